@@ -1,0 +1,764 @@
+// evxa_api.cpp -- host runtime and C ABI of libevxa.so (see include/evxa.h).
+//
+// Pictures, record tables, coefficients and levels stay resident in HBM; the
+// host enqueues frame jobs on the context's HIP stream.  Every analysis
+// launch is: a reset of the work-list counters and row-progress words (a fill
+// kernel, so the sequence can be captured into a hipGraph), then one
+// persistent analysis kernel over the MB wavefronts of all jobs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "evxa_internal.h"
+
+using namespace evxa;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Failure{code, msg}; }
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        fail(EVXA_ERROR_EXECUTION_FAILURE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return EVXA_SUCCESS;
+    } catch (const Failure &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return EVXA_ERROR_OUTOFMEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return EVXA_ERROR_EXECUTION_FAILURE;
+    }
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Plane geometry with apron; pitch rounded to 64 elements (128 bytes).
+struct PlaneGeom {
+    int32_t w, h, pitch, rows;
+    size_t bytes() const { return (size_t)pitch * rows * sizeof(int16_t); }
+    size_t origin() const { return (size_t)kApron * pitch + kApron; }
+};
+
+PlaneGeom plane_geom(int32_t w, int32_t h)
+{
+    PlaneGeom g;
+    g.w = w;
+    g.h = h;
+    g.pitch = (int32_t)align_up((size_t)w + 2 * kApron, 64);
+    g.rows = h + 2 * kApron;
+    return g;
+}
+
+size_t picture_bytes(int32_t w, int32_t h)
+{
+    return plane_geom(w, h).bytes() + 2 * plane_geom(w / 2, h / 2).bytes();
+}
+
+size_t outputs_bytes(int32_t w, int32_t h)
+{
+    const size_t n = (size_t)(w / 16) * (h / 16);
+    return 3 * n * sizeof(evxa_block_record) + 2 * n * EVXA_MB_SAMPLES * sizeof(int16_t);
+}
+
+// Device buffer owned by RAII.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    void alloc(size_t bytes)
+    {
+        release();
+        if (bytes == 0) return;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(EVXA_ERROR_OUTOFMEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        n = bytes;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t bytes)
+    {
+        if (bytes <= n) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(EVXA_ERROR_OUTOFMEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        n = bytes;
+    }
+    ~PinnedBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+float lambda_of(int32_t quality)
+{
+    // common.h:264-266, computed on the host with the same libm powf.
+    return 0.85f * powf(2.0f, ((float)quality - 12.0f) / 3.0f);
+}
+
+// Device-side resources of one launch configuration: job table, work list,
+// row-progress words and counters.  Graphs own theirs, so replays never see
+// another launch's tables.
+struct LaunchRes {
+    DevBuf jobs, tickets, progress, counters;
+    PinnedBuf h_jobs, h_tickets;
+    hipEvent_t staged = nullptr;     // host staging may be reused after this
+    std::vector<uint64_t> key;       // composition of the cached work list
+    int32_t n_tickets = 0;
+    int32_t n_progress = 0;
+    ~LaunchRes()
+    {
+        if (staged) (void)hipEventDestroy(staged);
+    }
+};
+
+}  // namespace
+
+struct evxa_stream;
+
+struct evxa_context {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int32_t grid = 0;
+    std::mutex mu;
+    LaunchRes res;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timed = false;
+    std::map<int32_t, std::unique_ptr<LaunchRes>> graph_res;
+    std::map<int32_t, hipGraphExec_t> graphs;
+    int32_t next_graph = 1;
+    std::vector<evxa_stream *> streams;
+};
+
+struct evxa_stream {
+    evxa_context *ctx = nullptr;
+    int32_t w = 0, h = 0, wb = 0, hb = 0, n_slots = 0;
+    PlaneGeom gy{}, gc{};
+    std::vector<std::unique_ptr<DevBuf>> slots;
+    DevBuf tables;         // 3 record tables: P ping, P pong, B
+    DevBuf coeffs, levels;
+    int32_t p_cur = 0;
+    bool temporal_valid = false;
+    int32_t last_table = -1;
+    uint64_t bytes = 0;
+
+    evxa_block_record *table(int k) const
+    {
+        return tables.as<evxa_block_record>() + (size_t)k * wb * hb;
+    }
+    DevPic pic(int32_t slot) const
+    {
+        const int16_t *base = slots[slot]->as<int16_t>();
+        DevPic p;
+        p.y = base + gy.origin();
+        const int16_t *ub = base + (size_t)gy.pitch * gy.rows;
+        p.u = ub + gc.origin();
+        p.v = ub + (size_t)gc.pitch * gc.rows + gc.origin();
+        p.py = gy.pitch;
+        p.pc = gc.pitch;
+        return p;
+    }
+};
+
+namespace {
+
+void set_device(const evxa_context *ctx) { check(hipSetDevice(ctx->device), "hipSetDevice"); }
+
+void check_slot(const evxa_stream *s, int32_t slot)
+{
+    if (slot < 0 || slot >= s->n_slots) fail(EVXA_ERROR_INVALIDARG, "slot out of range");
+}
+
+// Build the job descriptor of one frame and advance the stream's
+// temporal-table state (P frames ping-pong, B frames use a third table).
+DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
+{
+    if (d.type != EVXA_FRAME_P && d.type != EVXA_FRAME_B) fail(EVXA_ERROR_INVALIDARG, "frame type");
+    if (d.type == EVXA_FRAME_P && (d.n_refs < 1 || d.n_refs > EVXA_MAX_REFS))
+        fail(EVXA_ERROR_INVALIDARG, "P frames take 1..4 references");
+    if (d.type == EVXA_FRAME_B && d.n_refs != 2)
+        fail(EVXA_ERROR_INVALIDARG, "B frames take 2 references");
+    if (d.quality < 0 || d.quality > 255) fail(EVXA_ERROR_INVALIDARG, "quality");
+    check_slot(s, d.src_slot);
+    DevJob j;
+    memset(&j, 0, sizeof(j));
+    j.src = s->pic(d.src_slot);
+    for (int k = 0; k < d.n_refs; k++) {
+        check_slot(s, d.ref_slots[k]);
+        j.refs[k] = s->pic(d.ref_slots[k]);
+    }
+    j.width = s->w;
+    j.height = s->h;
+    j.wb = s->wb;
+    j.hb = s->hb;
+    j.type = d.type;
+    j.quality = d.quality;
+    j.n_refs = d.n_refs;
+    j.thresh = (d.quality >> 2) + 1;   // motion.cpp:589
+    j.lambda = lambda_of(d.quality);
+    j.target_ids = (uint32_t)d.target_ids[0] | ((uint32_t)d.target_ids[1] << 8) |
+                   ((uint32_t)d.target_ids[2] << 16) | ((uint32_t)d.target_ids[3] << 24);
+    int out;
+    if (d.type == EVXA_FRAME_P) {
+        out = s->p_cur;
+        j.prev = (d.temporal_mvp && s->temporal_valid) ? s->table(s->p_cur ^ 1) : nullptr;
+    } else {
+        out = 2;
+        j.prev = nullptr;
+    }
+    j.records = s->table(out);
+    j.coeffs = s->coeffs.as<int16_t>();
+    j.levels = s->levels.as<int16_t>();
+    if (advance) {
+        s->last_table = out;
+        if (d.type == EVXA_FRAME_P && d.update_temporal) {
+            s->p_cur ^= 1;
+            s->temporal_valid = true;
+        }
+    }
+    return j;
+}
+
+// Work list: anti-diagonal order t = bx + 2 by across all jobs, so every
+// ticket's predecessors (left, top-left, top, top-right) carry smaller
+// tickets and are owned by waves already running.
+void build_tickets(const std::vector<DevJob> &jobs, std::vector<uint32_t> &out)
+{
+    out.clear();
+    int32_t tmax = 0;
+    for (const auto &j : jobs) tmax = std::max(tmax, j.wb - 1 + 2 * (j.hb - 1));
+    for (int32_t t = 0; t <= tmax; t++)
+        for (size_t k = 0; k < jobs.size(); k++) {
+            const DevJob &j = jobs[k];
+            for (int32_t by = 0; by < j.hb; by++) {
+                const int32_t bx = t - 2 * by;
+                if (bx < 0) break;
+                if (bx >= j.wb) continue;
+                out.push_back(((uint32_t)k << kJobShift) | (uint32_t)(by * j.wb + bx));
+            }
+        }
+}
+
+// Stage jobs + work list into `r` and enqueue reset + analysis kernels.
+void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool time_it)
+{
+    if (jobs.empty()) return;
+    if ((int)jobs.size() > kMaxJobs) fail(EVXA_ERROR_CAPACITY_LIMIT, "too many jobs in one launch");
+    std::vector<uint64_t> key;
+    int32_t n_progress = 0;
+    for (const auto &j : jobs) {
+        if ((int64_t)j.wb * j.hb > (int64_t)kMbMask) fail(EVXA_ERROR_CAPACITY_LIMIT, "frame too large");
+        key.push_back(((uint64_t)j.wb << 32) | (uint32_t)j.hb);
+        n_progress += j.hb;
+    }
+    if (!r.staged) check(hipEventCreateWithFlags(&r.staged, hipEventDisableTiming), "hipEventCreate");
+    // The previous launch's host->device copies must have consumed the
+    // staging buffers before they are rewritten.
+    check(hipEventSynchronize(r.staged), "hipEventSynchronize");
+
+    if (key != r.key) {
+        std::vector<uint32_t> t;
+        build_tickets(jobs, t);
+        r.h_tickets.ensure(t.size() * sizeof(uint32_t));
+        memcpy(r.h_tickets.p, t.data(), t.size() * sizeof(uint32_t));
+        if (r.tickets.n < t.size() * sizeof(uint32_t)) r.tickets.alloc(t.size() * sizeof(uint32_t));
+        check(hipMemcpyAsync(r.tickets.p, r.h_tickets.p, t.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, ctx->stream),
+              "copy work list");
+        r.n_tickets = (int32_t)t.size();
+        r.key = key;
+    }
+    const size_t prog_bytes = (size_t)(n_progress + kCounterWords) * sizeof(int32_t);
+    if (r.progress.n < prog_bytes) r.progress.alloc(prog_bytes);
+    r.n_progress = n_progress;
+    int32_t *prog = r.progress.as<int32_t>();
+    int32_t off = kCounterWords;
+    for (auto &j : jobs) {
+        j.progress = prog + off;
+        off += j.hb;
+    }
+    const size_t jb = jobs.size() * sizeof(DevJob);
+    r.h_jobs.ensure(jb);
+    memcpy(r.h_jobs.p, jobs.data(), jb);
+    if (r.jobs.n < jb) r.jobs.alloc(jb);
+    check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, ctx->stream), "copy jobs");
+    check(hipEventRecord(r.staged, ctx->stream), "hipEventRecord");
+    check(launch_reset(prog, n_progress + kCounterWords, ctx->stream), "reset kernel");
+    const int32_t grid = std::max(1, std::min(ctx->grid, r.n_tickets));
+    if (time_it) check(hipEventRecord(ctx->ev_start, ctx->stream), "hipEventRecord");
+    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.n_tickets, prog, grid,
+                         ctx->stream),
+          "analysis kernel");
+    if (time_it) {
+        check(hipEventRecord(ctx->ev_stop, ctx->stream), "hipEventRecord");
+        ctx->timed = true;
+    }
+}
+
+void check_counters(evxa_context *ctx, LaunchRes &r)
+{
+    if (!r.progress.p) return;
+    int32_t c[kCounterWords];
+    check(hipMemcpy(c, r.progress.p, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    if (c[kCounterError] & kErrWaitTimeout)
+        fail(EVXA_ERROR_OPERATION_TIMEDOUT, "analysis kernel: dependency wait timed out");
+    if (c[kCounterError] & kErrSearchFailed)
+        fail(EVXA_ERROR_EXECUTION_FAILURE, "analysis kernel: motion search failed");
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t evxa_version(void) { return 100; }
+
+const char *evxa_last_error(void) { return g_last_error.c_str(); }
+
+float evxa_lambda(int32_t quality) { return lambda_of(quality); }
+
+uint64_t evxa_stream_bytes(int32_t width, int32_t height, int32_t n_slots)
+{
+    if (width <= 0 || height <= 0 || n_slots <= 0) return 0;
+    return (uint64_t)n_slots * picture_bytes(width, height) + outputs_bytes(width, height);
+}
+
+uint64_t evxa_context_bytes(const evxa_context *ctx) { return ctx ? ctx->used : 0; }
+
+int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context **out)
+{
+    return guarded([&] {
+        if (!out) fail(EVXA_ERROR_INVALIDARG, "out is null");
+        *out = nullptr;
+        int n = 0;
+        check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (device_ordinal < 0 || device_ordinal >= n) fail(EVXA_ERROR_INVALIDARG, "no such device");
+        auto ctx = std::make_unique<evxa_context>();
+        ctx->device = device_ordinal;
+        set_device(ctx.get());
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&ctx->ev_start), "hipEventCreate");
+        check(hipEventCreate(&ctx->ev_stop), "hipEventCreate");
+        if (hbm_budget_bytes == 0) {
+            size_t free_b = 0, total_b = 0;
+            check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            hbm_budget_bytes = (uint64_t)(free_b * 0.9);
+        }
+        ctx->budget = hbm_budget_bytes;
+        ctx->grid = analyze_resident_waves(device_ordinal);
+        *out = ctx.release();
+    });
+}
+
+int evxa_destroy(evxa_context *ctx)
+{
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+        ctx->graphs.clear();
+        ctx->graph_res.clear();
+        for (auto *s : ctx->streams) delete s;
+        ctx->streams.clear();
+        if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+        hipStream_t st = ctx->stream;
+        delete ctx;
+        if (st) (void)hipStreamDestroy(st);
+    });
+}
+
+int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n_slots,
+                     evxa_stream **out)
+{
+    return guarded([&] {
+        if (!ctx || !out) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        *out = nullptr;
+        if (width < 16 || height < 16 || width % 16 || height % 16 || width > 16384 ||
+            height > 16384)
+            fail(EVXA_ERROR_INVALIDARG, "width/height must be multiples of 16 in [16, 16384]");
+        if (n_slots < 1 || n_slots > 64) fail(EVXA_ERROR_INVALIDARG, "n_slots must be in [1, 64]");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        const uint64_t need = evxa_stream_bytes(width, height, n_slots);
+        if (ctx->used + need > ctx->budget)
+            fail(EVXA_ERROR_CAPACITY_LIMIT, "stream does not fit the HBM budget");
+        set_device(ctx);
+        auto s = std::make_unique<evxa_stream>();
+        s->ctx = ctx;
+        s->w = width;
+        s->h = height;
+        s->wb = width / 16;
+        s->hb = height / 16;
+        s->n_slots = n_slots;
+        s->gy = plane_geom(width, height);
+        s->gc = plane_geom(width / 2, height / 2);
+        const size_t pb = picture_bytes(width, height);
+        for (int k = 0; k < n_slots; k++) {
+            s->slots.emplace_back(new DevBuf(pb));
+            check(hipMemsetAsync(s->slots.back()->p, 0, pb, ctx->stream), "clear slot");
+        }
+        const size_t n = (size_t)s->wb * s->hb;
+        s->tables.alloc(3 * n * sizeof(evxa_block_record));
+        check(hipMemsetAsync(s->tables.p, 0, 3 * n * sizeof(evxa_block_record), ctx->stream),
+              "clear tables");
+        s->coeffs.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
+        s->levels.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        s->bytes = need;
+        ctx->used += need;
+        ctx->streams.push_back(s.get());
+        *out = s.release();
+    });
+}
+
+int evxa_stream_close(evxa_stream *s)
+{
+    return guarded([&] {
+        if (!s) return;
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->used -= s->bytes;
+        ctx->streams.erase(std::remove(ctx->streams.begin(), ctx->streams.end(), s),
+                           ctx->streams.end());
+        delete s;
+    });
+}
+
+int evxa_stream_reset_temporal(evxa_stream *s)
+{
+    return guarded([&] {
+        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+        std::lock_guard<std::mutex> lk(s->ctx->mu);
+        s->temporal_valid = false;
+    });
+}
+
+int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const int16_t *u,
+                        const int16_t *v, int32_t stride_y, int32_t stride_c)
+{
+    return guarded([&] {
+        if (!s || !y || !u || !v) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        check_slot(s, slot);
+        if (stride_y < s->w || stride_c < s->w / 2) fail(EVXA_ERROR_INVALIDARG, "stride too small");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        int16_t *base = s->slots[slot]->as<int16_t>();
+        int16_t *py = base;
+        int16_t *pu = base + (size_t)s->gy.pitch * s->gy.rows;
+        int16_t *pv = pu + (size_t)s->gc.pitch * s->gc.rows;
+        const PlaneGeom *g[3] = {&s->gy, &s->gc, &s->gc};
+        int16_t *dst[3] = {py, pu, pv};
+        const int16_t *src[3] = {y, u, v};
+        const int32_t strides[3] = {stride_y, stride_c, stride_c};
+        for (int k = 0; k < 3; k++) {
+            check(hipMemcpy2DAsync(dst[k] + g[k]->origin(), (size_t)g[k]->pitch * 2, src[k],
+                                   (size_t)strides[k] * 2, (size_t)g[k]->w * 2, g[k]->h,
+                                   hipMemcpyHostToDevice, ctx->stream),
+                  "upload plane");
+            check(launch_apron(dst[k] + g[k]->origin(), g[k]->pitch, g[k]->w, g[k]->h, ctx->stream),
+                  "apron kernel");
+        }
+        // Host buffers are the caller's: finish before returning.
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc)
+{
+    return guarded([&] {
+        if (!s || !desc) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        std::vector<DevJob> jobs{make_job(s, *desc, true)};
+        enqueue(ctx, ctx->res, jobs, true);
+    });
+}
+
+int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams, const evxa_frame_desc *descs,
+                       int32_t n)
+{
+    return guarded([&] {
+        if (!ctx || (n > 0 && (!streams || !descs)) || n < 0) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        for (int32_t a = 0; a < n; a++) {
+            if (!streams[a] || streams[a]->ctx != ctx) fail(EVXA_ERROR_INVALIDARG, "foreign stream");
+            for (int32_t b = 0; b < a; b++)
+                if (streams[a] == streams[b])
+                    fail(EVXA_ERROR_INVALIDARG, "a stream appears twice in one batch");
+        }
+        std::vector<DevJob> jobs;
+        jobs.reserve(n);
+        for (int32_t a = 0; a < n; a++) jobs.push_back(make_job(streams[a], descs[a], true));
+        enqueue(ctx, ctx->res, jobs, true);
+    });
+}
+
+int evxa_synchronize(evxa_context *ctx)
+{
+    return guarded([&] {
+        if (!ctx) fail(EVXA_ERROR_INVALIDARG, "null context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check_counters(ctx, ctx->res);
+    });
+}
+
+int evxa_last_launch_ms(evxa_context *ctx, float *ms)
+{
+    return guarded([&] {
+        if (!ctx || !ms) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (!ctx->timed) fail(EVXA_ERROR_INVALID_RESOURCE, "no timed launch yet");
+        set_device(ctx);
+        check(hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop), "hipEventElapsedTime");
+    });
+}
+
+int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeffs, int16_t *levels)
+{
+    return guarded([&] {
+        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
+        set_device(ctx);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check_counters(ctx, ctx->res);
+        const size_t n = (size_t)s->wb * s->hb;
+        if (records)
+            check(hipMemcpy(records, s->table(s->last_table), n * sizeof(evxa_block_record),
+                            hipMemcpyDeviceToHost),
+                  "read records");
+        if (coeffs)
+            check(hipMemcpy(coeffs, s->coeffs.p, n * EVXA_MB_SAMPLES * 2, hipMemcpyDeviceToHost),
+                  "read coefficients");
+        if (levels)
+            check(hipMemcpy(levels, s->levels.p, n * EVXA_MB_SAMPLES * 2, hipMemcpyDeviceToHost),
+                  "read levels");
+    });
+}
+
+int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams, const evxa_frame_desc *descs,
+                      int32_t n, int32_t *graph_id)
+{
+    return guarded([&] {
+        if (!ctx || !streams || !descs || !graph_id || n <= 0) fail(EVXA_ERROR_INVALIDARG, "bad argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        std::vector<DevJob> jobs;
+        for (int32_t a = 0; a < n; a++) {
+            if (!streams[a] || streams[a]->ctx != ctx) fail(EVXA_ERROR_INVALIDARG, "foreign stream");
+            jobs.push_back(make_job(streams[a], descs[a], false));
+            streams[a]->last_table = descs[a].type == EVXA_FRAME_P ? streams[a]->p_cur : 2;
+        }
+        auto res = std::make_unique<LaunchRes>();
+        // Stage the work list and job table outside capture (plain copies),
+        // then capture only the reset + analysis kernels.
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        enqueue(ctx, *res, jobs, false);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        hipError_t e1 = launch_reset(res->progress.as<int32_t>(), res->n_progress + kCounterWords,
+                                     ctx->stream);
+        const int32_t grid = std::max(1, std::min(ctx->grid, res->n_tickets));
+        hipError_t e2 = hipEventRecord(ctx->ev_start, ctx->stream);
+        hipError_t e3 = launch_analyze(res->jobs.as<DevJob>(), res->tickets.as<uint32_t>(),
+                                       res->n_tickets, res->progress.as<int32_t>(), grid, ctx->stream);
+        hipError_t e4 = hipEventRecord(ctx->ev_stop, ctx->stream);
+        hipError_t e5 = hipStreamEndCapture(ctx->stream, &graph);
+        check(e1, "capture reset");
+        check(e2, "capture event");
+        check(e3, "capture analysis");
+        check(e4, "capture event");
+        check(e5, "end capture");
+        hipGraphExec_t exec = nullptr;
+        hipError_t e6 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e6, "hipGraphInstantiate");
+        const int32_t id = ctx->next_graph++;
+        ctx->graphs[id] = exec;
+        ctx->graph_res[id] = std::move(res);
+        *graph_id = id;
+    });
+}
+
+int evxa_graph_launch(evxa_context *ctx, int32_t graph_id)
+{
+    return guarded([&] {
+        if (!ctx) fail(EVXA_ERROR_INVALIDARG, "null context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->graphs.find(graph_id);
+        if (it == ctx->graphs.end()) fail(EVXA_ERROR_INVALIDARG, "no such graph");
+        set_device(ctx);
+        check(hipGraphLaunch(it->second, ctx->stream), "hipGraphLaunch");
+        ctx->timed = true;
+    });
+}
+
+int evxa_graph_destroy(evxa_context *ctx, int32_t graph_id)
+{
+    return guarded([&] {
+        if (!ctx) fail(EVXA_ERROR_INVALIDARG, "null context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->graphs.find(graph_id);
+        if (it == ctx->graphs.end()) fail(EVXA_ERROR_INVALIDARG, "no such graph");
+        set_device(ctx);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        (void)hipGraphExecDestroy(it->second);
+        ctx->graphs.erase(it);
+        ctx->graph_res.erase(graph_id);
+    });
+}
+
+int evxa_interp_blocks(evxa_stream *s, int32_t slot, int32_t n, const int32_t *xs, const int32_t *ys,
+                       const int32_t *dir_x, const int32_t *dir_y, const int32_t *quarter, int16_t *out)
+{
+    return guarded([&] {
+        if (!s || n < 0 || (n > 0 && (!xs || !ys || !dir_x || !dir_y || !quarter || !out)))
+            fail(EVXA_ERROR_INVALIDARG, "bad argument");
+        if (n == 0) return;
+        check_slot(s, slot);
+        std::vector<int32_t> p((size_t)n * 5);
+        for (int32_t k = 0; k < n; k++) {
+            if (xs[k] < 0 || xs[k] > s->w - 16 || ys[k] < 0 || ys[k] > s->h - 16)
+                fail(EVXA_ERROR_INVALIDARG, "block base outside the picture");
+            if (dir_x[k] < -1 || dir_x[k] > 1 || dir_y[k] < -1 || dir_y[k] > 1)
+                fail(EVXA_ERROR_INVALIDARG, "direction components must be -1, 0 or 1");
+            p[k * 5] = xs[k]; p[k * 5 + 1] = ys[k]; p[k * 5 + 2] = dir_x[k];
+            p[k * 5 + 3] = dir_y[k]; p[k * 5 + 4] = quarter[k] ? 1 : 0;
+        }
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        DevBuf dp(p.size() * 4), dout((size_t)n * EVXA_MB_SAMPLES * 2);
+        check(hipMemcpy(dp.p, p.data(), p.size() * 4, hipMemcpyHostToDevice), "copy params");
+        check(launch_interp_blocks(s->pic(slot), s->w, s->h, n, dp.as<int32_t>(), dout.as<int16_t>(),
+                                   ctx->stream),
+              "interp kernel");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check(hipMemcpy(out, dout.p, (size_t)n * EVXA_MB_SAMPLES * 2, hipMemcpyDeviceToHost), "copy out");
+    });
+}
+
+int evxa_search_blocks(evxa_stream *s, int32_t src_slot, int32_t ref_slot, int32_t quality, int32_t n,
+                       const int32_t *xs, const int32_t *ys, const int16_t *pred_x,
+                       const int16_t *pred_y, evxa_search_result *out)
+{
+    return guarded([&] {
+        if (!s || n < 0 || (n > 0 && (!xs || !ys || !pred_x || !pred_y || !out)))
+            fail(EVXA_ERROR_INVALIDARG, "bad argument");
+        if (quality < 0 || quality > 255) fail(EVXA_ERROR_INVALIDARG, "quality");
+        if (n == 0) return;
+        check_slot(s, src_slot);
+        check_slot(s, ref_slot);
+        std::vector<int32_t> p((size_t)n * 4);
+        for (int32_t k = 0; k < n; k++) {
+            if (xs[k] < 0 || xs[k] > s->w - 16 || ys[k] < 0 || ys[k] > s->h - 16)
+                fail(EVXA_ERROR_INVALIDARG, "MB position outside the picture");
+            p[k * 4] = xs[k]; p[k * 4 + 1] = ys[k]; p[k * 4 + 2] = pred_x[k]; p[k * 4 + 3] = pred_y[k];
+        }
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        DevBuf dp(p.size() * 4), dout((size_t)n * sizeof(evxa_search_result));
+        check(hipMemcpy(dp.p, p.data(), p.size() * 4, hipMemcpyHostToDevice), "copy params");
+        check(launch_search_blocks(s->pic(src_slot), s->pic(ref_slot), s->w, s->h, quality,
+                                   lambda_of(quality), n, dp.as<int32_t>(),
+                                   dout.as<evxa_search_result>(), ctx->stream),
+              "search kernel");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check(hipMemcpy(out, dout.p, (size_t)n * sizeof(evxa_search_result), hipMemcpyDeviceToHost),
+              "copy out");
+    });
+}
+
+int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src, const int16_t *pred,
+                          const int32_t *quality, int16_t *coeffs, int16_t *levels, int32_t *qp,
+                          int32_t *variance)
+{
+    return guarded([&] {
+        if (!ctx || n < 0 || (n > 0 && (!src || !pred || !quality || !coeffs || !levels || !qp || !variance)))
+            fail(EVXA_ERROR_INVALIDARG, "bad argument");
+        if (n == 0) return;
+        for (int32_t k = 0; k < n; k++)
+            if (quality[k] < 0 || quality[k] > 255) fail(EVXA_ERROR_INVALIDARG, "quality");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        const size_t sb = (size_t)n * EVXA_MB_SAMPLES * 2;
+        DevBuf ds(sb), dp(sb), dq((size_t)n * 4), dc(sb), dl(sb), dqp((size_t)n * 4), dv((size_t)n * 4);
+        check(hipMemcpy(ds.p, src, sb, hipMemcpyHostToDevice), "copy src");
+        check(hipMemcpy(dp.p, pred, sb, hipMemcpyHostToDevice), "copy pred");
+        check(hipMemcpy(dq.p, quality, (size_t)n * 4, hipMemcpyHostToDevice), "copy quality");
+        check(launch_dct_quant_blocks(n, ds.as<int16_t>(), dp.as<int16_t>(), dq.as<int32_t>(),
+                                      dc.as<int16_t>(), dl.as<int16_t>(), dqp.as<int32_t>(),
+                                      dv.as<int32_t>(), ctx->stream),
+              "dct/quant kernel");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check(hipMemcpy(coeffs, dc.p, sb, hipMemcpyDeviceToHost), "copy coefficients");
+        check(hipMemcpy(levels, dl.p, sb, hipMemcpyDeviceToHost), "copy levels");
+        check(hipMemcpy(qp, dqp.p, (size_t)n * 4, hipMemcpyDeviceToHost), "copy qp");
+        check(hipMemcpy(variance, dv.p, (size_t)n * 4, hipMemcpyDeviceToHost), "copy variance");
+    });
+}
+
+}  // extern "C"

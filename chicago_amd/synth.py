@@ -1,0 +1,174 @@
+"""Seeded synthetic 8-bit 4:2:0 content for the analysis workloads.
+
+Frames are int16 planes holding 8-bit samples, aligned up to 16 with zero
+rows the way the reference pads its images (evx3enc.cpp:93-94,
+image.cpp:89).  Generated once on the host; the CPU and GPU paths consume the
+same buffers, so libm differences never reach a parity check.
+
+Recipes
+  texture:  Y = clamp(round(128 + sum_k (40/6) sin(2 pi (fx_k x + fy_k y) + phi_k)
+                 + U[-8, 8]), 0, 255), k = 1..6, f uniform in +-1/32 cycles/px.
+            Chroma planes are independent draws of the same recipe at half size.
+  moved:    every 16x16 block (with its 8x8 chroma) is fetched from the previous
+            picture at an integer displacement (clamped inside the picture) and
+            an optional half/quarter-pel state that is sampled with the 6-tap
+            rule of interp.cpp:52-164, then Gaussian noise (sigma 2) is added
+            and the result is clamped to [0, 255].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+DIRS = [(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)]
+
+
+def align16(n: int) -> int:
+    return (n + 15) & ~15
+
+
+def seed_of(*parts) -> int:
+    """Stable 64-bit seed from small integers (config, stream, purpose...)."""
+    h = 0x9E3779B97F4A7C15
+    for p in parts:
+        h ^= (int(p) + 0x9E3779B97F4A7C15 + ((h << 6) & 0xFFFFFFFFFFFFFFFF) + (h >> 2))
+        h &= 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _texture_plane(rng, w, h):
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    acc = np.full((h, w), 128.0)
+    for _ in range(6):
+        fx, fy = rng.uniform(-1 / 32, 1 / 32, size=2)
+        phi = rng.uniform(0, 2 * np.pi)
+        acc += (40.0 / 6.0) * np.sin(2 * np.pi * (fx * xx + fy * yy) + phi)
+    acc += rng.integers(-8, 9, size=(h, w))
+    return np.clip(np.rint(acc), 0, 255).astype(np.int16)
+
+
+def texture_frame(rng, width, height):
+    """(y, u, v) at the aligned size; rows past `height` are zero."""
+    aw, ah = align16(width), align16(height)
+    y = np.zeros((ah, aw), np.int16)
+    u = np.zeros((ah // 2, aw // 2), np.int16)
+    v = np.zeros((ah // 2, aw // 2), np.int16)
+    y[:height, :width] = _texture_plane(rng, width, height)
+    u[:height // 2, :width // 2] = _texture_plane(rng, width // 2, height // 2)
+    v[:height // 2, :width // 2] = _texture_plane(rng, width // 2, height // 2)
+    return y, u, v
+
+
+def _clamped(p, xs, ys):
+    h, w = p.shape
+    return p[np.clip(ys, 0, h - 1), np.clip(xs, 0, w - 1)].astype(np.int32)
+
+
+def _tap6(p, xs, ys, axis_x):
+    """6-tap half sample at (xs + 0.5, ys) if axis_x else (xs, ys + 0.5)."""
+    taps = (1, -5, 20, 20, -5, 1)
+    acc = 0
+    for k, t in enumerate(taps):
+        o = k - 2
+        acc = acc + t * (_clamped(p, xs + o, ys) if axis_x else _clamped(p, xs, ys + o))
+    return ((acc + 16) >> 5).astype(np.int16)
+
+
+def subpel_plane(p, dx, dy, quarter):
+    """Whole-plane sub-pel samples: value at every integer base position of
+    the 6-tap rule (interp.cpp:52-164).  Depends only on absolute position."""
+    h, w = p.shape
+    ys, xs = np.mgrid[0:h, 0:w]
+    xo = 0 if dx > 0 else -1
+    yo = 0 if dy > 0 else -1
+    if dx != 0 and dy == 0:
+        half = _tap6(p, xs + xo, ys, True)
+    elif dx == 0 and dy != 0:
+        half = _tap6(p, xs, ys + yo, False)
+    else:
+        taps = (1, -5, 20, 20, -5, 1)
+        acc = 0
+        for k, t in enumerate(taps):
+            acc = acc + t * _tap6(p, xs + xo, ys + yo + k - 2, True).astype(np.int32)
+        half = ((acc + 16) >> 5).astype(np.int16)
+    if quarter:
+        return ((p.astype(np.int32) + half.astype(np.int32) + 1) >> 1).astype(np.int16)
+    return half
+
+
+def move_blocks(rng, prev, mvx, mvy, sp_state, noise_sigma=2.0, visible=None):
+    """Build a picture from `prev` block by block.
+
+    mvx/mvy: int arrays (hb, wb) of integer displacements; sp_state: int array
+    (hb, wb) with -1 for none, else dir_index*2 + quarter (dir_index into DIRS).
+    """
+    y, u, v = prev
+    ah, aw = y.shape
+    hb, wb = ah // 16, aw // 16
+    out = [np.empty_like(y), np.empty_like(u), np.empty_like(v)]
+    planes = {}
+
+    def plane(ci, st):
+        key = (ci, st)
+        if key not in planes:
+            src = prev[ci]
+            if st < 0:
+                planes[key] = src
+            else:
+                dx, dy = DIRS[st // 2]
+                planes[key] = subpel_plane(src, dx, dy, st & 1)
+        return planes[key]
+
+    for by in range(hb):
+        for bx in range(wb):
+            x, yy = bx * 16, by * 16
+            sx = int(np.clip(x + mvx[by, bx], 0, aw - 16))
+            sy = int(np.clip(yy + mvy[by, bx], 0, ah - 16))
+            st = int(sp_state[by, bx])
+            out[0][yy:yy + 16, x:x + 16] = plane(0, st)[sy:sy + 16, sx:sx + 16]
+            cx, cy, scx, scy = x // 2, yy // 2, sx >> 1, sy >> 1
+            out[1][cy:cy + 8, cx:cx + 8] = plane(1, st)[scy:scy + 8, scx:scx + 8]
+            out[2][cy:cy + 8, cx:cx + 8] = plane(2, st)[scy:scy + 8, scx:scx + 8]
+    for i in range(3):
+        n = np.rint(rng.normal(0.0, noise_sigma, size=out[i].shape))
+        out[i] = np.clip(out[i].astype(np.float64) + n, 0, 255).astype(np.int16)
+    if visible is not None:
+        w, h = visible
+        out[0][h:, :] = 0
+        out[1][h // 2:, :] = 0
+        out[2][h // 2:, :] = 0
+    return tuple(out)
+
+
+def pair_c0(seed, width=1280, height=720, mv_range=16):
+    """Reference + current picture: random integer MV in [-16, 16]^2 and a
+    random sub-pel state per block (none, or one of 8 dirs x half/quarter)."""
+    rng = np.random.default_rng(seed)
+    ref = texture_frame(rng, width, height)
+    ah, aw = ref[0].shape
+    hb, wb = ah // 16, aw // 16
+    mvx = rng.integers(-mv_range, mv_range + 1, size=(hb, wb))
+    mvy = rng.integers(-mv_range, mv_range + 1, size=(hb, wb))
+    sp = rng.integers(-1, 16, size=(hb, wb))
+    cur = move_blocks(rng, ref, mvx, mvy, sp, visible=(width, height))
+    return ref, cur
+
+
+def sequence_c1(seed, n_frames, width=1920, height=1080, local_range=24, local_frac=0.3,
+                pan_quarter=True):
+    """Frame 0 from the texture recipe; frame t is frame t-1 moved by a global
+    pan of 1.25 px/frame (integer 1 plus the quarter-pel (-1, 0) state) with
+    uniform +-`local_range` px local motion on `local_frac` of the blocks."""
+    rng = np.random.default_rng(seed)
+    frames = [texture_frame(rng, width, height)]
+    ah, aw = frames[0][0].shape
+    hb, wb = ah // 16, aw // 16
+    st = DIRS.index((-1, 0)) * 2 + 1 if pan_quarter else -1
+    for _ in range(1, n_frames):
+        mvx = np.full((hb, wb), -1)
+        mvy = np.zeros((hb, wb), np.int64)
+        local = rng.random((hb, wb)) < local_frac
+        mvx = mvx + np.where(local, rng.integers(-local_range, local_range + 1, (hb, wb)), 0)
+        mvy = mvy + np.where(local, rng.integers(-local_range, local_range + 1, (hb, wb)), 0)
+        sp = np.full((hb, wb), st)
+        frames.append(move_blocks(rng, frames[-1], mvx, mvy, sp, visible=(width, height)))
+    return frames

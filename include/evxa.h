@@ -1,0 +1,185 @@
+/*
+ * evxa.h -- C ABI of the MI355X (gfx950) inter-analysis path for the
+ * Chicago/EVX3 video encoder: integer step search, 6-tap half/quarter-pel
+ * refinement, 8x8 forward DCT, adaptive QP and inter quantisation, run as
+ * hand-written HIP kernels on device-resident pictures.
+ *
+ * The reference has no FFI; its path sits behind encoder-internal C++ free
+ * functions (namespace evx).  Each entry point below names the reference
+ * interface it replaces (paths relative to the reference's src/).  The ABI
+ * uses plain pointers, sizes and int status codes; no C++ exception crosses
+ * it.  Status values equal the reference's evx_status codes (base.h:166-185).
+ *
+ * Threading: a context owns one HIP stream; calls on one context are
+ * serialised by an internal mutex and execute in submission order.
+ */
+#ifndef EVXA_H
+#define EVXA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: same values as base.h:166-185. */
+#define EVXA_SUCCESS 0
+#define EVXA_ERROR_INVALIDARG 1
+#define EVXA_ERROR_NOTIMPL 2
+#define EVXA_ERROR_OUTOFMEMORY 3
+#define EVXA_ERROR_HARDWAREFAIL 5
+#define EVXA_ERROR_CAPACITY_LIMIT 7
+#define EVXA_ERROR_INVALID_RESOURCE 8
+#define EVXA_ERROR_OPERATION_TIMEDOUT 9
+#define EVXA_ERROR_EXECUTION_FAILURE 10
+
+#define EVXA_FRAME_P 1
+#define EVXA_FRAME_B 2
+#define EVXA_MAX_REFS 4
+#define EVXA_MB_SAMPLES 384 /* 16x16 Y + 8x8 U + 8x8 V, in that order */
+
+/* Block type bits (types.h:72-91). */
+#define EVXA_BLOCK_INTRA 0x1
+#define EVXA_BLOCK_MOTION 0x2
+#define EVXA_BLOCK_COPY 0x4
+
+/* Per-macroblock analysis record, 64 bytes.  Fields up to `variance` carry
+ * evx_block_desc (common.h:87-115); `sad` is the SAD the block selection
+ * used (encode.cpp:194-208, 399-441); mad/cost are the primary search's
+ * best_mad and fp32 best_cost bits (motion.cpp:53-66); *_b are the backward
+ * search's for bidirectional blocks; n_int / n_axial / n_diag count the
+ * integer candidates and sub-pel directions evaluated (work accounting);
+ * flags bit0 = residual transformed, bit1 = bidirectional average. */
+typedef struct evxa_block_record {
+    uint8_t block_type;
+    uint8_t prediction_mode;
+    uint8_t prediction_target;
+    uint8_t prediction_target_b;
+    int16_t motion_x, motion_y;
+    int16_t motion_x_b, motion_y_b;
+    uint8_t sp_pred, sp_amount, sp_index, sp_pred_b;
+    uint8_t sp_amount_b, sp_index_b, q_index, reserved0;
+    int16_t variance;
+    int16_t reserved1;
+    int32_t sad;
+    int32_t mad;
+    uint32_t cost_bits;
+    int32_t sad_b, mad_b;
+    uint32_t cost_b_bits;
+    uint32_t n_int;
+    uint16_t n_axial, n_diag;
+    uint32_t flags;
+    uint32_t reserved2;
+} evxa_block_record;
+
+/* Result of one inter search (motion.cpp:567-679 plus its internal state). */
+typedef struct evxa_search_result {
+    int32_t best_x, best_y;
+    int32_t sad, mad, ssd;
+    uint32_t cost_bits;
+    int32_t sp_enabled, sp_amount, sp_index;
+    int32_t block_type;
+    int32_t motion_x, motion_y;
+    int32_t n_int, n_axial, n_diag;
+} evxa_search_result;
+
+/* One frame of analysis work on a stream. */
+typedef struct evxa_frame_desc {
+    int32_t type;                 /* EVXA_FRAME_P or EVXA_FRAME_B */
+    int32_t quality;              /* frame QP (evx3enc.cpp:64-82: q <= 16 maps 1:1) */
+    int32_t src_slot;             /* picture slot holding the current frame */
+    int32_t n_refs;               /* P: 1..4 (most recent first); B: 2 (fwd, bwd) */
+    int32_t ref_slots[EVXA_MAX_REFS];
+    uint8_t target_ids[EVXA_MAX_REFS]; /* written to prediction_target(_b) */
+    int32_t temporal_mvp;         /* P: use the stream's temporal table if one exists */
+    int32_t update_temporal;      /* P: this frame's table becomes the temporal table */
+} evxa_frame_desc;
+
+typedef struct evxa_context evxa_context;
+typedef struct evxa_stream evxa_stream;
+
+/* ---- lifecycle (replaces evx_context set-up, common.cpp:106-202) ------ */
+
+/* Create a context on `device_ordinal` that admits streams while their
+ * device footprint stays within `hbm_budget_bytes` (0 = 90% of free HBM). */
+int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context **out);
+int evxa_destroy(evxa_context *ctx);
+/* Last error message of the calling thread ("" if none). */
+const char *evxa_last_error(void);
+/* ABI version (major * 100 + minor). */
+int32_t evxa_version(void);
+/* Bytes of HBM a stream of this size would hold (admission accounting). */
+uint64_t evxa_stream_bytes(int32_t width, int32_t height, int32_t n_slots);
+/* HBM currently charged to streams of this context. */
+uint64_t evxa_context_bytes(const evxa_context *ctx);
+/* fp32 lambda of common.h:264-266 for `quality`, as the kernels use it. */
+float evxa_lambda(int32_t quality);
+
+/* ---- streams and picture slots (replaces evx_cache_bank's
+ *      prediction_cache, common.h:124-137) ---------------------------- */
+
+/* width/height: luma size, multiples of 16 (frames aligned with zero rows,
+ * image.cpp:89).  Fails with EVXA_ERROR_CAPACITY_LIMIT when the stream does
+ * not fit the context's HBM budget. */
+int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n_slots,
+                     evxa_stream **out);
+int evxa_stream_close(evxa_stream *s);
+/* Copy a host picture (int16 planes, strides in elements) into `slot`. */
+int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const int16_t *u,
+                        const int16_t *v, int32_t stride_y, int32_t stride_c);
+/* Forget the temporal MV table (next P frame has no temporal candidate). */
+int evxa_stream_reset_temporal(evxa_stream *s);
+
+/* ---- frame jobs (replaces the classify_block / classify_block_bidir /
+ *      encode_block inter analysis, encode.cpp:168-248, 374-477, 481-739,
+ *      without intra candidates, RDO and trellis) --------------------- */
+
+/* Enqueue one frame: one kernel launch over the frame's MB wavefront. */
+int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc);
+/* Enqueue frames of distinct streams as one fused persistent launch. */
+int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams,
+                       const evxa_frame_desc *descs, int32_t n);
+/* Block until the context's queue is idle; reports device-side failures. */
+int evxa_synchronize(evxa_context *ctx);
+/* Device time (ms) of the last analysis launch, from HIP events around it. */
+int evxa_last_launch_ms(evxa_context *ctx, float *ms);
+/* Copy the last analysed frame's outputs to host memory (any may be NULL):
+ * one record per MB in raster order, 384 coefficients and 384 pre-trellis
+ * levels per MB (zero for copy blocks).  Blocks until they are available. */
+int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeffs,
+                      int16_t *levels);
+
+/* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
+
+/* Capture the launch of `n` frame descriptors into a replayable graph.
+ * Replays re-run the same slots, qualities and temporal-table roles. */
+int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams,
+                      const evxa_frame_desc *descs, int32_t n, int32_t *graph_id);
+int evxa_graph_launch(evxa_context *ctx, int32_t graph_id);
+int evxa_graph_destroy(evxa_context *ctx, int32_t graph_id);
+
+/* ---- per-block entries (parity tests and encoder drop-in) ------------ */
+
+/* interpolate_subpixel_6tap (interp.cpp:166-202) at n integer bases of a
+ * slot's picture; out receives n * 384 samples. */
+int evxa_interp_blocks(evxa_stream *s, int32_t slot, int32_t n, const int32_t *xs,
+                       const int32_t *ys, const int32_t *dir_x, const int32_t *dir_y,
+                       const int32_t *quarter, int16_t *out);
+/* perform_inter_search (motion.cpp:567-679) for n MBs of src_slot against
+ * ref_slot, each with the predictor compute_mv_predictor would supply. */
+int evxa_search_blocks(evxa_stream *s, int32_t src_slot, int32_t ref_slot, int32_t quality,
+                       int32_t n, const int32_t *xs, const int32_t *ys, const int16_t *pred_x,
+                       const int16_t *pred_y, evxa_search_result *out);
+/* sub_transform_macroblock + query_block_quantization_parameter +
+ * compute_block_variance2 + quantize_macroblock (inter) for n residual
+ * pairs (macroblock.h:272-277, quantize.cpp:97-118, 372-397;
+ * analysis.h:253-275).  src/pred/coeffs/levels hold n * 384 samples. */
+int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src,
+                          const int16_t *pred, const int32_t *quality, int16_t *coeffs,
+                          int16_t *levels, int32_t *qp, int32_t *variance);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EVXA_H */

@@ -1,0 +1,742 @@
+/*
+ * evx_oracle.c -- CPU restatement of the Chicago/EVX3 inter analysis path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP path; never part of the
+ * product).  Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so that the
+ * fp32 RD cost is a multiply followed by an add, as on the reference's x86
+ * build with SSE arithmetic and no FMA contraction).
+ *
+ * Reference citations are path:line relative to the reference's src/.
+ * Integer semantics follow the reference's C++ on x86-64: truncating
+ * division, arithmetic right shift, int16 stores that truncate, int32 sums
+ * that wrap (emulated with uint32 arithmetic where the sum can overflow).
+ */
+#include "evx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EVO_INT32_MAX 0x7FFFFFFF
+#define EVO_MB 16
+#define EVO_SAD_THRESHOLD (8 * 1024)        /* motion.cpp:24 */
+#define EVO_SEARCH_RADIUS 16                /* motion.cpp:29 */
+#define EVO_QUANT_SCALE 16                  /* quantize.cpp:10 */
+#define EVO_MAX_QUANT_LEVELS 64             /* quantize.h:42 */
+#define EVO_CHROMA_QP_OFFSET 2              /* config.h:68 */
+
+/* 8x8 DCT-II basis scaled by 128: round(128 cos((2k+1) i pi / 16)), row i,
+ * column k (xftables.h:57-67). */
+static const int16_t kDct8[64] = {
+    128,  128,  128,  128,  128,  128,  128,  128,
+    126,  106,   71,   25,  -25,  -71, -106, -126,
+    118,   49,  -49, -118, -118,  -49,   49,  118,
+    106,  -25, -126,  -71,   71,  126,   25, -106,
+     91,  -91,  -91,   91,   91,  -91,  -91,   91,
+     71, -126,   25,  106, -106,  -25,  126,  -71,
+     49, -118,  118,  -49,  -49,  118, -118,   49,
+     25,  -71,  106, -126,  126, -106,   71,  -25};
+
+/* Inter quantisation matrix, quantize.cpp:26-36. */
+static const int16_t kInterQm[64] = {
+    16, 17, 18, 19, 20, 21, 22, 23,
+    17, 18, 19, 20, 21, 22, 23, 24,
+    18, 19, 20, 21, 22, 23, 24, 25,
+    19, 20, 21, 22, 23, 24, 26, 27,
+    20, 21, 22, 23, 25, 26, 27, 28,
+    21, 22, 23, 24, 26, 27, 28, 30,
+    22, 23, 24, 26, 27, 28, 30, 31,
+    23, 24, 25, 27, 28, 30, 31, 33};
+
+/* Frequency deadzone weights, quantize.cpp:61-71. */
+static const uint8_t kDeadzone[64] = {
+    2, 2, 2, 3, 3, 3, 4, 4,
+    2, 2, 3, 3, 3, 4, 4, 4,
+    2, 3, 3, 3, 4, 4, 4, 4,
+    3, 3, 3, 4, 4, 4, 4, 4,
+    3, 3, 4, 4, 4, 4, 4, 4,
+    3, 4, 4, 4, 4, 4, 4, 4,
+    4, 4, 4, 4, 4, 4, 4, 4,
+    4, 4, 4, 4, 4, 4, 4, 4};
+
+/* ---------------------------------------------------------------- math -- */
+
+/* evx_math.h:228-236: rounding division, halves away from zero, C `/`. */
+static int32_t rounded_div(int32_t numer, int32_t denom)
+{
+    if (((uint32_t)numer & 0x80000000u) ^ ((uint32_t)denom & 0x80000000u))
+        return (numer - denom / 2) / denom;
+    return (numer + denom / 2) / denom;
+}
+
+/* evx_math.h:197-211 */
+static int32_t abs32(int32_t v) { return v == INT32_MIN ? 0x7FFFFFFF : (v < 0 ? -v : v); }
+static int16_t abs16(int16_t v) { return v == INT16_MIN ? 0x7FFF : (int16_t)(v < 0 ? -v : v); }
+
+/* evx_math.h:149-154 */
+static int16_t sign16(int16_t v) { return (int16_t)((v > 0) - (v < 0)); }
+
+/* evx_math.h:69-138 (EVX_USE_FAST_32BIT_LOG2): floor(log2(v)), log2(0) = 0. */
+static uint32_t floor_log2_u32(uint32_t v)
+{
+    uint32_t r = 0;
+    while (v >>= 1) r++;
+    return r;
+}
+
+static int32_t clip_i32(int32_t v, int32_t lo, int32_t hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+/* common.h:264-266 */
+float evo_lambda(int32_t quality)
+{
+    return 0.85f * powf(2.0f, ((float)quality - 12.0f) / 3.0f);
+}
+
+/* -------------------------------------------------------- block access -- */
+
+void evo_load_block(const evo_planes *p, int32_t x, int32_t y, int16_t out[EVO_MB_SAMPLES])
+{
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++)
+            out[r * 16 + c] = p->y[(y + r) * p->stride_y + x + c];
+    int32_t cx = x >> 1, cy = y >> 1;
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 8; c++) {
+            out[256 + r * 8 + c] = p->u[(cy + r) * p->stride_c + cx + c];
+            out[320 + r * 8 + c] = p->v[(cy + r) * p->stride_c + cx + c];
+        }
+}
+
+/* ------------------------------------------------------- interpolation -- */
+
+/* interp.cpp:8-14 */
+static int16_t sample_clamped(const int16_t *d, int32_t stride, int32_t x, int32_t y,
+                              int32_t w, int32_t h)
+{
+    x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+    y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+    return d[y * stride + x];
+}
+
+/* interp.cpp:18-28: taps [1,-5,20,20,-5,1], (sum+16)>>5, int16 store. */
+static int16_t tap6_h(const int16_t *d, int32_t s, int32_t x, int32_t y, int32_t w, int32_t h)
+{
+    int32_t sum = sample_clamped(d, s, x - 2, y, w, h) - 5 * sample_clamped(d, s, x - 1, y, w, h) +
+                  20 * sample_clamped(d, s, x, y, w, h) + 20 * sample_clamped(d, s, x + 1, y, w, h) -
+                  5 * sample_clamped(d, s, x + 2, y, w, h) + sample_clamped(d, s, x + 3, y, w, h);
+    return (int16_t)((sum + 16) >> 5);
+}
+
+/* interp.cpp:31-41 */
+static int16_t tap6_v(const int16_t *d, int32_t s, int32_t x, int32_t y, int32_t w, int32_t h)
+{
+    int32_t sum = sample_clamped(d, s, x, y - 2, w, h) - 5 * sample_clamped(d, s, x, y - 1, w, h) +
+                  20 * sample_clamped(d, s, x, y, w, h) + 20 * sample_clamped(d, s, x, y + 1, w, h) -
+                  5 * sample_clamped(d, s, x, y + 2, w, h) + sample_clamped(d, s, x, y + 3, w, h);
+    return (int16_t)((sum + 16) >> 5);
+}
+
+/* interp.cpp:52-164 for one plane; out has row pitch `bs`. */
+static void interp_plane(const int16_t *d, int32_t s, int32_t w, int32_t h, int32_t bx,
+                         int32_t by, int32_t bs, int32_t dx, int32_t dy, int32_t quarter,
+                         int16_t *out)
+{
+    int32_t xo = dx > 0 ? 0 : -1;
+    int32_t yo = dy > 0 ? 0 : -1;
+    int16_t tmp[16 * 21];
+    if (dx != 0 && dy != 0) {
+        /* interp.cpp:125-131: H pass over rows -2 .. bs+2 (relative to yo). */
+        for (int32_t r = -2; r < bs + 3; r++)
+            for (int32_t c = 0; c < bs; c++)
+                tmp[(r + 2) * bs + c] = tap6_h(d, s, bx + c + xo, by + r + yo, w, h);
+    }
+    for (int32_t r = 0; r < bs; r++)
+        for (int32_t c = 0; c < bs; c++) {
+            int16_t half;
+            if (dx != 0 && dy == 0) {
+                half = tap6_h(d, s, bx + c + xo, by + r, w, h);
+            } else if (dx == 0 && dy != 0) {
+                half = tap6_v(d, s, bx + c, by + r + yo, w, h);
+            } else {
+                /* interp.cpp:134-149 */
+                const int16_t *t = tmp + r * bs + c;
+                int32_t sum = t[0] - 5 * t[bs] + 20 * t[2 * bs] + 20 * t[3 * bs] - 5 * t[4 * bs] +
+                              t[5 * bs];
+                half = (int16_t)((sum + 16) >> 5);
+            }
+            if (quarter) {
+                /* interp.cpp:74-80: average with the integer sample. */
+                int16_t full = sample_clamped(d, s, bx + c, by + r, w, h);
+                out[r * bs + c] = (int16_t)((full + half + 1) >> 1);
+            } else {
+                out[r * bs + c] = half;
+            }
+        }
+}
+
+void evo_interp_block(const evo_planes *ref, int32_t bx, int32_t by, int32_t dir_x,
+                      int32_t dir_y, int32_t quarter, int16_t out[EVO_MB_SAMPLES])
+{
+    /* interp.cpp:186-201: luma 16x16, chroma 8x8 at (bx>>1, by>>1). */
+    interp_plane(ref->y, ref->stride_y, ref->width, ref->height, bx, by, 16, dir_x, dir_y,
+                 quarter, out);
+    int32_t cw = ref->width >> 1, ch = ref->height >> 1;
+    interp_plane(ref->u, ref->stride_c, cw, ch, bx >> 1, by >> 1, 8, dir_x, dir_y, quarter,
+                 out + 256);
+    interp_plane(ref->v, ref->stride_c, cw, ch, bx >> 1, by >> 1, 8, dir_x, dir_y, quarter,
+                 out + 320);
+}
+
+/* ------------------------------------------------------------- metrics -- */
+
+void evo_hadamard_4x4(const int16_t *src, int32_t stride, int32_t out[16])
+{
+    /* analysis.h:44-85 */
+    int32_t t[16];
+    for (int i = 0; i < 4; i++) {
+        int32_t a = src[i * stride + 0], b = src[i * stride + 1];
+        int32_t c = src[i * stride + 2], d = src[i * stride + 3];
+        int32_t s0 = a + b, s1 = a - b, s2 = c + d, s3 = c - d;
+        t[i * 4 + 0] = s0 + s2;
+        t[i * 4 + 1] = s1 + s3;
+        t[i * 4 + 2] = s0 - s2;
+        t[i * 4 + 3] = s1 - s3;
+    }
+    for (int j = 0; j < 4; j++) {
+        int32_t a = t[j], b = t[4 + j], c = t[8 + j], d = t[12 + j];
+        int32_t s0 = a + b, s1 = a - b, s2 = c + d, s3 = c - d;
+        out[j] = s0 + s2;
+        out[4 + j] = s1 + s3;
+        out[8 + j] = s0 - s2;
+        out[12 + j] = s1 - s3;
+    }
+}
+
+int32_t evo_block_satd(const int16_t *a, const int16_t *b)
+{
+    /* analysis.h:89-116: the 4x4 difference is stored as int16. */
+    int32_t satd = 0;
+    int16_t diff[16];
+    int32_t coef[16];
+    for (int by = 0; by < 16; by += 4)
+        for (int bx = 0; bx < 16; bx += 4) {
+            for (int r = 0; r < 4; r++)
+                for (int c = 0; c < 4; c++)
+                    diff[r * 4 + c] = (int16_t)(a[(by + r) * 16 + bx + c] - b[(by + r) * 16 + bx + c]);
+            evo_hadamard_4x4(diff, 4, coef);
+            for (int k = 0; k < 16; k++) satd += abs32(coef[k]);
+        }
+    return (satd + 1) >> 1;
+}
+
+int32_t evo_block_sad(const int16_t *a, const int16_t *b)
+{
+    /* analysis.h:119-132: luma only. */
+    int32_t sad = 0;
+    for (int k = 0; k < 256; k++) sad += abs32(a[k] - b[k]);
+    return sad;
+}
+
+int32_t evo_block_mad(const int16_t *a, const int16_t *b)
+{
+    /* analysis.h:180-202: luma then U/V, running maximum. */
+    int32_t mad = 0;
+    for (int k = 0; k < 256; k++) {
+        int32_t t = abs32(a[k] - b[k]);
+        if (t > mad) mad = t;
+    }
+    for (int k = 0; k < 64; k++) {
+        int32_t tu = abs32(a[256 + k] - b[256 + k]);
+        int32_t tv = abs32(a[320 + k] - b[320 + k]);
+        if (tu > mad) mad = tu;
+        if (tv > mad) mad = tv;
+    }
+    return mad;
+}
+
+/* --------------------------------------------------------- mv predictor -- */
+
+/* motion.cpp:263-269 */
+static int16_t median3(int16_t a, int16_t b, int16_t c)
+{
+    if (a > b) { int16_t t = a; a = b; b = t; }
+    if (b > c) b = c;
+    if (a > b) b = a;
+    return b;
+}
+
+void evo_mv_predictor(const evo_record *table, uint32_t wb, uint32_t idx, const evo_record *prev,
+                      int16_t *pred_x, int16_t *pred_y)
+{
+    /* motion.cpp:271-346 */
+    uint32_t bx = idx % wb, by = idx / wb;
+    int16_t mx[3], my[3];
+    int n = 0;
+    if (bx > 0 && (table[idx - 1].block_type & EVO_BT_MOTION)) {
+        mx[n] = table[idx - 1].motion_x; my[n] = table[idx - 1].motion_y; n++;
+    }
+    if (by > 0 && (table[idx - wb].block_type & EVO_BT_MOTION)) {
+        mx[n] = table[idx - wb].motion_x; my[n] = table[idx - wb].motion_y; n++;
+    }
+    if (by > 0 && bx + 1 < wb && (table[idx - wb + 1].block_type & EVO_BT_MOTION)) {
+        mx[n] = table[idx - wb + 1].motion_x; my[n] = table[idx - wb + 1].motion_y; n++;
+    } else if (by > 0 && bx > 0 && (table[idx - wb - 1].block_type & EVO_BT_MOTION)) {
+        mx[n] = table[idx - wb - 1].motion_x; my[n] = table[idx - wb - 1].motion_y; n++;
+    }
+    if (n < 3 && prev && (prev[idx].block_type & EVO_BT_MOTION)) {
+        mx[n] = prev[idx].motion_x; my[n] = prev[idx].motion_y; n++;
+    }
+    if (n == 0) {
+        *pred_x = 0; *pred_y = 0;
+    } else if (n == 1) {
+        *pred_x = mx[0]; *pred_y = my[0];
+    } else if (n == 2) {
+        *pred_x = (int16_t)((mx[0] + mx[1]) / 2);
+        *pred_y = (int16_t)((my[0] + my[1]) / 2);
+    } else {
+        *pred_x = median3(mx[0], mx[1], mx[2]);
+        *pred_y = median3(my[0], my[1], my[2]);
+    }
+}
+
+/* --------------------------------------------------------- inter search -- */
+
+typedef struct search_state {
+    /* evx_motion_selection, motion.cpp:53-66 */
+    int32_t best_x, best_y, best_sad, best_mad, best_ssd;
+    float best_cost;
+    int32_t sp_index, sp_amount, sp_enabled;
+    /* evx_prediction_params, motion.cpp:43-51 */
+    const evo_planes *ref;
+    const int16_t *src;
+    int32_t px, py, thresh;
+    float lambda;
+    int32_t n_int, n_axial, n_diag;
+} search_state;
+
+/* motion.cpp:125-178.  `counted` is 0 for the re-evaluated centre of a
+ * search level, whose outcome is fixed (it is the level's starting best), so
+ * the work counter n_int only prices candidates that can change the state. */
+static void eval_candidate(search_state *s, int32_t cx, int32_t cy, int counted)
+{
+    int16_t blk[EVO_MB_SAMPLES];
+    evo_load_block(s->ref, cx, cy, blk);
+    s->n_int += counted;
+    int32_t sad = evo_block_sad(s->src, blk);
+    int32_t ssd = (cx - s->px) * (cx - s->px) + (cy - s->py) * (cy - s->py);
+    int32_t mad = evo_block_mad(s->src, blk);
+    float cost = (float)sad;
+    if (s->lambda > 0.0f) {
+        int16_t mvx = (int16_t)(cx - s->px), mvy = (int16_t)(cy - s->py);
+        int32_t satd = evo_block_satd(s->src, blk);
+        float bits = (float)(abs16(mvx) + abs16(mvy) + 1);   /* motion.cpp:120-123 */
+        float lb = s->lambda * bits;                            /* no FMA contraction */
+        cost = (float)satd + lb;
+    }
+    int accept;
+    if (s->best_mad < s->thresh) {
+        accept = mad < s->best_mad || (mad == s->best_mad && ssd < s->best_ssd);
+    } else {
+        /* motion.cpp:166-168, && binds tighter than || */
+        accept = (cost < s->best_cost ||
+                  ((cost == s->best_cost && ssd < s->best_ssd) && sad < EVO_SAD_THRESHOLD)) ||
+                 mad < s->thresh;
+    }
+    if (accept) {
+        s->best_x = cx; s->best_y = cy;
+        s->best_sad = sad; s->best_ssd = ssd; s->best_mad = mad;
+        s->best_cost = cost;
+    }
+}
+
+/* motion.cpp:381-406 */
+static void level_search(search_state *s, int32_t step)
+{
+    int32_t base_x = s->best_x, base_y = s->best_y;
+    int32_t max_x = s->ref->width - EVO_MB, max_y = s->ref->height - EVO_MB;
+    for (int32_t j = -step; j <= step; j += step)
+        for (int32_t i = -step; i <= step; i += step) {
+            int32_t cx = base_x + i, cy = base_y + j;
+            if (cx < 0 || cx > max_x || cy < 0 || cy > max_y) continue;
+            eval_candidate(s, cx, cy, i != 0 || j != 0);
+            if (s->best_mad < s->thresh) return;
+        }
+}
+
+/* motion.cpp:68-91 */
+static int32_t frac_index(int32_t i, int32_t j)
+{
+    i++, j++;
+    if (j == 0) return i;
+    if (j == 1) return i == 0 ? 3 : 4;
+    return i + 5;
+}
+
+/* motion.cpp:93-116 */
+static void frac_direction(int32_t idx, int32_t *dx, int32_t *dy)
+{
+    if (idx <= 2) { *dy = -1; *dx = idx - 1; }
+    else if (idx == 3) { *dx = -1; *dy = 0; }
+    else if (idx == 4) { *dx = 1; *dy = 0; }
+    else { *dy = 1; *dx = idx - 6; }
+}
+
+/* motion.cpp:180-261, one sub-pel candidate (half when quarter == 0). */
+static void subpel_accept(search_state *s, int32_t sad, int32_t mad, int32_t i, int32_t j,
+                          int32_t quarter)
+{
+    int accept;
+    if (s->best_mad < s->thresh)
+        accept = mad < s->best_mad;
+    else
+        accept = (sad < s->best_sad && sad < EVO_SAD_THRESHOLD) || mad < s->thresh;
+    if (accept) {
+        s->sp_enabled = 1;
+        s->sp_amount = quarter;
+        s->sp_index = frac_index(i, j);
+        s->best_sad = sad;
+        s->best_mad = mad;
+    }
+}
+
+/* motion.cpp:450-483 */
+static void subpel_search(search_state *s)
+{
+    int16_t blk[EVO_MB_SAMPLES];
+    int32_t max_x = s->ref->width - EVO_MB, max_y = s->ref->height - EVO_MB;
+    s->sp_index = 0; s->sp_amount = 0; s->sp_enabled = 0;
+    for (int32_t j = -1; j <= 1; j++)
+        for (int32_t i = -1; i <= 1; i++) {
+            int32_t tx = s->best_x + i, ty = s->best_y + j;
+            if (i == 0 && j == 0) continue;
+            if (tx < 0 || tx > max_x || ty < 0 || ty > max_y) continue;
+            if (i != 0 && j != 0) s->n_diag++; else s->n_axial++;
+            for (int32_t q = 0; q < 2; q++) {
+                evo_interp_block(s->ref, s->best_x, s->best_y, i, j, q, blk);
+                int32_t sad = evo_block_sad(s->src, blk);
+                int32_t mad = evo_block_mad(s->src, blk);
+                subpel_accept(s, sad, mad, i, j, q);
+            }
+        }
+}
+
+int32_t evo_inter_search(const int16_t src[EVO_MB_SAMPLES], const evo_planes *ref, int32_t x,
+                         int32_t y, int32_t quality, int16_t pred_mx, int16_t pred_my,
+                         evo_search_result *res)
+{
+    /* motion.cpp:567-679 */
+    search_state s;
+    memset(&s, 0, sizeof(s));
+    s.ref = ref; s.src = src; s.px = x; s.py = y;
+    s.thresh = (quality >> 2) + 1;
+    s.lambda = evo_lambda(quality);
+    s.best_x = x; s.best_y = y;
+    s.best_ssd = EVO_INT32_MAX;
+
+    /* Initial candidate at (x, y), motion.cpp:600-611. */
+    int16_t blk[EVO_MB_SAMPLES];
+    evo_load_block(ref, x, y, blk);
+    s.n_int = 1;
+    s.best_sad = evo_block_sad(src, blk);
+    s.best_mad = evo_block_mad(src, blk);
+    s.best_cost = (float)s.best_sad;
+    if (s.lambda > 0.0f) {
+        float lb = s.lambda * 1.0f;
+        s.best_cost = (float)evo_block_satd(src, blk) + lb;
+    }
+
+    /* Predictor candidate, motion.cpp:613-635. */
+    if (pred_mx != 0 || pred_my != 0) {
+        int32_t cx = x + pred_mx, cy = y + pred_my;
+        int32_t max_x = ref->width - EVO_MB, max_y = ref->height - EVO_MB;
+        if (cx < 0) cx = 0;
+        if (cx > max_x) cx = max_x;
+        if (cy < 0) cy = 0;
+        if (cy > max_y) cy = max_y;
+        eval_candidate(&s, cx, cy, 1);
+    }
+
+    /* motion.cpp:638-655 */
+    if (s.best_mad >= s.thresh) {
+        int32_t radius = EVO_SEARCH_RADIUS;
+        if (s.best_sad < EVO_SAD_THRESHOLD / 4) radius = EVO_SEARCH_RADIUS / 4;
+        else if (s.best_sad < EVO_SAD_THRESHOLD / 2) radius = EVO_SEARCH_RADIUS / 2;
+        for (int32_t i = radius; i > 0; i >>= 1) {
+            level_search(&s, i);
+            if (s.best_mad < s.thresh) break;
+        }
+        subpel_search(&s);
+    }
+
+    /* motion.cpp:657-678 */
+    int32_t bt = 0;
+    if (s.best_x != x || s.best_y != y || s.sp_enabled) bt |= EVO_BT_MOTION;
+    if (s.best_mad < s.thresh) bt |= EVO_BT_COPY;
+    res->best_x = s.best_x; res->best_y = s.best_y;
+    res->sad = s.best_sad; res->mad = s.best_mad; res->ssd = s.best_ssd;
+    memcpy(&res->cost_bits, &s.best_cost, 4);
+    res->sp_enabled = s.sp_enabled; res->sp_amount = s.sp_amount; res->sp_index = s.sp_index;
+    res->block_type = bt;
+    res->motion_x = s.best_x - x; res->motion_y = s.best_y - y;
+    res->n_int = s.n_int; res->n_axial = s.n_axial; res->n_diag = s.n_diag;
+    return s.best_sad;
+}
+
+void evo_predict_block(const evo_planes *ref, int32_t x, int32_t y, const evo_search_result *r,
+                       int16_t out[EVO_MB_SAMPLES])
+{
+    /* encode.cpp:696-723 (and motion.cpp:729-776 for the bidir halves) */
+    int32_t bx = x + r->motion_x, by = y + r->motion_y;
+    if (r->sp_enabled) {
+        int32_t dx, dy;
+        frac_direction(r->sp_index, &dx, &dy);
+        evo_interp_block(ref, bx, by, dx, dy, r->sp_amount, out);
+    } else {
+        evo_load_block(ref, bx, by, out);
+    }
+}
+
+int32_t evo_bidir_average(const int16_t src[EVO_MB_SAMPLES], const int16_t fwd[EVO_MB_SAMPLES],
+                          const int16_t bwd[EVO_MB_SAMPLES], int16_t avg[EVO_MB_SAMPLES])
+{
+    /* motion.cpp:778-813: SAD over Y, U and V against the int32 average. */
+    int32_t sad = 0;
+    for (int k = 0; k < EVO_MB_SAMPLES; k++) {
+        int32_t a = (fwd[k] + bwd[k] + 1) >> 1;
+        avg[k] = (int16_t)a;
+        sad += abs32(src[k] - a);
+    }
+    return sad;
+}
+
+/* ---------------------------------------------------------- transform -- */
+
+/* transform.cpp:264-284 */
+static void dct_line(const int16_t *src, int32_t sp, int16_t *dst, int32_t dp)
+{
+    for (int i = 0; i < 8; i++) {
+        int32_t t = 0;
+        for (int k = 0; k < 8; k++) t += src[k * sp] * kDct8[i * 8 + k];
+        t = (i == 0) ? (t * 45) / 128 : t / 2;
+        t = rounded_div(t, 128);
+        dst[i * dp] = (int16_t)t;
+    }
+}
+
+void evo_sub_transform_8x8(const int16_t *src, int32_t src_pitch, const int16_t *sub,
+                           int32_t sub_pitch, int16_t *dst, int32_t dst_pitch)
+{
+    /* transform.cpp:435-452 (difference stored as int16, transform.cpp:12-16) */
+    int16_t diff[64], tmp[64];
+    for (int j = 0; j < 8; j++) {
+        for (int i = 0; i < 8; i++)
+            diff[j * 8 + i] = (int16_t)(src[j * src_pitch + i] - sub[j * sub_pitch + i]);
+        dct_line(diff + j * 8, 1, tmp + j * 8, 1);
+    }
+    for (int j = 0; j < 8; j++) dct_line(tmp + j, 8, dst + j, dst_pitch);
+}
+
+void evo_sub_transform_mb(const int16_t src[EVO_MB_SAMPLES], const int16_t pred[EVO_MB_SAMPLES],
+                          int16_t coeffs[EVO_MB_SAMPLES])
+{
+    /* macroblock.h:272-277, transform.cpp:584-594 */
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        evo_sub_transform_8x8(src + off, 16, pred + off, 16, coeffs + off, 16);
+    }
+    evo_sub_transform_8x8(src + 256, 8, pred + 256, 8, coeffs + 256, 8);
+    evo_sub_transform_8x8(src + 320, 8, pred + 320, 8, coeffs + 320, 8);
+}
+
+/* ------------------------------------------------------- quantisation -- */
+
+int32_t evo_variance2(const int16_t coeffs[EVO_MB_SAMPLES])
+{
+    /* analysis.h:253-275; the int32 sums wrap on overflow. */
+    int32_t sum = 0, count = 0;
+    uint32_t sumsq = 0;
+    for (int k = 1; k < 256; k++) {
+        int32_t t = coeffs[k];
+        if (t) {
+            sum += t;
+            sumsq += (uint32_t)(t * t);
+            count++;
+        }
+    }
+    if (count == 0) return 0;
+    /* rounded_div(sum * sum, count): the square can only be negative after a
+     * (undefined) int32 overflow, so the reference's optimised build drops
+     * rounded_div's sign test and always rounds as for a non-negative
+     * numerator: (sq + count / 2) / count on the wrapped square.  Pinned by
+     * the golden vectors written by the reference's -O3 build. */
+    int32_t sq = (int32_t)((uint32_t)sum * (uint32_t)sum);
+    int32_t rd = (int32_t)((uint32_t)sq + (uint32_t)(count / 2)) / count;
+    return (int32_t)(sumsq - (uint32_t)rd);
+}
+
+int32_t evo_block_qp(int32_t quality, const int16_t coeffs[EVO_MB_SAMPLES])
+{
+    /* quantize.cpp:97-118 */
+    uint32_t variance = (uint32_t)evo_variance2(coeffs);
+    int32_t index = clip_i32((int32_t)(floor_log2_u32(variance) >> 1), 1, EVO_MAX_QUANT_LEVELS - 1);
+    int32_t q = (uint8_t)quality;
+    int32_t delta = (int16_t)(index - q) >> 1;
+    if (delta > 6) delta = 6;
+    if (delta < -6) delta = -6;
+    return clip_i32(q + delta, 1, EVO_MAX_QUANT_LEVELS - 1);
+}
+
+/* quantize.cpp:217-252 */
+static void quantize_inter_8x8(int32_t qp, const int16_t *src, int32_t sp, int16_t *dst, int32_t dp)
+{
+    for (int j = 0; j < 8; j++)
+        for (int k = 0; k < 8; k++) {
+            int16_t sv = src[j * sp + k];
+            int16_t qf = (int16_t)rounded_div(sv * EVO_QUANT_SCALE, kInterQm[j * 8 + k]);
+            int32_t w = kDeadzone[j * 8 + k];
+            if (qp >= 16) w = w + 1 < 6 ? w + 1 : 6;
+            int16_t dz = (int16_t)((sign16(qf) * qp * w) >> 1);
+            dst[j * dp + k] = (int16_t)rounded_div(qf - dz, qp << 1);
+        }
+}
+
+void evo_quantize_inter_mb(int32_t qp, const int16_t coeffs[EVO_MB_SAMPLES],
+                           int16_t levels[EVO_MB_SAMPLES])
+{
+    /* quantize.cpp:372-397 */
+    int32_t cqp = qp + EVO_CHROMA_QP_OFFSET;
+    if (cqp > EVO_MAX_QUANT_LEVELS - 1) cqp = EVO_MAX_QUANT_LEVELS - 1;
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        quantize_inter_8x8(qp, coeffs + off, 16, levels + off, 16);
+    }
+    quantize_inter_8x8(cqp, coeffs + 256, 8, levels + 256, 8);
+    quantize_inter_8x8(cqp, coeffs + 320, 8, levels + 320, 8);
+}
+
+/* ------------------------------------------------------- frame driver -- */
+
+static void fill_primary(evo_record *rec, const evo_search_result *r)
+{
+    rec->block_type = (uint8_t)r->block_type;
+    rec->motion_x = (int16_t)r->motion_x;
+    rec->motion_y = (int16_t)r->motion_y;
+    rec->sp_pred = (uint8_t)r->sp_enabled;
+    rec->sp_amount = (uint8_t)r->sp_amount;
+    rec->sp_index = (uint8_t)r->sp_index;
+    rec->mad = r->mad;
+    rec->cost_bits = r->cost_bits;
+}
+
+int32_t evo_analyze_frame(const evo_frame_job *job, evo_record *records, int16_t *coeffs,
+                          int16_t *levels)
+{
+    int32_t wb = job->src.width / EVO_MB, hb = job->src.height / EVO_MB;
+    float lambda = evo_lambda(job->quality);
+    int16_t src[EVO_MB_SAMPLES], pred[EVO_MB_SAMPLES], pb[EVO_MB_SAMPLES], tmp[EVO_MB_SAMPLES];
+    evo_search_result r[EVO_MAX_REFS];
+
+    for (int32_t by = 0; by < hb; by++)
+        for (int32_t bx = 0; bx < wb; bx++) {
+            uint32_t idx = (uint32_t)(by * wb + bx);
+            int32_t x = bx * EVO_MB, y = by * EVO_MB;
+            evo_record rec;
+            memset(&rec, 0, sizeof(rec));
+            evo_load_block(&job->src, x, y, src);
+
+            /* The temporal candidate exists for P frames only (encode.cpp:393, 414). */
+            int16_t pmx, pmy;
+            evo_mv_predictor(records, (uint32_t)wb, idx, job->type == 1 ? job->prev : NULL, &pmx,
+                             &pmy);
+
+            int32_t best_sad = EVO_INT32_MAX, best = -1, bidir = 0;
+            uint32_t n_int = 0, n_ax = 0, n_dg = 0;
+            if (job->type == 1) {
+                /* encode.cpp:215-247 against an empty sentinel (non-copy,
+                 * SAD INT32_MAX); with one reference this is 183-210. */
+                int32_t best_copy = 0;
+                for (int32_t k = 0; k < job->n_refs; k++) {
+                    int32_t sad = evo_inter_search(src, &job->refs[k], x, y, job->quality, pmx,
+                                                   pmy, &r[k]);
+                    n_int += r[k].n_int; n_ax += r[k].n_axial; n_dg += r[k].n_diag;
+                    int32_t bias = (int32_t)(lambda * (float)((k + 1) * 2));
+                    int32_t copy = (r[k].block_type & EVO_BT_COPY) != 0;
+                    if (copy ^ best_copy) {
+                        if (copy) { best = k; best_sad = sad; best_copy = 1; }
+                    } else if (sad + bias < best_sad) {
+                        best = k; best_sad = sad; best_copy = copy;
+                    }
+                }
+                if (best < 0 || best_sad == EVO_INT32_MAX) return 10;
+                fill_primary(&rec, &r[best]);
+                rec.prediction_mode = 0;
+                rec.prediction_target = job->target_ids[best];
+                rec.sad = best_sad;
+            } else {
+                /* encode.cpp:386-443 without the intra candidates. */
+                int32_t fs = evo_inter_search(src, &job->refs[0], x, y, job->quality, pmx, pmy, &r[0]);
+                int32_t bs = evo_inter_search(src, &job->refs[1], x, y, job->quality, pmx, pmy, &r[1]);
+                n_int = r[0].n_int + r[1].n_int;
+                n_ax = r[0].n_axial + r[1].n_axial;
+                n_dg = r[0].n_diag + r[1].n_diag;
+                if (fs < best_sad) { best = 0; best_sad = fs; }
+                if (bs < best_sad) { best = 1; best_sad = bs; }
+                /* motion.cpp:702-840 reuses the two searches above. */
+                evo_predict_block(&job->refs[0], x, y, &r[0], pred);
+                evo_predict_block(&job->refs[1], x, y, &r[1], pb);
+                int32_t bi_sad = evo_bidir_average(src, pred, pb, tmp);
+                int32_t bias = (int32_t)(lambda * 8.0f);
+                if (bi_sad + bias < best_sad) { bidir = 1; best_sad = bi_sad; }
+                if (best_sad == EVO_INT32_MAX) return 10;
+                if (bidir) {
+                    fill_primary(&rec, &r[0]);
+                    rec.block_type = EVO_BT_MOTION;   /* EVX_BLOCK_INTER_MOTION_DELTA */
+                    rec.prediction_mode = 2;
+                    rec.prediction_target = job->target_ids[0];
+                    rec.prediction_target_b = job->target_ids[1];
+                    rec.motion_x_b = (int16_t)r[1].motion_x;
+                    rec.motion_y_b = (int16_t)r[1].motion_y;
+                    rec.sp_pred_b = (uint8_t)r[1].sp_enabled;
+                    rec.sp_amount_b = (uint8_t)r[1].sp_amount;
+                    rec.sp_index_b = (uint8_t)r[1].sp_index;
+                    rec.sad_b = r[1].sad;
+                    rec.mad_b = r[1].mad;
+                    rec.cost_b_bits = r[1].cost_bits;
+                    memcpy(pred, tmp, sizeof(tmp));
+                } else {
+                    fill_primary(&rec, &r[best]);
+                    rec.prediction_mode = (uint8_t)best;
+                    rec.prediction_target = job->target_ids[best];
+                }
+                rec.sad = best_sad;
+            }
+            rec.n_int = n_int;
+            rec.n_axial = (uint16_t)n_ax;
+            rec.n_diag = (uint16_t)n_dg;
+
+            int16_t *co = coeffs + (size_t)idx * EVO_MB_SAMPLES;
+            int16_t *lv = levels + (size_t)idx * EVO_MB_SAMPLES;
+            if (rec.block_type & EVO_BT_COPY) {
+                /* encode.cpp:742-744: copies are neither transformed nor quantised. */
+                memset(co, 0, EVO_MB_SAMPLES * sizeof(int16_t));
+                memset(lv, 0, EVO_MB_SAMPLES * sizeof(int16_t));
+            } else {
+                if (!bidir) {
+                    const evo_planes *ref = &job->refs[job->type == 1 ? best : best];
+                    evo_predict_block(ref, x, y, &r[best], pred);
+                }
+                /* encode.cpp:717-728 */
+                evo_sub_transform_mb(src, pred, co);
+                int32_t qp = evo_block_qp(job->quality, co);
+                rec.q_index = (uint8_t)qp;
+                rec.variance = (int16_t)evo_variance2(co);
+                evo_quantize_inter_mb(qp, co, lv);
+                rec.flags = 1u | (bidir ? 2u : 0u);
+            }
+            records[idx] = rec;
+        }
+    return 0;
+}

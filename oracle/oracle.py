@@ -1,0 +1,322 @@
+"""ctypes bindings for the oracle (liboracle.so) and the reference build
+(_ref/libevxref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker of the HIP path.  Planes are numpy
+int16 arrays; MB-order blocks are int16[384] (16x16 Y, 8x8 U, 8x8 V).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libevxref.so")
+
+I16P = C.POINTER(C.c_int16)
+I32P = C.POINTER(C.c_int32)
+
+
+class Planes(C.Structure):
+    _fields_ = [("y", I16P), ("u", I16P), ("v", I16P), ("width", C.c_int32),
+                ("height", C.c_int32), ("stride_y", C.c_int32), ("stride_c", C.c_int32)]
+
+
+class SearchResult(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("best_x", "best_y", "sad", "mad", "ssd")] + [
+        ("cost_bits", C.c_uint32)] + [(n, C.c_int32) for n in (
+            "sp_enabled", "sp_amount", "sp_index", "block_type", "motion_x", "motion_y",
+            "n_int", "n_axial", "n_diag")]
+
+
+# Record layout shared with evxa_block_record (64 bytes).
+RECORD_DTYPE = np.dtype([
+    ("block_type", "u1"), ("prediction_mode", "u1"), ("prediction_target", "u1"),
+    ("prediction_target_b", "u1"), ("motion_x", "<i2"), ("motion_y", "<i2"),
+    ("motion_x_b", "<i2"), ("motion_y_b", "<i2"), ("sp_pred", "u1"), ("sp_amount", "u1"),
+    ("sp_index", "u1"), ("sp_pred_b", "u1"), ("sp_amount_b", "u1"), ("sp_index_b", "u1"),
+    ("q_index", "u1"), ("reserved0", "u1"), ("variance", "<i2"), ("reserved1", "<i2"),
+    ("sad", "<i4"), ("mad", "<i4"), ("cost_bits", "<u4"), ("sad_b", "<i4"), ("mad_b", "<i4"),
+    ("cost_b_bits", "<u4"), ("n_int", "<u4"), ("n_axial", "<u2"), ("n_diag", "<u2"),
+    ("flags", "<u4"), ("reserved2", "<u4")])
+assert RECORD_DTYPE.itemsize == 64
+
+# Fields the reference itself exposes (evx_block_desc + selection SAD).
+REF_FIELDS = ("block_type", "prediction_mode", "prediction_target", "prediction_target_b",
+              "motion_x", "motion_y", "motion_x_b", "motion_y_b", "sp_pred", "sp_amount",
+              "sp_index", "sp_pred_b", "sp_amount_b", "sp_index_b", "q_index", "variance",
+              "sad", "sad_b", "flags")
+
+
+class FrameJob(C.Structure):
+    _fields_ = [("type", C.c_int32), ("quality", C.c_int32), ("src", Planes),
+                ("n_refs", C.c_int32), ("refs", Planes * 4), ("target_ids", C.c_uint8 * 4),
+                ("prev", C.c_void_p)]
+
+
+def _ptr16(a):
+    assert a.dtype == np.int16 and a.flags.c_contiguous
+    return a.ctypes.data_as(I16P)
+
+
+class Picture:
+    """A 4:2:0 int16 picture; keeps the arrays alive for the Planes struct."""
+
+    def __init__(self, y, u, v):
+        self.y = np.ascontiguousarray(y, dtype=np.int16)
+        self.u = np.ascontiguousarray(u, dtype=np.int16)
+        self.v = np.ascontiguousarray(v, dtype=np.int16)
+        h, w = self.y.shape
+        assert w % 16 == 0 and h % 16 == 0
+        assert self.u.shape == (h // 2, w // 2) and self.v.shape == (h // 2, w // 2)
+        self.planes = Planes(_ptr16(self.y), _ptr16(self.u), _ptr16(self.v), w, h, w, w // 2)
+
+    @property
+    def width(self):
+        return self.y.shape[1]
+
+    @property
+    def height(self):
+        return self.y.shape[0]
+
+    def block(self, x, y):
+        out = np.empty(384, np.int16)
+        out[:256] = self.y[y:y + 16, x:x + 16].ravel()
+        out[256:320] = self.u[y // 2:y // 2 + 8, x // 2:x // 2 + 8].ravel()
+        out[320:] = self.v[y // 2:y // 2 + 8, x // 2:x // 2 + 8].ravel()
+        return out
+
+
+class _Lib:
+    prefix = ""
+
+    def __init__(self, path):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+
+    def f(self, name, restype, *argtypes):
+        fn = getattr(self.lib, self.prefix + name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+        return fn
+
+
+class Oracle(_Lib):
+    """The C restatement."""
+    prefix = "evo_"
+
+    def __init__(self, path=ORACLE_SO):
+        super().__init__(path)
+        P = C.POINTER(Planes)
+        self._lambda = self.f("lambda", C.c_float, C.c_int32)
+        self._interp = self.f("interp_block", None, P, C.c_int32, C.c_int32, C.c_int32,
+                              C.c_int32, C.c_int32, I16P)
+        self._had = self.f("hadamard_4x4", None, I16P, C.c_int32, I32P)
+        self._sad = self.f("block_sad", C.c_int32, I16P, I16P)
+        self._mad = self.f("block_mad", C.c_int32, I16P, I16P)
+        self._satd = self.f("block_satd", C.c_int32, I16P, I16P)
+        self._pred = self.f("mv_predictor", None, C.c_void_p, C.c_uint32, C.c_uint32,
+                            C.c_void_p, I16P, I16P)
+        self._search = self.f("inter_search", C.c_int32, I16P, P, C.c_int32, C.c_int32,
+                              C.c_int32, C.c_int16, C.c_int16, C.POINTER(SearchResult))
+        self._predict = self.f("predict_block", None, P, C.c_int32, C.c_int32,
+                               C.POINTER(SearchResult), I16P)
+        self._bidir = self.f("bidir_average", C.c_int32, I16P, I16P, I16P, I16P)
+        self._subt = self.f("sub_transform_mb", None, I16P, I16P, I16P)
+        self._var2 = self.f("variance2", C.c_int32, I16P)
+        self._qp = self.f("block_qp", C.c_int32, C.c_int32, I16P)
+        self._quant = self.f("quantize_inter_mb", None, C.c_int32, I16P, I16P)
+        self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
+                             I16P, I16P)
+
+    def lam(self, q):
+        return self._lambda(q)
+
+    def interp(self, pic, bx, by, dx, dy, quarter):
+        out = np.empty(384, np.int16)
+        self._interp(C.byref(pic.planes), bx, by, dx, dy, int(quarter), _ptr16(out))
+        return out
+
+    def hadamard(self, blk4x4):
+        b = np.ascontiguousarray(blk4x4, np.int16).ravel()
+        out = np.empty(16, np.int32)
+        self._had(_ptr16(b), 4, out.ctypes.data_as(I32P))
+        return out
+
+    def metrics(self, a, b):
+        a = np.ascontiguousarray(a, np.int16)
+        b = np.ascontiguousarray(b, np.int16)
+        return (self._sad(_ptr16(a), _ptr16(b)), self._mad(_ptr16(a), _ptr16(b)),
+                self._satd(_ptr16(a), _ptr16(b)))
+
+    def mv_predictor(self, table, wb, idx, prev=None):
+        px, py = C.c_int16(), C.c_int16()
+        self._pred(table.ctypes.data, wb, idx, None if prev is None else prev.ctypes.data,
+                   C.byref(px), C.byref(py))
+        return px.value, py.value
+
+    def inter_search(self, src_blk, ref, x, y, quality, pmx, pmy):
+        r = SearchResult()
+        s = np.ascontiguousarray(src_blk, np.int16)
+        self._search(_ptr16(s), C.byref(ref.planes), x, y, quality, pmx, pmy, C.byref(r))
+        return r
+
+    def predict(self, ref, x, y, r):
+        out = np.empty(384, np.int16)
+        self._predict(C.byref(ref.planes), x, y, C.byref(r), _ptr16(out))
+        return out
+
+    def bidir(self, src, f, b):
+        avg = np.empty(384, np.int16)
+        src, f, b = (np.ascontiguousarray(a, np.int16) for a in (src, f, b))
+        sad = self._bidir(_ptr16(src), _ptr16(f), _ptr16(b), _ptr16(avg))
+        return sad, avg
+
+    def sub_transform(self, src, pred):
+        out = np.empty(384, np.int16)
+        src, pred = (np.ascontiguousarray(a, np.int16) for a in (src, pred))
+        self._subt(_ptr16(src), _ptr16(pred), _ptr16(out))
+        return out
+
+    def variance2(self, coeffs):
+        c = np.ascontiguousarray(coeffs, np.int16)
+        return self._var2(_ptr16(c))
+
+    def block_qp(self, quality, coeffs):
+        c = np.ascontiguousarray(coeffs, np.int16)
+        return self._qp(quality, _ptr16(c))
+
+    def quantize(self, qp, coeffs):
+        c = np.ascontiguousarray(coeffs, np.int16)
+        out = np.empty(384, np.int16)
+        self._quant(qp, _ptr16(c), _ptr16(out))
+        return out
+
+    def analyze_frame(self, ftype, quality, src, refs, prev=None, target_ids=None):
+        return _run_frame(self._frame, ftype, quality, src, refs, prev, target_ids)
+
+
+def make_job(ftype, quality, src, refs, prev, target_ids):
+    job = FrameJob()
+    job.type = ftype
+    job.quality = quality
+    job.src = src.planes
+    job.n_refs = len(refs)
+    for k, r in enumerate(refs):
+        job.refs[k] = r.planes
+    ids = target_ids if target_ids is not None else (
+        list(range(1, len(refs) + 1)) if ftype == 1 else [0, 1])
+    for k, t in enumerate(ids):
+        job.target_ids[k] = t
+    job.prev = None if prev is None else prev.ctypes.data
+    return job
+
+
+def _run_frame(fn, ftype, quality, src, refs, prev, target_ids):
+    n = (src.width // 16) * (src.height // 16)
+    recs = np.zeros(n, RECORD_DTYPE)
+    coeffs = np.zeros((n, 384), np.int16)
+    levels = np.zeros((n, 384), np.int16)
+    job = make_job(ftype, quality, src, refs, prev, target_ids)
+    rc = fn(C.byref(job), recs.ctypes.data, _ptr16(coeffs), _ptr16(levels))
+    if rc != 0:
+        raise RuntimeError(f"analyze_frame failed: {rc}")
+    return recs, coeffs, levels
+
+
+class Reference(_Lib):
+    """The reference's own functions (oracle/_ref, built from its sources)."""
+    prefix = "evr_"
+
+    def __init__(self, path=REF_SO):
+        super().__init__(path)
+        P = C.POINTER(Planes)
+        self._lambda = self.f("lambda", C.c_float, C.c_int32)
+        self._interp_many = self.f("interp_many", None, P, C.c_int32, I32P, I32P, I32P, I32P,
+                                   I32P, I16P)
+        self._had = self.f("hadamard_4x4", None, I16P, C.c_int32, I32P)
+        self._metrics = self.f("block_metrics", None, I16P, I16P, I32P, I32P, I32P)
+        self._subt = self.f("sub_transform_mb", None, I16P, I16P, I16P)
+        self._var2 = self.f("variance2", C.c_int32, I16P)
+        self._qp = self.f("block_qp", C.c_int32, C.c_int32, I16P)
+        self._quant = self.f("quantize_mb", None, C.c_int32, C.c_int32, I16P, I16P)
+        self._pred = self.f("mv_predictor", None, C.c_void_p, C.c_uint32, C.c_uint32,
+                            C.c_uint32, C.c_void_p, I16P, I16P)
+        self._search = self.f("inter_search", C.c_int32, I16P, P, C.c_int32, C.c_int32,
+                              C.c_int32, C.c_int16, C.c_int16, C.c_void_p)
+        self._bidir = self.f("bidir", C.c_int32, I16P, P, P, C.c_int32, C.c_int32, C.c_int32,
+                             C.c_int16, C.c_int16, C.c_void_p, I16P)
+        self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
+                             I16P, I16P)
+
+    def lam(self, q):
+        return self._lambda(q)
+
+    def interp_many(self, pic, bx, by, dx, dy, quarter):
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (bx, by, dx, dy, quarter)]
+        n = len(arrs[0])
+        out = np.empty((n, 384), np.int16)
+        self._interp_many(C.byref(pic.planes), n, *[a.ctypes.data_as(I32P) for a in arrs],
+                          _ptr16(out))
+        return out
+
+    def hadamard(self, blk4x4):
+        b = np.ascontiguousarray(blk4x4, np.int16).ravel()
+        out = np.empty(16, np.int32)
+        self._had(_ptr16(b), 4, out.ctypes.data_as(I32P))
+        return out
+
+    def metrics(self, a, b):
+        a = np.ascontiguousarray(a, np.int16)
+        b = np.ascontiguousarray(b, np.int16)
+        s, m, t = C.c_int32(), C.c_int32(), C.c_int32()
+        self._metrics(_ptr16(a), _ptr16(b), C.byref(s), C.byref(m), C.byref(t))
+        return s.value, m.value, t.value
+
+    def sub_transform(self, src, pred):
+        out = np.empty(384, np.int16)
+        src, pred = (np.ascontiguousarray(a, np.int16) for a in (src, pred))
+        self._subt(_ptr16(src), _ptr16(pred), _ptr16(out))
+        return out
+
+    def variance2(self, coeffs):
+        return self._var2(_ptr16(np.ascontiguousarray(coeffs, np.int16)))
+
+    def block_qp(self, quality, coeffs):
+        return self._qp(quality, _ptr16(np.ascontiguousarray(coeffs, np.int16)))
+
+    def quantize(self, qp, coeffs, block_type=2):
+        out = np.empty(384, np.int16)
+        self._quant(qp, block_type, _ptr16(np.ascontiguousarray(coeffs, np.int16)), _ptr16(out))
+        return out
+
+    def mv_predictor(self, table, wb, idx, prev=None):
+        px, py = C.c_int16(), C.c_int16()
+        self._pred(table.ctypes.data, wb, len(table), idx,
+                   None if prev is None else prev.ctypes.data, C.byref(px), C.byref(py))
+        return px.value, py.value
+
+    def inter_search(self, src_blk, ref, x, y, quality, pmx, pmy):
+        rec = np.zeros(1, RECORD_DTYPE)
+        s = np.ascontiguousarray(src_blk, np.int16)
+        self._search(_ptr16(s), C.byref(ref.planes), x, y, quality, pmx, pmy, rec.ctypes.data)
+        return rec[0]
+
+    def bidir(self, src_blk, fwd, bwd, x, y, quality, pmx, pmy):
+        rec = np.zeros(1, RECORD_DTYPE)
+        avg = np.empty(384, np.int16)
+        s = np.ascontiguousarray(src_blk, np.int16)
+        self._bidir(_ptr16(s), C.byref(fwd.planes), C.byref(bwd.planes), x, y, quality, pmx, pmy,
+                    rec.ctypes.data, _ptr16(avg))
+        return rec[0], avg
+
+    def analyze_frame(self, ftype, quality, src, refs, prev=None, target_ids=None):
+        return _run_frame(self._frame, ftype, quality, src, refs, prev, target_ids)
+
+
+def reference_available():
+    return os.path.exists(REF_SO)

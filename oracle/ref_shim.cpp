@@ -1,0 +1,414 @@
+// ref_shim.cpp -- extern "C" entry points over the reference's own inter
+// analysis functions, compiled together with the reference sources into
+// oracle/_ref/libevxref.so by oracle/Makefile.
+//
+// TEST INFRASTRUCTURE ONLY: used to write golden vectors, to check the C
+// restatement (evx_oracle.c) on fresh inputs, and as the reference CPU
+// baseline.  Nothing in the product loads it.
+//
+// The per-block functions call the reference directly.  The frame drivers
+// compose reference calls the way the encoder does (encode.cpp:183-248 for
+// P, 386-443 for B, 569-739 for the residual prediction, 726-728 for QP and
+// quantisation) with the intra candidates replaced by an empty sentinel and
+// trellis left out (levels are pre-trellis).
+// <stdint.h> first: base.h includes it inside namespace evx (base.h:137).
+#include <stdint.h>
+
+#include "analysis.h"
+#include "common.h"
+#include "imageset.h"
+#include "interp.h"
+#include "macroblock.h"
+#include "motion.h"
+#include "quantize.h"
+#include "transform.h"
+
+#include <cstring>
+#include <vector>
+
+#include "evx_oracle.h"
+
+using namespace evx;
+
+namespace {
+
+void fill_set(image_set *set, const evo_planes &p)
+{
+    set->initialize(EVX_IMAGE_FORMAT_R16S, (uint16)p.width, (uint16)p.height);
+    int16 *y = reinterpret_cast<int16 *>(set->query_y_image()->query_data());
+    int16 *u = reinterpret_cast<int16 *>(set->query_u_image()->query_data());
+    int16 *v = reinterpret_cast<int16 *>(set->query_v_image()->query_data());
+    for (int r = 0; r < p.height; r++)
+        memcpy(y + r * p.width, p.y + r * p.stride_y, p.width * sizeof(int16));
+    int cw = p.width >> 1, ch = p.height >> 1;
+    for (int r = 0; r < ch; r++) {
+        memcpy(u + r * cw, p.u + r * p.stride_c, cw * sizeof(int16));
+        memcpy(v + r * cw, p.v + r * p.stride_c, cw * sizeof(int16));
+    }
+}
+
+macroblock view(int16_t *buf)
+{
+    macroblock m;
+    m.data_y = buf;
+    m.data_u = buf + 256;
+    m.data_v = buf + 320;
+    m.stride = 16;
+    return m;
+}
+
+void copy_out(const macroblock &m, int16_t *out)
+{
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) out[r * 16 + c] = m.data_y[r * m.stride + c];
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 8; c++) {
+            out[256 + r * 8 + c] = m.data_u[r * (m.stride >> 1) + c];
+            out[320 + r * 8 + c] = m.data_v[r * (m.stride >> 1) + c];
+        }
+}
+
+struct bank_holder {
+    evx_cache_bank bank;
+    bank_holder()
+    {
+        bank.transform_cache.initialize(EVX_IMAGE_FORMAT_R16S, 16, 16);
+        bank.motion_cache.initialize(EVX_IMAGE_FORMAT_R16S, 16, 16);
+        bank.staging_cache.initialize(EVX_IMAGE_FORMAT_R16S, 16, 16);
+        create_macroblock(bank.transform_cache, 0, 0, &bank.transform_block);
+        create_macroblock(bank.motion_cache, 0, 0, &bank.motion_block);
+        create_macroblock(bank.staging_cache, 0, 0, &bank.staging_block);
+    }
+};
+
+void desc_to_record(const evx_block_desc &d, evo_record *r)
+{
+    memset(r, 0, sizeof(*r));
+    r->block_type = (uint8_t)d.block_type;
+    r->prediction_mode = d.prediction_mode;
+    r->prediction_target = d.prediction_target;
+    r->prediction_target_b = d.prediction_target_b;
+    r->motion_x = d.motion_x;
+    r->motion_y = d.motion_y;
+    r->motion_x_b = d.motion_x_b;
+    r->motion_y_b = d.motion_y_b;
+    r->sp_pred = d.sp_pred;
+    r->sp_amount = d.sp_amount;
+    r->sp_index = d.sp_index;
+    r->sp_pred_b = d.sp_pred_b;
+    r->sp_amount_b = d.sp_amount_b;
+    r->sp_index_b = d.sp_index_b;
+    r->q_index = d.q_index;
+    r->variance = d.variance;
+}
+
+// Residual prediction for a chosen descriptor (encode.cpp:625-723), written
+// into `out` (an MB-order buffer).
+void form_prediction(evx_cache_bank *bank, const evx_block_desc &d, int32 x, int32 y,
+                     int16_t *out)
+{
+    if (d.prediction_mode == 2) {
+        macroblock fwd, bwd;
+        create_macroblock(bank->prediction_cache[d.prediction_target], x + d.motion_x,
+                          y + d.motion_y, &fwd);
+        const macroblock *fr = &fwd;
+        if (d.sp_pred) {
+            int16 i, j;
+            compute_motion_direction_from_frac_index(d.sp_index, &i, &j);
+            interpolate_subpixel_6tap(bank->prediction_cache[d.prediction_target], x + d.motion_x,
+                                      y + d.motion_y, i, j, d.sp_amount, &bank->motion_block);
+            fr = &bank->motion_block;
+        }
+        create_macroblock(bank->prediction_cache[d.prediction_target_b], x + d.motion_x_b,
+                          y + d.motion_y_b, &bwd);
+        const macroblock *br = &bwd;
+        if (d.sp_pred_b) {
+            int16 i, j;
+            compute_motion_direction_from_frac_index(d.sp_index_b, &i, &j);
+            interpolate_subpixel_6tap(bank->prediction_cache[d.prediction_target_b],
+                                      x + d.motion_x_b, y + d.motion_y_b, i, j, d.sp_amount_b,
+                                      &bank->transform_block);
+            br = &bank->transform_block;
+        }
+        for (int r = 0; r < 16; r++)
+            for (int c = 0; c < 16; c++)
+                out[r * 16 + c] = (int16)((fr->data_y[r * fr->stride + c] +
+                                           br->data_y[r * br->stride + c] + 1) >> 1);
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                uint32 fs = fr->stride >> 1, bs = br->stride >> 1;
+                out[256 + r * 8 + c] =
+                    (int16)((fr->data_u[r * fs + c] + br->data_u[r * bs + c] + 1) >> 1);
+                out[320 + r * 8 + c] =
+                    (int16)((fr->data_v[r * fs + c] + br->data_v[r * bs + c] + 1) >> 1);
+            }
+        return;
+    }
+    macroblock beta;
+    create_macroblock(bank->prediction_cache[d.prediction_target], x + d.motion_x,
+                      y + d.motion_y, &beta);
+    if (d.sp_pred) {
+        int16 i, j;
+        compute_motion_direction_from_frac_index(d.sp_index, &i, &j);
+        interpolate_subpixel_6tap(bank->prediction_cache[d.prediction_target], x + d.motion_x,
+                                  y + d.motion_y, i, j, d.sp_amount, &bank->motion_block);
+        copy_out(bank->motion_block, out);
+    } else {
+        copy_out(beta, out);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+float evr_lambda(int32_t quality) { return compute_lambda((uint16)quality); }
+
+void evr_interp_block(const evo_planes *ref, int32_t bx, int32_t by, int32_t dx, int32_t dy,
+                      int32_t quarter, int16_t *out)
+{
+    image_set set;
+    fill_set(&set, *ref);
+    std::vector<int16_t> buf(384);
+    macroblock m = view(buf.data());
+    interpolate_subpixel_6tap(set, bx, by, (int16)dx, (int16)dy, quarter != 0, &m);
+    memcpy(out, buf.data(), 384 * sizeof(int16_t));
+}
+
+// Batched interpolation over one picture: n positions/directions.
+void evr_interp_many(const evo_planes *ref, int32_t n, const int32_t *bx, const int32_t *by,
+                     const int32_t *dx, const int32_t *dy, const int32_t *quarter, int16_t *out)
+{
+    image_set set;
+    fill_set(&set, *ref);
+    std::vector<int16_t> buf(384);
+    for (int32_t k = 0; k < n; k++) {
+        macroblock m = view(buf.data());
+        interpolate_subpixel_6tap(set, bx[k], by[k], (int16)dx[k], (int16)dy[k], quarter[k] != 0,
+                                  &m);
+        memcpy(out + (size_t)k * 384, buf.data(), 384 * sizeof(int16_t));
+    }
+}
+
+void evr_hadamard_4x4(const int16_t *src, int32_t stride, int32_t *out)
+{
+    hadamard_4x4(src, stride, out);
+}
+
+void evr_block_metrics(const int16_t *a, const int16_t *b, int32_t *sad, int32_t *mad,
+                       int32_t *satd)
+{
+    macroblock ma = view(const_cast<int16_t *>(a)), mb = view(const_cast<int16_t *>(b));
+    *sad = compute_block_sad(ma, mb);
+    *mad = compute_block_mad(ma, mb);
+    *satd = compute_block_satd(ma, mb);
+}
+
+void evr_sub_transform_mb(const int16_t *src, const int16_t *pred, int16_t *coeffs)
+{
+    macroblock ms = view(const_cast<int16_t *>(src)), mp = view(const_cast<int16_t *>(pred));
+    macroblock mc = view(coeffs);
+    sub_transform_macroblock(ms, mp, &mc);
+}
+
+int32_t evr_variance2(const int16_t *coeffs)
+{
+    macroblock mc = view(const_cast<int16_t *>(coeffs));
+    return compute_block_variance2(mc);
+}
+
+int32_t evr_block_qp(int32_t quality, const int16_t *coeffs)
+{
+    macroblock mc = view(const_cast<int16_t *>(coeffs));
+    return query_block_quantization_parameter((uint8)quality, mc, EVX_BLOCK_INTER_MOTION_DELTA);
+}
+
+void evr_quantize_mb(int32_t qp, int32_t block_type, const int16_t *coeffs, int16_t *levels)
+{
+    macroblock mc = view(const_cast<int16_t *>(coeffs));
+    macroblock ml = view(levels);
+    quantize_macroblock((uint8)qp, (EVX_BLOCK_TYPE)block_type, mc, &ml);
+}
+
+void evr_mv_predictor(const evo_record *table, uint32_t wb, uint32_t n_blocks, uint32_t idx,
+                      const evo_record *prev, int16_t *px, int16_t *py)
+{
+    std::vector<evx_block_desc> t(n_blocks), p(n_blocks);
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        clear_block_desc(&t[k]);
+        t[k].block_type = (EVX_BLOCK_TYPE)table[k].block_type;
+        t[k].motion_x = table[k].motion_x;
+        t[k].motion_y = table[k].motion_y;
+        clear_block_desc(&p[k]);
+        if (prev) {
+            p[k].block_type = (EVX_BLOCK_TYPE)prev[k].block_type;
+            p[k].motion_x = prev[k].motion_x;
+            p[k].motion_y = prev[k].motion_y;
+        }
+    }
+    compute_mv_predictor(t.data(), wb, idx, prev ? p.data() : nullptr, px, py);
+}
+
+// perform_inter_search with the predictor supplied through a one-neighbour
+// block table (a single motion neighbour makes compute_mv_predictor return
+// its vector, motion.cpp:331-335).
+int32_t evr_inter_search(const int16_t *src, const evo_planes *ref, int32_t x, int32_t y,
+                         int32_t quality, int16_t pmx, int16_t pmy, evo_record *out)
+{
+    bank_holder h;
+    fill_set(&h.bank.prediction_cache[0], *ref);
+    evx_block_desc table[2];
+    clear_block_desc(&table[0]);
+    clear_block_desc(&table[1]);
+    if (pmx != 0 || pmy != 0) {
+        table[0].block_type = EVX_BLOCK_INTER_MOTION_DELTA;
+        table[0].motion_x = pmx;
+        table[0].motion_y = pmy;
+    } else {
+        table[0].block_type = EVX_BLOCK_INTER_DELTA;
+    }
+    evx_frame frame;
+    clear_frame(&frame);
+    frame.type = EVX_FRAME_INTER;
+    frame.quality = (uint16)quality;
+    std::vector<int16_t> s(src, src + 384);
+    macroblock ms = view(s.data());
+    evx_block_desc d;
+    int32 sad = perform_inter_search(frame, ms, x, y, &h.bank, 0, table, 2, 1, nullptr, &d);
+    desc_to_record(d, out);
+    out->sad = sad;
+    return sad;
+}
+
+int32_t evr_bidir(const int16_t *src, const evo_planes *fwd, const evo_planes *bwd, int32_t x,
+                  int32_t y, int32_t quality, int16_t pmx, int16_t pmy, evo_record *out,
+                  int16_t *avg)
+{
+    bank_holder h;
+    fill_set(&h.bank.prediction_cache[0], *fwd);
+    fill_set(&h.bank.prediction_cache[1], *bwd);
+    evx_block_desc table[2];
+    clear_block_desc(&table[0]);
+    clear_block_desc(&table[1]);
+    if (pmx != 0 || pmy != 0) {
+        table[0].block_type = EVX_BLOCK_INTER_MOTION_DELTA;
+        table[0].motion_x = pmx;
+        table[0].motion_y = pmy;
+    }
+    evx_frame frame;
+    clear_frame(&frame);
+    frame.type = EVX_FRAME_BIDIR;
+    frame.quality = (uint16)quality;
+    std::vector<int16_t> s(src, src + 384);
+    macroblock ms = view(s.data());
+    evx_block_desc d;
+    int32 sad = calculate_bidir_prediction(frame, ms, x, y, &h.bank, 0, 1, table, 2, 1, &d);
+    desc_to_record(d, out);
+    out->sad = sad;
+    copy_out(h.bank.staging_block, avg);
+    return sad;
+}
+
+// Frame driver over reference calls; the same composition as
+// evo_analyze_frame.  mad/cost/counter fields are left zero (the reference
+// does not expose them).
+int32_t evr_analyze_frame(const evo_frame_job *job, evo_record *records, int16_t *coeffs,
+                          int16_t *levels)
+{
+    int32 w = job->src.width, h = job->src.height;
+    uint32 wb = w / 16, hb = h / 16;
+    bank_holder bh;
+    evx_cache_bank *bank = &bh.bank;
+    image_set src_set;
+    fill_set(&src_set, job->src);
+    for (int32 k = 0; k < job->n_refs; k++) fill_set(&bank->prediction_cache[k], job->refs[k]);
+    // Slot numbers are the reference indices; prediction_target is mapped
+    // to the caller's ids when the records are written out.
+    std::vector<evx_block_desc> table(wb * hb), prev(wb * hb);
+    const evx_block_desc *prev_ptr = nullptr;
+    if (job->type == 1 && job->prev) {
+        for (uint32 k = 0; k < wb * hb; k++) {
+            clear_block_desc(&prev[k]);
+            prev[k].block_type = (EVX_BLOCK_TYPE)job->prev[k].block_type;
+            prev[k].motion_x = job->prev[k].motion_x;
+            prev[k].motion_y = job->prev[k].motion_y;
+        }
+        prev_ptr = prev.data();
+    }
+    evx_frame frame;
+    clear_frame(&frame);
+    frame.type = job->type == 1 ? EVX_FRAME_INTER : EVX_FRAME_BIDIR;
+    frame.quality = (uint16)job->quality;
+    float lambda = compute_lambda(frame.quality);
+    std::vector<int16_t> pred(384), co(384), lv(384);
+
+    for (uint32 by = 0; by < hb; by++)
+        for (uint32 bx = 0; bx < wb; bx++) {
+            uint32 idx = by * wb + bx;
+            int32 x = bx * 16, y = by * 16;
+            macroblock src_block;
+            create_macroblock(src_set, x, y, &src_block);
+            evx_block_desc best;
+            int32 best_sad = EVX_MAX_INT32;
+            bool best_copy = false;
+            int32 sad_b = 0;
+            if (job->type == 1) {
+                for (int32 k = 0; k < job->n_refs; k++) {
+                    evx_block_desc d;
+                    int32 sad = perform_inter_search(frame, src_block, x, y, bank, k, table.data(),
+                                                     wb, idx, prev_ptr, &d);
+                    d.prediction_target = (uint8)k;
+                    int32 bias = (int32)(lambda * (float)((k + 1) * 2));
+                    bool copy = EVX_IS_COPY_BLOCK_TYPE(d.block_type);
+                    if (copy ^ best_copy) {
+                        if (copy) { best = d; best_sad = sad; best_copy = true; }
+                    } else if (sad + bias < best_sad) {
+                        best = d; best_sad = sad; best_copy = copy;
+                    }
+                }
+            } else {
+                evx_block_desc fd, bd, bi;
+                int32 fs = perform_inter_search(frame, src_block, x, y, bank, 0, table.data(), wb,
+                                                idx, nullptr, &fd);
+                fd.prediction_mode = 0;
+                fd.prediction_target = 0;
+                if (fs < best_sad) { best = fd; best_sad = fs; }
+                int32 bs = perform_inter_search(frame, src_block, x, y, bank, 1, table.data(), wb,
+                                                idx, nullptr, &bd);
+                bd.prediction_mode = 1;
+                bd.prediction_target = 1;
+                if (bs < best_sad) { best = bd; best_sad = bs; }
+                int32 bis = calculate_bidir_prediction(frame, src_block, x, y, bank, 0, 1,
+                                                       table.data(), wb, idx, &bi);
+                int32 bias = (int32)(lambda * 8.0f);
+                if (bis + bias < best_sad) { best = bi; best_sad = bis; sad_b = bs; }
+            }
+            if (best_sad == EVX_MAX_INT32) return 10;
+            table[idx] = best;
+
+            int16_t *c = coeffs + (size_t)idx * 384, *l = levels + (size_t)idx * 384;
+            evx_block_desc &d = table[idx];
+            bool transformed = !EVX_IS_COPY_BLOCK_TYPE(d.block_type);
+            if (transformed) {
+                form_prediction(bank, d, x, y, pred.data());
+                macroblock mp = view(pred.data()), mc = view(c), ml = view(l);
+                sub_transform_macroblock(src_block, mp, &mc);
+                d.q_index = query_block_quantization_parameter((uint8)frame.quality, mc, d.block_type);
+                d.variance = compute_block_variance2(mc);
+                quantize_macroblock(d.q_index, d.block_type, mc, &ml);
+            } else {
+                memset(c, 0, 384 * sizeof(int16_t));
+                memset(l, 0, 384 * sizeof(int16_t));
+            }
+            evo_record *r = &records[idx];
+            desc_to_record(d, r);
+            r->prediction_target = job->target_ids[d.prediction_target];
+            if (d.prediction_mode == 2) r->prediction_target_b = job->target_ids[d.prediction_target_b];
+            r->sad = best_sad;
+            if (d.prediction_mode == 2) r->sad_b = sad_b;
+            r->flags = (transformed ? 1u : 0u) | (d.prediction_mode == 2 ? 2u : 0u);
+        }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Writes the golden vectors of tests/golden/*.npz from the reference itself.
+
+Runs only where the reference tree is present: it uses oracle/_ref/libevxref.so,
+which oracle/Makefile (`make -C oracle ref`) compiles from the reference's own
+sources plus oracle/ref_shim.cpp.  Inputs are stored next to the outputs, so
+the fixtures never depend on a random generator at test time.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from chicago_amd import synth  # noqa: E402
+from oracle.oracle import RECORD_DTYPE, REF_FIELDS, Picture, Reference  # noqa: E402
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def wide_picture(rng, w, h, lo, hi):
+    return (rng.integers(lo, hi, (h, w)).astype(np.int16),
+            rng.integers(lo, hi, (h // 2, w // 2)).astype(np.int16),
+            rng.integers(lo, hi, (h // 2, w // 2)).astype(np.int16))
+
+
+def records_to_fields(rec):
+    return np.stack([rec[f].astype(np.int64) for f in REF_FIELDS], axis=-1)
+
+
+def main():
+    ref = Reference()
+    rng = np.random.default_rng(20261203)
+
+    # 1. lambda table (pins powf), common.h:264-266.
+    lam = np.array([ref.lam(q) for q in range(64)], np.float32)
+    save("lambda.npz", quality=np.arange(64), bits=lam.view(np.uint32))
+
+    # 2. interpolation, interp.cpp:166-202, incl. edges and out-of-8-bit values.
+    out = {}
+    for tag, (lo, hi) in (("u8", (0, 256)), ("wide", (-2000, 2400))):
+        w, h = 64, 48
+        pic = wide_picture(rng, w, h, lo, hi)
+        P = Picture(*pic)
+        bx, by, dx, dy, qq = [], [], [], [], []
+        for x in (0, 1, 5, 23, w - 17, w - 16):
+            for y in (0, 2, 17, h - 16):
+                for (ddx, ddy) in synth.DIRS:
+                    for q in (0, 1):
+                        bx.append(x); by.append(y); dx.append(ddx); dy.append(ddy); qq.append(q)
+        res = ref.interp_many(P, bx, by, dx, dy, qq)
+        out.update({f"{tag}_y": pic[0], f"{tag}_u": pic[1], f"{tag}_v": pic[2],
+                    f"{tag}_params": np.array([bx, by, dx, dy, qq], np.int32).T,
+                    f"{tag}_out": res})
+    save("interp.npz", **out)
+
+    # 3. block metrics (analysis.h:44-202) and the 4x4 Hadamard.
+    a = rng.integers(0, 256, (200, 384)).astype(np.int16)
+    b = rng.integers(0, 256, (200, 384)).astype(np.int16)
+    a[100:] = rng.integers(-4000, 4000, (100, 384))
+    b[100:] = rng.integers(-4000, 4000, (100, 384))
+    m = np.array([ref.metrics(a[k], b[k]) for k in range(200)], np.int32)
+    h4 = rng.integers(-2000, 2000, (50, 16)).astype(np.int16)
+    hout = np.array([ref.hadamard(h4[k]) for k in range(50)], np.int32)
+    save("metrics.npz", a=a, b=b, sad_mad_satd=m, had_in=h4, had_out=hout)
+
+    # 4. DCT, variance2, QP and quantisation (transform.cpp, analysis.h:253,
+    #    quantize.cpp:97-118, 217-252, 454).
+    n = 96
+    src = np.zeros((n, 384), np.int16)
+    pred = np.zeros((n, 384), np.int16)
+    for k in range(n):
+        amp = (255, 600, 4095)[k % 3]
+        src[k] = rng.integers(-amp, amp + 1, 384)
+        pred[k] = rng.integers(-amp, amp + 1, 384)
+    src[0] = pred[0]
+    coeffs = np.array([ref.sub_transform(src[k], pred[k]) for k in range(n)])
+    var2 = np.array([ref.variance2(coeffs[k]) for k in range(n)], np.int32)
+    quals = np.array([0, 1, 4, 8, 12, 15, 16, 20, 31, 63], np.int32)
+    qp = np.array([[ref.block_qp(int(q), coeffs[k]) for q in quals] for k in range(n)], np.int32)
+    qps = np.array([1, 2, 7, 8, 15, 16, 17, 30, 62, 63], np.int32)
+    levels = np.array([[ref.quantize(int(q), coeffs[k]) for q in qps] for k in range(n)])
+    save("dctquant.npz", src=src, pred=pred, coeffs=coeffs, variance2=var2, qualities=quals,
+         qp=qp, qps=qps, levels=levels)
+
+    # 5. MV predictor tables (motion.cpp:271-346).
+    wb, hb = 7, 5
+    tabs = np.zeros((40, wb * hb), RECORD_DTYPE)
+    prevs = np.zeros((40, wb * hb), RECORD_DTYPE)
+    for t in range(40):
+        for T in (tabs[t], prevs[t]):
+            T["block_type"] = rng.choice([0, 1, 2, 3, 4, 6], wb * hb)
+            T["motion_x"] = rng.integers(-40, 41, wb * hb)
+            T["motion_y"] = rng.integers(-40, 41, wb * hb)
+    preds = np.zeros((40, 2, wb * hb, 2), np.int16)
+    for t in range(40):
+        for use_prev in (0, 1):
+            for idx in range(wb * hb):
+                preds[t, use_prev, idx] = ref.mv_predictor(tabs[t], wb, idx,
+                                                           prevs[t] if use_prev else None)
+    save("mvpred.npz", wb=np.int32(wb), tables=tabs, prevs=prevs, preds=preds)
+
+    # 6. perform_inter_search with supplied predictors (motion.cpp:567-679).
+    out = {}
+    for tag, (w, h, noise) in (("s64", (64, 64, True)), ("s128", (128, 128, True)),
+                               ("s128c", (128, 128, False))):
+        r_pic, c_pic = synth.pair_c0(int(rng.integers(1 << 30)), w, h)
+        if not noise:
+            hb_, wb_ = h // 16, w // 16
+            c_pic = synth.move_blocks(rng, r_pic, rng.integers(-16, 17, (hb_, wb_)),
+                                      rng.integers(-16, 17, (hb_, wb_)),
+                                      rng.integers(-1, 16, (hb_, wb_)), noise_sigma=0.0)
+        R, S = Picture(*r_pic), Picture(*c_pic)
+        params, res = [], []
+        for q in (8, 9, 10, 12, 13, 14):
+            for yy in range(0, h, 16):
+                for xx in range(0, w, 16):
+                    for pm in ((0, 0), tuple(int(v) for v in rng.integers(-24, 25, 2))):
+                        rec = ref.inter_search(S.block(xx, yy), R, xx, yy, q, pm[0], pm[1])
+                        params.append((xx, yy, q, pm[0], pm[1]))
+                        res.append(rec)
+        res = np.array(res, RECORD_DTYPE)
+        out.update({f"{tag}_ref_y": r_pic[0], f"{tag}_ref_u": r_pic[1], f"{tag}_ref_v": r_pic[2],
+                    f"{tag}_cur_y": c_pic[0], f"{tag}_cur_u": c_pic[1], f"{tag}_cur_v": c_pic[2],
+                    f"{tag}_params": np.array(params, np.int32), f"{tag}_records": res})
+    save("search.npz", **out)
+
+    # 7. calculate_bidir_prediction (motion.cpp:702-840).
+    frames = synth.sequence_c1(int(rng.integers(1 << 30)), 3, 96, 64, local_range=10)
+    F, Cc, B = (Picture(*f) for f in frames)
+    params, recs, avgs = [], [], []
+    for q in (9, 13):
+        for yy in range(0, 64, 16):
+            for xx in range(0, 96, 16):
+                pm = tuple(int(v) for v in rng.integers(-8, 9, 2))
+                r, a_ = ref.bidir(Cc.block(xx, yy), F, B, xx, yy, q, pm[0], pm[1])
+                params.append((xx, yy, q, pm[0], pm[1])); recs.append(r); avgs.append(a_)
+    save("bidir.npz", fwd_y=frames[0][0], fwd_u=frames[0][1], fwd_v=frames[0][2],
+         cur_y=frames[1][0], cur_u=frames[1][1], cur_v=frames[1][2],
+         bwd_y=frames[2][0], bwd_u=frames[2][1], bwd_v=frames[2][2],
+         params=np.array(params, np.int32), records=np.array(recs, RECORD_DTYPE),
+         averages=np.array(avgs))
+
+    # 8. whole analysis-mode frames: P (1 and 4 refs, with a temporal table)
+    #    and B, composed from the reference's functions.
+    frames = synth.sequence_c1(int(rng.integers(1 << 30)), 6, 160, 96, local_range=16)
+    pics = [Picture(*f) for f in frames]
+    out = {}
+    for k, f in enumerate(frames):
+        out[f"f{k}_y"], out[f"f{k}_u"], out[f"f{k}_v"] = f
+    rec1, co1, lv1 = ref.analyze_frame(1, 8, pics[1], [pics[0]])
+    rec2, co2, lv2 = ref.analyze_frame(1, 8, pics[2], [pics[1], pics[0]], prev=rec1)
+    rec4, co4, lv4 = ref.analyze_frame(1, 12, pics[5], [pics[4], pics[3], pics[2], pics[1]],
+                                       prev=rec2)
+    recb, cob, lvb = ref.analyze_frame(2, 9, pics[3], [pics[2], pics[4]])
+    for tag, (r_, c_, l_) in (("p1", (rec1, co1, lv1)), ("p2", (rec2, co2, lv2)),
+                              ("p4", (rec4, co4, lv4)), ("b", (recb, cob, lvb))):
+        out[f"{tag}_records"], out[f"{tag}_coeffs"], out[f"{tag}_levels"] = r_, c_, l_
+    save("frames.npz", **out)
+
+
+if __name__ == "__main__":
+    main()

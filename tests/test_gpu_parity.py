@@ -1,0 +1,230 @@
+"""Bit-exact parity of the HIP path (through the C ABI) with the oracle.
+
+Every comparison is exact: integer outputs, int16 samples, and the fp32 bits
+of the RD cost.  Sizes are small enough for the oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+
+from chicago_amd import evxa, synth
+from oracle.oracle import Picture
+
+pytestmark = pytest.mark.gpu
+
+RECORD_FIELDS = [n for n in evxa.RECORD_DTYPE.names if not n.startswith("reserved")]
+
+
+def _wide_picture(rng, w, h, lo, hi):
+    return (rng.integers(lo, hi, (h, w)).astype(np.int16),
+            rng.integers(lo, hi, (h // 2, w // 2)).astype(np.int16),
+            rng.integers(lo, hi, (h // 2, w // 2)).astype(np.int16))
+
+
+def _upload(stream, slot, pic):
+    stream.upload(slot, *pic)
+
+
+def assert_records_equal(got, want, fields=RECORD_FIELDS):
+    for f in fields:
+        bad = np.nonzero(got[f] != want[f])[0]
+        assert bad.size == 0, (f"field {f}: {bad.size} MBs differ, first {bad[:8]} "
+                               f"got {got[f][bad[:4]]} want {want[f][bad[:4]]}")
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 256), (-3000, 3000)])
+def test_interp_blocks(ctx, oracle, lo, hi):
+    rng = np.random.default_rng(11 + abs(lo))
+    w, h = 96, 64
+    pic = _wide_picture(rng, w, h, lo, hi)
+    s = ctx.open_stream(w, h, 1)
+    _upload(s, 0, pic)
+    P = Picture(*pic)
+    xs, ys, dxs, dys, qs = [], [], [], [], []
+    for bx in (0, 1, 7, 40, w - 17, w - 16):
+        for by in (0, 3, 24, h - 16):
+            for k, (dx, dy) in enumerate(synth.DIRS):
+                for q in (0, 1):
+                    xs.append(bx); ys.append(by); dxs.append(dx); dys.append(dy); qs.append(q)
+    got = s.interpolate_subpixel_6tap(0, xs, ys, dxs, dys, qs)
+    for k in range(len(xs)):
+        want = oracle.interp(P, xs[k], ys[k], dxs[k], dys[k], qs[k])
+        assert np.array_equal(got[k], want), (xs[k], ys[k], dxs[k], dys[k], qs[k])
+    s.close()
+
+
+@pytest.mark.parametrize("amp", [255, 600, 4095])
+def test_dct_quant_blocks(ctx, oracle, amp):
+    rng = np.random.default_rng(amp)
+    n = 64
+    src = rng.integers(-amp, amp + 1, (n, 384)).astype(np.int16)
+    pred = rng.integers(-amp, amp + 1, (n, 384)).astype(np.int16)
+    src[0] = pred[0]                       # zero residual: zero out (test_transform.cpp:169-196)
+    quality = rng.integers(0, 64, n).astype(np.int32)
+    quality[:8] = [0, 1, 8, 12, 15, 16, 31, 63]
+    co, lv, qp, var = ctx.dct_quant_blocks(src, pred, quality)
+    for k in range(n):
+        c = oracle.sub_transform(src[k], pred[k])
+        assert np.array_equal(co[k], c), k
+        assert var[k] == oracle.variance2(c), k
+        q = oracle.block_qp(int(quality[k]), c)
+        assert qp[k] == q, k
+        assert np.array_equal(lv[k], oracle.quantize(q, c)), k
+    assert not co[0].any() and not lv[0].any()
+
+
+def _pair(seed, w, h, noise=True):
+    ref, cur = synth.pair_c0(seed, w, h)
+    if not noise:
+        rng = np.random.default_rng(seed)
+        hb, wb = h // 16, w // 16
+        mvx = rng.integers(-16, 17, (hb, wb))
+        mvy = rng.integers(-16, 17, (hb, wb))
+        sp = rng.integers(-1, 16, (hb, wb))
+        cur = synth.move_blocks(rng, ref, mvx, mvy, sp, noise_sigma=0.0)
+    return ref, cur
+
+
+@pytest.mark.parametrize("quality", [8, 9, 12, 14])
+def test_search_blocks(ctx, oracle, quality):
+    w, h = 192, 128
+    ref, cur = _pair(quality, w, h)
+    s = ctx.open_stream(w, h, 2)
+    _upload(s, 0, cur)
+    _upload(s, 1, ref)
+    R, S = Picture(*ref), Picture(*cur)
+    rng = np.random.default_rng(quality)
+    xs, ys, pxs, pys = [], [], [], []
+    for by in range(h // 16):
+        for bx in range(w // 16):
+            for pm in ((0, 0), tuple(rng.integers(-20, 21, 2)), (200, -300)):
+                xs.append(bx * 16); ys.append(by * 16); pxs.append(pm[0]); pys.append(pm[1])
+    got = s.perform_inter_search(0, 1, quality, xs, ys, pxs, pys)
+    for k in range(len(xs)):
+        r = oracle.inter_search(S.block(xs[k], ys[k]), R, xs[k], ys[k], quality, pxs[k], pys[k])
+        for f in evxa.SEARCH_DTYPE.names:
+            assert got[f][k] == getattr(r, f), (k, f, got[k], xs[k], ys[k], pxs[k], pys[k])
+    s.close()
+
+
+def _run_p(ctx, cur, refs, quality, prev_frames=()):
+    h, w = cur[0].shape
+    s = ctx.open_stream(w, h, 1 + len(refs))
+    _upload(s, 0, cur)
+    for k, r in enumerate(refs):
+        _upload(s, 1 + k, r)
+    s.analyze_p(quality, 0, list(range(1, 1 + len(refs))))
+    out = s.results()
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("quality,noise", [(8, True), (12, True), (8, False), (13, False)])
+def test_frame_p_single_ref(ctx, oracle, quality, noise):
+    w, h = 160, 96
+    ref, cur = _pair(100 + quality, w, h, noise)
+    rec, co, lv = _run_p(ctx, cur, [ref], quality)
+    orec, oco, olv = oracle.analyze_frame(1, quality, Picture(*cur), [Picture(*ref)])
+    assert_records_equal(rec, orec)
+    assert np.array_equal(co, oco)
+    assert np.array_equal(lv, olv)
+    if not noise:
+        assert (rec["block_type"] & 4).any(), "expected copy blocks on noise-free content"
+
+
+@pytest.mark.parametrize("n_refs", [2, 4])
+def test_frame_p_multi_ref(ctx, oracle, n_refs):
+    frames = synth.sequence_c1(7, n_refs + 1, 160, 96, local_range=12)
+    cur = frames[-1]
+    refs = [frames[-2 - k] for k in range(n_refs)]
+    rec, co, lv = _run_p(ctx, cur, refs, 8)
+    orec, oco, olv = oracle.analyze_frame(1, 8, Picture(*cur), [Picture(*r) for r in refs])
+    assert_records_equal(rec, orec)
+    assert np.array_equal(co, oco)
+    assert np.array_equal(lv, olv)
+
+
+def test_frame_p_temporal_chain(ctx, oracle):
+    """Three P frames; frames 2 and 3 use the previous frame's table (H14)."""
+    frames = synth.sequence_c1(21, 4, 128, 96, local_range=10)
+    h, w = frames[0][0].shape
+    s = ctx.open_stream(w, h, 4)
+    for k, f in enumerate(frames):
+        _upload(s, k, f)
+    prev = None
+    for t in range(1, 4):
+        refs = list(range(t - 1, max(-1, t - 3), -1))
+        s.analyze_p(9, t, refs)
+        rec, co, lv = s.results()
+        orec, oco, olv = oracle.analyze_frame(1, 9, Picture(*frames[t]),
+                                              [Picture(*frames[r]) for r in refs], prev=prev)
+        assert_records_equal(rec, orec)
+        assert np.array_equal(co, oco) and np.array_equal(lv, olv)
+        prev = orec
+    s.close()
+
+
+@pytest.mark.parametrize("quality", [9, 13])
+def test_frame_b(ctx, oracle, quality):
+    frames = synth.sequence_c1(31, 3, 160, 96, local_range=12)
+    fwd, cur, bwd = frames
+    h, w = cur[0].shape
+    s = ctx.open_stream(w, h, 3)
+    for k, f in enumerate((cur, fwd, bwd)):
+        _upload(s, k, f)
+    s.analyze_b(quality, 0, 1, 2, target_ids=[0, 1])
+    rec, co, lv = s.results()
+    orec, oco, olv = oracle.analyze_frame(2, quality, Picture(*cur), [Picture(*fwd), Picture(*bwd)])
+    assert_records_equal(rec, orec)
+    assert np.array_equal(co, oco) and np.array_equal(lv, olv)
+    s.close()
+
+
+def test_batch_uneven_streams_match_alone(ctx, oracle):
+    """Streams of different sizes in one fused launch give the same results
+    as each analysed alone, and as the oracle."""
+    sizes = [(64, 48), (160, 96), (96, 64), (128, 128)]
+    streams, descs, expect = [], [], []
+    for k, (w, h) in enumerate(sizes):
+        frames = synth.sequence_c1(40 + k, 3, w, h, local_range=8)
+        s = ctx.open_stream(w, h, 3)
+        for i, f in enumerate(frames):
+            _upload(s, i, f)
+        streams.append(s)
+        nref = 1 + k % 2
+        descs.append(evxa.frame_desc(evxa.FRAME_P, 8 + k, 2, [1, 0][:nref]))
+        expect.append(oracle.analyze_frame(1, 8 + k, Picture(*frames[2]),
+                                           [Picture(*frames[1]), Picture(*frames[0])][:nref]))
+    ctx.analyze_batch(streams, descs)
+    for s, (orec, oco, olv) in zip(streams, expect):
+        rec, co, lv = s.results()
+        assert_records_equal(rec, orec)
+        assert np.array_equal(co, oco) and np.array_equal(lv, olv)
+    for s, d, (orec, _, _) in zip(streams, descs, expect):
+        s.reset_temporal()
+        s.analyze(d)
+        rec, _, _ = s.results()
+        assert_records_equal(rec, orec)
+    for s in streams:
+        s.close()
+
+
+def test_graph_replays_match_eager(ctx, oracle):
+    frames = synth.sequence_c1(55, 2, 128, 64, local_range=8)
+    h, w = frames[0][0].shape
+    s = ctx.open_stream(w, h, 2)
+    for i, f in enumerate(frames):
+        _upload(s, i, f)
+    d = evxa.frame_desc(evxa.FRAME_P, 10, 1, [0], temporal_mvp=False, update_temporal=False)
+    s.analyze(d)
+    eager = s.results()
+    gid = ctx.graph_create([s], [d])
+    for _ in range(3):
+        ctx.graph_launch(gid)
+        ctx.synchronize()
+        got = s.results()
+        assert_records_equal(got[0], eager[0])
+        assert np.array_equal(got[1], eager[1]) and np.array_equal(got[2], eager[2])
+        s.analyze(d)              # eager work between replays
+        ctx.synchronize()
+    ctx.graph_destroy(gid)
+    s.close()

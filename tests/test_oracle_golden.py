@@ -1,0 +1,121 @@
+"""The oracle against the golden vectors the reference itself wrote
+(tests/golden/make_golden.py over oracle/_ref).  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import REF_FIELDS, Picture
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def test_lambda_table(oracle):
+    g = load("lambda.npz")
+    got = np.array([oracle.lam(int(q)) for q in g["quality"]], np.float32).view(np.uint32)
+    assert np.array_equal(got, g["bits"])
+
+
+@pytest.mark.parametrize("tag", ["u8", "wide"])
+def test_interp(oracle, tag):
+    g = load("interp.npz")
+    P = Picture(g[f"{tag}_y"], g[f"{tag}_u"], g[f"{tag}_v"])
+    for p, want in zip(g[f"{tag}_params"], g[f"{tag}_out"]):
+        assert np.array_equal(oracle.interp(P, *[int(v) for v in p]), want), p
+
+
+def test_metrics(oracle):
+    g = load("metrics.npz")
+    for a, b, m in zip(g["a"], g["b"], g["sad_mad_satd"]):
+        assert oracle.metrics(a, b) == tuple(int(v) for v in m)
+    for x, want in zip(g["had_in"], g["had_out"]):
+        assert np.array_equal(oracle.hadamard(x.reshape(4, 4)), want)
+
+
+def test_dct_quant(oracle):
+    g = load("dctquant.npz")
+    for k in range(len(g["src"])):
+        c = oracle.sub_transform(g["src"][k], g["pred"][k])
+        assert np.array_equal(c, g["coeffs"][k]), k
+        assert oracle.variance2(c) == g["variance2"][k], k
+        for q, want in zip(g["qualities"], g["qp"][k]):
+            assert oracle.block_qp(int(q), c) == want, (k, q)
+        for q, want in zip(g["qps"], g["levels"][k]):
+            assert np.array_equal(oracle.quantize(int(q), c), want), (k, q)
+
+
+def test_mv_predictor(oracle):
+    g = load("mvpred.npz")
+    wb = int(g["wb"])
+    for t in range(len(g["tables"])):
+        for use_prev in (0, 1):
+            for idx in range(g["tables"].shape[1]):
+                got = oracle.mv_predictor(g["tables"][t], wb, idx,
+                                          g["prevs"][t] if use_prev else None)
+                assert got == tuple(int(v) for v in g["preds"][t, use_prev, idx]), (t, use_prev, idx)
+
+
+@pytest.mark.parametrize("tag", ["s64", "s128", "s128c"])
+def test_inter_search(oracle, tag):
+    g = load("search.npz")
+    R = Picture(g[f"{tag}_ref_y"], g[f"{tag}_ref_u"], g[f"{tag}_ref_v"])
+    S = Picture(g[f"{tag}_cur_y"], g[f"{tag}_cur_u"], g[f"{tag}_cur_v"])
+    copies = 0
+    for (x, y, q, pmx, pmy), want in zip(g[f"{tag}_params"], g[f"{tag}_records"]):
+        r = oracle.inter_search(S.block(int(x), int(y)), R, int(x), int(y), int(q), int(pmx), int(pmy))
+        got = (r.block_type, r.motion_x, r.motion_y, r.sp_enabled, r.sp_amount, r.sp_index, r.sad)
+        exp = tuple(int(want[f]) for f in ("block_type", "motion_x", "motion_y", "sp_pred",
+                                            "sp_amount", "sp_index", "sad"))
+        assert got == exp, (x, y, q, pmx, pmy)
+        copies += bool(r.block_type & 4)
+    if tag == "s128c":
+        assert copies > 0
+
+
+def test_bidir(oracle):
+    g = load("bidir.npz")
+    F = Picture(g["fwd_y"], g["fwd_u"], g["fwd_v"])
+    B = Picture(g["bwd_y"], g["bwd_u"], g["bwd_v"])
+    S = Picture(g["cur_y"], g["cur_u"], g["cur_v"])
+    for (x, y, q, pmx, pmy), want, avg in zip(g["params"], g["records"], g["averages"]):
+        x, y, q, pmx, pmy = (int(v) for v in (x, y, q, pmx, pmy))
+        src = S.block(x, y)
+        f = oracle.inter_search(src, F, x, y, q, pmx, pmy)
+        b = oracle.inter_search(src, B, x, y, q, pmx, pmy)
+        sad, a = oracle.bidir(src, oracle.predict(F, x, y, f), oracle.predict(B, x, y, b))
+        assert sad == want["sad"]
+        assert np.array_equal(a, avg)
+        assert (f.motion_x, f.motion_y, b.motion_x, b.motion_y) == (
+            want["motion_x"], want["motion_y"], want["motion_x_b"], want["motion_y_b"])
+        assert (f.sp_enabled, f.sp_index, f.sp_amount, b.sp_enabled, b.sp_index, b.sp_amount) == (
+            want["sp_pred"], want["sp_index"], want["sp_amount"], want["sp_pred_b"],
+            want["sp_index_b"], want["sp_amount_b"])
+
+
+def _frame_fixture():
+    g = load("frames.npz")
+    pics = [Picture(g[f"f{k}_y"], g[f"f{k}_u"], g[f"f{k}_v"]) for k in range(6)]
+    return g, pics
+
+
+def _check(got, g, tag):
+    rec, co, lv = got
+    for f in REF_FIELDS:
+        assert np.array_equal(rec[f], g[f"{tag}_records"][f]), (tag, f)
+    assert np.array_equal(co, g[f"{tag}_coeffs"]), tag
+    assert np.array_equal(lv, g[f"{tag}_levels"]), tag
+
+
+def test_frames(oracle):
+    g, pics = _frame_fixture()
+    r1 = oracle.analyze_frame(1, 8, pics[1], [pics[0]])
+    _check(r1, g, "p1")
+    r2 = oracle.analyze_frame(1, 8, pics[2], [pics[1], pics[0]], prev=r1[0])
+    _check(r2, g, "p2")
+    _check(oracle.analyze_frame(1, 12, pics[5], [pics[4], pics[3], pics[2], pics[1]], prev=r2[0]),
+           g, "p4")
+    _check(oracle.analyze_frame(2, 9, pics[3], [pics[2], pics[4]]), g, "b")
