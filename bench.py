@@ -1,0 +1,375 @@
+#!/usr/bin/env python3
+"""Benchmark of the EVX3 inter-analysis path on one MI355X.
+
+Workload (default, `--workload plain`): a seeded 60-frame 1920x1080 (aligned
+1920x1088, 8160 MBs) 8-bit 4:2:0 synthetic sequence; every frame after the
+first is a P frame with up to 4 past references (4 from the fifth frame on),
+quality 8, temporal MV prediction from the previous P frame, one frame per
+launch.  A step is one frame: integer step search over all references,
+half/quarter-pel refinement, selection, residual prediction, 8x8 DCT,
+adaptive QP and quantisation of every MB.  Pictures are resident in HBM before
+the timed region starts.
+
+Prints one JSON line (the driver's contract) with the plain-path throughput,
+the roofline of the analysis kernel (algorithmic integer ops per launch over
+its HIP-event duration), per-frame latency percentiles, a bit-exactness check
+of sampled frames against the CPU path, and the reference CPU path timed on
+this host's cores over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "1080p ME+DCT/quant frames/s (bit-exact), batched vs plain; p99 frame latency"
+
+# Algorithmic integer ops per unit of work (one 32-bit-lane add, sub, abs,
+# max, mul or shift each; loads not counted): an integer candidate's SAD,
+# MAD, SATD and cost; a half-pel interpolation per axial / diagonal direction;
+# the quarter-pel average; a sub-pel candidate's SAD + MAD; the DCT + QP +
+# quantisation of a non-copy MB; the bidirectional average + SAD.
+OPS_INT = 3729
+OPS_HALF_AXIAL = 3456
+OPS_HALF_DIAG = 8352
+OPS_QAVG = 1152
+OPS_SUBPEL = 1920
+OPS_DCTQ = 22405
+OPS_BIDIR = 2304
+VALU_PEAK_TOPS = 78.6      # 256 CUs x 64 lanes x 2 (packed 16-bit) x 2.4 GHz
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def record_ops(rec) -> int:
+    """Algorithmic ops of one frame's records."""
+    dirs = rec["n_axial"].astype(np.int64) + rec["n_diag"]
+    ops = (OPS_INT * rec["n_int"].astype(np.int64) + OPS_HALF_AXIAL * rec["n_axial"].astype(np.int64)
+           + OPS_HALF_DIAG * rec["n_diag"].astype(np.int64) + OPS_QAVG * dirs + 2 * OPS_SUBPEL * dirs
+           + OPS_DCTQ * (rec["flags"] & 1).astype(np.int64)
+           + OPS_BIDIR * ((rec["flags"] >> 1) & 1).astype(np.int64))
+    return int(ops.sum())
+
+
+def frame_refs(t, max_refs=4):
+    return [t - 1 - k for k in range(min(max_refs, t))]
+
+
+def frame_bytes(w, h, n_refs):
+    """Compulsory HBM bytes of one frame: source + references read once,
+    coefficients + levels + records written once (int16 samples)."""
+    n = (w // 16) * (h // 16)
+    pic = w * h * 3 // 2 * 2
+    return pic * (1 + n_refs) + n * (384 * 2 * 2 + 64)
+
+
+def cpu_baseline(frames, gpu_out, sample, quality, width, height):
+    """Time the CPU path on `sample` frames (single thread) and compare its
+    outputs with the GPU's.  Uses the reference's own build when present."""
+    from oracle.oracle import Oracle, Picture, Reference, reference_available
+    if reference_available():
+        impl, kind = Reference(), "reference"
+    else:
+        impl, kind = Oracle(), "port"
+    pics = {}
+
+    def pic(t):
+        if t not in pics:
+            pics[t] = Picture(*frames[t])
+        return pics[t]
+
+    prev = gpu_out[sample[0] - 1][0]
+    exact = True
+    elapsed = 0.0
+    fields = [n for n in gpu_out[sample[0]][0].dtype.names
+              if n not in ("mad", "cost_bits", "mad_b", "cost_b_bits", "n_int", "n_axial",
+                           "n_diag") and not n.startswith("reserved")]
+    for t in sample:
+        refs = [pic(r) for r in frame_refs(t)]
+        t0 = time.perf_counter()
+        rec, co, lv = impl.analyze_frame(1, quality, pic(t), refs, prev=prev)
+        elapsed += time.perf_counter() - t0
+        g = gpu_out[t]
+        same = all(np.array_equal(rec[f], g[0][f]) for f in fields)
+        same = same and np.array_equal(co, g[1]) and np.array_equal(lv, g[2])
+        exact = exact and same
+        log(f"cpu {kind} frame {t}: {elapsed:.1f}s so far, bit-exact={same}")
+        prev = rec
+    return {
+        "value": round(len(sample) / elapsed, 4),
+        "unit": "frames/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"frames {sample[0]}-{sample[-1]} of the {width}x{height} sequence "
+                   f"({len(frame_refs(sample[0]))} refs each, quality {quality}), one thread, "
+                   f"{elapsed:.1f} s"),
+    }, exact
+
+
+# 64 concurrent streams of uneven sizes: (width, height, count).
+BATCH_MIX = [(640, 360, 16), (1280, 720, 24), (1920, 1080, 16), (2560, 1440, 8)]
+BATCH_FRAMES = 5          # frames 0..3 are references, frame 4 is analysed
+
+
+def _gen_stream(spec):
+    from chicago_amd import synth
+    w, h, seed = spec
+    return synth.sequence_c1(seed, BATCH_FRAMES, w, h)
+
+
+def run_batched(args, ctx):
+    """64 streams, one P frame each, fused into one persistent launch per step
+    and replayed from a hipGraph.  Stream i uses 1 + (i mod 4) references."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from chicago_amd import evxa, synth
+
+    specs = []
+    for w, h, n in BATCH_MIX:
+        for _ in range(n):
+            specs.append((w, h, synth.seed_of(3, len(specs), 0)))
+    log(f"batched: generating {len(specs)} streams x {BATCH_FRAMES} frames")
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        seqs = list(ex.map(_gen_stream, specs))
+    streams, descs = [], []
+    for i, ((w, h, _), frames) in enumerate(zip(specs, seqs)):
+        s = ctx.open_stream(synth.align16(w), synth.align16(h), BATCH_FRAMES)
+        for t, f in enumerate(frames):
+            s.upload(t, *f)
+        nref = 1 + i % 4
+        streams.append(s)
+        descs.append(evxa.frame_desc(evxa.FRAME_P, args.quality, BATCH_FRAMES - 1,
+                                     [BATCH_FRAMES - 2 - k for k in range(nref)],
+                                     temporal_mvp=False, update_temporal=False))
+    mbs = sum(s.n_mbs for s in streams)
+    log(f"batched: uploaded, {mbs} MBs per step")
+
+    # Untimed eager step: outputs and work accounting.
+    ctx.analyze_batch(streams, descs)
+    ctx.synchronize()
+    outs = [s.results() for s in streams]
+    ops_step = sum(record_ops(o[0]) for o in outs)
+    ctx.launch_times()
+    # Eager steps, timed by HIP events around each fused launch.
+    for _ in range(args.warmup):
+        ctx.analyze_batch(streams, descs)
+    ctx.synchronize()
+    ctx.launch_times()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        ctx.analyze_batch(streams, descs)
+    ctx.synchronize()
+    wall_eager = time.perf_counter() - t0
+    kms = ctx.launch_times()
+    # Graph replays of the same step.
+    gid = ctx.graph_create(streams, descs)
+    for _ in range(args.warmup):
+        ctx.graph_launch(gid)
+    ctx.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        t1 = time.perf_counter()
+        ctx.graph_launch(gid)
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t1)
+    wall_graph = time.perf_counter() - t0
+    same = all(np.array_equal(a, b) for o, s in zip(outs, streams)
+               for a, b in zip(o, s.results()))
+    ctx.graph_destroy(gid)
+    log(f"batched: eager {wall_eager / args.batch_steps * 1e3:.2f} ms/step, "
+        f"graph {wall_graph / args.batch_steps * 1e3:.2f} ms/step, replay bit-identical={same}")
+    k_avg_s = float(np.mean(kms)) / 1e3
+    frames_step = len(streams)
+    step_s = wall_graph / args.batch_steps
+    res = {
+        "workload": ("batched: 64 streams (16x 640x360, 24x 1280x720, 16x 1920x1080, "
+                     "8x 2560x1440), 1-4 refs, quality %d, one fused launch per step, "
+                     "hipGraph replays" % args.quality),
+        "frames_per_s": round(frames_step / step_s, 2),
+        "frames_1080p_eq_per_s": round(mbs / 8160 / step_s, 2),
+        "ms_per_step": round(step_s * 1e3, 3),
+        "ms_per_step_eager": round(wall_eager / args.batch_steps * 1e3, 3),
+        "mbs_per_step": mbs,
+        "kernel_ms_avg": round(k_avg_s * 1e3, 3),
+        "achieved_tops": round(ops_step / k_avg_s / 1e12, 3),
+        "frac_of_valu_peak": round(ops_step / k_avg_s / 1e12 / VALU_PEAK_TOPS, 5),
+        "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "replays_bit_identical": same,
+    }
+    return res, streams, descs, seqs, outs
+
+
+def batched_cpu_check(seqs, descs, outs, quality, picks):
+    """Reference CPU path on a few of the batched streams; bit-exact check."""
+    from oracle.oracle import Oracle, Picture, Reference, reference_available
+    impl = Reference() if reference_available() else Oracle()
+    ok = True
+    for i in picks:
+        frames = seqs[i]
+        d = descs[i]
+        refs = [Picture(*frames[d.ref_slots[k]]) for k in range(d.n_refs)]
+        rec, co, lv = impl.analyze_frame(1, quality, Picture(*frames[d.src_slot]), refs)
+        g = outs[i]
+        fields = [n for n in rec.dtype.names if n not in (
+            "mad", "cost_bits", "mad_b", "cost_b_bits", "n_int", "n_axial", "n_diag")
+            and not n.startswith("reserved")]
+        same = all(np.array_equal(rec[f], g[0][f]) for f in fields)
+        same = same and np.array_equal(co, g[1]) and np.array_equal(lv, g[2])
+        log(f"batched stream {i}: CPU bit-exact={same}")
+        ok = ok and same
+    return ok
+
+
+def run_plain(args):
+    from chicago_amd import evxa, synth
+
+    w, h = 1920, 1080
+    aw, ah = synth.align16(w), synth.align16(h)
+    log(f"generating {args.frames} frames {w}x{h}")
+    frames = synth.sequence_c1(synth.seed_of(1, 0, 0), args.frames, w, h)
+    ctx = evxa.Context(0)
+    s = ctx.open_stream(aw, ah, args.frames)
+    for t, f in enumerate(frames):
+        s.upload(t, *f)
+    log("uploaded")
+    q = args.quality
+    seq = list(range(1, args.frames))
+
+    def step(t):
+        if t == 1:
+            s.reset_temporal()
+        s.analyze_p(q, t, frame_refs(t))
+
+    # Untimed pass: work accounting, outputs for the CPU comparison and
+    # per-frame latency (enqueue -> device idle).
+    keep = set(range(args.cpu_start - 1, args.cpu_start + args.cpu_frames))
+    ops, outs, lat = {}, {}, []
+    for t in seq:
+        t0 = time.perf_counter()
+        step(t)
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+        rec, co, lv = s.results(coeffs=t in keep, levels=t in keep)
+        ops[t] = record_ops(rec)
+        if t in keep:
+            outs[t] = (rec, co, lv)
+    ctx.launch_times()
+    log(f"accounting pass done: {np.mean(list(ops.values())) / 1e9:.2f} Gops/frame")
+
+    order = [seq[k % len(seq)] for k in range(args.warmup + args.steps)]
+    for t in order[:args.warmup]:
+        step(t)
+    ctx.synchronize()
+    ctx.launch_times()
+    timed = order[args.warmup:]
+    t0 = time.perf_counter()
+    for t in timed:
+        step(t)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    kms = ctx.launch_times()
+    assert len(kms) == len(timed), (len(kms), len(timed))
+    log(f"timed {len(timed)} frames in {wall:.3f}s")
+
+    ops_launch = float(np.mean([ops[t] for t in timed]))
+    k_avg_s = float(np.mean(kms)) / 1e3
+    achieved = ops_launch / k_avg_s / 1e12
+    nrefs = np.mean([len(frame_refs(t)) for t in timed])
+    bytes_launch = float(np.mean([frame_bytes(aw, ah, len(frame_refs(t))) for t in timed]))
+
+    result = {
+        "metric": METRIC,
+        "value": round(len(timed) / wall, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": len(timed),
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / len(timed) * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"plain: {w}x{h} 4:2:0 P frames, up to 4 refs ({nrefs:.2f} mean), "
+                         f"quality {q}, quarter-pel, one frame per launch"),
+            "frames": args.frames,
+            "mbs_per_frame": (aw // 16) * (ah // 16),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 4),
+            "peak": VALU_PEAK_TOPS,
+            "unit": "TOP/s",
+            "frac": round(achieved / VALU_PEAK_TOPS, 5),
+            "traffic": None,
+            "kernel": "analyze_kernel",
+            "kernel_ms_avg": round(k_avg_s * 1e3, 4),
+            "algorithmic_ops_per_launch": int(ops_launch),
+            "algorithmic_bytes_per_launch": int(bytes_launch),
+            "hbm_gbs_algorithmic": round(bytes_launch / k_avg_s / 1e9, 2),
+        },
+        "latency_ms": {
+            "p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "what": "enqueue of one frame to device idle, sequential frames",
+        },
+    }
+    if not args.no_cpu:
+        sample = list(range(args.cpu_start, args.cpu_start + args.cpu_frames))
+        cb, exact = cpu_baseline(frames, outs, sample, q, w, h)
+        result["cpu_baseline"] = cb
+        result["bit_exact_vs_cpu"] = exact
+        result["gpu_vs_cpu"] = round(result["value"] / cb["value"], 1)
+    s.close()
+    ctx.close()
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=118)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--frames", type=int, default=60)
+    ap.add_argument("--quality", type=int, default=8)
+    ap.add_argument("--workload", choices=["plain", "batched", "all"], default="all")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--cpu-start", type=int, default=5)
+    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by "
+              "stream with no cross-GPU exchange, so per-GPU numbers are the unit.",
+              file=sys.stderr)
+        sys.exit(2)
+    res = run_plain(args)
+    if args.workload in ("all", "batched"):
+        from chicago_amd import evxa
+        ctx = evxa.Context(0)
+        b, streams, descs, seqs, outs = run_batched(args, ctx)
+        if not args.no_cpu:
+            b["bit_exact_vs_cpu"] = batched_cpu_check(seqs, descs, outs, args.quality, [0, 17, 41])
+        b["vs_plain"] = round(b["frames_1080p_eq_per_s"] / res["value"], 2)
+        res["batched"] = b
+        for s in streams:
+            s.close()
+        ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -85,7 +86,8 @@ size_t picture_bytes(int32_t w, int32_t h)
 size_t outputs_bytes(int32_t w, int32_t h)
 {
     const size_t n = (size_t)(w / 16) * (h / 16);
-    return 3 * n * sizeof(evxa_block_record) + 2 * n * EVXA_MB_SAMPLES * sizeof(int16_t);
+    return 3 * n * sizeof(evxa_block_record) + 2 * n * EVXA_MB_SAMPLES * sizeof(int16_t) +
+           n * EVXA_MAX_REFS * sizeof(SearchRec) + n * sizeof(int32_t);
 }
 
 // Device buffer owned by RAII.
@@ -153,12 +155,16 @@ struct LaunchRes {
     DevBuf jobs, tickets, progress, counters;
     PinnedBuf h_jobs, h_tickets;
     hipEvent_t staged = nullptr;     // host staging may be reused after this
+    hipEvent_t g_start = nullptr, g_stop = nullptr;   // captured timing (graphs)
     std::vector<uint64_t> key;       // composition of the cached work list
     int32_t n_tickets = 0;
     int32_t n_progress = 0;
+    int32_t grid = 0;                // waves to launch for this composition
     ~LaunchRes()
     {
         if (staged) (void)hipEventDestroy(staged);
+        if (g_start) (void)hipEventDestroy(g_start);
+        if (g_stop) (void)hipEventDestroy(g_stop);
     }
 };
 
@@ -174,8 +180,12 @@ struct evxa_context {
     int32_t grid = 0;
     std::mutex mu;
     LaunchRes res;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    bool timed = false;
+    // Ring of (start, stop) event pairs around analysis kernels; launches
+    // [ring_tail, ring_head) have not been reported by evxa_launch_times.
+    static constexpr size_t kRing = 8192;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ring;
+    uint64_t ring_head = 0, ring_tail = 0;
+    hipEvent_t last_start = nullptr, last_stop = nullptr;
     std::map<int32_t, std::unique_ptr<LaunchRes>> graph_res;
     std::map<int32_t, hipGraphExec_t> graphs;
     int32_t next_graph = 1;
@@ -189,6 +199,8 @@ struct evxa_stream {
     std::vector<std::unique_ptr<DevBuf>> slots;
     DevBuf tables;         // 3 record tables: P ping, P pong, B
     DevBuf coeffs, levels;
+    DevBuf searches;       // EVXA_MAX_REFS SearchRec per MB
+    DevBuf mb_done;        // per-MB finished-search counters (kept at zero between launches)
     int32_t p_cur = 0;
     bool temporal_valid = false;
     int32_t last_table = -1;
@@ -261,6 +273,8 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     j.records = s->table(out);
     j.coeffs = s->coeffs.as<int16_t>();
     j.levels = s->levels.as<int16_t>();
+    j.searches = s->searches.as<SearchRec>();
+    j.mb_done = s->mb_done.as<int32_t>();
     if (advance) {
         s->last_table = out;
         if (d.type == EVXA_FRAME_P && d.update_temporal) {
@@ -271,24 +285,31 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     return j;
 }
 
-// Work list: anti-diagonal order t = bx + 2 by across all jobs, so every
-// ticket's predecessors (left, top-left, top, top-right) carry smaller
-// tickets and are owned by waves already running.
-void build_tickets(const std::vector<DevJob> &jobs, std::vector<uint32_t> &out)
+// Work list: anti-diagonal order t = bx + 2 by across all jobs, the
+// reference searches of a MB adjacent, so every ticket's predecessors (left,
+// top-left, top, top-right MBs) carry smaller tickets and are owned by waves
+// already running.  Returns the largest number of items on one diagonal.
+int32_t build_tickets(const std::vector<DevJob> &jobs, std::vector<uint32_t> &out)
 {
     out.clear();
-    int32_t tmax = 0;
+    int32_t tmax = 0, widest = 0;
     for (const auto &j : jobs) tmax = std::max(tmax, j.wb - 1 + 2 * (j.hb - 1));
-    for (int32_t t = 0; t <= tmax; t++)
+    for (int32_t t = 0; t <= tmax; t++) {
+        const size_t before = out.size();
         for (size_t k = 0; k < jobs.size(); k++) {
             const DevJob &j = jobs[k];
             for (int32_t by = 0; by < j.hb; by++) {
                 const int32_t bx = t - 2 * by;
                 if (bx < 0) break;
                 if (bx >= j.wb) continue;
-                out.push_back(((uint32_t)k << kJobShift) | (uint32_t)(by * j.wb + bx));
+                for (int32_t r = 0; r < j.n_refs; r++)
+                    out.push_back(((uint32_t)k << kJobShift) | ((uint32_t)r << kRefShift) |
+                                  (uint32_t)(by * j.wb + bx));
             }
         }
+        widest = std::max(widest, (int32_t)(out.size() - before));
+    }
+    return widest;
 }
 
 // Stage jobs + work list into `r` and enqueue reset + analysis kernels.
@@ -300,8 +321,8 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     int32_t n_progress = 0;
     for (const auto &j : jobs) {
         if ((int64_t)j.wb * j.hb > (int64_t)kMbMask) fail(EVXA_ERROR_CAPACITY_LIMIT, "frame too large");
-        key.push_back(((uint64_t)j.wb << 32) | (uint32_t)j.hb);
-        n_progress += j.hb;
+        key.push_back(((uint64_t)j.wb << 40) | ((uint64_t)j.hb << 8) | (uint32_t)j.n_refs);
+        n_progress += j.hb * kProgressStride;
     }
     if (!r.staged) check(hipEventCreateWithFlags(&r.staged, hipEventDisableTiming), "hipEventCreate");
     // The previous launch's host->device copies must have consumed the
@@ -310,7 +331,11 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
 
     if (key != r.key) {
         std::vector<uint32_t> t;
-        build_tickets(jobs, t);
+        const int32_t widest = build_tickets(jobs, t);
+        // Enough waves for the widest diagonal (plus slack for the MBs still
+        // finishing the previous one), never more than fit at once: waiting
+        // waves beyond that only add polling traffic.
+        r.grid = std::max(64, std::min(ctx->grid, widest + widest / 2 + 32));
         r.h_tickets.ensure(t.size() * sizeof(uint32_t));
         memcpy(r.h_tickets.p, t.data(), t.size() * sizeof(uint32_t));
         if (r.tickets.n < t.size() * sizeof(uint32_t)) r.tickets.alloc(t.size() * sizeof(uint32_t));
@@ -327,7 +352,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     int32_t off = kCounterWords;
     for (auto &j : jobs) {
         j.progress = prog + off;
-        off += j.hb;
+        off += j.hb * kProgressStride;
     }
     const size_t jb = jobs.size() * sizeof(DevJob);
     r.h_jobs.ensure(jb);
@@ -336,14 +361,21 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, ctx->stream), "copy jobs");
     check(hipEventRecord(r.staged, ctx->stream), "hipEventRecord");
     check(launch_reset(prog, n_progress + kCounterWords, ctx->stream), "reset kernel");
-    const int32_t grid = std::max(1, std::min(ctx->grid, r.n_tickets));
-    if (time_it) check(hipEventRecord(ctx->ev_start, ctx->stream), "hipEventRecord");
+    const int32_t grid = std::max(1, std::min(r.grid, r.n_tickets));
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (time_it) {
+        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+        ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+        check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
+    }
     check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.n_tickets, prog, grid,
                          ctx->stream),
           "analysis kernel");
     if (time_it) {
-        check(hipEventRecord(ctx->ev_stop, ctx->stream), "hipEventRecord");
-        ctx->timed = true;
+        check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
+        ctx->ring_head++;
+        ctx->last_start = ev.first;
+        ctx->last_stop = ev.second;
     }
 }
 
@@ -388,8 +420,11 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
         ctx->device = device_ordinal;
         set_device(ctx.get());
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
-        check(hipEventCreate(&ctx->ev_start), "hipEventCreate");
-        check(hipEventCreate(&ctx->ev_stop), "hipEventCreate");
+        ctx->ring.resize(evxa_context::kRing);
+        for (auto &e : ctx->ring) {
+            check(hipEventCreate(&e.first), "hipEventCreate");
+            check(hipEventCreate(&e.second), "hipEventCreate");
+        }
         if (hbm_budget_bytes == 0) {
             size_t free_b = 0, total_b = 0;
             check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
@@ -397,6 +432,10 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
         }
         ctx->budget = hbm_budget_bytes;
         ctx->grid = analyze_resident_waves(device_ordinal);
+        if (const char *g = getenv("EVXA_ANALYZE_GRID")) {
+            const int v = atoi(g);
+            if (v > 0) ctx->grid = v;
+        }
         *out = ctx.release();
     });
 }
@@ -412,8 +451,10 @@ int evxa_destroy(evxa_context *ctx)
         ctx->graph_res.clear();
         for (auto *s : ctx->streams) delete s;
         ctx->streams.clear();
-        if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
-        if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+        for (auto &e : ctx->ring) {
+            if (e.first) (void)hipEventDestroy(e.first);
+            if (e.second) (void)hipEventDestroy(e.second);
+        }
         hipStream_t st = ctx->stream;
         delete ctx;
         if (st) (void)hipStreamDestroy(st);
@@ -455,6 +496,9 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
               "clear tables");
         s->coeffs.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
         s->levels.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
+        s->searches.alloc(n * EVXA_MAX_REFS * sizeof(SearchRec));
+        s->mb_done.alloc(n * sizeof(int32_t));
+        check(hipMemsetAsync(s->mb_done.p, 0, n * sizeof(int32_t), ctx->stream), "clear counters");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         s->bytes = need;
         ctx->used += need;
@@ -566,10 +610,28 @@ int evxa_last_launch_ms(evxa_context *ctx, float *ms)
     return guarded([&] {
         if (!ctx || !ms) fail(EVXA_ERROR_INVALIDARG, "null argument");
         std::lock_guard<std::mutex> lk(ctx->mu);
-        if (!ctx->timed) fail(EVXA_ERROR_INVALID_RESOURCE, "no timed launch yet");
+        if (!ctx->last_stop) fail(EVXA_ERROR_INVALID_RESOURCE, "no timed launch yet");
         set_device(ctx);
-        check(hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize");
-        check(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop), "hipEventElapsedTime");
+        check(hipEventSynchronize(ctx->last_stop), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, ctx->last_start, ctx->last_stop), "hipEventElapsedTime");
+    });
+}
+
+int evxa_launch_times(evxa_context *ctx, float *ms, int32_t max, int32_t *n_out)
+{
+    return guarded([&] {
+        if (!ctx || !n_out || max < 0 || (max > 0 && !ms)) fail(EVXA_ERROR_INVALIDARG, "bad argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        int32_t n = 0;
+        while (ctx->ring_tail < ctx->ring_head && n < max) {
+            const auto &e = ctx->ring[ctx->ring_tail % evxa_context::kRing];
+            check(hipEventElapsedTime(ms + n, e.first, e.second), "hipEventElapsedTime");
+            ctx->ring_tail++;
+            n++;
+        }
+        *n_out = n;
     });
 }
 
@@ -620,11 +682,13 @@ int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams, const evxa
         check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
         hipError_t e1 = launch_reset(res->progress.as<int32_t>(), res->n_progress + kCounterWords,
                                      ctx->stream);
-        const int32_t grid = std::max(1, std::min(ctx->grid, res->n_tickets));
-        hipError_t e2 = hipEventRecord(ctx->ev_start, ctx->stream);
+        const int32_t grid = std::max(1, std::min(res->grid, res->n_tickets));
+        check(hipEventCreate(&res->g_start), "hipEventCreate");
+        check(hipEventCreate(&res->g_stop), "hipEventCreate");
+        hipError_t e2 = hipEventRecord(res->g_start, ctx->stream);
         hipError_t e3 = launch_analyze(res->jobs.as<DevJob>(), res->tickets.as<uint32_t>(),
                                        res->n_tickets, res->progress.as<int32_t>(), grid, ctx->stream);
-        hipError_t e4 = hipEventRecord(ctx->ev_stop, ctx->stream);
+        hipError_t e4 = hipEventRecord(res->g_stop, ctx->stream);
         hipError_t e5 = hipStreamEndCapture(ctx->stream, &graph);
         check(e1, "capture reset");
         check(e2, "capture event");
@@ -651,7 +715,9 @@ int evxa_graph_launch(evxa_context *ctx, int32_t graph_id)
         if (it == ctx->graphs.end()) fail(EVXA_ERROR_INVALIDARG, "no such graph");
         set_device(ctx);
         check(hipGraphLaunch(it->second, ctx->stream), "hipGraphLaunch");
-        ctx->timed = true;
+        const LaunchRes &r = *ctx->graph_res[graph_id];
+        ctx->last_start = r.g_start;
+        ctx->last_stop = r.g_stop;
     });
 }
 

@@ -13,7 +13,7 @@ namespace evxa {
 // kApron samples on every side, so every 6-tap read lands inside the buffer
 // and equals the clamped read of interp.cpp:8-14.  Pointers address sample
 // (0, 0); pitches are in elements and keep rows 128-byte aligned.
-constexpr int kApron = 16;
+constexpr int kApron = 24;
 
 struct DevPic {
     const int16_t *y;
@@ -22,6 +22,19 @@ struct DevPic {
     int32_t py;
     int32_t pc;
 };
+
+// Outcome of one (MB, reference) search, kept in HBM until the MB's last
+// search finishes and selects among them (32 bytes).
+struct SearchRec {
+    int16_t best_x, best_y;
+    int32_t sad, mad, ssd;
+    uint32_t cost_bits;
+    uint8_t sp_en, sp_amt, sp_idx, btype;
+    uint16_t n_int;
+    uint8_t n_ax, n_dg;
+    uint32_t reserved;
+};
+static_assert(sizeof(SearchRec) == 32, "SearchRec layout");
 
 // One frame of work inside a launch.
 struct DevJob {
@@ -35,18 +48,26 @@ struct DevJob {
     evxa_block_record *records;
     int16_t *coeffs;
     int16_t *levels;
-    int32_t *progress;                 // hb counters: MBs finished per row
+    SearchRec *searches;               // EVXA_MAX_REFS per MB
+    int32_t *mb_done;                  // searches finished per MB (returns to 0)
+    int32_t *progress;                 // per row, kProgressStride words apart
 };
 
-// Work-list entry: job index in the top 12 bits, MB index in the low 20.
-constexpr int kJobShift = 20;
-constexpr uint32_t kMbMask = (1u << kJobShift) - 1;
+// Row-progress words sit on separate 256-byte lines so that waiting waves
+// poll different cache lines.
+constexpr int kProgressStride = 64;
+
+// Work-list entry: job (10 bits) | reference (2 bits) | MB index (20 bits).
+constexpr int kMbBits = 20;
+constexpr uint32_t kMbMask = (1u << kMbBits) - 1;
+constexpr int kRefShift = 20;
+constexpr int kJobShift = 22;
 constexpr int kMaxJobs = 1 << (32 - kJobShift);
 
 // Counter block of a launch: [0] next ticket, [1] error flags.
 constexpr int kCounterTicket = 0;
 constexpr int kCounterError = 1;
-constexpr int kCounterWords = 16;
+constexpr int kCounterWords = 64;
 
 // Error flag bits.
 constexpr int32_t kErrSearchFailed = 1;
@@ -65,7 +86,7 @@ hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, in
 hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t *pred,
                                    const int32_t *quality, int16_t *coeffs, int16_t *levels,
                                    int32_t *qp, int32_t *variance, hipStream_t st);
-// Grid size that keeps every wave of the analysis kernel resident.
+// Waves of the analysis kernel that fit the device at once.
 int32_t analyze_resident_waves(int32_t device);
 
 }  // namespace evxa

@@ -1,22 +1,27 @@
 // evxa_kernels.hip -- gfx950 kernels of the EVX3 inter analysis path.
 //
-// One wavefront (64 lanes, one workgroup) analyses one macroblock at a time.
-// Candidate metrics run wide: the wave splits into four 16-lane groups, each
-// group evaluating one candidate position with one lane per 4x4 luma block
-// (plus 8 chroma samples), and DPP row reductions give each group's SAD,
-// SATD and MAD.  The decision is the reference's ordered fold over the
-// candidates (motion.cpp:125-178, 180-261), done on wave-uniform values.
+// Work items are (macroblock, reference) searches, one wavefront (64 lanes,
+// one workgroup) each.  Candidate metrics run wide: the wave splits into four
+// 16-lane groups, each evaluating one candidate position with one lane per
+// 4x4 luma block (plus 8 chroma samples), and DPP row reductions give each
+// group's SAD, SATD and MAD.  The decision is the reference's ordered fold
+// over the candidates (motion.cpp:125-178, 180-261) on wave-uniform values.
+// Sub-pel refinement builds the horizontal, vertical and diagonal half-pel
+// planes around the integer best once, in LDS, and reads all 16 candidates
+// from them.
 //
-// Work is scheduled by a persistent work list: waves take MB tickets in
-// anti-diagonal order (t = bx + 2*by, all jobs of the launch interleaved) and
-// wait until the MB's left and top-right neighbours are published, because
-// the MV predictor reads their final records (motion.cpp:271-346).  A MB's
-// record is published before its DCT/quantisation, which is off the
+// A persistent work list hands out items in anti-diagonal order
+// (t = bx + 2*by, all jobs of a launch interleaved, the references of a MB
+// adjacent).  An item waits until the MB's left and top-right neighbours are
+// published, because the MV predictor reads their final records
+// (motion.cpp:271-346).  The wave that finishes a MB's last search selects
+// among the references, publishes the record, then forms the residual
+// prediction and runs DCT, adaptive QP and quantisation, which are off the
 // dependency chain.
 //
-// All arithmetic is integer except the RD cost, which is an fp32 multiply
-// followed by an fp32 add with round-to-nearest and no contraction
-// (motion.cpp:144), as on the reference's x86 build.
+// Arithmetic is integer except the RD cost: an fp32 multiply followed by an
+// fp32 add with round-to-nearest and no contraction (motion.cpp:144), as on
+// the reference's x86 build.
 
 #include "evxa_internal.h"
 
@@ -26,7 +31,7 @@ namespace {
 constexpr int kSadThreshold = 8 * 1024;  // motion.cpp:24
 constexpr int kSearchRadius = 16;        // motion.cpp:29
 constexpr int kInt32Max = 0x7FFFFFFF;
-constexpr int kSpinLimit = 1 << 22;
+constexpr int kSpinLimit = 1 << 20;
 
 // xftables.h:57-67: round(128 cos((2k+1) i pi / 16)).
 __constant__ int16_t c_dct8[64] = {
@@ -100,12 +105,18 @@ __device__ __forceinline__ int abs16(int v)
     return s == INT16_MIN ? 0x7FFF : (s < 0 ? -s : s);
 }
 
-// evx_math.h:228-236
-__device__ __forceinline__ int rounded_div(int numer, int denom)
+// evx_math.h:228-236 with a positive divisor d and |numer| + d/2 < 2^20:
+// the truncating quotient of |numer| + d/2 is the high half of its product
+// with m = ceil(2^32 / d) (the error term stays below one unit because
+// (|numer| + d/2) * (m d - 2^32) < 2^32).
+__device__ __forceinline__ int rdiv_magic(int numer, int d, uint32_t m)
 {
-    if (((uint32_t)numer ^ (uint32_t)denom) & 0x80000000u) return (numer - denom / 2) / denom;
-    return (numer + denom / 2) / denom;
+    const uint32_t a = (uint32_t)(numer < 0 ? -numer : numer) + (uint32_t)(d >> 1);
+    const int q = (int)__umulhi(a, m);
+    return numer < 0 ? -q : q;
 }
+
+__device__ __forceinline__ uint32_t magic_of(int d) { return 0xFFFFFFFFu / (uint32_t)d + 1u; }
 
 // interp.cpp:18-41: six taps starting at p with spacing `step`.
 __device__ __forceinline__ int tap6(const int16_t *p, int step)
@@ -113,43 +124,6 @@ __device__ __forceinline__ int tap6(const int16_t *p, int step)
     int s = p[0] - 5 * p[step] + 20 * p[2 * step] + 20 * p[3 * step] - 5 * p[4 * step] +
             p[5 * step];
     return (int16_t)((s + 16) >> 5);
-}
-
-// Half-pel sample of interp.cpp:52-164 at integer position (X, Y) of a plane
-// (apron-backed, so no clamping is needed).
-__device__ __forceinline__ int half_sample(const int16_t *plane, int pitch, int X, int Y, int dx,
-                                           int dy)
-{
-    const int xo = dx > 0 ? 0 : -1;
-    const int yo = dy > 0 ? 0 : -1;
-    if (dy == 0) return tap6(plane + Y * pitch + X + xo - 2, 1);
-    if (dx == 0) return tap6(plane + (Y + yo - 2) * pitch + X, pitch);
-    const int16_t *p = plane + (Y + yo - 2) * pitch + X + xo - 2;
-    int h0 = tap6(p, 1), h1 = tap6(p + pitch, 1), h2 = tap6(p + 2 * pitch, 1);
-    int h3 = tap6(p + 3 * pitch, 1), h4 = tap6(p + 4 * pitch, 1), h5 = tap6(p + 5 * pitch, 1);
-    int s = h0 - 5 * h1 + 20 * h2 + 20 * h3 - 5 * h4 + h5;
-    return (int16_t)((s + 16) >> 5);
-}
-
-// Sample i (0..383, MB order) of the prediction at integer base (bx, by),
-// optionally sub-pel (interp.cpp:166-202) or a plain copy (macroblock.h:82).
-__device__ __forceinline__ int pred_sample(const DevPic &ref, int i, int bx, int by, int sp,
-                                           int dx, int dy, int quarter)
-{
-    const int16_t *plane;
-    int pitch, X, Y;
-    if (i < 256) {
-        plane = ref.y; pitch = ref.py;
-        X = bx + (i & 15); Y = by + (i >> 4);
-    } else {
-        int k = (i - 256) & 63;
-        plane = i < 320 ? ref.u : ref.v; pitch = ref.pc;
-        X = (bx >> 1) + (k & 7); Y = (by >> 1) + (k >> 3);
-    }
-    int full = plane[Y * pitch + X];
-    if (!sp) return full;
-    int half = half_sample(plane, pitch, X, Y, dx, dy);
-    return quarter ? (int16_t)((full + half + 1) >> 1) : half;
 }
 
 // motion.cpp:93-116
@@ -161,22 +135,6 @@ __device__ __forceinline__ void frac_direction(int idx, int &dx, int &dy)
     else { dy = 1; dx = idx - 6; }
 }
 
-// Direction k (0..7) of the sub-pel scan order (j outer, i inner, centre
-// skipped; motion.cpp:464-473).  k equals the frac index of motion.cpp:68-91.
-__device__ __forceinline__ void scan_direction(int k, int &i, int &j)
-{
-    int m = k < 4 ? k : k + 1;
-    i = m % 3 - 1;
-    j = m / 3 - 1;
-}
-
-// The k-th set bit (k = 0..3) of an 8-bit mask, or -1.
-__device__ __forceinline__ int nth_bit(uint32_t mask, int k)
-{
-    for (int t = 0; t < k; t++) mask &= mask - 1;
-    return mask ? __builtin_ctz(mask) : -1;
-}
-
 // Source MB samples held in registers.  Lane b = lane & 15 owns luma 4x4
 // block (b & 3, b >> 2) and chroma row (b & 7), columns (b >> 3) * 4 .. +3.
 struct Src {
@@ -186,38 +144,67 @@ struct Src {
     int ox, oy, cr, cc;
 };
 
-__device__ __forceinline__ void load_src(const DevPic &pic, int x, int y, Src &s)
+__device__ __forceinline__ void lane_geometry(Src &s)
 {
     const int b = lane_id() & 15;
     s.ox = (b & 3) * 4; s.oy = (b >> 2) * 4;
     s.cr = b & 7; s.cc = (b >> 3) * 4;
-    const int16_t *p = pic.y + (y + s.oy) * pic.py + x + s.ox;
+}
+
+// Samples of the MB at (x, y) of `pic` in the lane layout of Src.
+__device__ __forceinline__ void load_lane_block(const DevPic &pic, int x, int y, const Src &g,
+                                                int16_t (&ly)[16], int16_t (&lu)[4],
+                                                int16_t (&lv)[4])
+{
+    const int16_t *p = pic.y + (y + g.oy) * pic.py + x + g.ox;
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) s.y[r * 4 + c] = p[r * pic.py + c];
-    const int co = ((y >> 1) + s.cr) * pic.pc + (x >> 1) + s.cc;
+        for (int c = 0; c < 4; c++) ly[r * 4 + c] = p[r * pic.py + c];
+    const int co = ((y >> 1) + g.cr) * pic.pc + (x >> 1) + g.cc;
 #pragma unroll
-    for (int k = 0; k < 4; k++) { s.u[k] = pic.u[co + k]; s.v[k] = pic.v[co + k]; }
+    for (int k = 0; k < 4; k++) { lu[k] = pic.u[co + k]; lv[k] = pic.v[co + k]; }
+}
+
+// Reference samples of one candidate position in the lane layout of Src.
+struct RefBlk {
+    int16_t y[16];
+    int16_t u[4];
+    int16_t v[4];
+};
+
+__device__ __forceinline__ void load_cand(const DevPic &ref, const Src &s, int cx, int cy,
+                                          RefBlk &rb)
+{
+    load_lane_block(ref, cx, cy, s, rb.y, rb.u, rb.v);
+}
+
+__device__ __forceinline__ void select_blk(bool take_b, const RefBlk &a, const RefBlk &b,
+                                           RefBlk &out)
+{
+#pragma unroll
+    for (int k = 0; k < 16; k++) out.y[k] = take_b ? b.y[k] : a.y[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        out.u[k] = take_b ? b.u[k] : a.u[k];
+        out.v[k] = take_b ? b.v[k] : a.v[k];
+    }
 }
 
 // Per-lane part of an integer candidate (analysis.h:89-132, 180-202).
-__device__ __forceinline__ void eval_int_lane(const DevPic &ref, const Src &s, int cx, int cy,
-                                              int &sad, int &mad, int &satd)
+__device__ __forceinline__ void cand_metrics(const Src &s, const RefBlk &rb, int &sad, int &mad,
+                                             int &satd)
 {
-    const int16_t *p = ref.y + (cy + s.oy) * ref.py + cx + s.ox;
     int d[16];
     sad = 0; mad = 0;
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            int dd = s.y[r * 4 + c] - p[r * ref.py + c];
-            int a = dd < 0 ? -dd : dd;
-            sad += a;
-            mad = max(mad, a);
-            d[r * 4 + c] = (int16_t)dd;   // analysis.h:102 stores the difference as int16
-        }
+    for (int k = 0; k < 16; k++) {
+        const int dd = s.y[k] - rb.y[k];
+        const int a = dd < 0 ? -dd : dd;
+        sad += a;
+        mad = max(mad, a);
+        d[k] = (int16_t)dd;   // analysis.h:102 stores the difference as int16
+    }
     // analysis.h:44-85
     int t[16];
 #pragma unroll
@@ -233,49 +220,169 @@ __device__ __forceinline__ void eval_int_lane(const DevPic &ref, const Src &s, i
         int s0 = a + b, s1 = a - b, s2 = cc + e, s3 = cc - e;
         satd += abs(s0 + s2) + abs(s1 + s3) + abs(s0 - s2) + abs(s1 - s3);
     }
-    const int co = ((cy >> 1) + s.cr) * ref.pc + (cx >> 1) + s.cc;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        mad = max(mad, abs(s.u[k] - ref.u[co + k]));
-        mad = max(mad, abs(s.v[k] - ref.v[co + k]));
+        mad = max(mad, abs(s.u[k] - rb.u[k]));
+        mad = max(mad, abs(s.v[k] - rb.v[k]));
     }
 }
 
-// Per-lane part of one sub-pel direction: half and quarter SAD (luma) and
-// MAD (luma + chroma), motion.cpp:180-261 with interp.cpp:52-164.
-__device__ __forceinline__ void eval_subpel_lane(const DevPic &ref, const Src &s, int bx, int by,
-                                                 int dx, int dy, int &sad_h, int &mad_h,
-                                                 int &sad_q, int &mad_q)
+// Half-pel planes around an integer base (bx, by), interp.cpp:52-164 split
+// into its separable passes.  For a block of size B (16 luma, 8 chroma):
+//   h: horizontal halves between columns c and c+1, rows -3..B+2, cols -1..B-1
+//   v: vertical halves between rows r and r+1, rows -1..B-1, cols 0..B-1
+//   d: vertical 6-tap over h (the diagonal), rows -1..B-1, cols -1..B-1
+// Every direction's half-pel block is a B x B window of one plane.
+template <int B>
+struct HalfPlanes {
+    static constexpr int HR = B + 6, HC = B + 1;
+    static constexpr int VR = B + 1, VC = B;
+    static constexpr int DR = B + 1, DC = B + 1;
+    int16_t h[HR * HC];
+    int16_t v[VR * VC];
+    int16_t d[DR * DC];
+};
+
+// Reference window staged in LDS for the search levels with step <= 8 and
+// the sub-pel refinement.  Around a base position b and a reach r (the
+// largest offset the remaining levels can move the best, 2R - 1 <= 15) the
+// reads stay within luma [b - r - 3, b + r + 18] and chroma
+// [((b - r) >> 1) - 3, ((b + r) >> 1) + 10] on each axis.  Origins are even so
+// rows load as aligned dwords.
+constexpr int kReachMax = 15;
+constexpr int kWinLS = 54;    // luma row stride (>= 2 * 15 + 23, even)
+constexpr int kWinLR = 52;    // luma rows (2 * 15 + 22)
+constexpr int kWinCS = 32;    // chroma row stride (>= 15 + 16, even)
+constexpr int kWinCR = 30;    // chroma rows (15 + 15)
+
+struct Window {
+    int16_t y[kWinLR * kWinLS];
+    int16_t u[kWinCR * kWinCS];
+    int16_t v[kWinCR * kWinCS];
+};
+
+struct SubpelLds {
+    Window win;
+    HalfPlanes<16> y;
+    HalfPlanes<8> u;
+    HalfPlanes<8> v;
+    int32_t org[4];           // luma x0, y0, chroma x0, y0
+};
+
+// Coalesced copy of the window around (bx, by) with reach `reach` into LDS
+// (all 64 lanes; each lane keeps one column and strides over rows, so the
+// loop has no divisions and its loads issue back to back).
+__device__ void stage_window(const DevPic &ref, int bx, int by, int reach, SubpelLds &sp)
 {
-    sad_h = 0; mad_h = 0; sad_q = 0; mad_q = 0;
+    const int lane = lane_id();
+    const int x0 = (bx - reach - 3) & ~1, y0 = by - reach - 3;
+    const int lw = (bx + reach + 18 - x0 + 2) >> 1;          // dwords per luma row (<= 27)
+    const int lh = 2 * reach + 22;
+    const int cx0 = (((bx - reach) >> 1) - 3) & ~1, cy0 = ((by - reach) >> 1) - 3;
+    const int cw = ((((bx + reach) >> 1) + 10) - cx0 + 2) >> 1;   // <= 16
+    const int ch = (((by + reach) >> 1) + 10) - cy0 + 1;
+    {
+        const int per = 64 / lw, r0 = lane / lw, c = lane - r0 * lw;
+        const uint32_t *g = reinterpret_cast<const uint32_t *>(ref.y + y0 * ref.py + x0) + c;
+        uint32_t *l = reinterpret_cast<uint32_t *>(sp.win.y) + c;
+        if (r0 < per) {
+            int r = r0;
+            for (; r + 3 * per < lh; r += 4 * per) {
+                const uint32_t v0 = g[r * (ref.py / 2)];
+                const uint32_t v1 = g[(r + per) * (ref.py / 2)];
+                const uint32_t v2 = g[(r + 2 * per) * (ref.py / 2)];
+                const uint32_t v3 = g[(r + 3 * per) * (ref.py / 2)];
+                l[r * (kWinLS / 2)] = v0;
+                l[(r + per) * (kWinLS / 2)] = v1;
+                l[(r + 2 * per) * (kWinLS / 2)] = v2;
+                l[(r + 3 * per) * (kWinLS / 2)] = v3;
+            }
+            for (; r < lh; r += per) l[r * (kWinLS / 2)] = g[r * (ref.py / 2)];
+        }
+    }
+    {
+        // U and V rows interleaved: lanes 0..31 take U, 32..63 take V.
+        const int half = lane >> 5, ln = lane & 31;
+        const int per = 32 / cw, r0 = ln / cw, c = ln - r0 * cw;
+        const int16_t *gp = half ? ref.v : ref.u;
+        int16_t *lp = half ? sp.win.v : sp.win.u;
+        const uint32_t *g = reinterpret_cast<const uint32_t *>(gp + cy0 * ref.pc + cx0) + c;
+        uint32_t *l = reinterpret_cast<uint32_t *>(lp) + c;
+        if (r0 < per) {
+            int r = r0;
+            for (; r + per < ch; r += 2 * per) {
+                const uint32_t v0 = g[r * (ref.pc / 2)];
+                const uint32_t v1 = g[(r + per) * (ref.pc / 2)];
+                l[r * (kWinCS / 2)] = v0;
+                l[(r + per) * (kWinCS / 2)] = v1;
+            }
+            for (; r < ch; r += per) l[r * (kWinCS / 2)] = g[r * (ref.pc / 2)];
+        }
+    }
+    if (lane == 0) { sp.org[0] = x0; sp.org[1] = y0; sp.org[2] = cx0; sp.org[3] = cy0; }
+    __syncthreads();
+}
+
+// A candidate block from the staged window, in the lane layout of Src.
+__device__ __forceinline__ void load_cand_win(const SubpelLds &sp, const Src &s, int cx, int cy,
+                                              RefBlk &rb)
+{
+    const int16_t *p = sp.win.y + (cy + s.oy - sp.org[1]) * kWinLS + (cx + s.ox - sp.org[0]);
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int X = bx + s.ox + c, Y = by + s.oy + r;
-            const int full = ref.y[Y * ref.py + X];
-            const int half = half_sample(ref.y, ref.py, X, Y, dx, dy);
-            const int quarter = (int16_t)((full + half + 1) >> 1);
-            const int sv = s.y[r * 4 + c];
-            const int ah = abs(sv - half), aq = abs(sv - quarter);
-            sad_h += ah; mad_h = max(mad_h, ah);
-            sad_q += aq; mad_q = max(mad_q, aq);
-        }
-    const int cbx = bx >> 1, cby = by >> 1;
+        for (int c = 0; c < 4; c++) rb.y[r * 4 + c] = p[r * kWinLS + c];
+    const int co = ((cy >> 1) + s.cr - sp.org[3]) * kWinCS + (cx >> 1) + s.cc - sp.org[2];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int X = cbx + s.cc + k, Y = cby + s.cr;
-        int full = ref.u[Y * ref.pc + X];
-        int half = half_sample(ref.u, ref.pc, X, Y, dx, dy);
-        int quarter = (int16_t)((full + half + 1) >> 1);
-        mad_h = max(mad_h, abs(s.u[k] - half));
-        mad_q = max(mad_q, abs(s.u[k] - quarter));
-        full = ref.v[Y * ref.pc + X];
-        half = half_sample(ref.v, ref.pc, X, Y, dx, dy);
-        quarter = (int16_t)((full + half + 1) >> 1);
-        mad_h = max(mad_h, abs(s.v[k] - half));
-        mad_q = max(mad_q, abs(s.v[k] - quarter));
+    for (int k = 0; k < 4; k++) { rb.u[k] = sp.win.u[co + k]; rb.v[k] = sp.win.v[co + k]; }
+}
+
+// First pass: the h and v planes of one picture plane.  `plane` addresses
+// the block base (bx, by) and has row pitch `pitch`.
+template <int B>
+__device__ __forceinline__ void build_hv(const int16_t *plane, int pitch, HalfPlanes<B> &hp)
+{
+    using P = HalfPlanes<B>;
+    const int nh = P::HR * P::HC, nv = P::VR * P::VC;
+    for (int i = lane_id(); i < nh + nv; i += 64) {
+        if (i < nh) {
+            const int r = i / P::HC - 3, c = i % P::HC - 1;
+            hp.h[i] = (int16_t)tap6(plane + r * pitch + c - 2, 1);
+        } else {
+            const int k = i - nh;
+            const int r = k / P::VC - 1, c = k % P::VC;
+            hp.v[k] = (int16_t)tap6(plane + (r - 2) * pitch + c, pitch);
+        }
     }
+}
+
+// Second pass: the diagonal plane from the h plane in LDS.
+template <int B>
+__device__ __forceinline__ void build_d(HalfPlanes<B> &hp)
+{
+    using P = HalfPlanes<B>;
+    for (int i = lane_id(); i < P::DR * P::DC; i += 64) {
+        const int r = i / P::DC, c = i % P::DC;   // plane rows -1.., h rows -3..
+        hp.d[i] = (int16_t)tap6(hp.h + r * P::HC + c, P::HC);
+    }
+}
+
+// Where direction (dx, dy)'s half-pel block sits inside the planes.
+struct PlaneView {
+    const int16_t *base;
+    int stride;
+};
+
+template <int B>
+__device__ __forceinline__ PlaneView plane_view(const HalfPlanes<B> &hp, int dx, int dy)
+{
+    using P = HalfPlanes<B>;
+    const int xo = dx > 0 ? 0 : -1, yo = dy > 0 ? 0 : -1;
+    PlaneView pv;
+    if (dy == 0) { pv.base = hp.h + 3 * P::HC + (xo + 1); pv.stride = P::HC; }
+    else if (dx == 0) { pv.base = hp.v + (yo + 1) * P::VC; pv.stride = P::VC; }
+    else { pv.base = hp.d + (yo + 1) * P::DC + (xo + 1); pv.stride = P::DC; }
+    return pv;
 }
 
 struct SearchOut {
@@ -285,174 +392,320 @@ struct SearchOut {
     int n_int, n_ax, n_dg;
 };
 
+// Offsets of scan slot k (0..7) of a 3x3 level without its centre, in scan
+// order (j outer, i inner; motion.cpp:388-389, 464-465).  Slot k is also the
+// frac index of the sub-pel direction (motion.cpp:68-91).
+__device__ __forceinline__ constexpr int slot_i(int k) { return (k < 4 ? k : k + 1) % 3 - 1; }
+__device__ __forceinline__ constexpr int slot_j(int k) { return (k < 4 ? k : k + 1) / 3 - 1; }
+
+// RD cost of a candidate (motion.cpp:135-145): fp32 lambda * bits, then
+// + (float)SATD, no contraction.  Costs are non-negative, so their IEEE bits
+// order like the values and the fold compares them as integers.
+__device__ __forceinline__ uint32_t cost_bits(int sad, int satd_raw, int mvx, int mvy, float lambda)
+{
+    if (!(lambda > 0.0f)) return __float_as_uint(__int2float_rn(sad));
+    const int bits = abs16(mvx) + abs16(mvy) + 1;   // motion.cpp:120-123
+    return __float_as_uint(__fadd_rn(__int2float_rn((satd_raw + 1) >> 1),
+                                     __fmul_rn(lambda, __int2float_rn(bits))));
+}
+
 // perform_inter_search (motion.cpp:567-679) for the MB at (x, y) against
 // `ref`, predictor (pmx, pmy) as compute_mv_predictor produced it.  All
-// lanes of the wave call it with uniform arguments; it returns uniform
-// values.
-__device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s, int x, int y,
-                                  int thresh, float lambda, int pmx, int pmy)
+// lanes call it with uniform arguments; it returns uniform values.
+// `pre` holds the reference block at (x, y) in the lane layout of Src (it is
+// loaded while the MB waits for its neighbours).
+//
+// Lane group g (lanes 16g .. 16g+15) evaluates scan slots g and g+4 of every
+// level and sub-pel round; each lane owns one 4x4 luma block and 8 chroma
+// samples of the candidate.  Metrics and costs are computed for all slots in
+// parallel; the ordered fold then runs on wave-uniform integers.
+__device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s, const RefBlk &pre,
+                                  int x, int y, int thresh, float lambda, int pmx, int pmy,
+                                  SubpelLds &sp)
 {
     const int g = lane_id() >> 4;
+    const int ia = slot_i(g), ja = slot_j(g), ib = slot_i(g + 4), jb = slot_j(g + 4);
     const int max_x = W - 16, max_y = H - 16;
-    SearchOut o;
-    o.best_x = x; o.best_y = y;
-    o.ssd = kInt32Max;
-    o.sp_en = 0; o.sp_amt = 0; o.sp_idx = 0;
-    o.n_ax = 0; o.n_dg = 0;
+    int best_x = x, best_y = y, best_sad, best_mad, best_ssd = kInt32Max;
+    uint32_t best_cost;
+    int sp_en = 0, sp_amt = 0, sp_idx = 0, n_int = 1, n_ax = 0, n_dg = 0;
 
     // Initial candidate (motion.cpp:600-611) in group 0 and the predictor
     // candidate (motion.cpp:621-634, clamped into the picture) in group 1.
     const bool has_pred = pmx != 0 || pmy != 0;
     const int px = min(max(x + pmx, 0), max_x), py = min(max(y + pmy, 0), max_y);
     int sad, mad, satd;
-    {
-        const int cx = (g == 1 && has_pred) ? px : x;
-        const int cy = (g == 1 && has_pred) ? py : y;
-        eval_int_lane(ref, s, cx, cy, sad, mad, satd);
-        sad = row_sum(sad); mad = row_max(mad); satd = row_sum(satd);
+    if (has_pred) {
+        RefBlk rp, rb;
+        load_cand(ref, s, px, py, rp);
+        select_blk(g == 1, pre, rp, rb);
+        cand_metrics(s, rb, sad, mad, satd);
+    } else {
+        cand_metrics(s, pre, sad, mad, satd);
     }
-    o.sad = lane_value(sad, 0);
-    o.mad = lane_value(mad, 0);
-    o.cost = (float)o.sad;
-    if (lambda > 0.0f) {
-        const int st = (lane_value(satd, 0) + 1) >> 1;
-        o.cost = __fadd_rn(__int2float_rn(st), __fmul_rn(lambda, 1.0f));
-    }
-    o.n_int = 1;
+    sad = row_sum(sad); mad = row_max(mad); satd = row_sum(satd);
+    const uint32_t cst0 = g == 1 ? cost_bits(sad, satd, px - x, py - y, lambda)
+                                 : cost_bits(sad, satd, 0, 0, lambda);
+    best_sad = lane_value(sad, 0);
+    best_mad = lane_value(mad, 0);
+    best_cost = (uint32_t)lane_value((int)cst0, 0);
 
-    // evaluate_motion_candidate's acceptance (motion.cpp:125-178) for a
-    // candidate whose RD cost is already known.
-    auto fold_cost = [&](int cx, int cy, int csad, int cmad, float cost) {
+    // evaluate_motion_candidate's acceptance (motion.cpp:125-178), C
+    // precedence: && binds tighter than ||.
+    auto fold = [&](int cx, int cy, int csad, int cmad, uint32_t ccost) {
         const int ssd = (cx - x) * (cx - x) + (cy - y) * (cy - y);
         bool accept;
-        if (o.mad < thresh)
-            accept = cmad < o.mad || (cmad == o.mad && ssd < o.ssd);
+        if (best_mad < thresh)
+            accept = cmad < best_mad || (cmad == best_mad && ssd < best_ssd);
         else
-            accept = (cost < o.cost || ((cost == o.cost && ssd < o.ssd) && csad < kSadThreshold)) ||
+            accept = (ccost < best_cost ||
+                      ((ccost == best_cost && ssd < best_ssd) && csad < kSadThreshold)) ||
                      cmad < thresh;
         if (accept) {
-            o.best_x = cx; o.best_y = cy;
-            o.sad = csad; o.ssd = ssd; o.mad = cmad; o.cost = cost;
+            best_x = cx; best_y = cy;
+            best_sad = csad; best_ssd = ssd; best_mad = cmad; best_cost = ccost;
         }
-    };
-    auto fold = [&](int cx, int cy, int csad, int cmad, int csatd_raw) {
-        float cost = (float)csad;
-        if (lambda > 0.0f) {
-            const int bits = abs16(cx - x) + abs16(cy - y) + 1;   // motion.cpp:120-123
-            cost = __fadd_rn(__int2float_rn((csatd_raw + 1) >> 1),
-                             __fmul_rn(lambda, __int2float_rn(bits)));
-        }
-        fold_cost(cx, cy, csad, cmad, cost);
     };
 
     if (has_pred) {
-        fold(px, py, lane_value(sad, 16), lane_value(mad, 16), lane_value(satd, 16));
-        o.n_int++;
+        fold(px, py, lane_value(sad, 16), lane_value(mad, 16), (uint32_t)lane_value((int)cst0, 16));
+        n_int++;
     }
 
-    if (o.mad >= thresh) {
+    if (best_mad >= thresh) {
         int radius = kSearchRadius;
-        if (o.sad < kSadThreshold / 4) radius = kSearchRadius / 4;
-        else if (o.sad < kSadThreshold / 2) radius = kSearchRadius / 2;
+        if (best_sad < kSadThreshold / 4) radius = kSearchRadius / 4;
+        else if (best_sad < kSadThreshold / 2) radius = kSearchRadius / 2;
 
-        bool copy = false;
+        bool copy = false, staged = false;
         for (int step = radius; step > 0 && !copy; step >>= 1) {
             // perform_inter_motion_search (motion.cpp:381-406).  The centre of
-            // each 3x3 level is the level's starting best, so its metrics are
-            // known without evaluation; it is still folded at its place in
-            // the scan (fifth), because the initial candidate carries
-            // best_ssd = INT32_MAX and a tie on cost then moves best_ssd to
-            // the centre's true distance.
-            const int base_x = o.best_x, base_y = o.best_y;
-            const int c_sad = o.sad, c_mad = o.mad;
-            const float c_cost = o.cost;
-            bool centre_done = false;
+            // the level is its starting best, so its metrics are known; it is
+            // still folded at its place in the scan (after slot 3), because
+            // the initial candidate carries best_ssd = INT32_MAX and a tie on
+            // cost then moves best_ssd to the centre's true distance.
+            const int base_x = best_x, base_y = best_y;
+            if (step <= 8 && !staged) {
+                stage_window(ref, base_x, base_y, 2 * step - 1, sp);
+                staged = true;
+            }
             uint32_t mask = 0;
+#pragma unroll
             for (int k = 0; k < 8; k++) {
-                int i, j;
-                scan_direction(k, i, j);
-                const int cx = base_x + i * step, cy = base_y + j * step;
+                const int cx = base_x + slot_i(k) * step, cy = base_y + slot_j(k) * step;
                 if (cx >= 0 && cx <= max_x && cy >= 0 && cy <= max_y) mask |= 1u << k;
             }
-            while (mask && !copy) {
-                const int k = nth_bit(mask, g);
-                int i, j;
-                scan_direction(k < 0 ? 0 : k, i, j);
-                const int cx = k < 0 ? base_x : base_x + i * step;
-                const int cy = k < 0 ? base_y : base_y + j * step;
-                eval_int_lane(ref, s, cx, cy, sad, mad, satd);
-                sad = row_sum(sad); mad = row_max(mad); satd = row_sum(satd);
-                for (int gg = 0; gg < 4 && mask; gg++) {
-                    const int kk = __builtin_ctz(mask);
-                    mask &= mask - 1;
-                    if (kk >= 4 && !centre_done) {
-                        fold_cost(base_x, base_y, c_sad, c_mad, c_cost);
-                        centre_done = true;
-                    }
-                    int ii, jj;
-                    scan_direction(kk, ii, jj);
-                    fold(base_x + ii * step, base_y + jj * step, lane_value(sad, gg * 16),
-                         lane_value(mad, gg * 16), lane_value(satd, gg * 16));
-                    o.n_int++;
-                    if (o.mad < thresh) { copy = true; break; }   // motion.cpp:402-403
+            // This lane's two slots, clamped into the picture (an out-of-range
+            // slot is evaluated at a safe position and never folded).
+            const int ax = min(max(base_x + ia * step, 0), max_x);
+            const int ay = min(max(base_y + ja * step, 0), max_y);
+            const int bxx = min(max(base_x + ib * step, 0), max_x);
+            const int byy = min(max(base_y + jb * step, 0), max_y);
+            RefBlk ra, rb;
+            const bool two = (mask & 0xF0u) != 0;
+            if (staged) load_cand_win(sp, s, ax, ay, ra);
+            else load_cand(ref, s, ax, ay, ra);
+            if (two) {
+                if (staged) load_cand_win(sp, s, bxx, byy, rb);
+                else load_cand(ref, s, bxx, byy, rb);
+            }
+            int sad_a, mad_a, satd_a, sad_b = 0, mad_b = 0, satd_b = 0;
+            cand_metrics(s, ra, sad_a, mad_a, satd_a);
+            sad_a = row_sum(sad_a); mad_a = row_max(mad_a); satd_a = row_sum(satd_a);
+            const uint32_t cst_a = cost_bits(sad_a, satd_a, ax - x, ay - y, lambda);
+            uint32_t cst_b = 0;
+            if (two) {
+                cand_metrics(s, rb, sad_b, mad_b, satd_b);
+                sad_b = row_sum(sad_b); mad_b = row_max(mad_b); satd_b = row_sum(satd_b);
+                cst_b = cost_bits(sad_b, satd_b, bxx - x, byy - y, lambda);
+            }
+            const int c_sad = best_sad, c_mad = best_mad;
+            const uint32_t c_cost = best_cost;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (k == 4 && !copy) fold(base_x, base_y, c_sad, c_mad, c_cost);
+                if (!copy && (mask >> k & 1u)) {
+                    const int ln = (k & 3) * 16;
+                    const int csad = lane_value(k < 4 ? sad_a : sad_b, ln);
+                    const int cmad = lane_value(k < 4 ? mad_a : mad_b, ln);
+                    const uint32_t cc = (uint32_t)lane_value((int)(k < 4 ? cst_a : cst_b), ln);
+                    fold(base_x + slot_i(k) * step, base_y + slot_j(k) * step, csad, cmad, cc);
+                    n_int++;
+                    if (best_mad < thresh) copy = true;   // motion.cpp:402-403
                 }
             }
-            if (!copy && !centre_done) fold_cost(base_x, base_y, c_sad, c_mad, c_cost);
         }
 
-        // perform_inter_subpixel_motion_search (motion.cpp:450-483).
+        // perform_inter_subpixel_motion_search (motion.cpp:450-483): all
+        // candidates interpolate around the integer best (motion.cpp:186, 226).
         uint32_t mask = 0;
+#pragma unroll
         for (int k = 0; k < 8; k++) {
-            int i, j;
-            scan_direction(k, i, j);
-            const int tx = o.best_x + i, ty = o.best_y + j;
+            const int tx = best_x + slot_i(k), ty = best_y + slot_j(k);
             if (tx >= 0 && tx <= max_x && ty >= 0 && ty <= max_y) mask |= 1u << k;
         }
-        while (mask) {
-            const int k = nth_bit(mask, g);
-            int i, j;
-            scan_direction(k < 0 ? 0 : k, i, j);
-            int sh, mh, sq, mq;
-            eval_subpel_lane(ref, s, o.best_x, o.best_y, i, j, sh, mh, sq, mq);
-            sh = row_sum(sh); mh = row_max(mh); sq = row_sum(sq); mq = row_max(mq);
-            for (int gg = 0; gg < 4 && mask; gg++) {
-                const int kk = __builtin_ctz(mask);
-                mask &= mask - 1;
-                int ii, jj;
-                scan_direction(kk, ii, jj);
-                if (ii != 0 && jj != 0) o.n_dg++; else o.n_ax++;
-                const int csad[2] = {lane_value(sh, gg * 16), lane_value(sq, gg * 16)};
-                const int cmad[2] = {lane_value(mh, gg * 16), lane_value(mq, gg * 16)};
+        if (mask) {
+            if (!staged) stage_window(ref, best_x, best_y, 0, sp);
+            const int bx = best_x, by = best_y;
+            const int wlx = bx - sp.org[0], wly = by - sp.org[1];
+            const int wcx = (bx >> 1) - sp.org[2], wcy = (by >> 1) - sp.org[3];
+            build_hv<16>(sp.win.y + wly * kWinLS + wlx, kWinLS, sp.y);
+            build_hv<8>(sp.win.u + wcy * kWinCS + wcx, kWinCS, sp.u);
+            build_hv<8>(sp.win.v + wcy * kWinCS + wcx, kWinCS, sp.v);
+            RefBlk full;
+            load_cand_win(sp, s, bx, by, full);
+            __syncthreads();
+            build_d<16>(sp.y);
+            build_d<8>(sp.u);
+            build_d<8>(sp.v);
+            __syncthreads();
+            // Half and quarter metrics of this lane's two directions.
+            int m[8];
+#pragma unroll
+            for (int slot = 0; slot < 2; slot++) {
+                const int di = slot ? ib : ia, dj = slot ? jb : ja;
+                const PlaneView py_ = plane_view<16>(sp.y, di, dj);
+                const PlaneView pu_ = plane_view<8>(sp.u, di, dj);
+                const PlaneView pv_ = plane_view<8>(sp.v, di, dj);
+                int sh = 0, mh = 0, sq = 0, mq = 0;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int half = py_.base[(s.oy + r) * py_.stride + s.ox + c];
+                        const int quarter = (int16_t)((full.y[r * 4 + c] + half + 1) >> 1);
+                        const int sv = s.y[r * 4 + c];
+                        const int ah = abs(sv - half), aq = abs(sv - quarter);
+                        sh += ah; mh = max(mh, ah);
+                        sq += aq; mq = max(mq, aq);
+                    }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    int half = pu_.base[s.cr * pu_.stride + s.cc + c];
+                    int quarter = (int16_t)((full.u[c] + half + 1) >> 1);
+                    mh = max(mh, abs(s.u[c] - half));
+                    mq = max(mq, abs(s.u[c] - quarter));
+                    half = pv_.base[s.cr * pv_.stride + s.cc + c];
+                    quarter = (int16_t)((full.v[c] + half + 1) >> 1);
+                    mh = max(mh, abs(s.v[c] - half));
+                    mq = max(mq, abs(s.v[c] - quarter));
+                }
+                m[slot * 4 + 0] = row_sum(sh);
+                m[slot * 4 + 1] = row_max(mh);
+                m[slot * 4 + 2] = row_sum(sq);
+                m[slot * 4 + 3] = row_max(mq);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (!(mask >> k & 1u)) continue;
+                if (slot_i(k) != 0 && slot_j(k) != 0) n_dg++; else n_ax++;
+                const int ln = (k & 3) * 16, base = (k >> 2) * 4;
 #pragma unroll
                 for (int q = 0; q < 2; q++) {
+                    const int csad = lane_value(m[base + 2 * q], ln);
+                    const int cmad = lane_value(m[base + 2 * q + 1], ln);
                     bool accept;
-                    if (o.mad < thresh)
-                        accept = cmad[q] < o.mad;
+                    if (best_mad < thresh)
+                        accept = cmad < best_mad;
                     else
-                        accept = (csad[q] < o.sad && csad[q] < kSadThreshold) || cmad[q] < thresh;
+                        accept = (csad < best_sad && csad < kSadThreshold) || cmad < thresh;
                     if (accept) {
-                        o.sp_en = 1; o.sp_amt = q; o.sp_idx = kk;
-                        o.sad = csad[q]; o.mad = cmad[q];
+                        sp_en = 1; sp_amt = q; sp_idx = k;
+                        best_sad = csad; best_mad = cmad;
                     }
                 }
             }
         }
+        __syncthreads();
     }
 
+    SearchOut o;
+    o.best_x = best_x; o.best_y = best_y;
+    o.sad = best_sad; o.mad = best_mad; o.ssd = best_ssd;
+    o.cost = __uint_as_float(best_cost);
+    o.sp_en = sp_en; o.sp_amt = sp_amt; o.sp_idx = sp_idx;
+    o.n_int = n_int; o.n_ax = n_ax; o.n_dg = n_dg;
     o.btype = 0;
-    if (o.best_x != x || o.best_y != y || o.sp_en) o.btype |= EVXA_BLOCK_MOTION;
-    if (o.mad < thresh) o.btype |= EVXA_BLOCK_COPY;
+    if (best_x != x || best_y != y || sp_en) o.btype |= EVXA_BLOCK_MOTION;
+    if (best_mad < thresh) o.btype |= EVXA_BLOCK_COPY;
     return o;
 }
 
-// Prediction of a search result into LDS (encode.cpp:696-723).
-__device__ __forceinline__ void predict_mb(const DevPic &ref, int x, int y, const SearchOut &r,
-                                           int16_t *out)
+// Scratch of the MB-level stages (selection, prediction, DCT).
+struct MbLds {
+    int16_t src[EVXA_MB_SAMPLES];
+    int16_t pred[EVXA_MB_SAMPLES];
+    int16_t pred_b[EVXA_MB_SAMPLES];
+    int16_t res[EVXA_MB_SAMPLES];
+    int16_t tmp[16 * 21 + 2 * 8 * 13];   // diagonal H pass / DCT rows
+};
+
+union Shared {
+    SubpelLds sp;
+    MbLds mb;
+};
+
+// Residual prediction for a search result into `out` (encode.cpp:696-723;
+// interp.cpp:166-202 when sub-pel).  `tmp` holds the diagonal H pass.
+__device__ void predict_mb(const DevPic &ref, const SearchOut &r, int16_t *out, int16_t *tmp)
 {
+    const int lane = lane_id();
+    const int bx = r.best_x, by = r.best_y, cbx = bx >> 1, cby = by >> 1;
     int dx = 0, dy = 0;
     if (r.sp_en) frac_direction(r.sp_idx, dx, dy);
-    for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64)
-        out[i] = (int16_t)pred_sample(ref, i, r.best_x, r.best_y, r.sp_en, dx, dy, r.sp_amt);
+    const int xo = dx > 0 ? 0 : -1, yo = dy > 0 ? 0 : -1;
+    auto plane_of = [&](int i, const int16_t *&pl, int &pitch, int &X, int &Y) {
+        if (i < 256) { pl = ref.y; pitch = ref.py; X = bx + (i & 15); Y = by + (i >> 4); }
+        else {
+            const int k = (i - 256) & 63;
+            pl = i < 320 ? ref.u : ref.v; pitch = ref.pc;
+            X = cbx + (k & 7); Y = cby + (k >> 3);
+        }
+    };
+    if (!r.sp_en) {
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+            const int16_t *pl; int pitch, X, Y;
+            plane_of(i, pl, pitch, X, Y);
+            out[i] = pl[Y * pitch + X];
+        }
+        __syncthreads();
+        return;
+    }
+    if (dx != 0 && dy != 0) {
+        // interp.cpp:125-131: H pass over 21 (luma) / 13 (chroma) rows.
+        for (int i = lane; i < 16 * 21 + 2 * 8 * 13; i += 64) {
+            const int16_t *pl; int pitch, X, Y;
+            if (i < 336) { pl = ref.y; pitch = ref.py; X = bx + (i & 15); Y = by + (i >> 4) - 2; }
+            else {
+                const int k = (i - 336) % 104;
+                pl = i < 440 ? ref.u : ref.v; pitch = ref.pc;
+                X = cbx + (k & 7); Y = cby + (k >> 3) - 2;
+            }
+            tmp[i] = (int16_t)tap6(pl + (Y + yo) * pitch + X + xo - 2, 1);
+        }
+        __syncthreads();
+    }
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        const int16_t *pl; int pitch, X, Y;
+        plane_of(i, pl, pitch, X, Y);
+        int half;
+        if (dy == 0) half = tap6(pl + Y * pitch + X + xo - 2, 1);
+        else if (dx == 0) half = tap6(pl + (Y + yo - 2) * pitch + X, pitch);
+        else {
+            // interp.cpp:134-149 over the H pass.
+            const int16_t *t;
+            int ts;
+            if (i < 256) { t = tmp + (i >> 4) * 16 + (i & 15); ts = 16; }
+            else {
+                const int k = (i - 256) & 63;
+                t = tmp + 336 + (i < 320 ? 0 : 104) + (k >> 3) * 8 + (k & 7); ts = 8;
+            }
+            half = tap6(t, ts);
+        }
+        const int full = pl[Y * pitch + X];
+        out[i] = (int16_t)(r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half);
+    }
+    __syncthreads();
 }
 
 // transform.cpp:264-284 on a line of 8 samples.
@@ -467,7 +720,9 @@ __device__ __forceinline__ void dct_line(const int16_t *src, int sp, int16_t *ds
 #pragma unroll
         for (int k = 0; k < 8; k++) t += v[k] * c_dct8[i * 8 + k];
         t = (i == 0) ? (t * 45) / 128 : t / 2;
-        dst[i * dp] = (int16_t)rounded_div(t, 128);
+        // rounded_div(t, 128), evx_math.h:228-236
+        t = t < 0 ? (t - 64) / 128 : (t + 64) / 128;
+        dst[i * dp] = (int16_t)t;
     }
 }
 
@@ -482,8 +737,7 @@ __device__ __forceinline__ void block_origin(int blk, int &origin, int &pitch)
 // Residual, forward DCT, adaptive QP, variance and inter quantisation of one
 // MB held in LDS (encode.cpp:717-728; transform.cpp:435-452, 584-594;
 // quantize.cpp:97-118, 217-252, 372-397; analysis.h:253-275).
-// res/tmp are LDS scratch; coefficients end in `res`.  Returns qp and
-// variance2 (uniform); levels are written to `lv` (global).
+// Coefficients and levels go to global memory; returns qp and variance2.
 __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res, int16_t *tmp,
                           int quality, int16_t *co_out, int16_t *lv_out, int &qp_out,
                           int &var_out)
@@ -506,8 +760,8 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
     }
     __syncthreads();
 
-    // compute_block_variance2 over the 16x16 luma coefficients, (0,0) skipped;
-    // the int32 sums wrap like the reference's.
+    // compute_block_variance2 over the 16x16 luma coefficients, (0,0)
+    // skipped; the int32 sums wrap like the reference's.
     int sum = 0, cnt = 0;
     uint32_t sq = 0;
 #pragma unroll
@@ -537,18 +791,21 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
     delta = min(max(delta, -6), 6);
     const int qp = min(max(q8 + delta, 1), 63);
     const int cqp = min(qp + 2, 63);
+    const uint32_t m_luma = magic_of(qp << 1), m_chroma = magic_of(cqp << 1);
 
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         int pos, bq;
-        if (i < 256) { pos = ((i >> 4) & 7) * 8 + (i & 7); bq = qp; }
-        else { pos = (i - 256) & 63; bq = cqp; }
+        uint32_t mq;
+        if (i < 256) { pos = ((i >> 4) & 7) * 8 + (i & 7); bq = qp; mq = m_luma; }
+        else { pos = (i - 256) & 63; bq = cqp; mq = m_chroma; }
         const int sv = res[i];
-        const int qf = (int16_t)rounded_div(sv * 16, c_inter_qm[pos]);
+        const int qm = c_inter_qm[pos];
+        const int qf = (int16_t)rdiv_magic(sv * 16, qm, magic_of(qm));
         int w = c_deadzone[pos];
         if (bq >= 16) w = min(w + 1, 6);
         const int sg = (qf > 0) - (qf < 0);
         const int dz = (int16_t)((sg * bq * w) >> 1);
-        lv_out[i] = (int16_t)rounded_div(qf - dz, bq << 1);
+        lv_out[i] = (int16_t)rdiv_magic(qf - dz, bq << 1, mq);
         co_out[i] = (int16_t)sv;
     }
     qp_out = qp;
@@ -563,37 +820,47 @@ __device__ __forceinline__ int16_t load_src_sample(const DevPic &pic, int x, int
     return pl[((y >> 1) + (k >> 3)) * pic.pc + (x >> 1) + (k & 7)];
 }
 
-__device__ __forceinline__ uint32_t ld_acquire(const int32_t *p)
-{
-    return (uint32_t)__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// compute_mv_predictor (motion.cpp:271-346) from published records.
+// compute_mv_predictor (motion.cpp:271-346) from published records.  The
+// five candidate records are fetched together (one memory latency), then
+// combined in the reference's order.
 __device__ void mv_predictor(const DevJob &job, int bx, int by, int &pmx, int &pmy)
 {
     const int idx = by * job.wb + bx;
-    int mx[3], my[3], n = 0;
-    auto take = [&](const evxa_block_record *r, bool cond) -> bool {
-        if (!cond) return false;
-        const uint32_t *w = reinterpret_cast<const uint32_t *>(r);
-        const uint32_t w0 = ld_relaxed(w);
-        if (!(w0 & EVXA_BLOCK_MOTION)) return false;
-        const uint32_t w1 = ld_relaxed(w + 1);
-        mx[n] = (int16_t)(w1 & 0xFFFF);
-        my[n] = (int16_t)(w1 >> 16);
+    const uint32_t *rec = reinterpret_cast<const uint32_t *>(job.records);
+    const bool has_l = bx > 0, has_t = by > 0, has_tr = by > 0 && bx + 1 < job.wb;
+    const bool has_tl = by > 0 && bx > 0, has_p = job.prev != nullptr;
+    const int il = has_l ? idx - 1 : idx, it = has_t ? idx - job.wb : idx;
+    const int itr = has_tr ? idx - job.wb + 1 : idx, itl = has_tl ? idx - job.wb - 1 : idx;
+    // Two words per record (type, MV); out-of-range neighbours read this MB's
+    // own (not yet written) slot and are ignored.
+    const uint32_t l0 = ld_relaxed(rec + il * 16), l1 = ld_relaxed(rec + il * 16 + 1);
+    const uint32_t t0 = ld_relaxed(rec + it * 16), t1 = ld_relaxed(rec + it * 16 + 1);
+    const uint32_t r0 = ld_relaxed(rec + itr * 16), r1 = ld_relaxed(rec + itr * 16 + 1);
+    const uint32_t q0 = ld_relaxed(rec + itl * 16), q1 = ld_relaxed(rec + itl * 16 + 1);
+    uint32_t p0 = 0, p1 = 0;
+    if (has_p) {
+        const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + idx * 16;
+        p0 = prv[0];
+        p1 = prv[1];
+    }
+    int mx[3] = {0, 0, 0}, my[3] = {0, 0, 0}, n = 0;
+    auto take = [&](uint32_t w1) {
+        const int vx = (int16_t)(w1 & 0xFFFF), vy = (int16_t)(w1 >> 16);
+        if (n == 0) { mx[0] = vx; my[0] = vy; }
+        else if (n == 1) { mx[1] = vx; my[1] = vy; }
+        else { mx[2] = vx; my[2] = vy; }
         n++;
-        return true;
     };
-    take(job.records + idx - 1, bx > 0);
-    take(job.records + idx - job.wb, by > 0);
-    if (!take(job.records + idx - job.wb + 1, by > 0 && bx + 1 < job.wb))
-        take(job.records + idx - job.wb - 1, by > 0 && bx > 0);
-    if (n < 3 && job.prev) take(job.prev + idx, true);
+    if (has_l && (l0 & EVXA_BLOCK_MOTION)) take(l1);
+    if (has_t && (t0 & EVXA_BLOCK_MOTION)) take(t1);
+    if (has_tr && (r0 & EVXA_BLOCK_MOTION)) take(r1);
+    else if (has_tl && (q0 & EVXA_BLOCK_MOTION)) take(q1);
+    if (n < 3 && has_p && (p0 & EVXA_BLOCK_MOTION)) take(p1);
     if (n == 0) { pmx = 0; pmy = 0; }
     else if (n == 1) { pmx = mx[0]; pmy = my[0]; }
     else if (n == 2) { pmx = (int16_t)((mx[0] + mx[1]) / 2); pmy = (int16_t)((my[0] + my[1]) / 2); }
@@ -609,50 +876,77 @@ __device__ void mv_predictor(const DevJob &job, int bx, int by, int &pmx, int &p
     }
 }
 
-__device__ __forceinline__ void write_record(evxa_block_record *dst, const uint32_t (&w)[16])
+__device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, int n)
 {
     const int lane = lane_id();
-    if (lane < 16) {
+    if (lane < n) {
         uint32_t v = w[0];
-#pragma unroll
-        for (int k = 1; k < 16; k++) v = lane == k ? w[k] : v;
-        reinterpret_cast<uint32_t *>(dst)[lane] = v;
+        for (int k = 1; k < n; k++) v = lane == k ? w[k] : v;
+        dst[lane] = v;
     }
 }
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 
-struct MbScratch {
-    int16_t src[EVXA_MB_SAMPLES];
-    int16_t pred[EVXA_MB_SAMPLES];
-    int16_t pred_b[EVXA_MB_SAMPLES];
-    int16_t res[EVXA_MB_SAMPLES];
-    int16_t tmp[EVXA_MB_SAMPLES];
-};
+__device__ __forceinline__ void pack_search(const SearchOut &o, uint32_t (&w)[8])
+{
+    w[0] = (uint32_t)(uint16_t)o.best_x | ((uint32_t)(uint16_t)o.best_y << 16);
+    w[1] = (uint32_t)o.sad;
+    w[2] = (uint32_t)o.mad;
+    w[3] = (uint32_t)o.ssd;
+    w[4] = fbits(o.cost);
+    w[5] = (uint32_t)o.sp_en | ((uint32_t)o.sp_amt << 8) | ((uint32_t)o.sp_idx << 16) |
+           ((uint32_t)o.btype << 24);
+    w[6] = (uint32_t)(o.n_int & 0xFFFF) | ((uint32_t)(o.n_ax & 0xFF) << 16) |
+           ((uint32_t)(o.n_dg & 0xFF) << 24);
+    w[7] = 0;
+}
 
-// Analyse one MB of a job: searches, selection, record publication, then
-// prediction, DCT and quantisation.
-__device__ void analyze_mb(const DevJob &job, int bx, int by, int32_t *counters, MbScratch &sh)
+// Search result k of a MB, read wave-uniformly from the words in `v`
+// (lane l holds word l of the MB's SearchRec array).
+__device__ __forceinline__ SearchOut unpack_search(int v, int k)
+{
+    SearchOut o;
+    const uint32_t w0 = (uint32_t)lane_value(v, 8 * k);
+    o.best_x = (int16_t)(w0 & 0xFFFF);
+    o.best_y = (int16_t)(w0 >> 16);
+    o.sad = lane_value(v, 8 * k + 1);
+    o.mad = lane_value(v, 8 * k + 2);
+    o.ssd = lane_value(v, 8 * k + 3);
+    o.cost = __uint_as_float((uint32_t)lane_value(v, 8 * k + 4));
+    const uint32_t w5 = (uint32_t)lane_value(v, 8 * k + 5);
+    o.sp_en = w5 & 0xFF; o.sp_amt = (w5 >> 8) & 0xFF; o.sp_idx = (w5 >> 16) & 0xFF;
+    o.btype = w5 >> 24;
+    const uint32_t w6 = (uint32_t)lane_value(v, 8 * k + 6);
+    o.n_int = w6 & 0xFFFF; o.n_ax = (w6 >> 16) & 0xFF; o.n_dg = w6 >> 24;
+    return o;
+}
+
+// Selection among a MB's searches, record publication, residual prediction,
+// DCT and quantisation; run by the wave that finished the MB's last search.
+// With a single reference the caller passes its search result in `own`;
+// otherwise all results are read back from job.searches.
+__device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
+                          const SearchOut *own)
 {
     const int lane = lane_id();
     const int x = bx * 16, y = by * 16;
     const int idx = by * job.wb + bx;
     const float lambda = job.lambda;
-
-    Src s;
-    load_src(job.src, x, y, s);
-    int pmx, pmy;
-    mv_predictor(job, bx, by, pmx, pmy);
+    const uint32_t *words =
+        reinterpret_cast<const uint32_t *>(job.searches + (size_t)idx * EVXA_MAX_REFS);
+    const int v = (!own && lane < 8 * job.n_refs) ? (int)ld_relaxed(words + lane) : 0;
 
     SearchOut best, other;
+    other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
     int best_k = -1, best_sad = kInt32Max, bidir = 0;
     int n_int = 0, n_ax = 0, n_dg = 0;
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.src[i] = load_src_sample(job.src, x, y, i);
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel.
         bool best_copy = false;
         for (int k = 0; k < job.n_refs; k++) {
-            SearchOut r = inter_search(job.refs[k], job.width, job.height, s, x, y, job.thresh,
-                                       lambda, pmx, pmy);
+            const SearchOut r = own ? *own : unpack_search(v, k);
             n_int += r.n_int; n_ax += r.n_ax; n_dg += r.n_dg;
             const int bias = __float2int_rz(__fmul_rn(lambda, __int2float_rn((k + 1) * 2)));
             const bool copy = (r.btype & EVXA_BLOCK_COPY) != 0;
@@ -665,21 +959,16 @@ __device__ void analyze_mb(const DevJob &job, int bx, int by, int32_t *counters,
     } else {
         // encode.cpp:386-443 without the intra candidates; the bidirectional
         // candidate (motion.cpp:702-840) reuses both searches.
-        SearchOut f = inter_search(job.refs[0], job.width, job.height, s, x, y, job.thresh, lambda,
-                                   pmx, pmy);
-        SearchOut b = inter_search(job.refs[1], job.width, job.height, s, x, y, job.thresh, lambda,
-                                   pmx, pmy);
+        const SearchOut f = unpack_search(v, 0);
+        const SearchOut b = unpack_search(v, 1);
         n_int = f.n_int + b.n_int; n_ax = f.n_ax + b.n_ax; n_dg = f.n_dg + b.n_dg;
         if (f.sad < best_sad) { best = f; best_k = 0; best_sad = f.sad; }
         if (b.sad < best_sad) { best = b; best_k = 1; best_sad = b.sad; }
-        predict_mb(job.refs[0], x, y, f, sh.pred);
-        predict_mb(job.refs[1], x, y, b, sh.pred_b);
-        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.src[i] = load_src_sample(job.src, x, y, i);
-        __syncthreads();
+        predict_mb(job.refs[0], f, sh.pred, sh.tmp);
+        predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
         int bsad = 0;
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
             const int a = (sh.pred[i] + sh.pred_b[i] + 1) >> 1;
-            sh.tmp[i] = (int16_t)a;
             bsad += abs(sh.src[i] - a);
         }
         bsad = wave_sum(bsad);
@@ -688,10 +977,9 @@ __device__ void analyze_mb(const DevJob &job, int bx, int by, int32_t *counters,
             bidir = 1; best_sad = bsad; best = f; best_k = 0; other = b;
         }
         __syncthreads();
-        if (!bidir && best_k == 1) {
-            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.pred[i] = sh.pred_b[i];
-        } else if (bidir) {
-            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.pred[i] = sh.tmp[i];
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+            if (bidir) sh.pred[i] = (int16_t)((sh.pred[i] + sh.pred_b[i] + 1) >> 1);
+            else if (best_k == 1) sh.pred[i] = sh.pred_b[i];
         }
         __syncthreads();
     }
@@ -727,33 +1015,34 @@ __device__ void analyze_mb(const DevJob &job, int bx, int by, int32_t *counters,
     w[13] = (uint32_t)n_ax | ((uint32_t)n_dg << 16);
     w[14] = 0;
     w[15] = 0;
-    evxa_block_record *rec = job.records + idx;
-    write_record(rec, w);
+    uint32_t *rec = reinterpret_cast<uint32_t *>(job.records + idx);
+    write_words(rec, w, 16);
 
     // Publish: the record is visible before the row counter moves past it.
     __threadfence();
     __syncthreads();
     if (lane == 0)
-        __hip_atomic_store(job.progress + by, bx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(job.progress + by * kProgressStride, bx + 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
 
     int16_t *co = job.coeffs + (size_t)idx * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)idx * EVXA_MB_SAMPLES;
     if (btype & EVXA_BLOCK_COPY) {
         // encode.cpp:742-744: copy blocks are not transformed.
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) { co[i] = 0; lv[i] = 0; }
+        __syncthreads();
         return;
     }
-    if (job.type == EVXA_FRAME_P) {
-        predict_mb(job.refs[best_k], x, y, best, sh.pred);
-        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.src[i] = load_src_sample(job.src, x, y, i);
-        __syncthreads();
-    }
+    if (job.type == EVXA_FRAME_P) predict_mb(job.refs[best_k], best, sh.pred, sh.tmp);
+    __syncthreads();
     int qp, var2;
     dct_quant(sh.src, sh.pred, sh.res, sh.tmp, job.quality, co, lv, qp, var2);
-    w[4] |= (uint32_t)qp << 16;
-    w[5] = (uint32_t)(uint16_t)(int16_t)var2;
-    w[14] = 1u | (bidir ? 2u : 0u);
-    write_record(rec, w);
+    uint32_t w2[2];
+    w2[0] = w[4] | ((uint32_t)qp << 16);
+    w2[1] = (uint32_t)(uint16_t)(int16_t)var2;
+    write_words(rec + 4, w2, 2);
+    w2[0] = 1u | (bidir ? 2u : 0u);
+    write_words(rec + 14, w2, 1);
     __syncthreads();
 }
 
@@ -761,38 +1050,75 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
                                                      const uint32_t *__restrict__ tickets,
                                                      int32_t n_tickets, int32_t *counters)
 {
-    __shared__ MbScratch sh;
-    __shared__ int32_t s_ticket;
+    __shared__ Shared sh;
+    __shared__ int32_t s_word;
     const int lane = lane_id();
     for (;;) {
-        if (lane == 0) s_ticket = atomicAdd(&counters[kCounterTicket], 1);
+        if (lane == 0) s_word = atomicAdd(&counters[kCounterTicket], 1);
         __syncthreads();
-        const int t = s_ticket;
+        const int t = s_word;
         __syncthreads();
         if (t >= n_tickets) break;
         const uint32_t e = tickets[t];
         const DevJob &job = jobs[e >> kJobShift];
+        const int ref = (int)((e >> kRefShift) & 3);
         const int mb = (int)(e & kMbMask);
         const int bx = mb % job.wb, by = mb / job.wb;
+        const int x = bx * 16, y = by * 16;
+
+        // Load the source block and the initial candidate while the
+        // neighbours finish.
+        Src s;
+        lane_geometry(s);
+        load_lane_block(job.src, x, y, s, s.y, s.u, s.v);
+        RefBlk pre;
+        load_cand(job.refs[ref], s, x, y, pre);
+
         // Wait for the left neighbour (same row) and the top-right one (row
-        // above; the whole row above at the right edge).
-        int spins = 0;
+        // above; the whole row above at the right edge), backing off.
+        int spins = 0, nap = 1;
         bool timed_out = false;
-        if (bx > 0) {
-            while ((int)ld_acquire(job.progress + by) < bx) {
-                if (++spins > kSpinLimit) { timed_out = true; break; }
-                __builtin_amdgcn_s_sleep(2);
+        // Polling uses relaxed loads (served by L2 without invalidating this
+        // CU's L1 under the waves that are computing); one acquire fence
+        // after the condition holds orders the neighbours' records.
+        auto wait_for = [&](const int32_t *p, int need) {
+            while ((int)ld_relaxed(reinterpret_cast<const uint32_t *>(p)) < need) {
+                if (++spins > kSpinLimit) { timed_out = true; return; }
+                for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(2);
+                nap = min(nap * 2, 4);
             }
-        }
-        if (by > 0) {
-            const int need = min(bx + 2, job.wb);
-            while ((int)ld_acquire(job.progress + by - 1) < need) {
-                if (++spins > kSpinLimit) { timed_out = true; break; }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
+        };
+        if (bx > 0) wait_for(job.progress + by * kProgressStride, bx);
+        if (by > 0 && !timed_out)
+            wait_for(job.progress + (by - 1) * kProgressStride, min(bx + 2, job.wb));
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (timed_out && lane == 0) atomicOr(&counters[kCounterError], kErrWaitTimeout);
-        analyze_mb(job, bx, by, counters, sh);
+
+        int pmx, pmy;
+        mv_predictor(job, bx, by, pmx, pmy);
+        const SearchOut o = inter_search(job.refs[ref], job.width, job.height, s, pre, x, y,
+                                         job.thresh, job.lambda, pmx, pmy, sh.sp);
+        if (job.n_refs == 1) {
+            // Single reference: this wave finishes the MB directly.
+            finish_mb(job, bx, by, counters, sh.mb, &o);
+            continue;
+        }
+        uint32_t w[8];
+        pack_search(o, w);
+        write_words(reinterpret_cast<uint32_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref),
+                    w, 8);
+        __threadfence();
+        __syncthreads();
+        if (lane == 0) {
+            const int old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            s_word = old;
+            if (old == job.n_refs - 1) job.mb_done[mb] = 0;   // last one: back to zero
+        }
+        __syncthreads();
+        const int done_before = s_word;
+        __syncthreads();
+        if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, nullptr);
     }
 }
 
@@ -819,14 +1145,20 @@ __global__ void apron_kernel(int16_t *plane0, int32_t pitch, int32_t w, int32_t 
 __global__ __launch_bounds__(64) void interp_blocks_kernel(DevPic pic, int32_t n,
                                                            const int32_t *params, int16_t *out)
 {
+    __shared__ MbLds sh;
     const int b = blockIdx.x;
     if (b >= n) return;
     const int32_t *p = params + b * 5;
-    const int dx = p[2], dy = p[3];
-    const int sp = (dx != 0 || dy != 0);
+    SearchOut r{};
+    r.best_x = p[0]; r.best_y = p[1];
+    r.sp_en = (p[2] != 0 || p[3] != 0);
+    r.sp_amt = p[4];
+    // frac index of the direction (motion.cpp:68-91)
+    const int m = (p[3] + 1) * 3 + (p[2] + 1);
+    r.sp_idx = m < 4 ? m : m - 1;
+    predict_mb(pic, r, sh.pred, sh.tmp);
     for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64)
-        out[(size_t)b * EVXA_MB_SAMPLES + i] =
-            (int16_t)pred_sample(pic, i, p[0], p[1], sp, dx, dy, p[4]);
+        out[(size_t)b * EVXA_MB_SAMPLES + i] = sh.pred[i];
 }
 
 // params: n x {x, y, pred_x, pred_y}
@@ -836,12 +1168,17 @@ __global__ __launch_bounds__(64) void search_blocks_kernel(DevPic src, DevPic re
                                                            const int32_t *params,
                                                            evxa_search_result *out)
 {
+    __shared__ SubpelLds sp;
     const int b = blockIdx.x;
     if (b >= n) return;
     const int32_t *p = params + b * 4;
     Src s;
-    load_src(src, p[0], p[1], s);
-    SearchOut o = inter_search(ref, w, h, s, p[0], p[1], (quality >> 2) + 1, lambda, p[2], p[3]);
+    lane_geometry(s);
+    load_lane_block(src, p[0], p[1], s, s.y, s.u, s.v);
+    RefBlk pre;
+    load_cand(ref, s, p[0], p[1], pre);
+    SearchOut o = inter_search(ref, w, h, s, pre, p[0], p[1], (quality >> 2) + 1, lambda, p[2], p[3],
+                               sp);
     if (lane_id() == 0) {
         evxa_search_result r;
         r.best_x = o.best_x; r.best_y = o.best_y;
@@ -861,7 +1198,7 @@ __global__ __launch_bounds__(64) void dct_quant_blocks_kernel(int32_t n, const i
                                                               int16_t *coeffs, int16_t *levels,
                                                               int32_t *qp, int32_t *variance)
 {
-    __shared__ MbScratch sh;
+    __shared__ MbLds sh;
     const int b = blockIdx.x;
     if (b >= n) return;
     for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64) {

@@ -59,7 +59,8 @@ EXPORTS = (
     "evxa_create", "evxa_destroy", "evxa_last_error", "evxa_version", "evxa_stream_bytes",
     "evxa_context_bytes", "evxa_lambda", "evxa_stream_open", "evxa_stream_close",
     "evxa_upload_picture", "evxa_stream_reset_temporal", "evxa_analyze_frame",
-    "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_read_results",
+    "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_launch_times",
+    "evxa_read_results",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks")
 
@@ -100,6 +101,8 @@ def load_library(path: str = LIB_PATH):
         "evxa_analyze_batch": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32]),
         "evxa_synchronize": (C.c_int, [vp]),
         "evxa_last_launch_ms": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "evxa_launch_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32,
+                                        C.POINTER(C.c_int32)]),
         "evxa_read_results": (C.c_int, [vp, vp, I16P, I16P]),
         "evxa_graph_create": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32,
                                         C.POINTER(C.c_int32)]),
@@ -204,6 +207,13 @@ class Context:
         ms = C.c_float()
         _check(_lib.evxa_last_launch_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def launch_times(self, max_n=8192):
+        """Device ms of each analysis launch not reported before (oldest first)."""
+        buf = (C.c_float * max_n)()
+        n = C.c_int32()
+        _check(_lib.evxa_launch_times(self.h, buf, max_n, C.byref(n)))
+        return np.array(buf[:n.value], np.float64)
 
     def analyze_batch(self, streams, descs):
         n = len(streams)

@@ -101,35 +101,27 @@ def move_blocks(rng, prev, mvx, mvy, sp_state, noise_sigma=2.0, visible=None):
     mvx/mvy: int arrays (hb, wb) of integer displacements; sp_state: int array
     (hb, wb) with -1 for none, else dir_index*2 + quarter (dir_index into DIRS).
     """
-    y, u, v = prev
-    ah, aw = y.shape
+    ah, aw = prev[0].shape
     hb, wb = ah // 16, aw // 16
-    out = [np.empty_like(y), np.empty_like(u), np.empty_like(v)]
-    planes = {}
-
-    def plane(ci, st):
-        key = (ci, st)
-        if key not in planes:
-            src = prev[ci]
-            if st < 0:
-                planes[key] = src
-            else:
-                dx, dy = DIRS[st // 2]
-                planes[key] = subpel_plane(src, dx, dy, st & 1)
-        return planes[key]
-
-    for by in range(hb):
-        for bx in range(wb):
-            x, yy = bx * 16, by * 16
-            sx = int(np.clip(x + mvx[by, bx], 0, aw - 16))
-            sy = int(np.clip(yy + mvy[by, bx], 0, ah - 16))
-            st = int(sp_state[by, bx])
-            out[0][yy:yy + 16, x:x + 16] = plane(0, st)[sy:sy + 16, sx:sx + 16]
-            cx, cy, scx, scy = x // 2, yy // 2, sx >> 1, sy >> 1
-            out[1][cy:cy + 8, cx:cx + 8] = plane(1, st)[scy:scy + 8, scx:scx + 8]
-            out[2][cy:cy + 8, cx:cx + 8] = plane(2, st)[scy:scy + 8, scx:scx + 8]
+    bys, bxs = np.mgrid[0:hb, 0:wb]
+    sx = np.clip(bxs * 16 + np.asarray(mvx), 0, aw - 16)
+    sy = np.clip(bys * 16 + np.asarray(mvy), 0, ah - 16)
+    sp_state = np.asarray(sp_state)
+    states = np.unique(sp_state)
+    slot = np.searchsorted(states, sp_state)
+    out = []
+    for ci, src in enumerate(prev):
+        bs, sh = (16, 0) if ci == 0 else (8, 1)
+        stack = np.stack([src if st < 0 else
+                          subpel_plane(src, DIRS[st // 2][0], DIRS[st // 2][1], st & 1)
+                          for st in states])
+        r = np.arange(bs)
+        rows = (sy >> sh)[:, :, None, None] + r[None, None, :, None]
+        cols = (sx >> sh)[:, :, None, None] + r[None, None, None, :]
+        blocks = stack[slot[:, :, None, None], rows, cols]          # (hb, wb, bs, bs)
+        out.append(blocks.transpose(0, 2, 1, 3).reshape(hb * bs, wb * bs))
     for i in range(3):
-        n = np.rint(rng.normal(0.0, noise_sigma, size=out[i].shape))
+        n = np.rint(rng.normal(0.0, noise_sigma, size=out[i].shape)) if noise_sigma > 0 else 0
         out[i] = np.clip(out[i].astype(np.float64) + n, 0, 255).astype(np.int16)
     if visible is not None:
         w, h = visible

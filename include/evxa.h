@@ -143,6 +143,10 @@ int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams,
 int evxa_synchronize(evxa_context *ctx);
 /* Device time (ms) of the last analysis launch, from HIP events around it. */
 int evxa_last_launch_ms(evxa_context *ctx, float *ms);
+/* Waits for the queue, then reports the device times (ms, HIP events around
+ * each analysis kernel) of up to `max` launches not reported before, oldest
+ * first; *n_out receives the count.  Graph replays are not included. */
+int evxa_launch_times(evxa_context *ctx, float *ms, int32_t max, int32_t *n_out);
 /* Copy the last analysed frame's outputs to host memory (any may be NULL):
  * one record per MB in raster order, 384 coefficients and 384 pre-trellis
  * levels per MB (zero for copy blocks).  Blocks until they are available. */
