@@ -1,0 +1,88 @@
+"""The oracle against the reference's own build (oracle/_ref) on fresh
+random inputs.  Skipped where the reference build is absent (it is built from
+the reference tree by `make -C oracle ref`).  CPU only."""
+import numpy as np
+import pytest
+
+from chicago_amd import synth
+from oracle.oracle import REF_FIELDS, Picture, reference_available
+
+pytestmark = pytest.mark.skipif(not reference_available(), reason="oracle/_ref not built")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    from oracle.oracle import Reference
+    return Reference()
+
+
+def test_lambda(oracle, ref):
+    for q in range(0, 256):
+        assert np.float32(oracle.lam(q)).view(np.uint32) == np.float32(ref.lam(q)).view(np.uint32)
+
+
+def test_interp_random(oracle, ref):
+    rng = np.random.default_rng(101)
+    for lo, hi in ((0, 256), (-32768, 32767)):
+        y = rng.integers(lo, hi, (48, 80)).astype(np.int16)
+        u = rng.integers(lo, hi, (24, 40)).astype(np.int16)
+        v = rng.integers(lo, hi, (24, 40)).astype(np.int16)
+        P = Picture(y, u, v)
+        n = 300
+        bx = rng.integers(0, 80 - 15, n); by = rng.integers(0, 48 - 15, n)
+        d = rng.integers(0, 8, n)
+        dx = np.array([synth.DIRS[k][0] for k in d]); dy = np.array([synth.DIRS[k][1] for k in d])
+        q = rng.integers(0, 2, n)
+        want = ref.interp_many(P, bx, by, dx, dy, q)
+        for k in range(n):
+            assert np.array_equal(oracle.interp(P, int(bx[k]), int(by[k]), int(dx[k]), int(dy[k]),
+                                                int(q[k])), want[k])
+
+
+def test_metrics_dct_quant_random(oracle, ref):
+    rng = np.random.default_rng(202)
+    for k in range(400):
+        amp = (255, 600, 4095, 32767)[k % 4]
+        a = rng.integers(-amp, amp + 1, 384).astype(np.int16)
+        b = rng.integers(-amp, amp + 1, 384).astype(np.int16)
+        if amp <= 4095:
+            assert oracle.metrics(a, b) == ref.metrics(a, b)
+            c = oracle.sub_transform(a, b)
+            assert np.array_equal(c, ref.sub_transform(a, b))
+            assert oracle.variance2(c) == ref.variance2(c)
+            for q in (0, 3, 8, 12, 16, 24, 40, 63):
+                assert oracle.block_qp(q, c) == ref.block_qp(q, c)
+            for qp in (1, 2, 8, 15, 16, 17, 33, 63):
+                assert np.array_equal(oracle.quantize(qp, c), ref.quantize(qp, c))
+
+
+@pytest.mark.parametrize("quality", [8, 10, 13])
+def test_inter_search_random(oracle, ref, quality):
+    r_pic, c_pic = synth.pair_c0(900 + quality, 128, 96)
+    R, S = Picture(*r_pic), Picture(*c_pic)
+    rng = np.random.default_rng(quality)
+    for yy in range(0, 96, 16):
+        for xx in range(0, 128, 16):
+            for pm in ((0, 0), tuple(int(v) for v in rng.integers(-30, 31, 2))):
+                r = oracle.inter_search(S.block(xx, yy), R, xx, yy, quality, *pm)
+                w = ref.inter_search(S.block(xx, yy), R, xx, yy, quality, *pm)
+                assert (r.block_type, r.motion_x, r.motion_y, r.sp_enabled, r.sp_amount,
+                        r.sp_index, r.sad) == (w["block_type"], w["motion_x"], w["motion_y"],
+                                               w["sp_pred"], w["sp_amount"], w["sp_index"],
+                                               w["sad"])
+
+
+@pytest.mark.parametrize("ftype,nrefs,quality", [(1, 1, 8), (1, 3, 9), (1, 4, 14), (2, 2, 10)])
+def test_frames_random(oracle, ref, ftype, nrefs, quality):
+    frames = synth.sequence_c1(1000 + nrefs * 7 + quality, 6, 144, 80, local_range=14)
+    pics = [Picture(*f) for f in frames]
+    if ftype == 1:
+        cur, refs = pics[5], [pics[4 - k] for k in range(nrefs)]
+        prev = oracle.analyze_frame(1, quality, pics[4], [pics[3]])[0]
+    else:
+        cur, refs, prev = pics[3], [pics[2], pics[5]], None
+    a = oracle.analyze_frame(ftype, quality, cur, refs, prev=prev)
+    b = ref.analyze_frame(ftype, quality, cur, refs, prev=prev)
+    for f in REF_FIELDS:
+        assert np.array_equal(a[0][f], b[0][f]), f
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
