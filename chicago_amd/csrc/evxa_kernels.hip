@@ -519,20 +519,78 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
                 sad_b = row_sum(sad_b); mad_b = row_max(mad_b); satd_b = row_sum(satd_b);
                 cst_b = cost_bits(sad_b, satd_b, bxx - x, byy - y, lambda);
             }
-            const int c_sad = best_sad, c_mad = best_mad;
-            const uint32_t c_cost = best_cost;
+            // The level's fold (motion.cpp:388-405 over evaluate_motion_candidate)
+            // in closed form.  Order: the level's starting best b0, slots
+            // 0..3, the centre (b0 again, with its true distance), slots 4..7.
+            //  * If a valid slot has MAD < T, the first such slot is accepted
+            //    unconditionally and the level ends there (copy state).
+            //  * Otherwise the best ends at the minimum cost m: the first
+            //    element (in order) at cost m, replaced by any later element
+            //    at cost m with a strictly smaller ssd and SAD < 8192.
+            // Branch-free over wave-uniform values.
+            uint32_t sc[8];
+            int ss[8], sm[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                if (k == 4 && !copy) fold(base_x, base_y, c_sad, c_mad, c_cost);
-                if (!copy && (mask >> k & 1u)) {
-                    const int ln = (k & 3) * 16;
-                    const int csad = lane_value(k < 4 ? sad_a : sad_b, ln);
-                    const int cmad = lane_value(k < 4 ? mad_a : mad_b, ln);
-                    const uint32_t cc = (uint32_t)lane_value((int)(k < 4 ? cst_a : cst_b), ln);
-                    fold(base_x + slot_i(k) * step, base_y + slot_j(k) * step, csad, cmad, cc);
-                    n_int++;
-                    if (best_mad < thresh) copy = true;   // motion.cpp:402-403
+                const int ln = (k & 3) * 16;
+                sc[k] = (uint32_t)lane_value((int)(k < 4 ? cst_a : cst_b), ln);
+                ss[k] = lane_value(k < 4 ? sad_a : sad_b, ln);
+                sm[k] = lane_value(k < 4 ? mad_a : mad_b, ln);
+            }
+            uint32_t copym = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                copym |= (((mask >> k) & 1u) && sm[k] < thresh) ? (1u << k) : 0u;
+            auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
+            if (copym) {
+                const int p = __builtin_ctz(copym);
+                n_int += __builtin_popcount(mask & ((2u << p) - 1u));
+                int nx = base_x, ny = base_y, nsad = 0, nmad = 0;
+                uint32_t nc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const bool t = k == p;
+                    nx = t ? base_x + slot_i(k) * step : nx;
+                    ny = t ? base_y + slot_j(k) * step : ny;
+                    nsad = t ? ss[k] : nsad;
+                    nmad = t ? sm[k] : nmad;
+                    nc = t ? sc[k] : nc;
                 }
+                best_x = nx; best_y = ny; best_sad = nsad; best_mad = nmad; best_cost = nc;
+                best_ssd = dist(nx, ny);
+                copy = true;                                   // motion.cpp:402-403
+            } else {
+                n_int += __builtin_popcount(mask);
+                uint32_t m = best_cost;
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    m = ((mask >> k) & 1u) ? min(m, sc[k]) : m;
+                bool have = best_cost == m;
+                int cx = best_x, cy = best_y, csad = best_sad, cmad = best_mad, cssd = best_ssd;
+                auto visit = [&](bool valid, uint32_t cost, int ex, int ey, int esad, int emad,
+                                 int essd) {
+                    const bool eq = valid && cost == m;
+                    const bool take = eq && (!have || (essd < cssd && esad < kSadThreshold));
+                    cx = take ? ex : cx;
+                    cy = take ? ey : cy;
+                    csad = take ? esad : csad;
+                    cmad = take ? emad : cmad;
+                    cssd = take ? essd : cssd;
+                    have = have || eq;
+                };
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int ex = base_x + slot_i(k) * step, ey = base_y + slot_j(k) * step;
+                    visit(((mask >> k) & 1u) != 0, sc[k], ex, ey, ss[k], sm[k], dist(ex, ey));
+                }
+                visit(true, best_cost, base_x, base_y, best_sad, best_mad, dist(base_x, base_y));
+#pragma unroll
+                for (int k = 4; k < 8; k++) {
+                    const int ex = base_x + slot_i(k) * step, ey = base_y + slot_j(k) * step;
+                    visit(((mask >> k) & 1u) != 0, sc[k], ex, ey, ss[k], sm[k], dist(ex, ey));
+                }
+                best_x = cx; best_y = cy; best_sad = csad; best_mad = cmad; best_ssd = cssd;
+                best_cost = m;
             }
         }
 
@@ -547,8 +605,10 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
         if (mask) {
             if (!staged) stage_window(ref, best_x, best_y, 0, sp);
             const int bx = best_x, by = best_y;
-            const int wlx = bx - sp.org[0], wly = by - sp.org[1];
-            const int wcx = (bx >> 1) - sp.org[2], wcy = (by >> 1) - sp.org[3];
+            const int wlx = bx - __builtin_amdgcn_readfirstlane(sp.org[0]);
+            const int wly = by - __builtin_amdgcn_readfirstlane(sp.org[1]);
+            const int wcx = (bx >> 1) - __builtin_amdgcn_readfirstlane(sp.org[2]);
+            const int wcy = (by >> 1) - __builtin_amdgcn_readfirstlane(sp.org[3]);
             build_hv<16>(sp.win.y + wly * kWinLS + wlx, kWinLS, sp.y);
             build_hv<8>(sp.win.u + wcy * kWinCS + wcx, kWinCS, sp.u);
             build_hv<8>(sp.win.v + wcy * kWinCS + wcx, kWinCS, sp.v);
@@ -595,24 +655,29 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
                 m[slot * 4 + 2] = row_sum(sq);
                 m[slot * 4 + 3] = row_max(mq);
             }
+            // evaluate_subpel_motion_candidate's acceptance (motion.cpp:196-260)
+            // over the 16 candidates in order, branch-free; the copy mode can
+            // switch on mid-way and there is no early exit.
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                if (!(mask >> k & 1u)) continue;
-                if (slot_i(k) != 0 && slot_j(k) != 0) n_dg++; else n_ax++;
+                const bool valid = ((mask >> k) & 1u) != 0;
+                const bool diag = slot_i(k) != 0 && slot_j(k) != 0;
+                n_dg += (valid && diag) ? 1 : 0;
+                n_ax += (valid && !diag) ? 1 : 0;
                 const int ln = (k & 3) * 16, base = (k >> 2) * 4;
 #pragma unroll
                 for (int q = 0; q < 2; q++) {
                     const int csad = lane_value(m[base + 2 * q], ln);
                     const int cmad = lane_value(m[base + 2 * q + 1], ln);
-                    bool accept;
-                    if (best_mad < thresh)
-                        accept = cmad < best_mad;
-                    else
-                        accept = (csad < best_sad && csad < kSadThreshold) || cmad < thresh;
-                    if (accept) {
-                        sp_en = 1; sp_amt = q; sp_idx = k;
-                        best_sad = csad; best_mad = cmad;
-                    }
+                    const bool in_copy = best_mad < thresh;
+                    const bool accept =
+                        valid && (in_copy ? cmad < best_mad
+                                          : ((csad < best_sad && csad < kSadThreshold) || cmad < thresh));
+                    sp_en = accept ? 1 : sp_en;
+                    sp_amt = accept ? q : sp_amt;
+                    sp_idx = accept ? k : sp_idx;
+                    best_sad = accept ? csad : best_sad;
+                    best_mad = accept ? cmad : best_mad;
                 }
             }
         }
@@ -1056,7 +1121,10 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
     for (;;) {
         if (lane == 0) s_word = atomicAdd(&counters[kCounterTicket], 1);
         __syncthreads();
-        const int t = s_word;
+        // LDS hands the value back in a vector register; broadcasting it makes
+        // the work item, and all control derived from it, wave-uniform
+        // (scalar registers and scalar branches).
+        const int t = __builtin_amdgcn_readfirstlane(s_word);
         __syncthreads();
         if (t >= n_tickets) break;
         const uint32_t e = tickets[t];
@@ -1096,6 +1164,8 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
 
         int pmx, pmy;
         mv_predictor(job, bx, by, pmx, pmy);
+        pmx = __builtin_amdgcn_readfirstlane(pmx);
+        pmy = __builtin_amdgcn_readfirstlane(pmy);
         const SearchOut o = inter_search(job.refs[ref], job.width, job.height, s, pre, x, y,
                                          job.thresh, job.lambda, pmx, pmy, sh.sp);
         if (job.n_refs == 1) {
@@ -1116,7 +1186,7 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
             if (old == job.n_refs - 1) job.mb_done[mb] = 0;   // last one: back to zero
         }
         __syncthreads();
-        const int done_before = s_word;
+        const int done_before = __builtin_amdgcn_readfirstlane(s_word);
         __syncthreads();
         if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, nullptr);
     }
