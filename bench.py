@@ -314,7 +314,7 @@ def run_plain(args):
             "peak": VALU_PEAK_TOPS,
             "unit": "TOP/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "analyze_kernel",
             "kernel_ms_avg": round(k_avg_s * 1e3, 4),
             "algorithmic_ops_per_launch": int(ops_launch),
@@ -338,6 +338,54 @@ def run_plain(args):
     return result
 
 
+TRAFFIC_COPY_BYTES = 256 << 20
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "analyze_kernel_traffic_1080p.md")
+
+
+def run_traffic(args):
+    """Launches for the HBM-traffic counter passes: a plain device-to-device
+    copy of TRAFFIC_COPY_BYTES (the counters' calibration: it reads and
+    writes exactly that many bytes), then analysis launches of frame 5 of the
+    plain sequence (4 references)."""
+    import torch
+
+    from chicago_amd import evxa, synth
+
+    w, h = 1920, 1080
+    frames = synth.sequence_c1(synth.seed_of(1, 0, 0), 6, w, h)
+    a = torch.empty(TRAFFIC_COPY_BYTES, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ctx = evxa.Context(0)
+    s = ctx.open_stream(synth.align16(w), synth.align16(h), 6)
+    for t, f in enumerate(frames):
+        s.upload(t, *f)
+    for t in range(1, 6):
+        if t == 1:
+            s.reset_temporal()
+        s.analyze_p(args.quality, t, frame_refs(t))
+    ctx.synchronize()
+    s.close()
+    ctx.close()
+    log(f"traffic: {TRAFFIC_COPY_BYTES} B copy x3, analysis of frames 1-5")
+
+
+def measured_traffic():
+    """HBM bytes per analysis launch (frame 5, 4 refs) from the committed
+    counter summary: the line `traffic_bytes_per_launch: N`."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,7 +393,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--quality", type=int, default=8)
-    ap.add_argument("--workload", choices=["plain", "batched", "all"], default="all")
+    ap.add_argument("--workload", choices=["plain", "batched", "all", "traffic"], default="all")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=4)
@@ -356,6 +404,9 @@ def main():
               "stream with no cross-GPU exchange, so per-GPU numbers are the unit.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "traffic":
+        run_traffic(args)
+        return
     res = run_plain(args) if args.workload in ("all", "plain") else {"value": None}
     if args.workload in ("all", "batched"):
         from chicago_amd import evxa
