@@ -177,7 +177,9 @@ struct evxa_context {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
-    int32_t grid = 0;
+    int32_t grid = 0;                // most analysis waves per launch
+    int32_t lists = 0;               // work lists per launch (0: automatic)
+    int32_t cap_pct = 60;            // tickets per list and virtual diagonal, % of its waves
     std::mutex mu;
     LaunchRes res;
     // Ring of (start, stop) event pairs around analysis kernels; launches
@@ -196,7 +198,11 @@ struct evxa_stream {
     evxa_context *ctx = nullptr;
     int32_t w = 0, h = 0, wb = 0, hb = 0, n_slots = 0;
     PlaneGeom gy{}, gc{};
-    std::vector<std::unique_ptr<DevBuf>> slots;
+    // All picture slots in one allocation, slot k at k * slot_stride bytes:
+    // one large contiguous buffer maps with the largest page fragments, so the
+    // per-CU translation cache covers a stream's pictures with few entries.
+    DevBuf pics;
+    size_t slot_stride = 0;
     DevBuf tables;         // 3 record tables: P ping, P pong, B
     DevBuf coeffs, levels;
     DevBuf searches;       // EVXA_MAX_REFS SearchRec per MB
@@ -206,13 +212,17 @@ struct evxa_stream {
     int32_t last_table = -1;
     uint64_t bytes = 0;
 
+    int16_t *slot_base(int32_t slot) const
+    {
+        return reinterpret_cast<int16_t *>(static_cast<char *>(pics.p) + (size_t)slot * slot_stride);
+    }
     evxa_block_record *table(int k) const
     {
         return tables.as<evxa_block_record>() + (size_t)k * wb * hb;
     }
     DevPic pic(int32_t slot) const
     {
-        const int16_t *base = slots[slot]->as<int16_t>();
+        const int16_t *base = slot_base(slot);
         DevPic p;
         p.y = base + gy.origin();
         const int16_t *ub = base + (size_t)gy.pitch * gy.rows;
@@ -289,26 +299,106 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
 // reference searches of a MB adjacent, so every ticket's predecessors (left,
 // top-left, top, top-right MBs) carry smaller tickets and are owned by waves
 // already running.  Returns the largest number of items on one diagonal.
-int32_t build_tickets(const std::vector<DevJob> &jobs, std::vector<uint32_t> &out)
+// Work buffer of a launch (header + tickets, layout at kWorkHeader).
+//
+// Lists: each job's MB rows are laid end to end, weighted by their work
+// (MBs x refs), and cut into n_lists equal segments, so a large frame splits
+// into row bands over several lists (XCDs) and small ones share a list.
+//
+// Order: ticket (job j, MB (bx, by), ref r) has virtual diagonal
+// v = bx + 2 by + start[j]; each list runs by v, then job, row and ref.  The
+// per-job start (the same in every list) staggers the jobs so that each list
+// keeps about `per_list_cap` tickets per virtual diagonal: an XCD then works
+// on few pictures at a time and its L2 holds their search fronts.  Within a
+// job v follows the dependency order (left, top and top-right neighbours
+// have smaller v), and across jobs there are no dependencies, so the item
+// with the smallest unfinished v is always runnable: no list can deadlock.
+// Returns the most tickets any list holds on one virtual diagonal.
+int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per_list_cap,
+                   std::vector<uint32_t> &out)
 {
-    out.clear();
-    int32_t tmax = 0, widest = 0;
-    for (const auto &j : jobs) tmax = std::max(tmax, j.wb - 1 + 2 * (j.hb - 1));
-    for (int32_t t = 0; t <= tmax; t++) {
-        const size_t before = out.size();
+    int64_t total = 0;
+    for (const auto &j : jobs) total += (int64_t)j.wb * j.hb * j.n_refs;
+    std::vector<std::vector<uint8_t>> row_list(jobs.size());
+    int64_t cum = 0;
+    for (size_t k = 0; k < jobs.size(); k++) {
+        const DevJob &j = jobs[k];
+        row_list[k].resize(j.hb);
+        for (int32_t by = 0; by < j.hb; by++) {
+            const int64_t w = (int64_t)j.wb * j.n_refs;
+            row_list[k][by] = (uint8_t)std::min<int64_t>(n_lists - 1, (cum + w / 2) * n_lists / total);
+            cum += w;
+        }
+    }
+    // Tickets per (list, diagonal) of each job.
+    auto diagonals = [](const DevJob &j) { return j.wb + 2 * (j.hb - 1); };
+    std::vector<std::vector<int32_t>> prof(jobs.size());
+    for (size_t k = 0; k < jobs.size(); k++) {
+        const DevJob &j = jobs[k];
+        prof[k].assign((size_t)n_lists * diagonals(j), 0);
+        for (int32_t by = 0; by < j.hb; by++)
+            for (int32_t bx = 0; bx < j.wb; bx++)
+                prof[k][(size_t)row_list[k][by] * diagonals(j) + bx + 2 * by] += j.n_refs;
+    }
+    // Starts: largest jobs first, each at the earliest start that keeps every
+    // list within the cap (or, failing that, at the end of what is placed).
+    std::vector<int32_t> order(jobs.size()), start(jobs.size(), 0);
+    for (size_t k = 0; k < jobs.size(); k++) order[k] = (int32_t)k;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return (int64_t)jobs[a].wb * jobs[a].hb * jobs[a].n_refs >
+               (int64_t)jobs[b].wb * jobs[b].hb * jobs[b].n_refs;
+    });
+    std::vector<std::vector<int32_t>> load(n_lists);
+    int32_t horizon = 0;
+    for (int32_t k : order) {
+        const int32_t nd = diagonals(jobs[k]);
+        auto fits = [&](int32_t s0) {
+            for (int32_t l = 0; l < n_lists; l++)
+                for (int32_t t = 0; t < nd; t++) {
+                    const int32_t add = prof[k][(size_t)l * nd + t];
+                    if (!add) continue;
+                    const int32_t have = s0 + t < (int32_t)load[l].size() ? load[l][s0 + t] : 0;
+                    if (have > 0 && have + add > per_list_cap) return false;
+                }
+            return true;
+        };
+        int32_t s0 = 0;
+        while (s0 < horizon && !fits(s0)) s0 += 2;
+        start[k] = s0;
+        for (int32_t l = 0; l < n_lists; l++) {
+            if ((int32_t)load[l].size() < s0 + nd) load[l].resize(s0 + nd, 0);
+            for (int32_t t = 0; t < nd; t++) load[l][s0 + t] += prof[k][(size_t)l * nd + t];
+        }
+        horizon = std::max(horizon, s0 + nd);
+    }
+    int32_t widest = 0;
+    for (int32_t l = 0; l < n_lists; l++)
+        for (int32_t v : load[l]) widest = std::max(widest, v);
+    std::vector<std::vector<uint32_t>> lists(n_lists);
+    for (int32_t v = 0; v < horizon; v++)
         for (size_t k = 0; k < jobs.size(); k++) {
             const DevJob &j = jobs[k];
+            const int32_t t = v - start[k];
+            if (t < 0 || t >= diagonals(j)) continue;
             for (int32_t by = 0; by < j.hb; by++) {
                 const int32_t bx = t - 2 * by;
                 if (bx < 0) break;
                 if (bx >= j.wb) continue;
+                auto &dst = lists[row_list[k][by]];
                 for (int32_t r = 0; r < j.n_refs; r++)
-                    out.push_back(((uint32_t)k << kJobShift) | ((uint32_t)r << kRefShift) |
+                    dst.push_back(((uint32_t)k << kJobShift) | ((uint32_t)r << kRefShift) |
                                   (uint32_t)(by * j.wb + bx));
             }
         }
-        widest = std::max(widest, (int32_t)(out.size() - before));
+    out.assign(kWorkHeader, 0u);
+    out[0] = (uint32_t)n_lists;
+    uint32_t off = 0;
+    for (int32_t l = 0; l < n_lists; l++) {
+        out[1 + l] = off;
+        off += (uint32_t)lists[l].size();
+        out.insert(out.end(), lists[l].begin(), lists[l].end());
     }
+    out[1 + n_lists] = off;
     return widest;
 }
 
@@ -331,18 +421,25 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
 
     if (key != r.key) {
         std::vector<uint32_t> t;
-        const int32_t widest = build_tickets(jobs, t);
-        // Enough waves for the widest diagonal (plus slack for the MBs still
-        // finishing the previous one), never more than fit at once: waiting
-        // waves beyond that only add polling traffic.
-        r.grid = std::max(64, std::min(ctx->grid, widest + widest / 2 + 32));
+        // One list for a single picture (its rows all stay in flight), one
+        // per XCD for batches.
+        const int32_t lists = ctx->lists > 0 ? ctx->lists : (jobs.size() > 1 ? kLists : 1);
+        const int32_t n_lists = std::max(1, std::min(lists, ctx->grid));
+        const int32_t cap = std::max(16, ctx->grid / n_lists * ctx->cap_pct / 100);
+        const int32_t widest = build_work(jobs, n_lists, cap, t);
+        // Per list, enough waves for its widest diagonal (plus slack for the
+        // MBs still finishing the previous one), never more than fit at once:
+        // waiting waves beyond that only add polling traffic.  The grid is a
+        // multiple of n_lists so that every list has the same number of waves.
+        const int32_t per_list = std::max(8, widest + widest / 2 + 4);
+        r.grid = n_lists * std::max(1, std::min(ctx->grid / n_lists, per_list));
         r.h_tickets.ensure(t.size() * sizeof(uint32_t));
         memcpy(r.h_tickets.p, t.data(), t.size() * sizeof(uint32_t));
         if (r.tickets.n < t.size() * sizeof(uint32_t)) r.tickets.alloc(t.size() * sizeof(uint32_t));
         check(hipMemcpyAsync(r.tickets.p, r.h_tickets.p, t.size() * sizeof(uint32_t),
                              hipMemcpyHostToDevice, ctx->stream),
               "copy work list");
-        r.n_tickets = (int32_t)t.size();
+        r.n_tickets = (int32_t)t.size() - kWorkHeader;
         r.key = key;
     }
     const size_t prog_bytes = (size_t)(n_progress + kCounterWords) * sizeof(int32_t);
@@ -361,15 +458,14 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, ctx->stream), "copy jobs");
     check(hipEventRecord(r.staged, ctx->stream), "hipEventRecord");
     check(launch_reset(prog, n_progress + kCounterWords, ctx->stream), "reset kernel");
-    const int32_t grid = std::max(1, std::min(r.grid, r.n_tickets));
+    const int32_t grid = r.grid;
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (time_it) {
         if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
         ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
         check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
     }
-    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.n_tickets, prog, grid,
-                         ctx->stream),
+    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), prog, grid, ctx->stream),
           "analysis kernel");
     if (time_it) {
         check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
@@ -382,7 +478,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
 void check_counters(evxa_context *ctx, LaunchRes &r)
 {
     if (!r.progress.p) return;
-    int32_t c[kCounterWords];
+    int32_t c[kCounterError + 1];
     check(hipMemcpy(c, r.progress.p, sizeof(c), hipMemcpyDeviceToHost), "read counters");
     if (c[kCounterError] & kErrWaitTimeout)
         fail(EVXA_ERROR_OPERATION_TIMEDOUT, "analysis kernel: dependency wait timed out");
@@ -403,7 +499,8 @@ float evxa_lambda(int32_t quality) { return lambda_of(quality); }
 uint64_t evxa_stream_bytes(int32_t width, int32_t height, int32_t n_slots)
 {
     if (width <= 0 || height <= 0 || n_slots <= 0) return 0;
-    return (uint64_t)n_slots * picture_bytes(width, height) + outputs_bytes(width, height);
+    return (uint64_t)n_slots * align_up(picture_bytes(width, height), 4096) +
+           outputs_bytes(width, height);
 }
 
 uint64_t evxa_context_bytes(const evxa_context *ctx) { return ctx ? ctx->used : 0; }
@@ -435,6 +532,14 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
         if (const char *g = getenv("EVXA_ANALYZE_GRID")) {
             const int v = atoi(g);
             if (v > 0) ctx->grid = v;
+        }
+        if (const char *g = getenv("EVXA_LISTS")) {
+            const int v = atoi(g);
+            if (v > 0 && v <= kLists) ctx->lists = v;
+        }
+        if (const char *g = getenv("EVXA_CAP_PCT")) {
+            const int v = atoi(g);
+            if (v > 0) ctx->cap_pct = v;
         }
         *out = ctx.release();
     });
@@ -486,10 +591,9 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
         s->gy = plane_geom(width, height);
         s->gc = plane_geom(width / 2, height / 2);
         const size_t pb = picture_bytes(width, height);
-        for (int k = 0; k < n_slots; k++) {
-            s->slots.emplace_back(new DevBuf(pb));
-            check(hipMemsetAsync(s->slots.back()->p, 0, pb, ctx->stream), "clear slot");
-        }
+        s->slot_stride = align_up(pb, 4096);
+        s->pics.alloc(s->slot_stride * n_slots);
+        check(hipMemsetAsync(s->pics.p, 0, s->slot_stride * n_slots, ctx->stream), "clear slots");
         const size_t n = (size_t)s->wb * s->hb;
         s->tables.alloc(3 * n * sizeof(evxa_block_record));
         check(hipMemsetAsync(s->tables.p, 0, 3 * n * sizeof(evxa_block_record), ctx->stream),
@@ -541,7 +645,7 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
         evxa_context *ctx = s->ctx;
         std::lock_guard<std::mutex> lk(ctx->mu);
         set_device(ctx);
-        int16_t *base = s->slots[slot]->as<int16_t>();
+        int16_t *base = s->slot_base(slot);
         int16_t *py = base;
         int16_t *pu = base + (size_t)s->gy.pitch * s->gy.rows;
         int16_t *pv = pu + (size_t)s->gc.pitch * s->gc.rows;
@@ -682,12 +786,12 @@ int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams, const evxa
         check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
         hipError_t e1 = launch_reset(res->progress.as<int32_t>(), res->n_progress + kCounterWords,
                                      ctx->stream);
-        const int32_t grid = std::max(1, std::min(res->grid, res->n_tickets));
+        const int32_t grid = res->grid;
         check(hipEventCreate(&res->g_start), "hipEventCreate");
         check(hipEventCreate(&res->g_stop), "hipEventCreate");
         hipError_t e2 = hipEventRecord(res->g_start, ctx->stream);
         hipError_t e3 = launch_analyze(res->jobs.as<DevJob>(), res->tickets.as<uint32_t>(),
-                                       res->n_tickets, res->progress.as<int32_t>(), grid, ctx->stream);
+                                       res->progress.as<int32_t>(), grid, ctx->stream);
         hipError_t e4 = hipEventRecord(res->g_stop, ctx->stream);
         hipError_t e5 = hipStreamEndCapture(ctx->stream, &graph);
         check(e1, "capture reset");

@@ -64,10 +64,23 @@ constexpr int kRefShift = 20;
 constexpr int kJobShift = 22;
 constexpr int kMaxJobs = 1 << (32 - kJobShift);
 
-// Counter block of a launch: [0] next ticket, [1] error flags.
-constexpr int kCounterTicket = 0;
+// Work lists.  The work of a launch is split into up to kLists lists, one
+// per XCD of the MI355X: the dispatcher places workgroup w on XCD w % 8, so
+// workgroup w takes its tickets from list w % n_lists and each XCD's L2 holds
+// the pictures of its own share of rows.  Every list is ordered by
+// anti-diagonal, which keeps the wavefront deadlock-free whatever the
+// placement.  The work buffer starts with a header of kWorkHeader words:
+// [0] n_lists, [1 + k] first ticket of list k (k = 0 .. n_lists, the last
+// entry closes the final list); tickets follow.
+constexpr int kLists = 8;
+constexpr int kWorkHeader = 16;
+
+// Counter block of a launch: [1] error flags; the next-ticket counter of
+// list k at kCounterList + k * kCounterListStride (own 256-byte line each).
 constexpr int kCounterError = 1;
-constexpr int kCounterWords = 64;
+constexpr int kCounterList = 64;
+constexpr int kCounterListStride = 64;
+constexpr int kCounterWords = kCounterList + kLists * kCounterListStride;
 
 // Error flag bits.
 constexpr int32_t kErrSearchFailed = 1;
@@ -75,8 +88,8 @@ constexpr int32_t kErrWaitTimeout = 2;
 
 // Kernel launchers (evxa_kernels.hip).
 hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st);
-hipError_t launch_analyze(const DevJob *jobs, const uint32_t *tickets, int32_t n_tickets,
-                          int32_t *counters, int32_t grid, hipStream_t st);
+hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
+                          int32_t grid, hipStream_t st);
 hipError_t launch_apron(int16_t *plane0, int32_t pitch, int32_t w, int32_t h, hipStream_t st);
 hipError_t launch_interp_blocks(DevPic pic, int32_t w, int32_t h, int32_t n, const int32_t *params,
                                 int16_t *out, hipStream_t st);

@@ -31,7 +31,11 @@ namespace {
 constexpr int kSadThreshold = 8 * 1024;  // motion.cpp:24
 constexpr int kSearchRadius = 16;        // motion.cpp:29
 constexpr int kInt32Max = 0x7FFFFFFF;
-constexpr int kSpinLimit = 1 << 20;
+constexpr int kSpinLimit = 1 << 19;
+// Longest back-off between two polls of a dependency, in s_sleep(2) units
+// (about 128 cycles each): waits last tens of thousands of cycles, and every
+// poll is a trip to L2 through the CU's memory pipeline.
+constexpr int kMaxNap = 16;
 
 // xftables.h:57-67: round(128 cos((2k+1) i pi / 16)).
 __constant__ int16_t c_dct8[64] = {
@@ -151,19 +155,30 @@ __device__ __forceinline__ void lane_geometry(Src &s)
     s.cr = b & 7; s.cc = (b >> 3) * 4;
 }
 
-// Samples of the MB at (x, y) of `pic` in the lane layout of Src.
+// Four consecutive samples as one 8-byte access (global or LDS; the address
+// only needs 2-byte alignment, which gfx950 serves as one access).
+__device__ __forceinline__ void load4(const int16_t *p, int16_t *dst)
+{
+    uint2 q;
+    __builtin_memcpy(&q, p, 8);
+    dst[0] = (int16_t)(q.x & 0xFFFFu);
+    dst[1] = (int16_t)(q.x >> 16);
+    dst[2] = (int16_t)(q.y & 0xFFFFu);
+    dst[3] = (int16_t)(q.y >> 16);
+}
+
+// Samples of the MB at (x, y) of `pic` in the lane layout of Src: four
+// 8-byte rows of luma and one 8-byte run of each chroma plane per lane.
 __device__ __forceinline__ void load_lane_block(const DevPic &pic, int x, int y, const Src &g,
                                                 int16_t (&ly)[16], int16_t (&lu)[4],
                                                 int16_t (&lv)[4])
 {
     const int16_t *p = pic.y + (y + g.oy) * pic.py + x + g.ox;
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) ly[r * 4 + c] = p[r * pic.py + c];
+    for (int r = 0; r < 4; r++) load4(p + r * pic.py, ly + r * 4);
     const int co = ((y >> 1) + g.cr) * pic.pc + (x >> 1) + g.cc;
-#pragma unroll
-    for (int k = 0; k < 4; k++) { lu[k] = pic.u[co + k]; lv[k] = pic.v[co + k]; }
+    load4(pic.u + co, lu);
+    load4(pic.v + co, lv);
 }
 
 // Reference samples of one candidate position in the lane layout of Src.
@@ -269,57 +284,50 @@ struct SubpelLds {
     int32_t org[4];           // luma x0, y0, chroma x0, y0
 };
 
-// Coalesced copy of the window around (bx, by) with reach `reach` into LDS
-// (all 64 lanes; each lane keeps one column and strides over rows, so the
-// loop has no divisions and its loads issue back to back).
-__device__ void stage_window(const DevPic &ref, int bx, int by, int reach, SubpelLds &sp)
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Copies rows of `row_dw` dwords (row r of the source at g + r * pitch_dw)
+// into LDS packed with stride row_dw, n_rows rows, straight from memory to
+// LDS (global_load_lds_dword: lane i of the k-th load writes dword 64k + i).
+// All loads are issued back to back and complete under one wait.
+template <int ROW_DW>
+__device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_rows, uint32_t *lds)
 {
     const int lane = lane_id();
-    const int x0 = (bx - reach - 3) & ~1, y0 = by - reach - 3;
-    const int lw = (bx + reach + 18 - x0 + 2) >> 1;          // dwords per luma row (<= 27)
+    const int n = ROW_DW * n_rows;
+    int r = lane / ROW_DW, c = lane - r * ROW_DW;            // of dword 64k + lane
+    constexpr int kStepR = 64 / ROW_DW, kStepC = 64 - kStepR * ROW_DW;
+    for (int k = 0; k * 64 < n; k++) {
+        if (k * 64 + lane < n)
+            __builtin_amdgcn_global_load_lds((global_void *)(g + r * pitch_dw + c),
+                                             (lds_void *)(lds + k * 64), 4, 0, 0);
+        r += kStepR;
+        c += kStepC;
+        if (c >= ROW_DW) { c -= ROW_DW; r++; }
+    }
+}
+
+// The window around (bx, by) with reach `reach` into LDS.  Luma rows are
+// kWinLS samples and chroma rows kWinCS, always; the column origins are even
+// (dword-aligned) and clamped so that a row never runs past the right apron of
+// its own picture row while still covering the reads of the remaining levels.
+__device__ void stage_window(const DevPic &ref, int bx, int by, int reach, SubpelLds &sp)
+{
+    const int x0 = min((bx - reach - 3) & ~1, ref.py - kApron - kWinLS);
+    const int y0 = by - reach - 3;
     const int lh = 2 * reach + 22;
-    const int cx0 = (((bx - reach) >> 1) - 3) & ~1, cy0 = ((by - reach) >> 1) - 3;
-    const int cw = ((((bx + reach) >> 1) + 10) - cx0 + 2) >> 1;   // <= 16
+    const int cx0 = min((((bx - reach) >> 1) - 3) & ~1, ref.pc - kApron - kWinCS);
+    const int cy0 = ((by - reach) >> 1) - 3;
     const int ch = (((by + reach) >> 1) + 10) - cy0 + 1;
-    {
-        const int per = 64 / lw, r0 = lane / lw, c = lane - r0 * lw;
-        const uint32_t *g = reinterpret_cast<const uint32_t *>(ref.y + y0 * ref.py + x0) + c;
-        uint32_t *l = reinterpret_cast<uint32_t *>(sp.win.y) + c;
-        if (r0 < per) {
-            int r = r0;
-            for (; r + 3 * per < lh; r += 4 * per) {
-                const uint32_t v0 = g[r * (ref.py / 2)];
-                const uint32_t v1 = g[(r + per) * (ref.py / 2)];
-                const uint32_t v2 = g[(r + 2 * per) * (ref.py / 2)];
-                const uint32_t v3 = g[(r + 3 * per) * (ref.py / 2)];
-                l[r * (kWinLS / 2)] = v0;
-                l[(r + per) * (kWinLS / 2)] = v1;
-                l[(r + 2 * per) * (kWinLS / 2)] = v2;
-                l[(r + 3 * per) * (kWinLS / 2)] = v3;
-            }
-            for (; r < lh; r += per) l[r * (kWinLS / 2)] = g[r * (ref.py / 2)];
-        }
-    }
-    {
-        // U and V rows interleaved: lanes 0..31 take U, 32..63 take V.
-        const int half = lane >> 5, ln = lane & 31;
-        const int per = 32 / cw, r0 = ln / cw, c = ln - r0 * cw;
-        const int16_t *gp = half ? ref.v : ref.u;
-        int16_t *lp = half ? sp.win.v : sp.win.u;
-        const uint32_t *g = reinterpret_cast<const uint32_t *>(gp + cy0 * ref.pc + cx0) + c;
-        uint32_t *l = reinterpret_cast<uint32_t *>(lp) + c;
-        if (r0 < per) {
-            int r = r0;
-            for (; r + per < ch; r += 2 * per) {
-                const uint32_t v0 = g[r * (ref.pc / 2)];
-                const uint32_t v1 = g[(r + per) * (ref.pc / 2)];
-                l[r * (kWinCS / 2)] = v0;
-                l[(r + per) * (kWinCS / 2)] = v1;
-            }
-            for (; r < ch; r += per) l[r * (kWinCS / 2)] = g[r * (ref.pc / 2)];
-        }
-    }
-    if (lane == 0) { sp.org[0] = x0; sp.org[1] = y0; sp.org[2] = cx0; sp.org[3] = cy0; }
+    dma_rows<kWinLS / 2>(reinterpret_cast<const uint32_t *>(ref.y + y0 * ref.py + x0), ref.py / 2,
+                         lh, reinterpret_cast<uint32_t *>(sp.win.y));
+    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.u + cy0 * ref.pc + cx0), ref.pc / 2,
+                         ch, reinterpret_cast<uint32_t *>(sp.win.u));
+    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.v + cy0 * ref.pc + cx0), ref.pc / 2,
+                         ch, reinterpret_cast<uint32_t *>(sp.win.v));
+    if (lane_id() == 0) { sp.org[0] = x0; sp.org[1] = y0; sp.org[2] = cx0; sp.org[3] = cy0; }
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0): the LDS copies have landed
     __syncthreads();
 }
 
@@ -329,12 +337,10 @@ __device__ __forceinline__ void load_cand_win(const SubpelLds &sp, const Src &s,
 {
     const int16_t *p = sp.win.y + (cy + s.oy - sp.org[1]) * kWinLS + (cx + s.ox - sp.org[0]);
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) rb.y[r * 4 + c] = p[r * kWinLS + c];
+    for (int r = 0; r < 4; r++) load4(p + r * kWinLS, rb.y + r * 4);
     const int co = ((cy >> 1) + s.cr - sp.org[3]) * kWinCS + (cx >> 1) + s.cc - sp.org[2];
-#pragma unroll
-    for (int k = 0; k < 4; k++) { rb.u[k] = sp.win.u[co + k]; rb.v[k] = sp.win.v[co + k]; }
+    load4(sp.win.u + co, rb.u);
+    load4(sp.win.v + co, rb.v);
 }
 
 // First pass: the h and v planes of one picture plane.  `plane` addresses
@@ -877,12 +883,21 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
     var_out = var2;
 }
 
-__device__ __forceinline__ int16_t load_src_sample(const DevPic &pic, int x, int y, int i)
+// The source MB from the lanes that hold it (lanes 0..15 own the whole MB in
+// the layout of Src) into `dst` (16x16 Y, 8x8 U, 8x8 V, row-major).
+__device__ __forceinline__ void store_src(const Src &s, int16_t *dst)
 {
-    if (i < 256) return pic.y[(y + (i >> 4)) * pic.py + x + (i & 15)];
-    const int k = (i - 256) & 63;
-    const int16_t *pl = i < 320 ? pic.u : pic.v;
-    return pl[((y >> 1) + (k >> 3)) * pic.pc + (x >> 1) + (k & 7)];
+    if (lane_id() < 16) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) dst[(s.oy + r) * 16 + s.ox + c] = s.y[r * 4 + c];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            dst[256 + s.cr * 8 + s.cc + c] = s.u[c];
+            dst[320 + s.cr * 8 + s.cc + c] = s.v[c];
+        }
+    }
 }
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
@@ -992,7 +1007,7 @@ __device__ __forceinline__ SearchOut unpack_search(int v, int k)
 // With a single reference the caller passes its search result in `own`;
 // otherwise all results are read back from job.searches.
 __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
-                          const SearchOut *own)
+                          const Src &src, const SearchOut *own)
 {
     const int lane = lane_id();
     const int x = bx * 16, y = by * 16;
@@ -1006,7 +1021,7 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
     int best_k = -1, best_sad = kInt32Max, bidir = 0;
     int n_int = 0, n_ax = 0, n_dg = 0;
-    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.src[i] = load_src_sample(job.src, x, y, i);
+    store_src(src, sh.src);
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel.
         bool best_copy = false;
@@ -1083,8 +1098,8 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     uint32_t *rec = reinterpret_cast<uint32_t *>(job.records + idx);
     write_words(rec, w, 16);
 
-    // Publish: the record is visible before the row counter moves past it.
-    __threadfence();
+    // Publish: the record is visible before the row counter moves past it
+    // (the barrier orders every lane's stores before lane 0's release store).
     __syncthreads();
     if (lane == 0)
         __hip_atomic_store(job.progress + by * kProgressStride, bx + 1, __ATOMIC_RELEASE,
@@ -1112,21 +1127,26 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
 }
 
 __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ jobs,
-                                                     const uint32_t *__restrict__ tickets,
-                                                     int32_t n_tickets, int32_t *counters)
+                                                     const uint32_t *__restrict__ work,
+                                                     int32_t *counters)
 {
     __shared__ Shared sh;
     __shared__ int32_t s_word;
     const int lane = lane_id();
+    // This workgroup's list (see kLists): tickets [lo, hi) of the work buffer.
+    const int list = (int)(blockIdx.x % work[0]);
+    const int lo = (int)work[1 + list], hi = (int)work[2 + list];
+    const uint32_t *tickets = work + kWorkHeader;
+    int32_t *next = counters + kCounterList + list * kCounterListStride;
     for (;;) {
-        if (lane == 0) s_word = atomicAdd(&counters[kCounterTicket], 1);
+        if (lane == 0) s_word = lo + atomicAdd(next, 1);
         __syncthreads();
         // LDS hands the value back in a vector register; broadcasting it makes
         // the work item, and all control derived from it, wave-uniform
         // (scalar registers and scalar branches).
         const int t = __builtin_amdgcn_readfirstlane(s_word);
         __syncthreads();
-        if (t >= n_tickets) break;
+        if (t >= hi) break;
         const uint32_t e = tickets[t];
         const DevJob &job = jobs[e >> kJobShift];
         const int ref = (int)((e >> kRefShift) & 3);
@@ -1150,10 +1170,15 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         // CU's L1 under the waves that are computing); one acquire fence
         // after the condition holds orders the neighbours' records.
         auto wait_for = [&](const int32_t *p, int need) {
-            while ((int)ld_relaxed(reinterpret_cast<const uint32_t *>(p)) < need) {
+            for (;;) {
+                // One lane polls (a one-lane load costs the memory pipeline
+                // far less than a full-wave one); the value is broadcast.
+                int seen = 0;
+                if (lane == 0) seen = (int)ld_relaxed(reinterpret_cast<const uint32_t *>(p));
+                if (__builtin_amdgcn_readfirstlane(seen) >= need) return;
                 if (++spins > kSpinLimit) { timed_out = true; return; }
                 for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(2);
-                nap = min(nap * 2, 4);
+                nap = min(nap * 2, kMaxNap);
             }
         };
         if (bx > 0) wait_for(job.progress + by * kProgressStride, bx);
@@ -1170,14 +1195,14 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
                                          job.thresh, job.lambda, pmx, pmy, sh.sp);
         if (job.n_refs == 1) {
             // Single reference: this wave finishes the MB directly.
-            finish_mb(job, bx, by, counters, sh.mb, &o);
+            finish_mb(job, bx, by, counters, sh.mb, s, &o);
             continue;
         }
         uint32_t w[8];
         pack_search(o, w);
         write_words(reinterpret_cast<uint32_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref),
                     w, 8);
-        __threadfence();
+        // The barrier orders every lane's stores before lane 0's release.
         __syncthreads();
         if (lane == 0) {
             const int old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_ACQ_REL,
@@ -1188,7 +1213,7 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         __syncthreads();
         const int done_before = __builtin_amdgcn_readfirstlane(s_word);
         __syncthreads();
-        if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, nullptr);
+        if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, s, nullptr);
     }
 }
 
@@ -1291,11 +1316,10 @@ hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_analyze(const DevJob *jobs, const uint32_t *tickets, int32_t n_tickets,
-                          int32_t *counters, int32_t grid, hipStream_t st)
+hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
+                          int32_t grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(analyze_kernel, dim3(grid), dim3(64), 0, st, jobs, tickets, n_tickets,
-                       counters);
+    hipLaunchKernelGGL(analyze_kernel, dim3(grid), dim3(64), 0, st, jobs, work, counters);
     return hipGetLastError();
 }
 
