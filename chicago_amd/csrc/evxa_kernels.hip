@@ -900,9 +900,32 @@ __device__ __forceinline__ void store_src(const Src &s, int16_t *dst)
     }
 }
 
+// Cross-wave hand-off without cache-wide fences.  The MI355X has one L2 per
+// XCD, and an agent-scope release or acquire fence writes back or
+// invalidates a whole L2 (every coefficient and level store in flight).  The
+// few words other waves read -- neighbour records, search results, row
+// progress and finished-search counts -- are instead written and read with
+// device-scope (sc1) accesses, which go to the coherent side of the L2s.  A
+// producer stores its data words, waits for their completion (vmcnt(0)) in
+// every lane, passes a barrier, then stores the flag; a consumer reads the
+// flag, then the data, both device-scope.  Compiler fences keep the order.
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every lane's device-scope stores have completed once this returns.
+__device__ __forceinline__ void stores_done()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // compute_mv_predictor (motion.cpp:271-346) from published records.  The
@@ -956,13 +979,17 @@ __device__ void mv_predictor(const DevJob &job, int bx, int by, int &pmx, int &p
     }
 }
 
-__device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, int n)
+// Lane k < n stores word k; `coherent` makes the stores device-scope (read
+// by other waves inside the launch).
+__device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, int n,
+                                            bool coherent = false)
 {
     const int lane = lane_id();
     if (lane < n) {
         uint32_t v = w[0];
         for (int k = 1; k < n; k++) v = lane == k ? w[k] : v;
-        dst[lane] = v;
+        if (coherent) st_relaxed(dst + lane, v);
+        else dst[lane] = v;
     }
 }
 
@@ -1096,14 +1123,12 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     w[14] = 0;
     w[15] = 0;
     uint32_t *rec = reinterpret_cast<uint32_t *>(job.records + idx);
-    write_words(rec, w, 16);
+    write_words(rec, w, 16, true);
 
-    // Publish: the record is visible before the row counter moves past it
-    // (the barrier orders every lane's stores before lane 0's release store).
-    __syncthreads();
-    if (lane == 0)
-        __hip_atomic_store(job.progress + by * kProgressStride, bx + 1, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
+    // Publish: the record's device-scope stores complete in every lane
+    // before lane 0 moves the row counter past it.
+    stores_done();
+    if (lane == 0) st_relaxed(reinterpret_cast<uint32_t *>(job.progress + by * kProgressStride), bx + 1);
 
     int16_t *co = job.coeffs + (size_t)idx * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)idx * EVXA_MB_SAMPLES;
@@ -1120,9 +1145,9 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     uint32_t w2[2];
     w2[0] = w[4] | ((uint32_t)qp << 16);
     w2[1] = (uint32_t)(uint16_t)(int16_t)var2;
-    write_words(rec + 4, w2, 2);
+    write_words(rec + 4, w2, 2, true);
     w2[0] = 1u | (bidir ? 2u : 0u);
-    write_words(rec + 14, w2, 1);
+    write_words(rec + 14, w2, 1, true);
     __syncthreads();
 }
 
@@ -1184,7 +1209,8 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         if (bx > 0) wait_for(job.progress + by * kProgressStride, bx);
         if (by > 0 && !timed_out)
             wait_for(job.progress + (by - 1) * kProgressStride, min(bx + 2, job.wb));
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // The neighbours' records are read device-scope after the wait.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (timed_out && lane == 0) atomicOr(&counters[kCounterError], kErrWaitTimeout);
 
         int pmx, pmy;
@@ -1201,14 +1227,16 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         uint32_t w[8];
         pack_search(o, w);
         write_words(reinterpret_cast<uint32_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref),
-                    w, 8);
-        // The barrier orders every lane's stores before lane 0's release.
-        __syncthreads();
+                    w, 8, true);
+        // This search's result is complete in memory before it is counted;
+        // the wave that counts last reads every result device-scope.
+        stores_done();
         if (lane == 0) {
-            const int old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_ACQ_REL,
+            const int old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
             s_word = old;
-            if (old == job.n_refs - 1) job.mb_done[mb] = 0;   // last one: back to zero
+            if (old == job.n_refs - 1)   // last one: back to zero for the next launch
+                st_relaxed(reinterpret_cast<uint32_t *>(job.mb_done + mb), 0u);
         }
         __syncthreads();
         const int done_before = __builtin_amdgcn_readfirstlane(s_word);
