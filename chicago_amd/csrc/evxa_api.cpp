@@ -207,6 +207,7 @@ struct evxa_stream {
     DevBuf coeffs, levels;
     DevBuf searches;       // EVXA_MAX_REFS SearchRec per MB
     DevBuf mb_done;        // per-MB finished-search counters (kept at zero between launches)
+    DevBuf wide;           // per slot: nonzero if a sample lies outside [0, 2047]
     int32_t p_cur = 0;
     bool temporal_valid = false;
     int32_t last_table = -1;
@@ -230,6 +231,7 @@ struct evxa_stream {
         p.v = ub + (size_t)gc.pitch * gc.rows + gc.origin();
         p.py = gy.pitch;
         p.pc = gc.pitch;
+        p.wide = wide.as<int32_t>() + slot;
         return p;
     }
 };
@@ -499,7 +501,7 @@ float evxa_lambda(int32_t quality) { return lambda_of(quality); }
 uint64_t evxa_stream_bytes(int32_t width, int32_t height, int32_t n_slots)
 {
     if (width <= 0 || height <= 0 || n_slots <= 0) return 0;
-    return (uint64_t)n_slots * align_up(picture_bytes(width, height), 4096) +
+    return (uint64_t)n_slots * (align_up(picture_bytes(width, height), 4096) + sizeof(int32_t)) +
            outputs_bytes(width, height);
 }
 
@@ -603,6 +605,9 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
         s->searches.alloc(n * EVXA_MAX_REFS * sizeof(SearchRec));
         s->mb_done.alloc(n * sizeof(int32_t));
         check(hipMemsetAsync(s->mb_done.p, 0, n * sizeof(int32_t), ctx->stream), "clear counters");
+        s->wide.alloc((size_t)n_slots * sizeof(int32_t));
+        check(hipMemsetAsync(s->wide.p, 0, (size_t)n_slots * sizeof(int32_t), ctx->stream),
+              "clear range flags");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         s->bytes = need;
         ctx->used += need;
@@ -653,6 +658,8 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
         int16_t *dst[3] = {py, pu, pv};
         const int16_t *src[3] = {y, u, v};
         const int32_t strides[3] = {stride_y, stride_c, stride_c};
+        int32_t *wide = s->wide.as<int32_t>() + slot;
+        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
         for (int k = 0; k < 3; k++) {
             check(hipMemcpy2DAsync(dst[k] + g[k]->origin(), (size_t)g[k]->pitch * 2, src[k],
                                    (size_t)strides[k] * 2, (size_t)g[k]->w * 2, g[k]->h,
@@ -660,6 +667,9 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
                   "upload plane");
             check(launch_apron(dst[k] + g[k]->origin(), g[k]->pitch, g[k]->w, g[k]->h, ctx->stream),
                   "apron kernel");
+            check(launch_range(dst[k] + g[k]->origin(), g[k]->pitch, g[k]->w, g[k]->h, wide,
+                               ctx->stream),
+                  "range kernel");
         }
         // Host buffers are the caller's: finish before returning.
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

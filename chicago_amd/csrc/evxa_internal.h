@@ -21,6 +21,9 @@ struct DevPic {
     const int16_t *v;
     int32_t py;
     int32_t pc;
+    // Nonzero when some sample of the picture lies outside [0, 2047] (set at
+    // upload); pictures without it take the packed 16-bit metric paths.
+    const int32_t *wide;
 };
 
 // Outcome of one (MB, reference) search, kept in HBM until the MB's last
@@ -91,6 +94,9 @@ hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st);
 hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
                           int32_t grid, hipStream_t st);
 hipError_t launch_apron(int16_t *plane0, int32_t pitch, int32_t w, int32_t h, hipStream_t st);
+// ORs 1 into *wide if a sample of the w x h plane lies outside [0, 2047].
+hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,
+                        hipStream_t st);
 hipError_t launch_interp_blocks(DevPic pic, int32_t w, int32_t h, int32_t n, const int32_t *params,
                                 int16_t *out, hipStream_t st);
 hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, int32_t quality,

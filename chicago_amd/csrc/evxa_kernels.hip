@@ -139,94 +139,149 @@ __device__ __forceinline__ void frac_direction(int idx, int &dx, int &dy)
     else { dy = 1; dx = idx - 6; }
 }
 
-// Source MB samples held in registers.  Lane b = lane & 15 owns luma 4x4
-// block (b & 3, b >> 2) and chroma row (b & 7), columns (b >> 3) * 4 .. +3.
-struct Src {
-    int16_t y[16];
-    int16_t u[4];
-    int16_t v[4];
+// Packed 16-bit lanes: two int16 samples per 32-bit register, low half first.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 pk(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
+__device__ __forceinline__ s16x2 pabs(s16x2 x) { return __builtin_elementwise_max(x, -x); }
+__device__ __forceinline__ s16x2 pmax(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ s16x2 pswap(s16x2 x) { return __builtin_shufflevector(x, x, 1, 0); }
+// Low half + high half, both taken as unsigned 16-bit.
+__device__ __forceinline__ uint32_t hsum(s16x2 x)
+{
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, x), (u16x2)(1), 0u, false);
+}
+__device__ __forceinline__ int lo16(uint32_t x) { return (int16_t)(x & 0xFFFFu); }
+__device__ __forceinline__ int hi16(uint32_t x) { return (int16_t)(x >> 16); }
+
+// Samples of one MB-sized block in the lane layout: lane b = lane & 15 owns
+// luma 4x4 block (b & 3, b >> 2) -- rows oy .. oy + 3, columns ox .. ox + 3,
+// two packed pairs per row (y[2r], y[2r + 1]) -- and chroma row cr, columns
+// cc .. cc + 3 of U and V (two pairs each).
+struct Blk {
+    uint32_t y[8];
+    uint32_t u[2];
+    uint32_t v[2];
+};
+
+struct Geo {
     int ox, oy, cr, cc;
 };
 
-__device__ __forceinline__ void lane_geometry(Src &s)
+__device__ __forceinline__ Geo lane_geometry()
 {
     const int b = lane_id() & 15;
-    s.ox = (b & 3) * 4; s.oy = (b >> 2) * 4;
-    s.cr = b & 7; s.cc = (b >> 3) * 4;
+    Geo g;
+    g.ox = (b & 3) * 4; g.oy = (b >> 2) * 4;
+    g.cr = b & 7; g.cc = (b >> 3) * 4;
+    return g;
 }
 
 // Four consecutive samples as one 8-byte access (global or LDS; the address
 // only needs 2-byte alignment, which gfx950 serves as one access).
-__device__ __forceinline__ void load4(const int16_t *p, int16_t *dst)
+__device__ __forceinline__ void load4(const int16_t *p, uint32_t &lo, uint32_t &hi)
 {
     uint2 q;
     __builtin_memcpy(&q, p, 8);
-    dst[0] = (int16_t)(q.x & 0xFFFFu);
-    dst[1] = (int16_t)(q.x >> 16);
-    dst[2] = (int16_t)(q.y & 0xFFFFu);
-    dst[3] = (int16_t)(q.y >> 16);
+    lo = q.x;
+    hi = q.y;
 }
 
-// Samples of the MB at (x, y) of `pic` in the lane layout of Src: four
-// 8-byte rows of luma and one 8-byte run of each chroma plane per lane.
-__device__ __forceinline__ void load_lane_block(const DevPic &pic, int x, int y, const Src &g,
-                                                int16_t (&ly)[16], int16_t (&lu)[4],
-                                                int16_t (&lv)[4])
+// The block at (x, y) of planes with pitches (py, pc) in the lane layout.
+__device__ __forceinline__ void load_blk(const int16_t *py_, const int16_t *pu_, const int16_t *pv_,
+                                         int pitch_y, int pitch_c, int x, int y, const Geo &g,
+                                         Blk &b)
 {
-    const int16_t *p = pic.y + (y + g.oy) * pic.py + x + g.ox;
+    const int16_t *p = py_ + (y + g.oy) * pitch_y + x + g.ox;
 #pragma unroll
-    for (int r = 0; r < 4; r++) load4(p + r * pic.py, ly + r * 4);
-    const int co = ((y >> 1) + g.cr) * pic.pc + (x >> 1) + g.cc;
-    load4(pic.u + co, lu);
-    load4(pic.v + co, lv);
+    for (int r = 0; r < 4; r++) load4(p + r * pitch_y, b.y[2 * r], b.y[2 * r + 1]);
+    const int co = ((y >> 1) + g.cr) * pitch_c + (x >> 1) + g.cc;
+    load4(pu_ + co, b.u[0], b.u[1]);
+    load4(pv_ + co, b.v[0], b.v[1]);
 }
 
-// Reference samples of one candidate position in the lane layout of Src.
-struct RefBlk {
-    int16_t y[16];
-    int16_t u[4];
-    int16_t v[4];
-};
-
-__device__ __forceinline__ void load_cand(const DevPic &ref, const Src &s, int cx, int cy,
-                                          RefBlk &rb)
+__device__ __forceinline__ void load_cand(const DevPic &pic, const Geo &g, int x, int y, Blk &b)
 {
-    load_lane_block(ref, cx, cy, s, rb.y, rb.u, rb.v);
+    load_blk(pic.y, pic.u, pic.v, pic.py, pic.pc, x, y, g, b);
 }
 
-__device__ __forceinline__ void select_blk(bool take_b, const RefBlk &a, const RefBlk &b,
-                                           RefBlk &out)
+__device__ __forceinline__ void select_blk(bool take_b, const Blk &a, const Blk &b, Blk &out)
 {
 #pragma unroll
-    for (int k = 0; k < 16; k++) out.y[k] = take_b ? b.y[k] : a.y[k];
+    for (int k = 0; k < 8; k++) out.y[k] = take_b ? b.y[k] : a.y[k];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 2; k++) {
         out.u[k] = take_b ? b.u[k] : a.u[k];
         out.v[k] = take_b ? b.v[k] : a.v[k];
     }
 }
 
-// Per-lane part of an integer candidate (analysis.h:89-132, 180-202).
-__device__ __forceinline__ void cand_metrics(const Src &s, const RefBlk &rb, int &sad, int &mad,
+// Per-lane part of an integer candidate (analysis.h:89-132, 180-202) when
+// every sample lies in [0, 2047] ("narrow" pictures): |differences| <= 2047,
+// so the differences, their sums over a lane's 16 samples and the 4x4
+// Hadamard (intermediates <= 16 * 2047) are exact in packed int16.  The
+// Hadamard runs the vertical butterflies first (the set of coefficients does
+// not depend on the pass order); the last horizontal stage uses
+// |x + y| + |x - y| = 2 max(|x|, |y|).
+__device__ __forceinline__ void metrics_narrow(const Blk &s, const Blk &r, int &sad, int &mad,
+                                               int &satd)
+{
+    s16x2 d[8], a[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        d[k] = pk(s.y[k]) - pk(r.y[k]);
+        a[k] = pabs(d[k]);
+    }
+    const s16x2 sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    s16x2 mx = pmax(pmax(pmax(a[0], a[1]), pmax(a[2], a[3])), pmax(pmax(a[4], a[5]), pmax(a[6], a[7])));
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        mx = pmax(mx, pabs(pk(s.u[k]) - pk(r.u[k])));
+        mx = pmax(mx, pabs(pk(s.v[k]) - pk(r.v[k])));
+    }
+    sad = (int)hsum(sum);
+    mad = pmax(mx, pswap(mx)).x;
+    s16x2 u[4][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const s16x2 s0 = d[h] + d[2 + h], s1 = d[h] - d[2 + h];
+        const s16x2 s2 = d[4 + h] + d[6 + h], s3 = d[4 + h] - d[6 + h];
+        u[0][h] = s0 + s2; u[1][h] = s1 + s3; u[2][h] = s0 - s2; u[3][h] = s1 - s3;
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const s16x2 aS = pabs(u[i][0] + u[i][1]), aD = pabs(u[i][0] - u[i][1]);
+        acc += (uint32_t)(uint16_t)pmax(aS, pswap(aS)).x + (uint32_t)(uint16_t)pmax(aD, pswap(aD)).x;
+    }
+    satd = (int)(2 * acc);
+}
+
+// The same for general int16 samples, in 32-bit arithmetic as the reference:
+// SAD and MAD on int32 differences, the Hadamard on differences stored as
+// int16 (analysis.h:102).
+__device__ __forceinline__ void metrics_wide(const Blk &s, const Blk &r, int &sad, int &mad,
                                              int &satd)
 {
     int d[16];
     sad = 0; mad = 0;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int dd = s.y[k] - rb.y[k];
-        const int a = dd < 0 ? -dd : dd;
-        sad += a;
-        mad = max(mad, a);
-        d[k] = (int16_t)dd;   // analysis.h:102 stores the difference as int16
+    for (int k = 0; k < 8; k++) {
+        const int d0 = lo16(s.y[k]) - lo16(r.y[k]), d1 = hi16(s.y[k]) - hi16(r.y[k]);
+        const int a0 = d0 < 0 ? -d0 : d0, a1 = d1 < 0 ? -d1 : d1;
+        sad += a0 + a1;
+        mad = max(mad, max(a0, a1));
+        d[2 * k] = (int16_t)d0;
+        d[2 * k + 1] = (int16_t)d1;
     }
     // analysis.h:44-85
     int t[16];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        int a = d[r * 4], b = d[r * 4 + 1], c = d[r * 4 + 2], e = d[r * 4 + 3];
+    for (int rr = 0; rr < 4; rr++) {
+        int a = d[rr * 4], b = d[rr * 4 + 1], c = d[rr * 4 + 2], e = d[rr * 4 + 3];
         int s0 = a + b, s1 = a - b, s2 = c + e, s3 = c - e;
-        t[r * 4] = s0 + s2; t[r * 4 + 1] = s1 + s3; t[r * 4 + 2] = s0 - s2; t[r * 4 + 3] = s1 - s3;
+        t[rr * 4] = s0 + s2; t[rr * 4 + 1] = s1 + s3; t[rr * 4 + 2] = s0 - s2; t[rr * 4 + 3] = s1 - s3;
     }
     satd = 0;
 #pragma unroll
@@ -236,10 +291,22 @@ __device__ __forceinline__ void cand_metrics(const Src &s, const RefBlk &rb, int
         satd += abs(s0 + s2) + abs(s1 + s3) + abs(s0 - s2) + abs(s1 - s3);
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        mad = max(mad, abs(s.u[k] - rb.u[k]));
-        mad = max(mad, abs(s.v[k] - rb.v[k]));
+    for (int k = 0; k < 2; k++) {
+        mad = max(mad, max(abs(lo16(s.u[k]) - lo16(r.u[k])), abs(hi16(s.u[k]) - hi16(r.u[k]))));
+        mad = max(mad, max(abs(lo16(s.v[k]) - lo16(r.v[k])), abs(hi16(s.v[k]) - hi16(r.v[k]))));
     }
+}
+
+// One integer candidate over the 16 lanes of a group: SAD, MAD and raw
+// SATD (before the final (x + 1) >> 1) in every lane of the group.
+__device__ __forceinline__ void cand_metrics(bool narrow, const Blk &s, const Blk &r, int &sad,
+                                             int &mad, int &satd)
+{
+    if (narrow) metrics_narrow(s, r, sad, mad, satd);
+    else metrics_wide(s, r, sad, mad, satd);
+    sad = row_sum(sad);
+    mad = row_max(mad);
+    satd = row_sum(satd);
 }
 
 // Half-pel planes around an integer base (bx, by), interp.cpp:52-164 split
@@ -258,12 +325,14 @@ struct HalfPlanes {
     int16_t d[DR * DC];
 };
 
-// Reference window staged in LDS for the search levels with step <= 8 and
-// the sub-pel refinement.  Around a base position b and a reach r (the
-// largest offset the remaining levels can move the best, 2R - 1 <= 15) the
-// reads stay within luma [b - r - 3, b + r + 18] and chroma
-// [((b - r) >> 1) - 3, ((b + r) >> 1) + 10] on each axis.  Origins are even so
-// rows load as aligned dwords.
+// Reference window staged in LDS.  A window around a base position b with
+// reach r covers the reads of any level whose candidates stay within b +- r
+// and of the sub-pel refinement around any such position: luma
+// [b - r - 3, b + r + 18] and chroma [((b - r) >> 1) - 3, ((b + r) >> 1) + 10]
+// on each axis.  The largest window (r = 15) is staged around the MB itself
+// while it waits for its neighbours; a level or the sub-pel step whose reads
+// fall outside the current window stages a new one around its base.  Column
+// origins are even so rows load as aligned dwords.
 constexpr int kReachMax = 15;
 constexpr int kWinLS = 54;    // luma row stride (>= 2 * 15 + 23, even)
 constexpr int kWinLR = 52;    // luma rows (2 * 15 + 22)
@@ -281,8 +350,42 @@ struct SubpelLds {
     HalfPlanes<16> y;
     HalfPlanes<8> u;
     HalfPlanes<8> v;
-    int32_t org[4];           // luma x0, y0, chroma x0, y0
 };
+
+// Where the staged window sits in picture coordinates (wave-uniform): luma
+// columns [x0, x0 + kWinLS) and rows [y0, y1), chroma columns
+// [cx0, cx0 + kWinCS) and rows [cy0, cy1).
+struct WinPos {
+    int x0, y0, y1, cx0, cy0, cy1;
+};
+
+// Does the window hold luma [lx0, lx1) x [ly0, ly1) and chroma
+// [cx0, cx1) x [cy0, cy1)?
+__device__ __forceinline__ bool win_holds(const WinPos &w, int lx0, int lx1, int ly0, int ly1,
+                                          int cx0, int cx1, int cy0, int cy1)
+{
+    return lx0 >= w.x0 && lx1 <= w.x0 + kWinLS && ly0 >= w.y0 && ly1 <= w.y1 && cx0 >= w.cx0 &&
+           cx1 <= w.cx0 + kWinCS && cy0 >= w.cy0 && cy1 <= w.cy1;
+}
+
+// Reads of the candidates of a level with step `step` around (bx, by),
+// clamped into the picture like the candidates themselves.
+__device__ __forceinline__ bool win_holds_level(const WinPos &w, int bx, int by, int step, int max_x,
+                                                int max_y)
+{
+    const int lx0 = max(bx - step, 0), lx1 = min(bx + step, max_x) + 16;
+    const int ly0 = max(by - step, 0), ly1 = min(by + step, max_y) + 16;
+    return win_holds(w, lx0, lx1, ly0, ly1, lx0 >> 1, ((lx1 - 16) >> 1) + 8, ly0 >> 1,
+                     ((ly1 - 16) >> 1) + 8);
+}
+
+// Reads of the sub-pel refinement around (bx, by) (interp.cpp:52-164 with
+// its 3-sample apron, luma 16 and chroma 8 samples wide).
+__device__ __forceinline__ bool win_holds_subpel(const WinPos &w, int bx, int by)
+{
+    return win_holds(w, bx - 3, bx + 19, by - 3, by + 19, (bx >> 1) - 3, (bx >> 1) + 11,
+                     (by >> 1) - 3, (by >> 1) + 11);
+}
 
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
@@ -290,7 +393,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 // Copies rows of `row_dw` dwords (row r of the source at g + r * pitch_dw)
 // into LDS packed with stride row_dw, n_rows rows, straight from memory to
 // LDS (global_load_lds_dword: lane i of the k-th load writes dword 64k + i).
-// All loads are issued back to back and complete under one wait.
+// All loads are issued back to back; they complete under vmcnt(0).
 template <int ROW_DW>
 __device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_rows, uint32_t *lds)
 {
@@ -308,69 +411,102 @@ __device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_
     }
 }
 
-// The window around (bx, by) with reach `reach` into LDS.  Luma rows are
-// kWinLS samples and chroma rows kWinCS, always; the column origins are even
-// (dword-aligned) and clamped so that a row never runs past the right apron of
-// its own picture row while still covering the reads of the remaining levels.
-__device__ void stage_window(const DevPic &ref, int bx, int by, int reach, SubpelLds &sp)
+// Issue the copy of the window around (bx, by) with reach `reach` into LDS
+// (call window_ready before reading it).  Luma rows are kWinLS samples and
+// chroma rows kWinCS, always; the column origins are even and clamped so that
+// a row never runs past the right apron of its own picture row.
+__device__ WinPos stage_window(const DevPic &ref, int bx, int by, int reach, SubpelLds &sp)
 {
-    const int x0 = min((bx - reach - 3) & ~1, ref.py - kApron - kWinLS);
-    const int y0 = by - reach - 3;
-    const int lh = 2 * reach + 22;
-    const int cx0 = min((((bx - reach) >> 1) - 3) & ~1, ref.pc - kApron - kWinCS);
-    const int cy0 = ((by - reach) >> 1) - 3;
-    const int ch = (((by + reach) >> 1) + 10) - cy0 + 1;
-    dma_rows<kWinLS / 2>(reinterpret_cast<const uint32_t *>(ref.y + y0 * ref.py + x0), ref.py / 2,
-                         lh, reinterpret_cast<uint32_t *>(sp.win.y));
-    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.u + cy0 * ref.pc + cx0), ref.pc / 2,
-                         ch, reinterpret_cast<uint32_t *>(sp.win.u));
-    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.v + cy0 * ref.pc + cx0), ref.pc / 2,
-                         ch, reinterpret_cast<uint32_t *>(sp.win.v));
-    if (lane_id() == 0) { sp.org[0] = x0; sp.org[1] = y0; sp.org[2] = cx0; sp.org[3] = cy0; }
+    WinPos w;
+    w.x0 = min((bx - reach - 3) & ~1, ref.py - kApron - kWinLS);
+    w.y0 = by - reach - 3;
+    w.y1 = w.y0 + 2 * reach + 22;
+    w.cx0 = min((((bx - reach) >> 1) - 3) & ~1, ref.pc - kApron - kWinCS);
+    w.cy0 = ((by - reach) >> 1) - 3;
+    w.cy1 = ((by + reach) >> 1) + 11;
+    dma_rows<kWinLS / 2>(reinterpret_cast<const uint32_t *>(ref.y + w.y0 * ref.py + w.x0),
+                         ref.py / 2, w.y1 - w.y0, reinterpret_cast<uint32_t *>(sp.win.y));
+    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.u + w.cy0 * ref.pc + w.cx0),
+                         ref.pc / 2, w.cy1 - w.cy0, reinterpret_cast<uint32_t *>(sp.win.u));
+    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.v + w.cy0 * ref.pc + w.cx0),
+                         ref.pc / 2, w.cy1 - w.cy0, reinterpret_cast<uint32_t *>(sp.win.v));
+    return w;
+}
+
+__device__ __forceinline__ void window_ready()
+{
     __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0): the LDS copies have landed
     __syncthreads();
 }
 
-// A candidate block from the staged window, in the lane layout of Src.
-__device__ __forceinline__ void load_cand_win(const SubpelLds &sp, const Src &s, int cx, int cy,
-                                              RefBlk &rb)
+// A candidate block from the staged window, in the lane layout.
+__device__ __forceinline__ void load_cand_win(const SubpelLds &sp, const WinPos &w, const Geo &g,
+                                              int cx, int cy, Blk &b)
 {
-    const int16_t *p = sp.win.y + (cy + s.oy - sp.org[1]) * kWinLS + (cx + s.ox - sp.org[0]);
-#pragma unroll
-    for (int r = 0; r < 4; r++) load4(p + r * kWinLS, rb.y + r * 4);
-    const int co = ((cy >> 1) + s.cr - sp.org[3]) * kWinCS + (cx >> 1) + s.cc - sp.org[2];
-    load4(sp.win.u + co, rb.u);
-    load4(sp.win.v + co, rb.v);
+    load_blk(sp.win.y + (cy - w.y0) * kWinLS + (cx - w.x0),
+             sp.win.u + ((cy >> 1) - w.cy0) * kWinCS + ((cx >> 1) - w.cx0),
+             sp.win.v + ((cy >> 1) - w.cy0) * kWinCS + ((cx >> 1) - w.cx0), kWinLS, kWinCS, 0, 0,
+             g, b);
 }
 
-// First pass: the h and v planes of one picture plane.  `plane` addresses
-// the block base (bx, by) and has row pitch `pitch`.
-template <int B>
-__device__ __forceinline__ void build_hv(const int16_t *plane, int pitch, HalfPlanes<B> &hp)
+// Six-tap outputs of an R x C plane into dst (row-major, row stride C):
+// output (r, c) filters src[r * sp + c + t * ts], t = 0..5 (interp.cpp:18-41).
+// Lane l computes outputs l, l + 64, ...; positions advance by constants
+// (no division in the loop), and each chunk of kChunk outputs issues all of
+// its LDS loads before any arithmetic, so their latencies overlap.
+template <int R, int C>
+__device__ __forceinline__ void filter_plane(const int16_t *src, int sp, int ts, int16_t *dst)
 {
-    using P = HalfPlanes<B>;
-    const int nh = P::HR * P::HC, nv = P::VR * P::VC;
-    for (int i = lane_id(); i < nh + nv; i += 64) {
-        if (i < nh) {
-            const int r = i / P::HC - 3, c = i % P::HC - 1;
-            hp.h[i] = (int16_t)tap6(plane + r * pitch + c - 2, 1);
-        } else {
-            const int k = i - nh;
-            const int r = k / P::VC - 1, c = k % P::VC;
-            hp.v[k] = (int16_t)tap6(plane + (r - 2) * pitch + c, pitch);
+    constexpr int N = R * C;
+    constexpr int K = (N + 63) / 64;
+    constexpr int kChunk = 4;
+    const int lane = lane_id();
+    const int r0 = lane / C, c0 = lane - (lane / C) * C;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += kChunk) {
+        int v[kChunk][6];
+#pragma unroll
+        for (int j = 0; j < kChunk; j++) {
+            const int k = k0 + j;
+            if (k < K && (k < K - 1 || lane + 64 * k < N)) {
+                int c = c0 + (64 * k) % C, r = r0 + (64 * k) / C;
+                if (c >= C) { c -= C; r++; }
+                const int16_t *q = src + r * sp + c;
+#pragma unroll
+                for (int t = 0; t < 6; t++) v[j][t] = q[t * ts];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; j++) {
+            const int k = k0 + j;
+            if (k < K && (k < K - 1 || lane + 64 * k < N)) {
+                const int sum = v[j][0] - 5 * v[j][1] + 20 * v[j][2] + 20 * v[j][3] -
+                                5 * v[j][4] + v[j][5];
+                dst[lane + 64 * k] = (int16_t)((sum + 16) >> 5);
+            }
         }
     }
 }
 
-// Second pass: the diagonal plane from the h plane in LDS.
+// First pass: the h and v planes of one picture plane.  `plane` addresses
+// the block base (bx, by) and has row pitch `pitch`.  h row r, column c is
+// the horizontal half between columns c - 1 and c of row r - 3; v row r is
+// the vertical half between rows r - 1 and r.
+template <int B>
+__device__ __forceinline__ void build_hv(const int16_t *plane, int pitch, HalfPlanes<B> &hp)
+{
+    using P = HalfPlanes<B>;
+    filter_plane<P::HR, P::HC>(plane - 3 * pitch - 3, pitch, 1, hp.h);
+    filter_plane<P::VR, P::VC>(plane - 3 * pitch, pitch, pitch, hp.v);
+}
+
+// Second pass: the diagonal plane, a vertical six-tap over the h plane
+// (plane row -1 starts at h row 0, i.e. picture row -3).
 template <int B>
 __device__ __forceinline__ void build_d(HalfPlanes<B> &hp)
 {
     using P = HalfPlanes<B>;
-    for (int i = lane_id(); i < P::DR * P::DC; i += 64) {
-        const int r = i / P::DC, c = i % P::DC;   // plane rows -1.., h rows -3..
-        hp.d[i] = (int16_t)tap6(hp.h + r * P::HC + c, P::HC);
-    }
+    filter_plane<P::DR, P::DC>(hp.h, P::HC, P::HC, hp.d);
 }
 
 // Where direction (dx, dy)'s half-pel block sits inside the planes.
@@ -415,19 +551,83 @@ __device__ __forceinline__ uint32_t cost_bits(int sad, int satd_raw, int mvx, in
                                      __fmul_rn(lambda, __int2float_rn(bits))));
 }
 
+// Bits 0, 16, 32 and 48 of a wave mask (lane 0 of each 16-lane group) as
+// bits 0..3.
+__device__ __forceinline__ uint32_t group_bits(uint64_t m)
+{
+    return (uint32_t)((m & 1u) | ((m >> 15) & 2u) | ((m >> 30) & 4u) | ((m >> 45) & 8u));
+}
+
+// SAD and MAD of this lane's samples against the half-pel block of one
+// direction (its plane views) and against the quarter-pel block
+// (full + half + 1) >> 1 (interp.cpp:166-202), per lane before reduction.
+// Narrow pictures (samples in [0, 2047]) keep every half-pel sample in
+// [-1680, 3727], so differences and sums stay exact in packed int16.
+__device__ __forceinline__ void subpel_metrics(bool narrow, const Blk &s, const Blk &full,
+                                               const PlaneView &vy, const PlaneView &vu,
+                                               const PlaneView &vv, const Geo &g, int &sh, int &mh,
+                                               int &sq, int &mq)
+{
+    Blk h;
+    load_blk(vy.base, vu.base, vv.base, vy.stride, vu.stride, 0, 0, g, h);
+    if (narrow) {
+        const s16x2 one = (s16x2)(1);
+        s16x2 sumh = (s16x2)(0), sumq = (s16x2)(0), mxh = (s16x2)(0), mxq = (s16x2)(0);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const s16x2 hv = pk(h.y[k]), qv = (pk(full.y[k]) + hv + one) >> one;
+            const s16x2 ah = pabs(pk(s.y[k]) - hv), aq = pabs(pk(s.y[k]) - qv);
+            sumh += ah; sumq += aq;
+            mxh = pmax(mxh, ah); mxq = pmax(mxq, aq);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const s16x2 hu = pk(h.u[k]), qu = (pk(full.u[k]) + hu + one) >> one;
+            const s16x2 hw = pk(h.v[k]), qw = (pk(full.v[k]) + hw + one) >> one;
+            mxh = pmax(mxh, pmax(pabs(pk(s.u[k]) - hu), pabs(pk(s.v[k]) - hw)));
+            mxq = pmax(mxq, pmax(pabs(pk(s.u[k]) - qu), pabs(pk(s.v[k]) - qw)));
+        }
+        sh = (int)hsum(sumh);
+        sq = (int)hsum(sumq);
+        mh = pmax(mxh, pswap(mxh)).x;
+        mq = pmax(mxq, pswap(mxq)).x;
+        return;
+    }
+    sh = 0; mh = 0; sq = 0; mq = 0;
+    auto one_sample = [&](int sv, int fv, int hv, bool luma) {
+        const int qv = (int16_t)((fv + hv + 1) >> 1);
+        const int ah = abs(sv - hv), aq = abs(sv - qv);
+        if (luma) { sh += ah; sq += aq; }
+        mh = max(mh, ah);
+        mq = max(mq, aq);
+    };
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        one_sample(lo16(s.y[k]), lo16(full.y[k]), lo16(h.y[k]), true);
+        one_sample(hi16(s.y[k]), hi16(full.y[k]), hi16(h.y[k]), true);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        one_sample(lo16(s.u[k]), lo16(full.u[k]), lo16(h.u[k]), false);
+        one_sample(hi16(s.u[k]), hi16(full.u[k]), hi16(h.u[k]), false);
+        one_sample(lo16(s.v[k]), lo16(full.v[k]), lo16(h.v[k]), false);
+        one_sample(hi16(s.v[k]), hi16(full.v[k]), hi16(h.v[k]), false);
+    }
+}
+
 // perform_inter_search (motion.cpp:567-679) for the MB at (x, y) against
 // `ref`, predictor (pmx, pmy) as compute_mv_predictor produced it.  All
 // lanes call it with uniform arguments; it returns uniform values.
-// `pre` holds the reference block at (x, y) in the lane layout of Src (it is
-// loaded while the MB waits for its neighbours).
+// `s` is the source MB and `pre` the reference block at (x, y), both in the
+// lane layout; `w` is the window already staged in `sp` (any position).
 //
 // Lane group g (lanes 16g .. 16g+15) evaluates scan slots g and g+4 of every
 // level and sub-pel round; each lane owns one 4x4 luma block and 8 chroma
 // samples of the candidate.  Metrics and costs are computed for all slots in
-// parallel; the ordered fold then runs on wave-uniform integers.
-__device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s, const RefBlk &pre,
-                                  int x, int y, int thresh, float lambda, int pmx, int pmy,
-                                  SubpelLds &sp)
+// parallel; the ordered fold then runs on wave-uniform values.
+__device__ SearchOut inter_search(const DevPic &ref, int W, int H, bool narrow, const Blk &s,
+                                  const Geo &geo, const Blk &pre, WinPos w, int x, int y,
+                                  int thresh, float lambda, int pmx, int pmy, SubpelLds &sp)
 {
     const int g = lane_id() >> 4;
     const int ia = slot_i(g), ja = slot_j(g), ib = slot_i(g + 4), jb = slot_j(g + 4);
@@ -435,6 +635,7 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
     int best_x = x, best_y = y, best_sad, best_mad, best_ssd = kInt32Max;
     uint32_t best_cost;
     int sp_en = 0, sp_amt = 0, sp_idx = 0, n_int = 1, n_ax = 0, n_dg = 0;
+    auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
 
     // Initial candidate (motion.cpp:600-611) in group 0 and the predictor
     // candidate (motion.cpp:621-634, clamped into the picture) in group 1.
@@ -442,24 +643,26 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
     const int px = min(max(x + pmx, 0), max_x), py = min(max(y + pmy, 0), max_y);
     int sad, mad, satd;
     if (has_pred) {
-        RefBlk rp, rb;
-        load_cand(ref, s, px, py, rp);
+        Blk rp, rb;
+        if (win_holds_level(w, px, py, 0, max_x, max_y)) load_cand_win(sp, w, geo, px, py, rp);
+        else load_cand(ref, geo, px, py, rp);
         select_blk(g == 1, pre, rp, rb);
-        cand_metrics(s, rb, sad, mad, satd);
+        cand_metrics(narrow, s, rb, sad, mad, satd);
     } else {
-        cand_metrics(s, pre, sad, mad, satd);
+        cand_metrics(narrow, s, pre, sad, mad, satd);
     }
-    sad = row_sum(sad); mad = row_max(mad); satd = row_sum(satd);
     const uint32_t cst0 = g == 1 ? cost_bits(sad, satd, px - x, py - y, lambda)
                                  : cost_bits(sad, satd, 0, 0, lambda);
     best_sad = lane_value(sad, 0);
     best_mad = lane_value(mad, 0);
     best_cost = (uint32_t)lane_value((int)cst0, 0);
 
-    // evaluate_motion_candidate's acceptance (motion.cpp:125-178), C
-    // precedence: && binds tighter than ||.
-    auto fold = [&](int cx, int cy, int csad, int cmad, uint32_t ccost) {
-        const int ssd = (cx - x) * (cx - x) + (cy - y) * (cy - y);
+    if (has_pred) {
+        // evaluate_motion_candidate's acceptance (motion.cpp:125-178), C
+        // precedence: && binds tighter than ||.
+        const int csad = lane_value(sad, 16), cmad = lane_value(mad, 16);
+        const uint32_t ccost = (uint32_t)lane_value((int)cst0, 16);
+        const int ssd = dist(px, py);
         bool accept;
         if (best_mad < thresh)
             accept = cmad < best_mad || (cmad == best_mad && ssd < best_ssd);
@@ -468,13 +671,9 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
                       ((ccost == best_cost && ssd < best_ssd) && csad < kSadThreshold)) ||
                      cmad < thresh;
         if (accept) {
-            best_x = cx; best_y = cy;
+            best_x = px; best_y = py;
             best_sad = csad; best_ssd = ssd; best_mad = cmad; best_cost = ccost;
         }
-    };
-
-    if (has_pred) {
-        fold(px, py, lane_value(sad, 16), lane_value(mad, 16), (uint32_t)lane_value((int)cst0, 16));
         n_int++;
     }
 
@@ -483,17 +682,15 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
         if (best_sad < kSadThreshold / 4) radius = kSearchRadius / 4;
         else if (best_sad < kSadThreshold / 2) radius = kSearchRadius / 2;
 
-        bool copy = false, staged = false;
+        bool copy = false;
         for (int step = radius; step > 0 && !copy; step >>= 1) {
-            // perform_inter_motion_search (motion.cpp:381-406).  The centre of
-            // the level is its starting best, so its metrics are known; it is
-            // still folded at its place in the scan (after slot 3), because
-            // the initial candidate carries best_ssd = INT32_MAX and a tie on
-            // cost then moves best_ssd to the centre's true distance.
+            // perform_inter_motion_search (motion.cpp:381-406).
             const int base_x = best_x, base_y = best_y;
-            if (step <= 8 && !staged) {
-                stage_window(ref, base_x, base_y, 2 * step - 1, sp);
-                staged = true;
+            bool in_win = win_holds_level(w, base_x, base_y, step, max_x, max_y);
+            if (!in_win && step <= 8) {
+                w = stage_window(ref, base_x, base_y, 2 * step - 1, sp);
+                window_ready();
+                in_win = true;
             }
             uint32_t mask = 0;
 #pragma unroll
@@ -507,22 +704,20 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
             const int ay = min(max(base_y + ja * step, 0), max_y);
             const int bxx = min(max(base_x + ib * step, 0), max_x);
             const int byy = min(max(base_y + jb * step, 0), max_y);
-            RefBlk ra, rb;
+            Blk ra, rb;
             const bool two = (mask & 0xF0u) != 0;
-            if (staged) load_cand_win(sp, s, ax, ay, ra);
-            else load_cand(ref, s, ax, ay, ra);
+            if (in_win) load_cand_win(sp, w, geo, ax, ay, ra);
+            else load_cand(ref, geo, ax, ay, ra);
             if (two) {
-                if (staged) load_cand_win(sp, s, bxx, byy, rb);
-                else load_cand(ref, s, bxx, byy, rb);
+                if (in_win) load_cand_win(sp, w, geo, bxx, byy, rb);
+                else load_cand(ref, geo, bxx, byy, rb);
             }
             int sad_a, mad_a, satd_a, sad_b = 0, mad_b = 0, satd_b = 0;
-            cand_metrics(s, ra, sad_a, mad_a, satd_a);
-            sad_a = row_sum(sad_a); mad_a = row_max(mad_a); satd_a = row_sum(satd_a);
+            cand_metrics(narrow, s, ra, sad_a, mad_a, satd_a);
             const uint32_t cst_a = cost_bits(sad_a, satd_a, ax - x, ay - y, lambda);
             uint32_t cst_b = 0;
             if (two) {
-                cand_metrics(s, rb, sad_b, mad_b, satd_b);
-                sad_b = row_sum(sad_b); mad_b = row_max(mad_b); satd_b = row_sum(satd_b);
+                cand_metrics(narrow, s, rb, sad_b, mad_b, satd_b);
                 cst_b = cost_bits(sad_b, satd_b, bxx - x, byy - y, lambda);
             }
             // The level's fold (motion.cpp:388-405 over evaluate_motion_candidate)
@@ -533,67 +728,49 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
             //  * Otherwise the best ends at the minimum cost m: the first
             //    element (in order) at cost m, replaced by any later element
             //    at cost m with a strictly smaller ssd and SAD < 8192.
-            // Branch-free over wave-uniform values.
-            uint32_t sc[8];
-            int ss[8], sm[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int ln = (k & 3) * 16;
-                sc[k] = (uint32_t)lane_value((int)(k < 4 ? cst_a : cst_b), ln);
-                ss[k] = lane_value(k < 4 ? sad_a : sad_b, ln);
-                sm[k] = lane_value(k < 4 ? mad_a : mad_b, ln);
-            }
-            uint32_t copym = 0;
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                copym |= (((mask >> k) & 1u) && sm[k] < thresh) ? (1u << k) : 0u;
-            auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
+            const bool va = ((mask >> g) & 1u) != 0, vb = ((mask >> (g + 4)) & 1u) != 0;
+            const uint32_t copym = group_bits(__ballot(va && mad_a < thresh)) |
+                                   (group_bits(__ballot(vb && mad_b < thresh)) << 4);
             if (copym) {
                 const int p = __builtin_ctz(copym);
                 n_int += __builtin_popcount(mask & ((2u << p) - 1u));
-                int nx = base_x, ny = base_y, nsad = 0, nmad = 0;
-                uint32_t nc = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const bool t = k == p;
-                    nx = t ? base_x + slot_i(k) * step : nx;
-                    ny = t ? base_y + slot_j(k) * step : ny;
-                    nsad = t ? ss[k] : nsad;
-                    nmad = t ? sm[k] : nmad;
-                    nc = t ? sc[k] : nc;
-                }
-                best_x = nx; best_y = ny; best_sad = nsad; best_mad = nmad; best_cost = nc;
-                best_ssd = dist(nx, ny);
+                const int ln = (p & 3) * 16;
+                best_sad = lane_value(p < 4 ? sad_a : sad_b, ln);
+                best_mad = lane_value(p < 4 ? mad_a : mad_b, ln);
+                best_cost = (uint32_t)lane_value((int)(p < 4 ? cst_a : cst_b), ln);
+                best_x = base_x + slot_i(p) * step;
+                best_y = base_y + slot_j(p) * step;
+                best_ssd = dist(best_x, best_y);
                 copy = true;                                   // motion.cpp:402-403
             } else {
                 n_int += __builtin_popcount(mask);
-                uint32_t m = best_cost;
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    m = ((mask >> k) & 1u) ? min(m, sc[k]) : m;
+                const uint32_t ml = min(va ? cst_a : 0xFFFFFFFFu, vb ? cst_b : 0xFFFFFFFFu);
+                const uint32_t m = min(min(best_cost, min((uint32_t)lane_value((int)ml, 0),
+                                                          (uint32_t)lane_value((int)ml, 16))),
+                                       min((uint32_t)lane_value((int)ml, 32),
+                                           (uint32_t)lane_value((int)ml, 48)));
+                const uint32_t eq = group_bits(__ballot(va && cst_a == m)) |
+                                    (group_bits(__ballot(vb && cst_b == m)) << 4);
                 bool have = best_cost == m;
+                // Visit order as bits: slots 0..3, the centre (bit 4), slots 4..7.
+                uint32_t seq = (eq & 0xFu) | (have ? 0x10u : 0u) | ((eq & 0xF0u) << 1);
                 int cx = best_x, cy = best_y, csad = best_sad, cmad = best_mad, cssd = best_ssd;
-                auto visit = [&](bool valid, uint32_t cost, int ex, int ey, int esad, int emad,
-                                 int essd) {
-                    const bool eq = valid && cost == m;
-                    const bool take = eq && (!have || (essd < cssd && esad < kSadThreshold));
-                    cx = take ? ex : cx;
-                    cy = take ? ey : cy;
-                    csad = take ? esad : csad;
-                    cmad = take ? emad : cmad;
-                    cssd = take ? essd : cssd;
-                    have = have || eq;
-                };
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int ex = base_x + slot_i(k) * step, ey = base_y + slot_j(k) * step;
-                    visit(((mask >> k) & 1u) != 0, sc[k], ex, ey, ss[k], sm[k], dist(ex, ey));
-                }
-                visit(true, best_cost, base_x, base_y, best_sad, best_mad, dist(base_x, base_y));
-#pragma unroll
-                for (int k = 4; k < 8; k++) {
-                    const int ex = base_x + slot_i(k) * step, ey = base_y + slot_j(k) * step;
-                    visit(((mask >> k) & 1u) != 0, sc[k], ex, ey, ss[k], sm[k], dist(ex, ey));
+                while (seq) {
+                    const int q = __builtin_ctz(seq);
+                    seq &= seq - 1;
+                    int ex = base_x, ey = base_y, esad = best_sad, emad = best_mad;
+                    if (q != 4) {
+                        const int k = q < 4 ? q : q - 1, ln = (k & 3) * 16;
+                        ex = base_x + slot_i(k) * step;
+                        ey = base_y + slot_j(k) * step;
+                        esad = lane_value(k < 4 ? sad_a : sad_b, ln);
+                        emad = lane_value(k < 4 ? mad_a : mad_b, ln);
+                    }
+                    const int essd = dist(ex, ey);
+                    if (!have || (essd < cssd && esad < kSadThreshold)) {
+                        cx = ex; cy = ey; csad = esad; cmad = emad; cssd = essd;
+                    }
+                    have = true;
                 }
                 best_x = cx; best_y = cy; best_sad = csad; best_mad = cmad; best_ssd = cssd;
                 best_cost = m;
@@ -609,17 +786,18 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
             if (tx >= 0 && tx <= max_x && ty >= 0 && ty <= max_y) mask |= 1u << k;
         }
         if (mask) {
-            if (!staged) stage_window(ref, best_x, best_y, 0, sp);
             const int bx = best_x, by = best_y;
-            const int wlx = bx - __builtin_amdgcn_readfirstlane(sp.org[0]);
-            const int wly = by - __builtin_amdgcn_readfirstlane(sp.org[1]);
-            const int wcx = (bx >> 1) - __builtin_amdgcn_readfirstlane(sp.org[2]);
-            const int wcy = (by >> 1) - __builtin_amdgcn_readfirstlane(sp.org[3]);
+            if (!win_holds_subpel(w, bx, by)) {
+                w = stage_window(ref, bx, by, 0, sp);
+                window_ready();
+            }
+            const int wlx = bx - w.x0, wly = by - w.y0;
+            const int wcx = (bx >> 1) - w.cx0, wcy = (by >> 1) - w.cy0;
             build_hv<16>(sp.win.y + wly * kWinLS + wlx, kWinLS, sp.y);
             build_hv<8>(sp.win.u + wcy * kWinCS + wcx, kWinCS, sp.u);
             build_hv<8>(sp.win.v + wcy * kWinCS + wcx, kWinCS, sp.v);
-            RefBlk full;
-            load_cand_win(sp, s, bx, by, full);
+            Blk full;
+            load_cand_win(sp, w, geo, bx, by, full);
             __syncthreads();
             build_d<16>(sp.y);
             build_d<8>(sp.u);
@@ -630,32 +808,10 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, const Src &s,
 #pragma unroll
             for (int slot = 0; slot < 2; slot++) {
                 const int di = slot ? ib : ia, dj = slot ? jb : ja;
-                const PlaneView py_ = plane_view<16>(sp.y, di, dj);
-                const PlaneView pu_ = plane_view<8>(sp.u, di, dj);
-                const PlaneView pv_ = plane_view<8>(sp.v, di, dj);
-                int sh = 0, mh = 0, sq = 0, mq = 0;
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int half = py_.base[(s.oy + r) * py_.stride + s.ox + c];
-                        const int quarter = (int16_t)((full.y[r * 4 + c] + half + 1) >> 1);
-                        const int sv = s.y[r * 4 + c];
-                        const int ah = abs(sv - half), aq = abs(sv - quarter);
-                        sh += ah; mh = max(mh, ah);
-                        sq += aq; mq = max(mq, aq);
-                    }
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    int half = pu_.base[s.cr * pu_.stride + s.cc + c];
-                    int quarter = (int16_t)((full.u[c] + half + 1) >> 1);
-                    mh = max(mh, abs(s.u[c] - half));
-                    mq = max(mq, abs(s.u[c] - quarter));
-                    half = pv_.base[s.cr * pv_.stride + s.cc + c];
-                    quarter = (int16_t)((full.v[c] + half + 1) >> 1);
-                    mh = max(mh, abs(s.v[c] - half));
-                    mq = max(mq, abs(s.v[c] - quarter));
-                }
+                int sh, mh, sq, mq;
+                subpel_metrics(narrow, s, full, plane_view<16>(sp.y, di, dj),
+                               plane_view<8>(sp.u, di, dj), plane_view<8>(sp.v, di, dj), geo, sh,
+                               mh, sq, mq);
                 m[slot * 4 + 0] = row_sum(sh);
                 m[slot * 4 + 1] = row_max(mh);
                 m[slot * 4 + 2] = row_sum(sq);
@@ -711,7 +867,7 @@ struct MbLds {
     int16_t tmp[16 * 21 + 2 * 8 * 13];   // diagonal H pass / DCT rows
 };
 
-union Shared {
+union alignas(16) Shared {
     SubpelLds sp;
     MbLds mb;
 };
@@ -884,18 +1040,20 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
 }
 
 // The source MB from the lanes that hold it (lanes 0..15 own the whole MB in
-// the layout of Src) into `dst` (16x16 Y, 8x8 U, 8x8 V, row-major).
-__device__ __forceinline__ void store_src(const Src &s, int16_t *dst)
+// the lane layout) into `dst` (16x16 Y, 8x8 U, 8x8 V, row-major).
+__device__ __forceinline__ void store_src(const Blk &s, const Geo &g, int16_t *dst)
 {
     if (lane_id() < 16) {
+        uint32_t *d = reinterpret_cast<uint32_t *>(dst);
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int r = 0; r < 4; r++) {
+            d[((g.oy + r) * 16 + g.ox) / 2] = s.y[2 * r];
+            d[((g.oy + r) * 16 + g.ox) / 2 + 1] = s.y[2 * r + 1];
+        }
 #pragma unroll
-            for (int c = 0; c < 4; c++) dst[(s.oy + r) * 16 + s.ox + c] = s.y[r * 4 + c];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            dst[256 + s.cr * 8 + s.cc + c] = s.u[c];
-            dst[320 + s.cr * 8 + s.cc + c] = s.v[c];
+        for (int k = 0; k < 2; k++) {
+            d[(256 + g.cr * 8 + g.cc) / 2 + k] = s.u[k];
+            d[(320 + g.cr * 8 + g.cc) / 2 + k] = s.v[k];
         }
     }
 }
@@ -1034,7 +1192,7 @@ __device__ __forceinline__ SearchOut unpack_search(int v, int k)
 // With a single reference the caller passes its search result in `own`;
 // otherwise all results are read back from job.searches.
 __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
-                          const Src &src, const SearchOut *own)
+                          const Blk &src, const Geo &geo, const SearchOut *own)
 {
     const int lane = lane_id();
     const int x = bx * 16, y = by * 16;
@@ -1048,7 +1206,7 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
     int best_k = -1, best_sad = kInt32Max, bidir = 0;
     int n_int = 0, n_ax = 0, n_dg = 0;
-    store_src(src, sh.src);
+    store_src(src, geo, sh.src);
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel.
         bool best_copy = false;
@@ -1179,13 +1337,15 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         const int bx = mb % job.wb, by = mb / job.wb;
         const int x = bx * 16, y = by * 16;
 
-        // Load the source block and the initial candidate while the
-        // neighbours finish.
-        Src s;
-        lane_geometry(s);
-        load_lane_block(job.src, x, y, s, s.y, s.u, s.v);
-        RefBlk pre;
-        load_cand(job.refs[ref], s, x, y, pre);
+        // Load the source block and the initial candidate, and stage the
+        // largest search window around the MB, while the neighbours finish.
+        const DevPic &rp = job.refs[ref];
+        const Geo geo = lane_geometry();
+        Blk s, pre;
+        load_cand(job.src, geo, x, y, s);
+        load_cand(rp, geo, x, y, pre);
+        const WinPos win = stage_window(rp, x, y, kReachMax, sh.sp);
+        const bool narrow = (*job.src.wide | *rp.wide) == 0;
 
         // Wait for the left neighbour (same row) and the top-right one (row
         // above; the whole row above at the right edge), backing off.
@@ -1217,11 +1377,12 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         mv_predictor(job, bx, by, pmx, pmy);
         pmx = __builtin_amdgcn_readfirstlane(pmx);
         pmy = __builtin_amdgcn_readfirstlane(pmy);
-        const SearchOut o = inter_search(job.refs[ref], job.width, job.height, s, pre, x, y,
+        window_ready();
+        const SearchOut o = inter_search(rp, job.width, job.height, narrow, s, geo, pre, win, x, y,
                                          job.thresh, job.lambda, pmx, pmy, sh.sp);
         if (job.n_refs == 1) {
             // Single reference: this wave finishes the MB directly.
-            finish_mb(job, bx, by, counters, sh.mb, s, &o);
+            finish_mb(job, bx, by, counters, sh.mb, s, geo, &o);
             continue;
         }
         uint32_t w[8];
@@ -1241,7 +1402,7 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         __syncthreads();
         const int done_before = __builtin_amdgcn_readfirstlane(s_word);
         __syncthreads();
-        if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, s, nullptr);
+        if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, s, geo, nullptr);
     }
 }
 
@@ -1262,6 +1423,19 @@ __global__ void apron_kernel(int16_t *plane0, int32_t pitch, int32_t w, int32_t 
         const int cx = min(max(X, 0), w - 1), cy = min(max(Y, 0), h - 1);
         plane0[Y * pitch + X] = plane0[cy * pitch + cx];
     }
+}
+
+// Flags pictures holding a sample outside [0, 2047] (one atomic per wave
+// that finds one).
+__global__ void range_kernel(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h,
+                             int32_t *wide)
+{
+    bool out = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
+        const int v = plane0[(i / w) * pitch + i % w];
+        out = out || v < 0 || v > 2047;
+    }
+    if (__ballot(out) != 0 && lane_id() == 0) atomicOr(wide, 1);
 }
 
 // params: n x {x, y, dir_x, dir_y, quarter}
@@ -1295,13 +1469,15 @@ __global__ __launch_bounds__(64) void search_blocks_kernel(DevPic src, DevPic re
     const int b = blockIdx.x;
     if (b >= n) return;
     const int32_t *p = params + b * 4;
-    Src s;
-    lane_geometry(s);
-    load_lane_block(src, p[0], p[1], s, s.y, s.u, s.v);
-    RefBlk pre;
-    load_cand(ref, s, p[0], p[1], pre);
-    SearchOut o = inter_search(ref, w, h, s, pre, p[0], p[1], (quality >> 2) + 1, lambda, p[2], p[3],
-                               sp);
+    const Geo geo = lane_geometry();
+    Blk s, pre;
+    load_cand(src, geo, p[0], p[1], s);
+    load_cand(ref, geo, p[0], p[1], pre);
+    const WinPos win = stage_window(ref, p[0], p[1], kReachMax, sp);
+    const bool narrow = (*src.wide | *ref.wide) == 0;
+    window_ready();
+    SearchOut o = inter_search(ref, w, h, narrow, s, geo, pre, win, p[0], p[1], (quality >> 2) + 1,
+                               lambda, p[2], p[3], sp);
     if (lane_id() == 0) {
         evxa_search_result r;
         r.best_x = o.best_x; r.best_y = o.best_y;
@@ -1356,6 +1532,15 @@ hipError_t launch_apron(int16_t *plane0, int32_t pitch, int32_t w, int32_t h, hi
     const int total = (w + 2 * kApron) * (h + 2 * kApron);
     const int blocks = min((total + 255) / 256, 2048);
     hipLaunchKernelGGL(apron_kernel, dim3(blocks), dim3(256), 0, st, plane0, pitch, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,
+                        hipStream_t st)
+{
+    const int blocks = min((w * h + 255) / 256, 1024);
+    hipLaunchKernelGGL(range_kernel, dim3(max(blocks, 1)), dim3(256), 0, st, plane0, pitch, w, h,
+                       wide);
     return hipGetLastError();
 }
 

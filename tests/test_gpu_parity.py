@@ -106,6 +106,57 @@ def test_search_blocks(ctx, oracle, quality):
     s.close()
 
 
+def _stretch(pics, lo, hi):
+    """Map the samples of all `pics` affinely onto [lo, hi] (same map for all)."""
+    a = min(int(c.min()) for p in pics for c in p)
+    b = max(int(c.max()) for p in pics for c in p)
+    return [tuple(((c.astype(np.int64) - a) * (hi - lo) // (b - a) + lo).astype(np.int16)
+                  for c in p) for p in pics]
+
+
+# Sample ranges that select the metric paths: 8-bit content as generated
+# (packed 16-bit path), stretched onto [0, 2047] (packed path at the edge of
+# its range, largest differences), onto [1, 2048] (one past it: 32-bit path),
+# onto negative samples, and onto [-32000, 32000] (differences beyond int16,
+# which the Hadamard input truncates like analysis.h:102).
+RANGE_CASES = [None, (0, 2047), (1, 2048), (-3000, -1000), (-32000, 32000)]
+
+
+@pytest.mark.parametrize("span", RANGE_CASES)
+def test_search_blocks_sample_ranges(ctx, oracle, span):
+    w, h = 128, 96
+    ref, cur = _pair(77, w, h)
+    if span is not None:
+        ref, cur = _stretch([ref, cur], *span)
+    s = ctx.open_stream(w, h, 2)
+    _upload(s, 0, cur)
+    _upload(s, 1, ref)
+    R, S = Picture(*ref), Picture(*cur)
+    rng = np.random.default_rng(5)
+    xs, ys, pxs, pys = [], [], [], []
+    for by in range(h // 16):
+        for bx in range(w // 16):
+            pm = tuple(rng.integers(-12, 13, 2)) if (bx + by) % 2 else (0, 0)
+            xs.append(bx * 16); ys.append(by * 16); pxs.append(pm[0]); pys.append(pm[1])
+    got = s.perform_inter_search(0, 1, 8, xs, ys, pxs, pys)
+    for k in range(len(xs)):
+        r = oracle.inter_search(S.block(xs[k], ys[k]), R, xs[k], ys[k], 8, pxs[k], pys[k])
+        for f in evxa.SEARCH_DTYPE.names:
+            assert got[f][k] == getattr(r, f), (k, f, got[k], xs[k], ys[k], pxs[k], pys[k])
+    s.close()
+
+
+@pytest.mark.parametrize("span", RANGE_CASES[1:])
+def test_frame_p_sample_ranges(ctx, oracle, span):
+    frames = _stretch(synth.sequence_c1(91, 3, 128, 96, local_range=10), *span)
+    cur, refs = frames[2], [frames[1], frames[0]]
+    rec, co, lv = _run_p(ctx, cur, refs, 10)
+    orec, oco, olv = oracle.analyze_frame(1, 10, Picture(*cur), [Picture(*r) for r in refs])
+    assert_records_equal(rec, orec)
+    assert np.array_equal(co, oco)
+    assert np.array_equal(lv, olv)
+
+
 def _run_p(ctx, cur, refs, quality, prev_frames=()):
     h, w = cur[0].shape
     s = ctx.open_stream(w, h, 1 + len(refs))
