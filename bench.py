@@ -338,6 +338,99 @@ def run_plain(args):
     return result
 
 
+def run_gop(args):
+    """Hierarchical-B workload: a seeded 33-frame 3840x2160 sequence (zoom
+    0.5%/frame about the centre plus +-32 px local motion on 30% of blocks),
+    GOP 8 with I/P anchors and a 3-level B pyramid, quality 12 (reference Bs
+    +1, non-reference Bs +2).  One frame per launch in decode order; B frames
+    run forward, backward and bidirectional prediction."""
+    from chicago_amd import evxa, gop, synth
+
+    w, h = 3840, 2160
+    n = args.gop_frames
+    log(f"gop: generating {n} frames {w}x{h}")
+    frames = synth.sequence_c2(synth.seed_of(2, 0, 0), n, w, h)
+    roles = gop.gop8_schedule(n, 12)
+    ctx = evxa.Context(0)
+    s = ctx.open_stream(synth.align16(w), synth.align16(h), n)
+    for t, f in enumerate(frames):
+        s.upload(t, *f)
+    log("gop: uploaded")
+    work = [r for r in roles if r.kind != "I"]
+
+    def step(r):
+        if r.kind == "P":
+            if not r.temporal_mvp:
+                s.reset_temporal()
+            s.analyze_p(r.quality, r.index, r.refs)
+        else:
+            s.analyze_b(r.quality, r.index, r.refs[0], r.refs[1], target_ids=[0, 1])
+
+    # Untimed pass: per-frame latency, work accounting, outputs of a sample.
+    sample = {work[0].index, next(r.index for r in work if r.kind == "B")}
+    ops, lat, outs, ptables = 0, [], {}, {}
+    for r in work:
+        t0 = time.perf_counter()
+        step(r)
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+        rec, co, lv = s.results(coeffs=r.index in sample, levels=r.index in sample)
+        ops += record_ops(rec)
+        if r.kind == "P":
+            ptables[r.index] = rec
+        if r.index in sample:
+            outs[r.index] = (rec, co, lv)
+    ctx.launch_times()
+    for _ in range(args.warmup and 1):
+        for r in work:
+            step(r)
+    ctx.synchronize()
+    ctx.launch_times()
+    t0 = time.perf_counter()
+    for r in work:
+        step(r)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    kms = ctx.launch_times()
+    res = {
+        "workload": (f"gop: {w}x{h} GOP 8 (I/P anchors, 3-level hierarchical B), "
+                     f"{len(work)} analysed frames in decode order, quality 12, one frame per launch"),
+        "frames_per_s": round(len(work) / wall, 3),
+        "frames_1080p_eq_per_s": round(len(work) * (s.n_mbs / 8160) / wall, 3),
+        "ms_per_frame": round(wall / len(work) * 1e3, 3),
+        "kernel_ms_avg": round(float(np.mean(kms)), 3),
+        "achieved_tops": round(ops / (float(np.sum(kms)) / 1e3) / 1e12, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+    }
+    if not args.no_cpu:
+        from oracle.oracle import Oracle, Picture, Reference, reference_available
+        impl = Reference() if reference_available() else Oracle()
+        exact = True
+        for r in work:
+            if r.index not in outs:
+                continue
+            cur = Picture(*frames[r.index])
+            if r.kind == "P":
+                prev = ptables.get(r.refs[0]) if r.temporal_mvp else None
+                want = impl.analyze_frame(1, r.quality, cur, [Picture(*frames[r.refs[0]])], prev=prev)
+            else:
+                want = impl.analyze_frame(2, r.quality, cur, [Picture(*frames[k]) for k in r.refs],
+                                          target_ids=[0, 1])
+            got = outs[r.index]
+            fields = [f for f in want[0].dtype.names if f not in (
+                "mad", "cost_bits", "mad_b", "cost_b_bits", "n_int", "n_axial", "n_diag")
+                and not f.startswith("reserved")]
+            same = all(np.array_equal(want[0][f], got[0][f]) for f in fields)
+            same = same and np.array_equal(want[1], got[1]) and np.array_equal(want[2], got[2])
+            log(f"gop frame {r.index} ({r.kind}): CPU bit-exact={same}")
+            exact = exact and same
+        res["bit_exact_vs_cpu"] = exact
+    s.close()
+    ctx.close()
+    return res
+
+
 TRAFFIC_COPY_BYTES = 256 << 20
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "analyze_kernel_traffic_1080p.md")
 
@@ -393,7 +486,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--quality", type=int, default=8)
-    ap.add_argument("--workload", choices=["plain", "batched", "all", "traffic"], default="all")
+    ap.add_argument("--workload", choices=["plain", "batched", "gop", "all", "traffic"],
+                    default="all")
+    ap.add_argument("--gop-frames", type=int, default=33)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=4)
@@ -420,6 +515,8 @@ def main():
         for s in streams:
             s.close()
         ctx.close()
+    if args.workload in ("all", "gop"):
+        res["gop_b"] = run_gop(args)
     print(json.dumps(res), flush=True)
 
 

@@ -95,31 +95,80 @@ def subpel_plane(p, dx, dy, quarter):
     return half
 
 
+_TAPS = (1, -5, 20, 20, -5, 1)
+
+
+def _patches(p, x0, y0, size):
+    """(n, size, size) int32 patches of plane p with top-left corners
+    (x0, y0), coordinates clamped into the plane like interp.cpp:8-14."""
+    h, w = p.shape
+    r = np.arange(size)
+    rows = np.clip(y0[:, None] + r[None, :], 0, h - 1)
+    cols = np.clip(x0[:, None] + r[None, :], 0, w - 1)
+    return p[rows[:, :, None], cols[:, None, :]].astype(np.int32)
+
+
+def _subpel_blocks(patch, bs, dx, dy, quarter):
+    """subpel_plane's samples for bs x bs blocks whose patches start 3 samples
+    above and left of the block (patch size bs + 6)."""
+    xo = 0 if dx > 0 else -1
+    yo = 0 if dy > 0 else -1
+
+    def hpass(r0, nrows):
+        # Horizontal halves between columns c + xo and c + xo + 1 of patch rows
+        # r0 .. r0 + nrows - 1 for block columns c = 0 .. bs - 1.
+        acc = 0
+        for k, t in enumerate(_TAPS):
+            c0 = 3 + xo + k - 2
+            acc = acc + t * patch[:, r0:r0 + nrows, c0:c0 + bs]
+        return ((acc + 16) >> 5).astype(np.int16)
+
+    if dx != 0 and dy == 0:
+        half = hpass(3, bs)
+    elif dx == 0 and dy != 0:
+        acc = 0
+        for k, t in enumerate(_TAPS):
+            r0 = 3 + yo + k - 2
+            acc = acc + t * patch[:, r0:r0 + bs, 3:3 + bs]
+        half = ((acc + 16) >> 5).astype(np.int16)
+    else:
+        rows = hpass(3 + yo - 2, bs + 5).astype(np.int32)
+        acc = 0
+        for k, t in enumerate(_TAPS):
+            acc = acc + t * rows[:, k:k + bs, :]
+        half = ((acc + 16) >> 5).astype(np.int16)
+    if quarter:
+        full = patch[:, 3:3 + bs, 3:3 + bs]
+        return ((full + half.astype(np.int32) + 1) >> 1).astype(np.int16)
+    return half
+
+
 def move_blocks(rng, prev, mvx, mvy, sp_state, noise_sigma=2.0, visible=None):
     """Build a picture from `prev` block by block.
 
     mvx/mvy: int arrays (hb, wb) of integer displacements; sp_state: int array
     (hb, wb) with -1 for none, else dir_index*2 + quarter (dir_index into DIRS).
+    Sub-pel samples follow subpel_plane (interp.cpp:52-164), computed only on
+    each block's clamped neighbourhood.
     """
     ah, aw = prev[0].shape
     hb, wb = ah // 16, aw // 16
     bys, bxs = np.mgrid[0:hb, 0:wb]
-    sx = np.clip(bxs * 16 + np.asarray(mvx), 0, aw - 16)
-    sy = np.clip(bys * 16 + np.asarray(mvy), 0, ah - 16)
-    sp_state = np.asarray(sp_state)
-    states = np.unique(sp_state)
-    slot = np.searchsorted(states, sp_state)
+    sx = np.clip(bxs * 16 + np.asarray(mvx), 0, aw - 16).ravel()
+    sy = np.clip(bys * 16 + np.asarray(mvy), 0, ah - 16).ravel()
+    sp_state = np.broadcast_to(np.asarray(sp_state), (hb, wb)).ravel()
     out = []
     for ci, src in enumerate(prev):
         bs, sh = (16, 0) if ci == 0 else (8, 1)
-        stack = np.stack([src if st < 0 else
-                          subpel_plane(src, DIRS[st // 2][0], DIRS[st // 2][1], st & 1)
-                          for st in states])
-        r = np.arange(bs)
-        rows = (sy >> sh)[:, :, None, None] + r[None, None, :, None]
-        cols = (sx >> sh)[:, :, None, None] + r[None, None, None, :]
-        blocks = stack[slot[:, :, None, None], rows, cols]          # (hb, wb, bs, bs)
-        out.append(blocks.transpose(0, 2, 1, 3).reshape(hb * bs, wb * bs))
+        blocks = np.empty((hb * wb, bs, bs), np.int16)
+        for st in np.unique(sp_state):
+            idx = np.nonzero(sp_state == st)[0]
+            patch = _patches(src, (sx[idx] >> sh) - 3, (sy[idx] >> sh) - 3, bs + 6)
+            if st < 0:
+                blocks[idx] = patch[:, 3:3 + bs, 3:3 + bs]
+            else:
+                blocks[idx] = _subpel_blocks(patch, bs, DIRS[st // 2][0], DIRS[st // 2][1], st & 1)
+        out.append(blocks.reshape(hb, wb, bs, bs).transpose(0, 2, 1, 3).reshape(hb * bs, wb * bs))
     for i in range(3):
         n = np.rint(rng.normal(0.0, noise_sigma, size=out[i].shape)) if noise_sigma > 0 else 0
         out[i] = np.clip(out[i].astype(np.float64) + n, 0, 255).astype(np.int16)
@@ -163,4 +212,43 @@ def sequence_c1(seed, n_frames, width=1920, height=1080, local_range=24, local_f
         mvy = mvy + np.where(local, rng.integers(-local_range, local_range + 1, (hb, wb)), 0)
         sp = np.full((hb, wb), st)
         frames.append(move_blocks(rng, frames[-1], mvx, mvy, sp, visible=(width, height)))
+    return frames
+
+
+def _subpel_state(fx, fy):
+    """Nearest representable sub-pel state for a fractional offset in
+    [-0.5, 0.5]^2: none, or a direction with a half (0.5) or quarter (0.25)
+    step along it (the states of move_blocks)."""
+    ax, ay = abs(fx), abs(fy)
+    if max(ax, ay) < 0.125:
+        return -1
+    dx = 0 if ax < 0.125 else (1 if fx > 0 else -1)
+    dy = 0 if ay < 0.125 else (1 if fy > 0 else -1)
+    quarter = 1 if max(ax, ay) < 0.375 else 0
+    return DIRS.index((dx, dy)) * 2 + quarter
+
+
+def sequence_c2(seed, n_frames=33, width=3840, height=2160, zoom=0.005, local_range=32,
+                local_frac=0.3):
+    """Frame 0 from the texture recipe; frame t is frame t-1 zoomed in by
+    `zoom` about the picture centre (each block fetched from where the zoom
+    maps its centre, rounded to the nearest integer plus half/quarter-pel
+    state) with uniform +-`local_range` px local motion on `local_frac` of
+    the blocks."""
+    rng = np.random.default_rng(seed)
+    frames = [texture_frame(rng, width, height)]
+    ah, aw = frames[0][0].shape
+    hb, wb = ah // 16, aw // 16
+    bys, bxs = np.mgrid[0:hb, 0:wb]
+    cx, cy = (width - 1) / 2.0, (height - 1) / 2.0
+    # Source position of a block centre p under a zoom-in by s: c + (p - c) / s.
+    dxf = ((bxs * 16 + 7.5) - cx) * (1.0 / (1.0 + zoom) - 1.0)
+    dyf = ((bys * 16 + 7.5) - cy) * (1.0 / (1.0 + zoom) - 1.0)
+    ix, iy = np.rint(dxf).astype(np.int64), np.rint(dyf).astype(np.int64)
+    sp_zoom = np.vectorize(_subpel_state)(dxf - ix, dyf - iy)
+    for _ in range(1, n_frames):
+        local = rng.random((hb, wb)) < local_frac
+        mvx = ix + np.where(local, rng.integers(-local_range, local_range + 1, (hb, wb)), 0)
+        mvy = iy + np.where(local, rng.integers(-local_range, local_range + 1, (hb, wb)), 0)
+        frames.append(move_blocks(rng, frames[-1], mvx, mvy, sp_zoom, visible=(width, height)))
     return frames

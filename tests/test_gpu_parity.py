@@ -230,6 +230,40 @@ def test_frame_b(ctx, oracle, quality):
     s.close()
 
 
+def test_gop8_hierarchical_sequence(ctx, oracle):
+    """A GOP-8 sequence with a 3-level B pyramid in decode order: P anchors
+    chain their temporal tables, Bs use forward/backward/bidirectional
+    prediction at the pyramid's qualities."""
+    from chicago_amd import gop
+    n = 10
+    frames = synth.sequence_c2(61, n, 128, 96, local_range=12)
+    h, w = frames[0][0].shape
+    s = ctx.open_stream(w, h, n)
+    for k, f in enumerate(frames):
+        _upload(s, k, f)
+    ptables = {}
+    for r in gop.gop8_schedule(n, 12):
+        if r.kind == "I":
+            continue
+        cur = Picture(*frames[r.index])
+        if r.kind == "P":
+            if not r.temporal_mvp:
+                s.reset_temporal()
+            s.analyze_p(r.quality, r.index, r.refs)
+            prev = ptables.get(r.refs[0]) if r.temporal_mvp else None
+            want = oracle.analyze_frame(1, r.quality, cur, [Picture(*frames[r.refs[0]])], prev=prev)
+        else:
+            s.analyze_b(r.quality, r.index, r.refs[0], r.refs[1], target_ids=[0, 1])
+            want = oracle.analyze_frame(2, r.quality, cur, [Picture(*frames[k]) for k in r.refs],
+                                        target_ids=[0, 1])
+        rec, co, lv = s.results()
+        assert_records_equal(rec, want[0])
+        assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2]), r
+        if r.kind == "P":
+            ptables[r.index] = want[0]
+    s.close()
+
+
 def test_batch_uneven_streams_match_alone(ctx, oracle):
     """Streams of different sizes in one fused launch give the same results
     as each analysed alone, and as the oracle."""
