@@ -431,6 +431,68 @@ def run_gop(args):
     return res
 
 
+def run_serve(args):
+    """Serving: a real-time request loop over live streams at 60 fps (2 ms
+    exponential arrival jitter; 50% 1280x720, 35% 1920x1080, 15% 2560x1440;
+    exponential lifetimes of mean 20 s), P frames with one reference and the
+    previous frame's MV table, continuous batching through the native
+    dispatcher.  Streams are admitted by HBM budget and by a compute capacity
+    measured first: a burst of frames of the mix through the same dispatcher
+    gives the sustainable MB/s, and admission keeps the offered MB/s within
+    `--serve-util` of it."""
+    from chicago_amd import evxa, serve
+
+    ctx = evxa.Context(0)
+    cap = None
+    if args.serve_util > 0:
+        cap = calibrate_serving(ctx, args) * args.serve_util
+        log(f"serve: admission capacity {cap / 1e6:.2f} M MB/s")
+    res = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
+                    duration=args.serve_seconds, quality=args.quality,
+                    max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
+                    capacity_mbps=cap, log=log)
+    res["workload"] = (f"serve: {args.serve_streams} live streams offered at 60 fps "
+                       f"(50% 720p / 35% 1080p / 15% 1440p, mean life 20 s, 2 ms jitter), "
+                       f"{args.serve_seconds:g} s schedule, 1 ref, quality {args.quality}, "
+                       f"{args.serve_inflight} launches in flight")
+    res["admission_capacity_mb_per_s"] = cap
+    ctx.close()
+    return res
+
+
+def calibrate_serving(ctx, args):
+    """Sustained MB/s of the dispatcher on a backlog of mix frames (every
+    stream of a 64-stream mix queued for several frames)."""
+    from chicago_amd import evxa, serve, synth
+    frames = serve.pool_content(args.serve_seed, [m[:2] for m in serve.C4_MIX], 4)
+    descs = serve.descs_for(4, args.quality)
+    rng = np.random.default_rng(args.serve_seed)
+    shares = np.array([m[2] for m in serve.C4_MIX])
+    streams = []
+    for _ in range(64):
+        w, h, _ = serve.C4_MIX[rng.choice(3, p=shares)]
+        s = ctx.open_stream(synth.align16(w), synth.align16(h), 4)
+        for k, f in enumerate(frames[(w, h)]):
+            s.upload(k, *f)
+        streams.append(s)
+    n_per = 6
+    with evxa.Server(ctx, args.serve_inflight, args.serve_max_frames) as srv:
+        t0 = time.perf_counter()
+        for k in range(1, n_per + 1):
+            for i, s in enumerate(streams):
+                srv.submit(s, descs[k % 4], i * 100 + k)
+        n = 0
+        while n < 64 * n_per:
+            n += len(srv.poll())
+            time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+    mbs = sum(s.n_mbs for s in streams) * n_per
+    for s in streams:
+        s.close()
+    log(f"serve: calibration {mbs / wall / 1e6:.2f} M MB/s")
+    return mbs / wall
+
+
 TRAFFIC_COPY_BYTES = 256 << 20
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "analyze_kernel_traffic_1080p.md")
 
@@ -486,9 +548,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--quality", type=int, default=8)
-    ap.add_argument("--workload", choices=["plain", "batched", "gop", "all", "traffic"],
+    ap.add_argument("--workload", choices=["plain", "batched", "gop", "serve", "all", "traffic"],
                     default="all")
     ap.add_argument("--gop-frames", type=int, default=33)
+    ap.add_argument("--serve-streams", type=int, default=200)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-seed", type=int, default=1)
+    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-max-frames", type=int, default=64)
+    ap.add_argument("--serve-util", type=float, default=0.6)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=4)
@@ -517,6 +585,8 @@ def main():
         ctx.close()
     if args.workload in ("all", "gop"):
         res["gop_b"] = run_gop(args)
+    if args.workload in ("all", "serve"):
+        res["serve"] = run_serve(args)
     print(json.dumps(res), flush=True)
 
 

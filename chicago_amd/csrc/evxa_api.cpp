@@ -8,7 +8,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <set>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -210,6 +216,7 @@ struct evxa_stream {
     DevBuf wide;           // per slot: nonzero if a sample lies outside [0, 2047]
     int32_t p_cur = 0;
     bool temporal_valid = false;
+    std::atomic<int32_t> inflight{0};   // frames queued on a server and not yet complete
     int32_t last_table = -1;
     uint64_t bytes = 0;
 
@@ -317,7 +324,7 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
 // with the smallest unfinished v is always runnable: no list can deadlock.
 // Returns the most tickets any list holds on one virtual diagonal.
 int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per_list_cap,
-                   std::vector<uint32_t> &out)
+                   bool stagger, std::vector<uint32_t> &out)
 {
     int64_t total = 0;
     for (const auto &j : jobs) total += (int64_t)j.wb * j.hb * j.n_refs;
@@ -365,7 +372,7 @@ int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per
             return true;
         };
         int32_t s0 = 0;
-        while (s0 < horizon && !fits(s0)) s0 += 2;
+        while (stagger && s0 < horizon && !fits(s0)) s0 += 2;
         start[k] = s0;
         for (int32_t l = 0; l < n_lists; l++) {
             if ((int32_t)load[l].size() < s0 + nd) load[l].resize(s0 + nd, 0);
@@ -404,9 +411,18 @@ int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per
     return widest;
 }
 
-// Stage jobs + work list into `r` and enqueue reset + analysis kernels.
-void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool time_it)
+// Stage jobs + work list into `r` and enqueue reset + analysis kernels on
+// `st` (the context's queue unless given).  `grid_cap` bounds the waves of
+// the launch (0: all that fit the device); `stagger` staggers the jobs'
+// diagonals per list (batches of whole streams) or starts all at once
+// (serving, where compositions change every step).  `ev0`/`ev1`, if given,
+// are recorded around the analysis kernel instead of the context's ring.
+void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool time_it,
+             hipStream_t st = nullptr, int32_t grid_cap = 0, bool stagger = true,
+             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
+    if (!st) st = ctx->stream;
+    const int32_t max_grid = grid_cap > 0 ? std::min(grid_cap, ctx->grid) : ctx->grid;
     if (jobs.empty()) return;
     if ((int)jobs.size() > kMaxJobs) fail(EVXA_ERROR_CAPACITY_LIMIT, "too many jobs in one launch");
     std::vector<uint64_t> key;
@@ -426,20 +442,20 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
         // One list for a single picture (its rows all stay in flight), one
         // per XCD for batches.
         const int32_t lists = ctx->lists > 0 ? ctx->lists : (jobs.size() > 1 ? kLists : 1);
-        const int32_t n_lists = std::max(1, std::min(lists, ctx->grid));
-        const int32_t cap = std::max(16, ctx->grid / n_lists * ctx->cap_pct / 100);
-        const int32_t widest = build_work(jobs, n_lists, cap, t);
+        const int32_t n_lists = std::max(1, std::min(lists, max_grid));
+        const int32_t cap = std::max(16, max_grid / n_lists * ctx->cap_pct / 100);
+        const int32_t widest = build_work(jobs, n_lists, cap, stagger, t);
         // Per list, enough waves for its widest diagonal (plus slack for the
         // MBs still finishing the previous one), never more than fit at once:
         // waiting waves beyond that only add polling traffic.  The grid is a
         // multiple of n_lists so that every list has the same number of waves.
         const int32_t per_list = std::max(8, widest + widest / 2 + 4);
-        r.grid = n_lists * std::max(1, std::min(ctx->grid / n_lists, per_list));
+        r.grid = n_lists * std::max(1, std::min(max_grid / n_lists, per_list));
         r.h_tickets.ensure(t.size() * sizeof(uint32_t));
         memcpy(r.h_tickets.p, t.data(), t.size() * sizeof(uint32_t));
         if (r.tickets.n < t.size() * sizeof(uint32_t)) r.tickets.alloc(t.size() * sizeof(uint32_t));
         check(hipMemcpyAsync(r.tickets.p, r.h_tickets.p, t.size() * sizeof(uint32_t),
-                             hipMemcpyHostToDevice, ctx->stream),
+                             hipMemcpyHostToDevice, st),
               "copy work list");
         r.n_tickets = (int32_t)t.size() - kWorkHeader;
         r.key = key;
@@ -457,20 +473,20 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     r.h_jobs.ensure(jb);
     memcpy(r.h_jobs.p, jobs.data(), jb);
     if (r.jobs.n < jb) r.jobs.alloc(jb);
-    check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, ctx->stream), "copy jobs");
-    check(hipEventRecord(r.staged, ctx->stream), "hipEventRecord");
-    check(launch_reset(prog, n_progress + kCounterWords, ctx->stream), "reset kernel");
+    check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, st), "copy jobs");
+    check(hipEventRecord(r.staged, st), "hipEventRecord");
+    check(launch_reset(prog, n_progress + kCounterWords, st), "reset kernel");
     const int32_t grid = r.grid;
-    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    std::pair<hipEvent_t, hipEvent_t> ev{ev0, ev1};
     if (time_it) {
         if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
         ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
-        check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
     }
-    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), prog, grid, ctx->stream),
+    if (ev.first) check(hipEventRecord(ev.first, st), "hipEventRecord");
+    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), prog, grid, st),
           "analysis kernel");
+    if (ev.second) check(hipEventRecord(ev.second, st), "hipEventRecord");
     if (time_it) {
-        check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
         ctx->ring_head++;
         ctx->last_start = ev.first;
         ctx->last_stop = ev.second;
@@ -622,6 +638,7 @@ int evxa_stream_close(evxa_stream *s)
         if (!s) return;
         evxa_context *ctx = s->ctx;
         std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         ctx->used -= s->bytes;
@@ -682,6 +699,7 @@ int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc)
         if (!s || !desc) fail(EVXA_ERROR_INVALIDARG, "null argument");
         evxa_context *ctx = s->ctx;
         std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
         set_device(ctx);
         std::vector<DevJob> jobs{make_job(s, *desc, true)};
         enqueue(ctx, ctx->res, jobs, true);
@@ -701,6 +719,9 @@ int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams, const evx
                 if (streams[a] == streams[b])
                     fail(EVXA_ERROR_INVALIDARG, "a stream appears twice in one batch");
         }
+        for (int32_t a = 0; a < n; a++)
+            if (streams[a]->inflight.load() > 0)
+                fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
         std::vector<DevJob> jobs;
         jobs.reserve(n);
         for (int32_t a = 0; a < n; a++) jobs.push_back(make_job(streams[a], descs[a], true));
@@ -756,6 +777,7 @@ int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeff
         evxa_context *ctx = s->ctx;
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         check_counters(ctx, ctx->res);
@@ -938,6 +960,278 @@ int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src, cons
         check(hipMemcpy(levels, dl.p, sb, hipMemcpyDeviceToHost), "copy levels");
         check(hipMemcpy(qp, dqp.p, (size_t)n * 4, hipMemcpyDeviceToHost), "copy qp");
         check(hipMemcpy(variance, dv.p, (size_t)n * 4, hipMemcpyDeviceToHost), "copy variance");
+    });
+}
+
+}  // extern "C"
+
+// ---- serving: continuous batching over concurrent launches ----------------
+//
+// A server owns a dispatcher thread and `max_inflight` launch slots, each with
+// its own HIP stream, job/work buffers and events.  Submitted frames queue in
+// arrival order; whenever a slot is free, the dispatcher fuses every queued
+// frame whose stream has no frame in flight (up to `max_frames`) into one
+// persistent launch on that slot, sized to its share of the device, so
+// launches overlap on the GPU and frames that arrive while others run start
+// in the next free slot.  A stream has at most one frame in flight: the next
+// P frame of a stream reads the previous one's MV table, and a stream's
+// outputs are overwritten by its next frame.  Completion is detected by
+// polling the slot's event; each frame then yields a completion record.
+
+namespace {
+
+double clock_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct evxa_server {
+    evxa_context *ctx = nullptr;
+    int32_t max_inflight = 2;
+    int32_t max_frames = 64;
+    std::thread thread;
+    std::mutex mu;                       // guards everything below
+    std::condition_variable cv;
+    bool stop = false;
+    struct Req {
+        evxa_stream *s;
+        evxa_frame_desc d;
+        uint64_t tag;
+        double t_arrival;
+    };
+    struct Slot {
+        hipStream_t st = nullptr;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        LaunchRes res;
+        PinnedBuf flags;                 // error word of the launch, copied back
+        std::vector<Req> reqs;
+        bool busy = false;
+        double t_launch = 0.0;
+    };
+    std::deque<Req> pending;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::set<evxa_stream *> active;
+    std::deque<evxa_completion> done;
+    int32_t error = 0;
+    std::string error_msg;
+    uint64_t launches = 0, frames = 0;
+};
+
+namespace {
+
+void server_fail(evxa_server *srv, int code, const std::string &msg)
+{
+    if (!srv->error) {
+        srv->error = code;
+        srv->error_msg = msg;
+    }
+}
+
+// Launch `reqs` on `sl` (called with srv->mu held; takes ctx->mu).
+void server_launch(evxa_server *srv, evxa_server::Slot &sl, std::vector<evxa_server::Req> &reqs)
+{
+    evxa_context *ctx = srv->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    set_device(ctx);
+    std::vector<DevJob> jobs;
+    jobs.reserve(reqs.size());
+    for (auto &r : reqs) jobs.push_back(make_job(r.s, r.d, true));
+    const int32_t cap = std::max(1, ctx->grid / srv->max_inflight);
+    enqueue(ctx, sl.res, jobs, false, sl.st, cap, false, sl.t0, nullptr);
+    check(hipMemcpyAsync(sl.flags.p, sl.res.progress.p, (kCounterError + 1) * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, sl.st),
+          "copy launch flags");
+    check(hipEventRecord(sl.t1, sl.st), "hipEventRecord");
+    sl.reqs = std::move(reqs);
+    sl.busy = true;
+    sl.t_launch = clock_s();
+    srv->launches++;
+}
+
+void server_complete(evxa_server *srv, evxa_server::Slot &sl)
+{
+    const double now = clock_s();
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, sl.t0, sl.t1);
+    const int32_t flags = static_cast<const int32_t *>(sl.flags.p)[kCounterError];
+    int32_t status = EVXA_SUCCESS;
+    if (flags & kErrWaitTimeout) status = EVXA_ERROR_OPERATION_TIMEDOUT;
+    else if (flags & kErrSearchFailed) status = EVXA_ERROR_EXECUTION_FAILURE;
+    for (auto &r : sl.reqs) {
+        evxa_completion c;
+        memset(&c, 0, sizeof(c));
+        c.tag = r.tag;
+        c.status = status;
+        c.batch_frames = (int32_t)sl.reqs.size();
+        c.t_arrival = r.t_arrival;
+        c.t_launch = sl.t_launch;
+        c.t_done = now;
+        c.device_ms = ms;
+        srv->done.push_back(c);
+        srv->active.erase(r.s);
+        r.s->inflight--;
+        srv->frames++;
+    }
+    sl.reqs.clear();
+    sl.busy = false;
+}
+
+void server_run(evxa_server *srv)
+{
+    (void)hipSetDevice(srv->ctx->device);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        bool any_busy = false;
+        for (auto &sl : srv->slots) {
+            if (!sl->busy) continue;
+            const hipError_t q = hipEventQuery(sl->t1);
+            if (q == hipSuccess) {
+                server_complete(srv, *sl);
+            } else if (q == hipErrorNotReady) {
+                any_busy = true;
+            } else {
+                server_fail(srv, EVXA_ERROR_EXECUTION_FAILURE,
+                            std::string("analysis launch: ") + hipGetErrorString(q));
+                (void)hipStreamSynchronize(sl->st);
+                server_complete(srv, *sl);
+            }
+        }
+        evxa_server::Slot *free_slot = nullptr;
+        for (auto &sl : srv->slots)
+            if (!sl->busy) { free_slot = sl.get(); break; }
+        if (free_slot && !srv->pending.empty()) {
+            std::vector<evxa_server::Req> pick;
+            std::set<evxa_stream *> taken;
+            for (auto it = srv->pending.begin();
+                 it != srv->pending.end() && (int32_t)pick.size() < srv->max_frames;) {
+                if (srv->active.count(it->s) || taken.count(it->s)) { ++it; continue; }
+                taken.insert(it->s);
+                pick.push_back(*it);
+                it = srv->pending.erase(it);
+            }
+            if (!pick.empty()) {
+                for (auto &r : pick) srv->active.insert(r.s);
+                try {
+                    server_launch(srv, *free_slot, pick);
+                } catch (const Failure &e) {
+                    server_fail(srv, e.code, e.msg);
+                    for (auto &r : pick) {
+                        evxa_completion c;
+                        memset(&c, 0, sizeof(c));
+                        c.tag = r.tag;
+                        c.status = e.code;
+                        c.t_arrival = r.t_arrival;
+                        c.t_launch = c.t_done = clock_s();
+                        srv->done.push_back(c);
+                        srv->active.erase(r.s);
+                        r.s->inflight--;
+                    }
+                }
+                continue;
+            }
+        }
+        if (srv->stop && srv->pending.empty() && !any_busy) break;
+        if (any_busy) srv->cv.wait_for(lk, std::chrono::microseconds(20));
+        else srv->cv.wait(lk);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+double evxa_clock(void) { return clock_s(); }
+
+int evxa_server_create(evxa_context *ctx, int32_t max_inflight, int32_t max_frames,
+                       evxa_server **out)
+{
+    return guarded([&] {
+        if (!ctx || !out) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        *out = nullptr;
+        if (max_inflight < 1 || max_inflight > 4) fail(EVXA_ERROR_INVALIDARG, "max_inflight must be 1..4");
+        if (max_frames < 1 || max_frames > kMaxJobs) fail(EVXA_ERROR_INVALIDARG, "max_frames out of range");
+        auto srv = std::make_unique<evxa_server>();
+        srv->ctx = ctx;
+        srv->max_inflight = max_inflight;
+        srv->max_frames = max_frames;
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            set_device(ctx);
+            for (int32_t k = 0; k < max_inflight; k++) {
+                auto sl = std::make_unique<evxa_server::Slot>();
+                check(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking), "hipStreamCreate");
+                check(hipEventCreate(&sl->t0), "hipEventCreate");
+                check(hipEventCreate(&sl->t1), "hipEventCreate");
+                sl->flags.ensure(64);
+                srv->slots.push_back(std::move(sl));
+            }
+        }
+        evxa_server *raw = srv.get();
+        raw->thread = std::thread(server_run, raw);
+        *out = srv.release();
+    });
+}
+
+int evxa_server_submit(evxa_server *srv, evxa_stream *s, const evxa_frame_desc *desc, uint64_t tag,
+                       double t_arrival)
+{
+    return guarded([&] {
+        if (!srv || !s || !desc) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        if (s->ctx != srv->ctx) fail(EVXA_ERROR_INVALIDARG, "foreign stream");
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            if (srv->stop) fail(EVXA_ERROR_INVALID_RESOURCE, "server is stopping");
+            srv->pending.push_back({s, *desc, tag, t_arrival > 0.0 ? t_arrival : clock_s()});
+            s->inflight++;
+        }
+        srv->cv.notify_one();
+    });
+}
+
+int evxa_server_poll(evxa_server *srv, evxa_completion *out, int32_t max, int32_t *n_out)
+{
+    return guarded([&] {
+        if (!srv || !n_out || (max > 0 && !out)) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        int32_t n = 0;
+        while (n < max && !srv->done.empty()) {
+            out[n++] = srv->done.front();
+            srv->done.pop_front();
+        }
+        *n_out = n;
+        if (srv->error) fail(srv->error, srv->error_msg);
+    });
+}
+
+int evxa_server_pending(evxa_server *srv, int32_t *queued, int32_t *inflight)
+{
+    return guarded([&] {
+        if (!srv) fail(EVXA_ERROR_INVALIDARG, "null server");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (queued) *queued = (int32_t)srv->pending.size();
+        if (inflight) *inflight = (int32_t)srv->active.size();
+    });
+}
+
+int evxa_server_destroy(evxa_server *srv)
+{
+    return guarded([&] {
+        if (!srv) return;
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->stop = true;
+        }
+        srv->cv.notify_one();
+        if (srv->thread.joinable()) srv->thread.join();
+        for (auto &sl : srv->slots) {
+            if (sl->st) (void)hipStreamSynchronize(sl->st);
+            if (sl->t0) (void)hipEventDestroy(sl->t0);
+            if (sl->t1) (void)hipEventDestroy(sl->t1);
+            if (sl->st) (void)hipStreamDestroy(sl->st);
+        }
+        delete srv;
     });
 }
 

@@ -47,6 +47,12 @@ SEARCH_DTYPE = np.dtype([(n, "<i4") for n in (
 assert SEARCH_DTYPE.itemsize == 60
 
 
+COMPLETION_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("batch_frames", "<i4"),
+                             ("t_arrival", "<f8"), ("t_launch", "<f8"), ("t_done", "<f8"),
+                             ("device_ms", "<f4"), ("reserved", "<i4")])
+assert COMPLETION_DTYPE.itemsize == 48
+
+
 class FrameDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("quality", C.c_int32), ("src_slot", C.c_int32),
                 ("n_refs", C.c_int32), ("ref_slots", C.c_int32 * 4),
@@ -62,7 +68,8 @@ EXPORTS = (
     "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_launch_times",
     "evxa_read_results",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
-    "evxa_search_blocks", "evxa_dct_quant_blocks")
+    "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
+    "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy")
 
 
 class EvxaError(RuntimeError):
@@ -114,6 +121,12 @@ def load_library(path: str = LIB_PATH):
                                          I32P, I16P, I16P, vp]),
         "evxa_dct_quant_blocks": (C.c_int, [vp, C.c_int32, I16P, I16P, I32P, I16P, I16P, I32P,
                                             I32P]),
+        "evxa_clock": (C.c_double, []),
+        "evxa_server_create": (C.c_int, [vp, C.c_int32, C.c_int32, C.POINTER(vp)]),
+        "evxa_server_submit": (C.c_int, [vp, vp, C.POINTER(FrameDesc), C.c_uint64, C.c_double]),
+        "evxa_server_poll": (C.c_int, [vp, vp, C.c_int32, C.POINTER(C.c_int32)]),
+        "evxa_server_pending": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "evxa_server_destroy": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -319,3 +332,51 @@ class Stream:
                                        _p32(ys), _p16(px), _p16(py),
                                        out.ctypes.data_as(C.c_void_p)))
         return out
+
+
+def clock() -> float:
+    """The dispatcher's monotonic clock (seconds)."""
+    return load_library().evxa_clock()
+
+
+class Server:
+    """Continuous batching: a native dispatcher keeps up to `max_inflight`
+    fused launches on the device and fuses every frame that is ready (its
+    stream idle) into the next free one, up to `max_frames` per launch."""
+
+    def __init__(self, ctx: Context, max_inflight: int = 2, max_frames: int = 64):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(_lib.evxa_server_create(ctx.h, max_inflight, max_frames, C.byref(h)))
+        self.h = h
+        self._buf = np.zeros(4096, COMPLETION_DTYPE)
+
+    def submit(self, stream: "Stream", desc: FrameDesc, tag: int, t_arrival: float = 0.0):
+        _check(_lib.evxa_server_submit(self.h, stream.h, C.byref(desc), tag, t_arrival))
+
+    def poll(self):
+        """Completions available now (structured array of COMPLETION_DTYPE)."""
+        out = []
+        while True:
+            n = C.c_int32()
+            _check(_lib.evxa_server_poll(self.h, self._buf.ctypes.data_as(C.c_void_p),
+                                         len(self._buf), C.byref(n)))
+            out.append(self._buf[:n.value].copy())
+            if n.value < len(self._buf):
+                return np.concatenate(out)
+
+    def pending(self):
+        q, f = C.c_int32(), C.c_int32()
+        _check(_lib.evxa_server_pending(self.h, C.byref(q), C.byref(f)))
+        return q.value, f.value
+
+    def close(self):
+        if self.h:
+            _check(_lib.evxa_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

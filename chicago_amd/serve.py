@@ -1,0 +1,303 @@
+"""Serving loop: many live streams, continuous batching on one GPU.
+
+A seeded schedule of live streams (resolution mix, exponential lifetimes)
+emits P frames at a fixed rate with exponential arrival jitter.  Streams are
+admitted against the context's HBM budget (each admitted stream holds a
+device stream of picture slots from a pool, returned when it leaves) and,
+optionally, against a compute capacity in MB searches per second.  Arriving
+frames are submitted in real time to the native dispatcher (`evxa.Server`),
+which fuses every ready frame into the next free launch; per-frame latency is
+completion time minus arrival time.
+
+The reference encoder analyses one frame at a time inside
+engine_encode_frame (encode.cpp:945) and has no serving path; this loop is
+the multi-stream request path built on the same per-frame analysis.
+
+Content: each resolution class uses one seeded config-1-style sequence
+(global pan plus local block motion, synth.sequence_c1) of `pool_frames`
+frames, uploaded once per pooled device stream; logical frame k of a stream
+analyses slot k mod pool_frames against slot (k - 1) mod pool_frames, with
+the previous frame's MV table as temporal candidate.  Reference pictures
+therefore stay resident in HBM and only the request crosses the host.
+"""
+from __future__ import annotations
+
+import heapq
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# (width, height, share of streams)
+C4_MIX = [(1280, 720, 0.50), (1920, 1080, 0.35), (2560, 1440, 0.15)]
+
+
+@dataclass
+class StreamLife:
+    sid: int
+    width: int
+    height: int
+    t_join: float
+    t_leave: float
+    phase: float
+
+
+def make_schedule(seed, n_live=200, duration=10.0, mean_life=20.0, mix=C4_MIX):
+    """Seeded join/leave schedule keeping `n_live` streams live: each seat's
+    stream lives an exponential time (mean `mean_life` s) and is replaced at
+    once by a new stream with a fresh resolution draw.  Returns the lives
+    that overlap [0, duration)."""
+    rng = np.random.default_rng(seed)
+    shares = np.array([m[2] for m in mix])
+    lives = []
+    sid = 0
+    for _ in range(n_live):
+        t = 0.0
+        while t < duration:
+            k = rng.choice(len(mix), p=shares / shares.sum())
+            life = rng.exponential(mean_life)
+            w, h, _ = mix[k]
+            lives.append(StreamLife(sid, w, h, t, t + life, float(rng.uniform(0.0, 1.0))))
+            sid += 1
+            t += life
+    return lives
+
+
+def frame_arrivals(lives, seed, fps=60.0, jitter=0.002, duration=10.0):
+    """(arrival time, stream id, frame number) of every frame, sorted: frame k
+    of a stream is due at t_join + (phase + k) / fps plus an exponential delay
+    of mean `jitter`.  Frame 0 of a stream is its intra picture and is not
+    analysed; frames 1.. are P frames."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for life in lives:
+        end = min(life.t_leave, duration)
+        k = 1
+        while True:
+            due = life.t_join + (life.phase + k) / fps
+            if due >= end:
+                break
+            out.append((due + rng.exponential(jitter), life.sid, k))
+            k += 1
+    out.sort()
+    return out
+
+
+@dataclass
+class Pool:
+    """Device streams of one resolution class, reused across stream lives."""
+    width: int
+    height: int
+    free: list = field(default_factory=list)
+    opened: int = 0
+
+
+class Admission:
+    """HBM-budget and compute-capacity admission of stream lives."""
+
+    def __init__(self, ctx, pool_frames, frames, capacity_mbps=None, fps=60.0):
+        self.ctx, self.pool_frames, self.frames = ctx, pool_frames, frames
+        self.capacity, self.fps = capacity_mbps, fps
+        self.load = 0.0
+        self.pools = {}
+        self.rejected_memory = 0
+        self.rejected_compute = 0
+
+    def demand(self, w, h):
+        return (w // 16) * (((h + 15) // 16)) * self.fps
+
+    def prefill(self, w, h, n):
+        """Open up to `n` device streams of a class ahead of time (within the
+        HBM budget), so joins during the run find one in the pool."""
+        from chicago_amd import evxa, synth
+        pool = self.pools.setdefault((w, h), Pool(w, h))
+        while pool.opened < n:
+            try:
+                s = self.ctx.open_stream(synth.align16(w), synth.align16(h), self.pool_frames)
+            except evxa.EvxaError as e:
+                if e.code == 7:
+                    return
+                raise
+            for k, f in enumerate(self.frames[(w, h)]):
+                s.upload(k, *f)
+            pool.opened += 1
+            pool.free.append(s)
+
+    def acquire(self, w, h):
+        """A device stream for a new life, or None if it is not admitted."""
+        from chicago_amd import evxa, synth
+        d = self.demand(w, h)
+        if self.capacity is not None and self.load + d > self.capacity:
+            self.rejected_compute += 1
+            return None
+        pool = self.pools.setdefault((w, h), Pool(w, h))
+        if pool.free:
+            s = pool.free.pop()
+        else:
+            try:
+                s = self.ctx.open_stream(synth.align16(w), synth.align16(h), self.pool_frames)
+            except evxa.EvxaError as e:
+                if e.code == 7:          # EVXA_ERROR_CAPACITY_LIMIT: HBM budget
+                    self.rejected_memory += 1
+                    return None
+                raise
+            for k, f in enumerate(self.frames[(w, h)]):
+                s.upload(k, *f)
+            pool.opened += 1
+        s.reset_temporal()
+        self.load += d
+        return s
+
+    def release(self, w, h, s):
+        self.pools[(w, h)].free.append(s)
+        self.load -= self.demand(w, h)
+
+
+def pool_content(seed, classes, pool_frames):
+    from chicago_amd import synth
+    return {(w, h): synth.sequence_c1(synth.seed_of(4, w, h, seed), pool_frames, w, h)
+            for (w, h) in classes}
+
+
+def descs_for(pool_frames, quality):
+    """Frame descriptor of logical frame k (by k mod pool_frames)."""
+    from chicago_amd import evxa
+    return [evxa.frame_desc(evxa.FRAME_P, quality, k % pool_frames, [(k - 1) % pool_frames])
+            for k in range(pool_frames)]
+
+
+def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_life=20.0,
+        quality=8, pool_frames=6, max_inflight=2, max_frames=64, capacity_mbps=None,
+        mix=C4_MIX, log=None):
+    """Run the request loop in real time for `duration` seconds of schedule
+    and return latency and throughput statistics."""
+    from chicago_amd import evxa
+
+    lives = make_schedule(seed, n_live, duration, mean_life, mix)
+    arrivals = frame_arrivals(lives, seed + 1, fps, jitter, duration)
+    by_sid = {life.sid: life for life in lives}
+    frames = pool_content(seed, sorted({(l.width, l.height) for l in lives}), pool_frames)
+    adm = Admission(ctx, pool_frames, frames, capacity_mbps, fps)
+    descs = descs_for(pool_frames, quality)
+
+    # Device streams for the most lives of each class alive at once are
+    # opened before the clock starts; streams live at time 0 are admitted
+    # then, later joins when their first frame is due.
+    events = sorted([(l.t_join, 1, (l.width, l.height)) for l in lives] +
+                    [(l.t_leave, -1, (l.width, l.height)) for l in lives])
+    alive, most = {}, {}
+    for _, d, key in events:
+        alive[key] = alive.get(key, 0) + d
+        most[key] = max(most.get(key, 0), alive[key])
+    if capacity_mbps is not None:
+        # no more streams of a class than the capacity admits
+        for key in most:
+            most[key] = min(most[key], int(capacity_mbps // adm.demand(*key)) + 1)
+    for key, n in sorted(most.items()):
+        adm.prefill(key[0], key[1], n)
+    live = {}                 # sid -> device stream
+    refused = set()
+    for life in lives:
+        if life.t_join == 0.0:
+            s = adm.acquire(life.width, life.height)
+            if s is None:
+                refused.add(life.sid)
+            else:
+                live[life.sid] = s
+    if log:
+        log(f"serve: {len(lives)} stream lives, {len(arrivals)} frames offered, "
+            f"{len(live)} admitted at start")
+
+    # Leaves, in time order, release a stream once its last frame is done.
+    last_frame = {}
+    for t, sid, k in arrivals:
+        last_frame[sid] = k
+    outstanding = {}          # sid -> frames submitted and not complete
+    leaving = []              # heap of (t_leave, sid)
+    for life in lives:
+        heapq.heappush(leaving, (life.t_leave, life.sid))
+    tags = {}                 # tag -> (sid, k, nmbs)
+    lat, dev, batch, mbs_done = [], [], [], 0
+    srv = evxa.Server(ctx, max_inflight, max_frames)
+    t0 = evxa.clock() + 0.05
+    i, n_sub = 0, 0
+
+    def drain():
+        nonlocal mbs_done
+        comp = srv.poll()
+        for c in comp:
+            sid, k, nmbs = tags.pop(int(c["tag"]))
+            if c["status"] != 0:
+                raise evxa.EvxaError(int(c["status"]), f"frame {k} of stream {sid} failed")
+            lat.append(c["t_done"] - c["t_arrival"])
+            dev.append(c["device_ms"])
+            batch.append(c["batch_frames"])
+            mbs_done += nmbs
+            outstanding[sid] -= 1
+        return len(comp)
+
+    def release_left(now_rel):
+        while leaving and leaving[0][0] <= now_rel:
+            t_leave, sid = leaving[0]
+            if sid in live and outstanding.get(sid, 0) > 0:
+                break                    # its last frames are still running
+            heapq.heappop(leaving)
+            if sid in live:
+                life = by_sid[sid]
+                adm.release(life.width, life.height, live.pop(sid))
+
+    while i < len(arrivals) or tags:
+        now = evxa.clock() - t0
+        while i < len(arrivals) and arrivals[i][0] <= now:
+            t, sid, k = arrivals[i]
+            i += 1
+            if sid in refused:
+                continue
+            if sid not in live:
+                life = by_sid[sid]
+                s = adm.acquire(life.width, life.height)
+                if s is None:
+                    refused.add(sid)
+                    continue
+                live[sid] = s
+            s = live[sid]
+            d = descs[k % pool_frames]
+            if k == 1:
+                s.reset_temporal()
+            tag = n_sub
+            n_sub += 1
+            tags[tag] = (sid, k, s.n_mbs)
+            outstanding[sid] = outstanding.get(sid, 0) + 1
+            srv.submit(s, d, tag, t0 + t)
+        drain()
+        release_left(now)
+        if i < len(arrivals):
+            wait = arrivals[i][0] - (evxa.clock() - t0)
+            if wait > 0:
+                time.sleep(min(wait, 0.0005))
+        else:
+            time.sleep(0.0002)
+    wall = evxa.clock() - t0
+    srv.close()
+    lat = np.array(lat) * 1e3
+    res = {
+        "offered_streams_live": n_live,
+        "stream_lives": len(lives),
+        "admitted_lives": len(lives) - len(refused),
+        "refused_compute": adm.rejected_compute,
+        "refused_memory": adm.rejected_memory,
+        "frames": int(len(lat)),
+        "schedule_s": duration,
+        "wall_s": round(wall, 3),
+        "frames_per_s": round(len(lat) / wall, 2),
+        "frames_1080p_eq_per_s": round(mbs_done / 8160 / wall, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p90": round(float(np.percentile(lat, 90)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)} if len(lat) else None,
+        "mean_frames_per_launch": round(float(np.mean(batch)), 2) if batch else 0,
+        "mean_launch_ms": round(float(np.mean(dev)), 3) if dev else 0,
+        "pooled_device_streams": {f"{w}x{h}": p.opened for (w, h), p in adm.pools.items()},
+        "hbm_bytes": ctx.bytes_used,
+    }
+    return res

@@ -182,6 +182,45 @@ int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src,
                           const int16_t *pred, const int32_t *quality, int16_t *coeffs,
                           int16_t *levels, int32_t *qp, int32_t *variance);
 
+/* ---- serving: continuous batching (no counterpart in the reference, whose
+ *      encoder analyses one frame at a time inside engine_encode_frame,
+ *      encode.cpp:945; this is the multi-stream request path) ------------ */
+
+typedef struct evxa_server evxa_server;
+
+/* One finished frame.  Times are evxa_clock() seconds: t_arrival as given at
+ * submit, t_launch when its fused launch was enqueued, t_done when the
+ * dispatcher saw the launch complete.  device_ms is the launch's kernel
+ * time; batch_frames the number of frames fused into it. */
+typedef struct evxa_completion {
+    uint64_t tag;
+    int32_t status;
+    int32_t batch_frames;
+    double t_arrival;
+    double t_launch;
+    double t_done;
+    float device_ms;
+    int32_t reserved;
+} evxa_completion;
+
+/* Monotonic clock (seconds) shared with the dispatcher. */
+double evxa_clock(void);
+/* Start a dispatcher with `max_inflight` (1..4) concurrent launches, each
+ * fusing up to `max_frames` ready frames. */
+int evxa_server_create(evxa_context *ctx, int32_t max_inflight, int32_t max_frames,
+                       evxa_server **out);
+/* Queue one frame (as for evxa_analyze_frame).  Frames of one stream run in
+ * submission order, one at a time; t_arrival <= 0 means now. */
+int evxa_server_submit(evxa_server *srv, evxa_stream *s, const evxa_frame_desc *desc,
+                       uint64_t tag, double t_arrival);
+/* Move up to `max` completions to `out` (non-blocking); *n_out receives the
+ * count.  Returns the first dispatcher error, if any. */
+int evxa_server_poll(evxa_server *srv, evxa_completion *out, int32_t max, int32_t *n_out);
+/* Frames queued and frames in flight. */
+int evxa_server_pending(evxa_server *srv, int32_t *queued, int32_t *inflight);
+/* Run every queued frame to completion, then stop the dispatcher. */
+int evxa_server_destroy(evxa_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
