@@ -2,7 +2,7 @@
 //
 // Pictures, record tables, coefficients and levels stay resident in HBM; the
 // host enqueues frame jobs on the context's HIP stream.  Every analysis
-// launch is: a reset of the work-list counters and row-progress words (a fill
+// launch is: a reset of the work-list counters and MB predictor words (a fill
 // kernel, so the sequence can be captured into a hipGraph), then one
 // persistent analysis kernel over the MB wavefronts of all jobs.
 #include <hip/hip_runtime.h>
@@ -155,7 +155,7 @@ float lambda_of(int32_t quality)
 }
 
 // Device-side resources of one launch configuration: job table, work list,
-// row-progress words and counters.  Graphs own theirs, so replays never see
+// predictor words and counters.  Graphs own theirs, so replays never see
 // another launch's tables.
 struct LaunchRes {
     DevBuf jobs, tickets, progress, counters;
@@ -430,7 +430,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     for (const auto &j : jobs) {
         if ((int64_t)j.wb * j.hb > (int64_t)kMbMask) fail(EVXA_ERROR_CAPACITY_LIMIT, "frame too large");
         key.push_back(((uint64_t)j.wb << 40) | ((uint64_t)j.hb << 8) | (uint32_t)j.n_refs);
-        n_progress += j.hb * kProgressStride;
+        n_progress += j.wb * j.hb * 2;          // one 64-bit predictor word per MB
     }
     if (!r.staged) check(hipEventCreateWithFlags(&r.staged, hipEventDisableTiming), "hipEventCreate");
     // The previous launch's host->device copies must have consumed the
@@ -466,8 +466,8 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     int32_t *prog = r.progress.as<int32_t>();
     int32_t off = kCounterWords;
     for (auto &j : jobs) {
-        j.progress = prog + off;
-        off += j.hb * kProgressStride;
+        j.mvw = reinterpret_cast<uint64_t *>(prog + off);
+        off += j.wb * j.hb * 2;
     }
     const size_t jb = jobs.size() * sizeof(DevJob);
     r.h_jobs.ensure(jb);

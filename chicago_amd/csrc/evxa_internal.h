@@ -53,12 +53,8 @@ struct DevJob {
     int16_t *levels;
     SearchRec *searches;               // EVXA_MAX_REFS per MB
     int32_t *mb_done;                  // searches finished per MB (returns to 0)
-    int32_t *progress;                 // per row, kProgressStride words apart
+    uint64_t *mvw;                     // per MB: predictor word, zeroed per launch
 };
-
-// Row-progress words sit on separate 256-byte lines so that waiting waves
-// poll different cache lines.
-constexpr int kProgressStride = 64;
 
 // Work-list entry: job (10 bits) | reference (2 bits) | MB index (20 bits).
 constexpr int kMbBits = 20;

@@ -32,10 +32,10 @@ constexpr int kSadThreshold = 8 * 1024;  // motion.cpp:24
 constexpr int kSearchRadius = 16;        // motion.cpp:29
 constexpr int kInt32Max = 0x7FFFFFFF;
 constexpr int kSpinLimit = 1 << 19;
-// Longest back-off between two polls of a dependency, in s_sleep(2) units
-// (about 128 cycles each): waits last tens of thousands of cycles, and every
-// poll is a trip to L2 through the CU's memory pipeline.
-constexpr int kMaxNap = 16;
+// Longest back-off between two polls of a dependency, in s_sleep(1) units
+// (about 64 cycles each): a MB's wait is on the dependency chain, so the
+// back-off stays short; every poll is one memory instruction of four lanes.
+constexpr int kMaxNap = 8;
 
 // xftables.h:57-67: round(128 cos((2k+1) i pi / 16)).
 __constant__ int16_t c_dct8[64] = {
@@ -1077,6 +1077,23 @@ __device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A MB's predictor word: bit 63 = published, bits 32..39 = block type,
+// bits 0..31 = MV (x low, y high; record word 1).  Zero until published.
+__device__ __forceinline__ uint64_t pred_word(int btype, uint32_t mv)
+{
+    return (1ull << 63) | ((uint64_t)(btype & 0xFF) << 32) | mv;
+}
+
 // Every lane's device-scope stores have completed once this returns.
 __device__ __forceinline__ void stores_done()
 {
@@ -1086,29 +1103,22 @@ __device__ __forceinline__ void stores_done()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// compute_mv_predictor (motion.cpp:271-346) from published records.  The
-// five candidate records are fetched together (one memory latency), then
-// combined in the reference's order.
-__device__ void mv_predictor(const DevJob &job, int bx, int by, int &pmx, int &pmy)
+// compute_mv_predictor (motion.cpp:271-346) from the predictor words of
+// the left, top, top-right and top-left MBs (0 where absent) and the
+// temporal table's record, combined in the reference's order.
+__device__ void mv_predictor(const DevJob &job, int bx, int by, uint64_t wl, uint64_t wt,
+                             uint64_t wtr, uint64_t wtl, int &pmx, int &pmy)
 {
     const int idx = by * job.wb + bx;
-    const uint32_t *rec = reinterpret_cast<const uint32_t *>(job.records);
     const bool has_l = bx > 0, has_t = by > 0, has_tr = by > 0 && bx + 1 < job.wb;
     const bool has_tl = by > 0 && bx > 0, has_p = job.prev != nullptr;
-    const int il = has_l ? idx - 1 : idx, it = has_t ? idx - job.wb : idx;
-    const int itr = has_tr ? idx - job.wb + 1 : idx, itl = has_tl ? idx - job.wb - 1 : idx;
-    // Two words per record (type, MV); out-of-range neighbours read this MB's
-    // own (not yet written) slot and are ignored.
-    const uint32_t l0 = ld_relaxed(rec + il * 16), l1 = ld_relaxed(rec + il * 16 + 1);
-    const uint32_t t0 = ld_relaxed(rec + it * 16), t1 = ld_relaxed(rec + it * 16 + 1);
-    const uint32_t r0 = ld_relaxed(rec + itr * 16), r1 = ld_relaxed(rec + itr * 16 + 1);
-    const uint32_t q0 = ld_relaxed(rec + itl * 16), q1 = ld_relaxed(rec + itl * 16 + 1);
     uint32_t p0 = 0, p1 = 0;
     if (has_p) {
         const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + idx * 16;
         p0 = prv[0];
         p1 = prv[1];
     }
+    auto motion = [](uint64_t w) { return ((uint32_t)(w >> 32) & EVXA_BLOCK_MOTION) != 0; };
     int mx[3] = {0, 0, 0}, my[3] = {0, 0, 0}, n = 0;
     auto take = [&](uint32_t w1) {
         const int vx = (int16_t)(w1 & 0xFFFF), vy = (int16_t)(w1 >> 16);
@@ -1117,10 +1127,10 @@ __device__ void mv_predictor(const DevJob &job, int bx, int by, int &pmx, int &p
         else { mx[2] = vx; my[2] = vy; }
         n++;
     };
-    if (has_l && (l0 & EVXA_BLOCK_MOTION)) take(l1);
-    if (has_t && (t0 & EVXA_BLOCK_MOTION)) take(t1);
-    if (has_tr && (r0 & EVXA_BLOCK_MOTION)) take(r1);
-    else if (has_tl && (q0 & EVXA_BLOCK_MOTION)) take(q1);
+    if (has_l && motion(wl)) take((uint32_t)wl);
+    if (has_t && motion(wt)) take((uint32_t)wt);
+    if (has_tr && motion(wtr)) take((uint32_t)wtr);
+    else if (has_tl && motion(wtl)) take((uint32_t)wtl);
     if (n < 3 && has_p && (p0 & EVXA_BLOCK_MOTION)) take(p1);
     if (n == 0) { pmx = 0; pmy = 0; }
     else if (n == 1) { pmx = mx[0]; pmy = my[0]; }
@@ -1137,7 +1147,6 @@ __device__ void mv_predictor(const DevJob &job, int bx, int by, int &pmx, int &p
     }
 }
 
-// Lane k < n stores word k; `coherent` makes the stores device-scope (read
 // by other waves inside the launch).
 __device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, int n,
                                             bool coherent = false)
@@ -1281,12 +1290,11 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     w[14] = 0;
     w[15] = 0;
     uint32_t *rec = reinterpret_cast<uint32_t *>(job.records + idx);
-    write_words(rec, w, 16, true);
+    write_words(rec, w, 16);
 
-    // Publish: the record's device-scope stores complete in every lane
-    // before lane 0 moves the row counter past it.
-    stores_done();
-    if (lane == 0) st_relaxed(reinterpret_cast<uint32_t *>(job.progress + by * kProgressStride), bx + 1);
+    // Publish what the MV predictor of later MBs reads (block type and MV)
+    // as one device-scope word: they poll it directly.
+    if (lane == 0) st_relaxed64(job.mvw + idx, pred_word(btype, w[1]));
 
     int16_t *co = job.coeffs + (size_t)idx * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)idx * EVXA_MB_SAMPLES;
@@ -1303,9 +1311,9 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     uint32_t w2[2];
     w2[0] = w[4] | ((uint32_t)qp << 16);
     w2[1] = (uint32_t)(uint16_t)(int16_t)var2;
-    write_words(rec + 4, w2, 2, true);
+    write_words(rec + 4, w2, 2);
     w2[0] = 1u | (bidir ? 2u : 0u);
-    write_words(rec + 14, w2, 1, true);
+    write_words(rec + 14, w2, 1);
     __syncthreads();
 }
 
@@ -1347,34 +1355,34 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         const WinPos win = stage_window(rp, x, y, kReachMax, sh.sp);
         const bool narrow = (*job.src.wide | *rp.wide) == 0;
 
-        // Wait for the left neighbour (same row) and the top-right one (row
-        // above; the whole row above at the right edge), backing off.
+        // Wait until the predictor words of the left, top, top-right and
+        // top-left MBs are published (the row above up to its end at the
+        // right edge), polling with a short back-off.  Lanes 0..3 load one
+        // word each; absent neighbours need nothing.
+        const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
+        const bool has_tl = has_t && has_l;
+        const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
+                       : lane == 1 ? (has_t ? mb - job.wb : -1)
+                       : lane == 2 ? (has_tr ? mb - job.wb + 1 : -1)
+                       : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1) : -1;
+        uint64_t word = 0;
         int spins = 0, nap = 1;
         bool timed_out = false;
-        // Polling uses relaxed loads (served by L2 without invalidating this
-        // CU's L1 under the waves that are computing); one acquire fence
-        // after the condition holds orders the neighbours' records.
-        auto wait_for = [&](const int32_t *p, int need) {
-            for (;;) {
-                // One lane polls (a one-lane load costs the memory pipeline
-                // far less than a full-wave one); the value is broadcast.
-                int seen = 0;
-                if (lane == 0) seen = (int)ld_relaxed(reinterpret_cast<const uint32_t *>(p));
-                if (__builtin_amdgcn_readfirstlane(seen) >= need) return;
-                if (++spins > kSpinLimit) { timed_out = true; return; }
-                for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(2);
-                nap = min(nap * 2, kMaxNap);
-            }
-        };
-        if (bx > 0) wait_for(job.progress + by * kProgressStride, bx);
-        if (by > 0 && !timed_out)
-            wait_for(job.progress + (by - 1) * kProgressStride, min(bx + 2, job.wb));
-        // The neighbours' records are read device-scope after the wait.
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        for (;;) {
+            if (nidx >= 0) word = ld_relaxed64(job.mvw + nidx);
+            const uint64_t missing = __ballot(nidx >= 0 && (word >> 63) == 0);
+            if (missing == 0) break;
+            if (++spins > kSpinLimit) { timed_out = true; break; }
+            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+            nap = min(nap * 2, kMaxNap);
+        }
         if (timed_out && lane == 0) atomicOr(&counters[kCounterError], kErrWaitTimeout);
-
+        auto lane_word = [&](int l) {
+            return ((uint64_t)(uint32_t)lane_value((int)(word >> 32), l) << 32) |
+                   (uint32_t)lane_value((int)(uint32_t)word, l);
+        };
         int pmx, pmy;
-        mv_predictor(job, bx, by, pmx, pmy);
+        mv_predictor(job, bx, by, lane_word(0), lane_word(1), lane_word(2), lane_word(3), pmx, pmy);
         pmx = __builtin_amdgcn_readfirstlane(pmx);
         pmy = __builtin_amdgcn_readfirstlane(pmy);
         window_ready();
