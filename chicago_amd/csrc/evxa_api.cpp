@@ -165,7 +165,8 @@ struct LaunchRes {
     std::vector<uint64_t> key;       // composition of the cached work list
     int32_t n_tickets = 0;
     int32_t n_progress = 0;
-    int32_t grid = 0;                // waves to launch for this composition
+    int32_t grid = 0;                // workgroups to launch for this composition
+    int32_t waves = 1;               // waves per workgroup
     ~LaunchRes()
     {
         if (staged) (void)hipEventDestroy(staged);
@@ -183,9 +184,11 @@ struct evxa_context {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
-    int32_t grid = 0;                // most analysis waves per launch
+    int32_t resident[3] = {0, 0, 0}; // workgroups of 1- and 2-wave analysis kernels that fit
+    int32_t waves = 1;               // waves per search (1 or 2)
+    int32_t grid = 0;                // most analysis workgroups per launch (0: all that fit)
     int32_t lists = 0;               // work lists per launch (0: automatic)
-    int32_t cap_pct = 60;            // tickets per list and virtual diagonal, % of its waves
+    int32_t cap_pct = 60;            // tickets per list and virtual diagonal, % of its workgroups
     std::mutex mu;
     LaunchRes res;
     // Ring of (start, stop) event pairs around analysis kernels; launches
@@ -422,7 +425,9 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     if (!st) st = ctx->stream;
-    const int32_t max_grid = grid_cap > 0 ? std::min(grid_cap, ctx->grid) : ctx->grid;
+    const int32_t fit = ctx->resident[ctx->waves];
+    const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
+    const int32_t max_grid = grid_cap > 0 ? std::min(grid_cap, all) : all;
     if (jobs.empty()) return;
     if ((int)jobs.size() > kMaxJobs) fail(EVXA_ERROR_CAPACITY_LIMIT, "too many jobs in one launch");
     std::vector<uint64_t> key;
@@ -432,6 +437,9 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
         key.push_back(((uint64_t)j.wb << 40) | ((uint64_t)j.hb << 8) | (uint32_t)j.n_refs);
         n_progress += j.wb * j.hb * 2;          // one 64-bit predictor word per MB
     }
+    key.push_back(((uint64_t)max_grid << 32) | ((uint64_t)ctx->waves << 24) |
+                  ((uint64_t)ctx->lists << 16) | ((uint64_t)(stagger ? 1 : 0) << 15) |
+                  (uint64_t)(ctx->cap_pct & 0x7FFF));
     if (!r.staged) check(hipEventCreateWithFlags(&r.staged, hipEventDisableTiming), "hipEventCreate");
     // The previous launch's host->device copies must have consumed the
     // staging buffers before they are rewritten.
@@ -451,6 +459,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
         // multiple of n_lists so that every list has the same number of waves.
         const int32_t per_list = std::max(8, widest + widest / 2 + 4);
         r.grid = n_lists * std::max(1, std::min(max_grid / n_lists, per_list));
+        r.waves = ctx->waves;
         r.h_tickets.ensure(t.size() * sizeof(uint32_t));
         memcpy(r.h_tickets.p, t.data(), t.size() * sizeof(uint32_t));
         if (r.tickets.n < t.size() * sizeof(uint32_t)) r.tickets.alloc(t.size() * sizeof(uint32_t));
@@ -483,7 +492,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
         ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
     }
     if (ev.first) check(hipEventRecord(ev.first, st), "hipEventRecord");
-    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), prog, grid, st),
+    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), prog, grid, r.waves, st),
           "analysis kernel");
     if (ev.second) check(hipEventRecord(ev.second, st), "hipEventRecord");
     if (time_it) {
@@ -546,7 +555,12 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
             hbm_budget_bytes = (uint64_t)(free_b * 0.9);
         }
         ctx->budget = hbm_budget_bytes;
-        ctx->grid = analyze_resident_waves(device_ordinal);
+        ctx->resident[1] = analyze_resident_groups(device_ordinal, 1);
+        ctx->resident[2] = analyze_resident_groups(device_ordinal, 2);
+        if (const char *g = getenv("EVXA_WAVES")) {
+            const int v = atoi(g);
+            if (v == 1 || v == 2) ctx->waves = v;
+        }
         if (const char *g = getenv("EVXA_ANALYZE_GRID")) {
             const int v = atoi(g);
             if (v > 0) ctx->grid = v;
@@ -823,7 +837,7 @@ int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams, const evxa
         check(hipEventCreate(&res->g_stop), "hipEventCreate");
         hipError_t e2 = hipEventRecord(res->g_start, ctx->stream);
         hipError_t e3 = launch_analyze(res->jobs.as<DevJob>(), res->tickets.as<uint32_t>(),
-                                       res->progress.as<int32_t>(), grid, ctx->stream);
+                                       res->progress.as<int32_t>(), grid, res->waves, ctx->stream);
         hipError_t e4 = hipEventRecord(res->g_stop, ctx->stream);
         hipError_t e5 = hipStreamEndCapture(ctx->stream, &graph);
         check(e1, "capture reset");
@@ -1038,7 +1052,9 @@ void server_launch(evxa_server *srv, evxa_server::Slot &sl, std::vector<evxa_ser
     std::vector<DevJob> jobs;
     jobs.reserve(reqs.size());
     for (auto &r : reqs) jobs.push_back(make_job(r.s, r.d, true));
-    const int32_t cap = std::max(1, ctx->grid / srv->max_inflight);
+    const int32_t fit = ctx->resident[ctx->waves];
+    const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
+    const int32_t cap = std::max(1, all / srv->max_inflight);
     enqueue(ctx, sl.res, jobs, false, sl.st, cap, false, sl.t0, nullptr);
     check(hipMemcpyAsync(sl.flags.p, sl.res.progress.p, (kCounterError + 1) * sizeof(int32_t),
                          hipMemcpyDeviceToHost, sl.st),
@@ -1143,6 +1159,34 @@ void server_run(evxa_server *srv)
 extern "C" {
 
 double evxa_clock(void) { return clock_s(); }
+
+int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t)
+{
+    return guarded([&] {
+        if (!ctx || !t) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        if (t->waves_per_search < 0 || t->waves_per_search > 2 || t->max_groups < 0 ||
+            t->lists < 0 || t->lists > kLists || t->cap_pct < 0 || t->cap_pct > 10000)
+            fail(EVXA_ERROR_INVALIDARG, "tuning out of range");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (t->waves_per_search) ctx->waves = t->waves_per_search;
+        ctx->grid = t->max_groups;
+        ctx->lists = t->lists;
+        ctx->cap_pct = t->cap_pct ? t->cap_pct : 60;
+    });
+}
+
+int evxa_get_tuning(evxa_context *ctx, evxa_tuning *t, int32_t *resident_groups)
+{
+    return guarded([&] {
+        if (!ctx || !t) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        t->waves_per_search = ctx->waves;
+        t->max_groups = ctx->grid;
+        t->lists = ctx->lists;
+        t->cap_pct = ctx->cap_pct;
+        if (resident_groups) *resident_groups = ctx->resident[ctx->waves];
+    });
+}
 
 int evxa_server_create(evxa_context *ctx, int32_t max_inflight, int32_t max_frames,
                        evxa_server **out)

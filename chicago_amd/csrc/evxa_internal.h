@@ -87,8 +87,9 @@ constexpr int32_t kErrWaitTimeout = 2;
 
 // Kernel launchers (evxa_kernels.hip).
 hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st);
+// `waves` (1 or 2) per workgroup; `grid` workgroups.
 hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
-                          int32_t grid, hipStream_t st);
+                          int32_t grid, int32_t waves, hipStream_t st);
 hipError_t launch_apron(int16_t *plane0, int32_t pitch, int32_t w, int32_t h, hipStream_t st);
 // ORs 1 into *wide if a sample of the w x h plane lies outside [0, 2047].
 hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,
@@ -101,7 +102,8 @@ hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, in
 hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t *pred,
                                    const int32_t *quality, int16_t *coeffs, int16_t *levels,
                                    int32_t *qp, int32_t *variance, hipStream_t st);
-// Waves of the analysis kernel that fit the device at once.
-int32_t analyze_resident_waves(int32_t device);
+// Workgroups of the analysis kernel with `waves` waves each that fit the
+// device at once.
+int32_t analyze_resident_groups(int32_t device, int32_t waves);
 
 }  // namespace evxa

@@ -62,6 +62,18 @@ __constant__ uint8_t c_deadzone[64] = {
     4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+
+// Orders one wave's LDS accesses across its lanes: a wave's LDS
+// instructions execute in order, so completing them suffices.  Used by the
+// per-MB stages that a single wave runs (selection, prediction, DCT).
+__device__ __forceinline__ void wave_sync()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);     // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 template <int CTRL>
 __device__ __forceinline__ int dpp(int v)
@@ -345,11 +357,21 @@ struct Window {
     int16_t v[kWinCR * kWinCS];
 };
 
+// Per-slot results exchanged between the waves of a two-wave search:
+// integer levels (double-buffered by level parity) and sub-pel directions.
+struct Exchange {
+    uint32_t cost[2][8];
+    int32_t sad[2][8];
+    int32_t mad[2][8];
+    int32_t sub[8][4];        // half SAD, half MAD, quarter SAD, quarter MAD
+};
+
 struct SubpelLds {
     Window win;
     HalfPlanes<16> y;
     HalfPlanes<8> u;
     HalfPlanes<8> v;
+    Exchange xch;
 };
 
 // Where the staged window sits in picture coordinates (wave-uniform): luma
@@ -394,14 +416,22 @@ typedef __attribute__((address_space(3))) void lds_void;
 // into LDS packed with stride row_dw, n_rows rows, straight from memory to
 // LDS (global_load_lds_dword: lane i of the k-th load writes dword 64k + i).
 // All loads are issued back to back; they complete under vmcnt(0).
-template <int ROW_DW>
+// Copies rows of `row_dw` dwords (row r of the source at g + r * pitch_dw)
+// into LDS packed with stride row_dw, n_rows rows, straight from memory to
+// LDS (global_load_lds_dword: lane i of the k-th load writes dword 64k + i).
+// The W waves of the workgroup take every W-th load.  All loads are issued
+// back to back; they complete under vmcnt(0).
+template <int ROW_DW, int W>
 __device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_rows, uint32_t *lds)
 {
     const int lane = lane_id();
+    const int w0 = W > 1 ? wave_id() : 0;
     const int n = ROW_DW * n_rows;
-    int r = lane / ROW_DW, c = lane - r * ROW_DW;            // of dword 64k + lane
-    constexpr int kStepR = 64 / ROW_DW, kStepC = 64 - kStepR * ROW_DW;
-    for (int k = 0; k * 64 < n; k++) {
+    const int first = 64 * w0 + lane;
+    int r = first / ROW_DW, c = first - (first / ROW_DW) * ROW_DW;   // of dword 64k + lane
+    constexpr int kStep = 64 * W;
+    constexpr int kStepR = kStep / ROW_DW, kStepC = kStep - kStepR * ROW_DW;
+    for (int k = w0; k * 64 < n; k += W) {
         if (k * 64 + lane < n)
             __builtin_amdgcn_global_load_lds((global_void *)(g + r * pitch_dw + c),
                                              (lds_void *)(lds + k * 64), 4, 0, 0);
@@ -415,6 +445,7 @@ __device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_
 // (call window_ready before reading it).  Luma rows are kWinLS samples and
 // chroma rows kWinCS, always; the column origins are even and clamped so that
 // a row never runs past the right apron of its own picture row.
+template <int W>
 __device__ WinPos stage_window(const DevPic &ref, int bx, int by, int reach, SubpelLds &sp)
 {
     WinPos w;
@@ -424,18 +455,19 @@ __device__ WinPos stage_window(const DevPic &ref, int bx, int by, int reach, Sub
     w.cx0 = min((((bx - reach) >> 1) - 3) & ~1, ref.pc - kApron - kWinCS);
     w.cy0 = ((by - reach) >> 1) - 3;
     w.cy1 = ((by + reach) >> 1) + 11;
-    dma_rows<kWinLS / 2>(reinterpret_cast<const uint32_t *>(ref.y + w.y0 * ref.py + w.x0),
-                         ref.py / 2, w.y1 - w.y0, reinterpret_cast<uint32_t *>(sp.win.y));
-    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.u + w.cy0 * ref.pc + w.cx0),
-                         ref.pc / 2, w.cy1 - w.cy0, reinterpret_cast<uint32_t *>(sp.win.u));
-    dma_rows<kWinCS / 2>(reinterpret_cast<const uint32_t *>(ref.v + w.cy0 * ref.pc + w.cx0),
-                         ref.pc / 2, w.cy1 - w.cy0, reinterpret_cast<uint32_t *>(sp.win.v));
+    dma_rows<kWinLS / 2, W>(reinterpret_cast<const uint32_t *>(ref.y + w.y0 * ref.py + w.x0),
+                            ref.py / 2, w.y1 - w.y0, reinterpret_cast<uint32_t *>(sp.win.y));
+    dma_rows<kWinCS / 2, W>(reinterpret_cast<const uint32_t *>(ref.u + w.cy0 * ref.pc + w.cx0),
+                            ref.pc / 2, w.cy1 - w.cy0, reinterpret_cast<uint32_t *>(sp.win.u));
+    dma_rows<kWinCS / 2, W>(reinterpret_cast<const uint32_t *>(ref.v + w.cy0 * ref.pc + w.cx0),
+                            ref.pc / 2, w.cy1 - w.cy0, reinterpret_cast<uint32_t *>(sp.win.v));
     return w;
 }
 
+// Every wave's LDS copies have landed and are visible to the workgroup.
 __device__ __forceinline__ void window_ready()
 {
-    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0): the LDS copies have landed
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
     __syncthreads();
 }
 
@@ -451,25 +483,27 @@ __device__ __forceinline__ void load_cand_win(const SubpelLds &sp, const WinPos 
 
 // Six-tap outputs of an R x C plane into dst (row-major, row stride C):
 // output (r, c) filters src[r * sp + c + t * ts], t = 0..5 (interp.cpp:18-41).
-// Lane l computes outputs l, l + 64, ...; positions advance by constants
-// (no division in the loop), and each chunk of kChunk outputs issues all of
-// its LDS loads before any arithmetic, so their latencies overlap.
-template <int R, int C>
+// Thread t of the W waves computes outputs t, t + 64W, ...; positions
+// advance by constants (no division in the loop), and each chunk of kChunk
+// outputs issues all of its LDS loads before any arithmetic, so their
+// latencies overlap.
+template <int R, int C, int W>
 __device__ __forceinline__ void filter_plane(const int16_t *src, int sp, int ts, int16_t *dst)
 {
     constexpr int N = R * C;
-    constexpr int K = (N + 63) / 64;
+    constexpr int NT = 64 * W;
+    constexpr int K = (N + NT - 1) / NT;
     constexpr int kChunk = 4;
-    const int lane = lane_id();
-    const int r0 = lane / C, c0 = lane - (lane / C) * C;
+    const int tid = W > 1 ? (int)threadIdx.x : lane_id();
+    const int r0 = tid / C, c0 = tid - (tid / C) * C;
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += kChunk) {
         int v[kChunk][6];
 #pragma unroll
         for (int j = 0; j < kChunk; j++) {
             const int k = k0 + j;
-            if (k < K && (k < K - 1 || lane + 64 * k < N)) {
-                int c = c0 + (64 * k) % C, r = r0 + (64 * k) / C;
+            if (k < K && (k < K - 1 || tid + NT * k < N)) {
+                int c = c0 + (NT * k) % C, r = r0 + (NT * k) / C;
                 if (c >= C) { c -= C; r++; }
                 const int16_t *q = src + r * sp + c;
 #pragma unroll
@@ -479,10 +513,10 @@ __device__ __forceinline__ void filter_plane(const int16_t *src, int sp, int ts,
 #pragma unroll
         for (int j = 0; j < kChunk; j++) {
             const int k = k0 + j;
-            if (k < K && (k < K - 1 || lane + 64 * k < N)) {
+            if (k < K && (k < K - 1 || tid + NT * k < N)) {
                 const int sum = v[j][0] - 5 * v[j][1] + 20 * v[j][2] + 20 * v[j][3] -
                                 5 * v[j][4] + v[j][5];
-                dst[lane + 64 * k] = (int16_t)((sum + 16) >> 5);
+                dst[tid + NT * k] = (int16_t)((sum + 16) >> 5);
             }
         }
     }
@@ -492,21 +526,21 @@ __device__ __forceinline__ void filter_plane(const int16_t *src, int sp, int ts,
 // the block base (bx, by) and has row pitch `pitch`.  h row r, column c is
 // the horizontal half between columns c - 1 and c of row r - 3; v row r is
 // the vertical half between rows r - 1 and r.
-template <int B>
+template <int B, int W>
 __device__ __forceinline__ void build_hv(const int16_t *plane, int pitch, HalfPlanes<B> &hp)
 {
     using P = HalfPlanes<B>;
-    filter_plane<P::HR, P::HC>(plane - 3 * pitch - 3, pitch, 1, hp.h);
-    filter_plane<P::VR, P::VC>(plane - 3 * pitch, pitch, pitch, hp.v);
+    filter_plane<P::HR, P::HC, W>(plane - 3 * pitch - 3, pitch, 1, hp.h);
+    filter_plane<P::VR, P::VC, W>(plane - 3 * pitch, pitch, pitch, hp.v);
 }
 
 // Second pass: the diagonal plane, a vertical six-tap over the h plane
 // (plane row -1 starts at h row 0, i.e. picture row -3).
-template <int B>
+template <int B, int W>
 __device__ __forceinline__ void build_d(HalfPlanes<B> &hp)
 {
     using P = HalfPlanes<B>;
-    filter_plane<P::DR, P::DC>(hp.h, P::HC, P::HC, hp.d);
+    filter_plane<P::DR, P::DC, W>(hp.h, P::HC, P::HC, hp.d);
 }
 
 // Where direction (dx, dy)'s half-pel block sits inside the planes.
@@ -615,30 +649,103 @@ __device__ __forceinline__ void subpel_metrics(bool narrow, const Blk &s, const 
     }
 }
 
+// The closed-form fold of one level (motion.cpp:388-405 over
+// evaluate_motion_candidate), given lane k (k < 8) holding scan slot k's
+// validity, cost bits, SAD and MAD.  Order: the level's starting best b0,
+// slots 0..3, the centre (b0 again, with its true distance), slots 4..7.
+//  * If a valid slot has MAD < T, the first such slot is accepted
+//    unconditionally and the level ends there (copy state).
+//  * Otherwise the best ends at the minimum cost m: the first element (in
+//    order) at cost m, replaced by any later element at cost m with a
+//    strictly smaller ssd and SAD < 8192.
+struct Best {
+    int x, y, sad, mad, ssd;
+    uint32_t cost;
+};
+
+__device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int step, int x, int y,
+                                           int thresh, uint32_t mask, bool valid, uint32_t cost,
+                                           int sad, int mad, int &n_int)
+{
+    auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
+    const uint32_t copym = (uint32_t)__ballot(valid && mad < thresh) & 0xFFu;
+    if (copym) {
+        const int p = __builtin_ctz(copym);
+        n_int += __builtin_popcount(mask & ((2u << p) - 1u));
+        b.sad = lane_value(sad, p);
+        b.mad = lane_value(mad, p);
+        b.cost = (uint32_t)lane_value((int)cost, p);
+        b.x = base_x + slot_i(p) * step;
+        b.y = base_y + slot_j(p) * step;
+        b.ssd = dist(b.x, b.y);
+        return true;                                   // motion.cpp:402-403
+    }
+    n_int += __builtin_popcount(mask);
+    // Minimum over the slots in lanes 0..15 (lanes 8..15 hold no slot).
+    uint32_t ml = valid ? cost : 0xFFFFFFFFu;
+    ml = min(ml, (uint32_t)dpp<0xB1>((int)ml));
+    ml = min(ml, (uint32_t)dpp<0x4E>((int)ml));
+    ml = min(ml, (uint32_t)dpp<0x141>((int)ml));
+    ml = min(ml, (uint32_t)dpp<0x140>((int)ml));
+    const uint32_t m = min(b.cost, (uint32_t)lane_value((int)ml, 0));
+    const uint32_t eq = (uint32_t)__ballot(valid && cost == m) & 0xFFu;
+    bool have = b.cost == m;
+    // Visit order as bits: slots 0..3, the centre (bit 4), slots 4..7.
+    uint32_t seq = (eq & 0xFu) | (have ? 0x10u : 0u) | ((eq & 0xF0u) << 1);
+    Best c = b;
+    while (seq) {
+        const int q = __builtin_ctz(seq);
+        seq &= seq - 1;
+        int ex = base_x, ey = base_y, esad = b.sad, emad = b.mad;
+        if (q != 4) {
+            const int k = q < 4 ? q : q - 1;
+            ex = base_x + slot_i(k) * step;
+            ey = base_y + slot_j(k) * step;
+            esad = lane_value(sad, k);
+            emad = lane_value(mad, k);
+        }
+        const int essd = dist(ex, ey);
+        if (!have || (essd < c.ssd && esad < kSadThreshold)) {
+            c.x = ex; c.y = ey; c.sad = esad; c.mad = emad; c.ssd = essd;
+        }
+        have = true;
+    }
+    c.cost = m;
+    b = c;
+    return false;
+}
+
 // perform_inter_search (motion.cpp:567-679) for the MB at (x, y) against
-// `ref`, predictor (pmx, pmy) as compute_mv_predictor produced it.  All
-// lanes call it with uniform arguments; it returns uniform values.
-// `s` is the source MB and `pre` the reference block at (x, y), both in the
-// lane layout; `w` is the window already staged in `sp` (any position).
+// `ref` (a pw x ph picture), predictor (pmx, pmy) as compute_mv_predictor
+// produced it.  All lanes of the NW waves call it with uniform arguments; it
+// returns uniform values.  `s` is the source MB and `pre` the reference block
+// at (x, y), both in the lane layout; `w` is the window already staged in
+// `sp` (any position).
 //
-// Lane group g (lanes 16g .. 16g+15) evaluates scan slots g and g+4 of every
-// level and sub-pel round; each lane owns one 4x4 luma block and 8 chroma
-// samples of the candidate.  Metrics and costs are computed for all slots in
-// parallel; the ordered fold then runs on wave-uniform values.
-__device__ SearchOut inter_search(const DevPic &ref, int W, int H, bool narrow, const Blk &s,
+// Each lane group (16 lanes) evaluates one candidate position at a time,
+// one lane per 4x4 luma block plus 8 chroma samples.  With one wave, group
+// g takes scan slots g and g + 4 of every level and sub-pel round; with two
+// waves, group g of wave h takes slot 4h + g and the waves exchange their
+// slots' results through LDS.  The ordered fold then runs on wave-uniform
+// values, identically in every wave.
+template <int NW>
+__device__ SearchOut inter_search(const DevPic &ref, int pw, int ph, bool narrow, const Blk &s,
                                   const Geo &geo, const Blk &pre, WinPos w, int x, int y,
                                   int thresh, float lambda, int pmx, int pmy, SubpelLds &sp)
 {
-    const int g = lane_id() >> 4;
-    const int ia = slot_i(g), ja = slot_j(g), ib = slot_i(g + 4), jb = slot_j(g + 4);
-    const int max_x = W - 16, max_y = H - 16;
-    int best_x = x, best_y = y, best_sad, best_mad, best_ssd = kInt32Max;
-    uint32_t best_cost;
+    const int lane = lane_id();
+    const int g = lane >> 4;
+    const int sa = NW == 1 ? g : 4 * wave_id() + g;       // this group's (first) slot
+    const int ia = slot_i(sa), ja = slot_j(sa), ib = slot_i(g + 4), jb = slot_j(g + 4);
+    const int max_x = pw - 16, max_y = ph - 16;
+    Best b;
+    b.x = x; b.y = y; b.ssd = kInt32Max;
     int sp_en = 0, sp_amt = 0, sp_idx = 0, n_int = 1, n_ax = 0, n_dg = 0;
     auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
 
     // Initial candidate (motion.cpp:600-611) in group 0 and the predictor
-    // candidate (motion.cpp:621-634, clamped into the picture) in group 1.
+    // candidate (motion.cpp:621-634, clamped into the picture) in group 1
+    // (every wave evaluates both).
     const bool has_pred = pmx != 0 || pmy != 0;
     const int px = min(max(x + pmx, 0), max_x), py = min(max(y + pmy, 0), max_y);
     int sad, mad, satd;
@@ -653,9 +760,9 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, bool narrow, 
     }
     const uint32_t cst0 = g == 1 ? cost_bits(sad, satd, px - x, py - y, lambda)
                                  : cost_bits(sad, satd, 0, 0, lambda);
-    best_sad = lane_value(sad, 0);
-    best_mad = lane_value(mad, 0);
-    best_cost = (uint32_t)lane_value((int)cst0, 0);
+    b.sad = lane_value(sad, 0);
+    b.mad = lane_value(mad, 0);
+    b.cost = (uint32_t)lane_value((int)cst0, 0);
 
     if (has_pred) {
         // evaluate_motion_candidate's acceptance (motion.cpp:125-178), C
@@ -664,31 +771,32 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, bool narrow, 
         const uint32_t ccost = (uint32_t)lane_value((int)cst0, 16);
         const int ssd = dist(px, py);
         bool accept;
-        if (best_mad < thresh)
-            accept = cmad < best_mad || (cmad == best_mad && ssd < best_ssd);
+        if (b.mad < thresh)
+            accept = cmad < b.mad || (cmad == b.mad && ssd < b.ssd);
         else
-            accept = (ccost < best_cost ||
-                      ((ccost == best_cost && ssd < best_ssd) && csad < kSadThreshold)) ||
+            accept = (ccost < b.cost ||
+                      ((ccost == b.cost && ssd < b.ssd) && csad < kSadThreshold)) ||
                      cmad < thresh;
         if (accept) {
-            best_x = px; best_y = py;
-            best_sad = csad; best_ssd = ssd; best_mad = cmad; best_cost = ccost;
+            b.x = px; b.y = py;
+            b.sad = csad; b.ssd = ssd; b.mad = cmad; b.cost = ccost;
         }
         n_int++;
     }
 
-    if (best_mad >= thresh) {
+    if (b.mad >= thresh) {
         int radius = kSearchRadius;
-        if (best_sad < kSadThreshold / 4) radius = kSearchRadius / 4;
-        else if (best_sad < kSadThreshold / 2) radius = kSearchRadius / 2;
+        if (b.sad < kSadThreshold / 4) radius = kSearchRadius / 4;
+        else if (b.sad < kSadThreshold / 2) radius = kSearchRadius / 2;
 
         bool copy = false;
+        int parity = 0;
         for (int step = radius; step > 0 && !copy; step >>= 1) {
             // perform_inter_motion_search (motion.cpp:381-406).
-            const int base_x = best_x, base_y = best_y;
+            const int base_x = b.x, base_y = b.y;
             bool in_win = win_holds_level(w, base_x, base_y, step, max_x, max_y);
             if (!in_win && step <= 8) {
-                w = stage_window(ref, base_x, base_y, 2 * step - 1, sp);
+                w = stage_window<NW>(ref, base_x, base_y, 2 * step - 1, sp);
                 window_ready();
                 in_win = true;
             }
@@ -698,82 +806,61 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, bool narrow, 
                 const int cx = base_x + slot_i(k) * step, cy = base_y + slot_j(k) * step;
                 if (cx >= 0 && cx <= max_x && cy >= 0 && cy <= max_y) mask |= 1u << k;
             }
-            // This lane's two slots, clamped into the picture (an out-of-range
+            // This group's slot(s), clamped into the picture (an out-of-range
             // slot is evaluated at a safe position and never folded).
             const int ax = min(max(base_x + ia * step, 0), max_x);
             const int ay = min(max(base_y + ja * step, 0), max_y);
-            const int bxx = min(max(base_x + ib * step, 0), max_x);
-            const int byy = min(max(base_y + jb * step, 0), max_y);
-            Blk ra, rb;
-            const bool two = (mask & 0xF0u) != 0;
+            Blk ra;
             if (in_win) load_cand_win(sp, w, geo, ax, ay, ra);
             else load_cand(ref, geo, ax, ay, ra);
-            if (two) {
-                if (in_win) load_cand_win(sp, w, geo, bxx, byy, rb);
-                else load_cand(ref, geo, bxx, byy, rb);
-            }
-            int sad_a, mad_a, satd_a, sad_b = 0, mad_b = 0, satd_b = 0;
-            cand_metrics(narrow, s, ra, sad_a, mad_a, satd_a);
-            const uint32_t cst_a = cost_bits(sad_a, satd_a, ax - x, ay - y, lambda);
-            uint32_t cst_b = 0;
-            if (two) {
-                cand_metrics(narrow, s, rb, sad_b, mad_b, satd_b);
-                cst_b = cost_bits(sad_b, satd_b, bxx - x, byy - y, lambda);
-            }
-            // The level's fold (motion.cpp:388-405 over evaluate_motion_candidate)
-            // in closed form.  Order: the level's starting best b0, slots
-            // 0..3, the centre (b0 again, with its true distance), slots 4..7.
-            //  * If a valid slot has MAD < T, the first such slot is accepted
-            //    unconditionally and the level ends there (copy state).
-            //  * Otherwise the best ends at the minimum cost m: the first
-            //    element (in order) at cost m, replaced by any later element
-            //    at cost m with a strictly smaller ssd and SAD < 8192.
-            const bool va = ((mask >> g) & 1u) != 0, vb = ((mask >> (g + 4)) & 1u) != 0;
-            const uint32_t copym = group_bits(__ballot(va && mad_a < thresh)) |
-                                   (group_bits(__ballot(vb && mad_b < thresh)) << 4);
-            if (copym) {
-                const int p = __builtin_ctz(copym);
-                n_int += __builtin_popcount(mask & ((2u << p) - 1u));
-                const int ln = (p & 3) * 16;
-                best_sad = lane_value(p < 4 ? sad_a : sad_b, ln);
-                best_mad = lane_value(p < 4 ? mad_a : mad_b, ln);
-                best_cost = (uint32_t)lane_value((int)(p < 4 ? cst_a : cst_b), ln);
-                best_x = base_x + slot_i(p) * step;
-                best_y = base_y + slot_j(p) * step;
-                best_ssd = dist(best_x, best_y);
-                copy = true;                                   // motion.cpp:402-403
-            } else {
-                n_int += __builtin_popcount(mask);
-                const uint32_t ml = min(va ? cst_a : 0xFFFFFFFFu, vb ? cst_b : 0xFFFFFFFFu);
-                const uint32_t m = min(min(best_cost, min((uint32_t)lane_value((int)ml, 0),
-                                                          (uint32_t)lane_value((int)ml, 16))),
-                                       min((uint32_t)lane_value((int)ml, 32),
-                                           (uint32_t)lane_value((int)ml, 48)));
-                const uint32_t eq = group_bits(__ballot(va && cst_a == m)) |
-                                    (group_bits(__ballot(vb && cst_b == m)) << 4);
-                bool have = best_cost == m;
-                // Visit order as bits: slots 0..3, the centre (bit 4), slots 4..7.
-                uint32_t seq = (eq & 0xFu) | (have ? 0x10u : 0u) | ((eq & 0xF0u) << 1);
-                int cx = best_x, cy = best_y, csad = best_sad, cmad = best_mad, cssd = best_ssd;
-                while (seq) {
-                    const int q = __builtin_ctz(seq);
-                    seq &= seq - 1;
-                    int ex = base_x, ey = base_y, esad = best_sad, emad = best_mad;
-                    if (q != 4) {
-                        const int k = q < 4 ? q : q - 1, ln = (k & 3) * 16;
-                        ex = base_x + slot_i(k) * step;
-                        ey = base_y + slot_j(k) * step;
-                        esad = lane_value(k < 4 ? sad_a : sad_b, ln);
-                        emad = lane_value(k < 4 ? mad_a : mad_b, ln);
-                    }
-                    const int essd = dist(ex, ey);
-                    if (!have || (essd < cssd && esad < kSadThreshold)) {
-                        cx = ex; cy = ey; csad = esad; cmad = emad; cssd = essd;
-                    }
-                    have = true;
+            int sad_a, mad_a, satd_a;
+            if constexpr (NW == 1) {
+                const int bxx = min(max(base_x + ib * step, 0), max_x);
+                const int byy = min(max(base_y + jb * step, 0), max_y);
+                Blk rb;
+                const bool two = (mask & 0xF0u) != 0;
+                if (two) {
+                    if (in_win) load_cand_win(sp, w, geo, bxx, byy, rb);
+                    else load_cand(ref, geo, bxx, byy, rb);
                 }
-                best_x = cx; best_y = cy; best_sad = csad; best_mad = cmad; best_ssd = cssd;
-                best_cost = m;
+                cand_metrics(narrow, s, ra, sad_a, mad_a, satd_a);
+                const uint32_t cst_a = cost_bits(sad_a, satd_a, ax - x, ay - y, lambda);
+                int sad_b = 0, mad_b = 0, satd_b = 0;
+                uint32_t cst_b = 0;
+                if (two) {
+                    cand_metrics(narrow, s, rb, sad_b, mad_b, satd_b);
+                    cst_b = cost_bits(sad_b, satd_b, bxx - x, byy - y, lambda);
+                }
+                // Slot k's values into lane k (k < 8): slot k sits in lane
+                // 16 * (k & 3), value a for k < 4 and b for k >= 4.
+                const int from = ((lane & 3) << 4) << 2;             // byte address of that lane
+                const bool hi = (lane & 4) != 0;
+                auto pull = [&](int va, int vb) {
+                    const int ra_ = __builtin_amdgcn_ds_bpermute(from, va);
+                    const int rb_ = __builtin_amdgcn_ds_bpermute(from, vb);
+                    return hi ? rb_ : ra_;
+                };
+                const uint32_t kc = (uint32_t)pull((int)cst_a, (int)cst_b);
+                const int ks = pull(sad_a, sad_b);
+                const int km = pull(mad_a, mad_b);
+                const bool kv = lane < 8 && ((mask >> (lane & 7)) & 1u) != 0;
+                copy = fold_level(b, base_x, base_y, step, x, y, thresh, mask, kv, kc, ks, km, n_int);
+            } else {
+                cand_metrics(narrow, s, ra, sad_a, mad_a, satd_a);
+                const uint32_t cst_a = cost_bits(sad_a, satd_a, ax - x, ay - y, lambda);
+                Exchange &xc = sp.xch;
+                if ((lane & 15) == 0) {
+                    xc.cost[parity][sa] = cst_a;
+                    xc.sad[parity][sa] = sad_a;
+                    xc.mad[parity][sa] = mad_a;
+                }
+                __syncthreads();
+                const int k = lane & 7;
+                const uint32_t kc = xc.cost[parity][k];
+                const int ks = xc.sad[parity][k], km = xc.mad[parity][k];
+                const bool kv = lane < 8 && ((mask >> k) & 1u) != 0;
+                copy = fold_level(b, base_x, base_y, step, x, y, thresh, mask, kv, kc, ks, km, n_int);
+                parity ^= 1;
             }
         }
 
@@ -782,79 +869,132 @@ __device__ SearchOut inter_search(const DevPic &ref, int W, int H, bool narrow, 
         uint32_t mask = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int tx = best_x + slot_i(k), ty = best_y + slot_j(k);
+            const int tx = b.x + slot_i(k), ty = b.y + slot_j(k);
             if (tx >= 0 && tx <= max_x && ty >= 0 && ty <= max_y) mask |= 1u << k;
         }
         if (mask) {
-            const int bx = best_x, by = best_y;
+            const int bx = b.x, by = b.y;
             if (!win_holds_subpel(w, bx, by)) {
-                w = stage_window(ref, bx, by, 0, sp);
+                w = stage_window<NW>(ref, bx, by, 0, sp);
                 window_ready();
             }
             const int wlx = bx - w.x0, wly = by - w.y0;
             const int wcx = (bx >> 1) - w.cx0, wcy = (by >> 1) - w.cy0;
-            build_hv<16>(sp.win.y + wly * kWinLS + wlx, kWinLS, sp.y);
-            build_hv<8>(sp.win.u + wcy * kWinCS + wcx, kWinCS, sp.u);
-            build_hv<8>(sp.win.v + wcy * kWinCS + wcx, kWinCS, sp.v);
+            build_hv<16, NW>(sp.win.y + wly * kWinLS + wlx, kWinLS, sp.y);
+            build_hv<8, NW>(sp.win.u + wcy * kWinCS + wcx, kWinCS, sp.u);
+            build_hv<8, NW>(sp.win.v + wcy * kWinCS + wcx, kWinCS, sp.v);
             Blk full;
             load_cand_win(sp, w, geo, bx, by, full);
             __syncthreads();
-            build_d<16>(sp.y);
-            build_d<8>(sp.u);
-            build_d<8>(sp.v);
+            build_d<16, NW>(sp.y);
+            build_d<8, NW>(sp.u);
+            build_d<8, NW>(sp.v);
             __syncthreads();
-            // Half and quarter metrics of this lane's two directions.
-            int m[8];
+            // Half and quarter SAD/MAD of candidate j = 2 * direction +
+            // (quarter) in lane j (j < 16).
+            int jsad, jmad;
+            const int j = lane & 15, jd = j >> 1, jq = j & 1;
+            if constexpr (NW == 1) {
+                int m[8];
 #pragma unroll
-            for (int slot = 0; slot < 2; slot++) {
-                const int di = slot ? ib : ia, dj = slot ? jb : ja;
+                for (int slot = 0; slot < 2; slot++) {
+                    const int di = slot ? ib : ia, dj = slot ? jb : ja;
+                    int sh, mh, sq, mq;
+                    subpel_metrics(narrow, s, full, plane_view<16>(sp.y, di, dj),
+                                   plane_view<8>(sp.u, di, dj), plane_view<8>(sp.v, di, dj), geo,
+                                   sh, mh, sq, mq);
+                    m[slot * 4 + 0] = row_sum(sh);
+                    m[slot * 4 + 1] = row_max(mh);
+                    m[slot * 4 + 2] = row_sum(sq);
+                    m[slot * 4 + 3] = row_max(mq);
+                }
+                // Direction d's values sit in lane 16 * (d & 3), slot d >> 2.
+                const int from = ((jd & 3) << 4) << 2;
+                int pulled[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) pulled[i] = __builtin_amdgcn_ds_bpermute(from, m[i]);
+                const int base = (jd >= 4 ? 4 : 0) + (jq ? 2 : 0);
+                jsad = base == 0 ? pulled[0] : base == 2 ? pulled[2] : base == 4 ? pulled[4] : pulled[6];
+                jmad = base == 0 ? pulled[1] : base == 2 ? pulled[3] : base == 4 ? pulled[5] : pulled[7];
+            } else {
                 int sh, mh, sq, mq;
-                subpel_metrics(narrow, s, full, plane_view<16>(sp.y, di, dj),
-                               plane_view<8>(sp.u, di, dj), plane_view<8>(sp.v, di, dj), geo, sh,
-                               mh, sq, mq);
-                m[slot * 4 + 0] = row_sum(sh);
-                m[slot * 4 + 1] = row_max(mh);
-                m[slot * 4 + 2] = row_sum(sq);
-                m[slot * 4 + 3] = row_max(mq);
+                subpel_metrics(narrow, s, full, plane_view<16>(sp.y, ia, ja),
+                               plane_view<8>(sp.u, ia, ja), plane_view<8>(sp.v, ia, ja), geo, sh, mh,
+                               sq, mq);
+                sh = row_sum(sh); mh = row_max(mh); sq = row_sum(sq); mq = row_max(mq);
+                if ((lane & 15) == 0) {
+                    sp.xch.sub[sa][0] = sh; sp.xch.sub[sa][1] = mh;
+                    sp.xch.sub[sa][2] = sq; sp.xch.sub[sa][3] = mq;
+                }
+                __syncthreads();
+                jsad = sp.xch.sub[jd][2 * jq];
+                jmad = sp.xch.sub[jd][2 * jq + 1];
             }
             // evaluate_subpel_motion_candidate's acceptance (motion.cpp:196-260)
-            // over the 16 candidates in order, branch-free; the copy mode can
-            // switch on mid-way and there is no early exit.
+            // over the 16 candidates in order (lane j holds candidate j), in
+            // closed form; there is no early exit.
+            //  * Copy state from the start (best MAD < T): a candidate is taken
+            //    iff its MAD is below the running best's, so the result is the
+            //    first candidate at the smallest MAD, if that is below the
+            //    starting MAD.
+            //  * Otherwise the first candidate j* with MAD < T is taken
+            //    unconditionally and switches to copy state; from j* on the
+            //    result is the first candidate at the smallest MAD.
+            //  * Without such a candidate, one is taken iff its SAD is below
+            //    the running best's and below 8192: the first candidate at
+            //    the smallest such SAD, if that is below the starting SAD.
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const bool valid = ((mask >> k) & 1u) != 0;
                 const bool diag = slot_i(k) != 0 && slot_j(k) != 0;
                 n_dg += (valid && diag) ? 1 : 0;
                 n_ax += (valid && !diag) ? 1 : 0;
-                const int ln = (k & 3) * 16, base = (k >> 2) * 4;
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    const int csad = lane_value(m[base + 2 * q], ln);
-                    const int cmad = lane_value(m[base + 2 * q + 1], ln);
-                    const bool in_copy = best_mad < thresh;
-                    const bool accept =
-                        valid && (in_copy ? cmad < best_mad
-                                          : ((csad < best_sad && csad < kSadThreshold) || cmad < thresh));
-                    sp_en = accept ? 1 : sp_en;
-                    sp_amt = accept ? q : sp_amt;
-                    sp_idx = accept ? k : sp_idx;
-                    best_sad = accept ? csad : best_sad;
-                    best_mad = accept ? cmad : best_mad;
+            }
+            const bool vj = lane < 16 && ((mask >> (lane >> 1)) & 1u) != 0;
+            auto row_min = [](int v) {
+                v = min(v, dpp<0xB1>(v));
+                v = min(v, dpp<0x4E>(v));
+                v = min(v, dpp<0x141>(v));
+                v = min(v, dpp<0x140>(v));
+                return lane_value(v, 0);
+            };
+            int win = -1;
+            if (b.mad < thresh) {
+                const int m2 = row_min(vj ? jmad : kInt32Max);
+                if (m2 < b.mad) win = (int)__builtin_ctzll(__ballot(vj && jmad == m2));
+            } else {
+                const uint64_t cm = __ballot(vj && jmad < thresh);
+                if (cm) {
+                    const int js = (int)__builtin_ctzll(cm);
+                    const bool after = vj && lane >= js;
+                    const int m2 = row_min(after ? jmad : kInt32Max);
+                    win = (int)__builtin_ctzll(__ballot(after && jmad == m2));
+                } else {
+                    const bool ok = vj && jsad < kSadThreshold;
+                    const int m1 = row_min(ok ? jsad : kInt32Max);
+                    if (m1 < b.sad) win = (int)__builtin_ctzll(__ballot(ok && jsad == m1));
                 }
+            }
+            if (win >= 0) {
+                sp_en = 1;
+                sp_amt = win & 1;
+                sp_idx = win >> 1;
+                b.sad = lane_value(jsad, win);
+                b.mad = lane_value(jmad, win);
             }
         }
         __syncthreads();
     }
 
     SearchOut o;
-    o.best_x = best_x; o.best_y = best_y;
-    o.sad = best_sad; o.mad = best_mad; o.ssd = best_ssd;
-    o.cost = __uint_as_float(best_cost);
+    o.best_x = b.x; o.best_y = b.y;
+    o.sad = b.sad; o.mad = b.mad; o.ssd = b.ssd;
+    o.cost = __uint_as_float(b.cost);
     o.sp_en = sp_en; o.sp_amt = sp_amt; o.sp_idx = sp_idx;
     o.n_int = n_int; o.n_ax = n_ax; o.n_dg = n_dg;
     o.btype = 0;
-    if (best_x != x || best_y != y || sp_en) o.btype |= EVXA_BLOCK_MOTION;
-    if (best_mad < thresh) o.btype |= EVXA_BLOCK_COPY;
+    if (b.x != x || b.y != y || sp_en) o.btype |= EVXA_BLOCK_MOTION;
+    if (b.mad < thresh) o.btype |= EVXA_BLOCK_COPY;
     return o;
 }
 
@@ -895,7 +1035,7 @@ __device__ void predict_mb(const DevPic &ref, const SearchOut &r, int16_t *out, 
             plane_of(i, pl, pitch, X, Y);
             out[i] = pl[Y * pitch + X];
         }
-        __syncthreads();
+        wave_sync();
         return;
     }
     if (dx != 0 && dy != 0) {
@@ -910,7 +1050,7 @@ __device__ void predict_mb(const DevPic &ref, const SearchOut &r, int16_t *out, 
             }
             tmp[i] = (int16_t)tap6(pl + (Y + yo) * pitch + X + xo - 2, 1);
         }
-        __syncthreads();
+        wave_sync();
     }
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         const int16_t *pl; int pitch, X, Y;
@@ -932,7 +1072,7 @@ __device__ void predict_mb(const DevPic &ref, const SearchOut &r, int16_t *out, 
         const int full = pl[Y * pitch + X];
         out[i] = (int16_t)(r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half);
     }
-    __syncthreads();
+    wave_sync();
 }
 
 // transform.cpp:264-284 on a line of 8 samples.
@@ -971,21 +1111,21 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
 {
     const int lane = lane_id();
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) res[i] = (int16_t)(src[i] - pred[i]);
-    __syncthreads();
+    wave_sync();
     if (lane < 48) {
         int origin, pitch;
         block_origin(lane >> 3, origin, pitch);
         const int row = origin + (lane & 7) * pitch;
         dct_line(res + row, 1, tmp + row, 1);
     }
-    __syncthreads();
+    wave_sync();
     if (lane < 48) {
         int origin, pitch;
         block_origin(lane >> 3, origin, pitch);
         const int col = origin + (lane & 7);
         dct_line(tmp + col, pitch, res + col, pitch);
     }
-    __syncthreads();
+    wave_sync();
 
     // compute_block_variance2 over the 16x16 luma coefficients, (0,0)
     // skipped; the int32 sums wrap like the reference's.
@@ -1094,30 +1234,26 @@ __device__ __forceinline__ uint64_t pred_word(int btype, uint32_t mv)
     return (1ull << 63) | ((uint64_t)(btype & 0xFF) << 32) | mv;
 }
 
-// Every lane's device-scope stores have completed once this returns.
+// Every lane's device-scope stores (of the calling wave) have completed
+// once this returns.
 __device__ __forceinline__ void stores_done()
 {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // compute_mv_predictor (motion.cpp:271-346) from the predictor words of
-// the left, top, top-right and top-left MBs (0 where absent) and the
-// temporal table's record, combined in the reference's order.
+// the left, top, top-right and top-left MBs (0 where absent) and the first
+// two words (type, MV) of the temporal table's record, combined in the
+// reference's order.
 __device__ void mv_predictor(const DevJob &job, int bx, int by, uint64_t wl, uint64_t wt,
-                             uint64_t wtr, uint64_t wtl, int &pmx, int &pmy)
+                             uint64_t wtr, uint64_t wtl, uint32_t p0, uint32_t p1, int &pmx,
+                             int &pmy)
 {
-    const int idx = by * job.wb + bx;
     const bool has_l = bx > 0, has_t = by > 0, has_tr = by > 0 && bx + 1 < job.wb;
     const bool has_tl = by > 0 && bx > 0, has_p = job.prev != nullptr;
-    uint32_t p0 = 0, p1 = 0;
-    if (has_p) {
-        const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + idx * 16;
-        p0 = prv[0];
-        p1 = prv[1];
-    }
     auto motion = [](uint64_t w) { return ((uint32_t)(w >> 32) & EVXA_BLOCK_MOTION) != 0; };
     int mx[3] = {0, 0, 0}, my[3] = {0, 0, 0}, n = 0;
     auto take = [&](uint32_t w1) {
@@ -1250,12 +1386,12 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
         if (bsad + bias < best_sad) {
             bidir = 1; best_sad = bsad; best = f; best_k = 0; other = b;
         }
-        __syncthreads();
+        wave_sync();
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
             if (bidir) sh.pred[i] = (int16_t)((sh.pred[i] + sh.pred_b[i] + 1) >> 1);
             else if (best_k == 1) sh.pred[i] = sh.pred_b[i];
         }
-        __syncthreads();
+        wave_sync();
     }
 
     if (best_k < 0 || best_sad == kInt32Max) {
@@ -1301,11 +1437,11 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     if (btype & EVXA_BLOCK_COPY) {
         // encode.cpp:742-744: copy blocks are not transformed.
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) { co[i] = 0; lv[i] = 0; }
-        __syncthreads();
+        wave_sync();
         return;
     }
     if (job.type == EVXA_FRAME_P) predict_mb(job.refs[best_k], best, sh.pred, sh.tmp);
-    __syncthreads();
+    wave_sync();
     int qp, var2;
     dct_quant(sh.src, sh.pred, sh.res, sh.tmp, job.quality, co, lv, qp, var2);
     uint32_t w2[2];
@@ -1314,12 +1450,15 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     write_words(rec + 4, w2, 2);
     w2[0] = 1u | (bidir ? 2u : 0u);
     write_words(rec + 14, w2, 1);
-    __syncthreads();
+    wave_sync();
 }
 
-__global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ jobs,
-                                                     const uint32_t *__restrict__ work,
-                                                     int32_t *counters)
+// The persistent analysis kernel: NW waves per workgroup search one
+// (MB, reference) ticket together; wave 0 then finishes it.
+template <int NW>
+__global__ __launch_bounds__(64 * NW, 2) void analyze_kernel(const DevJob *__restrict__ jobs,
+                                                          const uint32_t *__restrict__ work,
+                                                          int32_t *counters)
 {
     __shared__ Shared sh;
     __shared__ int32_t s_word;
@@ -1330,7 +1469,7 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
     const uint32_t *tickets = work + kWorkHeader;
     int32_t *next = counters + kCounterList + list * kCounterListStride;
     for (;;) {
-        if (lane == 0) s_word = lo + atomicAdd(next, 1);
+        if (threadIdx.x == 0) s_word = lo + atomicAdd(next, 1);
         __syncthreads();
         // LDS hands the value back in a vector register; broadcasting it makes
         // the work item, and all control derived from it, wave-uniform
@@ -1352,13 +1491,21 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         Blk s, pre;
         load_cand(job.src, geo, x, y, s);
         load_cand(rp, geo, x, y, pre);
-        const WinPos win = stage_window(rp, x, y, kReachMax, sh.sp);
-        const bool narrow = (*job.src.wide | *rp.wide) == 0;
+        const WinPos win = stage_window<NW>(rp, x, y, kReachMax, sh.sp);
+        const bool narrow = __builtin_amdgcn_readfirstlane((*job.src.wide | *rp.wide) == 0) != 0;
+        // The temporal table's record of this MB (written by an earlier launch).
+        uint32_t p0 = 0, p1 = 0;
+        if (job.prev) {
+            const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + (size_t)mb * 16;
+            p0 = prv[0];
+            p1 = prv[1];
+        }
 
         // Wait until the predictor words of the left, top, top-right and
         // top-left MBs are published (the row above up to its end at the
         // right edge), polling with a short back-off.  Lanes 0..3 load one
-        // word each; absent neighbours need nothing.
+        // word each; absent neighbours need nothing.  Every wave of the
+        // workgroup reads the same published words.
         const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
         const bool has_tl = has_t && has_l;
         const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
@@ -1382,12 +1529,15 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
                    (uint32_t)lane_value((int)(uint32_t)word, l);
         };
         int pmx, pmy;
-        mv_predictor(job, bx, by, lane_word(0), lane_word(1), lane_word(2), lane_word(3), pmx, pmy);
+        mv_predictor(job, bx, by, lane_word(0), lane_word(1), lane_word(2), lane_word(3), p0, p1,
+                     pmx, pmy);
         pmx = __builtin_amdgcn_readfirstlane(pmx);
         pmy = __builtin_amdgcn_readfirstlane(pmy);
         window_ready();
-        const SearchOut o = inter_search(rp, job.width, job.height, narrow, s, geo, pre, win, x, y,
-                                         job.thresh, job.lambda, pmx, pmy, sh.sp);
+        const SearchOut o = inter_search<NW>(rp, job.width, job.height, narrow, s, geo, pre, win,
+                                             x, y, job.thresh, job.lambda, pmx, pmy, sh.sp);
+        // Wave 0 finishes the ticket; the other waves go on to the next claim.
+        if (NW > 1 && wave_id() != 0) continue;
         if (job.n_refs == 1) {
             // Single reference: this wave finishes the MB directly.
             finish_mb(job, bx, by, counters, sh.mb, s, geo, &o);
@@ -1400,16 +1550,14 @@ __global__ __launch_bounds__(64) void analyze_kernel(const DevJob *__restrict__ 
         // This search's result is complete in memory before it is counted;
         // the wave that counts last reads every result device-scope.
         stores_done();
+        int old = 0;
         if (lane == 0) {
-            const int old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-            s_word = old;
+            old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
             if (old == job.n_refs - 1)   // last one: back to zero for the next launch
                 st_relaxed(reinterpret_cast<uint32_t *>(job.mb_done + mb), 0u);
         }
-        __syncthreads();
-        const int done_before = __builtin_amdgcn_readfirstlane(s_word);
-        __syncthreads();
+        const int done_before = __builtin_amdgcn_readfirstlane(old);
         if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, s, geo, nullptr);
     }
 }
@@ -1481,10 +1629,10 @@ __global__ __launch_bounds__(64) void search_blocks_kernel(DevPic src, DevPic re
     Blk s, pre;
     load_cand(src, geo, p[0], p[1], s);
     load_cand(ref, geo, p[0], p[1], pre);
-    const WinPos win = stage_window(ref, p[0], p[1], kReachMax, sp);
+    const WinPos win = stage_window<1>(ref, p[0], p[1], kReachMax, sp);
     const bool narrow = (*src.wide | *ref.wide) == 0;
     window_ready();
-    SearchOut o = inter_search(ref, w, h, narrow, s, geo, pre, win, p[0], p[1], (quality >> 2) + 1,
+    SearchOut o = inter_search<1>(ref, w, h, narrow, s, geo, pre, win, p[0], p[1], (quality >> 2) + 1,
                                lambda, p[2], p[3], sp);
     if (lane_id() == 0) {
         evxa_search_result r;
@@ -1529,9 +1677,12 @@ hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st)
 }
 
 hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
-                          int32_t grid, hipStream_t st)
+                          int32_t grid, int32_t waves, hipStream_t st)
 {
-    hipLaunchKernelGGL(analyze_kernel, dim3(grid), dim3(64), 0, st, jobs, work, counters);
+    if (waves == 2)
+        hipLaunchKernelGGL(analyze_kernel<2>, dim3(grid), dim3(128), 0, st, jobs, work, counters);
+    else
+        hipLaunchKernelGGL(analyze_kernel<1>, dim3(grid), dim3(64), 0, st, jobs, work, counters);
     return hipGetLastError();
 }
 
@@ -1578,16 +1729,17 @@ hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t 
     return hipGetLastError();
 }
 
-int32_t analyze_resident_waves(int32_t device)
+int32_t analyze_resident_groups(int32_t device, int32_t waves)
 {
     int n_cu = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         n_cu <= 0)
         n_cu = 256;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel, 64, 0) != hipSuccess ||
-        per_cu <= 0)
-        per_cu = 4;
+    const hipError_t e = waves == 2
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel<2>, 128, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel<1>, 64, 0);
+    if (e != hipSuccess || per_cu <= 0) per_cu = waves == 2 ? 2 : 4;
     return n_cu * per_cu;
 }
 

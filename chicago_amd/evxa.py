@@ -53,6 +53,14 @@ COMPLETION_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("batch_frames",
 assert COMPLETION_DTYPE.itemsize == 48
 
 
+class Tuning(C.Structure):
+    _fields_ = [("waves_per_search", C.c_int32), ("max_groups", C.c_int32), ("lists", C.c_int32),
+                ("cap_pct", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class FrameDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("quality", C.c_int32), ("src_slot", C.c_int32),
                 ("n_refs", C.c_int32), ("ref_slots", C.c_int32 * 4),
@@ -69,7 +77,8 @@ EXPORTS = (
     "evxa_read_results",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
-    "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy")
+    "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
+    "evxa_set_tuning", "evxa_get_tuning")
 
 
 class EvxaError(RuntimeError):
@@ -127,6 +136,8 @@ def load_library(path: str = LIB_PATH):
         "evxa_server_poll": (C.c_int, [vp, vp, C.c_int32, C.POINTER(C.c_int32)]),
         "evxa_server_pending": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "evxa_server_destroy": (C.c_int, [vp]),
+        "evxa_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "evxa_get_tuning": (C.c_int, [vp, C.POINTER(Tuning), C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -209,6 +220,17 @@ class Context:
     @property
     def bytes_used(self) -> int:
         return _lib.evxa_context_bytes(self.h)
+
+    def set_tuning(self, waves_per_search=0, max_groups=0, lists=0, cap_pct=0):
+        t = Tuning(waves_per_search, max_groups, lists, cap_pct)
+        _check(_lib.evxa_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        """(current Tuning as a dict, workgroups of that shape resident at once)."""
+        t = Tuning()
+        n = C.c_int32()
+        _check(_lib.evxa_get_tuning(self.h, C.byref(t), C.byref(n)))
+        return t.as_dict(), n.value
 
     def open_stream(self, width, height, n_slots):
         return Stream(self, width, height, n_slots)

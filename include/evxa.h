@@ -182,6 +182,21 @@ int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src,
                           const int16_t *pred, const int32_t *quality, int16_t *coeffs,
                           int16_t *levels, int32_t *qp, int32_t *variance);
 
+/* ---- launch shape (no counterpart in the reference: its analysis runs on
+ *      CPU threads, encode.cpp:817-941).  Set by the autotuner. ---------- */
+
+typedef struct evxa_tuning {
+    int32_t waves_per_search;     /* 1 or 2 waves search one (MB, reference) pair (0: keep) */
+    int32_t max_groups;           /* workgroups per launch (0: all that fit the device) */
+    int32_t lists;                /* work lists per launch, 1..8 (0: one per XCD for batches) */
+    int32_t cap_pct;              /* tickets per list and diagonal, % of its workgroups (0: 60) */
+} evxa_tuning;
+
+int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t);
+/* Current tuning; *resident_groups (optional) receives the workgroups of
+ * the current shape that fit the device at once. */
+int evxa_get_tuning(evxa_context *ctx, evxa_tuning *t, int32_t *resident_groups);
+
 /* ---- serving: continuous batching (no counterpart in the reference, whose
  *      encoder analyses one frame at a time inside engine_encode_frame,
  *      encode.cpp:945; this is the multi-stream request path) ------------ */
