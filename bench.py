@@ -154,6 +154,13 @@ def run_batched(args, ctx):
     mbs = sum(s.n_mbs for s in streams)
     log(f"batched: uploaded, {mbs} MBs per step")
 
+    tuned = None
+    if not args.no_tune:
+        from chicago_amd import autotune
+        tuned, _ = autotune.tune(ctx, lambda: ctx.analyze_batch(streams, descs),
+                                 lambda: [s.results()[0] for s in streams], autotune.BATCH,
+                                 reps=2, log=log)
+
     # Untimed eager step: outputs and work accounting.
     ctx.analyze_batch(streams, descs)
     ctx.synchronize()
@@ -207,6 +214,7 @@ def run_batched(args, ctx):
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "replays_bit_identical": same,
+        "launch_shape": tuned,
     }
     return res, streams, descs, seqs, outs
 
@@ -251,6 +259,18 @@ def run_plain(args):
         if t == 1:
             s.reset_temporal()
         s.analyze_p(q, t, frame_refs(t))
+
+    # Launch shape: the autotuner times one 4-reference frame (temporal table
+    # reset, so every shape sees the same inputs) under each candidate.
+    tuned = None
+    if not args.no_tune:
+        from chicago_amd import autotune
+
+        def tune_step():
+            s.reset_temporal()
+            s.analyze_p(q, 5, frame_refs(5))
+        tuned, table = autotune.tune(ctx, tune_step, lambda: s.results(), autotune.SINGLE,
+                                     log=log)
 
     # Untimed pass: work accounting, outputs for the CPU comparison and
     # per-frame latency (enqueue -> device idle).
@@ -326,6 +346,7 @@ def run_plain(args):
             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "what": "enqueue of one frame to device idle, sequential frames",
         },
+        "launch_shape": tuned,
     }
     if not args.no_cpu:
         sample = list(range(args.cpu_start, args.cpu_start + args.cpu_frames))
@@ -366,6 +387,13 @@ def run_gop(args):
         else:
             s.analyze_b(r.quality, r.index, r.refs[0], r.refs[1], target_ids=[0, 1])
 
+    tuned = None
+    if not args.no_tune:
+        from chicago_amd import autotune
+        rb = next(r for r in work if r.kind == "B")
+        tuned, _ = autotune.tune(ctx, lambda: step(rb), lambda: s.results(), autotune.SINGLE,
+                                 reps=2, log=log)
+
     # Untimed pass: per-frame latency, work accounting, outputs of a sample.
     sample = {work[0].index, next(r.index for r in work if r.kind == "B")}
     ops, lat, outs, ptables = 0, [], {}, {}
@@ -402,6 +430,7 @@ def run_gop(args):
         "achieved_tops": round(ops / (float(np.sum(kms)) / 1e3) / 1e12, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "launch_shape": tuned,
     }
     if not args.no_cpu:
         from oracle.oracle import Oracle, Picture, Reference, reference_available
@@ -443,6 +472,10 @@ def run_serve(args):
     from chicago_amd import evxa, serve
 
     ctx = evxa.Context(0)
+    # Launch shape for serving: the fewest-latency shape of a single frame
+    # (two waves per search) unless the tuner is off.
+    if not args.no_tune:
+        ctx.set_tuning(waves_per_search=args.serve_waves)
     cap = None
     if args.serve_util > 0:
         cap = calibrate_serving(ctx, args) * args.serve_util
@@ -456,6 +489,7 @@ def run_serve(args):
                        f"{args.serve_seconds:g} s schedule, 1 ref, quality {args.quality}, "
                        f"{args.serve_inflight} launches in flight")
     res["admission_capacity_mb_per_s"] = cap
+    res["launch_shape"] = ctx.tuning()[0]
     ctx.close()
     return res
 
@@ -557,6 +591,8 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-max-frames", type=int, default=64)
     ap.add_argument("--serve-util", type=float, default=0.6)
+    ap.add_argument("--serve-waves", type=int, default=2)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=5)
     ap.add_argument("--cpu-frames", type=int, default=4)

@@ -1,0 +1,64 @@
+"""Autotuner of the analysis launch shape.
+
+The knobs (evxa_set_tuning): waves per (MB, reference) search (one wave
+evaluates a level's 8 candidates in two passes, two waves in one pass and
+exchange results through LDS), work lists per launch (row bands spread over
+the XCDs' L2 caches), and the per-list cap of tickets per virtual diagonal
+that staggers the frames of a batch.  Which shape is fastest depends on the
+workload: a single frame is bound by its MB dependency chain and wants the
+lowest per-MB latency, a batch of many frames wants throughput.
+
+`tune` runs a repeatable step of the real workload under each candidate
+shape, times its launches with HIP events, requires every shape's outputs to
+equal the first shape's bit for bit, and leaves the fastest applied.  The
+reference has no counterpart: its analysis runs on CPU threads
+(encode.cpp:817-941).
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+# Candidate shapes per workload class.
+SINGLE = [dict(waves_per_search=w, lists=l) for w, l in itertools.product((1, 2), (1, 8))]
+BATCH = [dict(waves_per_search=w, lists=8, cap_pct=c)
+         for w, c in itertools.product((1, 2), (60, 100, 200, 400, 800))]
+SERVE = [dict(waves_per_search=w) for w in (1, 2)]
+
+
+def _same(a, b):
+    if a is None or b is None:
+        return True
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def tune(ctx, step, outputs=None, candidates=SINGLE, reps=3, warmup=1, log=None):
+    """Time `step()` (which enqueues the launches of one step) under each
+    candidate shape; `outputs()` returns arrays to compare across shapes.
+    Applies and returns the fastest shape with the per-shape table."""
+    table = []
+    base = None
+    for cand in candidates:
+        ctx.set_tuning(**cand)
+        for _ in range(warmup):
+            step()
+        ctx.synchronize()
+        ctx.launch_times()
+        for _ in range(reps):
+            step()
+        ctx.synchronize()
+        ms = ctx.launch_times()
+        step_ms = float(np.sum(ms)) / reps
+        out = outputs() if outputs else None
+        if base is None:
+            base = out
+        same = _same(base, out)
+        table.append({"tuning": dict(cand), "step_ms": round(step_ms, 4), "bit_identical": same})
+        if log:
+            log(f"autotune {cand}: {step_ms:.3f} ms/step, bit-identical={same}")
+        if not same:
+            raise AssertionError(f"launch shape {cand} changed the results")
+    best = min(table, key=lambda r: r["step_ms"])
+    ctx.set_tuning(**best["tuning"])
+    return best, table
