@@ -983,8 +983,9 @@ int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src, cons
 //
 // A server owns a dispatcher thread and `max_inflight` launch slots, each with
 // its own HIP stream, job/work buffers and events.  Submitted frames queue in
-// arrival order; whenever a slot is free, the dispatcher fuses every queued
-// frame whose stream has no frame in flight (up to `max_frames`) into one
+// arrival order; whenever a slot is free, the dispatcher fuses the queued
+// frames whose stream has no frame in flight and whose size equals the oldest
+// such frame's (up to `max_frames`; a launch completes as a whole) into one
 // persistent launch on that slot, sized to its share of the device, so
 // launches overlap on the GPU and frames that arrive while others run start
 // in the next free slot.  A stream has at most one frame in flight: the next
@@ -1118,11 +1119,19 @@ void server_run(evxa_server *srv)
         for (auto &sl : srv->slots)
             if (!sl->busy) { free_slot = sl.get(); break; }
         if (free_slot && !srv->pending.empty()) {
+            // A launch completes as a whole, so it fuses frames of one size
+            // class (MBs per frame): the class of the oldest ready frame.
             std::vector<evxa_server::Req> pick;
             std::set<evxa_stream *> taken;
+            int64_t klass = -1;
             for (auto it = srv->pending.begin();
                  it != srv->pending.end() && (int32_t)pick.size() < srv->max_frames;) {
-                if (srv->active.count(it->s) || taken.count(it->s)) { ++it; continue; }
+                const int64_t k = (int64_t)it->s->wb * it->s->hb;
+                if (srv->active.count(it->s) || taken.count(it->s) || (klass >= 0 && k != klass)) {
+                    ++it;
+                    continue;
+                }
+                klass = k;
                 taken.insert(it->s);
                 pick.push_back(*it);
                 it = srv->pending.erase(it);

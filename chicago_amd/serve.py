@@ -218,6 +218,7 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
         heapq.heappush(leaving, (life.t_leave, life.sid))
     tags = {}                 # tag -> (sid, k, nmbs)
     lat, dev, batch, mbs_done = [], [], [], 0
+    queue, run_t, klass = [], [], []
     srv = evxa.Server(ctx, max_inflight, max_frames)
     t0 = evxa.clock() + 0.05
     i, n_sub = 0, 0
@@ -230,6 +231,9 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
             if c["status"] != 0:
                 raise evxa.EvxaError(int(c["status"]), f"frame {k} of stream {sid} failed")
             lat.append(c["t_done"] - c["t_arrival"])
+            queue.append(c["t_launch"] - c["t_arrival"])
+            run_t.append(c["t_done"] - c["t_launch"])
+            klass.append(by_sid[sid].height)
             dev.append(c["device_ms"])
             batch.append(c["batch_frames"])
             mbs_done += nmbs
@@ -280,6 +284,11 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
     wall = evxa.clock() - t0
     srv.close()
     lat = np.array(lat) * 1e3
+    queue, run_t, klass = np.array(queue) * 1e3, np.array(run_t) * 1e3, np.array(klass)
+
+    def pct(a):
+        return {"p50": round(float(np.percentile(a, 50)), 3),
+                "p99": round(float(np.percentile(a, 99)), 3)} if len(a) else None
     res = {
         "offered_streams_live": n_live,
         "stream_lives": len(lives),
@@ -295,6 +304,9 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
                        "p90": round(float(np.percentile(lat, 90)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3)} if len(lat) else None,
+        "queue_ms": pct(queue),
+        "launch_to_done_ms": pct(run_t),
+        "latency_ms_by_height": {int(hh): pct(lat[klass == hh]) for hh in np.unique(klass)},
         "mean_frames_per_launch": round(float(np.mean(batch)), 2) if batch else 0,
         "mean_launch_ms": round(float(np.mean(dev)), 3) if dev else 0,
         "pooled_device_streams": {f"{w}x{h}": p.opened for (w, h), p in adm.pools.items()},
