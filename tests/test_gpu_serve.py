@@ -17,10 +17,11 @@ def _same(a, b):
 
 
 def test_served_frames_match_alone_and_oracle(ctx, oracle):
-    sizes = [(64, 48), (160, 96), (96, 64), (128, 128), (80, 48)]
+    # Repeated sizes: the dispatcher fuses frames of one size class.
+    sizes = [(64, 48), (160, 96), (64, 48), (160, 96), (96, 64), (64, 48)]
     n_frames, pool = 5, 4
     content = {sz: synth.sequence_c1(synth.seed_of(9, *sz), pool, *sz, local_range=8)
-               for sz in sizes}
+               for sz in set(sizes)}
     descs = serve.descs_for(pool, 9)
     streams = []
     for sz in sizes:
