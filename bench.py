@@ -27,6 +27,10 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# The serving dispatcher keeps up to --serve-inflight launches on streams of
+# their own; HIP maps streams onto this many hardware queues per process (its
+# default is 4, which the dispatcher's streams would share with the others).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 sys.path.insert(0, ROOT)
 
 METRIC = "1080p ME+DCT/quant frames/s (bit-exact), batched vs plain; p99 frame latency"
@@ -478,7 +482,8 @@ def run_serve(args):
         ctx.set_tuning(waves_per_search=args.serve_waves)
     cap = None
     if args.serve_util > 0:
-        cap = calibrate_serving(ctx, args) * args.serve_util
+        full = args.serve_capacity if args.serve_capacity > 0 else calibrate_serving(ctx, args)
+        cap = full * args.serve_util
         log(f"serve: admission capacity {cap / 1e6:.2f} M MB/s")
     res = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
                     duration=args.serve_seconds, quality=args.quality,
@@ -510,7 +515,9 @@ def calibrate_serving(ctx, args):
             s.upload(k, *f)
         streams.append(s)
     n_per = 6
-    with evxa.Server(ctx, args.serve_inflight, args.serve_max_frames) as srv:
+    # The burst runs two launches in flight whatever the serving setting, so
+    # the capacity (and the admitted load) does not depend on it.
+    with evxa.Server(ctx, 2, args.serve_max_frames) as srv:
         t0 = time.perf_counter()
         for k in range(1, n_per + 1):
             for i, s in enumerate(streams):
@@ -588,9 +595,11 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=200)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-seed", type=int, default=1)
-    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-max-frames", type=int, default=64)
     ap.add_argument("--serve-util", type=float, default=0.6)
+    ap.add_argument("--serve-capacity", type=float, default=0.0,
+                    help="admission capacity in MB/s (0: measure it with a burst)")
     ap.add_argument("--serve-waves", type=int, default=2)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--batch-steps", type=int, default=20)

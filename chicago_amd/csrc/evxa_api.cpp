@@ -1053,9 +1053,12 @@ void server_launch(evxa_server *srv, evxa_server::Slot &sl, std::vector<evxa_ser
     std::vector<DevJob> jobs;
     jobs.reserve(reqs.size());
     for (auto &r : reqs) jobs.push_back(make_job(r.s, r.d, true));
+    // Each launch is sized by its own work (the widest diagonal of its
+    // lists), capped at half the device: launches of chain-bound frames need
+    // few workgroups, and several in flight may share the device.
     const int32_t fit = ctx->resident[ctx->waves];
     const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
-    const int32_t cap = std::max(1, all / srv->max_inflight);
+    const int32_t cap = std::max(1, all / std::min(srv->max_inflight, 2));
     enqueue(ctx, sl.res, jobs, false, sl.st, cap, false, sl.t0, nullptr);
     check(hipMemcpyAsync(sl.flags.p, sl.res.progress.p, (kCounterError + 1) * sizeof(int32_t),
                          hipMemcpyDeviceToHost, sl.st),
@@ -1203,7 +1206,7 @@ int evxa_server_create(evxa_context *ctx, int32_t max_inflight, int32_t max_fram
     return guarded([&] {
         if (!ctx || !out) fail(EVXA_ERROR_INVALIDARG, "null argument");
         *out = nullptr;
-        if (max_inflight < 1 || max_inflight > 4) fail(EVXA_ERROR_INVALIDARG, "max_inflight must be 1..4");
+        if (max_inflight < 1 || max_inflight > 16) fail(EVXA_ERROR_INVALIDARG, "max_inflight must be 1..16");
         if (max_frames < 1 || max_frames > kMaxJobs) fail(EVXA_ERROR_INVALIDARG, "max_frames out of range");
         auto srv = std::make_unique<evxa_server>();
         srv->ctx = ctx;

@@ -167,7 +167,7 @@ def descs_for(pool_frames, quality):
 
 
 def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_life=20.0,
-        quality=8, pool_frames=6, max_inflight=2, max_frames=64, capacity_mbps=None,
+        quality=8, pool_frames=6, max_inflight=4, max_frames=64, capacity_mbps=None,
         mix=C4_MIX, log=None):
     """Run the request loop in real time for `duration` seconds of schedule
     and return latency and throughput statistics."""

@@ -220,7 +220,7 @@ typedef struct evxa_completion {
 
 /* Monotonic clock (seconds) shared with the dispatcher. */
 double evxa_clock(void);
-/* Start a dispatcher with `max_inflight` (1..4) concurrent launches, each
+/* Start a dispatcher with `max_inflight` (1..16) concurrent launches, each
  * fusing up to `max_frames` ready frames. */
 int evxa_server_create(evxa_context *ctx, int32_t max_inflight, int32_t max_frames,
                        evxa_server **out);
