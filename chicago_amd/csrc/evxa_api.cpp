@@ -93,7 +93,7 @@ size_t outputs_bytes(int32_t w, int32_t h)
 {
     const size_t n = (size_t)(w / 16) * (h / 16);
     return 3 * n * sizeof(evxa_block_record) + 2 * n * EVXA_MB_SAMPLES * sizeof(int16_t) +
-           n * EVXA_MAX_REFS * sizeof(SearchRec) + n * sizeof(int32_t);
+           n * EVXA_MAX_REFS * sizeof(SearchRec);
 }
 
 // Device buffer owned by RAII.
@@ -215,7 +215,6 @@ struct evxa_stream {
     DevBuf tables;         // 3 record tables: P ping, P pong, B
     DevBuf coeffs, levels;
     DevBuf searches;       // EVXA_MAX_REFS SearchRec per MB
-    DevBuf mb_done;        // per-MB finished-search counters (kept at zero between launches)
     DevBuf wide;           // per slot: nonzero if a sample lies outside [0, 2047]
     int32_t p_cur = 0;
     bool temporal_valid = false;
@@ -296,7 +295,6 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     j.coeffs = s->coeffs.as<int16_t>();
     j.levels = s->levels.as<int16_t>();
     j.searches = s->searches.as<SearchRec>();
-    j.mb_done = s->mb_done.as<int32_t>();
     if (advance) {
         s->last_table = out;
         if (d.type == EVXA_FRAME_P && d.update_temporal) {
@@ -633,8 +631,8 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
         s->coeffs.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
         s->levels.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
         s->searches.alloc(n * EVXA_MAX_REFS * sizeof(SearchRec));
-        s->mb_done.alloc(n * sizeof(int32_t));
-        check(hipMemsetAsync(s->mb_done.p, 0, n * sizeof(int32_t), ctx->stream), "clear counters");
+        check(hipMemsetAsync(s->searches.p, 0, n * EVXA_MAX_REFS * sizeof(SearchRec), ctx->stream),
+              "clear search words");
         s->wide.alloc((size_t)n_slots * sizeof(int32_t));
         check(hipMemsetAsync(s->wide.p, 0, (size_t)n_slots * sizeof(int32_t), ctx->stream),
               "clear range flags");

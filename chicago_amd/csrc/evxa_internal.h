@@ -26,16 +26,19 @@ struct DevPic {
     const int32_t *wide;
 };
 
-// Outcome of one (MB, reference) search, kept in HBM until the MB's last
-// search finishes and selects among them (32 bytes).
+// Outcome of one (MB, reference) search, handed to the wave that finishes
+// the MB (32 bytes).  Three 64-bit words, each stored as one access with
+// bit 63 set, so a reader that sees the bit sees the whole word:
+//   w[0]: bits 0..15 best x, 16..31 best y, 32..62 MAD
+//   w[1]: bits 0..31 RD cost (fp32 bits), 32..62 SAD
+//   w[2]: bits 0..15 integer candidates, 16..23 axial and 24..31 diagonal
+//         sub-pel directions, 32 sub-pel enable, 33 quarter, 34..36 sub-pel
+//         direction index, 40..47 block type
+// SAD and MAD are below 2^25 (256 luma or 128 chroma differences of int16
+// samples).  The finishing wave clears the words it read, so they are zero
+// between launches.
 struct SearchRec {
-    int16_t best_x, best_y;
-    int32_t sad, mad, ssd;
-    uint32_t cost_bits;
-    uint8_t sp_en, sp_amt, sp_idx, btype;
-    uint16_t n_int;
-    uint8_t n_ax, n_dg;
-    uint32_t reserved;
+    uint64_t w[4];
 };
 static_assert(sizeof(SearchRec) == 32, "SearchRec layout");
 
@@ -52,7 +55,6 @@ struct DevJob {
     int16_t *coeffs;
     int16_t *levels;
     SearchRec *searches;               // EVXA_MAX_REFS per MB
-    int32_t *mb_done;                  // searches finished per MB (returns to 0)
     uint64_t *mvw;                     // per MB: predictor word, zeroed per launch
 };
 

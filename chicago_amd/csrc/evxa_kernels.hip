@@ -1201,22 +1201,11 @@ __device__ __forceinline__ void store_src(const Blk &s, const Geo &g, int16_t *d
 // Cross-wave hand-off without cache-wide fences.  The MI355X has one L2 per
 // XCD, and an agent-scope release or acquire fence writes back or
 // invalidates a whole L2 (every coefficient and level store in flight).  The
-// few words other waves read -- neighbour records, search results, row
-// progress and finished-search counts -- are instead written and read with
-// device-scope (sc1) accesses, which go to the coherent side of the L2s.  A
-// producer stores its data words, waits for their completion (vmcnt(0)) in
-// every lane, passes a barrier, then stores the flag; a consumer reads the
-// flag, then the data, both device-scope.  Compiler fences keep the order.
-__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// few words other waves read -- MB predictor words and search results -- are
+// instead written and read with device-scope (sc1) accesses, which go to the
+// coherent side of the L2s.  Each such word is one 64-bit access carrying
+// its own published bit, so a reader needs no flag, no ordering between
+// words and no wait on the writer's side.
 __device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1232,16 +1221,6 @@ __device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
 __device__ __forceinline__ uint64_t pred_word(int btype, uint32_t mv)
 {
     return (1ull << 63) | ((uint64_t)(btype & 0xFF) << 32) | mv;
-}
-
-// Every lane's device-scope stores (of the calling wave) have completed
-// once this returns.
-__device__ __forceinline__ void stores_done()
-{
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // compute_mv_predictor (motion.cpp:271-346) from the predictor words of
@@ -1283,69 +1262,89 @@ __device__ void mv_predictor(const DevJob &job, int bx, int by, uint64_t wl, uin
     }
 }
 
-// by other waves inside the launch).
-__device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, int n,
-                                            bool coherent = false)
+// Lane k < n stores word k of `w` to dst[k] (one coalesced store of a
+// record that no wave of the launch reads).
+__device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, int n)
 {
     const int lane = lane_id();
     if (lane < n) {
         uint32_t v = w[0];
         for (int k = 1; k < n; k++) v = lane == k ? w[k] : v;
-        if (coherent) st_relaxed(dst + lane, v);
-        else dst[lane] = v;
+        dst[lane] = v;
     }
 }
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 
-__device__ __forceinline__ void pack_search(const SearchOut &o, uint32_t (&w)[8])
+// Lanes 0..2 store this search's three result words (SearchRec layout,
+// bit 63 set) for the wave that finishes the MB.
+__device__ __forceinline__ void publish_search(uint64_t *dst, const SearchOut &o)
 {
-    w[0] = (uint32_t)(uint16_t)o.best_x | ((uint32_t)(uint16_t)o.best_y << 16);
-    w[1] = (uint32_t)o.sad;
-    w[2] = (uint32_t)o.mad;
-    w[3] = (uint32_t)o.ssd;
-    w[4] = fbits(o.cost);
-    w[5] = (uint32_t)o.sp_en | ((uint32_t)o.sp_amt << 8) | ((uint32_t)o.sp_idx << 16) |
-           ((uint32_t)o.btype << 24);
-    w[6] = (uint32_t)(o.n_int & 0xFFFF) | ((uint32_t)(o.n_ax & 0xFF) << 16) |
-           ((uint32_t)(o.n_dg & 0xFF) << 24);
-    w[7] = 0;
+    const int lane = lane_id();
+    const uint64_t pub = 1ull << 63;
+    const uint32_t pos = (uint32_t)(uint16_t)o.best_x | ((uint32_t)(uint16_t)o.best_y << 16);
+    const uint32_t cnt = (uint32_t)(o.n_int & 0xFFFF) | ((uint32_t)(o.n_ax & 0xFF) << 16) |
+                         ((uint32_t)(o.n_dg & 0xFF) << 24);
+    const uint32_t fl = (uint32_t)(o.sp_en & 1) | ((uint32_t)(o.sp_amt & 1) << 1) |
+                        ((uint32_t)(o.sp_idx & 7) << 2) | ((uint32_t)(o.btype & 0xFF) << 8);
+    const uint64_t w0 = pub | ((uint64_t)(uint32_t)o.mad << 32) | pos;
+    const uint64_t w1 = pub | ((uint64_t)(uint32_t)o.sad << 32) | fbits(o.cost);
+    const uint64_t w2 = pub | ((uint64_t)fl << 32) | cnt;
+    if (lane < 3) st_relaxed64(dst + lane, lane == 0 ? w0 : lane == 1 ? w1 : w2);
 }
 
-// Search result k of a MB, read wave-uniformly from the words in `v`
-// (lane l holds word l of the MB's SearchRec array).
-__device__ __forceinline__ SearchOut unpack_search(int v, int k)
+// Search result k of a MB, read wave-uniformly from the words in `sw`
+// (lane 3k + i holds word i of reference k's SearchRec).
+__device__ __forceinline__ SearchOut unpack_search(uint64_t sw, int k)
 {
+    auto lo = [&](int i) { return (uint32_t)lane_value((int)(uint32_t)sw, 3 * k + i); };
+    auto hi = [&](int i) { return (uint32_t)lane_value((int)(uint32_t)(sw >> 32), 3 * k + i) & 0x7FFFFFFFu; };
     SearchOut o;
-    const uint32_t w0 = (uint32_t)lane_value(v, 8 * k);
-    o.best_x = (int16_t)(w0 & 0xFFFF);
-    o.best_y = (int16_t)(w0 >> 16);
-    o.sad = lane_value(v, 8 * k + 1);
-    o.mad = lane_value(v, 8 * k + 2);
-    o.ssd = lane_value(v, 8 * k + 3);
-    o.cost = __uint_as_float((uint32_t)lane_value(v, 8 * k + 4));
-    const uint32_t w5 = (uint32_t)lane_value(v, 8 * k + 5);
-    o.sp_en = w5 & 0xFF; o.sp_amt = (w5 >> 8) & 0xFF; o.sp_idx = (w5 >> 16) & 0xFF;
-    o.btype = w5 >> 24;
-    const uint32_t w6 = (uint32_t)lane_value(v, 8 * k + 6);
-    o.n_int = w6 & 0xFFFF; o.n_ax = (w6 >> 16) & 0xFF; o.n_dg = w6 >> 24;
+    const uint32_t pos = lo(0), cnt = lo(2), fl = hi(2);
+    o.best_x = (int16_t)(pos & 0xFFFF);
+    o.best_y = (int16_t)(pos >> 16);
+    o.mad = (int)hi(0);
+    o.sad = (int)hi(1);
+    o.cost = __uint_as_float(lo(1));
+    o.ssd = 0;
+    o.sp_en = fl & 1; o.sp_amt = (fl >> 1) & 1; o.sp_idx = (fl >> 2) & 7;
+    o.btype = (fl >> 8) & 0xFF;
+    o.n_int = cnt & 0xFFFF; o.n_ax = (cnt >> 16) & 0xFF; o.n_dg = cnt >> 24;
     return o;
 }
 
 // Selection among a MB's searches, record publication, residual prediction,
-// DCT and quantisation; run by the wave that finished the MB's last search.
-// With a single reference the caller passes its search result in `own`;
-// otherwise all results are read back from job.searches.
+// DCT and quantisation; run by the wave that searched the MB's last
+// reference, which passes its own result in `own`.  The results of the
+// other references are polled from job.searches (they were claimed earlier,
+// so their waves are running) and cleared for the next launch.
 __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
-                          const Blk &src, const Geo &geo, const SearchOut *own)
+                          const Blk &src, const Geo &geo, const SearchOut &own)
 {
     const int lane = lane_id();
     const int x = bx * 16, y = by * 16;
     const int idx = by * job.wb + bx;
     const float lambda = job.lambda;
-    const uint32_t *words =
-        reinterpret_cast<const uint32_t *>(job.searches + (size_t)idx * EVXA_MAX_REFS);
-    const int v = (!own && lane < 8 * job.n_refs) ? (int)ld_relaxed(words + lane) : 0;
+    // Lane l < 3 (n_refs - 1) polls word l % 3 of reference l / 3.
+    const int n_words = 3 * (job.n_refs - 1);
+    uint64_t sw = 0;
+    if (n_words > 0) {
+        uint64_t *p = reinterpret_cast<uint64_t *>(job.searches + (size_t)idx * EVXA_MAX_REFS) +
+                      (lane / 3) * 4 + lane % 3;
+        int spins = 0, nap = 1;
+        for (;;) {
+            if (lane < n_words) sw = ld_relaxed64(p);
+            if (__ballot(lane < n_words && (sw >> 63) == 0) == 0) break;
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(&counters[kCounterError], kErrWaitTimeout);
+                break;
+            }
+            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+            nap = min(nap * 2, kMaxNap);
+        }
+        if (lane < n_words) st_relaxed64(p, 0ull);
+    }
+    auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
 
     SearchOut best, other;
     other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
@@ -1356,7 +1355,7 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
         // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel.
         bool best_copy = false;
         for (int k = 0; k < job.n_refs; k++) {
-            const SearchOut r = own ? *own : unpack_search(v, k);
+            const SearchOut r = result(k);
             n_int += r.n_int; n_ax += r.n_ax; n_dg += r.n_dg;
             const int bias = __float2int_rz(__fmul_rn(lambda, __int2float_rn((k + 1) * 2)));
             const bool copy = (r.btype & EVXA_BLOCK_COPY) != 0;
@@ -1369,8 +1368,8 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     } else {
         // encode.cpp:386-443 without the intra candidates; the bidirectional
         // candidate (motion.cpp:702-840) reuses both searches.
-        const SearchOut f = unpack_search(v, 0);
-        const SearchOut b = unpack_search(v, 1);
+        const SearchOut f = result(0);
+        const SearchOut b = result(1);
         n_int = f.n_int + b.n_int; n_ax = f.n_ax + b.n_ax; n_dg = f.n_dg + b.n_dg;
         if (f.sad < best_sad) { best = f; best_k = 0; best_sad = f.sad; }
         if (b.sad < best_sad) { best = b; best_k = 1; best_sad = b.sad; }
@@ -1538,27 +1537,14 @@ __global__ __launch_bounds__(64 * NW, 2) void analyze_kernel(const DevJob *__res
                                              x, y, job.thresh, job.lambda, pmx, pmy, sh.sp);
         // Wave 0 finishes the ticket; the other waves go on to the next claim.
         if (NW > 1 && wave_id() != 0) continue;
-        if (job.n_refs == 1) {
-            // Single reference: this wave finishes the MB directly.
-            finish_mb(job, bx, by, counters, sh.mb, s, geo, &o);
+        if (ref < job.n_refs - 1) {
+            // Hand the result to the wave of the MB's last reference (a later
+            // ticket of the same list, so never one this wave waits on).
+            publish_search(reinterpret_cast<uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref),
+                           o);
             continue;
         }
-        uint32_t w[8];
-        pack_search(o, w);
-        write_words(reinterpret_cast<uint32_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref),
-                    w, 8, true);
-        // This search's result is complete in memory before it is counted;
-        // the wave that counts last reads every result device-scope.
-        stores_done();
-        int old = 0;
-        if (lane == 0) {
-            old = __hip_atomic_fetch_add(job.mb_done + mb, 1, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (old == job.n_refs - 1)   // last one: back to zero for the next launch
-                st_relaxed(reinterpret_cast<uint32_t *>(job.mb_done + mb), 0u);
-        }
-        const int done_before = __builtin_amdgcn_readfirstlane(old);
-        if (done_before == job.n_refs - 1) finish_mb(job, bx, by, counters, sh.mb, s, geo, nullptr);
+        finish_mb(job, bx, by, counters, sh.mb, s, geo, o);
     }
 }
 
