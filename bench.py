@@ -77,9 +77,23 @@ def frame_bytes(w, h, n_refs):
     return pic * (1 + n_refs) + n * (384 * 2 * 2 + 64)
 
 
-def cpu_baseline(frames, gpu_out, sample, quality, width, height):
-    """Time the CPU path on `sample` frames (single thread) and compare its
-    outputs with the GPU's.  Uses the reference's own build when present."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(frames, gpu_out, sample, quality, width, height, threads=1, mt_sample=()):
+    """Time the CPU path on `sample` frames on one thread and, with the
+    reference's own build, on `mt_sample` frames on `threads` threads in the
+    reference's wavefront row order; compare every output with the GPU's.
+    Returns the multi-threaded figure (the single-threaded one nested) when
+    it was run."""
     from oracle.oracle import Oracle, Picture, Reference, reference_available
     if reference_available():
         impl, kind = Reference(), "reference"
@@ -92,32 +106,53 @@ def cpu_baseline(frames, gpu_out, sample, quality, width, height):
             pics[t] = Picture(*frames[t])
         return pics[t]
 
-    prev = gpu_out[sample[0] - 1][0]
-    exact = True
-    elapsed = 0.0
     fields = [n for n in gpu_out[sample[0]][0].dtype.names
               if n not in ("mad", "cost_bits", "mad_b", "cost_b_bits", "n_int", "n_axial",
                            "n_diag") and not n.startswith("reserved")]
-    for t in sample:
-        refs = [pic(r) for r in frame_refs(t)]
-        t0 = time.perf_counter()
-        rec, co, lv = impl.analyze_frame(1, quality, pic(t), refs, prev=prev)
-        elapsed += time.perf_counter() - t0
-        g = gpu_out[t]
-        same = all(np.array_equal(rec[f], g[0][f]) for f in fields)
-        same = same and np.array_equal(co, g[1]) and np.array_equal(lv, g[2])
-        exact = exact and same
-        log(f"cpu {kind} frame {t}: {elapsed:.1f}s so far, bit-exact={same}")
-        prev = rec
-    return {
-        "value": round(len(sample) / elapsed, 4),
-        "unit": "frames/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": (f"frames {sample[0]}-{sample[-1]} of the {width}x{height} sequence "
-                   f"({len(frame_refs(sample[0]))} refs each, quality {quality}), one thread, "
-                   f"{elapsed:.1f} s"),
-    }, exact
+
+    def run(frames_, n_threads):
+        prev = gpu_out[frames_[0] - 1][0]
+        exact, times = True, []
+        for t in frames_:
+            refs = [pic(r) for r in frame_refs(t)]
+            t0 = time.perf_counter()
+            if n_threads > 1:
+                rec, co, lv = impl.analyze_frame_wpp(1, quality, pic(t), refs, prev=prev,
+                                                     threads=n_threads)
+            else:
+                rec, co, lv = impl.analyze_frame(1, quality, pic(t), refs, prev=prev)
+            times.append(time.perf_counter() - t0)
+            g = gpu_out[t]
+            same = all(np.array_equal(rec[f], g[0][f]) for f in fields)
+            same = same and np.array_equal(co, g[1]) and np.array_equal(lv, g[2])
+            exact = exact and same
+            log(f"cpu {kind} frame {t} on {n_threads} thread(s): {sum(times):.1f}s so far, "
+                f"bit-exact={same}")
+            prev = rec
+        elapsed = sum(times)
+        ms = np.array(times) * 1e3
+        return {
+            "value": round(len(frames_) / elapsed, 4),
+            "unit": "frames/s",
+            "cores": n_threads,
+            "kind": kind,
+            "sample": (f"frames {frames_[0]}-{frames_[-1]} of the {width}x{height} sequence "
+                       f"({len(frame_refs(frames_[0]))} refs each, quality {quality}), "
+                       f"{n_threads} thread(s)"
+                       + (" in wavefront row order" if n_threads > 1 else "")
+                       + f", {elapsed:.1f} s"),
+            "latency_ms": {"p50": round(float(np.percentile(ms, 50)), 1),
+                           "p99": round(float(np.percentile(ms, 99)), 1)},
+        }, exact
+
+    single, exact = run(sample, 1)
+    if threads <= 1 or kind != "reference" or not mt_sample:
+        single["cpu_model"] = cpu_model()
+        return single, exact
+    multi, exact_mt = run(list(mt_sample), threads)
+    multi["cpu_model"] = cpu_model()
+    multi["single_thread"] = single
+    return multi, exact and exact_mt
 
 
 # 64 concurrent streams of uneven sizes: (width, height, count).
@@ -278,7 +313,8 @@ def run_plain(args):
 
     # Untimed pass: work accounting, outputs for the CPU comparison and
     # per-frame latency (enqueue -> device idle).
-    keep = set(range(args.cpu_start - 1, args.cpu_start + args.cpu_frames))
+    keep = set(range(args.cpu_start - 1,
+                     args.cpu_start + max(args.cpu_frames, args.cpu_mt_frames)))
     ops, outs, lat = {}, {}, []
     for t in seq:
         t0 = time.perf_counter()
@@ -354,7 +390,10 @@ def run_plain(args):
     }
     if not args.no_cpu:
         sample = list(range(args.cpu_start, args.cpu_start + args.cpu_frames))
-        cb, exact = cpu_baseline(frames, outs, sample, q, w, h)
+        mt_sample = [t for t in range(args.cpu_start, args.cpu_start + args.cpu_mt_frames)
+                     if t in outs]
+        cb, exact = cpu_baseline(frames, outs, sample, q, w, h, threads=args.cpu_threads,
+                                 mt_sample=mt_sample)
         result["cpu_baseline"] = cb
         result["bit_exact_vs_cpu"] = exact
         result["gpu_vs_cpu"] = round(result["value"] / cb["value"], 1)
@@ -604,6 +643,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="threads of the multi-threaded CPU baseline (the host's share of cores)")
+    ap.add_argument("--cpu-mt-frames", type=int, default=24)
     ap.add_argument("--cpu-frames", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -252,9 +252,19 @@ class Reference(_Lib):
                              C.c_int16, C.c_int16, C.c_void_p, I16P)
         self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
                              I16P, I16P)
+        self._frame_wpp = self.f("analyze_frame_wpp", C.c_int32, C.POINTER(FrameJob),
+                                 C.c_void_p, I16P, I16P, C.c_int32)
 
     def lam(self, q):
         return self._lambda(q)
+
+    def analyze_frame_wpp(self, ftype, quality, src, refs, prev=None, target_ids=None,
+                          threads=1):
+        """analyze_frame on `threads` threads in the reference's wavefront row
+        order (identical results)."""
+        def fn(job, rec, co, lv):
+            return self._frame_wpp(job, rec, co, lv, threads)
+        return _run_frame(fn, ftype, quality, src, refs, prev, target_ids)
 
     def interp_many(self, pic, bx, by, dx, dy, quarter):
         arrs = [np.ascontiguousarray(a, np.int32) for a in (bx, by, dx, dy, quarter)]

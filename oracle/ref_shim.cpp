@@ -23,7 +23,10 @@
 #include "quantize.h"
 #include "transform.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "evx_oracle.h"
@@ -309,106 +312,190 @@ int32_t evr_bidir(const int16_t *src, const evo_planes *fwd, const evo_planes *b
     return sad;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Per-frame state shared by the frame drivers: the source and reference
+// pictures (filled once), the frame descriptor and the block table.
+struct frame_state {
+    const evo_frame_job *job;
+    uint32 wb, hb;
+    image_set src_set;
+    evx_cache_bank refs;                  // prediction_cache[k] = reference k
+    evx_frame frame;
+    float lambda;
+    std::vector<evx_block_desc> table, prev;
+    const evx_block_desc *prev_ptr = nullptr;
+    evo_record *records;
+    int16_t *coeffs, *levels;
+
+    frame_state(const evo_frame_job *j, evo_record *r, int16_t *c, int16_t *l)
+        : job(j), records(r), coeffs(c), levels(l)
+    {
+        wb = j->src.width / 16;
+        hb = j->src.height / 16;
+        fill_set(&src_set, j->src);
+        for (int32 k = 0; k < j->n_refs; k++) fill_set(&refs.prediction_cache[k], j->refs[k]);
+        table.resize(wb * hb);
+        if (j->type == 1 && j->prev) {
+            prev.resize(wb * hb);
+            for (uint32 k = 0; k < wb * hb; k++) {
+                clear_block_desc(&prev[k]);
+                prev[k].block_type = (EVX_BLOCK_TYPE)j->prev[k].block_type;
+                prev[k].motion_x = j->prev[k].motion_x;
+                prev[k].motion_y = j->prev[k].motion_y;
+            }
+            prev_ptr = prev.data();
+        }
+        clear_frame(&frame);
+        frame.type = j->type == 1 ? EVX_FRAME_INTER : EVX_FRAME_BIDIR;
+        frame.quality = (uint16)j->quality;
+        lambda = compute_lambda(frame.quality);
+    }
+};
+
+// One MB: the searches, selection, residual prediction, DCT, QP and
+// quantisation, and its record.  `bank` holds the references (its own or
+// borrowed) and the worker's scratch blocks.  Slot numbers are the
+// reference indices; prediction_target is mapped to the caller's ids when
+// the record is written out.
+int32_t analyze_mb(frame_state &fs, evx_cache_bank *bank, uint32 bx, uint32 by)
+{
+    const evo_frame_job *job = fs.job;
+    const uint32 wb = fs.wb;
+    const uint32 idx = by * wb + bx;
+    const int32 x = bx * 16, y = by * 16;
+    macroblock src_block;
+    create_macroblock(fs.src_set, x, y, &src_block);
+    evx_block_desc best;
+    int32 best_sad = EVX_MAX_INT32;
+    bool best_copy = false;
+    int32 sad_b = 0;
+    if (job->type == 1) {
+        for (int32 k = 0; k < job->n_refs; k++) {
+            evx_block_desc d;
+            int32 sad = perform_inter_search(fs.frame, src_block, x, y, bank, k, fs.table.data(),
+                                             wb, idx, fs.prev_ptr, &d);
+            d.prediction_target = (uint8)k;
+            int32 bias = (int32)(fs.lambda * (float)((k + 1) * 2));
+            bool copy = EVX_IS_COPY_BLOCK_TYPE(d.block_type);
+            if (copy ^ best_copy) {
+                if (copy) { best = d; best_sad = sad; best_copy = true; }
+            } else if (sad + bias < best_sad) {
+                best = d; best_sad = sad; best_copy = copy;
+            }
+        }
+    } else {
+        evx_block_desc fd, bd, bi;
+        int32 fsad = perform_inter_search(fs.frame, src_block, x, y, bank, 0, fs.table.data(), wb,
+                                          idx, nullptr, &fd);
+        fd.prediction_mode = 0;
+        fd.prediction_target = 0;
+        if (fsad < best_sad) { best = fd; best_sad = fsad; }
+        int32 bsad = perform_inter_search(fs.frame, src_block, x, y, bank, 1, fs.table.data(), wb,
+                                          idx, nullptr, &bd);
+        bd.prediction_mode = 1;
+        bd.prediction_target = 1;
+        if (bsad < best_sad) { best = bd; best_sad = bsad; }
+        int32 bis = calculate_bidir_prediction(fs.frame, src_block, x, y, bank, 0, 1,
+                                               fs.table.data(), wb, idx, &bi);
+        int32 bias = (int32)(fs.lambda * 8.0f);
+        if (bis + bias < best_sad) { best = bi; best_sad = bis; sad_b = bsad; }
+    }
+    if (best_sad == EVX_MAX_INT32) return 10;
+    fs.table[idx] = best;
+
+    int16_t pred[384];
+    int16_t *c = fs.coeffs + (size_t)idx * 384, *l = fs.levels + (size_t)idx * 384;
+    evx_block_desc &d = fs.table[idx];
+    bool transformed = !EVX_IS_COPY_BLOCK_TYPE(d.block_type);
+    if (transformed) {
+        form_prediction(bank, d, x, y, pred);
+        macroblock mp = view(pred), mc = view(c), ml = view(l);
+        sub_transform_macroblock(src_block, mp, &mc);
+        d.q_index = query_block_quantization_parameter((uint8)fs.frame.quality, mc, d.block_type);
+        d.variance = compute_block_variance2(mc);
+        quantize_macroblock(d.q_index, d.block_type, mc, &ml);
+    } else {
+        memset(c, 0, 384 * sizeof(int16_t));
+        memset(l, 0, 384 * sizeof(int16_t));
+    }
+    evo_record *r = &fs.records[idx];
+    desc_to_record(d, r);
+    r->prediction_target = job->target_ids[d.prediction_target];
+    if (d.prediction_mode == 2) r->prediction_target_b = job->target_ids[d.prediction_target_b];
+    r->sad = best_sad;
+    if (d.prediction_mode == 2) r->sad_b = sad_b;
+    r->flags = (transformed ? 1u : 0u) | (d.prediction_mode == 2 ? 2u : 0u);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 // Frame driver over reference calls; the same composition as
-// evo_analyze_frame.  mad/cost/counter fields are left zero (the reference
-// does not expose them).
+// evo_analyze_frame, in raster order on the calling thread.  mad/cost/
+// counter fields are left zero (the reference does not expose them).
 int32_t evr_analyze_frame(const evo_frame_job *job, evo_record *records, int16_t *coeffs,
                           int16_t *levels)
 {
-    int32 w = job->src.width, h = job->src.height;
-    uint32 wb = w / 16, hb = h / 16;
+    frame_state fs(job, records, coeffs, levels);
     bank_holder bh;
-    evx_cache_bank *bank = &bh.bank;
-    image_set src_set;
-    fill_set(&src_set, job->src);
-    for (int32 k = 0; k < job->n_refs; k++) fill_set(&bank->prediction_cache[k], job->refs[k]);
-    // Slot numbers are the reference indices; prediction_target is mapped
-    // to the caller's ids when the records are written out.
-    std::vector<evx_block_desc> table(wb * hb), prev(wb * hb);
-    const evx_block_desc *prev_ptr = nullptr;
-    if (job->type == 1 && job->prev) {
-        for (uint32 k = 0; k < wb * hb; k++) {
-            clear_block_desc(&prev[k]);
-            prev[k].block_type = (EVX_BLOCK_TYPE)job->prev[k].block_type;
-            prev[k].motion_x = job->prev[k].motion_x;
-            prev[k].motion_y = job->prev[k].motion_y;
-        }
-        prev_ptr = prev.data();
-    }
-    evx_frame frame;
-    clear_frame(&frame);
-    frame.type = job->type == 1 ? EVX_FRAME_INTER : EVX_FRAME_BIDIR;
-    frame.quality = (uint16)job->quality;
-    float lambda = compute_lambda(frame.quality);
-    std::vector<int16_t> pred(384), co(384), lv(384);
-
-    for (uint32 by = 0; by < hb; by++)
-        for (uint32 bx = 0; bx < wb; bx++) {
-            uint32 idx = by * wb + bx;
-            int32 x = bx * 16, y = by * 16;
-            macroblock src_block;
-            create_macroblock(src_set, x, y, &src_block);
-            evx_block_desc best;
-            int32 best_sad = EVX_MAX_INT32;
-            bool best_copy = false;
-            int32 sad_b = 0;
-            if (job->type == 1) {
-                for (int32 k = 0; k < job->n_refs; k++) {
-                    evx_block_desc d;
-                    int32 sad = perform_inter_search(frame, src_block, x, y, bank, k, table.data(),
-                                                     wb, idx, prev_ptr, &d);
-                    d.prediction_target = (uint8)k;
-                    int32 bias = (int32)(lambda * (float)((k + 1) * 2));
-                    bool copy = EVX_IS_COPY_BLOCK_TYPE(d.block_type);
-                    if (copy ^ best_copy) {
-                        if (copy) { best = d; best_sad = sad; best_copy = true; }
-                    } else if (sad + bias < best_sad) {
-                        best = d; best_sad = sad; best_copy = copy;
-                    }
-                }
-            } else {
-                evx_block_desc fd, bd, bi;
-                int32 fs = perform_inter_search(frame, src_block, x, y, bank, 0, table.data(), wb,
-                                                idx, nullptr, &fd);
-                fd.prediction_mode = 0;
-                fd.prediction_target = 0;
-                if (fs < best_sad) { best = fd; best_sad = fs; }
-                int32 bs = perform_inter_search(frame, src_block, x, y, bank, 1, table.data(), wb,
-                                                idx, nullptr, &bd);
-                bd.prediction_mode = 1;
-                bd.prediction_target = 1;
-                if (bs < best_sad) { best = bd; best_sad = bs; }
-                int32 bis = calculate_bidir_prediction(frame, src_block, x, y, bank, 0, 1,
-                                                       table.data(), wb, idx, &bi);
-                int32 bias = (int32)(lambda * 8.0f);
-                if (bis + bias < best_sad) { best = bi; best_sad = bis; sad_b = bs; }
-            }
-            if (best_sad == EVX_MAX_INT32) return 10;
-            table[idx] = best;
-
-            int16_t *c = coeffs + (size_t)idx * 384, *l = levels + (size_t)idx * 384;
-            evx_block_desc &d = table[idx];
-            bool transformed = !EVX_IS_COPY_BLOCK_TYPE(d.block_type);
-            if (transformed) {
-                form_prediction(bank, d, x, y, pred.data());
-                macroblock mp = view(pred.data()), mc = view(c), ml = view(l);
-                sub_transform_macroblock(src_block, mp, &mc);
-                d.q_index = query_block_quantization_parameter((uint8)frame.quality, mc, d.block_type);
-                d.variance = compute_block_variance2(mc);
-                quantize_macroblock(d.q_index, d.block_type, mc, &ml);
-            } else {
-                memset(c, 0, 384 * sizeof(int16_t));
-                memset(l, 0, 384 * sizeof(int16_t));
-            }
-            evo_record *r = &records[idx];
-            desc_to_record(d, r);
-            r->prediction_target = job->target_ids[d.prediction_target];
-            if (d.prediction_mode == 2) r->prediction_target_b = job->target_ids[d.prediction_target_b];
-            r->sad = best_sad;
-            if (d.prediction_mode == 2) r->sad_b = sad_b;
-            r->flags = (transformed ? 1u : 0u) | (d.prediction_mode == 2 ? 2u : 0u);
+    for (int32 k = 0; k < job->n_refs; k++)
+        bh.bank.prediction_cache[k].borrow(fs.refs.prediction_cache[k]);
+    for (uint32 by = 0; by < fs.hb; by++)
+        for (uint32 bx = 0; bx < fs.wb; bx++) {
+            const int32_t rc = analyze_mb(fs, &bh.bank, bx, by);
+            if (rc) return rc;
         }
     return 0;
+}
+
+// The same frame on `n_threads` threads in wavefront order, as the
+// reference's encode_slice_wpp schedules rows (encode.cpp:817-941): workers
+// claim rows, each with a private cache bank borrowing the shared pictures,
+// and MB (bx, by) waits until row by - 1 has finished column
+// min(bx + 1, wb - 1) -- the top-right neighbour the MV predictor reads
+// (the reference waits for col + 4 because its intra search reads further).
+// Results are identical to evr_analyze_frame's.
+int32_t evr_analyze_frame_wpp(const evo_frame_job *job, evo_record *records, int16_t *coeffs,
+                              int16_t *levels, int32_t n_threads)
+{
+    frame_state fs(job, records, coeffs, levels);
+    const uint32 rows = fs.hb, cols = fs.wb;
+    std::vector<std::atomic<int32>> progress(rows);
+    for (auto &p : progress) p.store(-1, std::memory_order_relaxed);
+    std::atomic<uint32> next_row(0);
+    std::atomic<int32_t> error(0);
+    auto worker = [&]() {
+        bank_holder bh;
+        for (int32 k = 0; k < job->n_refs; k++)
+            bh.bank.prediction_cache[k].borrow(fs.refs.prediction_cache[k]);
+        for (;;) {
+            const uint32 row = next_row.fetch_add(1, std::memory_order_relaxed);
+            if (row >= rows) return;
+            for (uint32 col = 0; col < cols; col++) {
+                if (row > 0) {
+                    const int32 need = (int32)std::min(col + 1, cols - 1);
+                    while (progress[row - 1].load(std::memory_order_acquire) < need) {
+                        if (error.load(std::memory_order_relaxed)) return;
+                        std::this_thread::yield();
+                    }
+                }
+                const int32_t rc = analyze_mb(fs, &bh.bank, col, row);
+                if (rc) { error.store(rc); return; }
+                progress[row].store((int32)col, std::memory_order_release);
+            }
+        }
+    };
+    const int32_t n = std::max(1, std::min<int32_t>(n_threads, (int32_t)rows));
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < n; t++) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    return error.load();
 }
 
 }  // extern "C"

@@ -86,3 +86,22 @@ def test_frames_random(oracle, ref, ftype, nrefs, quality):
     for f in REF_FIELDS:
         assert np.array_equal(a[0][f], b[0][f]), f
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("ftype,nrefs,threads", [(1, 4, 3), (1, 2, 8), (2, 2, 5)])
+def test_frame_wavefront_threads_match_raster(ref, ftype, nrefs, threads):
+    """The reference's frame driver on several threads in wavefront row order
+    (the multi-threaded CPU baseline) gives the single-threaded results."""
+    frames = synth.sequence_c1(3000 + nrefs + threads, 6, 208, 144, local_range=14)
+    pics = [Picture(*f) for f in frames]
+    if ftype == 1:
+        cur, refs = pics[5], [pics[4 - k] for k in range(nrefs)]
+        prev = ref.analyze_frame(1, 8, pics[4], [pics[3]])[0]
+    else:
+        cur, refs, prev = pics[3], [pics[2], pics[5]], None
+    a = ref.analyze_frame(ftype, 9, cur, refs, prev=prev)
+    for rep in range(3):
+        b = ref.analyze_frame_wpp(ftype, 9, cur, refs, prev=prev, threads=threads)
+        for f in a[0].dtype.names:
+            assert np.array_equal(a[0][f], b[0][f]), f
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
