@@ -37,29 +37,45 @@ constexpr int kSpinLimit = 1 << 19;
 // back-off stays short; every poll is one memory instruction of four lanes.
 constexpr int kMaxNap = 8;
 
-// xftables.h:57-67: round(128 cos((2k+1) i pi / 16)).
-__constant__ int16_t c_dct8[64] = {
-    128, 128,  128,  128,  128,  128,  128,  128,
-    126, 106,   71,   25,  -25,  -71, -106, -126,
-    118,  49,  -49, -118, -118,  -49,   49,  118,
-    106, -25, -126,  -71,   71,  126,   25, -106,
-     91, -91,  -91,   91,   91,  -91,  -91,   91,
-     71, -126,  25,  106, -106,  -25,  126,  -71,
-     49, -118, 118,  -49,  -49,  118, -118,   49,
-     25,  -71, 106, -126,  126, -106,   71,  -25};
+// xftables.h:57-67: round(128 cos((2k+1) i pi / 16)), columns 0..3 of each
+// row; row i is symmetric (even i) or antisymmetric (odd i) about its middle,
+// c[i][7 - k] = (-1)^i c[i][k].  Compile-time constants, so they fold into
+// the multiplies.
+constexpr int kDct8Half[8][4] = {
+    {128, 128, 128, 128},
+    {126, 106, 71, 25},
+    {118, 49, -49, -118},
+    {106, -25, -126, -71},
+    {91, -91, -91, 91},
+    {71, -126, 25, 106},
+    {49, -118, 118, -49},
+    {25, -71, 106, -126}};
 
-// quantize.cpp:26-36
-__constant__ int16_t c_inter_qm[64] = {
-    16, 17, 18, 19, 20, 21, 22, 23, 17, 18, 19, 20, 21, 22, 23, 24,
-    18, 19, 20, 21, 22, 23, 24, 25, 19, 20, 21, 22, 23, 24, 26, 27,
-    20, 21, 22, 23, 25, 26, 27, 28, 21, 22, 23, 24, 26, 27, 28, 30,
-    22, 23, 24, 26, 27, 28, 30, 31, 23, 24, 25, 27, 28, 30, 31, 33};
+// Quantiser tables: the inter matrix (quantize.cpp:26-36), the dead-zone
+// widths (quantize.cpp:61-71) and, per matrix entry, the reciprocal
+// multiplier ceil(2^32 / qm) that rdiv_magic divides by.
+struct QuantTables {
+    int16_t qm[64];
+    uint8_t dz[64];
+    uint32_t qm_magic[64];
+};
 
-// quantize.cpp:61-71
-__constant__ uint8_t c_deadzone[64] = {
-    2, 2, 2, 3, 3, 3, 4, 4, 2, 2, 3, 3, 3, 4, 4, 4, 2, 3, 3, 3, 4, 4, 4, 4,
-    3, 3, 3, 4, 4, 4, 4, 4, 3, 3, 4, 4, 4, 4, 4, 4, 3, 4, 4, 4, 4, 4, 4, 4,
-    4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+constexpr QuantTables make_quant_tables()
+{
+    QuantTables t{
+        {16, 17, 18, 19, 20, 21, 22, 23, 17, 18, 19, 20, 21, 22, 23, 24,
+         18, 19, 20, 21, 22, 23, 24, 25, 19, 20, 21, 22, 23, 24, 26, 27,
+         20, 21, 22, 23, 25, 26, 27, 28, 21, 22, 23, 24, 26, 27, 28, 30,
+         22, 23, 24, 26, 27, 28, 30, 31, 23, 24, 25, 27, 28, 30, 31, 33},
+        {2, 2, 2, 3, 3, 3, 4, 4, 2, 2, 3, 3, 3, 4, 4, 4, 2, 3, 3, 3, 4, 4, 4, 4,
+         3, 3, 3, 4, 4, 4, 4, 4, 3, 3, 4, 4, 4, 4, 4, 4, 3, 4, 4, 4, 4, 4, 4, 4,
+         4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4},
+        {}};
+    for (int i = 0; i < 64; i++) t.qm_magic[i] = 0xFFFFFFFFu / (uint32_t)t.qm[i] + 1u;
+    return t;
+}
+
+__constant__ QuantTables c_quant = make_quant_tables();
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
@@ -1075,17 +1091,24 @@ __device__ void predict_mb(const DevPic &ref, const SearchOut &r, int16_t *out, 
     wave_sync();
 }
 
-// transform.cpp:264-284 on a line of 8 samples.
+// transform.cpp:264-284 on a line of 8 samples.  The 8-term products fold
+// the table's symmetry: even rows take sums v[k] + v[7 - k], odd rows
+// differences v[k] - v[7 - k] (the same integers; no term exceeds
+// 8 * 65535 * 128 in magnitude, so nothing wraps).
 __device__ __forceinline__ void dct_line(const int16_t *src, int sp, int16_t *dst, int dp)
 {
     int v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = src[k * sp];
+    int e[4], o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) { e[k] = v[k] + v[7 - k]; o[k] = v[k] - v[7 - k]; }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
+        const int *x = (i & 1) ? o : e;
         int t = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) t += v[k] * c_dct8[i * 8 + k];
+        for (int k = 0; k < 4; k++) t += x[k] * kDct8Half[i][k];
         t = (i == 0) ? (t * 45) / 128 : t / 2;
         // rounded_div(t, 128), evx_math.h:228-236
         t = t < 0 ? (t - 64) / 128 : (t + 64) / 128;
@@ -1160,15 +1183,19 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
     const int cqp = min(qp + 2, 63);
     const uint32_t m_luma = magic_of(qp << 1), m_chroma = magic_of(cqp << 1);
 
+    // One sample per lane per round, rolled: the rounds' divisions would
+    // otherwise all be in flight at once and raise the kernel's register
+    // count (and lower its occupancy).
+#pragma unroll 1
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         int pos, bq;
         uint32_t mq;
         if (i < 256) { pos = ((i >> 4) & 7) * 8 + (i & 7); bq = qp; mq = m_luma; }
         else { pos = (i - 256) & 63; bq = cqp; mq = m_chroma; }
         const int sv = res[i];
-        const int qm = c_inter_qm[pos];
-        const int qf = (int16_t)rdiv_magic(sv * 16, qm, magic_of(qm));
-        int w = c_deadzone[pos];
+        const int qm = c_quant.qm[pos];
+        const int qf = (int16_t)rdiv_magic(sv * 16, qm, c_quant.qm_magic[pos]);
+        int w = c_quant.dz[pos];
         if (bq >= 16) w = min(w + 1, 6);
         const int sg = (qf > 0) - (qf < 0);
         const int dz = (int16_t)((sg * bq * w) >> 1);
@@ -1453,9 +1480,11 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
 }
 
 // The persistent analysis kernel: NW waves per workgroup search one
-// (MB, reference) ticket together; wave 0 then finishes it.
+// (MB, reference) ticket together; wave 0 then finishes it.  Register
+// budgets: the one-wave kernel may take 256 VGPRs (2 waves per SIMD); the
+// two-wave kernel fits 168 without spills, so 3 waves share each SIMD.
 template <int NW>
-__global__ __launch_bounds__(64 * NW, 2) void analyze_kernel(const DevJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const DevJob *__restrict__ jobs,
                                                           const uint32_t *__restrict__ work,
                                                           int32_t *counters)
 {

@@ -22,6 +22,7 @@ therefore stay resident in HBM and only the request crosses the host.
 """
 from __future__ import annotations
 
+import gc
 import heapq
 import time
 from dataclasses import dataclass, field
@@ -220,6 +221,11 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
     lat, dev, batch, mbs_done = [], [], [], 0
     queue, run_t, klass = [], [], []
     srv = evxa.Server(ctx, max_inflight, max_frames)
+    # No cyclic garbage collection inside the real-time loop: a full
+    # collection over a large heap would stall submissions for milliseconds.
+    gc.collect()
+    gc_was_enabled = gc.isenabled()
+    gc.disable()
     t0 = evxa.clock() + 0.05
     i, n_sub = 0, 0
 
@@ -250,38 +256,42 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
                 life = by_sid[sid]
                 adm.release(life.width, life.height, live.pop(sid))
 
-    while i < len(arrivals) or tags:
-        now = evxa.clock() - t0
-        while i < len(arrivals) and arrivals[i][0] <= now:
-            t, sid, k = arrivals[i]
-            i += 1
-            if sid in refused:
-                continue
-            if sid not in live:
-                life = by_sid[sid]
-                s = adm.acquire(life.width, life.height)
-                if s is None:
-                    refused.add(sid)
+    try:
+        while i < len(arrivals) or tags:
+            now = evxa.clock() - t0
+            while i < len(arrivals) and arrivals[i][0] <= now:
+                t, sid, k = arrivals[i]
+                i += 1
+                if sid in refused:
                     continue
-                live[sid] = s
-            s = live[sid]
-            d = descs[k % pool_frames]
-            if k == 1:
-                s.reset_temporal()
-            tag = n_sub
-            n_sub += 1
-            tags[tag] = (sid, k, s.n_mbs)
-            outstanding[sid] = outstanding.get(sid, 0) + 1
-            srv.submit(s, d, tag, t0 + t)
-        drain()
-        release_left(now)
-        if i < len(arrivals):
-            wait = arrivals[i][0] - (evxa.clock() - t0)
-            if wait > 0:
-                time.sleep(min(wait, 0.0005))
-        else:
-            time.sleep(0.0002)
-    wall = evxa.clock() - t0
+                if sid not in live:
+                    life = by_sid[sid]
+                    s = adm.acquire(life.width, life.height)
+                    if s is None:
+                        refused.add(sid)
+                        continue
+                    live[sid] = s
+                s = live[sid]
+                d = descs[k % pool_frames]
+                if k == 1:
+                    s.reset_temporal()
+                tag = n_sub
+                n_sub += 1
+                tags[tag] = (sid, k, s.n_mbs)
+                outstanding[sid] = outstanding.get(sid, 0) + 1
+                srv.submit(s, d, tag, t0 + t)
+            drain()
+            release_left(now)
+            if i < len(arrivals):
+                wait = arrivals[i][0] - (evxa.clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.0005))
+            else:
+                time.sleep(0.0002)
+        wall = evxa.clock() - t0
+    finally:
+        if gc_was_enabled:
+            gc.enable()
     srv.close()
     lat = np.array(lat) * 1e3
     queue, run_t, klass = np.array(queue) * 1e3, np.array(run_t) * 1e3, np.array(klass)
