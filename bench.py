@@ -528,6 +528,21 @@ def run_serve(args):
                     duration=args.serve_seconds, quality=args.quality,
                     max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
                     capacity_mbps=cap, log=log)
+    # The same schedule at lower admitted loads: latency against throughput.
+    if cap is not None and args.serve_curve:
+        res["curve"] = []
+        for u in (float(x) for x in args.serve_curve.split(",")):
+            r = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
+                          duration=args.serve_seconds, quality=args.quality,
+                          max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
+                          capacity_mbps=full * u, log=log)
+            res["curve"].append({"util": u, "frames_per_s": r["frames_per_s"],
+                                 "frames_1080p_eq_per_s": r["frames_1080p_eq_per_s"],
+                                 "admitted_lives": r["admitted_lives"],
+                                 "latency_ms": r["latency_ms"],
+                                 "latency_ms_by_height": r["latency_ms_by_height"]})
+            log(f"serve curve util {u}: {r['frames_per_s']} frames/s, p99 "
+                f"{r['latency_ms']['p99'] if r['latency_ms'] else None} ms")
     res["workload"] = (f"serve: {args.serve_streams} live streams offered at 60 fps "
                        f"(50% 720p / 35% 1080p / 15% 1440p, mean life 20 s, 2 ms jitter), "
                        f"{args.serve_seconds:g} s schedule, 1 ref, quality {args.quality}, "
@@ -640,6 +655,8 @@ def main():
     ap.add_argument("--serve-capacity", type=float, default=0.0,
                     help="admission capacity in MB/s (0: measure it with a burst)")
     ap.add_argument("--serve-waves", type=int, default=2)
+    ap.add_argument("--serve-curve", default="0.15,0.3",
+                    help="further admitted-load fractions to run (latency against throughput)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=5)

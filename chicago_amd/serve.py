@@ -293,6 +293,12 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
         if gc_was_enabled:
             gc.enable()
     srv.close()
+    hbm = ctx.bytes_used
+    for s in live.values():
+        s.close()
+    for pool in adm.pools.values():
+        for s in pool.free:
+            s.close()
     lat = np.array(lat) * 1e3
     queue, run_t, klass = np.array(queue) * 1e3, np.array(run_t) * 1e3, np.array(klass)
 
@@ -320,6 +326,6 @@ def run(ctx, *, seed=1, n_live=200, duration=10.0, fps=60.0, jitter=0.002, mean_
         "mean_frames_per_launch": round(float(np.mean(batch)), 2) if batch else 0,
         "mean_launch_ms": round(float(np.mean(dev)), 3) if dev else 0,
         "pooled_device_streams": {f"{w}x{h}": p.opened for (w, h), p in adm.pools.items()},
-        "hbm_bytes": ctx.bytes_used,
+        "hbm_bytes": hbm,
     }
     return res
