@@ -731,12 +731,44 @@ __device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int 
     return false;
 }
 
+// Cross-wave hand-off without cache-wide fences.  The MI355X has one L2 per
+// XCD, and an agent-scope release or acquire fence writes back or
+// invalidates a whole L2 (every coefficient and level store in flight).  The
+// few words other waves read -- MB predictor words and search results -- are
+// instead written and read with device-scope (sc1) accesses, which go to the
+// coherent side of the L2s.  Each such word is one 64-bit access carrying
+// its own published bit, so a reader needs no flag, no ordering between
+// words and no wait on the writer's side.
+__device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A MB's predictor word: bit 63 = published, bits 32..39 = the block type's
+// motion flag (the only type bit the MV predictor reads), bits 0..31 = the
+// integer MV (x low, y high; record word 1).  Zero until published.
+__device__ __forceinline__ uint64_t pred_word(bool motion, uint32_t mv)
+{
+    return (1ull << 63) | ((uint64_t)(motion ? EVXA_BLOCK_MOTION : 0) << 32) | mv;
+}
+
+__device__ __forceinline__ uint32_t mv_word(int mvx, int mvy)
+{
+    return (uint32_t)(uint16_t)mvx | ((uint32_t)(uint16_t)mvy << 16);
+}
+
 // perform_inter_search (motion.cpp:567-679) for the MB at (x, y) against
 // `ref` (a pw x ph picture), predictor (pmx, pmy) as compute_mv_predictor
 // produced it.  All lanes of the NW waves call it with uniform arguments; it
 // returns uniform values.  `s` is the source MB and `pre` the reference block
 // at (x, y), both in the lane layout; `w` is the window already staged in
-// `sp` (any position).
+// `sp` (any position).  `early`, if given, is the MB's predictor word, to be
+// published as soon as it is final (single-reference P frames).
 //
 // Each lane group (16 lanes) evaluates one candidate position at a time,
 // one lane per 4x4 luma block plus 8 chroma samples.  With one wave, group
@@ -747,7 +779,8 @@ __device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int 
 template <int NW>
 __device__ SearchOut inter_search(const DevPic &ref, int pw, int ph, bool narrow, const Blk &s,
                                   const Geo &geo, const Blk &pre, WinPos w, int x, int y,
-                                  int thresh, float lambda, int pmx, int pmy, SubpelLds &sp)
+                                  int thresh, float lambda, int pmx, int pmy, SubpelLds &sp,
+                                  uint64_t *early = nullptr)
 {
     const int lane = lane_id();
     const int g = lane >> 4;
@@ -879,6 +912,14 @@ __device__ SearchOut inter_search(const DevPic &ref, int pw, int ph, bool narrow
                 parity ^= 1;
             }
         }
+
+        // Once the integer search has left the MB's own position, the block
+        // type has its motion flag whatever the sub-pel step finds, and the
+        // sub-pel step leaves the integer MV alone: with a single reference
+        // that is the MB's final predictor word, so dependent MBs can start
+        // during this MB's sub-pel round.
+        if (early && (b.x != x || b.y != y) && (NW == 1 || wave_id() == 0) && lane == 0)
+            st_relaxed64(early, pred_word(true, mv_word(b.x - x, b.y - y)));
 
         // perform_inter_subpixel_motion_search (motion.cpp:450-483): all
         // candidates interpolate around the integer best (motion.cpp:186, 226).
@@ -1225,31 +1266,6 @@ __device__ __forceinline__ void store_src(const Blk &s, const Geo &g, int16_t *d
     }
 }
 
-// Cross-wave hand-off without cache-wide fences.  The MI355X has one L2 per
-// XCD, and an agent-scope release or acquire fence writes back or
-// invalidates a whole L2 (every coefficient and level store in flight).  The
-// few words other waves read -- MB predictor words and search results -- are
-// instead written and read with device-scope (sc1) accesses, which go to the
-// coherent side of the L2s.  Each such word is one 64-bit access carrying
-// its own published bit, so a reader needs no flag, no ordering between
-// words and no wait on the writer's side.
-__device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// A MB's predictor word: bit 63 = published, bits 32..39 = block type,
-// bits 0..31 = MV (x low, y high; record word 1).  Zero until published.
-__device__ __forceinline__ uint64_t pred_word(int btype, uint32_t mv)
-{
-    return (1ull << 63) | ((uint64_t)(btype & 0xFF) << 32) | mv;
-}
-
 // compute_mv_predictor (motion.cpp:271-346) from the predictor words of
 // the left, top, top-right and top-left MBs (0 where absent) and the first
 // two words (type, MV) of the temporal table's record, combined in the
@@ -1435,7 +1451,7 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     const int mvx = best.best_x - x, mvy = best.best_y - y;
     const int mvx_b = bidir ? other.best_x - x : 0, mvy_b = bidir ? other.best_y - y : 0;
     w[0] = (uint32_t)btype | ((uint32_t)mode << 8) | ((uint32_t)tgt << 16) | ((uint32_t)tgt_b << 24);
-    w[1] = (uint32_t)(uint16_t)mvx | ((uint32_t)(uint16_t)mvy << 16);
+    w[1] = mv_word(mvx, mvy);
     w[2] = (uint32_t)(uint16_t)mvx_b | ((uint32_t)(uint16_t)mvy_b << 16);
     w[3] = (uint32_t)best.sp_en | ((uint32_t)best.sp_amt << 8) | ((uint32_t)best.sp_idx << 16) |
            ((uint32_t)(bidir ? other.sp_en : 0) << 24);
@@ -1456,7 +1472,7 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
 
     // Publish what the MV predictor of later MBs reads (block type and MV)
     // as one device-scope word: they poll it directly.
-    if (lane == 0) st_relaxed64(job.mvw + idx, pred_word(btype, w[1]));
+    if (lane == 0) st_relaxed64(job.mvw + idx, pred_word((btype & EVXA_BLOCK_MOTION) != 0, w[1]));
 
     int16_t *co = job.coeffs + (size_t)idx * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)idx * EVXA_MB_SAMPLES;
@@ -1562,8 +1578,9 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
         pmx = __builtin_amdgcn_readfirstlane(pmx);
         pmy = __builtin_amdgcn_readfirstlane(pmy);
         window_ready();
+        uint64_t *early = job.type == EVXA_FRAME_P && job.n_refs == 1 ? job.mvw + mb : nullptr;
         const SearchOut o = inter_search<NW>(rp, job.width, job.height, narrow, s, geo, pre, win,
-                                             x, y, job.thresh, job.lambda, pmx, pmy, sh.sp);
+                                             x, y, job.thresh, job.lambda, pmx, pmy, sh.sp, early);
         // Wave 0 finishes the ticket; the other waves go on to the next claim.
         if (NW > 1 && wave_id() != 0) continue;
         if (ref < job.n_refs - 1) {
