@@ -348,9 +348,12 @@ struct HalfPlanes {
     static constexpr int HR = B + 6, HC = B + 1;
     static constexpr int VR = B + 1, VC = B;
     static constexpr int DR = B + 1, DC = B + 1;
-    int16_t h[HR * HC];
-    int16_t v[VR * VC];
-    int16_t d[DR * DC];
+    // Each plane is padded to whole dwords plus one: blocks are read as
+    // aligned dwords, up to one dword past their last sample.
+    static constexpr int pad(int n) { return (n + 3) & ~1; }
+    alignas(4) int16_t h[pad(HR * HC)];
+    alignas(4) int16_t v[pad(VR * VC)];
+    alignas(4) int16_t d[pad(DR * DC)];
 };
 
 // Reference window staged in LDS.  A window around a base position b with
@@ -487,14 +490,29 @@ __device__ __forceinline__ void window_ready()
     __syncthreads();
 }
 
-// A candidate block from the staged window, in the lane layout.
+// Four consecutive samples from sample index i of an LDS array with a
+// dword-aligned base: three aligned dword reads and two funnel shifts, so
+// no LDS access straddles a dword whatever the parity of i.
+__device__ __forceinline__ void lds_load4(const int16_t *base, int i, uint32_t &lo, uint32_t &hi)
+{
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(base) + (i >> 1);
+    const uint32_t d0 = q[0], d1 = q[1], d2 = q[2];
+    const uint32_t sh = (uint32_t)(i & 1) << 4;
+    lo = __builtin_amdgcn_alignbit(d1, d0, sh);
+    hi = __builtin_amdgcn_alignbit(d2, d1, sh);
+}
+
+// A candidate block from the staged window, in the lane layout (window
+// rows have even strides, so a row's parity is its column's).
 __device__ __forceinline__ void load_cand_win(const SubpelLds &sp, const WinPos &w, const Geo &g,
                                               int cx, int cy, Blk &b)
 {
-    load_blk(sp.win.y + (cy - w.y0) * kWinLS + (cx - w.x0),
-             sp.win.u + ((cy >> 1) - w.cy0) * kWinCS + ((cx >> 1) - w.cx0),
-             sp.win.v + ((cy >> 1) - w.cy0) * kWinCS + ((cx >> 1) - w.cx0), kWinLS, kWinCS, 0, 0,
-             g, b);
+    const int ly = (cy - w.y0 + g.oy) * kWinLS + (cx - w.x0 + g.ox);
+#pragma unroll
+    for (int r = 0; r < 4; r++) lds_load4(sp.win.y, ly + r * kWinLS, b.y[2 * r], b.y[2 * r + 1]);
+    const int lc = ((cy >> 1) - w.cy0 + g.cr) * kWinCS + ((cx >> 1) - w.cx0 + g.cc);
+    lds_load4(sp.win.u, lc, b.u[0], b.u[1]);
+    lds_load4(sp.win.v, lc, b.v[0], b.v[1]);
 }
 
 // Six-tap outputs of an R x C plane into dst (row-major, row stride C):
@@ -561,8 +579,8 @@ __device__ __forceinline__ void build_d(HalfPlanes<B> &hp)
 
 // Where direction (dx, dy)'s half-pel block sits inside the planes.
 struct PlaneView {
-    const int16_t *base;
-    int stride;
+    const int16_t *arr;     // the plane (dword-aligned)
+    int off, stride;        // the block's first sample and row stride in it
 };
 
 template <int B>
@@ -571,10 +589,21 @@ __device__ __forceinline__ PlaneView plane_view(const HalfPlanes<B> &hp, int dx,
     using P = HalfPlanes<B>;
     const int xo = dx > 0 ? 0 : -1, yo = dy > 0 ? 0 : -1;
     PlaneView pv;
-    if (dy == 0) { pv.base = hp.h + 3 * P::HC + (xo + 1); pv.stride = P::HC; }
-    else if (dx == 0) { pv.base = hp.v + (yo + 1) * P::VC; pv.stride = P::VC; }
-    else { pv.base = hp.d + (yo + 1) * P::DC + (xo + 1); pv.stride = P::DC; }
+    if (dy == 0) { pv.arr = hp.h; pv.off = 3 * P::HC + (xo + 1); pv.stride = P::HC; }
+    else if (dx == 0) { pv.arr = hp.v; pv.off = (yo + 1) * P::VC; pv.stride = P::VC; }
+    else { pv.arr = hp.d; pv.off = (yo + 1) * P::DC + (xo + 1); pv.stride = P::DC; }
     return pv;
+}
+
+// One direction's half-pel block from its plane views, in the lane layout.
+__device__ __forceinline__ void load_view(const PlaneView &vy, const PlaneView &vu,
+                                          const PlaneView &vv, const Geo &g, Blk &b)
+{
+    const int ly = vy.off + g.oy * vy.stride + g.ox;
+#pragma unroll
+    for (int r = 0; r < 4; r++) lds_load4(vy.arr, ly + r * vy.stride, b.y[2 * r], b.y[2 * r + 1]);
+    lds_load4(vu.arr, vu.off + g.cr * vu.stride + g.cc, b.u[0], b.u[1]);
+    lds_load4(vv.arr, vv.off + g.cr * vv.stride + g.cc, b.v[0], b.v[1]);
 }
 
 struct SearchOut {
@@ -619,7 +648,7 @@ __device__ __forceinline__ void subpel_metrics(bool narrow, const Blk &s, const 
                                                int &sq, int &mq)
 {
     Blk h;
-    load_blk(vy.base, vu.base, vv.base, vy.stride, vu.stride, 0, 0, g, h);
+    load_view(vy, vu, vv, g, h);
     if (narrow) {
         const s16x2 one = (s16x2)(1);
         s16x2 sumh = (s16x2)(0), sumq = (s16x2)(0), mxh = (s16x2)(0), mxq = (s16x2)(0);
