@@ -619,6 +619,15 @@ struct SearchOut {
 __device__ __forceinline__ constexpr int slot_i(int k) { return (k < 4 ? k : k + 1) % 3 - 1; }
 __device__ __forceinline__ constexpr int slot_j(int k) { return (k < 4 ? k : k + 1) / 3 - 1; }
 
+// The same for a run-time k (0..7): the offsets + 1 as 2-bit fields of one
+// constant, three scalar instructions instead of a division by 3.
+constexpr uint32_t kSlotI = 0x9224u, kSlotJ = 0xA940u;
+__device__ __forceinline__ int slot_i_rt(int k) { return (int)((kSlotI >> (2 * k)) & 3u) - 1; }
+__device__ __forceinline__ int slot_j_rt(int k) { return (int)((kSlotJ >> (2 * k)) & 3u) - 1; }
+static_assert(((kSlotI >> 0) & 3) == 0 && ((kSlotI >> 8) & 3) == 2 && ((kSlotI >> 12) & 3) == 1 &&
+              ((kSlotJ >> 6) & 3) == 1 && ((kSlotJ >> 10) & 3) == 2 && ((kSlotJ >> 4) & 3) == 0,
+              "slot offset tables");
+
 // RD cost of a candidate (motion.cpp:135-145): fp32 lambda * bits, then
 // + (float)SATD, no contraction.  Costs are non-negative, so their IEEE bits
 // order like the values and the fold compares them as integers.
@@ -713,29 +722,47 @@ __device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int 
                                            int sad, int mad, int &n_int)
 {
     auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
+    auto take = [&](int k) {
+        b.x = base_x + slot_i_rt(k) * step;
+        b.y = base_y + slot_j_rt(k) * step;
+        b.sad = lane_value(sad, k);
+        b.mad = lane_value(mad, k);
+        b.ssd = dist(b.x, b.y);
+    };
     const uint32_t copym = (uint32_t)__ballot(valid && mad < thresh) & 0xFFu;
     if (copym) {
         const int p = __builtin_ctz(copym);
         n_int += __builtin_popcount(mask & ((2u << p) - 1u));
-        b.sad = lane_value(sad, p);
-        b.mad = lane_value(mad, p);
+        take(p);
         b.cost = (uint32_t)lane_value((int)cost, p);
-        b.x = base_x + slot_i(p) * step;
-        b.y = base_y + slot_j(p) * step;
-        b.ssd = dist(b.x, b.y);
         return true;                                   // motion.cpp:402-403
     }
     n_int += __builtin_popcount(mask);
-    // Minimum over the slots in lanes 0..15 (lanes 8..15 hold no slot).
+    // Minimum over the slots in lanes 0..7 (quad swaps and the half-row
+    // mirror reach all eight).
     uint32_t ml = valid ? cost : 0xFFFFFFFFu;
     ml = min(ml, (uint32_t)dpp<0xB1>((int)ml));
     ml = min(ml, (uint32_t)dpp<0x4E>((int)ml));
     ml = min(ml, (uint32_t)dpp<0x141>((int)ml));
-    ml = min(ml, (uint32_t)dpp<0x140>((int)ml));
     const uint32_t m = min(b.cost, (uint32_t)lane_value((int)ml, 0));
     const uint32_t eq = (uint32_t)__ballot(valid && cost == m) & 0xFFu;
     bool have = b.cost == m;
-    // Visit order as bits: slots 0..3, the centre (bit 4), slots 4..7.
+    b.cost = m;
+    // The usual cases.  The level's start stays best alone: only the centre
+    // (the start again) is visited, which can lower best_ssd to the true
+    // distance.  Or exactly one slot reaches the minimum: it is taken (no
+    // earlier element ties with it).
+    if (eq == 0) {
+        const int d = dist(base_x, base_y);
+        if (d < b.ssd && b.sad < kSadThreshold) b.ssd = d;
+        return false;
+    }
+    if (!have && (eq & (eq - 1u)) == 0) {
+        take(__builtin_ctz(eq));
+        return false;
+    }
+    // Ties: visit order as bits -- slots 0..3, the centre (bit 4), slots
+    // 4..7.
     uint32_t seq = (eq & 0xFu) | (have ? 0x10u : 0u) | ((eq & 0xF0u) << 1);
     Best c = b;
     while (seq) {
@@ -744,8 +771,8 @@ __device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int 
         int ex = base_x, ey = base_y, esad = b.sad, emad = b.mad;
         if (q != 4) {
             const int k = q < 4 ? q : q - 1;
-            ex = base_x + slot_i(k) * step;
-            ey = base_y + slot_j(k) * step;
+            ex = base_x + slot_i_rt(k) * step;
+            ey = base_y + slot_j_rt(k) * step;
             esad = lane_value(sad, k);
             emad = lane_value(mad, k);
         }
@@ -755,7 +782,6 @@ __device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int 
         }
         have = true;
     }
-    c.cost = m;
     b = c;
     return false;
 }
