@@ -23,7 +23,7 @@ import numpy as np
 # Candidate shapes per workload class.
 SINGLE = [dict(waves_per_search=w, lists=l) for w, l in itertools.product((1, 2), (1, 8))]
 BATCH = [dict(waves_per_search=w, lists=8, cap_pct=c)
-         for w, c in itertools.product((1, 2), (60, 100, 200, 400, 800))]
+         for w, c in itertools.product((1, 2), (100, 150, 200, 250, 300, 400, 800))]
 SERVE = [dict(waves_per_search=w) for w in (1, 2)]
 
 

@@ -350,11 +350,16 @@ int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per
             for (int32_t bx = 0; bx < j.wb; bx++)
                 prof[k][(size_t)row_list[k][by] * diagonals(j) + bx + 2 * by] += j.n_refs;
     }
-    // Starts: largest jobs first, each at the earliest start that keeps every
-    // list within the cap (or, failing that, at the end of what is placed).
+    // Starts: longest MB chains (most diagonals) first, then the most work,
+    // each at the earliest start that keeps every list within the cap (or,
+    // failing that, at the end of what is placed).  A frame cannot finish
+    // sooner than its chain of diagonals, so long chains that start late
+    // would set the end of the launch.
     std::vector<int32_t> order(jobs.size()), start(jobs.size(), 0);
     for (size_t k = 0; k < jobs.size(); k++) order[k] = (int32_t)k;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        if (diagonals(jobs[a]) != diagonals(jobs[b]))
+            return diagonals(jobs[a]) > diagonals(jobs[b]);
         return (int64_t)jobs[a].wb * jobs[a].hb * jobs[a].n_refs >
                (int64_t)jobs[b].wb * jobs[b].hb * jobs[b].n_refs;
     });
