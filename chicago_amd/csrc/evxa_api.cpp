@@ -480,6 +480,11 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     for (auto &j : jobs) {
         j.mvw = reinterpret_cast<uint64_t *>(prog + off);
         off += j.wb * j.hb * 2;
+        // A lone picture is latency-bound: the wave of whichever reference
+        // finishes last can select and publish at once, without waiting for
+        // the designated wave to see its result.  In batches the extra read
+        // per search costs throughput, so only the designated wave selects.
+        j.eager = jobs.size() == 1 && j.type == EVXA_FRAME_P && j.n_refs > 1;
     }
     const size_t jb = jobs.size() * sizeof(DevJob);
     r.h_jobs.ensure(jb);

@@ -56,6 +56,8 @@ struct DevJob {
     int16_t *levels;
     SearchRec *searches;               // EVXA_MAX_REFS per MB
     uint64_t *mvw;                     // per MB: predictor word, zeroed per launch
+    int32_t eager;                     // P with several references, alone in its launch:
+                                       // every reference's wave tries the selection
 };
 
 // Work-list entry: job (10 bits) | reference (2 bits) | MB index (20 bits).

@@ -1411,6 +1411,49 @@ __device__ __forceinline__ SearchOut unpack_search(uint64_t sw, int k)
     return o;
 }
 
+// encode.cpp:215-247 for a P MB, against an empty (non-copy, INT32_MAX)
+// sentinel: reference k's result from result(k).  Returns the index chosen.
+template <typename F>
+__device__ __forceinline__ int select_p(int n_refs, float lambda, F result, SearchOut &best,
+                                        int &best_sad)
+{
+    int best_k = -1;
+    bool best_copy = false;
+    best_sad = kInt32Max;
+    for (int k = 0; k < n_refs; k++) {
+        const SearchOut r = result(k);
+        const int bias = __float2int_rz(__fmul_rn(lambda, __int2float_rn((k + 1) * 2)));
+        const bool copy = (r.btype & EVXA_BLOCK_COPY) != 0;
+        if (copy != best_copy) {
+            if (copy) { best = r; best_k = k; best_sad = r.sad; best_copy = true; }
+        } else if (r.sad + bias < best_sad) {
+            best = r; best_k = k; best_sad = r.sad; best_copy = copy;
+        }
+    }
+    return best_k;
+}
+
+// One look (no waiting) at all of a P MB's search results; if every one is
+// there, select and publish the MB's predictor word (the value the
+// finishing wave publishes as well).
+__device__ __forceinline__ void try_publish(const DevJob &job, int mb, int x, int y)
+{
+    const int lane = lane_id();
+    const int n_words = 3 * job.n_refs;
+    const uint64_t *p = reinterpret_cast<const uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS) +
+                        (lane / 3) * 4 + lane % 3;
+    const uint64_t sw = lane < n_words ? ld_relaxed64(p) : 0;
+    if (__ballot(lane < n_words && (sw >> 63) == 0) != 0) return;
+    SearchOut best;
+    int best_sad;
+    if (select_p(job.n_refs, job.lambda, [&](int k) { return unpack_search(sw, k); }, best,
+                 best_sad) < 0)
+        return;
+    if (lane == 0)
+        st_relaxed64(job.mvw + mb, pred_word((best.btype & EVXA_BLOCK_MOTION) != 0,
+                                             mv_word(best.best_x - x, best.best_y - y)));
+}
+
 // Selection among a MB's searches, record publication, residual prediction,
 // DCT and quantisation; run by the wave that searched the MB's last
 // reference, which passes its own result in `own`.  The results of the
@@ -1440,7 +1483,9 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
             for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
             nap = min(nap * 2, kMaxNap);
         }
+        // Clear what was read, and this wave's own words if it published them.
         if (lane < n_words) st_relaxed64(p, 0ull);
+        if (job.eager && lane >= n_words && lane < n_words + 3) st_relaxed64(p, 0ull);
     }
     auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
 
@@ -1450,7 +1495,9 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     int n_int = 0, n_ax = 0, n_dg = 0;
     store_src(src, geo, sh.src);
     if (job.type == EVXA_FRAME_P) {
-        // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel.
+        // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel
+        // (the same rule as select_p, written out: the compiler schedules
+        // this path better that way).
         bool best_copy = false;
         for (int k = 0; k < job.n_refs; k++) {
             const SearchOut r = result(k);
@@ -1638,12 +1685,18 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
                                              x, y, job.thresh, job.lambda, pmx, pmy, sh.sp, early);
         // Wave 0 finishes the ticket; the other waves go on to the next claim.
         if (NW > 1 && wave_id() != 0) continue;
+        uint64_t *own_words =
+            reinterpret_cast<uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref);
         if (ref < job.n_refs - 1) {
             // Hand the result to the wave of the MB's last reference (a later
             // ticket of the same list, so never one this wave waits on).
-            publish_search(reinterpret_cast<uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref),
-                           o);
+            publish_search(own_words, o);
+            if (job.eager) try_publish(job, mb, x, y);
             continue;
+        }
+        if (job.eager) {
+            publish_search(own_words, o);
+            try_publish(job, mb, x, y);
         }
         finish_mb(job, bx, by, counters, sh.mb, s, geo, o);
     }
