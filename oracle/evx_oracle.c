@@ -740,3 +740,115 @@ int32_t evo_analyze_frame(const evo_frame_job *job, evo_record *records, int16_t
         }
     return 0;
 }
+
+/* ----------------------------------------------------- reconstruction -- */
+
+/* quantize.cpp:322-332 */
+static void dequantize_inter_8x8(int32_t qp, const int16_t *src, int32_t sp, int16_t *dst,
+                                 int32_t dp)
+{
+    for (int j = 0; j < 8; j++)
+        for (int k = 0; k < 8; k++) {
+            int32_t v = src[j * sp + k];
+            dst[j * dp + k] = (int16_t)(((2 * v) * kInterQm[j * 8 + k] * qp) / EVO_QUANT_SCALE);
+        }
+}
+
+void evo_dequantize_inter_mb(int32_t qp, const int16_t levels[EVO_MB_SAMPLES],
+                             int16_t coeffs[EVO_MB_SAMPLES])
+{
+    /* quantize.cpp:427-449 */
+    int32_t cqp = qp + EVO_CHROMA_QP_OFFSET;
+    if (cqp > EVO_MAX_QUANT_LEVELS - 1) cqp = EVO_MAX_QUANT_LEVELS - 1;
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        dequantize_inter_8x8(qp, levels + off, 16, coeffs + off, 16);
+    }
+    dequantize_inter_8x8(cqp, levels + 256, 8, coeffs + 256, 8);
+    dequantize_inter_8x8(cqp, levels + 320, 8, coeffs + 320, 8);
+}
+
+/* transform.cpp:330-347 (and 396-417 with the addend): sample i of a line
+ * from its 8 frequencies, each term truncated on its own. */
+static int32_t idct_sample(const int16_t *src, int32_t sp, int i)
+{
+    int32_t t = ((src[0] * kDct8[i]) * 45) / 128;
+    for (int k = 1; k < 8; k++) t += (src[k * sp] * kDct8[k * 8 + i]) / 2;
+    return rounded_div(t, 128);
+}
+
+static void inverse_transform_add_8x8(const int16_t *src, int32_t sp, const int16_t *add,
+                                      int32_t ap, int16_t *dst, int32_t dp)
+{
+    /* transform.cpp:419-433: vertical lines into a scratch block, then
+     * horizontal lines plus the addend. */
+    int16_t tmp[64];
+    for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++) tmp[i * 8 + j] = (int16_t)idct_sample(src + j, sp, i);
+    for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++)
+            dst[j * dp + i] = (int16_t)(idct_sample(tmp + j * 8, 1, i) + add[j * ap + i]);
+}
+
+void evo_inverse_transform_add_mb(const int16_t coeffs[EVO_MB_SAMPLES],
+                                  const int16_t pred[EVO_MB_SAMPLES],
+                                  int16_t out[EVO_MB_SAMPLES])
+{
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        inverse_transform_add_8x8(coeffs + off, 16, pred + off, 16, out + off, 16);
+    }
+    inverse_transform_add_8x8(coeffs + 256, 8, pred + 256, 8, out + 256, 8);
+    inverse_transform_add_8x8(coeffs + 320, 8, pred + 320, 8, out + 320, 8);
+}
+
+static int ref_of_target(const evo_frame_job *job, int32_t target)
+{
+    for (int k = 0; k < job->n_refs; k++)
+        if (job->target_ids[k] == target) return k;
+    return -1;
+}
+
+int32_t evo_reconstruct_frame(const evo_frame_job *job, const evo_record *records,
+                              const int16_t *levels, int16_t *out_y, int16_t *out_u,
+                              int16_t *out_v)
+{
+    int32_t w = job->src.width, h = job->src.height, wb = w / EVO_MB, hb = h / EVO_MB;
+    int16_t pred[EVO_MB_SAMPLES], pb[EVO_MB_SAMPLES], co[EVO_MB_SAMPLES], rec[EVO_MB_SAMPLES];
+    for (int32_t by = 0; by < hb; by++)
+        for (int32_t bx = 0; bx < wb; bx++) {
+            int32_t idx = by * wb + bx, x = bx * EVO_MB, y = by * EVO_MB;
+            const evo_record *r = &records[idx];
+            int k = ref_of_target(job, r->prediction_target);
+            if (k < 0) return 3;
+            evo_search_result s;
+            memset(&s, 0, sizeof(s));
+            s.motion_x = r->motion_x; s.motion_y = r->motion_y;
+            s.sp_enabled = r->sp_pred; s.sp_amount = r->sp_amount; s.sp_index = r->sp_index;
+            evo_predict_block(&job->refs[k], x, y, &s, pred);
+            if (r->prediction_mode == 2) {
+                /* decode.cpp:126-197, 272-330: average with the backward block. */
+                int kb = ref_of_target(job, r->prediction_target_b);
+                if (kb < 0) return 3;
+                s.motion_x = r->motion_x_b; s.motion_y = r->motion_y_b;
+                s.sp_enabled = r->sp_pred_b; s.sp_amount = r->sp_amount_b;
+                s.sp_index = r->sp_index_b;
+                evo_predict_block(&job->refs[kb], x, y, &s, pb);
+                for (int i = 0; i < EVO_MB_SAMPLES; i++)
+                    pred[i] = (int16_t)((pred[i] + pb[i] + 1) >> 1);
+            }
+            if (r->block_type & EVO_BT_COPY) {
+                memcpy(rec, pred, sizeof(rec));        /* decode.cpp:123-269 */
+            } else {
+                evo_dequantize_inter_mb(r->q_index, levels + (size_t)idx * EVO_MB_SAMPLES, co);
+                evo_inverse_transform_add_mb(co, pred, rec);
+            }
+            for (int j = 0; j < 16; j++)
+                memcpy(out_y + (size_t)(y + j) * w + x, rec + j * 16, 16 * sizeof(int16_t));
+            for (int j = 0; j < 8; j++) {
+                memcpy(out_u + (size_t)(y / 2 + j) * (w / 2) + x / 2, rec + 256 + j * 8, 8 * sizeof(int16_t));
+                memcpy(out_v + (size_t)(y / 2 + j) * (w / 2) + x / 2, rec + 320 + j * 8, 8 * sizeof(int16_t));
+            }
+        }
+    return 0;
+}

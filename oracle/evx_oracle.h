@@ -169,6 +169,32 @@ void evo_quantize_inter_mb(int32_t qp, const int16_t coeffs[EVO_MB_SAMPLES],
 int32_t evo_analyze_frame(const evo_frame_job *job, evo_record *records,
                           int16_t *coeffs, int16_t *levels);
 
+/* ---- Reconstruction of inter MBs (decode side) ---- */
+
+/* quantize.cpp:427-449 (EVX_ENABLE_LINEAR_QUANTIZATION 0) with 322-332: the
+ * inter inverse quantisation of an MB's levels, luma at qp and chroma at
+ * min(qp + 2, 63); each value ((2 * level) * qm * qp) / 16 stored as int16. */
+void evo_dequantize_inter_mb(int32_t qp, const int16_t levels[EVO_MB_SAMPLES],
+                             int16_t coeffs[EVO_MB_SAMPLES]);
+
+/* macroblock.h:290-295 -> transform.cpp:330-347, 396-433 and the 16x16 split
+ * of 572-582: the inverse DCT of each 8x8 block added to the prediction,
+ * stored as int16 (no clipping). */
+void evo_inverse_transform_add_mb(const int16_t coeffs[EVO_MB_SAMPLES],
+                                  const int16_t pred[EVO_MB_SAMPLES],
+                                  int16_t out[EVO_MB_SAMPLES]);
+
+/* decode.cpp:123-433 (the inter block types) for every MB of a frame the
+ * job describes: the prediction each record names (its reference found by
+ * prediction_target among job->target_ids; bidirectional records average
+ * both), plus, for transformed (non-copy) MBs, the inverse-quantised,
+ * inverse-transformed levels.  out planes: luma width x height with stride
+ * width, chroma width/2 x height/2.  Returns 0, or 3 (EVX_ERROR_INVALIDARG)
+ * if a record names a target the job does not hold. */
+int32_t evo_reconstruct_frame(const evo_frame_job *job, const evo_record *records,
+                              const int16_t *levels, int16_t *out_y, int16_t *out_u,
+                              int16_t *out_v);
+
 #ifdef __cplusplus
 }
 #endif

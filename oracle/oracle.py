@@ -105,7 +105,46 @@ class _Lib:
         return fn
 
 
-class Oracle(_Lib):
+class _Recon:
+    """Reconstruction entry points shared by the restatement and the
+    reference build (same names and argument layout)."""
+
+    def _bind_recon(self):
+        self._deq = self.f("dequantize_inter_mb", None, C.c_int32, I16P, I16P)
+        self._idct = self.f("inverse_transform_add_mb", None, I16P, I16P, I16P)
+        self._recon = self.f("reconstruct_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
+                             I16P, I16P, I16P, I16P)
+
+    def dequantize(self, qp, levels):
+        lv = np.ascontiguousarray(levels, np.int16)
+        out = np.empty(384, np.int16)
+        self._deq(qp, _ptr16(lv), _ptr16(out))
+        return out
+
+    def idct_add(self, coeffs, pred):
+        c, p = (np.ascontiguousarray(a, np.int16) for a in (coeffs, pred))
+        out = np.empty(384, np.int16)
+        self._idct(_ptr16(c), _ptr16(p), _ptr16(out))
+        return out
+
+    def reconstruct_frame(self, ftype, quality, src, refs, records, levels, target_ids=None):
+        """(y, u, v) planes of the frame `records` and `levels` describe
+        (src only gives the size)."""
+        job = make_job(ftype, quality, src, refs, None, target_ids)
+        w, h = src.width, src.height
+        y = np.zeros((h, w), np.int16)
+        u = np.zeros((h // 2, w // 2), np.int16)
+        v = np.zeros((h // 2, w // 2), np.int16)
+        rec = np.ascontiguousarray(records)
+        lv = np.ascontiguousarray(levels, np.int16)
+        rc = self._recon(C.byref(job), rec.ctypes.data, _ptr16(lv), _ptr16(y), _ptr16(u),
+                         _ptr16(v))
+        if rc != 0:
+            raise RuntimeError(f"reconstruct_frame failed: {rc}")
+        return y, u, v
+
+
+class Oracle(_Lib, _Recon):
     """The C restatement."""
     prefix = "evo_"
 
@@ -132,6 +171,7 @@ class Oracle(_Lib):
         self._quant = self.f("quantize_inter_mb", None, C.c_int32, I16P, I16P)
         self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
                              I16P, I16P)
+        self._bind_recon()
 
     def lam(self, q):
         return self._lambda(q)
@@ -228,7 +268,7 @@ def _run_frame(fn, ftype, quality, src, refs, prev, target_ids):
     return recs, coeffs, levels
 
 
-class Reference(_Lib):
+class Reference(_Lib, _Recon):
     """The reference's own functions (oracle/_ref, built from its sources)."""
     prefix = "evr_"
 
@@ -254,6 +294,7 @@ class Reference(_Lib):
                              I16P, I16P)
         self._frame_wpp = self.f("analyze_frame_wpp", C.c_int32, C.POINTER(FrameJob),
                                  C.c_void_p, I16P, I16P, C.c_int32)
+        self._bind_recon()
 
     def lam(self, q):
         return self._lambda(q)

@@ -453,6 +453,83 @@ int32_t evr_analyze_frame(const evo_frame_job *job, evo_record *records, int16_t
     return 0;
 }
 
+// The inter inverse quantisation of an MB (quantize.cpp:466-476 for a
+// non-intra block type) and the inverse transform added to a prediction
+// (macroblock.h:290-295), on MB-order buffers.
+void evr_dequantize_inter_mb(int32_t qp, const int16_t *levels, int16_t *coeffs)
+{
+    std::vector<int16_t> in(levels, levels + 384);
+    macroblock ms = view(in.data()), md = view(coeffs);
+    inverse_quantize_macroblock((uint8)qp, EVX_BLOCK_INTER_MOTION_DELTA, ms, &md);
+}
+
+void evr_inverse_transform_add_mb(const int16_t *coeffs, const int16_t *pred, int16_t *out)
+{
+    std::vector<int16_t> c(coeffs, coeffs + 384), p(pred, pred + 384);
+    macroblock mc = view(c.data()), mp = view(p.data()), mo = view(out);
+    inverse_transform_add_macroblock(mc, mp, &mo);
+}
+
+// Reconstruction of an analysed frame the way decode_block treats the inter
+// block types (decode.cpp:123-433): the prediction each record names, plus
+// the inverse-quantised, inverse-transformed levels for non-copy blocks.
+// Output planes are packed (luma stride width, chroma width / 2).
+int32_t evr_reconstruct_frame(const evo_frame_job *job, const evo_record *records,
+                              const int16_t *levels, int16_t *out_y, int16_t *out_u,
+                              int16_t *out_v)
+{
+    const int32 w = job->src.width, h = job->src.height;
+    const uint32 wb = w / 16, hb = h / 16;
+    bank_holder bh;
+    evx_cache_bank *bank = &bh.bank;
+    for (int32 k = 0; k < job->n_refs; k++) fill_set(&bank->prediction_cache[k], job->refs[k]);
+    auto slot_of = [&](uint8_t target) {
+        for (int32 k = 0; k < job->n_refs; k++)
+            if (job->target_ids[k] == target) return k;
+        return -1;
+    };
+    std::vector<int16_t> pred(384), lv(384), co(384), rec(384);
+    for (uint32 by = 0; by < hb; by++)
+        for (uint32 bx = 0; bx < wb; bx++) {
+            const uint32 idx = by * wb + bx;
+            const int32 x = bx * 16, y = by * 16;
+            const evo_record &r = records[idx];
+            evx_block_desc d;
+            clear_block_desc(&d);
+            d.block_type = (EVX_BLOCK_TYPE)r.block_type;
+            d.prediction_mode = r.prediction_mode;
+            const int32 k = slot_of(r.prediction_target);
+            const int32 kb = r.prediction_mode == 2 ? slot_of(r.prediction_target_b) : 0;
+            if (k < 0 || kb < 0) return 3;
+            d.prediction_target = (uint8)k;
+            d.prediction_target_b = (uint8)kb;
+            d.motion_x = r.motion_x; d.motion_y = r.motion_y;
+            d.motion_x_b = r.motion_x_b; d.motion_y_b = r.motion_y_b;
+            d.sp_pred = r.sp_pred; d.sp_amount = r.sp_amount; d.sp_index = r.sp_index;
+            d.sp_pred_b = r.sp_pred_b; d.sp_amount_b = r.sp_amount_b; d.sp_index_b = r.sp_index_b;
+            d.q_index = r.q_index;
+            form_prediction(bank, d, x, y, pred.data());
+            if (EVX_IS_COPY_BLOCK_TYPE(d.block_type)) {
+                rec = pred;
+            } else {
+                memcpy(lv.data(), levels + (size_t)idx * 384, 384 * sizeof(int16_t));
+                macroblock ml = view(lv.data()), mc = view(co.data()), mp = view(pred.data()),
+                           mr = view(rec.data());
+                inverse_quantize_macroblock(d.q_index, d.block_type, ml, &mc);
+                inverse_transform_add_macroblock(mc, mp, &mr);
+            }
+            for (int j = 0; j < 16; j++)
+                memcpy(out_y + (size_t)(y + j) * w + x, rec.data() + j * 16, 16 * sizeof(int16_t));
+            for (int j = 0; j < 8; j++) {
+                memcpy(out_u + (size_t)(y / 2 + j) * (w / 2) + x / 2, rec.data() + 256 + j * 8,
+                       8 * sizeof(int16_t));
+                memcpy(out_v + (size_t)(y / 2 + j) * (w / 2) + x / 2, rec.data() + 320 + j * 8,
+                       8 * sizeof(int16_t));
+            }
+        }
+    return 0;
+}
+
 // The same frame on `n_threads` threads in wavefront order, as the
 // reference's encode_slice_wpp schedules rows (encode.cpp:817-941): workers
 // claim rows, each with a private cache bank borrowing the shared pictures,

@@ -5,7 +5,8 @@ which oracle/Makefile (`make -C oracle ref`) compiles from the reference's own
 sources plus oracle/ref_shim.cpp.  Inputs are stored next to the outputs, so
 the fixtures never depend on a random generator at test time.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py recon    # recon.npz only
 """
 import os
 import sys
@@ -167,5 +168,44 @@ def main():
     save("frames.npz", **out)
 
 
+def recon_fixtures():
+    """recon.npz: inverse quantisation and inverse transform + add on random
+    vectors, and the reconstruction of the frames of frames.npz (their own
+    levels, and levels perturbed as a trellis pass would leave them)."""
+    ref = Reference()
+    rng = np.random.default_rng(20261204)
+    out = {}
+    n = 96
+    amps = np.array([(3, 40, 600, 32767)[k % 4] for k in range(n)])
+    lv = np.stack([rng.integers(-a, a + 1, 384) for a in amps]).astype(np.int16)
+    qp = rng.integers(1, 64, n).astype(np.int32)
+    co = np.stack([rng.integers(-a * 8, a * 8 + 1, 384).clip(-32768, 32767)
+                   for a in amps]).astype(np.int16)
+    pred = rng.integers(-300, 600, (n, 384)).astype(np.int16)
+    out["dq_levels"], out["dq_qp"] = lv, qp
+    out["dq_coeffs"] = np.stack([ref.dequantize(int(q), l) for q, l in zip(qp, lv)])
+    out["it_coeffs"], out["it_pred"] = co, pred
+    out["it_out"] = np.stack([ref.idct_add(c, p_) for c, p_ in zip(co, pred)])
+    g = np.load(os.path.join(HERE, "frames.npz"))
+    pics = [Picture(g[f"f{k}_y"], g[f"f{k}_u"], g[f"f{k}_v"]) for k in range(6)]
+    jobs = {"p1": (1, 8, 1, [0]), "p2": (1, 8, 2, [1, 0]), "p4": (1, 12, 5, [4, 3, 2, 1]),
+            "b": (2, 9, 3, [2, 4])}
+    for tag, (ftype, q, cur, refs) in jobs.items():
+        rec, lv0 = g[f"{tag}_records"], g[f"{tag}_levels"]
+        lv1 = lv0.copy()
+        mask = rng.random(lv1.shape) < 0.05
+        lv1[mask] += rng.integers(-2, 3, int(mask.sum())).astype(np.int16)
+        out[f"{tag}_levels_perturbed"] = lv1
+        for name, levels in (("own", lv0), ("perturbed", lv1)):
+            y, u, v = ref.reconstruct_frame(ftype, q, pics[cur], [pics[r] for r in refs], rec,
+                                            levels)
+            out[f"{tag}_{name}_y"], out[f"{tag}_{name}_u"], out[f"{tag}_{name}_v"] = y, u, v
+    save("recon.npz", **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["recon"]:
+        recon_fixtures()
+    else:
+        main()
+        recon_fixtures()

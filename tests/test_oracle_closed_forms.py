@@ -248,3 +248,65 @@ def test_variance_and_qp_closed_form(oracle):
     for q in (1, 8, 20, 40):
         delta = max(-6, min(6, (idx - q) >> 1))
         assert oracle.block_qp(q, co) == max(1, min(63, q + delta))
+
+
+def test_reconstruction_zero_levels_is_prediction(oracle):
+    """Zero levels dequantise to zero and transform to zero: the
+    reconstruction is the prediction itself (decode.cpp:101-121)."""
+    rng = np.random.default_rng(8)
+    pred = rng.integers(-50, 300, 384).astype(np.int16)
+    z = np.zeros(384, np.int16)
+    for qp in (1, 8, 31, 63):
+        assert not oracle.dequantize(qp, z).any()
+    assert np.array_equal(oracle.idct_add(z, pred), pred)
+
+
+def test_dequantize_dc_and_chroma_qp(oracle):
+    """quantize.cpp:322-332: the matrix entry at (0,0) is 16, so a DC level
+    l comes back as (2 l 16 qp) / 16 = 2 l qp; chroma uses min(qp + 2, 63)."""
+    for l in (1, -1, 7, -300):
+        for qp in (1, 8, 40, 62, 63):
+            lv = np.zeros(384, np.int16)
+            lv[0] = l
+            lv[256] = l
+            co = oracle.dequantize(qp, lv)
+            wrap = lambda v: (v + 32768) % 65536 - 32768     # int16 store
+            assert co[0] == wrap(2 * l * qp)
+            assert co[256] == wrap(2 * l * min(qp + 2, 63))
+            assert np.count_nonzero(co) == 2
+
+
+def test_inverse_dc_closed_form(oracle):
+    """A lone DC coefficient c: the vertical pass gives column 0 the value
+    d = rdiv(45 c, 128) in every row (the k = 0 term is (c 128 45) / 128),
+    the horizontal pass spreads rdiv(45 d, 128) over the block
+    (transform.cpp:330-347, 396-417)."""
+    def rdiv(n, d):
+        return int(_rounded_div(np.int64(n), np.int64(d)))
+    pred = np.arange(384, dtype=np.int16) % 97
+    for c in (1, 5, 64, -64, 1000, -2047):
+        co = np.zeros(384, np.int16)
+        co[0] = c
+        co[256 + 0] = c
+        out = oracle.idct_add(co, pred)
+        v = rdiv(45 * rdiv(45 * c, 128), 128)
+        blk = out[:256].reshape(16, 16)[:8, :8] - pred[:256].reshape(16, 16)[:8, :8]
+        assert (blk == v).all()
+        assert np.array_equal(out[:256].reshape(16, 16)[8:, :], pred[:256].reshape(16, 16)[8:, :])
+        assert (out[256:320] - pred[256:320] == v).all()
+        assert np.array_equal(out[320:], pred[320:])
+
+
+def test_inverse_undoes_forward(oracle):
+    """The integer inverse transform undoes the forward one up to rounding
+    (both scaled to an orthonormal DCT-II): a transposed or misindexed
+    table would leave errors of the order of the samples."""
+    rng = np.random.default_rng(9)
+    for amp in (20, 255, 1000):
+        src = rng.integers(-amp, amp + 1, 384).astype(np.int16)
+        z = np.zeros(384, np.int16)
+        co = oracle.sub_transform(src, z)
+        back = oracle.idct_add(co, z)
+        err = np.abs(back.astype(int) - src.astype(int))
+        # the per-term truncations leave errors of about 1% of the amplitude
+        assert err.max() <= 2 + amp / 50 and err.mean() <= 0.5 + amp / 200

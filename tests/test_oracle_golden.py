@@ -119,3 +119,31 @@ def test_frames(oracle):
     _check(oracle.analyze_frame(1, 12, pics[5], [pics[4], pics[3], pics[2], pics[1]], prev=r2[0]),
            g, "p4")
     _check(oracle.analyze_frame(2, 9, pics[3], [pics[2], pics[4]]), g, "b")
+
+
+def test_dequantize_and_inverse_transform():
+    """quantize.cpp:322-332, 427-449 and transform.cpp:330-433 against the
+    reference build's outputs (recon.npz)."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    g = load("recon.npz")
+    for qp, lv, want in zip(g["dq_qp"], g["dq_levels"], g["dq_coeffs"]):
+        assert np.array_equal(o.dequantize(int(qp), lv), want)
+    for co, pred, want in zip(g["it_coeffs"], g["it_pred"], g["it_out"]):
+        assert np.array_equal(o.idct_add(co, pred), want)
+
+
+@pytest.mark.parametrize("tag,ftype,q,cur,refs", [("p1", 1, 8, 1, [0]), ("p2", 1, 8, 2, [1, 0]),
+                                                  ("p4", 1, 12, 5, [4, 3, 2, 1]),
+                                                  ("b", 2, 9, 3, [2, 4])])
+def test_reconstruct_frames(oracle, tag, ftype, q, cur, refs):
+    """decode.cpp:123-433 composed from the reference's functions, on the
+    analysed frames of frames.npz with their own and perturbed levels."""
+    g, pics = _frame_fixture()
+    r = load("recon.npz")
+    rec = g[f"{tag}_records"]
+    for name, levels in (("own", g[f"{tag}_levels"]), ("perturbed", r[f"{tag}_levels_perturbed"])):
+        y, u, v = oracle.reconstruct_frame(ftype, q, pics[cur], [pics[k] for k in refs], rec, levels)
+        assert np.array_equal(y, r[f"{tag}_{name}_y"]), (tag, name)
+        assert np.array_equal(u, r[f"{tag}_{name}_u"]), (tag, name)
+        assert np.array_equal(v, r[f"{tag}_{name}_v"]), (tag, name)

@@ -105,3 +105,41 @@ def test_frame_wavefront_threads_match_raster(ref, ftype, nrefs, threads):
         for f in a[0].dtype.names:
             assert np.array_equal(a[0][f], b[0][f]), f
         assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_dequantize_idct_random(oracle, ref):
+    """Inverse quantisation and inverse transform + add against the
+    reference's own functions, over level and coefficient ranges that
+    include int16 wrap."""
+    rng = np.random.default_rng(404)
+    for k in range(300):
+        amp = (3, 40, 600, 32767)[k % 4]
+        lv = rng.integers(-amp, amp + 1, 384).astype(np.int16)
+        qp = int(rng.integers(1, 64))
+        assert np.array_equal(oracle.dequantize(qp, lv), ref.dequantize(qp, lv))
+        co = rng.integers(-amp * 8, amp * 8 + 1, 384).clip(-32768, 32767).astype(np.int16)
+        pred = rng.integers(-300, 600, 384).astype(np.int16)
+        assert np.array_equal(oracle.idct_add(co, pred), ref.idct_add(co, pred))
+
+
+@pytest.mark.parametrize("ftype,nrefs,quality", [(1, 1, 8), (1, 3, 10), (2, 2, 12)])
+def test_reconstruct_frame_random(oracle, ref, ftype, nrefs, quality):
+    """Reconstruction of an analysed frame, with its own levels and with
+    perturbed ones (as a trellis pass would leave them), against the
+    reference's decode composition."""
+    frames = synth.sequence_c1(5000 + nrefs + quality, 6, 144, 96, local_range=14)
+    pics = [Picture(*f) for f in frames]
+    if ftype == 1:
+        cur, refs = pics[5], [pics[4 - k] for k in range(nrefs)]
+    else:
+        cur, refs = pics[3], [pics[2], pics[5]]
+    rec, co, lv = oracle.analyze_frame(ftype, quality, cur, refs)
+    rng = np.random.default_rng(quality)
+    lv2 = lv.copy()
+    mask = rng.random(lv.shape) < 0.05
+    lv2[mask] += rng.integers(-2, 3, mask.sum()).astype(np.int16)
+    for levels in (lv, lv2):
+        a = oracle.reconstruct_frame(ftype, quality, cur, refs, rec, levels)
+        b = ref.reconstruct_frame(ftype, quality, cur, refs, rec, levels)
+        for pa, pb in zip(a, b):
+            assert np.array_equal(pa, pb)
