@@ -220,6 +220,9 @@ struct evxa_stream {
     bool temporal_valid = false;
     std::atomic<int32_t> inflight{0};   // frames queued on a server and not yet complete
     int32_t last_table = -1;
+    DevJob last_job{};     // the last analysed frame (for reconstruction)
+    int32_t last_refs[EVXA_MAX_REFS] = {-1, -1, -1, -1};
+    DevBuf recon_levels;   // levels handed in for reconstruction
     uint64_t bytes = 0;
 
     int16_t *slot_base(int32_t slot) const
@@ -297,6 +300,8 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     j.searches = s->searches.as<SearchRec>();
     if (advance) {
         s->last_table = out;
+        s->last_job = j;
+        for (int k = 0; k < EVXA_MAX_REFS; k++) s->last_refs[k] = k < d.n_refs ? d.ref_slots[k] : -1;
         if (d.type == EVXA_FRAME_P && d.update_temporal) {
             s->p_cur ^= 1;
             s->temporal_valid = true;
@@ -715,6 +720,30 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
     });
 }
 
+int evxa_download_picture(evxa_stream *s, int32_t slot, int16_t *y, int16_t *u, int16_t *v,
+                          int32_t stride_y, int32_t stride_c)
+{
+    return guarded([&] {
+        if (!s || !y || !u || !v) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        check_slot(s, slot);
+        if (stride_y < s->w || stride_c < s->w / 2) fail(EVXA_ERROR_INVALIDARG, "stride too small");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        set_device(ctx);
+        const DevPic p = s->pic(slot);
+        const PlaneGeom *g[3] = {&s->gy, &s->gc, &s->gc};
+        const int16_t *src[3] = {p.y, p.u, p.v};
+        int16_t *dst[3] = {y, u, v};
+        const int32_t strides[3] = {stride_y, stride_c, stride_c};
+        for (int k = 0; k < 3; k++)
+            check(hipMemcpy2DAsync(dst[k], (size_t)strides[k] * 2, src[k], (size_t)g[k]->pitch * 2,
+                                   (size_t)g[k]->w * 2, g[k]->h, hipMemcpyDeviceToHost, ctx->stream),
+                  "download plane");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
 int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc)
 {
     return guarded([&] {
@@ -814,6 +843,57 @@ int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeff
         if (levels)
             check(hipMemcpy(levels, s->levels.p, n * EVXA_MB_SAMPLES * 2, hipMemcpyDeviceToHost),
                   "read levels");
+    });
+}
+
+int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
+{
+    return guarded([&] {
+        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        check_slot(s, dst_slot);
+        const DevJob &j = s->last_job;
+        for (int k = 0; k < j.n_refs; k++) {
+            if (s->last_refs[k] == dst_slot)
+                fail(EVXA_ERROR_INVALIDARG, "destination slot is a reference of the frame");
+            for (int m = 0; m < k; m++)
+                if (((j.target_ids >> (8 * k)) & 0xFF) == ((j.target_ids >> (8 * m)) & 0xFF))
+                    fail(EVXA_ERROR_INVALIDARG, "reconstruction needs distinct target ids");
+        }
+        set_device(ctx);
+        const size_t n = (size_t)s->wb * s->hb;
+        const int16_t *lv = j.levels;
+        if (levels) {
+            if (!s->recon_levels.p) s->recon_levels.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
+            check(hipMemcpyAsync(s->recon_levels.p, levels, n * EVXA_MB_SAMPLES * sizeof(int16_t),
+                                 hipMemcpyHostToDevice, ctx->stream),
+                  "upload levels");
+            lv = s->recon_levels.as<int16_t>();
+        }
+        const DevPic dst = s->pic(dst_slot);
+        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+        check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
+        check(launch_recon(j, lv, dst, ctx->stream), "reconstruction kernel");
+        check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
+        ctx->ring_head++;
+        ctx->last_start = ev.first;
+        ctx->last_stop = ev.second;
+        // The slot is a picture like an uploaded one: apron and range flag.
+        int32_t *wide = s->wide.as<int32_t>() + dst_slot;
+        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
+        const PlaneGeom *g[3] = {&s->gy, &s->gc, &s->gc};
+        const int16_t *planes[3] = {dst.y, dst.u, dst.v};
+        for (int k = 0; k < 3; k++) {
+            int16_t *p0 = const_cast<int16_t *>(planes[k]);
+            check(launch_apron(p0, g[k]->pitch, g[k]->w, g[k]->h, ctx->stream), "apron kernel");
+            check(launch_range(p0, g[k]->pitch, g[k]->w, g[k]->h, wide, ctx->stream), "range kernel");
+        }
+        // The caller's level buffer has been consumed once this returns.
+        if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
 }
 

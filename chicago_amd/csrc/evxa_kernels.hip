@@ -1702,6 +1702,136 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
     }
 }
 
+// ------------------------------------------------------ reconstruction --
+
+// The 8-point cosine table (xftables.h:57-67), row k = frequency, column i =
+// sample, as the inverse transform indexes it.
+constexpr int kDct8Full[8][8] = {
+    {128, 128, 128, 128, 128, 128, 128, 128},
+    {126, 106, 71, 25, -25, -71, -106, -126},
+    {118, 49, -49, -118, -118, -49, 49, 118},
+    {106, -25, -126, -71, 71, 126, 25, -106},
+    {91, -91, -91, 91, 91, -91, -91, 91},
+    {71, -126, 25, 106, -106, -25, 126, -71},
+    {49, -118, 118, -49, -49, 118, -118, 49},
+    {25, -71, 106, -126, 126, -106, 71, -25}};
+
+// transform.cpp:330-347 (and 396-417 before the addend): sample i of an
+// 8-point line from its frequencies v[0..7], every term truncated on its
+// own, then rounded_div(., 128) (evx_math.h:228-236).
+__device__ __forceinline__ int idct_sample(const int (&v)[8], int i)
+{
+    int t = ((v[0] * kDct8Full[0][i]) * 45) / 128;
+#pragma unroll
+    for (int k = 1; k < 8; k++) t += (v[k] * kDct8Full[k][i]) / 2;
+    return t < 0 ? (t - 64) / 128 : (t + 64) / 128;
+}
+
+constexpr int kReconMbs = 4;           // MBs (one wave each) per workgroup
+
+struct ReconLds {
+    int16_t pred[EVXA_MB_SAMPLES];
+    int16_t pred_b[EVXA_MB_SAMPLES];
+    int16_t co[EVXA_MB_SAMPLES];
+    int16_t tmp[16 * 21 + 2 * 8 * 13];  // diagonal H pass, then the vertical IDCT pass
+};
+
+// Reconstruction of the inter MBs of an analysed frame into `dst`, as the
+// reference decoder rebuilds them (decode.cpp:123-433): the prediction the
+// record names (the reference whose target id it carries; bidirectional
+// records average both, decode.cpp:126-197, 272-330), plus, for transformed
+// blocks, the levels inverse-quantised (quantize.cpp:322-332, 427-449) and
+// inverse-transformed onto it (macroblock.h:290-295, transform.cpp:419-433,
+// 572-582), stored as int16.  One wave per MB; MBs are independent.
+__global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
+                                                               const int16_t *__restrict levels,
+                                                               DevPic dst)
+{
+    __shared__ ReconLds lds[kReconMbs];
+    const int lane = lane_id();
+    const int mb = blockIdx.x * kReconMbs + wave_id();
+    if (mb >= job.wb * job.hb) return;   // the whole wave; no workgroup barriers follow
+    ReconLds &sh = lds[wave_id()];
+    const int x = (mb % job.wb) * 16, y = (mb / job.wb) * 16;
+    const uint32_t *rw = reinterpret_cast<const uint32_t *>(job.records + mb);
+    const uint32_t w0 = rw[0], w1 = rw[1], w2 = rw[2], w3 = rw[3], w4 = rw[4];
+    const int btype = w0 & 0xFF, mode = (w0 >> 8) & 0xFF;
+    auto ref_of = [&](uint32_t target) {
+        int k = 0;
+        for (int r = job.n_refs - 1; r >= 0; r--)
+            if (((job.target_ids >> (8 * r)) & 0xFF) == target) k = r;
+        return k;
+    };
+    SearchOut o{};
+    o.best_x = x + (int16_t)(w1 & 0xFFFF);
+    o.best_y = y + (int16_t)(w1 >> 16);
+    o.sp_en = w3 & 0xFF; o.sp_amt = (w3 >> 8) & 0xFF; o.sp_idx = (w3 >> 16) & 0xFF;
+    predict_mb(job.refs[ref_of((w0 >> 16) & 0xFF)], o, sh.pred, sh.tmp);
+    if (mode == 2) {
+        SearchOut b{};
+        b.best_x = x + (int16_t)(w2 & 0xFFFF);
+        b.best_y = y + (int16_t)(w2 >> 16);
+        b.sp_en = w3 >> 24; b.sp_amt = w4 & 0xFF; b.sp_idx = (w4 >> 8) & 0xFF;
+        predict_mb(job.refs[ref_of(w0 >> 24)], b, sh.pred_b, sh.tmp);
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64)
+            sh.pred[i] = (int16_t)((sh.pred[i] + sh.pred_b[i] + 1) >> 1);
+        wave_sync();
+    }
+    const bool coded = (btype & EVXA_BLOCK_COPY) == 0;
+    if (coded) {
+        // Inverse quantisation, luma at qp and chroma at min(qp + 2, 63).
+        const int qp = (w4 >> 16) & 0xFF, cqp = min(qp + 2, 63);
+        const int16_t *lv = levels + (size_t)mb * EVXA_MB_SAMPLES;
+#pragma unroll 1
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+            const int pos = i < 256 ? ((i >> 4) & 7) * 8 + (i & 7) : (i - 256) & 63;
+            const int q = i < 256 ? qp : cqp;
+            sh.co[i] = (int16_t)(((2 * (int)lv[i]) * c_quant.qm[pos] * q) / 16);
+        }
+        wave_sync();
+        // Vertical pass: lane = (8x8 block, column) into tmp.
+        if (lane < 48) {
+            int origin, pitch;
+            block_origin(lane >> 3, origin, pitch);
+            const int c = origin + (lane & 7);
+            int v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = sh.co[c + k * pitch];
+#pragma unroll
+            for (int i = 0; i < 8; i++) sh.tmp[c + i * pitch] = (int16_t)idct_sample(v, i);
+        }
+        wave_sync();
+    }
+    // Horizontal pass plus the prediction (or the prediction alone): lane =
+    // (8x8 block, row), eight samples stored as one 16-byte row segment.
+    if (lane < 48) {
+        const int blk = lane >> 3, j = lane & 7;
+        int origin, pitch;
+        block_origin(blk, origin, pitch);
+        const int r = origin + j * pitch;
+        int v[8];
+        if (coded) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = sh.tmp[r + k];
+        }
+        uint32_t out[4];
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            int a = sh.pred[r + i], b2 = sh.pred[r + i + 1];
+            if (coded) { a += idct_sample(v, i); b2 += idct_sample(v, i + 1); }
+            out[i / 2] = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b2 << 16);
+        }
+        // DevPic views are read-only for the analysis; the destination slot
+        // is this kernel's output.
+        int16_t *row = const_cast<int16_t *>(
+            blk < 4 ? dst.y + (size_t)(y + (blk >> 1) * 8 + j) * dst.py + x + (blk & 1) * 8
+                    : (blk == 4 ? dst.u : dst.v) + (size_t)(y / 2 + j) * dst.pc + x / 2);
+        uint4 q;
+        q.x = out[0]; q.y = out[1]; q.z = out[2]; q.w = out[3];
+        __builtin_memcpy(row, &q, 16);
+    }
+}
+
 __global__ void reset_kernel(int32_t *words, int32_t n)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -1823,6 +1953,15 @@ hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *cou
         hipLaunchKernelGGL(analyze_kernel<2>, dim3(grid), dim3(128), 0, st, jobs, work, counters);
     else
         hipLaunchKernelGGL(analyze_kernel<1>, dim3(grid), dim3(64), 0, st, jobs, work, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_recon(const DevJob &job, const int16_t *levels, const DevPic &dst,
+                        hipStream_t st)
+{
+    const int n = job.wb * job.hb;
+    hipLaunchKernelGGL(recon_kernel, dim3((n + kReconMbs - 1) / kReconMbs), dim3(64 * kReconMbs),
+                       0, st, job, levels, dst);
     return hipGetLastError();
 }
 

@@ -74,7 +74,7 @@ EXPORTS = (
     "evxa_context_bytes", "evxa_lambda", "evxa_stream_open", "evxa_stream_close",
     "evxa_upload_picture", "evxa_stream_reset_temporal", "evxa_analyze_frame",
     "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_launch_times",
-    "evxa_read_results",
+    "evxa_read_results", "evxa_download_picture", "evxa_reconstruct",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
@@ -120,6 +120,8 @@ def load_library(path: str = LIB_PATH):
         "evxa_launch_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32,
                                         C.POINTER(C.c_int32)]),
         "evxa_read_results": (C.c_int, [vp, vp, I16P, I16P]),
+        "evxa_download_picture": (C.c_int, [vp, C.c_int32, I16P, I16P, I16P, C.c_int32, C.c_int32]),
+        "evxa_reconstruct": (C.c_int, [vp, C.c_int32, I16P]),
         "evxa_graph_create": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32,
                                         C.POINTER(C.c_int32)]),
         "evxa_graph_launch": (C.c_int, [vp, C.c_int32]),
@@ -311,6 +313,24 @@ class Stream:
         assert u.shape == v.shape == (self.height // 2, self.width // 2)
         _check(_lib.evxa_upload_picture(self.h, slot, _p16(y), _p16(u), _p16(v), self.width,
                                         self.width // 2))
+
+    def download(self, slot):
+        """(y, u, v) planes of picture slot `slot`."""
+        y = np.empty((self.height, self.width), np.int16)
+        u = np.empty((self.height // 2, self.width // 2), np.int16)
+        v = np.empty((self.height // 2, self.width // 2), np.int16)
+        _check(_lib.evxa_download_picture(self.h, slot, _p16(y), _p16(u), _p16(v), self.width,
+                                          self.width // 2))
+        return y, u, v
+
+    def reconstruct(self, dst_slot, levels=None):
+        """Rebuild the last analysed frame into `dst_slot` (prediction plus
+        the inverse-quantised, inverse-transformed `levels`, n_mbs x 384 in
+        MB order; None: the stream's own levels)."""
+        if levels is not None:
+            levels = np.ascontiguousarray(levels, np.int16)
+            assert levels.shape == (self.n_mbs, MB_SAMPLES)
+        _check(_lib.evxa_reconstruct(self.h, dst_slot, None if levels is None else _p16(levels)))
 
     def reset_temporal(self):
         _check(_lib.evxa_stream_reset_temporal(self.h))

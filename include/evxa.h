@@ -127,6 +127,12 @@ int evxa_stream_close(evxa_stream *s);
 /* Copy a host picture (int16 planes, strides in elements) into `slot`. */
 int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const int16_t *u,
                         const int16_t *v, int32_t stride_y, int32_t stride_c);
+
+/* Copy picture slot `slot` (luma w x h, chroma w/2 x h/2 interiors) to host
+ * planes with the given strides (elements); blocks until done.  With
+ * evxa_reconstruct it returns the decoded picture. */
+int evxa_download_picture(evxa_stream *s, int32_t slot, int16_t *y, int16_t *u, int16_t *v,
+                          int32_t stride_y, int32_t stride_c);
 /* Forget the temporal MV table (next P frame has no temporal candidate). */
 int evxa_stream_reset_temporal(evxa_stream *s);
 
@@ -152,6 +158,20 @@ int evxa_launch_times(evxa_context *ctx, float *ms, int32_t max, int32_t *n_out)
  * levels per MB (zero for copy blocks).  Blocks until they are available. */
 int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeffs,
                       int16_t *levels);
+
+/* Rebuild the last analysed frame of `s` into picture slot `dst_slot`, as
+ * the reference decoder rebuilds inter MBs (decode_block, decode.cpp:31-433,
+ * for the inter block types): the prediction each record names plus, for
+ * transformed MBs, the levels inverse-quantised (quantize.cpp:427-449) and
+ * inverse-transformed onto it (macroblock.h:290-295), stored as int16.
+ * `levels`: n_mbs x 384 int16 in MB order on the host (for instance after
+ * a trellis pass on the CPU), or NULL for the stream's own pre-trellis
+ * levels.  The slot then serves as a reference like an uploaded picture,
+ * so the decoded picture buffer stays on the device.  dst_slot must not be
+ * a reference of that frame, and its references need distinct target ids.
+ * Replaces the encoder's per-MB reconstruction call, decode_block after
+ * quantisation (encode.cpp:806, 918 in the row workers). */
+int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels);
 
 /* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
 
