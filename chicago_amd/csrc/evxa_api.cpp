@@ -709,12 +709,13 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
                                    (size_t)strides[k] * 2, (size_t)g[k]->w * 2, g[k]->h,
                                    hipMemcpyHostToDevice, ctx->stream),
                   "upload plane");
-            check(launch_apron(dst[k] + g[k]->origin(), g[k]->pitch, g[k]->w, g[k]->h, ctx->stream),
-                  "apron kernel");
             check(launch_range(dst[k] + g[k]->origin(), g[k]->pitch, g[k]->w, g[k]->h, wide,
                                ctx->stream),
                   "range kernel");
         }
+        check(launch_apron3(py + s->gy.origin(), pu + s->gc.origin(), pv + s->gc.origin(),
+                            s->gy.pitch, s->gc.pitch, s->w, s->h, ctx->stream),
+              "apron kernel");
         // Host buffers are the caller's: finish before returning.
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
@@ -874,24 +875,21 @@ int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
             lv = s->recon_levels.as<int16_t>();
         }
         const DevPic dst = s->pic(dst_slot);
+        int32_t *wide = s->wide.as<int32_t>() + dst_slot;
+        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
         if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
         const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
         check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
-        check(launch_recon(j, lv, dst, ctx->stream), "reconstruction kernel");
+        check(launch_recon(j, lv, dst, wide, ctx->stream), "reconstruction kernel");
         check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
         ctx->ring_head++;
         ctx->last_start = ev.first;
         ctx->last_stop = ev.second;
-        // The slot is a picture like an uploaded one: apron and range flag.
-        int32_t *wide = s->wide.as<int32_t>() + dst_slot;
-        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
-        const PlaneGeom *g[3] = {&s->gy, &s->gc, &s->gc};
-        const int16_t *planes[3] = {dst.y, dst.u, dst.v};
-        for (int k = 0; k < 3; k++) {
-            int16_t *p0 = const_cast<int16_t *>(planes[k]);
-            check(launch_apron(p0, g[k]->pitch, g[k]->w, g[k]->h, ctx->stream), "apron kernel");
-            check(launch_range(p0, g[k]->pitch, g[k]->w, g[k]->h, wide, ctx->stream), "range kernel");
-        }
+        // The slot is a picture like an uploaded one: refresh its apron (the
+        // range flag came with the reconstruction).
+        check(launch_apron3(const_cast<int16_t *>(dst.y), const_cast<int16_t *>(dst.u),
+                            const_cast<int16_t *>(dst.v), dst.py, dst.pc, s->w, s->h, ctx->stream),
+              "apron kernel");
         // The caller's level buffer has been consumed once this returns.
         if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });

@@ -94,11 +94,14 @@ hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st);
 // `waves` (1 or 2) per workgroup; `grid` workgroups.
 hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
                           int32_t grid, int32_t waves, hipStream_t st);
-hipError_t launch_apron(int16_t *plane0, int32_t pitch, int32_t w, int32_t h, hipStream_t st);
 // Reconstruction of the MBs of `job` (its records and prediction sources)
-// from `levels` into the picture `dst`.
+// from `levels` into the picture `dst`; ORs 1 into *wide if a stored sample
+// lies outside [0, 2047] (clear it first).
 hipError_t launch_recon(const DevJob &job, const int16_t *levels, const DevPic &dst,
-                        hipStream_t st);
+                        int32_t *wide, hipStream_t st);
+// The aprons of a picture's three planes (interior origins, luma w x h).
+hipError_t launch_apron3(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int32_t pc, int32_t w,
+                         int32_t h, hipStream_t st);
 // ORs 1 into *wide if a sample of the w x h plane lies outside [0, 2047].
 hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,
                         hipStream_t st);

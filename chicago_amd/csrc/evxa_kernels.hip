@@ -1745,7 +1745,7 @@ struct ReconLds {
 // 572-582), stored as int16.  One wave per MB; MBs are independent.
 __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
                                                                const int16_t *__restrict levels,
-                                                               DevPic dst)
+                                                               DevPic dst, int32_t *wide)
 {
     __shared__ ReconLds lds[kReconMbs];
     const int lane = lane_id();
@@ -1815,12 +1815,18 @@ __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
             for (int k = 0; k < 8; k++) v[k] = sh.tmp[r + k];
         }
         uint32_t out[4];
+        bool outside = false;             // a stored sample outside [0, 2047]
 #pragma unroll
         for (int i = 0; i < 8; i += 2) {
             int a = sh.pred[r + i], b2 = sh.pred[r + i + 1];
             if (coded) { a += idct_sample(v, i); b2 += idct_sample(v, i + 1); }
+            const int sa = (int16_t)a, sb = (int16_t)b2;
+            outside = outside || sa < 0 || sa > 2047 || sb < 0 || sb > 2047;
             out[i / 2] = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b2 << 16);
         }
+        // The picture's range flag (range_kernel's rule), one atomic per
+        // wave that stores such a sample.
+        if (__ballot(outside) != 0 && lane == 0) atomicOr(wide, 1);
         // DevPic views are read-only for the analysis; the destination slot
         // is this kernel's output.
         int16_t *row = const_cast<int16_t *>(
@@ -1838,16 +1844,35 @@ __global__ void reset_kernel(int32_t *words, int32_t n)
         words[i] = 0;
 }
 
-// Replicate the plane's edge samples into its apron (equivalent to the
-// clamped sampling of interp.cpp:8-14).
-__global__ void apron_kernel(int16_t *plane0, int32_t pitch, int32_t w, int32_t h)
+// Replicate the edge samples of a picture's three planes into their aprons
+// (equivalent to the clamped sampling of interp.cpp:8-14), in one launch
+// (blockIdx.y = plane).  Only apron positions are visited: the top and
+// bottom bands at full padded width, then the left and right bands of the
+// interior rows; every value is read from its clamped interior position.
+__global__ void apron3_kernel(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int32_t pc,
+                              int32_t w, int32_t h)
 {
-    const int W = w + 2 * kApron, H = h + 2 * kApron;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < W * H; i += gridDim.x * blockDim.x) {
-        const int X = i % W - kApron, Y = i / W - kApron;
-        if (X >= 0 && X < w && Y >= 0 && Y < h) continue;
-        const int cx = min(max(X, 0), w - 1), cy = min(max(Y, 0), h - 1);
-        plane0[Y * pitch + X] = plane0[cy * pitch + cx];
+    const int plane = blockIdx.y;
+    int16_t *p0 = plane == 0 ? y0 : plane == 1 ? u0 : v0;
+    const int pw = plane == 0 ? w : w / 2, ph = plane == 0 ? h : h / 2;
+    const int pitch = plane == 0 ? py : pc;
+    const int W = pw + 2 * kApron;
+    const int band = kApron * W, side = kApron * ph;
+    const int n = 2 * band + 2 * side;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int X, Y;
+        if (i < 2 * band) {
+            const int j = i < band ? i : i - band;
+            X = j % W - kApron;
+            Y = i < band ? j / W - kApron : ph + j / W;
+        } else {
+            const int j = i - 2 * band;
+            const int k = j < side ? j : j - side;
+            Y = k / kApron;
+            X = j < side ? k % kApron - kApron : pw + k % kApron;
+        }
+        const int cx = min(max(X, 0), pw - 1), cy = min(max(Y, 0), ph - 1);
+        p0[Y * pitch + X] = p0[cy * pitch + cx];
     }
 }
 
@@ -1957,19 +1982,20 @@ hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *cou
 }
 
 hipError_t launch_recon(const DevJob &job, const int16_t *levels, const DevPic &dst,
-                        hipStream_t st)
+                        int32_t *wide, hipStream_t st)
 {
     const int n = job.wb * job.hb;
     hipLaunchKernelGGL(recon_kernel, dim3((n + kReconMbs - 1) / kReconMbs), dim3(64 * kReconMbs),
-                       0, st, job, levels, dst);
+                       0, st, job, levels, dst, wide);
     return hipGetLastError();
 }
 
-hipError_t launch_apron(int16_t *plane0, int32_t pitch, int32_t w, int32_t h, hipStream_t st)
+hipError_t launch_apron3(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int32_t pc, int32_t w,
+                         int32_t h, hipStream_t st)
 {
-    const int total = (w + 2 * kApron) * (h + 2 * kApron);
-    const int blocks = min((total + 255) / 256, 2048);
-    hipLaunchKernelGGL(apron_kernel, dim3(blocks), dim3(256), 0, st, plane0, pitch, w, h);
+    const int n = 2 * kApron * (w + 2 * kApron) + 2 * kApron * h;
+    const int blocks = min((n + 255) / 256, 512);
+    hipLaunchKernelGGL(apron3_kernel, dim3(blocks, 3), dim3(256), 0, st, y0, u0, v0, py, pc, w, h);
     return hipGetLastError();
 }
 
