@@ -1762,6 +1762,14 @@ __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
             if (((job.target_ids >> (8 * r)) & 0xFF) == target) k = r;
         return k;
     };
+    // Levels of a transformed MB: loads issued now, used after the prediction.
+    const bool coded = (btype & EVXA_BLOCK_COPY) == 0;
+    int16_t lvr[EVXA_MB_SAMPLES / 64];
+    if (coded) {
+        const int16_t *lv = levels + (size_t)mb * EVXA_MB_SAMPLES;
+#pragma unroll
+        for (int k = 0; k < EVXA_MB_SAMPLES / 64; k++) lvr[k] = lv[lane + 64 * k];
+    }
     SearchOut o{};
     o.best_x = x + (int16_t)(w1 & 0xFFFF);
     o.best_y = y + (int16_t)(w1 >> 16);
@@ -1777,16 +1785,15 @@ __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
             sh.pred[i] = (int16_t)((sh.pred[i] + sh.pred_b[i] + 1) >> 1);
         wave_sync();
     }
-    const bool coded = (btype & EVXA_BLOCK_COPY) == 0;
     if (coded) {
         // Inverse quantisation, luma at qp and chroma at min(qp + 2, 63).
         const int qp = (w4 >> 16) & 0xFF, cqp = min(qp + 2, 63);
-        const int16_t *lv = levels + (size_t)mb * EVXA_MB_SAMPLES;
-#pragma unroll 1
-        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+#pragma unroll
+        for (int k = 0; k < EVXA_MB_SAMPLES / 64; k++) {
+            const int i = lane + 64 * k;
             const int pos = i < 256 ? ((i >> 4) & 7) * 8 + (i & 7) : (i - 256) & 63;
             const int q = i < 256 ? qp : cqp;
-            sh.co[i] = (int16_t)(((2 * (int)lv[i]) * c_quant.qm[pos] * q) / 16);
+            sh.co[i] = (int16_t)(((2 * (int)lvr[k]) * c_quant.qm[pos] * q) / 16);
         }
         wave_sync();
         // Vertical pass: lane = (8x8 block, column) into tmp.
