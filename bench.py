@@ -597,13 +597,45 @@ def run_recon(args):
                      "algorithmic_bytes_per_launch": nbytes},
         "transformed_mbs": coded,
     }
+    got = s.download(6)
+    # Deblocking of that reconstruction with the frame's own records
+    # (evxa_deblock: a device copy of the picture, then deblock_kernel from
+    # the copy back into the slot, then the apron).
+    s.deblock(6)
+    got_db = s.download(6)
+    for _ in range(5):
+        s.deblock(6)
+    ctx.synchronize()
+    ctx.launch_times()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.deblock(6)
+    ctx.synchronize()
+    call_s = (time.perf_counter() - t0) / reps
+    d_s = float(np.mean(ctx.launch_times())) / 1e3
+    samples = aw * ah * 3 // 2
+    dbytes = samples * 4 + n * 64
+    res["deblock"] = {
+        "workload": "deblock: the reconstructed 1920x1080 frame, its own records, in place",
+        "kernel": "deblock_kernel",
+        "kernel_us_avg": round(d_s * 1e6, 2),
+        "call_us": round(call_s * 1e6, 2),
+        "frames_per_s": round(1.0 / d_s, 1),
+        "roofline": {"bound": "hbm", "achieved": round(dbytes / d_s / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dbytes / d_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_launch": dbytes},
+    }
     if not args.no_cpu:
         from oracle.oracle import Oracle, Picture
+        o = Oracle()
         pics = [Picture(*f) for f in frames]
-        want = Oracle().reconstruct_frame(1, args.quality, pics[5], [pics[r] for r in frame_refs(5)],
-                                          rec, lv)
-        got = s.download(6)
+        want = o.reconstruct_frame(1, args.quality, pics[5], [pics[r] for r in frame_refs(5)],
+                                   rec, lv)
         res["bit_exact_vs_cpu"] = all(np.array_equal(a, b) for a, b in zip(got, want))
+        want_db = o.deblock_frame(rec, *want)
+        res["deblock"]["bit_exact_vs_cpu"] = all(np.array_equal(a, b)
+                                                 for a, b in zip(got_db, want_db))
     s.close()
     ctx.close()
     return res

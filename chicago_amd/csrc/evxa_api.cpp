@@ -223,6 +223,8 @@ struct evxa_stream {
     DevJob last_job{};     // the last analysed frame (for reconstruction)
     int32_t last_refs[EVXA_MAX_REFS] = {-1, -1, -1, -1};
     DevBuf recon_levels;   // levels handed in for reconstruction
+    DevBuf deblock_src;    // unfiltered copy of the picture being deblocked (packed planes)
+    DevBuf deblock_records;  // block descriptors handed in for deblocking
     uint64_t bytes = 0;
 
     int16_t *slot_base(int32_t slot) const
@@ -255,6 +257,18 @@ void set_device(const evxa_context *ctx) { check(hipSetDevice(ctx->device), "hip
 void check_slot(const evxa_stream *s, int32_t slot)
 {
     if (slot < 0 || slot >= s->n_slots) fail(EVXA_ERROR_INVALIDARG, "slot out of range");
+}
+
+// Allocate a buffer a stream creates on first use and charge it to the
+// context's HBM budget (the caller holds ctx->mu).
+void lazy_alloc(evxa_stream *s, DevBuf &b, size_t bytes)
+{
+    if (b.p) return;
+    evxa_context *ctx = s->ctx;
+    if (ctx->used + bytes > ctx->budget) fail(EVXA_ERROR_CAPACITY_LIMIT, "buffer does not fit the HBM budget");
+    b.alloc(bytes);
+    s->bytes += bytes;
+    ctx->used += bytes;
 }
 
 // Build the job descriptor of one frame and advance the stream's
@@ -868,7 +882,7 @@ int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
         const size_t n = (size_t)s->wb * s->hb;
         const int16_t *lv = j.levels;
         if (levels) {
-            if (!s->recon_levels.p) s->recon_levels.alloc(n * EVXA_MB_SAMPLES * sizeof(int16_t));
+            lazy_alloc(s, s->recon_levels, n * EVXA_MB_SAMPLES * sizeof(int16_t));
             check(hipMemcpyAsync(s->recon_levels.p, levels, n * EVXA_MB_SAMPLES * sizeof(int16_t),
                                  hipMemcpyHostToDevice, ctx->stream),
                   "upload levels");
@@ -892,6 +906,57 @@ int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
               "apron kernel");
         // The caller's level buffer has been consumed once this returns.
         if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
+{
+    return guarded([&] {
+        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        check_slot(s, slot);
+        if (!records && s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
+        set_device(ctx);
+        const size_t n = (size_t)s->wb * s->hb;
+        const evxa_block_record *rec = s->last_job.records;
+        if (records) {
+            lazy_alloc(s, s->deblock_records, n * sizeof(evxa_block_record));
+            check(hipMemcpyAsync(s->deblock_records.p, records, n * sizeof(evxa_block_record),
+                                 hipMemcpyHostToDevice, ctx->stream),
+                  "upload records");
+            rec = s->deblock_records.as<evxa_block_record>();
+        }
+        // The filter reads the unfiltered picture around each tile, so it
+        // runs from a copy back into the slot.
+        const size_t ny = (size_t)s->w * s->h, nc = ny / 4;
+        lazy_alloc(s, s->deblock_src, (ny + 2 * nc) * sizeof(int16_t));
+        const DevPic p = s->pic(slot);
+        int16_t *scr = s->deblock_src.as<int16_t>();
+        int16_t *const src[3] = {scr, scr + ny, scr + ny + nc};
+        int16_t *const dst[3] = {const_cast<int16_t *>(p.y), const_cast<int16_t *>(p.u),
+                                 const_cast<int16_t *>(p.v)};
+        const int32_t sp[3] = {s->w, s->w / 2, s->w / 2}, dp[3] = {p.py, p.pc, p.pc};
+        for (int k = 0; k < 3; k++)
+            check(hipMemcpy2DAsync(src[k], (size_t)sp[k] * 2, dst[k], (size_t)dp[k] * 2,
+                                   (size_t)sp[k] * 2, k ? s->h / 2 : s->h, hipMemcpyDeviceToDevice,
+                                   ctx->stream),
+                  "copy picture");
+        int32_t *wide = s->wide.as<int32_t>() + slot;
+        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
+        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+        check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
+        check(launch_deblock(src, sp, dst, dp, s->w, s->h, rec, wide, ctx->stream), "deblock kernel");
+        check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
+        ctx->ring_head++;
+        ctx->last_start = ev.first;
+        ctx->last_stop = ev.second;
+        check(launch_apron3(dst[0], dst[1], dst[2], p.py, p.pc, s->w, s->h, ctx->stream),
+              "apron kernel");
+        // The caller's records have been consumed once this returns.
+        if (records) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
 }
 

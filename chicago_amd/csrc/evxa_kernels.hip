@@ -1896,6 +1896,252 @@ __global__ void range_kernel(const int16_t *plane0, int32_t pitch, int32_t w, in
     if (__ballot(out) != 0 && lane_id() == 0) atomicOr(wide, 1);
 }
 
+// -------------------------------------------------------------- deblock --
+
+// The in-loop deblocking filter (deblock.cpp:222-306) as a parallel stencil.
+// The reference filters in place in raster order: the vertical edges of the
+// first band of 8 rows, then per band the horizontal edge at row j = 8b and
+// the vertical edges of rows j .. j+7.  A horizontal edge reads rows
+// j-4 .. j+3 and writes j-3 .. j+2; a vertical edge reads and writes within
+// its row.  Within a band the segments of one direction touch disjoint
+// samples, and the horizontal edge at j reads rows j-4 .. j-1 after the
+// vertical edges of band b-1 but before anything else has written them, so
+// every sample's final value follows from three stages over the unfiltered
+// picture:
+//   1. vertical edges on rows 8b+3 .. 8b+7 (and all rows of band 0), which no
+//      horizontal edge writes before them; row 8b+3 is also read unfiltered
+//      by the horizontal edge at 8b, so its result is kept aside;
+//   2. horizontal edges on the result;
+//   3. vertical edges on rows 8b .. 8b+2 (b >= 1) after stage 2.
+// A workgroup filters a tile of kDbTh x kDbTw samples from an LDS copy with
+// a 4-sample apron on every side (the reach of one edge filter).
+constexpr int kDbTw = 128, kDbTh = 32;
+constexpr int kDbCols = kDbTw + 8, kDbRows = kDbTh + 8;
+constexpr int kDbEdges = kDbTw / 8 + 1;         // vertical edges at C0, C0+8, ..., C0+kDbTw
+constexpr int kDbThreads = 256;
+constexpr int kDbBlkRows = 8, kDbBlkCols = 20;   // MBs (luma) or 8x8 blocks (chroma) around a tile
+
+// deblock.cpp:13-35: edge thresholds by qp index.
+constexpr uint8_t kDeblockAlpha[64] = {
+    0,   0,   0,   0,   0,   0,   0,   1,   1,   1,   2,   2,   3,   3,   4,   5,
+    6,   7,   8,   9,   10,  12,  14,  16,  18,  20,  22,  24,  26,  29,  32,  35,
+    38,  42,  46,  50,  54,  59,  64,  70,  76,  82,  89,  96,  104, 112, 120, 128,
+    138, 148, 158, 170, 182, 196, 210, 224, 240, 255, 255, 255, 255, 255, 255, 255};
+constexpr uint8_t kDeblockBeta[64] = {
+    0,  0,  0,  0,  0,  0,  0,  0,  1,  1,  1,  1,  2,  2,  2,  3,
+    3,  3,  4,  4,  4,  5,  5,  6,  6,  7,  7,  8,  8,  9,  10, 11,
+    12, 13, 14, 15, 16, 17, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18,
+    18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18};
+
+struct DeblockLds {
+    alignas(16) int16_t t[kDbRows][kDbCols];     // rows R0-4 .., columns C0-4 ..
+    alignas(16) int16_t r3[kDbTh / 8][kDbCols];  // stage-1 results of each band's row 3
+    uint16_t blk[kDbBlkRows][kDbBlkCols];        // copy << 8 | q_index
+    uint16_t ab[64];                             // alpha | beta << 8
+};
+
+// deblock.cpp:91-165 on one line v = p3 .. q3 (strength 1 or 2), in place;
+// `ab` holds the thresholds of the edge's qp index.
+__device__ __forceinline__ void deblock_line(int (&v)[8], uint32_t ab, int strength, bool luma)
+{
+    const int p3 = v[0], p2 = v[1], p1 = v[2], p0 = v[3], q0 = v[4], q1 = v[5], q2 = v[6],
+              q3 = v[7];
+    const int alpha = ab & 0xFF, beta = ab >> 8;
+    // The differences are stored as int16 before the comparisons.
+    const int dpq = (int16_t)abs(p0 - q0), dp = (int16_t)abs(p1 - p0), dq = (int16_t)abs(q1 - q0);
+    if (dpq >= alpha || dp >= beta || dq >= beta) return;
+    auto rd = [](int x, int d) { return x < 0 ? (x - d / 2) / d : (x + d / 2) / d; };
+    if (strength == 2) {
+        v[3] = (int16_t)rd(p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1, 8);
+        v[2] = (int16_t)rd(p2 + p1 + p0 + q0, 4);
+        v[4] = (int16_t)rd(p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2, 8);
+        v[5] = (int16_t)rd(p0 + q0 + q1 + q2, 4);
+        if (luma) {
+            v[1] = (int16_t)rd(2 * p3 + 3 * p2 + p1 + p0 + q0, 8);
+            v[6] = (int16_t)rd(2 * q3 + 3 * q2 + q1 + q0 + p0, 8);
+        }
+    } else {
+        v[3] = (int16_t)rd(((q0 + p0) * 4) + p1 - q1, 8);
+        v[4] = (int16_t)rd(((q0 + p0) * 4) + q1 - p1, 8);
+        if (luma) {
+            v[2] = (int16_t)rd((p2 * 4) + (p0 * 2) + (q0 * 2), 8);
+            v[5] = (int16_t)rd((q2 * 4) + (q0 * 2) + (p0 * 2), 8);
+        }
+    }
+}
+
+// deblock.cpp:57-89 and 167-168: strength of the edge between blocks a (left
+// or top) and b, and the thresholds of its qp index.
+__device__ __forceinline__ int deblock_edge(const DeblockLds &sh, uint32_t a, uint32_t b,
+                                            bool internal, int qp_offset, uint32_t &ab)
+{
+    const bool ca = a >> 8, cb = b >> 8;
+    if (ca && cb) return 0;
+    const int avg = !ca && !cb ? ((a & 0xFF) + (b & 0xFF)) >> 1 : !ca ? (a & 0xFF) : (b & 0xFF);
+    const int s = ca != cb ? 1 : internal ? 1 : 2;
+    const int qp = min(avg + qp_offset, 63);
+    ab = sh.ab[s == 2 && qp > 0 ? min(qp + 2, 63) : qp];
+    return s;
+}
+
+// The planes are tiled kDbTh x kDbTw; tiles of plane k start at first[k].
+struct DeblockArgs {
+    const int16_t *src[3];
+    int16_t *dst[3];
+    int32_t sp[3], dp[3];      // pitches (elements)
+    int32_t first[4];          // first tile of each plane; first[3] = total
+    int32_t w, h, wb, hb;      // luma size and MB grid
+};
+
+__device__ __forceinline__ void lds_line(const int16_t *p, int (&v)[8])
+{
+    const int4 q = *reinterpret_cast<const int4 *>(p);
+    const int32_t d[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[2 * k] = (int16_t)(d[k] & 0xFFFF);
+        v[2 * k + 1] = d[k] >> 16;
+    }
+}
+
+__device__ __forceinline__ void lds_store_line(int16_t *p, const int (&v)[8])
+{
+    int4 q;
+    q.x = (v[0] & 0xFFFF) | (v[1] << 16);
+    q.y = (v[2] & 0xFFFF) | (v[3] << 16);
+    q.z = (v[4] & 0xFFFF) | (v[5] << 16);
+    q.w = (v[6] & 0xFFFF) | (v[7] << 16);
+    *reinterpret_cast<int4 *>(p) = q;
+}
+
+__global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a,
+                                                             const evxa_block_record *records,
+                                                             int32_t *wide)
+{
+    __shared__ DeblockLds sh;
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int pl = (tile >= a.first[1]) + (tile >= a.first[2]);
+    const bool luma = pl == 0;
+    const int pw = luma ? a.w : a.w / 2, ph = luma ? a.h : a.h / 2;
+    const int mb = luma ? 16 : 8, off = luma ? 0 : 2;   // config.h:68 for chroma
+    const int tiles_x = (pw + kDbTw - 1) / kDbTw;
+    const int t = tile - a.first[pl];
+    const int C0 = (t % tiles_x) * kDbTw, R0 = (t / tiles_x) * kDbTh;
+    const int16_t *src = a.src[pl];
+    const int sp = a.sp[pl];
+
+    // Unfiltered samples of the tile and its apron (zero outside the plane;
+    // no edge reaches them).  Chunks of 4 samples: w is a multiple of 8.
+    for (int i = tid; i < kDbRows * (kDbCols / 4); i += kDbThreads) {
+        const int lr = i / (kDbCols / 4), q = i % (kDbCols / 4);
+        const int r = R0 - 4 + lr, c = C0 - 4 + 4 * q;
+        int2 v = make_int2(0, 0);
+        if (r >= 0 && r < ph && c >= 0 && c < pw)
+            v = *reinterpret_cast<const int2 *>(src + (size_t)r * sp + c);
+        *reinterpret_cast<int2 *>(&sh.t[lr][4 * q]) = v;
+    }
+    // Block descriptors around the tile.
+    const int bx0 = max(C0 - 4, 0) / mb, by0 = max(R0 - 4, 0) / mb;
+    for (int i = tid; i < kDbBlkRows * kDbBlkCols; i += kDbThreads) {
+        const int bx = bx0 + i % kDbBlkCols, by = by0 + i / kDbBlkCols;
+        uint32_t e = 0;
+        if (bx < a.wb && by < a.hb) {
+            const uint32_t *rw = reinterpret_cast<const uint32_t *>(records + by * a.wb + bx);
+            // block_type: byte 0; q_index: byte 18.
+            e = ((rw[0] & 4) ? 0x100u : 0u) | ((rw[4] >> 16) & 0xFF);
+        }
+        sh.blk[i / kDbBlkCols][i % kDbBlkCols] = (uint16_t)e;
+    }
+    if (tid < 64) sh.ab[tid] = kDeblockAlpha[tid] | kDeblockBeta[tid] << 8;
+    __syncthreads();
+
+    auto blk = [&](int x, int y) -> uint32_t { return sh.blk[y / mb - by0][x / mb - bx0]; };
+    // The line of row r across vertical edge e (column C0 + 8e); false if
+    // the edge is not filtered.
+    auto vertical = [&](int r, int e, int (&v)[8]) -> bool {
+        const int i = C0 + 8 * e;
+        if (i < 8 || i >= pw) return false;
+        uint32_t ab;
+        const int s = deblock_edge(sh, blk(i - 1, r), blk(i, r), (i - 1) / mb == i / mb, off, ab);
+        if (!s) return false;
+        deblock_line(v, ab, s, luma);
+        return true;
+    };
+
+    // Stage 1: vertical edges of the rows no horizontal edge writes first.
+    for (int i = tid; i < (kDbTh + 4) * kDbEdges; i += kDbThreads) {
+        const int lr = i / kDbEdges, e = i % kDbEdges;
+        const int r = R0 - 4 + lr;
+        if (r < 0 || r >= ph || (r >= 8 && (r & 7) < 3)) continue;
+        int v[8];
+        lds_line(&sh.t[lr][8 * e], v);
+        const bool f = vertical(r, e, v);
+        if (r >= 8 && (r & 7) == 3) lds_store_line(&sh.r3[(r - R0) >> 3][8 * e], v);
+        else if (f) lds_store_line(&sh.t[lr][8 * e], v);
+    }
+    __syncthreads();
+
+    // Stage 2: horizontal edges, two columns per thread.
+    for (int i = tid; i < (kDbTh / 8 + 1) * (kDbCols / 2); i += kDbThreads) {
+        const int hb = i / (kDbCols / 2), cp = i % (kDbCols / 2);
+        const int j = R0 + 8 * hb, c = C0 - 4 + 2 * cp;
+        if (j < 8 || j >= ph || c < 0 || c >= pw) continue;
+        uint32_t ab;
+        const int s = deblock_edge(sh, blk(c, j - 1), blk(c, j), (j - 1) / mb == j / mb, off, ab);
+        if (!s) continue;
+        int v0[8], v1[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int d = *reinterpret_cast<const int32_t *>(&sh.t[8 * hb + k][2 * cp]);
+            v0[k] = (int16_t)(d & 0xFFFF);
+            v1[k] = d >> 16;
+        }
+        deblock_line(v0, ab, s, luma);
+        deblock_line(v1, ab, s, luma);
+#pragma unroll
+        for (int k = 1; k < 7; k++)
+            *reinterpret_cast<int32_t *>(&sh.t[8 * hb + k][2 * cp]) = (v0[k] & 0xFFFF) | (v1[k] << 16);
+    }
+    __syncthreads();
+
+    // Stage 3: vertical edges of rows 8b .. 8b+2 after the horizontal edges.
+    for (int i = tid; i < (kDbTh / 8) * 3 * kDbEdges; i += kDbThreads) {
+        const int k = i / kDbEdges, e = i % kDbEdges;
+        const int r = R0 + 8 * (k / 3) + k % 3;
+        if (r < 8 || r >= ph) continue;
+        const int lr = r - R0 + 4;
+        int v[8];
+        lds_line(&sh.t[lr][8 * e], v);
+        if (vertical(r, e, v)) lds_store_line(&sh.t[lr][8 * e], v);
+    }
+    __syncthreads();
+
+    // The tile's samples, 8 per thread and step; rows 8b+3 from stage 1.
+    int16_t *dst = a.dst[pl];
+    const int dpitch = a.dp[pl];
+    bool outside = false;
+    for (int i = tid; i < kDbTh * (kDbTw / 8); i += kDbThreads) {
+        const int rr = i / (kDbTw / 8), q = i % (kDbTw / 8);
+        const int r = R0 + rr, c = C0 + 8 * q;
+        if (r >= ph || c >= pw) continue;
+        const int16_t *s = r >= 8 && (r & 7) == 3 ? &sh.r3[rr >> 3][8 * q + 4]
+                                                  : &sh.t[rr + 4][8 * q + 4];
+        const int2 lo = *reinterpret_cast<const int2 *>(s);
+        const int2 hi = *reinterpret_cast<const int2 *>(s + 4);
+        const int32_t d[4] = {lo.x, lo.y, hi.x, hi.y};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int x0 = (int16_t)(d[k] & 0xFFFF), x1 = d[k] >> 16;
+            outside = outside || x0 < 0 || x0 > 2047 || x1 < 0 || x1 > 2047;
+        }
+        *reinterpret_cast<int4 *>(dst + (size_t)r * dpitch + c) = make_int4(lo.x, lo.y, hi.x, hi.y);
+    }
+    // The picture's range flag (range_kernel's rule).
+    if (__ballot(outside) != 0 && lane_id() == 0) atomicOr(wide, 1);
+}
+
+
 // params: n x {x, y, dir_x, dir_y, quarter}
 __global__ __launch_bounds__(64) void interp_blocks_kernel(DevPic pic, int32_t n,
                                                            const int32_t *params, int16_t *out)
@@ -2012,6 +2258,30 @@ hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t
     const int blocks = min((w * h + 255) / 256, 1024);
     hipLaunchKernelGGL(range_kernel, dim3(max(blocks, 1)), dim3(256), 0, st, plane0, pitch, w, h,
                        wide);
+    return hipGetLastError();
+}
+
+hipError_t launch_deblock(int16_t *const src[3], const int32_t sp[3], int16_t *const dst[3],
+                          const int32_t dp[3], int32_t w, int32_t h,
+                          const evxa_block_record *records, int32_t *wide, hipStream_t st)
+{
+    DeblockArgs a{};
+    int32_t n = 0;
+    for (int p = 0; p < 3; p++) {
+        const int pw = p ? w / 2 : w, ph = p ? h / 2 : h;
+        a.src[p] = src[p];
+        a.dst[p] = dst[p];
+        a.sp[p] = sp[p];
+        a.dp[p] = dp[p];
+        a.first[p] = n;
+        n += ((pw + kDbTw - 1) / kDbTw) * ((ph + kDbTh - 1) / kDbTh);
+    }
+    a.first[3] = n;
+    a.w = w;
+    a.h = h;
+    a.wb = w / 16;
+    a.hb = h / 16;
+    hipLaunchKernelGGL(deblock_kernel, dim3(n), dim3(kDbThreads), 0, st, a, records, wide);
     return hipGetLastError();
 }
 

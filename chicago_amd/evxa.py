@@ -74,7 +74,7 @@ EXPORTS = (
     "evxa_context_bytes", "evxa_lambda", "evxa_stream_open", "evxa_stream_close",
     "evxa_upload_picture", "evxa_stream_reset_temporal", "evxa_analyze_frame",
     "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_launch_times",
-    "evxa_read_results", "evxa_download_picture", "evxa_reconstruct",
+    "evxa_read_results", "evxa_download_picture", "evxa_reconstruct", "evxa_deblock",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
@@ -122,6 +122,7 @@ def load_library(path: str = LIB_PATH):
         "evxa_read_results": (C.c_int, [vp, vp, I16P, I16P]),
         "evxa_download_picture": (C.c_int, [vp, C.c_int32, I16P, I16P, I16P, C.c_int32, C.c_int32]),
         "evxa_reconstruct": (C.c_int, [vp, C.c_int32, I16P]),
+        "evxa_deblock": (C.c_int, [vp, C.c_int32, vp]),
         "evxa_graph_create": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32,
                                         C.POINTER(C.c_int32)]),
         "evxa_graph_launch": (C.c_int, [vp, C.c_int32]),
@@ -331,6 +332,16 @@ class Stream:
             levels = np.ascontiguousarray(levels, np.int16)
             assert levels.shape == (self.n_mbs, MB_SAMPLES)
         _check(_lib.evxa_reconstruct(self.h, dst_slot, None if levels is None else _p16(levels)))
+
+    def deblock(self, slot, records=None):
+        """Deblock picture slot `slot` in place (the block types and q_index
+        of `records`, n_mbs block records; None: the last analysed frame's)."""
+        ptr = None
+        if records is not None:
+            records = np.ascontiguousarray(records, RECORD_DTYPE)
+            assert records.shape == (self.n_mbs,)
+            ptr = records.ctypes.data
+        _check(_lib.evxa_deblock(self.h, slot, ptr))
 
     def reset_temporal(self):
         _check(_lib.evxa_stream_reset_temporal(self.h))

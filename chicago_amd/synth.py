@@ -252,3 +252,32 @@ def sequence_c2(seed, n_frames=33, width=3840, height=2160, zoom=0.005, local_ra
         mvy = iy + np.where(local, rng.integers(-local_range, local_range + 1, (hb, wb)), 0)
         frames.append(move_blocks(rng, frames[-1], mvx, mvy, sp_zoom, visible=(width, height)))
     return frames
+
+
+def blocky_picture(seed, width, height, amp=8, noise=2, base=120):
+    """A smooth picture with a random offset per 8x8 block and light noise:
+    the blocking a coarse quantiser leaves, for the deblocking filter.
+    Returns (y, u, v) int16 planes (chroma width/2 x height/2)."""
+    rng = np.random.default_rng(seed)
+
+    def plane(w, h):
+        yy, xx = np.mgrid[0:h, 0:w]
+        smooth = base + 40 * np.sin(xx / 37.0) + 30 * np.cos(yy / 23.0)
+        off = rng.integers(-amp, amp + 1, (h // 8 + 1, w // 8 + 1))
+        steps = np.repeat(np.repeat(off, 8, 0), 8, 1)[:h, :w]
+        return (smooth + steps + rng.integers(-noise, noise + 1, (h, w))).astype(np.int16)
+
+    return plane(width, height), plane(width // 2, height // 2), plane(width // 2, height // 2)
+
+
+def random_block_records(seed, width, height, q_lo=1, q_hi=63, copy_frac=0.25):
+    """Block records for a width x height frame with random block types
+    (delta or copy, with or without motion) and q_index in [q_lo, q_hi]."""
+    from .evxa import RECORD_DTYPE
+    rng = np.random.default_rng(seed)
+    n = (width // 16) * (height // 16)
+    rec = np.zeros(n, RECORD_DTYPE)
+    copy = rng.random(n) < copy_frac
+    rec["block_type"] = (copy * 4) | (rng.integers(0, 2, n) * 2)
+    rec["q_index"] = rng.integers(q_lo, q_hi + 1, n)
+    return rec

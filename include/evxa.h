@@ -173,6 +173,16 @@ int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeff
  * quantisation (encode.cpp:806, 918 in the row workers). */
 int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels);
 
+/* The in-loop deblocking filter of picture slot `slot`, in place, with the
+ * result of the reference's deblock_image_filter (deblock.cpp:222-306: luma
+ * on 16x16 blocks, chroma on 8x8 at qp + 2, edges every 8 samples, strength
+ * from the block types, thresholds from the averaged q_index).  `records`:
+ * n_mbs block records on the host (block_type and q_index are read), or
+ * NULL for the stream's last analysed frame.  Replaces the encoder's
+ * deblock_image_filter call after the frame's reconstruction
+ * (encode.cpp:968) and the decoder's (decode.cpp:583). */
+int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records);
+
 /* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
 
 /* Capture the launch of `n` frame descriptors into a replayable graph.

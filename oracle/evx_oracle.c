@@ -852,3 +852,122 @@ int32_t evo_reconstruct_frame(const evo_frame_job *job, const evo_record *record
         }
     return 0;
 }
+
+/* ------------------------------------------------------------ deblock -- */
+
+#define EVO_DEBLOCK_STEP 8                  /* deblock.cpp:9 */
+#define EVO_DEBLOCK_QP_OFFSET 2             /* config.h:74 */
+
+/* deblock.cpp:13-35: edge thresholds by qp index. */
+static const int16_t kDeblockAlpha[EVO_MAX_QUANT_LEVELS] = {
+    0,   0,   0,   0,   0,   0,   0,   1,   1,   1,   2,   2,   3,   3,   4,   5,
+    6,   7,   8,   9,   10,  12,  14,  16,  18,  20,  22,  24,  26,  29,  32,  35,
+    38,  42,  46,  50,  54,  59,  64,  70,  76,  82,  89,  96,  104, 112, 120, 128,
+    138, 148, 158, 170, 182, 196, 210, 224, 240, 255, 255, 255, 255, 255, 255, 255};
+static const int16_t kDeblockBeta[EVO_MAX_QUANT_LEVELS] = {
+    0,  0,  0,  0,  0,  0,  0,  0,  1,  1,  1,  1,  2,  2,  2,  3,
+    3,  3,  4,  4,  4,  5,  5,  6,  6,  7,  7,  8,  8,  9,  10, 11,
+    12, 13, 14, 15, 16, 17, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18,
+    18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18, 18};
+
+/* deblock.cpp:91-165 */
+void evo_deblock_line(int16_t v[8], int32_t qp, int32_t strength, int32_t is_luma)
+{
+    const int32_t p3 = v[0], p2 = v[1], p1 = v[2], p0 = v[3];
+    const int32_t q0 = v[4], q1 = v[5], q2 = v[6], q3 = v[7];
+    int32_t qi = qp;
+    if (strength == 2 && qp > 0)
+        qi = qp + EVO_DEBLOCK_QP_OFFSET < EVO_MAX_QUANT_LEVELS - 1 ? qp + EVO_DEBLOCK_QP_OFFSET
+                                                                  : EVO_MAX_QUANT_LEVELS - 1;
+    /* The differences are stored as int16 before the comparisons. */
+    const int16_t d_pq = (int16_t)abs32(p0 - q0);
+    const int16_t d_p = (int16_t)abs32(p1 - p0);
+    const int16_t d_q = (int16_t)abs32(q1 - q0);
+    if (d_pq >= kDeblockAlpha[qi] || d_p >= kDeblockBeta[qi] || d_q >= kDeblockBeta[qi]) return;
+    if (strength == 2) {
+        v[3] = (int16_t)rounded_div(p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1, 8);
+        v[2] = (int16_t)rounded_div(p2 + p1 + p0 + q0, 4);
+        v[4] = (int16_t)rounded_div(p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2, 8);
+        v[5] = (int16_t)rounded_div(p0 + q0 + q1 + q2, 4);
+        if (is_luma) {
+            v[1] = (int16_t)rounded_div(2 * p3 + 3 * p2 + p1 + p0 + q0, 8);
+            v[6] = (int16_t)rounded_div(2 * q3 + 3 * q2 + q1 + q0 + p0, 8);
+        }
+    } else if (strength == 1) {
+        v[3] = (int16_t)rounded_div(((q0 + p0) * 4) + p1 - q1, 8);
+        v[4] = (int16_t)rounded_div(((q0 + p0) * 4) + q1 - p1, 8);
+        if (is_luma) {
+            v[2] = (int16_t)rounded_div((p2 * 4) + (p0 * 2) + (q0 * 2), 8);
+            v[5] = (int16_t)rounded_div((q2 * 4) + (q0 * 2) + (p0 * 2), 8);
+        }
+    }
+}
+
+/* deblock.cpp:57-89 with 167-220: strength (0, 1, 2) and average qp of the
+ * edge between blocks a (left / top) and b; `internal` when both sides lie
+ * in one block of the table. */
+static int32_t deblock_edge(const evo_record *a, const evo_record *b, int32_t internal,
+                            int32_t *avg_qp)
+{
+    const int32_t ca = (a->block_type & EVO_BT_COPY) != 0, cb = (b->block_type & EVO_BT_COPY) != 0;
+    if (!ca && !cb) *avg_qp = (uint8_t)((a->q_index + b->q_index) >> 1);
+    else if (!ca) *avg_qp = a->q_index;
+    else if (!cb) *avg_qp = b->q_index;
+    else *avg_qp = 0;
+    if (ca && cb) return 0;
+    if (ca ^ cb) return 1;
+    return internal ? 1 : 2;
+}
+
+/* deblock.cpp:167-220: filter the 8 lines of one edge segment in place,
+ * gathering each line's samples at stride `along` (1: across a vertical
+ * edge; the row pitch: across a horizontal one) and stepping `next`. */
+static void deblock_segment(int16_t *at, int32_t along, int32_t next, int32_t avg_qp,
+                            int32_t strength, int32_t is_luma, int32_t qp_offset)
+{
+    int32_t qp = avg_qp + qp_offset;
+    if (qp > EVO_MAX_QUANT_LEVELS - 1) qp = EVO_MAX_QUANT_LEVELS - 1;
+    for (int32_t l = 0; l < EVO_DEBLOCK_STEP; l++, at += next) {
+        int16_t v[8];
+        for (int32_t t = 0; t < 8; t++) v[t] = at[(t - 4) * along];
+        evo_deblock_line(v, qp, strength, is_luma);
+        for (int32_t t = 1; t < 7; t++) at[(t - 4) * along] = v[t];
+    }
+}
+
+/* deblock.cpp:222-276: one plane in the reference's order -- the vertical
+ * edges of the first band of 8 rows, then per band its horizontal edges and,
+ * after each horizontal segment from the second on, the vertical edge at the
+ * same column.  Blocks of the table are `mb` samples square. */
+static void deblock_plane(const evo_record *records, int32_t wb, int32_t mb, int32_t is_luma,
+                          int16_t *d, int32_t w, int32_t h, int32_t qp_offset)
+{
+    int32_t avg, s;
+#define EVO_BLK(i, j) (&records[(i) / mb + ((j) / mb) * wb])
+    for (int32_t i = EVO_DEBLOCK_STEP; i < w; i += EVO_DEBLOCK_STEP) {
+        s = deblock_edge(EVO_BLK(i - 1, 0), EVO_BLK(i, 0), (i - 1) / mb == i / mb, &avg);
+        if (s) deblock_segment(d + i, 1, w, avg, s, is_luma, qp_offset);
+    }
+    for (int32_t j = EVO_DEBLOCK_STEP; j < h; j += EVO_DEBLOCK_STEP) {
+        int16_t *row = d + (size_t)j * w;
+        s = deblock_edge(EVO_BLK(0, j - 1), EVO_BLK(0, j), (j - 1) / mb == j / mb, &avg);
+        if (s) deblock_segment(row, w, 1, avg, s, is_luma, qp_offset);
+        for (int32_t i = EVO_DEBLOCK_STEP; i < w; i += EVO_DEBLOCK_STEP) {
+            s = deblock_edge(EVO_BLK(i, j - 1), EVO_BLK(i, j), (j - 1) / mb == j / mb, &avg);
+            if (s) deblock_segment(row + i, w, 1, avg, s, is_luma, qp_offset);
+            s = deblock_edge(EVO_BLK(i - 1, j), EVO_BLK(i, j), (i - 1) / mb == i / mb, &avg);
+            if (s) deblock_segment(row + i, 1, w, avg, s, is_luma, qp_offset);
+        }
+    }
+#undef EVO_BLK
+}
+
+/* deblock.cpp:278-306 */
+void evo_deblock_frame(const evo_record *records, int32_t width, int32_t height, int16_t *y,
+                       int16_t *u, int16_t *v)
+{
+    const int32_t wb = width / EVO_MB;
+    deblock_plane(records, wb, EVO_MB, 1, y, width, height, 0);
+    deblock_plane(records, wb, EVO_MB / 2, 0, u, width / 2, height / 2, EVO_CHROMA_QP_OFFSET);
+    deblock_plane(records, wb, EVO_MB / 2, 0, v, width / 2, height / 2, EVO_CHROMA_QP_OFFSET);
+}

@@ -195,6 +195,20 @@ int32_t evo_reconstruct_frame(const evo_frame_job *job, const evo_record *record
                               const int16_t *levels, int16_t *out_y, int16_t *out_u,
                               int16_t *out_v);
 
+/* ---- In-loop deblocking (decode and encode side) ---- */
+
+/* deblock.cpp:91-165: one line of 8 samples across an edge, v = p3 p2 p1 p0
+ * q0 q1 q2 q3, filtered in place at the plane's effective qp (0..63) and
+ * strength 0, 1 or 2. */
+void evo_deblock_line(int16_t v[8], int32_t qp, int32_t strength, int32_t is_luma);
+
+/* deblock.cpp:222-306 (encode.cpp:968): the deblocking filter of a frame in
+ * the reference's raster order, in place.  records: one per MB (block_type
+ * and q_index are read).  Planes are packed: luma stride width, chroma
+ * width / 2. */
+void evo_deblock_frame(const evo_record *records, int32_t width, int32_t height, int16_t *y,
+                       int16_t *u, int16_t *v);
+
 #ifdef __cplusplus
 }
 #endif

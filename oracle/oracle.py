@@ -114,6 +114,9 @@ class _Recon:
         self._idct = self.f("inverse_transform_add_mb", None, I16P, I16P, I16P)
         self._recon = self.f("reconstruct_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
                              I16P, I16P, I16P, I16P)
+        self._dbl_line = self.f("deblock_line", None, I16P, C.c_int32, C.c_int32, C.c_int32)
+        self._dbl_frame = self.f("deblock_frame", None, C.c_void_p, C.c_int32, C.c_int32, I16P,
+                                 I16P, I16P)
 
     def dequantize(self, qp, levels):
         lv = np.ascontiguousarray(levels, np.int16)
@@ -142,6 +145,23 @@ class _Recon:
         if rc != 0:
             raise RuntimeError(f"reconstruct_frame failed: {rc}")
         return y, u, v
+
+    def deblock_line(self, v, qp, strength, is_luma):
+        """The 8 samples p3 .. q3 of one line after the edge filter."""
+        a = np.array(v, np.int16)
+        self._dbl_line(_ptr16(a), qp, strength, int(bool(is_luma)))
+        return a
+
+    def deblock_frame(self, records, y, u, v):
+        """Deblocked copies of the planes (y: h x w, u/v: h/2 x w/2); the
+        records give each MB's block type and q_index."""
+        h, w = y.shape
+        rec = np.ascontiguousarray(records)
+        if rec.size != (w // 16) * (h // 16):
+            raise ValueError("one record per MB")
+        oy, ou, ov = (np.array(a, np.int16, order="C") for a in (y, u, v))
+        self._dbl_frame(rec.ctypes.data, w, h, _ptr16(oy), _ptr16(ou), _ptr16(ov))
+        return oy, ou, ov
 
 
 class Oracle(_Lib, _Recon):

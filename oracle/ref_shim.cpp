@@ -31,6 +31,15 @@
 
 #include "evx_oracle.h"
 
+// deblock.cpp defines these at namespace scope without a header.
+namespace evx {
+evx_status deblock_image_filter(evx_block_desc *block_table, image_set *target_image);
+void deblock_filter_values(int16 p3, int16 p2, int16 p1, int16 p0, int16 q0, int16 q1, int16 q2,
+                           int16 q3, int16 *out_p2, int16 *out_p1, int16 *out_p0, int16 *out_q0,
+                           int16 *out_q1, int16 *out_q2, uint8 average_qp, uint8 strength,
+                           bool is_luma);
+}  // namespace evx
+
 using namespace evx;
 
 namespace {
@@ -573,6 +582,38 @@ int32_t evr_analyze_frame_wpp(const evo_frame_job *job, evo_record *records, int
     worker();
     for (auto &t : pool) t.join();
     return error.load();
+}
+
+
+// deblock_filter_values (deblock.cpp:91-165) on one line v = p3 .. q3.
+void evr_deblock_line(int16_t *v, int32_t qp, int32_t strength, int32_t is_luma)
+{
+    deblock_filter_values(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], &v[1], &v[2], &v[3],
+                          &v[4], &v[5], &v[6], (uint8)qp, (uint8)strength, is_luma != 0);
+}
+
+// deblock_image_filter (deblock.cpp:298-306, encode.cpp:968) on a packed
+// frame, in place; the block table carries each record's type and q_index.
+void evr_deblock_frame(const evo_record *records, int32_t width, int32_t height, int16_t *y,
+                       int16_t *u, int16_t *v)
+{
+    evo_planes p = {y, u, v, width, height, width, width / 2};
+    image_set set;
+    fill_set(&set, p);
+    const size_t n = (size_t)(width / 16) * (height / 16);
+    std::vector<evx_block_desc> table(n);
+    for (size_t i = 0; i < n; i++) {
+        clear_block_desc(&table[i]);
+        table[i].block_type = (EVX_BLOCK_TYPE)records[i].block_type;
+        table[i].q_index = records[i].q_index;
+    }
+    deblock_image_filter(table.data(), &set);
+    const int16 *sy = reinterpret_cast<const int16 *>(set.query_y_image()->query_data());
+    const int16 *su = reinterpret_cast<const int16 *>(set.query_u_image()->query_data());
+    const int16 *sv = reinterpret_cast<const int16 *>(set.query_v_image()->query_data());
+    memcpy(y, sy, (size_t)width * height * sizeof(int16_t));
+    memcpy(u, su, (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
+    memcpy(v, sv, (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
 }
 
 }  // extern "C"

@@ -203,9 +203,57 @@ def recon_fixtures():
     save("recon.npz", **out)
 
 
+def deblock_fixtures():
+    """deblock.npz: the edge filter on random lines (realistic and extreme
+    samples, every qp and strength, luma and chroma), and the deblocking of
+    whole frames: the reconstructions of recon.npz with their own records,
+    a blocky picture with random block types and q_index, and one with
+    samples near the int16 limits."""
+    ref = Reference()
+    rng = np.random.default_rng(20261205)
+    out = {}
+    n = 4096
+    base = rng.integers(-40, 300, n)
+    lines = (base[:, None] + rng.integers(-12, 13, (n, 8))).astype(np.int16)
+    wild = rng.random(n) < 0.15
+    lines[wild] = rng.integers(-32768, 32768, (int(wild.sum()), 8)).astype(np.int16)
+    qp = rng.integers(0, 64, n).astype(np.int32)
+    st = rng.integers(0, 3, n).astype(np.int32)
+    luma = rng.integers(0, 2, n).astype(np.int32)
+    out["line_in"], out["line_qp"], out["line_strength"], out["line_luma"] = lines, qp, st, luma
+    out["line_out"] = np.stack([ref.deblock_line(l, int(q), int(s_), int(k))
+                                for l, q, s_, k in zip(lines, qp, st, luma)])
+    g = np.load(os.path.join(HERE, "frames.npz"))
+    r = np.load(os.path.join(HERE, "recon.npz"))
+    for tag in ("p1", "p4", "b"):
+        planes = [r[f"{tag}_perturbed_{c}"] for c in "yuv"]
+        y, u, v = ref.deblock_frame(g[f"{tag}_records"], *planes)
+        out[f"{tag}_y"], out[f"{tag}_u"], out[f"{tag}_v"] = y, u, v
+    w, h = 176, 144
+    pic = synth.blocky_picture(11, w, h, amp=10)
+    rec = synth.random_block_records(12, w, h)
+    out["blocky_in_y"], out["blocky_in_u"], out["blocky_in_v"] = pic
+    out["blocky_records"] = rec
+    out["blocky_y"], out["blocky_u"], out["blocky_v"] = ref.deblock_frame(rec, *pic)
+    # 8x8 blocks alternating near +32760 and -8: differences across edges
+    # wrap when stored as int16.
+    ext = []
+    for pw, ph in ((w, h), (w // 2, h // 2), (w // 2, h // 2)):
+        sign = (np.add.outer(np.arange(ph) // 8, np.arange(pw) // 8) % 2).astype(np.int32)
+        ext.append((np.where(sign == 1, 32760, -8) + rng.integers(-2, 3, (ph, pw))).astype(np.int16))
+    rec2 = synth.random_block_records(13, w, h, q_lo=40, q_hi=63, copy_frac=0.1)
+    out["extreme_in_y"], out["extreme_in_u"], out["extreme_in_v"] = ext
+    out["extreme_records"] = rec2
+    out["extreme_y"], out["extreme_u"], out["extreme_v"] = ref.deblock_frame(rec2, *ext)
+    save("deblock.npz", **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["recon"]:
         recon_fixtures()
+    elif sys.argv[1:] == ["deblock"]:
+        deblock_fixtures()
     else:
         main()
         recon_fixtures()
+        deblock_fixtures()

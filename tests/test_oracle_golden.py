@@ -147,3 +147,33 @@ def test_reconstruct_frames(oracle, tag, ftype, q, cur, refs):
         assert np.array_equal(y, r[f"{tag}_{name}_y"]), (tag, name)
         assert np.array_equal(u, r[f"{tag}_{name}_u"]), (tag, name)
         assert np.array_equal(v, r[f"{tag}_{name}_v"]), (tag, name)
+
+
+def test_deblock_lines(oracle):
+    """deblock.cpp:91-165 on random lines against the reference build
+    (deblock.npz), extreme samples included."""
+    g = load("deblock.npz")
+    for v, qp, s, lu, want in zip(g["line_in"], g["line_qp"], g["line_strength"], g["line_luma"],
+                                  g["line_out"]):
+        assert np.array_equal(oracle.deblock_line(v, int(qp), int(s), int(lu)), want), (v, qp, s, lu)
+
+
+@pytest.mark.parametrize("tag", ["p1", "p4", "b"])
+def test_deblock_reconstructed_frames(oracle, tag):
+    """deblock.cpp:222-306 on the reconstructions of recon.npz with the
+    records of their analysis."""
+    g, _ = _frame_fixture()
+    r, d = load("recon.npz"), load("deblock.npz")
+    out = oracle.deblock_frame(g[f"{tag}_records"], *(r[f"{tag}_perturbed_{c}"] for c in "yuv"))
+    for c, plane in zip("yuv", out):
+        assert np.array_equal(plane, d[f"{tag}_{c}"]), (tag, c)
+
+
+@pytest.mark.parametrize("tag", ["blocky", "extreme"])
+def test_deblock_synthetic_frames(oracle, tag):
+    """Random block types and q_index over a blocky picture, and over
+    samples near the int16 limits whose edge differences wrap."""
+    d = load("deblock.npz")
+    out = oracle.deblock_frame(d[f"{tag}_records"], *(d[f"{tag}_in_{c}"] for c in "yuv"))
+    for c, plane in zip("yuv", out):
+        assert np.array_equal(plane, d[f"{tag}_{c}"]), (tag, c)

@@ -143,3 +143,29 @@ def test_reconstruct_frame_random(oracle, ref, ftype, nrefs, quality):
         b = ref.reconstruct_frame(ftype, quality, cur, refs, rec, levels)
         for pa, pb in zip(a, b):
             assert np.array_equal(pa, pb)
+
+
+def test_deblock_lines_random(oracle, ref):
+    """The edge filter of one line (deblock.cpp:91-165) against the
+    reference's own, every qp and strength, luma and chroma."""
+    rng = np.random.default_rng(606)
+    for k in range(20000):
+        v = rng.integers(-40, 300) + rng.integers(-12, 13, 8)
+        if k % 7 == 0:
+            v = rng.integers(-32768, 32768, 8)
+        qp, s, lu = int(rng.integers(0, 64)), int(rng.integers(0, 3)), int(rng.integers(0, 2))
+        assert np.array_equal(oracle.deblock_line(v, qp, s, lu), ref.deblock_line(v, qp, s, lu))
+
+
+@pytest.mark.parametrize("w,h,seed", [(64, 48, 1), (176, 144, 2), (320, 192, 3), (272, 160, 4)])
+def test_deblock_frame_random(oracle, ref, w, h, seed):
+    """Whole-frame deblocking in the reference's raster order against
+    deblock_image_filter itself: blocky pictures, random block types and
+    q_index (copy pairs, mixed pairs, internal and MB edges)."""
+    pic = synth.blocky_picture(seed, w, h, amp=6 + 3 * seed)
+    rec = synth.random_block_records(seed + 100, w, h)
+    a = oracle.deblock_frame(rec, *pic)
+    b = ref.deblock_frame(rec, *pic)
+    for c, x, y in zip("yuv", a, b):
+        assert np.array_equal(x, y), c
+    assert any((x != p).any() for x, p in zip(a, pic)), "nothing filtered"
