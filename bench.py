@@ -613,6 +613,20 @@ def run_recon(args):
     ctx.synchronize()
     call_s = (time.perf_counter() - t0) / reps
     d_s = float(np.mean(ctx.launch_times())) / 1e3
+    # Reconstruction then deblocking, as two calls and as the fused call.
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.reconstruct(6)
+        s.deblock(6)
+    ctx.synchronize()
+    pair_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.reconstruct_deblocked(6)
+    ctx.synchronize()
+    fused_s = (time.perf_counter() - t0) / reps
+    ctx.launch_times()
+    got_fused = s.download(6)
     samples = aw * ah * 3 // 2
     dbytes = samples * 4 + n * 64
     res["deblock"] = {
@@ -620,6 +634,8 @@ def run_recon(args):
         "kernel": "deblock_kernel",
         "kernel_us_avg": round(d_s * 1e6, 2),
         "call_us": round(call_s * 1e6, 2),
+        "reconstruct_then_deblock_us": round(pair_s * 1e6, 2),
+        "reconstruct_deblocked_us": round(fused_s * 1e6, 2),
         "frames_per_s": round(1.0 / d_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(dbytes / d_s / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -634,8 +650,9 @@ def run_recon(args):
                                    rec, lv)
         res["bit_exact_vs_cpu"] = all(np.array_equal(a, b) for a, b in zip(got, want))
         want_db = o.deblock_frame(rec, *want)
-        res["deblock"]["bit_exact_vs_cpu"] = all(np.array_equal(a, b)
-                                                 for a, b in zip(got_db, want_db))
+        res["deblock"]["bit_exact_vs_cpu"] = all(
+            np.array_equal(a, b) and np.array_equal(c, b)
+            for a, b, c in zip(got_db, want_db, got_fused))
     s.close()
     ctx.close()
     return res

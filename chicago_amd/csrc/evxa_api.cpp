@@ -225,6 +225,7 @@ struct evxa_stream {
     DevBuf recon_levels;   // levels handed in for reconstruction
     DevBuf deblock_src;    // unfiltered copy of the picture being deblocked (packed planes)
     DevBuf deblock_records;  // block descriptors handed in for deblocking
+    DevBuf recon_flag;     // range flag of a reconstruction headed for the filter (unused)
     uint64_t bytes = 0;
 
     int16_t *slot_base(int32_t slot) const
@@ -861,52 +862,114 @@ int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeff
     });
 }
 
-int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
+}  // extern "C"
+
+namespace {
+
+// Events around one kernel of the context's queue, for evxa_launch_times.
+template <class F>
+void timed(evxa_context *ctx, F &&launch)
 {
-    return guarded([&] {
-        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
-        evxa_context *ctx = s->ctx;
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        if (s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
-        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
-        check_slot(s, dst_slot);
-        const DevJob &j = s->last_job;
-        for (int k = 0; k < j.n_refs; k++) {
-            if (s->last_refs[k] == dst_slot)
-                fail(EVXA_ERROR_INVALIDARG, "destination slot is a reference of the frame");
-            for (int m = 0; m < k; m++)
-                if (((j.target_ids >> (8 * k)) & 0xFF) == ((j.target_ids >> (8 * m)) & 0xFF))
-                    fail(EVXA_ERROR_INVALIDARG, "reconstruction needs distinct target ids");
-        }
-        set_device(ctx);
-        const size_t n = (size_t)s->wb * s->hb;
-        const int16_t *lv = j.levels;
-        if (levels) {
-            lazy_alloc(s, s->recon_levels, n * EVXA_MB_SAMPLES * sizeof(int16_t));
-            check(hipMemcpyAsync(s->recon_levels.p, levels, n * EVXA_MB_SAMPLES * sizeof(int16_t),
-                                 hipMemcpyHostToDevice, ctx->stream),
-                  "upload levels");
-            lv = s->recon_levels.as<int16_t>();
-        }
-        const DevPic dst = s->pic(dst_slot);
-        int32_t *wide = s->wide.as<int32_t>() + dst_slot;
+    if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+    const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+    check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
+    launch();
+    check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
+    ctx->ring_head++;
+    ctx->last_start = ev.first;
+    ctx->last_stop = ev.second;
+}
+
+// The packed unfiltered planes the deblocking kernel reads (luma pitch w,
+// chroma pitch w / 2), allocated on first use.
+void deblock_input(evxa_stream *s, int16_t **planes)
+{
+    const size_t ny = (size_t)s->w * s->h, nc = ny / 4;
+    lazy_alloc(s, s->deblock_src, (ny + 2 * nc) * sizeof(int16_t));
+    int16_t *scr = s->deblock_src.as<int16_t>();
+    planes[0] = scr;
+    planes[1] = scr + ny;
+    planes[2] = scr + ny + nc;
+}
+
+// evxa_reconstruct, optionally followed by the deblocking filter: then the
+// reconstruction goes to the filter's input planes, and the filter writes
+// the slot (and its apron).
+void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool filter)
+{
+    if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+    evxa_context *ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
+    if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+    check_slot(s, dst_slot);
+    const DevJob &j = s->last_job;
+    for (int k = 0; k < j.n_refs; k++) {
+        if (s->last_refs[k] == dst_slot)
+            fail(EVXA_ERROR_INVALIDARG, "destination slot is a reference of the frame");
+        for (int m = 0; m < k; m++)
+            if (((j.target_ids >> (8 * k)) & 0xFF) == ((j.target_ids >> (8 * m)) & 0xFF))
+                fail(EVXA_ERROR_INVALIDARG, "reconstruction needs distinct target ids");
+    }
+    set_device(ctx);
+    const size_t n = (size_t)s->wb * s->hb;
+    const int16_t *lv = j.levels;
+    if (levels) {
+        lazy_alloc(s, s->recon_levels, n * EVXA_MB_SAMPLES * sizeof(int16_t));
+        check(hipMemcpyAsync(s->recon_levels.p, levels, n * EVXA_MB_SAMPLES * sizeof(int16_t),
+                             hipMemcpyHostToDevice, ctx->stream),
+              "upload levels");
+        lv = s->recon_levels.as<int16_t>();
+    }
+    const DevPic dst = s->pic(dst_slot);
+    int32_t *wide = s->wide.as<int32_t>() + dst_slot;
+    if (!filter) {
         check(launch_reset(wide, 1, ctx->stream), "reset range flag");
-        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
-        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
-        check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
-        check(launch_recon(j, lv, dst, wide, ctx->stream), "reconstruction kernel");
-        check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
-        ctx->ring_head++;
-        ctx->last_start = ev.first;
-        ctx->last_stop = ev.second;
+        timed(ctx, [&] { check(launch_recon(j, lv, dst, wide, ctx->stream), "reconstruction kernel"); });
         // The slot is a picture like an uploaded one: refresh its apron (the
         // range flag came with the reconstruction).
         check(launch_apron3(const_cast<int16_t *>(dst.y), const_cast<int16_t *>(dst.u),
                             const_cast<int16_t *>(dst.v), dst.py, dst.pc, s->w, s->h, ctx->stream),
               "apron kernel");
-        // The caller's level buffer has been consumed once this returns.
-        if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    });
+    } else {
+        int16_t *src[3];
+        deblock_input(s, src);
+        lazy_alloc(s, s->recon_flag, sizeof(int32_t));
+        DevPic mid{};
+        mid.y = src[0];
+        mid.u = src[1];
+        mid.v = src[2];
+        mid.py = s->w;
+        mid.pc = s->w / 2;
+        const int32_t sp[3] = {s->w, s->w / 2, s->w / 2}, dp[3] = {dst.py, dst.pc, dst.pc};
+        int16_t *const out[3] = {const_cast<int16_t *>(dst.y), const_cast<int16_t *>(dst.u),
+                                 const_cast<int16_t *>(dst.v)};
+        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
+        timed(ctx, [&] {
+            check(launch_recon(j, lv, mid, s->recon_flag.as<int32_t>(), ctx->stream),
+                  "reconstruction kernel");
+        });
+        timed(ctx, [&] {
+            check(launch_deblock(src, sp, out, dp, s->w, s->h, j.records, wide, ctx->stream),
+                  "deblock kernel");
+        });
+    }
+    // The caller's level buffer has been consumed once this returns.
+    if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+}  // namespace
+
+extern "C" {
+
+int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
+{
+    return guarded([&] { reconstruct(s, dst_slot, levels, false); });
+}
+
+int evxa_reconstruct_deblocked(evxa_stream *s, int32_t dst_slot, const int16_t *levels)
+{
+    return guarded([&] { reconstruct(s, dst_slot, levels, true); });
 }
 
 int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
@@ -929,32 +992,20 @@ int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
             rec = s->deblock_records.as<evxa_block_record>();
         }
         // The filter reads the unfiltered picture around each tile, so it
-        // runs from a copy back into the slot.
-        const size_t ny = (size_t)s->w * s->h, nc = ny / 4;
-        lazy_alloc(s, s->deblock_src, (ny + 2 * nc) * sizeof(int16_t));
+        // runs from a packed copy back into the slot (and its apron).
+        int16_t *src[3];
+        deblock_input(s, src);
         const DevPic p = s->pic(slot);
-        int16_t *scr = s->deblock_src.as<int16_t>();
-        int16_t *const src[3] = {scr, scr + ny, scr + ny + nc};
         int16_t *const dst[3] = {const_cast<int16_t *>(p.y), const_cast<int16_t *>(p.u),
                                  const_cast<int16_t *>(p.v)};
         const int32_t sp[3] = {s->w, s->w / 2, s->w / 2}, dp[3] = {p.py, p.pc, p.pc};
-        for (int k = 0; k < 3; k++)
-            check(hipMemcpy2DAsync(src[k], (size_t)sp[k] * 2, dst[k], (size_t)dp[k] * 2,
-                                   (size_t)sp[k] * 2, k ? s->h / 2 : s->h, hipMemcpyDeviceToDevice,
-                                   ctx->stream),
-                  "copy picture");
         int32_t *wide = s->wide.as<int32_t>() + slot;
-        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
-        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
-        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
-        check(hipEventRecord(ev.first, ctx->stream), "hipEventRecord");
-        check(launch_deblock(src, sp, dst, dp, s->w, s->h, rec, wide, ctx->stream), "deblock kernel");
-        check(hipEventRecord(ev.second, ctx->stream), "hipEventRecord");
-        ctx->ring_head++;
-        ctx->last_start = ev.first;
-        ctx->last_stop = ev.second;
-        check(launch_apron3(dst[0], dst[1], dst[2], p.py, p.pc, s->w, s->h, ctx->stream),
-              "apron kernel");
+        check(launch_copy3(p.y, p.u, p.v, p.py, p.pc, src[0], s->w, s->h, wide, ctx->stream),
+              "copy kernel");
+        timed(ctx, [&] {
+            check(launch_deblock(src, sp, dst, dp, s->w, s->h, rec, wide, ctx->stream),
+                  "deblock kernel");
+        });
         // The caller's records have been consumed once this returns.
         if (records) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });

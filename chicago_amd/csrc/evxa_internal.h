@@ -106,12 +106,17 @@ hipError_t launch_apron3(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int3
 hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,
                         hipStream_t st);
 // The in-loop deblocking filter of a frame (luma w x h, MB descriptors in
-// `records`) from the unfiltered planes src (pitches sp) into dst (pitches
-// dp, distinct memory); ORs 1 into *wide if a stored sample lies outside
-// [0, 2047] (clear it first).  w and h are multiples of 16.
+// `records`) from the unfiltered planes src (pitches sp) into the picture
+// planes dst (pitches dp, distinct memory, with their kApron aprons, which
+// it refreshes); ORs 1 into *wide if a stored sample lies outside [0, 2047]
+// (clear it first).  w and h are multiples of 16.
 hipError_t launch_deblock(int16_t *const src[3], const int32_t sp[3], int16_t *const dst[3],
                           const int32_t dp[3], int32_t w, int32_t h,
                           const evxa_block_record *records, int32_t *wide, hipStream_t st);
+// Copies a picture's three plane interiors (luma w x h) to packed planes at
+// `out` and clears *wide.
+hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py, int32_t pc,
+                        int16_t *out, int32_t w, int32_t h, int32_t *wide, hipStream_t st);
 hipError_t launch_interp_blocks(DevPic pic, int32_t w, int32_t h, int32_t n, const int32_t *params,
                                 int16_t *out, hipStream_t st);
 hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, int32_t quality,

@@ -1950,22 +1950,25 @@ __device__ __forceinline__ void deblock_line(int (&v)[8], uint32_t ab, int stren
     // The differences are stored as int16 before the comparisons.
     const int dpq = (int16_t)abs(p0 - q0), dp = (int16_t)abs(p1 - p0), dq = (int16_t)abs(q1 - q0);
     if (dpq >= alpha || dp >= beta || dq >= beta) return;
-    auto rd = [](int x, int d) { return x < 0 ? (x - d / 2) / d : (x + d / 2) / d; };
+    // rounded_div (evx_math.h:228-236) by 8 and by 4 -- halves away from
+    // zero, C division -- as floor shifts: (x + d/2 - [x < 0]) >> log2(d).
+    auto rd8 = [](int x) { return (x + 4 + (x >> 31)) >> 3; };
+    auto rd4 = [](int x) { return (x + 2 + (x >> 31)) >> 2; };
     if (strength == 2) {
-        v[3] = (int16_t)rd(p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1, 8);
-        v[2] = (int16_t)rd(p2 + p1 + p0 + q0, 4);
-        v[4] = (int16_t)rd(p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2, 8);
-        v[5] = (int16_t)rd(p0 + q0 + q1 + q2, 4);
+        v[3] = (int16_t)rd8(p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1);
+        v[2] = (int16_t)rd4(p2 + p1 + p0 + q0);
+        v[4] = (int16_t)rd8(p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2);
+        v[5] = (int16_t)rd4(p0 + q0 + q1 + q2);
         if (luma) {
-            v[1] = (int16_t)rd(2 * p3 + 3 * p2 + p1 + p0 + q0, 8);
-            v[6] = (int16_t)rd(2 * q3 + 3 * q2 + q1 + q0 + p0, 8);
+            v[1] = (int16_t)rd8(2 * p3 + 3 * p2 + p1 + p0 + q0);
+            v[6] = (int16_t)rd8(2 * q3 + 3 * q2 + q1 + q0 + p0);
         }
     } else {
-        v[3] = (int16_t)rd(((q0 + p0) * 4) + p1 - q1, 8);
-        v[4] = (int16_t)rd(((q0 + p0) * 4) + q1 - p1, 8);
+        v[3] = (int16_t)rd8(((q0 + p0) * 4) + p1 - q1);
+        v[4] = (int16_t)rd8(((q0 + p0) * 4) + q1 - p1);
         if (luma) {
-            v[2] = (int16_t)rd((p2 * 4) + (p0 * 2) + (q0 * 2), 8);
-            v[5] = (int16_t)rd((q2 * 4) + (q0 * 2) + (p0 * 2), 8);
+            v[2] = (int16_t)rd8((p2 * 4) + (p0 * 2) + (q0 * 2));
+            v[5] = (int16_t)rd8((q2 * 4) + (q0 * 2) + (p0 * 2));
         }
     }
 }
@@ -2024,7 +2027,9 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
     const int pl = (tile >= a.first[1]) + (tile >= a.first[2]);
     const bool luma = pl == 0;
     const int pw = luma ? a.w : a.w / 2, ph = luma ? a.h : a.h / 2;
-    const int mb = luma ? 16 : 8, off = luma ? 0 : 2;   // config.h:68 for chroma
+    // Blocks of the table are 1 << lg samples square; chroma thresholds use
+    // qp + 2 (config.h:68).
+    const int lg = luma ? 4 : 3, off = luma ? 0 : 2;
     const int tiles_x = (pw + kDbTw - 1) / kDbTw;
     const int t = tile - a.first[pl];
     const int C0 = (t % tiles_x) * kDbTw, R0 = (t / tiles_x) * kDbTh;
@@ -2032,46 +2037,67 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
     const int sp = a.sp[pl];
 
     // Unfiltered samples of the tile and its apron (zero outside the plane;
-    // no edge reaches them).  Chunks of 4 samples: w is a multiple of 8.
-    for (int i = tid; i < kDbRows * (kDbCols / 4); i += kDbThreads) {
+    // no edge reaches them), in chunks of 4 samples (w is a multiple of 8),
+    // and the block descriptors around the tile.  Every global load is
+    // issued before the first LDS store, so their latencies overlap.
+    constexpr int kChunks = kDbRows * (kDbCols / 4);
+    constexpr int kLoadSteps = (kChunks + kDbThreads - 1) / kDbThreads;
+    static_assert(kDbBlkRows * kDbBlkCols <= kDbThreads, "one descriptor per thread");
+    int2 ld[kLoadSteps];
+#pragma unroll
+    for (int k = 0; k < kLoadSteps; k++) {
+        const int i = tid + k * kDbThreads;
         const int lr = i / (kDbCols / 4), q = i % (kDbCols / 4);
         const int r = R0 - 4 + lr, c = C0 - 4 + 4 * q;
-        int2 v = make_int2(0, 0);
-        if (r >= 0 && r < ph && c >= 0 && c < pw)
-            v = *reinterpret_cast<const int2 *>(src + (size_t)r * sp + c);
-        *reinterpret_cast<int2 *>(&sh.t[lr][4 * q]) = v;
+        ld[k] = make_int2(0, 0);
+        if (i < kChunks && r >= 0 && r < ph && c >= 0 && c < pw)
+            ld[k] = *reinterpret_cast<const int2 *>(src + (size_t)r * sp + c);
     }
-    // Block descriptors around the tile.
-    const int bx0 = max(C0 - 4, 0) / mb, by0 = max(R0 - 4, 0) / mb;
-    for (int i = tid; i < kDbBlkRows * kDbBlkCols; i += kDbThreads) {
-        const int bx = bx0 + i % kDbBlkCols, by = by0 + i / kDbBlkCols;
-        uint32_t e = 0;
-        if (bx < a.wb && by < a.hb) {
+    const int bx0 = max(C0 - 4, 0) >> lg, by0 = max(R0 - 4, 0) >> lg;
+    uint32_t bt = 0, bq = 0;
+    {
+        const int bx = bx0 + tid % kDbBlkCols, by = by0 + tid / kDbBlkCols;
+        if (tid < kDbBlkRows * kDbBlkCols && bx < a.wb && by < a.hb) {
             const uint32_t *rw = reinterpret_cast<const uint32_t *>(records + by * a.wb + bx);
-            // block_type: byte 0; q_index: byte 18.
-            e = ((rw[0] & 4) ? 0x100u : 0u) | ((rw[4] >> 16) & 0xFF);
+            bt = rw[0];   // block_type: byte 0
+            bq = rw[4];   // q_index: byte 18
         }
-        sh.blk[i / kDbBlkCols][i % kDbBlkCols] = (uint16_t)e;
     }
+#pragma unroll
+    for (int k = 0; k < kLoadSteps; k++) {
+        const int i = tid + k * kDbThreads;
+        if (i < kChunks)
+            *reinterpret_cast<int2 *>(&sh.t[i / (kDbCols / 4)][4 * (i % (kDbCols / 4))]) = ld[k];
+    }
+    if (tid < kDbBlkRows * kDbBlkCols)
+        sh.blk[tid / kDbBlkCols][tid % kDbBlkCols] =
+            (uint16_t)(((bt & 4) ? 0x100u : 0u) | ((bq >> 16) & 0xFF));
     if (tid < 64) sh.ab[tid] = kDeblockAlpha[tid] | kDeblockBeta[tid] << 8;
     __syncthreads();
 
-    auto blk = [&](int x, int y) -> uint32_t { return sh.blk[y / mb - by0][x / mb - bx0]; };
+    auto blk = [&](int x, int y) -> uint32_t { return sh.blk[(y >> lg) - by0][(x >> lg) - bx0]; };
+    const int inner = (1 << lg) - 1;   // an edge inside one block: position & inner != 0
     // The line of row r across vertical edge e (column C0 + 8e); false if
     // the edge is not filtered.
     auto vertical = [&](int r, int e, int (&v)[8]) -> bool {
         const int i = C0 + 8 * e;
         if (i < 8 || i >= pw) return false;
         uint32_t ab;
-        const int s = deblock_edge(sh, blk(i - 1, r), blk(i, r), (i - 1) / mb == i / mb, off, ab);
+        const int s = deblock_edge(sh, blk(i - 1, r), blk(i, r), (i & inner) != 0, off, ab);
         if (!s) return false;
         deblock_line(v, ab, s, luma);
         return true;
     };
 
-    // Stage 1: vertical edges of the rows no horizontal edge writes first.
-    for (int i = tid; i < (kDbTh + 4) * kDbEdges; i += kDbThreads) {
-        const int lr = i / kDbEdges, e = i % kDbEdges;
+    // Stage 1: vertical edges of the rows no horizontal edge writes first:
+    // the top apron (rows 4..7 of the band above), rows 3..7 of each band of
+    // the tile, and rows 0..2 of the picture's first band.
+    constexpr int kS1Rows = 4 + 5 * (kDbTh / 8) + 3;
+    for (int i = tid; i < kS1Rows * kDbEdges; i += kDbThreads) {
+        const int sr = i / kDbEdges, e = i % kDbEdges;
+        const int lr = sr < 4 ? sr
+                     : sr < kS1Rows - 3 ? 4 + 8 * ((sr - 4) / 5) + 3 + (sr - 4) % 5
+                                        : 4 + sr - (kS1Rows - 3);
         const int r = R0 - 4 + lr;
         if (r < 0 || r >= ph || (r >= 8 && (r & 7) < 3)) continue;
         int v[8];
@@ -2088,7 +2114,7 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
         const int j = R0 + 8 * hb, c = C0 - 4 + 2 * cp;
         if (j < 8 || j >= ph || c < 0 || c >= pw) continue;
         uint32_t ab;
-        const int s = deblock_edge(sh, blk(c, j - 1), blk(c, j), (j - 1) / mb == j / mb, off, ab);
+        const int s = deblock_edge(sh, blk(c, j - 1), blk(c, j), (j & inner) != 0, off, ab);
         if (!s) continue;
         int v0[8], v1[8];
 #pragma unroll
@@ -2139,6 +2165,74 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
     }
     // The picture's range flag (range_kernel's rule).
     if (__ballot(outside) != 0 && lane_id() == 0) atomicOr(wide, 1);
+
+    // The apron (kApron samples on every side, the clamped sampling of
+    // interp.cpp:8-14): tiles on the plane's border replicate their final
+    // edge samples, 8 per store, as apron3_kernel does for uploads.
+    auto final_row = [&](int r) -> const int16_t * {   // LDS row of r, column C0 at [0]
+        return r >= 8 && (r & 7) == 3 ? &sh.r3[(r - R0) >> 3][4] : &sh.t[r - R0 + 4][4];
+    };
+    auto fill8 = [&](int r, int c, int val) {
+        const int d = (val & 0xFFFF) | (val << 16);
+        *reinterpret_cast<int4 *>(dst + (ptrdiff_t)r * dpitch + c) = make_int4(d, d, d, d);
+    };
+    const bool left = C0 == 0, right = C0 + kDbTw >= pw, top = R0 == 0, bottom = R0 + kDbTh >= ph;
+    const int rows_here = min(kDbTh, ph - R0), chunks_here = min(kDbTw, pw - C0) / 8;
+    constexpr int kSide = kApron / 8;                    // 8-sample chunks per side row
+    if (left || right) {
+        for (int i = tid; i < 2 * rows_here * kSide; i += kDbThreads) {
+            const bool rs = i >= rows_here * kSide;
+            const int k = rs ? i - rows_here * kSide : i;
+            if (rs ? !right : !left) continue;
+            const int r = R0 + k / kSide;
+            fill8(r, rs ? pw + 8 * (k % kSide) : -kApron + 8 * (k % kSide),
+                  final_row(r)[rs ? pw - 1 - C0 : 0]);
+        }
+    }
+    if (top || bottom) {
+        const int cl = left ? kSide : 0, cr = right ? kSide : 0;
+        const int per = cl + chunks_here + cr;           // chunks per apron row
+        for (int i = tid; i < 2 * kApron * per; i += kDbThreads) {
+            const bool bs = i >= kApron * per;
+            const int k = bs ? i - kApron * per : i;
+            if (bs ? !bottom : !top) continue;
+            const int r = bs ? ph + k / per : -kApron + k / per;
+            const int16_t *src_row = final_row(bs ? ph - 1 : 0);
+            const int ch = k % per;
+            if (ch < cl) {
+                fill8(r, -kApron + 8 * ch, src_row[0]);
+            } else if (ch < cl + chunks_here) {
+                const int c = C0 + 8 * (ch - cl);
+                const int2 lo = *reinterpret_cast<const int2 *>(src_row + (c - C0));
+                const int2 hi = *reinterpret_cast<const int2 *>(src_row + (c - C0) + 4);
+                *reinterpret_cast<int4 *>(dst + (ptrdiff_t)r * dpitch + c) =
+                    make_int4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                fill8(r, pw + 8 * (ch - cl - chunks_here), src_row[pw - 1 - C0]);
+            }
+        }
+    }
+}
+
+// Copies the interiors of a picture's three planes (luma w x h) into packed
+// planes at `out` (luma, then the two chroma planes; 8 samples per thread
+// and step, blockIdx.y = plane) and clears the picture's range flag for the
+// deblocking kernel that writes the planes back.
+__global__ void copy3_kernel(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py,
+                             int32_t pc, int16_t *out, int32_t w, int32_t h, int32_t *wide)
+{
+    const int plane = blockIdx.y;
+    if (plane == 0 && blockIdx.x == 0 && threadIdx.x == 0) *wide = 0;
+    const int pw = plane ? w / 2 : w, ph = plane ? h / 2 : h;
+    const int16_t *src = plane == 0 ? y : plane == 1 ? u : v;
+    const int pitch = plane ? pc : py;
+    int16_t *o = out + (plane == 0 ? 0 : (size_t)w * h + (plane - 1) * (size_t)(w / 2) * (h / 2));
+    const int per = pw / 8, n = per * ph;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int r = i / per, c = 8 * (i % per);
+        *reinterpret_cast<int4 *>(o + (size_t)r * pw + c) =
+            *reinterpret_cast<const int4 *>(src + (size_t)r * pitch + c);
+    }
 }
 
 
@@ -2282,6 +2376,15 @@ hipError_t launch_deblock(int16_t *const src[3], const int32_t sp[3], int16_t *c
     a.wb = w / 16;
     a.hb = h / 16;
     hipLaunchKernelGGL(deblock_kernel, dim3(n), dim3(kDbThreads), 0, st, a, records, wide);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py, int32_t pc,
+                        int16_t *out, int32_t w, int32_t h, int32_t *wide, hipStream_t st)
+{
+    const int blocks = min((w / 8 * h + 255) / 256, 1024);
+    hipLaunchKernelGGL(copy3_kernel, dim3(blocks, 3), dim3(256), 0, st, y, u, v, py, pc, out, w, h,
+                       wide);
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ EXPORTS = (
     "evxa_upload_picture", "evxa_stream_reset_temporal", "evxa_analyze_frame",
     "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_launch_times",
     "evxa_read_results", "evxa_download_picture", "evxa_reconstruct", "evxa_deblock",
+    "evxa_reconstruct_deblocked",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
@@ -123,6 +124,7 @@ def load_library(path: str = LIB_PATH):
         "evxa_download_picture": (C.c_int, [vp, C.c_int32, I16P, I16P, I16P, C.c_int32, C.c_int32]),
         "evxa_reconstruct": (C.c_int, [vp, C.c_int32, I16P]),
         "evxa_deblock": (C.c_int, [vp, C.c_int32, vp]),
+        "evxa_reconstruct_deblocked": (C.c_int, [vp, C.c_int32, I16P]),
         "evxa_graph_create": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32,
                                         C.POINTER(C.c_int32)]),
         "evxa_graph_launch": (C.c_int, [vp, C.c_int32]),
@@ -332,6 +334,15 @@ class Stream:
             levels = np.ascontiguousarray(levels, np.int16)
             assert levels.shape == (self.n_mbs, MB_SAMPLES)
         _check(_lib.evxa_reconstruct(self.h, dst_slot, None if levels is None else _p16(levels)))
+
+    def reconstruct_deblocked(self, dst_slot, levels=None):
+        """reconstruct() then deblock() with the frame's own records, in one
+        call (no intermediate copy of the picture)."""
+        if levels is not None:
+            levels = np.ascontiguousarray(levels, np.int16)
+            assert levels.shape == (self.n_mbs, MB_SAMPLES)
+        _check(_lib.evxa_reconstruct_deblocked(self.h, dst_slot,
+                                               None if levels is None else _p16(levels)))
 
     def deblock(self, slot, records=None):
         """Deblock picture slot `slot` in place (the block types and q_index

@@ -183,6 +183,14 @@ int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels);
  * (encode.cpp:968) and the decoder's (decode.cpp:583). */
 int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records);
 
+/* evxa_reconstruct followed by evxa_deblock with the frame's own records,
+ * with the same result in `dst_slot`, in one call: the reconstruction is
+ * written straight to the filter's input, which saves the picture copy
+ * evxa_deblock makes.  The encoder's decode_block calls then its
+ * deblock_image_filter (encode.cpp:806, 918, 968); the decoder's
+ * decode_slice then deblock_image_filter (decode.cpp:466-583). */
+int evxa_reconstruct_deblocked(evxa_stream *s, int32_t dst_slot, const int16_t *levels);
+
 /* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
 
 /* Capture the launch of `n` frame descriptors into a replayable graph.

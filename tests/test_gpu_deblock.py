@@ -77,6 +77,14 @@ def test_deblock_after_reconstruct(ctx, oracle, ftype):
     recon = oracle.reconstruct_frame(*args, rec, lv2)
     want = oracle.deblock_frame(rec, *recon)
     _check_planes(s.download(3), want, f"type {ftype}")
+    # The fused entry point (reconstruction straight into the filter's
+    # input) leaves the same picture, also with the stream's own levels.
+    s.upload(3, *frames[0])
+    s.reconstruct_deblocked(3, lv2)
+    _check_planes(s.download(3), want, f"type {ftype}, fused")
+    s.reconstruct_deblocked(3)
+    want_own = oracle.deblock_frame(rec, *oracle.reconstruct_frame(*args, rec, lv))
+    _check_planes(s.download(3), want_own, f"type {ftype}, fused, own levels")
     s.close()
 
 
@@ -98,6 +106,15 @@ def test_deblocked_picture_as_reference(ctx, oracle):
     filt = oracle.deblock_frame(rec, *dark)
     assert min(int(p.min()) for p in filt) < 0, "no sample left the [0, 2047] range"
     _check_planes(s.download(0), filt, "filtered reference")
+    # Sub-pel blocks along the border read the refreshed apron.
+    xs = [0, w - 16, 0, w - 16, 48, 0, w - 16, 64]
+    ys = [0, 0, h - 16, h - 16, 0, 32, 48, h - 16]
+    fp = Picture(*filt)
+    for dx, dy in ((-1, -1), (1, 1), (-1, 1), (1, 0), (0, -1)):
+        for q in (0, 1):
+            got = s.interpolate_subpixel_6tap(0, xs, ys, [dx] * 8, [dy] * 8, [q] * 8)
+            for k in range(8):
+                assert np.array_equal(got[k], oracle.interp(fp, xs[k], ys[k], dx, dy, q)), (k, dx, dy, q)
     s.analyze_p(10, 1, [0], temporal_mvp=False)
     got, gco, glv = s.results()
     orec, oco, olv = oracle.analyze_frame(1, 10, Picture(*cur), [Picture(*filt)])
@@ -107,10 +124,36 @@ def test_deblocked_picture_as_reference(ctx, oracle):
     s.close()
 
 
+def test_fused_reconstruction_as_reference(ctx, oracle):
+    """reconstruct_deblocked refreshes the apron and range flag of the slot
+    it writes: frame 2 searched against the filtered reconstruction of
+    frame 1 matches the oracle."""
+    frames = synth.sequence_c1(83, 3, 128, 96, local_range=10)
+    h, w = frames[0][0].shape
+    s = ctx.open_stream(w, h, 4)
+    for k, f in enumerate(frames):
+        s.upload(k, *f)
+    pics = [Picture(*f) for f in frames]
+    s.analyze_p(20, 1, [0])
+    rec1, _, lv1 = s.results()
+    s.reconstruct_deblocked(3)
+    filt = oracle.deblock_frame(rec1, *oracle.reconstruct_frame(1, 20, pics[1], [pics[0]], rec1, lv1))
+    _check_planes(s.download(3), filt, "frame 1")
+    s.analyze_p(20, 2, [3], temporal_mvp=False)
+    rec2, co2, lv2 = s.results()
+    orec, oco, olv = oracle.analyze_frame(1, 20, pics[2], [Picture(*filt)])
+    for f in [n for n in evxa.RECORD_DTYPE.names if not n.startswith("reserved")]:
+        assert np.array_equal(rec2[f], orec[f]), f
+    assert np.array_equal(co2, oco) and np.array_equal(lv2, olv)
+    s.close()
+
+
 def test_deblock_refusals(ctx):
     s = ctx.open_stream(64, 48, 2)
     with pytest.raises(evxa.EvxaError):
         s.deblock(0)                           # no records and nothing analysed
+    with pytest.raises(evxa.EvxaError):
+        s.reconstruct_deblocked(1)             # nothing analysed
     with pytest.raises(evxa.EvxaError):
         s.deblock(2, synth.random_block_records(1, 64, 48))   # no such slot
     s.close()
