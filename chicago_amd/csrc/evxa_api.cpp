@@ -226,6 +226,9 @@ struct evxa_stream {
     DevBuf deblock_src;    // unfiltered copy of the picture being deblocked (packed planes)
     DevBuf deblock_records;  // block descriptors handed in for deblocking
     DevBuf recon_flag;     // range flag of a reconstruction headed for the filter (unused)
+    DevBuf sao_order;      // MB of each SAO ticket, anti-diagonal order
+    DevBuf sao_words;      // per MB: published SAO decision; then the SAO counters
+    DevBuf sao_table;      // evxa_sao_info per MB
     uint64_t bytes = 0;
 
     int16_t *slot_base(int32_t slot) const
@@ -1008,6 +1011,127 @@ int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
         });
         // The caller's records have been consumed once this returns.
         if (records) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int kSaoCounters = 64;
+
+// The SAO buffers of a stream (allocated on first use; the ticket order is
+// written once).
+void sao_buffers(evxa_stream *s)
+{
+    const size_t n = (size_t)s->wb * s->hb;
+    if (!s->sao_order.p) {
+        lazy_alloc(s, s->sao_order, n * sizeof(uint32_t));
+        std::vector<uint32_t> order;
+        order.reserve(n);
+        for (int32_t d = 0; d < s->wb + s->hb - 1; d++)
+            for (int32_t by = 0; by < s->hb; by++) {
+                const int32_t bx = d - by;
+                if (bx >= 0 && bx < s->wb) order.push_back((uint32_t)(by * s->wb + bx));
+            }
+        check(hipMemcpy(s->sao_order.p, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "upload SAO order");
+    }
+    lazy_alloc(s, s->sao_words, n * sizeof(uint64_t) + kSaoCounters * sizeof(int32_t));
+    lazy_alloc(s, s->sao_table, n * sizeof(evxa_sao_info));
+}
+
+int32_t *sao_counters(evxa_stream *s)
+{
+    return reinterpret_cast<int32_t *>(s->sao_words.as<uint64_t>() + (size_t)s->wb * s->hb);
+}
+
+// The unfiltered copy of `slot` SAO reads, then SAO per the stream's SAO
+// table into the slot and its apron; the error flags are read back.
+void sao_apply_slot(evxa_stream *s, int32_t slot, int16_t *const rec[3])
+{
+    evxa_context *ctx = s->ctx;
+    const DevPic p = s->pic(slot);
+    int16_t *const dst[3] = {const_cast<int16_t *>(p.y), const_cast<int16_t *>(p.u),
+                             const_cast<int16_t *>(p.v)};
+    const int32_t dp[3] = {p.py, p.pc, p.pc};
+    int32_t *counters = sao_counters(s);
+    timed(ctx, [&] {
+        check(launch_sao_apply(rec, dst, dp, s->w, s->h, s->sao_table.as<evxa_sao_info>(),
+                               s->wide.as<int32_t>() + slot, counters + 1, ctx->stream),
+              "SAO apply kernel");
+    });
+    check(launch_apron3(dst[0], dst[1], dst[2], p.py, p.pc, s->w, s->h, ctx->stream), "apron kernel");
+    int32_t err = 0;
+    check(hipMemcpyAsync(&err, counters + 1, sizeof(err), hipMemcpyDeviceToHost, ctx->stream),
+          "read SAO flags");
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (err & 1) fail(EVXA_ERROR_OPERATION_TIMEDOUT, "SAO dependency wait timed out");
+    if (err) fail(EVXA_ERROR_EXECUTION_FAILURE, "SAO chain deeper than three blocks");
+}
+
+}  // namespace
+
+extern "C" {
+
+int evxa_sao(evxa_stream *s, int32_t slot, int32_t src_slot, int32_t quality, evxa_sao_info *table)
+{
+    return guarded([&] {
+        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        check_slot(s, slot);
+        check_slot(s, src_slot);
+        if (slot == src_slot) fail(EVXA_ERROR_INVALIDARG, "the source and the reconstruction share a slot");
+        set_device(ctx);
+        int16_t *rec[3];
+        deblock_input(s, rec);
+        sao_buffers(s);
+        const size_t n = (size_t)s->wb * s->hb;
+        const DevPic p = s->pic(slot);
+        // The decisions and the filter read the unfiltered reconstruction.
+        check(launch_copy3(p.y, p.u, p.v, p.py, p.pc, rec[0], s->w, s->h,
+                           s->wide.as<int32_t>() + slot, ctx->stream),
+              "copy kernel");
+        check(launch_reset(reinterpret_cast<int32_t *>(s->sao_words.p), (int32_t)(2 * n) + kSaoCounters,
+                           ctx->stream),
+              "reset SAO words");
+        timed(ctx, [&] {
+            check(launch_sao_decide(rec[0], rec[1], s->pic(src_slot), s->w, s->h, lambda_of(quality),
+                                    s->sao_order.as<uint32_t>(), s->sao_words.as<uint64_t>(),
+                                    s->sao_table.as<evxa_sao_info>(), sao_counters(s), ctx->stream),
+                  "SAO decision kernel");
+        });
+        sao_apply_slot(s, slot, rec);
+        if (table)
+            check(hipMemcpy(table, s->sao_table.p, n * sizeof(evxa_sao_info), hipMemcpyDeviceToHost),
+                  "download SAO table");
+    });
+}
+
+int evxa_sao_filter(evxa_stream *s, int32_t slot, const evxa_sao_info *table)
+{
+    return guarded([&] {
+        if (!s || !table) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        check_slot(s, slot);
+        set_device(ctx);
+        int16_t *rec[3];
+        deblock_input(s, rec);
+        sao_buffers(s);
+        const size_t n = (size_t)s->wb * s->hb;
+        check(hipMemcpyAsync(s->sao_table.p, table, n * sizeof(evxa_sao_info), hipMemcpyHostToDevice,
+                             ctx->stream),
+              "upload SAO table");
+        check(launch_reset(sao_counters(s), kSaoCounters, ctx->stream), "reset SAO flags");
+        const DevPic p = s->pic(slot);
+        check(launch_copy3(p.y, p.u, p.v, p.py, p.pc, rec[0], s->w, s->h,
+                           s->wide.as<int32_t>() + slot, ctx->stream),
+              "copy kernel");
+        sao_apply_slot(s, slot, rec);
     });
 }
 

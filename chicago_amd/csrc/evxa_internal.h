@@ -117,6 +117,21 @@ hipError_t launch_deblock(int16_t *const src[3], const int32_t sp[3], int16_t *c
 // `out` and clears *wide.
 hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py, int32_t pc,
                         int16_t *out, int32_t w, int32_t h, int32_t *wide, hipStream_t st);
+// SAO decisions of a frame (sao.cpp:378-590): the unfiltered reconstruction
+// (packed luma and U planes), the frame's source picture, lambda; `order`
+// lists the MBs in anti-diagonal order, `words` (one per MB) and counters
+// [0..1] start at zero; decisions go to `table`; counters[1] gets error
+// flags (1: a wait timed out, 2: an SAO chain deeper than three blocks).
+hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const DevPic &src,
+                             int32_t w, int32_t h, float lambda, const uint32_t *order,
+                             uint64_t *words, evxa_sao_info *table, int32_t *counters,
+                             hipStream_t st);
+// SAO of a frame per `table` from the unfiltered packed planes `rec` into
+// the picture planes `dst` (pitches dp); ORs 1 into *wide if a stored sample
+// lies outside [0, 2047] (clear it first) and error flags into *error.
+hipError_t launch_sao_apply(int16_t *const rec[3], int16_t *const dst[3], const int32_t dp[3],
+                            int32_t w, int32_t h, const evxa_sao_info *table, int32_t *wide,
+                            int32_t *error, hipStream_t st);
 hipError_t launch_interp_blocks(DevPic pic, int32_t w, int32_t h, int32_t n, const int32_t *params,
                                 int16_t *out, hipStream_t st);
 hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, int32_t quality,

@@ -2236,6 +2236,464 @@ __global__ void copy3_kernel(const int16_t *y, const int16_t *u, const int16_t *
 }
 
 
+// ------------------------------------------------------------------ SAO --
+
+constexpr int kSaoOff = 0, kSaoEdge = 1, kSaoBand = 2;   // sao.h:46-48
+
+// sao.cpp:12-16
+__device__ __forceinline__ int sao_clamp(int v) { return min(max(v, 0), 255); }
+
+// sao.cpp:19-24: neighbour a of each edge class; neighbour b mirrors it.
+__device__ __forceinline__ void sao_neighbour(int cls, int &dx, int &dy)
+{
+    dx = cls == 3 ? 1 : cls == 1 ? 0 : -1;
+    dy = cls == 0 ? 0 : -1;
+}
+
+// sao.cpp:26-37: valley 0, concave 1, convex 2, peak 3, none 4.
+__device__ __forceinline__ int sao_category(int p, int a, int b)
+{
+    const int s = ((a > p) - (a < p)) + ((b > p) - (b < p));
+    return s == 2 ? 0 : s == 1 ? 1 : s == -1 ? 2 : s == -2 ? 3 : 4;
+}
+
+struct SaoP {
+    int type, cls, band;
+    int off[4];
+};
+
+// Decided params in 25 bits: type | class << 2 | band << 4 | the four
+// offsets (-7..7) as 4-bit two's complement from bit 9.
+__device__ __forceinline__ uint32_t sao_pack(const SaoP &p)
+{
+    uint32_t v = (uint32_t)p.type | (uint32_t)p.cls << 2 | (uint32_t)p.band << 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v |= ((uint32_t)p.off[k] & 0xF) << (9 + 4 * k);
+    return v;
+}
+
+__device__ __forceinline__ SaoP sao_unpack(uint32_t v)
+{
+    SaoP p;
+    p.type = v & 3;
+    p.cls = (v >> 2) & 3;
+    p.band = (v >> 4) & 31;
+#pragma unroll
+    for (int k = 0; k < 4; k++) p.off[k] = (int)(v << (32 - 13 - 4 * k)) >> 28;
+    return p;
+}
+
+// An evxa_sao_params as two 32-bit words.
+__device__ __forceinline__ SaoP sao_from_words(uint32_t w0, uint32_t w1)
+{
+    SaoP p;
+    p.type = w0 & 0xFF;
+    p.cls = (w0 >> 8) & 0xFF;
+    p.band = (w0 >> 16) & 0xFF;
+#pragma unroll
+    for (int k = 0; k < 4; k++) p.off[k] = (int8_t)(w1 >> (8 * k));
+    return p;
+}
+
+__device__ __forceinline__ void sao_to_words(const SaoP &p, uint32_t &w0, uint32_t &w1)
+{
+    w0 = (uint32_t)p.type | (uint32_t)p.cls << 8 | (uint32_t)p.band << 16;
+    w1 = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) w1 |= ((uint32_t)p.off[k] & 0xFF) << (8 * k);
+}
+
+// Offset i (0..3) of p, by selects (no indexed private array).
+__device__ __forceinline__ int sao_off(const SaoP &p, int i)
+{
+    return i == 0 ? p.off[0] : i == 1 ? p.off[1] : i == 2 ? p.off[2] : p.off[3];
+}
+
+// sao.cpp:136-146
+__device__ __forceinline__ int sao_bits(const SaoP &p)
+{
+    if (p.type == kSaoOff) return 2;
+    int bits = 2 + (p.type == kSaoEdge ? 2 : 5);
+#pragma unroll
+    for (int k = 0; k < 4; k++) bits += p.off[k] != 0 ? 4 : 1;
+    return bits;
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+struct SaoLds {
+    int16_t ry[18][18];          // luma reconstruction, rows and columns -1 .. 16 of the MB
+    int16_t ru[10][10];          // U, -1 .. 8
+    int32_t bsum[32], bcnt[32], boff[32];
+    int32_t esum[16], ecnt[16], eoff[16];    // [class * 4 + category]
+    unsigned long long bd[32], ed[4];        // SSD change per band / per edge class
+};
+
+// The rounded mean of src - rec over a class, clamped to +-7 (sao.cpp:199-214).
+__device__ __forceinline__ int sao_mean(int sum, int cnt)
+{
+    if (cnt <= 0) return 0;
+    const int avg = (sum + (sum >= 0 ? cnt / 2 : -cnt / 2)) / cnt;
+    return min(max(avg, -7), 7);
+}
+
+// The corrected value of sample (x, y) (window position r, c) under p, the
+// window holding the unfiltered reconstruction (sao.cpp:333-376 per sample).
+template <int N>
+__device__ __forceinline__ int sao_corrected(const int16_t *win, int r, int c, int x, int y, int pw,
+                                             int ph, const SaoP &p)
+{
+    constexpr int P = N + 2;
+    const int v = win[(r + 1) * P + c + 1];
+    if (p.type == kSaoBand) {
+        const int idx = (sao_clamp(v) >> 3) - p.band;
+        return idx >= 0 && idx < 4 ? sao_clamp(v + sao_off(p, idx)) : v;
+    }
+    if (p.type != kSaoEdge) return v;
+    int dx, dy;
+    sao_neighbour(p.cls, dx, dy);
+    if (x + dx < 0 || x + dx >= pw || y + dy < 0 || y + dy >= ph || x - dx < 0 || x - dx >= pw ||
+        y - dy < 0 || y - dy >= ph)
+        return v;
+    const int cat = sao_category(v, win[(r + 1 + dy) * P + c + 1 + dx], win[(r + 1 - dy) * P + c + 1 - dx]);
+    return cat < 4 ? sao_clamp(v + sao_off(p, cat)) : v;
+}
+
+// SSD of an N x N block of src (sv: this lane's samples) against the
+// reconstruction corrected by p.
+template <int N>
+__device__ __forceinline__ int64_t sao_block_ssd(const int16_t *win, const int (&sv)[N * N / 64],
+                                                 int x0, int y0, int pw, int ph, const SaoP &p)
+{
+    constexpr int K = N * N / 64;
+    int64_t ssd = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int idx = lane_id() * K + k, r = idx / N, c = idx % N;
+        const int d = sv[k] - sao_corrected<N>(win, r, c, x0 + c, y0 + r, pw, ph, p);
+        ssd += (int64_t)d * d;
+    }
+    return wave_sum64(ssd);
+}
+
+// sao.cpp:316-331 with 163-239 and 241-314: the best params of one N x N
+// block.  Class statistics, offsets and the SSD change of every band and
+// edge class are gathered once; the 29 band and 4 edge candidates are then
+// priced in the reference's order (ties keep the earlier).  best_ssd and
+// best_bits describe the winner.
+template <int N>
+__device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 64], int x0,
+                        int y0, int pw, int ph, float lambda, int64_t &best_ssd, int &best_bits)
+{
+    constexpr int K = N * N / 64, P = N + 2;
+    const int lane = lane_id();
+    if (lane < 32) {
+        sh.bsum[lane] = 0;
+        sh.bcnt[lane] = 0;
+        sh.bd[lane] = 0;
+    }
+    if (lane < 16) {
+        sh.esum[lane] = 0;
+        sh.ecnt[lane] = 0;
+    }
+    if (lane < 4) sh.ed[lane] = 0;
+    __syncthreads();
+    int v[K], band[K], cat[K][4];
+    int64_t base = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int idx = lane * K + k, r = idx / N, c = idx % N, x = x0 + c, y = y0 + r;
+        v[k] = win[(r + 1) * P + c + 1];
+        const int d = sv[k] - v[k];
+        base += (int64_t)d * d;
+        band[k] = sao_clamp(v[k]) >> 3;
+        atomicAdd(&sh.bsum[band[k]], d);
+        atomicAdd(&sh.bcnt[band[k]], 1);
+#pragma unroll
+        for (int cls = 0; cls < 4; cls++) {
+            int dx, dy;
+            sao_neighbour(cls, dx, dy);
+            const bool in = x + dx >= 0 && x + dx < pw && y + dy >= 0 && y + dy < ph && x - dx >= 0 &&
+                            x - dx < pw && y - dy >= 0 && y - dy < ph;
+            cat[k][cls] = in ? sao_category(v[k], win[(r + 1 + dy) * P + c + 1 + dx],
+                                            win[(r + 1 - dy) * P + c + 1 - dx])
+                             : 4;
+            if (cat[k][cls] < 4) {
+                atomicAdd(&sh.esum[cls * 4 + cat[k][cls]], d);
+                atomicAdd(&sh.ecnt[cls * 4 + cat[k][cls]], 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < 32) sh.boff[lane] = sao_mean(sh.bsum[lane], sh.bcnt[lane]);
+    else if (lane < 48) sh.eoff[lane - 32] = sao_mean(sh.esum[lane - 32], sh.ecnt[lane - 32]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int d = sv[k] - v[k];
+        const int64_t d0 = (int64_t)d * d;
+        const int cb = sv[k] - sao_clamp(v[k] + sh.boff[band[k]]);
+        atomicAdd(&sh.bd[band[k]], (unsigned long long)((int64_t)cb * cb - d0));
+#pragma unroll
+        for (int cls = 0; cls < 4; cls++) {
+            if (cat[k][cls] < 4) {
+                const int ce = sv[k] - sao_clamp(v[k] + sh.eoff[cls * 4 + cat[k][cls]]);
+                atomicAdd(&sh.ed[cls], (unsigned long long)((int64_t)ce * ce - d0));
+            }
+        }
+    }
+    base = wave_sum64(base);
+    __syncthreads();
+    SaoP best{};
+    float best_cost = (float)base + lambda * 2.0f;
+    best_ssd = base;
+    best_bits = 2;
+    for (int bp = 0; bp <= 28; bp++) {
+        SaoP c{};
+        c.type = kSaoBand;
+        c.band = bp;
+        bool nz = false;
+        int64_t ssd = base;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            c.off[j] = sh.boff[bp + j];
+            nz = nz || c.off[j] != 0;
+            ssd += (int64_t)sh.bd[bp + j];
+        }
+        if (!nz) continue;
+        const int bits = sao_bits(c);
+        const float cost = (float)ssd + lambda * (float)bits;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = c;
+            best_ssd = ssd;
+            best_bits = bits;
+        }
+    }
+    for (int cls = 0; cls < 4; cls++) {
+        SaoP c{};
+        c.type = kSaoEdge;
+        c.cls = cls;
+        bool nz = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            c.off[j] = sh.eoff[cls * 4 + j];
+            nz = nz || c.off[j] != 0;
+        }
+        if (!nz) continue;
+        const int64_t ssd = base + (int64_t)sh.ed[cls];
+        const int bits = sao_bits(c);
+        const float cost = (float)ssd + lambda * (float)bits;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = c;
+            best_ssd = ssd;
+            best_bits = bits;
+        }
+    }
+    return best;
+}
+
+struct SaoArgs {
+    const int16_t *rec_y, *rec_u;   // unfiltered reconstruction, packed (pitch w, w / 2)
+    const int16_t *src_y, *src_u;   // the frame's source planes
+    int32_t spy, spc;               // their pitches
+    int32_t w, h, wb, hb;
+    float lambda;
+    const uint32_t *order;          // MB of each ticket, anti-diagonal order
+    uint64_t *words;                // per MB: the published decision (zero until then)
+    evxa_sao_info *table;
+    int32_t *counters;              // [0] next ticket, [1] error flags
+};
+
+constexpr int kSaoErrWait = 1, kSaoErrDepth = 2;
+
+// The encoder's SAO decisions (sao.cpp:378-590, chroma chosen on U).  One
+// wave per MB ticket: the MB's own luma and chroma params first (no
+// dependency), then the merge with the left or upper MB's decided params,
+// which the reference takes in raster order: tickets run in anti-diagonal
+// order and each MB waits for the published words of its left and upper
+// neighbours.  A word (device-scope 64-bit access) is bit 63 = published |
+// merge_above << 51 | merge_left << 50 | chroma params << 25 | luma params.
+__global__ __launch_bounds__(64) void sao_decide_kernel(const SaoArgs a)
+{
+    __shared__ SaoLds sh;
+    const int lane = lane_id();
+    const int n = a.wb * a.hb, cw = a.w / 2, chh = a.h / 2;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&a.counters[0], 1);
+        t = __shfl(t, 0);
+        if (t >= n) return;
+        const int mb = a.order[t];
+        const int bx = mb % a.wb, by = mb / a.wb;
+        __syncthreads();                      // the previous ticket's LDS reads are done
+        for (int i = lane; i < 18 * 18; i += 64) {
+            const int X = bx * 16 + i % 18 - 1, Y = by * 16 + i / 18 - 1;
+            sh.ry[i / 18][i % 18] = X >= 0 && X < a.w && Y >= 0 && Y < a.h ? a.rec_y[(size_t)Y * a.w + X] : 0;
+        }
+        for (int i = lane; i < 10 * 10; i += 64) {
+            const int X = bx * 8 + i % 10 - 1, Y = by * 8 + i / 10 - 1;
+            sh.ru[i / 10][i % 10] = X >= 0 && X < cw && Y >= 0 && Y < chh ? a.rec_u[(size_t)Y * cw + X] : 0;
+        }
+        int sy[4], su[1];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int idx = lane * 4 + k;
+            sy[k] = a.src_y[(size_t)(by * 16 + idx / 16) * a.spy + bx * 16 + idx % 16];
+        }
+        su[0] = a.src_u[(size_t)(by * 8 + lane / 8) * a.spc + bx * 8 + lane % 8];
+        __syncthreads();
+        int64_t lssd, cssd;
+        int lbits, cbits;
+        const SaoP own_l = sao_rdo<16>(sh, &sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, a.lambda,
+                                       lssd, lbits);
+        const SaoP own_c = sao_rdo<8>(sh, &sh.ru[0][0], su, bx * 8, by * 8, cw, chh, a.lambda,
+                                      cssd, cbits);
+        // sao.cpp:538-550: the cost of the MB's own luma params.
+        float cur = own_l.type == kSaoOff
+                        ? (float)lssd + a.lambda * 2.0f
+                        : (((float)lssd + a.lambda * 1.0f) - a.lambda * 1.0f) + a.lambda * (float)lbits;
+        // The decided words of the left (lane 0) and upper (lane 1) MBs.
+        const int nidx = lane == 0 ? (bx > 0 ? mb - 1 : -1) : lane == 1 ? (by > 0 ? mb - a.wb : -1) : -1;
+        uint64_t word = 0;
+        int spins = 0, nap = 1;
+        bool timed_out = false;
+        for (;;) {
+            if (nidx >= 0) word = ld_relaxed64(a.words + nidx);
+            if (__ballot(nidx >= 0 && (word >> 63) == 0) == 0) break;
+            if (++spins > kSpinLimit) { timed_out = true; break; }
+            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+            nap = min(nap * 2, kMaxNap);
+        }
+        if (timed_out && lane == 0) atomicOr(&a.counters[1], kSaoErrWait);
+        const uint64_t wl = ((uint64_t)(uint32_t)lane_value((int)(word >> 32), 0) << 32) |
+                            (uint32_t)lane_value((int)(uint32_t)word, 0);
+        const uint64_t wa = ((uint64_t)(uint32_t)lane_value((int)(word >> 32), 1) << 32) |
+                            (uint32_t)lane_value((int)(uint32_t)word, 1);
+        constexpr uint32_t kMask25 = (1u << 25) - 1;
+        uint32_t fl = sao_pack(own_l), fc = sao_pack(own_c);
+        int merge = 0;
+        // sao.cpp:553-583: merge left, else merge up, when it costs less.
+        if (bx > 0) {
+            const SaoP L = sao_unpack((uint32_t)wl & kMask25);
+            if (L.type != kSaoOff) {
+                const float mc = (float)sao_block_ssd<16>(&sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, L) +
+                                 a.lambda * 1.0f;
+                if (mc < cur) {
+                    merge = 1;
+                    fl = (uint32_t)wl & kMask25;
+                    fc = (uint32_t)(wl >> 25) & kMask25;
+                    cur = mc;
+                }
+            }
+        }
+        if (merge == 0 && by > 0) {
+            const SaoP A = sao_unpack((uint32_t)wa & kMask25);
+            if (A.type != kSaoOff) {
+                const float mc = (float)sao_block_ssd<16>(&sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, A) +
+                                 a.lambda * 1.0f;
+                if (mc < cur) {
+                    merge = 2;
+                    fl = (uint32_t)wa & kMask25;
+                    fc = (uint32_t)(wa >> 25) & kMask25;
+                }
+            }
+        }
+        if (lane == 0)
+            st_relaxed64(a.words + mb, 1ull << 63 | (uint64_t)(merge == 2) << 51 |
+                                           (uint64_t)(merge == 1) << 50 | (uint64_t)fc << 25 | fl);
+        // The table entry (evxa_sao_info, five 32-bit words).
+        uint32_t lw0, lw1, cw0, cw1;
+        sao_to_words(sao_unpack(fl), lw0, lw1);
+        sao_to_words(sao_unpack(fc), cw0, cw1);
+        uint32_t *e = reinterpret_cast<uint32_t *>(a.table + mb);
+        if (lane < 5)
+            e[lane] = lane == 0 ? (uint32_t)(merge == 1) | (uint32_t)(merge == 2) << 8
+                    : lane == 1 ? lw0 : lane == 2 ? lw1 : lane == 3 ? cw0 : cw1;
+    }
+}
+
+struct SaoApplyArgs {
+    const int16_t *rec[3];     // unfiltered planes, packed
+    int16_t *dst[3];
+    int32_t dp[3];
+    int32_t w, h, wb;
+    const evxa_sao_info *table;
+    int32_t *wide;
+    int32_t *error;
+};
+
+// The params that apply to block (bx, by) (sao.cpp:111-119).
+__device__ __forceinline__ SaoP sao_params_at(const evxa_sao_info *table, int wb, int bx, int by,
+                                              bool luma)
+{
+    int m = by * wb + bx;
+    const uint32_t f = reinterpret_cast<const uint32_t *>(table + m)[0];
+    if ((f & 0xFF) && bx > 0) m -= 1;
+    else if (((f >> 8) & 0xFF) && by > 0) m -= wb;
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(table + m) + (luma ? 1 : 3);
+    return sao_from_words(q[0], q[1]);
+}
+
+// The final value of sample (x, y) of a plane after the reference's
+// in-place raster pass (sao.cpp:98-134): an edge-offset block classifies a
+// sample against neighbours as they stand when the block is processed --
+// final values in blocks before it in raster order, unfiltered values
+// elsewhere -- so such neighbours are evaluated the same way, recursively.
+// Such a chain steps into an earlier block each time and can continue only
+// from a block corner, which limits it to three steps; a deeper one would be
+// flagged.
+template <int D>
+__device__ int sao_final(const SaoApplyArgs &a, int pl, int x, int y)
+{
+    const int lg = pl ? 3 : 4, pw = pl ? a.w / 2 : a.w, ph = pl ? a.h / 2 : a.h;
+    const int16_t *rec = a.rec[pl];
+    const int bx = x >> lg, by = y >> lg;
+    const SaoP p = sao_params_at(a.table, a.wb, bx, by, pl == 0);
+    const int v = rec[(size_t)y * pw + x];
+    if (p.type == kSaoBand) {
+        const int idx = (sao_clamp(v) >> 3) - p.band;
+        return idx >= 0 && idx < 4 ? sao_clamp(v + sao_off(p, idx)) : v;
+    }
+    if (p.type != kSaoEdge) return v;
+    int dx, dy;
+    sao_neighbour(p.cls, dx, dy);
+    const int ax = x + dx, ay = y + dy, cx = x - dx, cy = y - dy;
+    if (ax < 0 || ax >= pw || ay < 0 || ay >= ph || cx < 0 || cx >= pw || cy < 0 || cy >= ph) return v;
+    const int me = by * a.wb + bx;
+    auto seen = [&](int qx, int qy) -> int {
+        if ((qy >> lg) * a.wb + (qx >> lg) < me) {
+            if constexpr (D < 3) return sao_final<D + 1>(a, pl, qx, qy);
+            atomicOr(a.error, kSaoErrDepth);
+        }
+        return rec[(size_t)qy * pw + qx];
+    };
+    const int cat = sao_category(v, seen(ax, ay), seen(cx, cy));
+    return cat < 4 ? sao_clamp(v + sao_off(p, cat)) : v;
+}
+
+// SAO of a frame per its table, from the unfiltered planes into the picture
+// slot (blockIdx.y = plane), setting the slot's range flag.
+__global__ __launch_bounds__(256) void sao_apply_kernel(const SaoApplyArgs a)
+{
+    const int pl = blockIdx.y;
+    const int pw = pl ? a.w / 2 : a.w, ph = pl ? a.h / 2 : a.h;
+    bool outside = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < pw * ph; i += gridDim.x * blockDim.x) {
+        const int x = i % pw, y = i / pw;
+        const int v = sao_final<0>(a, pl, x, y);
+        outside = outside || v < 0 || v > 2047;
+        a.dst[pl][(size_t)y * a.dp[pl] + x] = (int16_t)v;
+    }
+    if (__ballot(outside) != 0 && lane_id() == 0) atomicOr(a.wide, 1);
+}
+
 // params: n x {x, y, dir_x, dir_y, quarter}
 __global__ __launch_bounds__(64) void interp_blocks_kernel(DevPic pic, int32_t n,
                                                            const int32_t *params, int16_t *out)
@@ -2385,6 +2843,53 @@ hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, in
     const int blocks = min((w / 8 * h + 255) / 256, 1024);
     hipLaunchKernelGGL(copy3_kernel, dim3(blocks, 3), dim3(256), 0, st, y, u, v, py, pc, out, w, h,
                        wide);
+    return hipGetLastError();
+}
+
+hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const DevPic &src,
+                             int32_t w, int32_t h, float lambda, const uint32_t *order,
+                             uint64_t *words, evxa_sao_info *table, int32_t *counters,
+                             hipStream_t st)
+{
+    SaoArgs a{};
+    a.rec_y = rec_y;
+    a.rec_u = rec_u;
+    a.src_y = src.y;
+    a.src_u = src.u;
+    a.spy = src.py;
+    a.spc = src.pc;
+    a.w = w;
+    a.h = h;
+    a.wb = w / 16;
+    a.hb = h / 16;
+    a.lambda = lambda;
+    a.order = order;
+    a.words = words;
+    a.table = table;
+    a.counters = counters;
+    const int n = a.wb * a.hb;
+    hipLaunchKernelGGL(sao_decide_kernel, dim3(min(n, 4096)), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sao_apply(int16_t *const rec[3], int16_t *const dst[3], const int32_t dp[3],
+                            int32_t w, int32_t h, const evxa_sao_info *table, int32_t *wide,
+                            int32_t *error, hipStream_t st)
+{
+    SaoApplyArgs a{};
+    for (int p = 0; p < 3; p++) {
+        a.rec[p] = rec[p];
+        a.dst[p] = dst[p];
+        a.dp[p] = dp[p];
+    }
+    a.w = w;
+    a.h = h;
+    a.wb = w / 16;
+    a.table = table;
+    a.wide = wide;
+    a.error = error;
+    const int blocks = min((w * h + 255) / 256, 2048);
+    hipLaunchKernelGGL(sao_apply_kernel, dim3(blocks, 3), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -40,6 +40,15 @@ RECORD_DTYPE = np.dtype([
     ("flags", "<u4"), ("reserved2", "<u4")])
 assert RECORD_DTYPE.itemsize == 64
 
+# evxa_sao_info: one per MB (evxa.h; sao.h:62-74).
+SAO_DTYPE = np.dtype([("merge_left", "u1"), ("merge_above", "u1"), ("reserved", "u1", 2),
+                      ("luma_type", "u1"), ("luma_eo_class", "u1"), ("luma_band_position", "u1"),
+                      ("luma_reserved", "u1"), ("luma_offsets", "i1", 4),
+                      ("chroma_type", "u1"), ("chroma_eo_class", "u1"),
+                      ("chroma_band_position", "u1"), ("chroma_reserved", "u1"),
+                      ("chroma_offsets", "i1", 4)])
+assert SAO_DTYPE.itemsize == 20
+
 SEARCH_DTYPE = np.dtype([(n, "<i4") for n in (
     "best_x", "best_y", "sad", "mad", "ssd")] + [("cost_bits", "<u4")] + [(n, "<i4") for n in (
         "sp_enabled", "sp_amount", "sp_index", "block_type", "motion_x", "motion_y", "n_int",
@@ -75,7 +84,7 @@ EXPORTS = (
     "evxa_upload_picture", "evxa_stream_reset_temporal", "evxa_analyze_frame",
     "evxa_analyze_batch", "evxa_synchronize", "evxa_last_launch_ms", "evxa_launch_times",
     "evxa_read_results", "evxa_download_picture", "evxa_reconstruct", "evxa_deblock",
-    "evxa_reconstruct_deblocked",
+    "evxa_reconstruct_deblocked", "evxa_sao", "evxa_sao_filter",
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
@@ -125,6 +134,8 @@ def load_library(path: str = LIB_PATH):
         "evxa_reconstruct": (C.c_int, [vp, C.c_int32, I16P]),
         "evxa_deblock": (C.c_int, [vp, C.c_int32, vp]),
         "evxa_reconstruct_deblocked": (C.c_int, [vp, C.c_int32, I16P]),
+        "evxa_sao": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, vp]),
+        "evxa_sao_filter": (C.c_int, [vp, C.c_int32, vp]),
         "evxa_graph_create": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32,
                                         C.POINTER(C.c_int32)]),
         "evxa_graph_launch": (C.c_int, [vp, C.c_int32]),
@@ -343,6 +354,20 @@ class Stream:
             assert levels.shape == (self.n_mbs, MB_SAMPLES)
         _check(_lib.evxa_reconstruct_deblocked(self.h, dst_slot,
                                                None if levels is None else _p16(levels)))
+
+    def sao(self, slot, src_slot, quality, table=True):
+        """The encoder's SAO of the deblocked frame in `slot` against its
+        source in `src_slot`, applied in place; returns the decisions
+        (SAO_DTYPE, one per MB) unless table is False."""
+        t = np.zeros(self.n_mbs, SAO_DTYPE) if table else None
+        _check(_lib.evxa_sao(self.h, slot, src_slot, quality, None if t is None else t.ctypes.data))
+        return t
+
+    def sao_filter(self, slot, table):
+        """Apply an SAO table (SAO_DTYPE, one per MB) to `slot` in place."""
+        t = np.ascontiguousarray(table, SAO_DTYPE)
+        assert t.shape == (self.n_mbs,)
+        _check(_lib.evxa_sao_filter(self.h, slot, t.ctypes.data))
 
     def deblock(self, slot, records=None):
         """Deblock picture slot `slot` in place (the block types and q_index

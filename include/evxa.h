@@ -72,6 +72,21 @@ typedef struct evxa_block_record {
     uint32_t reserved2;
 } evxa_block_record;
 
+/* Sample adaptive offset parameters of one plane of an MB (evx_sao_params,
+ * sao.h:62-67, padded to 8 bytes).  type: 0 off, 1 edge offset, 2 band
+ * offset; eo_class: 0 horizontal, 1 vertical, 2 135 degrees, 3 45 degrees;
+ * band_position: first of the 4 bands of 8 levels that get offsets. */
+typedef struct evxa_sao_params {
+    uint8_t type, eo_class, band_position, reserved;
+    int8_t offsets[4];
+} evxa_sao_params;
+
+/* The SAO entry of one MB (evx_sao_info, sao.h:69-74), 20 bytes. */
+typedef struct evxa_sao_info {
+    uint8_t merge_left, merge_above, reserved[2];
+    evxa_sao_params luma, chroma;
+} evxa_sao_info;
+
 /* Result of one inter search (motion.cpp:567-679 plus its internal state). */
 typedef struct evxa_search_result {
     int32_t best_x, best_y;
@@ -190,6 +205,23 @@ int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
  * deblock_image_filter (encode.cpp:806, 918, 968); the decoder's
  * decode_slice then deblock_image_filter (decode.cpp:466-583). */
 int evxa_reconstruct_deblocked(evxa_stream *s, int32_t dst_slot, const int16_t *levels);
+
+/* The encoder's sample adaptive offset of the frame in picture slot `slot`
+ * (its deblocked reconstruction) against that frame's source picture in
+ * `src_slot`, at `quality` (lambda of common.h:264-266): per MB the best
+ * luma offsets and the best chroma offsets (chosen on U), band or edge,
+ * by cost SSD + lambda * bits, then a merge with the left or upper MB's
+ * parameters when that costs less, all in the reference's raster order;
+ * then the offsets are applied to `slot` in place.  The n_mbs decisions
+ * are copied to `table` when it is not NULL.  Replaces sao_encode_frame
+ * (sao.cpp:378-590) after deblocking (encode.cpp:974). */
+int evxa_sao(evxa_stream *s, int32_t slot, int32_t src_slot, int32_t quality,
+             evxa_sao_info *table);
+
+/* Apply a given SAO table (n_mbs entries on the host) to picture slot
+ * `slot` in place, as the decoder's sao_filter_image does after deblocking
+ * (sao.cpp:98-148; decode.cpp:589). */
+int evxa_sao_filter(evxa_stream *s, int32_t slot, const evxa_sao_info *table);
 
 /* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
 

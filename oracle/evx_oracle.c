@@ -971,3 +971,253 @@ void evo_deblock_frame(const evo_record *records, int32_t width, int32_t height,
     deblock_plane(records, wb, EVO_MB / 2, 0, u, width / 2, height / 2, EVO_CHROMA_QP_OFFSET);
     deblock_plane(records, wb, EVO_MB / 2, 0, v, width / 2, height / 2, EVO_CHROMA_QP_OFFSET);
 }
+
+/* ---------------------------------------------------------------- SAO -- */
+
+#define EVO_SAO_BAND_COUNT 32                /* sao.h:43-60 */
+#define EVO_SAO_MAX_OFFSET 7
+#define EVO_SAO_NUM_OFFSETS 4
+#define EVO_SAO_CAT_NONE 4
+
+/* sao.cpp:12-16 */
+static int16_t sao_clamp(int32_t v) { return v < 0 ? 0 : (v > 255 ? 255 : (int16_t)v); }
+
+/* sao.cpp:19-24: neighbours a and b of each edge class (dx_a, dy_a, dx_b, dy_b). */
+static const int8_t kSaoNeighbours[4][4] = {
+    {-1, 0, 1, 0}, {0, -1, 0, 1}, {-1, -1, 1, 1}, {1, -1, -1, 1}};
+
+/* sao.cpp:26-37: valley 0, concave 1, convex 2, peak 3, none 4. */
+static int32_t sao_category(int16_t pixel, int16_t a, int16_t b)
+{
+    const int32_t s = ((a > pixel) - (a < pixel)) + ((b > pixel) - (b < pixel));
+    return s == 2 ? 0 : s == 1 ? 1 : s == -1 ? 2 : s == -2 ? 3 : EVO_SAO_CAT_NONE;
+}
+
+/* The category of sample (x, y) under edge class `cls` in plane d (w x h,
+ * stride w), or none when a neighbour lies outside the plane. */
+static int32_t sao_edge_category(const int16_t *d, int32_t w, int32_t h, int32_t x, int32_t y,
+                                 int32_t cls)
+{
+    const int32_t ax = x + kSaoNeighbours[cls][0], ay = y + kSaoNeighbours[cls][1];
+    const int32_t bx = x + kSaoNeighbours[cls][2], by = y + kSaoNeighbours[cls][3];
+    if (ax < 0 || ax >= w || ay < 0 || ay >= h || bx < 0 || bx >= w || by < 0 || by >= h)
+        return EVO_SAO_CAT_NONE;
+    return sao_category(d[(size_t)y * w + x], d[(size_t)ay * w + ax], d[(size_t)by * w + bx]);
+}
+
+/* The corrected value of sample (x, y) of `rec` under params p (sao.cpp:39-53
+ * and 55-96 for one sample, classified on `rec`). */
+static int16_t sao_corrected(const int16_t *rec, int32_t w, int32_t h, int32_t x, int32_t y,
+                             const evo_sao_params *p)
+{
+    const int16_t pixel = rec[(size_t)y * w + x];
+    if (p->type == EVO_SAO_BAND) {
+        const int32_t idx = (sao_clamp(pixel) >> 3) - (int32_t)p->band_position;
+        if (idx >= 0 && idx < EVO_SAO_NUM_OFFSETS) return sao_clamp(pixel + p->offsets[idx]);
+    } else if (p->type == EVO_SAO_EDGE) {
+        const int32_t c = sao_edge_category(rec, w, h, x, y, p->eo_class);
+        if (c < EVO_SAO_NUM_OFFSETS) return sao_clamp(pixel + p->offsets[c]);
+    }
+    return pixel;
+}
+
+/* sao.cpp:98-134: one plane in raster order, in place (blocks of `mb`
+ * samples; edge categories of a block from the plane as it stands before the
+ * block is written). */
+static void sao_filter_plane(const evo_sao_info *table, int32_t wb, int32_t hb, int16_t *d,
+                             int32_t w, int32_t h, int32_t mb, int32_t luma)
+{
+    int8_t cat[EVO_MB * EVO_MB];
+    for (int32_t by = 0; by < hb; by++)
+        for (int32_t bx = 0; bx < wb; bx++) {
+            const int32_t idx = by * wb + bx;
+            const evo_sao_info *info = &table[idx];
+            const evo_sao_info *src = info;
+            if (info->merge_left && bx > 0) src = &table[idx - 1];
+            else if (info->merge_above && by > 0) src = &table[idx - wb];
+            const evo_sao_params *p = luma ? &src->luma : &src->chroma;
+            if (p->type == EVO_SAO_OFF) continue;
+            const int32_t x0 = bx * mb, y0 = by * mb;
+            const int32_t bw = x0 + mb > w ? w - x0 : mb, bh = y0 + mb > h ? h - y0 : mb;
+            if (p->type == EVO_SAO_BAND) {
+                for (int32_t j = y0; j < y0 + bh; j++)
+                    for (int32_t i = x0; i < x0 + bw; i++)
+                        d[(size_t)j * w + i] = sao_corrected(d, w, h, i, j, p);
+            } else if (p->type == EVO_SAO_EDGE) {
+                for (int32_t j = 0; j < bh; j++)
+                    for (int32_t i = 0; i < bw; i++)
+                        cat[j * bw + i] = (int8_t)sao_edge_category(d, w, h, x0 + i, y0 + j, p->eo_class);
+                for (int32_t j = 0; j < bh; j++)
+                    for (int32_t i = 0; i < bw; i++) {
+                        const int32_t c = cat[j * bw + i];
+                        int16_t *s = &d[(size_t)(y0 + j) * w + x0 + i];
+                        if (c < EVO_SAO_NUM_OFFSETS) *s = sao_clamp(*s + p->offsets[c]);
+                    }
+            }
+        }
+}
+
+/* sao.cpp:136-146 */
+static uint32_t sao_bits(const evo_sao_params *p)
+{
+    if (p->type == EVO_SAO_OFF) return 2;
+    uint32_t bits = 2 + (p->type == EVO_SAO_EDGE ? 2 : 5);
+    for (int k = 0; k < EVO_SAO_NUM_OFFSETS; k++) bits += p->offsets[k] != 0 ? 4 : 1;
+    return bits;
+}
+
+/* SSD of a block of `src` against `rec` corrected by p (sao.cpp:148-161 with
+ * p off; the SSD loops of 183-239, 241-314 and 333-376). */
+static int64_t sao_ssd(const int16_t *src, const int16_t *rec, int32_t w, int32_t h, int32_t x0,
+                       int32_t y0, int32_t bw, int32_t bh, const evo_sao_params *p)
+{
+    int64_t ssd = 0;
+    for (int32_t j = y0; j < y0 + bh; j++)
+        for (int32_t i = x0; i < x0 + bw; i++) {
+            const int32_t diff = (int32_t)src[(size_t)j * w + i] - sao_corrected(rec, w, h, i, j, p);
+            ssd += diff * diff;
+        }
+    return ssd;
+}
+
+/* The offsets of a class from its sums and counts (sao.cpp:199-214,
+ * 274-289): rounded mean of src - rec, clamped to +-7; 0 if no nonzero. */
+static int32_t sao_offsets(const int32_t *sum, const int32_t *cnt, int8_t *off)
+{
+    int32_t nonzero = 0;
+    for (int k = 0; k < EVO_SAO_NUM_OFFSETS; k++) {
+        int32_t avg = 0;
+        if (cnt[k] > 0) {
+            avg = (sum[k] + (sum[k] >= 0 ? cnt[k] / 2 : -cnt[k] / 2)) / cnt[k];
+            if (avg > EVO_SAO_MAX_OFFSET) avg = EVO_SAO_MAX_OFFSET;
+            if (avg < -EVO_SAO_MAX_OFFSET) avg = -EVO_SAO_MAX_OFFSET;
+        }
+        off[k] = (int8_t)avg;
+        nonzero |= avg != 0;
+    }
+    return nonzero;
+}
+
+static void sao_consider(const evo_sao_params *cand, int64_t ssd, float lambda,
+                         evo_sao_params *best, float *best_cost)
+{
+    const float cost = (float)ssd + lambda * (float)sao_bits(cand);
+    if (cost < *best_cost) {
+        *best_cost = cost;
+        *best = *cand;
+    }
+}
+
+/* sao.cpp:316-331 with 163-239 (band) and 241-314 (edge): the best params of
+ * one block of one plane. */
+static void sao_rdo_block(const int16_t *src, const int16_t *rec, int32_t w, int32_t h,
+                          int32_t x0, int32_t y0, int32_t bw, int32_t bh, float lambda,
+                          evo_sao_params *out)
+{
+    memset(out, 0, sizeof(*out));
+    float best = (float)sao_ssd(src, rec, w, h, x0, y0, bw, bh, out) + lambda * (float)sao_bits(out);
+    int32_t bsum[EVO_SAO_BAND_COUNT] = {0}, bcnt[EVO_SAO_BAND_COUNT] = {0};
+    for (int32_t j = y0; j < y0 + bh; j++)
+        for (int32_t i = x0; i < x0 + bw; i++) {
+            const int16_t pixel = rec[(size_t)j * w + i];
+            const int32_t band = sao_clamp(pixel) >> 3;
+            bsum[band] += (int32_t)src[(size_t)j * w + i] - pixel;
+            bcnt[band]++;
+        }
+    for (int32_t bp = 0; bp <= EVO_SAO_BAND_COUNT - EVO_SAO_NUM_OFFSETS; bp++) {
+        evo_sao_params c;
+        memset(&c, 0, sizeof(c));
+        c.type = EVO_SAO_BAND;
+        c.band_position = (uint8_t)bp;
+        if (!sao_offsets(bsum + bp, bcnt + bp, c.offsets)) continue;
+        sao_consider(&c, sao_ssd(src, rec, w, h, x0, y0, bw, bh, &c), lambda, out, &best);
+    }
+    for (int32_t cls = 0; cls < 4; cls++) {
+        int32_t csum[EVO_SAO_NUM_OFFSETS] = {0}, ccnt[EVO_SAO_NUM_OFFSETS] = {0};
+        for (int32_t j = y0; j < y0 + bh; j++)
+            for (int32_t i = x0; i < x0 + bw; i++) {
+                const int32_t cat = sao_edge_category(rec, w, h, i, j, cls);
+                if (cat < EVO_SAO_NUM_OFFSETS) {
+                    csum[cat] += (int32_t)src[(size_t)j * w + i] - rec[(size_t)j * w + i];
+                    ccnt[cat]++;
+                }
+            }
+        evo_sao_params c;
+        memset(&c, 0, sizeof(c));
+        c.type = EVO_SAO_EDGE;
+        c.eo_class = (uint8_t)cls;
+        if (!sao_offsets(csum, ccnt, c.offsets)) continue;
+        sao_consider(&c, sao_ssd(src, rec, w, h, x0, y0, bw, bh, &c), lambda, out, &best);
+    }
+}
+
+/* sao.cpp:333-376 */
+static float sao_merge_cost(const int16_t *src, const int16_t *rec, int32_t w, int32_t h,
+                            int32_t x0, int32_t y0, int32_t bw, int32_t bh,
+                            const evo_sao_params *p, float lambda)
+{
+    return (float)sao_ssd(src, rec, w, h, x0, y0, bw, bh, p) + lambda * 1.0f;
+}
+
+void evo_sao_filter_frame(const evo_sao_info *table, int32_t width, int32_t height, int16_t *y,
+                          int16_t *u, int16_t *v)
+{
+    const int32_t wb = width / EVO_MB, hb = height / EVO_MB;
+    sao_filter_plane(table, wb, hb, y, width, height, EVO_MB, 1);
+    sao_filter_plane(table, wb, hb, u, width / 2, height / 2, EVO_MB / 2, 0);
+    sao_filter_plane(table, wb, hb, v, width / 2, height / 2, EVO_MB / 2, 0);
+}
+
+/* sao.cpp:378-590 (EVX_ENABLE_SAO_CHROMA_MERGE 0): per MB in raster order
+ * the best luma params, the best chroma params from the U plane, then a
+ * merge with the left or upper MB's (final) params when its luma cost is
+ * lower; finally the table is applied to rec in place. */
+void evo_sao_encode_frame(const int16_t *src_y, const int16_t *src_u, const int16_t *src_v,
+                          int16_t *rec_y, int16_t *rec_u, int16_t *rec_v, int32_t width,
+                          int32_t height, float lambda, evo_sao_info *table)
+{
+    (void)src_v;
+    const int32_t wb = width / EVO_MB, hb = height / EVO_MB, cw = width / 2, chh = height / 2;
+    for (int32_t by = 0; by < hb; by++)
+        for (int32_t bx = 0; bx < wb; bx++) {
+            const int32_t idx = by * wb + bx;
+            evo_sao_info *info = &table[idx];
+            memset(info, 0, sizeof(*info));
+            const int32_t lx0 = bx * EVO_MB, ly0 = by * EVO_MB;
+            const int32_t lw = lx0 + EVO_MB > width ? width - lx0 : EVO_MB;
+            const int32_t lh = ly0 + EVO_MB > height ? height - ly0 : EVO_MB;
+            sao_rdo_block(src_y, rec_y, width, height, lx0, ly0, lw, lh, lambda, &info->luma);
+            const int32_t cx0 = bx * (EVO_MB / 2), cy0 = by * (EVO_MB / 2);
+            const int32_t cwb = cx0 + EVO_MB / 2 > cw ? cw - cx0 : EVO_MB / 2;
+            const int32_t chb = cy0 + EVO_MB / 2 > chh ? chh - cy0 : EVO_MB / 2;
+            sao_rdo_block(src_u, rec_u, cw, chh, cx0, cy0, cwb, chb, lambda, &info->chroma);
+            float cur;
+            if (info->luma.type == EVO_SAO_OFF) {
+                cur = (float)sao_ssd(src_y, rec_y, width, height, lx0, ly0, lw, lh, &info->luma) +
+                      lambda * 2.0f;
+            } else {
+                cur = sao_merge_cost(src_y, rec_y, width, height, lx0, ly0, lw, lh, &info->luma, lambda) -
+                      lambda * 1.0f + lambda * (float)sao_bits(&info->luma);
+            }
+            if (bx > 0 && table[idx - 1].luma.type != EVO_SAO_OFF) {
+                const float mc = sao_merge_cost(src_y, rec_y, width, height, lx0, ly0, lw, lh,
+                                                &table[idx - 1].luma, lambda);
+                if (mc < cur) {
+                    info->merge_left = 1;
+                    info->luma = table[idx - 1].luma;
+                    info->chroma = table[idx - 1].chroma;
+                    cur = mc;
+                }
+            }
+            if (!info->merge_left && by > 0 && table[idx - wb].luma.type != EVO_SAO_OFF) {
+                const float mc = sao_merge_cost(src_y, rec_y, width, height, lx0, ly0, lw, lh,
+                                                &table[idx - wb].luma, lambda);
+                if (mc < cur) {
+                    info->merge_above = 1;
+                    info->luma = table[idx - wb].luma;
+                    info->chroma = table[idx - wb].chroma;
+                }
+            }
+        }
+    evo_sao_filter_frame(table, width, height, rec_y, rec_u, rec_v);
+}

@@ -209,6 +209,39 @@ void evo_deblock_line(int16_t v[8], int32_t qp, int32_t strength, int32_t is_lum
 void evo_deblock_frame(const evo_record *records, int32_t width, int32_t height, int16_t *y,
                        int16_t *u, int16_t *v);
 
+/* ---- Sample adaptive offset (decode and encode side) ---- */
+
+#define EVO_SAO_OFF 0                /* sao.h:46-48 */
+#define EVO_SAO_EDGE 1
+#define EVO_SAO_BAND 2
+
+/* evx_sao_params (sao.h:62-67), padded to 8 bytes. */
+typedef struct evo_sao_params {
+    uint8_t type;                  /* off, edge or band */
+    uint8_t eo_class;              /* horizontal, vertical, 135, 45 degrees */
+    uint8_t band_position;         /* first of 4 bands of 8 levels */
+    uint8_t reserved;
+    int8_t offsets[4];
+} evo_sao_params;
+
+/* evx_sao_info (sao.h:69-74), 20 bytes: one per MB. */
+typedef struct evo_sao_info {
+    uint8_t merge_left, merge_above, reserved[2];
+    evo_sao_params luma, chroma;
+} evo_sao_info;
+
+/* sao.cpp:98-134 and sao_filter_image (decode.cpp:589): SAO of a frame per
+ * `table`, in place, MB by MB in raster order.  Planes are packed. */
+void evo_sao_filter_frame(const evo_sao_info *table, int32_t width, int32_t height, int16_t *y,
+                          int16_t *u, int16_t *v);
+
+/* sao_encode_frame (sao.cpp:378-590, encode.cpp:974): the SAO decisions of a
+ * frame from the source and the (deblocked) reconstruction, written to
+ * `table`, then applied to rec in place.  lambda: evo_lambda(quality). */
+void evo_sao_encode_frame(const int16_t *src_y, const int16_t *src_u, const int16_t *src_v,
+                          int16_t *rec_y, int16_t *rec_u, int16_t *rec_v, int32_t width,
+                          int32_t height, float lambda, evo_sao_info *table);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,15 @@ RECORD_DTYPE = np.dtype([
     ("flags", "<u4"), ("reserved2", "<u4")])
 assert RECORD_DTYPE.itemsize == 64
 
+# evo_sao_info: one per MB (sao.h:62-74).
+SAO_DTYPE = np.dtype([("merge_left", "u1"), ("merge_above", "u1"), ("reserved", "u1", 2),
+                      ("luma_type", "u1"), ("luma_eo_class", "u1"), ("luma_band_position", "u1"),
+                      ("luma_reserved", "u1"), ("luma_offsets", "i1", 4),
+                      ("chroma_type", "u1"), ("chroma_eo_class", "u1"),
+                      ("chroma_band_position", "u1"), ("chroma_reserved", "u1"),
+                      ("chroma_offsets", "i1", 4)])
+assert SAO_DTYPE.itemsize == 20
+
 # Fields the reference itself exposes (evx_block_desc + selection SAD).
 REF_FIELDS = ("block_type", "prediction_mode", "prediction_target", "prediction_target_b",
               "motion_x", "motion_y", "motion_x_b", "motion_y_b", "sp_pred", "sp_amount",
@@ -117,6 +126,10 @@ class _Recon:
         self._dbl_line = self.f("deblock_line", None, I16P, C.c_int32, C.c_int32, C.c_int32)
         self._dbl_frame = self.f("deblock_frame", None, C.c_void_p, C.c_int32, C.c_int32, I16P,
                                  I16P, I16P)
+        self._sao_filter = self.f("sao_filter_frame", None, C.c_void_p, C.c_int32, C.c_int32,
+                                  I16P, I16P, I16P)
+        self._sao_encode = self.f("sao_encode_frame", None, I16P, I16P, I16P, I16P, I16P, I16P,
+                                  C.c_int32, C.c_int32, C.c_float, C.c_void_p)
 
     def dequantize(self, qp, levels):
         lv = np.ascontiguousarray(levels, np.int16)
@@ -151,6 +164,27 @@ class _Recon:
         a = np.array(v, np.int16)
         self._dbl_line(_ptr16(a), qp, strength, int(bool(is_luma)))
         return a
+
+    def sao_filter_frame(self, table, y, u, v):
+        """Copies of the planes after SAO per `table` (SAO_DTYPE, one per MB)."""
+        h, w = y.shape
+        t = np.ascontiguousarray(table, SAO_DTYPE)
+        if t.size != (w // 16) * (h // 16):
+            raise ValueError("one SAO entry per MB")
+        oy, ou, ov = (np.array(a, np.int16, order="C") for a in (y, u, v))
+        self._sao_filter(t.ctypes.data, w, h, _ptr16(oy), _ptr16(ou), _ptr16(ov))
+        return oy, ou, ov
+
+    def sao_encode_frame(self, src, rec, lam):
+        """(table, y, u, v): the SAO decisions of the frame whose source
+        planes are `src` and deblocked planes `rec`, and rec after SAO."""
+        h, w = rec[0].shape
+        s3 = [np.ascontiguousarray(a, np.int16) for a in src]
+        oy, ou, ov = (np.array(a, np.int16, order="C") for a in rec)
+        t = np.zeros((w // 16) * (h // 16), SAO_DTYPE)
+        self._sao_encode(*(_ptr16(a) for a in s3), _ptr16(oy), _ptr16(ou), _ptr16(ov), w, h,
+                         C.c_float(lam), t.ctypes.data)
+        return t, oy, ou, ov
 
     def deblock_frame(self, records, y, u, v):
         """Deblocked copies of the planes (y: h x w, u/v: h/2 x w/2); the

@@ -21,6 +21,7 @@
 #include "macroblock.h"
 #include "motion.h"
 #include "quantize.h"
+#include "sao.h"
 #include "transform.h"
 
 #include <algorithm>
@@ -614,6 +615,83 @@ void evr_deblock_frame(const evo_record *records, int32_t width, int32_t height,
     memcpy(y, sy, (size_t)width * height * sizeof(int16_t));
     memcpy(u, su, (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
     memcpy(v, sv, (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
+}
+
+}  // extern "C"
+
+namespace {
+
+void to_ref(const evo_sao_params &a, evx_sao_params *b)
+{
+    b->type = a.type;
+    b->eo_class = a.eo_class;
+    b->band_position = a.band_position;
+    for (int k = 0; k < 4; k++) b->offsets[k] = a.offsets[k];
+}
+
+void from_ref(const evx_sao_params &a, evo_sao_params *b)
+{
+    memset(b, 0, sizeof(*b));
+    b->type = a.type;
+    b->eo_class = a.eo_class;
+    b->band_position = a.band_position;
+    for (int k = 0; k < 4; k++) b->offsets[k] = a.offsets[k];
+}
+
+void copy_back(const image_set &set, int32_t width, int32_t height, int16_t *y, int16_t *u,
+               int16_t *v)
+{
+    memcpy(y, set.query_y_image()->query_data(), (size_t)width * height * sizeof(int16_t));
+    memcpy(u, set.query_u_image()->query_data(), (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
+    memcpy(v, set.query_v_image()->query_data(), (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
+}
+
+}  // namespace
+
+extern "C" {
+
+// sao_filter_image (sao.cpp:136-146) on a packed frame, in place.
+void evr_sao_filter_frame(const evo_sao_info *table, int32_t width, int32_t height, int16_t *y,
+                          int16_t *u, int16_t *v)
+{
+    evo_planes p = {y, u, v, width, height, width, width / 2};
+    image_set set;
+    fill_set(&set, p);
+    const size_t n = (size_t)(width / 16) * (height / 16);
+    std::vector<evx_sao_info> t(n);
+    for (size_t i = 0; i < n; i++) {
+        clear_sao_info(&t[i]);
+        t[i].merge_left = table[i].merge_left != 0;
+        t[i].merge_above = table[i].merge_above != 0;
+        to_ref(table[i].luma, &t[i].luma);
+        to_ref(table[i].chroma, &t[i].chroma);
+    }
+    sao_filter_image(t.data(), width / 16, height / 16, &set);
+    copy_back(set, width, height, y, u, v);
+}
+
+// sao_encode_frame (sao.cpp:378-590) on packed frames: decisions into
+// `table`, the reconstruction filtered in place.
+void evr_sao_encode_frame(const int16_t *src_y, const int16_t *src_u, const int16_t *src_v,
+                          int16_t *rec_y, int16_t *rec_u, int16_t *rec_v, int32_t width,
+                          int32_t height, float lambda, evo_sao_info *table)
+{
+    evo_planes ps = {src_y, src_u, src_v, width, height, width, width / 2};
+    evo_planes pr = {rec_y, rec_u, rec_v, width, height, width, width / 2};
+    image_set src, rec;
+    fill_set(&src, ps);
+    fill_set(&rec, pr);
+    const size_t n = (size_t)(width / 16) * (height / 16);
+    std::vector<evx_sao_info> t(n);
+    sao_encode_frame(src, &rec, t.data(), width / 16, height / 16, lambda);
+    for (size_t i = 0; i < n; i++) {
+        memset(&table[i], 0, sizeof(table[i]));
+        table[i].merge_left = t[i].merge_left;
+        table[i].merge_above = t[i].merge_above;
+        from_ref(t[i].luma, &table[i].luma);
+        from_ref(t[i].chroma, &table[i].chroma);
+    }
+    copy_back(rec, width, height, rec_y, rec_u, rec_v);
 }
 
 }  // extern "C"

@@ -1,0 +1,142 @@
+"""Sample adaptive offset on the GPU (evxa_sao, evxa_sao_filter) against the
+oracle's raster-order restatement of sao.cpp, bit for bit: random tables
+through the filter (every type, class, band and merge flag, samples outside
+[0, 255]), the encoder's decisions and filtered planes over content that
+calls for edge offsets, band offsets and no offset, the chain after a
+reconstruction and deblocking, and an SAO picture as the next reference."""
+import numpy as np
+import pytest
+
+from chicago_amd import evxa, synth
+from oracle.oracle import SAO_DTYPE, Picture
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = [f for f in SAO_DTYPE.names if "reserved" not in f]
+
+
+def _check_planes(got, want, what):
+    for name, g, w in zip("yuv", got, want):
+        bad = np.argwhere(g != w)
+        assert bad.size == 0, f"{what}: plane {name} differs at {bad[:4].tolist()}"
+
+
+def _check_table(got, want, what):
+    for f in FIELDS:
+        bad = np.argwhere(np.asarray(got[f] != want[f]).reshape(len(got), -1).any(1))
+        assert bad.size == 0, f"{what}: field {f} differs at MBs {bad[:6].ravel().tolist()}"
+
+
+def _random_table(rng, n):
+    t = np.zeros(n, SAO_DTYPE)
+    for pre in ("luma", "chroma"):
+        t[pre + "_type"] = rng.choice([0, 1, 1, 2], n)
+        t[pre + "_eo_class"] = rng.integers(0, 4, n)
+        t[pre + "_band_position"] = rng.integers(0, 29, n)
+        t[pre + "_offsets"] = rng.integers(-7, 8, (n, 4))
+    t["merge_left"] = rng.random(n) < 0.2
+    t["merge_above"] = rng.random(n) < 0.2
+    return t
+
+
+def _noisy(rng, pic, amp):
+    return tuple((p.astype(np.int32) + rng.integers(-amp, amp + 1, p.shape)).astype(np.int16)
+                 for p in pic)
+
+
+@pytest.mark.parametrize("w,h,seed", [(64, 48, 1), (176, 144, 2), (400, 288, 3), (1920, 1088, 4)])
+def test_sao_filter_random_tables(ctx, oracle, w, h, seed):
+    rng = np.random.default_rng(seed)
+    pic = _noisy(rng, synth.blocky_picture(seed, w, h, amp=20), 40)
+    table = _random_table(rng, (w // 16) * (h // 16))
+    s = ctx.open_stream(w, h, 1)
+    s.upload(0, *pic)
+    s.sao_filter(0, table)
+    _check_planes(s.download(0), oracle.sao_filter_frame(table, *pic), f"{w}x{h}")
+    s.close()
+
+
+def _content(rng, w, h):
+    """Source and reconstruction: noise in most MBs (edge offsets), a
+    constant bias in some (band offsets), exact copies in others (off)."""
+    src = synth.blocky_picture(int(rng.integers(1 << 30)), w, h, amp=3)
+    rec = []
+    for k, p in enumerate(src):
+        mb = 16 if k == 0 else 8
+        r = p.astype(np.int32) + rng.integers(-4, 5, p.shape)
+        kind = rng.integers(0, 4, (p.shape[0] // mb, p.shape[1] // mb))
+        kind = np.repeat(np.repeat(kind, mb, 0), mb, 1)
+        r = np.where(kind == 1, p.astype(np.int32) + 5, r)
+        r = np.where(kind == 2, p, r)
+        rec.append(r.astype(np.int16))
+    return src, tuple(rec)
+
+
+@pytest.mark.parametrize("w,h,q", [(64, 48, 4), (176, 144, 12), (320, 192, 24), (1920, 1088, 16)])
+def test_sao_encode(ctx, oracle, w, h, q):
+    rng = np.random.default_rng(q)
+    src, rec = _content(rng, w, h)
+    s = ctx.open_stream(w, h, 2)
+    s.upload(0, *src)
+    s.upload(1, *rec)
+    table = s.sao(1, 0, q)
+    want_t, *want = oracle.sao_encode_frame(src, rec, oracle.lam(q))
+    _check_table(table, want_t, f"{w}x{h} q{q}")
+    _check_planes(s.download(1), want, f"{w}x{h} q{q}")
+    types = np.bincount(want_t["luma_type"], minlength=3)
+    assert types.min() > 0, f"content did not exercise every type: {types}"
+    if w >= 176:
+        assert want_t["merge_left"].any() and want_t["merge_above"].any()
+    s.close()
+
+
+def test_sao_after_reconstruct_deblocked(ctx, oracle):
+    """Analysis, reconstruction with deblocking, then SAO against the
+    frame's source: the encoder's in-loop chain (encode.cpp:806-974)."""
+    frames = synth.sequence_c1(91, 3, 192, 112, local_range=12)
+    h, w = frames[0][0].shape
+    s = ctx.open_stream(w, h, 4)
+    for k, f in enumerate(frames):
+        s.upload(k, *f)
+    pics = [Picture(*f) for f in frames]
+    s.analyze_p(14, 2, [1, 0])
+    rec, _, lv = s.results()
+    s.reconstruct_deblocked(3)
+    table = s.sao(3, 2, 14)
+    filt = oracle.deblock_frame(rec, *oracle.reconstruct_frame(1, 14, pics[2], [pics[1], pics[0]], rec, lv))
+    want_t, *want = oracle.sao_encode_frame(frames[2], filt, oracle.lam(14))
+    _check_table(table, want_t, "chain")
+    _check_planes(s.download(3), want, "chain")
+    s.close()
+
+
+def test_sao_picture_as_reference(ctx, oracle):
+    """After SAO the slot's apron and range flag are refreshed: a P frame
+    searched against it matches the oracle's search against the planes."""
+    rng = np.random.default_rng(5)
+    w, h = 160, 96
+    src, rec = _content(rng, w, h)
+    cur = synth.blocky_picture(77, w, h, amp=3)
+    s = ctx.open_stream(w, h, 3)
+    s.upload(0, *src)
+    s.upload(1, *rec)
+    s.upload(2, *cur)
+    s.sao(1, 0, 12)
+    _, *want = oracle.sao_encode_frame(src, rec, oracle.lam(12))
+    _check_planes(s.download(1), want, "reference")
+    s.analyze_p(10, 2, [1], temporal_mvp=False)
+    got, gco, glv = s.results()
+    orec, oco, olv = oracle.analyze_frame(1, 10, Picture(*cur), [Picture(*want)])
+    for f in [n for n in evxa.RECORD_DTYPE.names if not n.startswith("reserved")]:
+        assert np.array_equal(got[f], orec[f]), f
+    assert np.array_equal(gco, oco) and np.array_equal(glv, olv)
+    s.close()
+
+
+def test_sao_refusals(ctx):
+    s = ctx.open_stream(64, 48, 2)
+    with pytest.raises(evxa.EvxaError):
+        s.sao(0, 0, 8)                         # source and reconstruction in one slot
+    with pytest.raises(evxa.EvxaError):
+        s.sao(2, 0, 8)                         # no such slot
+    s.close()
