@@ -642,6 +642,34 @@ def run_recon(args):
                      "frac": round(dbytes / d_s / 1e9 / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": dbytes},
     }
+    # SAO of the deblocked reconstruction against the frame's source (slot 5).
+    s.reconstruct_deblocked(6)
+    table = s.sao(6, 5, args.quality)
+    got_sao = s.download(6)
+    ctx.synchronize()
+    ctx.launch_times()
+    sreps = 20
+    t0 = time.perf_counter()
+    for _ in range(sreps):
+        s.reconstruct_deblocked(6)
+        s.sao(6, 5, args.quality, table=False)
+    ctx.synchronize()
+    chain_s = (time.perf_counter() - t0) / sreps
+    ms = np.array(ctx.launch_times()).reshape(sreps, 4)     # recon, deblock, decide, apply
+    dec_s, app_s = float(np.mean(ms[:, 2])) / 1e3, float(np.mean(ms[:, 3])) / 1e3
+    res["sao"] = {
+        "workload": "SAO: the deblocked 1920x1080 reconstruction against its source frame",
+        "kernels": "sao_decide_kernel (per-MB search, raster-order merges), sao_apply_kernel",
+        "decide_us_avg": round(dec_s * 1e6, 2),
+        "apply_us_avg": round(app_s * 1e6, 2),
+        "reconstruct_deblock_sao_call_us": round(chain_s * 1e6, 2),
+        "luma_types_off_edge_band": np.bincount(table["luma_type"], minlength=3).tolist(),
+        "merges_left_up": [int(table["merge_left"].sum()), int(table["merge_above"].sum())],
+        "roofline": {"bound": "hbm", "kernel": "sao_apply_kernel",
+                     "achieved": round(samples * 4 / app_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(samples * 4 / app_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_launch": samples * 4},
+    }
     if not args.no_cpu:
         from oracle.oracle import Oracle, Picture
         o = Oracle()
@@ -653,6 +681,11 @@ def run_recon(args):
         res["deblock"]["bit_exact_vs_cpu"] = all(
             np.array_equal(a, b) and np.array_equal(c, b)
             for a, b, c in zip(got_db, want_db, got_fused))
+        want_t, *want_sao = o.sao_encode_frame(frames[5], want_db, o.lam(args.quality))
+        res["sao"]["bit_exact_vs_cpu"] = bool(
+            all(np.array_equal(table[f], want_t[f]) for f in want_t.dtype.names
+                if "reserved" not in f) and
+            all(np.array_equal(a, b) for a, b in zip(got_sao, want_sao)))
     s.close()
     ctx.close()
     return res

@@ -226,7 +226,6 @@ struct evxa_stream {
     DevBuf deblock_src;    // unfiltered copy of the picture being deblocked (packed planes)
     DevBuf deblock_records;  // block descriptors handed in for deblocking
     DevBuf recon_flag;     // range flag of a reconstruction headed for the filter (unused)
-    DevBuf sao_order;      // MB of each SAO ticket, anti-diagonal order
     DevBuf sao_words;      // per MB: published SAO decision; then the SAO counters
     DevBuf sao_table;      // evxa_sao_info per MB
     uint64_t bytes = 0;
@@ -1020,24 +1019,14 @@ namespace {
 
 constexpr int kSaoCounters = 64;
 
-// The SAO buffers of a stream (allocated on first use; the ticket order is
-// written once).
+// The SAO buffers of a stream, allocated on first use: sao_words holds the
+// decided words (one per MB), then kSaoCounters counters, then 5 words per
+// MB of per-stage scratch.
 void sao_buffers(evxa_stream *s)
 {
     const size_t n = (size_t)s->wb * s->hb;
-    if (!s->sao_order.p) {
-        lazy_alloc(s, s->sao_order, n * sizeof(uint32_t));
-        std::vector<uint32_t> order;
-        order.reserve(n);
-        for (int32_t d = 0; d < s->wb + s->hb - 1; d++)
-            for (int32_t by = 0; by < s->hb; by++) {
-                const int32_t bx = d - by;
-                if (bx >= 0 && bx < s->wb) order.push_back((uint32_t)(by * s->wb + bx));
-            }
-        check(hipMemcpy(s->sao_order.p, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice),
-              "upload SAO order");
-    }
-    lazy_alloc(s, s->sao_words, n * sizeof(uint64_t) + kSaoCounters * sizeof(int32_t));
+    lazy_alloc(s, s->sao_words, n * sizeof(uint64_t) + kSaoCounters * sizeof(int32_t) +
+                                    5 * n * sizeof(uint32_t));
     lazy_alloc(s, s->sao_table, n * sizeof(evxa_sao_info));
 }
 
@@ -1099,9 +1088,10 @@ int evxa_sao(evxa_stream *s, int32_t slot, int32_t src_slot, int32_t quality, ev
               "reset SAO words");
         timed(ctx, [&] {
             check(launch_sao_decide(rec[0], rec[1], s->pic(src_slot), s->w, s->h, lambda_of(quality),
-                                    s->sao_order.as<uint32_t>(), s->sao_words.as<uint64_t>(),
-                                    s->sao_table.as<evxa_sao_info>(), sao_counters(s), ctx->stream),
-                  "SAO decision kernel");
+                                    reinterpret_cast<uint32_t *>(sao_counters(s) + kSaoCounters),
+                                    s->sao_words.as<uint64_t>(), s->sao_table.as<evxa_sao_info>(),
+                                    sao_counters(s), ctx->stream),
+                  "SAO decision kernels");
         });
         sao_apply_slot(s, slot, rec);
         if (table)

@@ -118,12 +118,12 @@ hipError_t launch_deblock(int16_t *const src[3], const int32_t sp[3], int16_t *c
 hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py, int32_t pc,
                         int16_t *out, int32_t w, int32_t h, int32_t *wide, hipStream_t st);
 // SAO decisions of a frame (sao.cpp:378-590): the unfiltered reconstruction
-// (packed luma and U planes), the frame's source picture, lambda; `order`
-// lists the MBs in anti-diagonal order, `words` (one per MB) and counters
-// [0..1] start at zero; decisions go to `table`; counters[1] gets error
-// flags (1: a wait timed out, 2: an SAO chain deeper than three blocks).
+// (packed luma and U planes), the frame's source picture, lambda; `scratch`
+// holds 5 words per MB, `words` (one per MB) starts at zero; decisions go to
+// `table`; counters[1] (zero at the start) gets error flags (1: a wait
+// timed out, 2: an SAO chain deeper than three blocks).
 hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const DevPic &src,
-                             int32_t w, int32_t h, float lambda, const uint32_t *order,
+                             int32_t w, int32_t h, float lambda, uint32_t *scratch,
                              uint64_t *words, evxa_sao_info *table, int32_t *counters,
                              hipStream_t st);
 // SAO of a frame per `table` from the unfiltered packed planes `rec` into

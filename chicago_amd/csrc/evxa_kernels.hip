@@ -2505,103 +2505,199 @@ struct SaoArgs {
     int32_t spy, spc;               // their pitches
     int32_t w, h, wb, hb;
     float lambda;
-    const uint32_t *order;          // MB of each ticket, anti-diagonal order
-    uint64_t *words;                // per MB: the published decision (zero until then)
+    uint32_t *own_l, *own_c;        // per MB: its own best params (packed)
+    float *cur;                     // per MB: the cost of its own luma params
+    float *mcl, *mca;               // per MB: merge costs under the left / upper MB's own luma params
+    uint64_t *words;                // per MB: the decided params (zero until published)
     evxa_sao_info *table;
-    int32_t *counters;              // [0] next ticket, [1] error flags
+    int32_t *counters;              // [1] error flags
 };
 
 constexpr int kSaoErrWait = 1, kSaoErrDepth = 2;
+constexpr uint32_t kSaoMask25 = (1u << 25) - 1;
 
-// The encoder's SAO decisions (sao.cpp:378-590, chroma chosen on U).  One
-// wave per MB ticket: the MB's own luma and chroma params first (no
-// dependency), then the merge with the left or upper MB's decided params,
-// which the reference takes in raster order: tickets run in anti-diagonal
-// order and each MB waits for the published words of its left and upper
-// neighbours.  A word (device-scope 64-bit access) is bit 63 = published |
-// merge_above << 51 | merge_left << 50 | chroma params << 25 | luma params.
-__global__ __launch_bounds__(64) void sao_decide_kernel(const SaoArgs a)
+// An MB's luma reconstruction window (rows and columns -1 .. 16, zero
+// outside the plane) into LDS and this lane's 4 source samples.
+__device__ __forceinline__ void sao_load_luma(SaoLds &sh, const SaoArgs &a, int bx, int by,
+                                              int (&sy)[4])
+{
+    const int lane = lane_id();
+    for (int i = lane; i < 18 * 18; i += 64) {
+        const int X = bx * 16 + i % 18 - 1, Y = by * 16 + i / 18 - 1;
+        sh.ry[i / 18][i % 18] = X >= 0 && X < a.w && Y >= 0 && Y < a.h ? a.rec_y[(size_t)Y * a.w + X] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int idx = lane * 4 + k;
+        sy[k] = a.src_y[(size_t)(by * 16 + idx / 16) * a.spy + bx * 16 + idx % 16];
+    }
+}
+
+// The merge cost of an MB under luma params p (sao.cpp:333-376): its SSD
+// plus lambda for the merge flag.
+__device__ __forceinline__ float sao_merge_cost(const SaoLds &sh, const SaoArgs &a, int bx, int by,
+                                                const int (&sy)[4], const SaoP &p)
+{
+    return (float)sao_block_ssd<16>(&sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, p) + a.lambda * 1.0f;
+}
+
+// Stage 1 of the SAO decisions (sao.cpp:409-550, chroma chosen on U): each
+// MB's own best luma and chroma params and the cost of its own luma params,
+// one wave per MB, no dependency.
+__global__ __launch_bounds__(64) void sao_own_kernel(const SaoArgs a)
 {
     __shared__ SaoLds sh;
     const int lane = lane_id();
-    const int n = a.wb * a.hb, cw = a.w / 2, chh = a.h / 2;
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&a.counters[0], 1);
-        t = __shfl(t, 0);
-        if (t >= n) return;
-        const int mb = a.order[t];
-        const int bx = mb % a.wb, by = mb / a.wb;
-        __syncthreads();                      // the previous ticket's LDS reads are done
-        for (int i = lane; i < 18 * 18; i += 64) {
-            const int X = bx * 16 + i % 18 - 1, Y = by * 16 + i / 18 - 1;
-            sh.ry[i / 18][i % 18] = X >= 0 && X < a.w && Y >= 0 && Y < a.h ? a.rec_y[(size_t)Y * a.w + X] : 0;
-        }
-        for (int i = lane; i < 10 * 10; i += 64) {
-            const int X = bx * 8 + i % 10 - 1, Y = by * 8 + i / 10 - 1;
-            sh.ru[i / 10][i % 10] = X >= 0 && X < cw && Y >= 0 && Y < chh ? a.rec_u[(size_t)Y * cw + X] : 0;
-        }
-        int sy[4], su[1];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int idx = lane * 4 + k;
-            sy[k] = a.src_y[(size_t)(by * 16 + idx / 16) * a.spy + bx * 16 + idx % 16];
-        }
-        su[0] = a.src_u[(size_t)(by * 8 + lane / 8) * a.spc + bx * 8 + lane % 8];
-        __syncthreads();
-        int64_t lssd, cssd;
-        int lbits, cbits;
-        const SaoP own_l = sao_rdo<16>(sh, &sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, a.lambda,
-                                       lssd, lbits);
-        const SaoP own_c = sao_rdo<8>(sh, &sh.ru[0][0], su, bx * 8, by * 8, cw, chh, a.lambda,
-                                      cssd, cbits);
-        // sao.cpp:538-550: the cost of the MB's own luma params.
-        float cur = own_l.type == kSaoOff
-                        ? (float)lssd + a.lambda * 2.0f
-                        : (((float)lssd + a.lambda * 1.0f) - a.lambda * 1.0f) + a.lambda * (float)lbits;
-        // The decided words of the left (lane 0) and upper (lane 1) MBs.
-        const int nidx = lane == 0 ? (bx > 0 ? mb - 1 : -1) : lane == 1 ? (by > 0 ? mb - a.wb : -1) : -1;
-        uint64_t word = 0;
-        int spins = 0, nap = 1;
-        bool timed_out = false;
-        for (;;) {
-            if (nidx >= 0) word = ld_relaxed64(a.words + nidx);
-            if (__ballot(nidx >= 0 && (word >> 63) == 0) == 0) break;
-            if (++spins > kSpinLimit) { timed_out = true; break; }
-            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
-            nap = min(nap * 2, kMaxNap);
-        }
-        if (timed_out && lane == 0) atomicOr(&a.counters[1], kSaoErrWait);
-        const uint64_t wl = ((uint64_t)(uint32_t)lane_value((int)(word >> 32), 0) << 32) |
-                            (uint32_t)lane_value((int)(uint32_t)word, 0);
-        const uint64_t wa = ((uint64_t)(uint32_t)lane_value((int)(word >> 32), 1) << 32) |
-                            (uint32_t)lane_value((int)(uint32_t)word, 1);
-        constexpr uint32_t kMask25 = (1u << 25) - 1;
-        uint32_t fl = sao_pack(own_l), fc = sao_pack(own_c);
-        int merge = 0;
-        // sao.cpp:553-583: merge left, else merge up, when it costs less.
-        if (bx > 0) {
-            const SaoP L = sao_unpack((uint32_t)wl & kMask25);
-            if (L.type != kSaoOff) {
-                const float mc = (float)sao_block_ssd<16>(&sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, L) +
-                                 a.lambda * 1.0f;
-                if (mc < cur) {
-                    merge = 1;
-                    fl = (uint32_t)wl & kMask25;
-                    fc = (uint32_t)(wl >> 25) & kMask25;
-                    cur = mc;
+    const int mb = blockIdx.x, bx = mb % a.wb, by = mb / a.wb;
+    const int cw = a.w / 2, chh = a.h / 2;
+    int sy[4], su[1];
+    sao_load_luma(sh, a, bx, by, sy);
+    for (int i = lane; i < 10 * 10; i += 64) {
+        const int X = bx * 8 + i % 10 - 1, Y = by * 8 + i / 10 - 1;
+        sh.ru[i / 10][i % 10] = X >= 0 && X < cw && Y >= 0 && Y < chh ? a.rec_u[(size_t)Y * cw + X] : 0;
+    }
+    su[0] = a.src_u[(size_t)(by * 8 + lane / 8) * a.spc + bx * 8 + lane % 8];
+    __syncthreads();
+    int64_t lssd, cssd;
+    int lbits, cbits;
+    const SaoP own_l = sao_rdo<16>(sh, &sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, a.lambda, lssd, lbits);
+    const SaoP own_c = sao_rdo<8>(sh, &sh.ru[0][0], su, bx * 8, by * 8, cw, chh, a.lambda, cssd, cbits);
+    // sao.cpp:538-550: the merge cost of the own params, less the merge
+    // flag, plus their bits; the plain SSD and 2 bits when off.
+    const float cur = own_l.type == kSaoOff
+                          ? (float)lssd + a.lambda * 2.0f
+                          : (((float)lssd + a.lambda * 1.0f) - a.lambda * 1.0f) + a.lambda * (float)lbits;
+    if (lane == 0) {
+        a.own_l[mb] = sao_pack(own_l);
+        a.own_c[mb] = sao_pack(own_c);
+        a.cur[mb] = cur;
+    }
+}
+
+// Stage 2: each MB's merge costs under the left and upper MBs' own luma
+// params (used when the neighbour keeps them), one wave per MB.
+__global__ __launch_bounds__(64) void sao_cost_kernel(const SaoArgs a)
+{
+    __shared__ SaoLds sh;
+    const int mb = blockIdx.x, bx = mb % a.wb, by = mb / a.wb;
+    int sy[4];
+    sao_load_luma(sh, a, bx, by, sy);
+    __syncthreads();
+    float ml = 0.0f, ma = 0.0f;
+    if (bx > 0) {
+        const SaoP p = sao_unpack(a.own_l[mb - 1]);
+        if (p.type != kSaoOff) ml = sao_merge_cost(sh, a, bx, by, sy, p);
+    }
+    if (by > 0) {
+        const SaoP p = sao_unpack(a.own_l[mb - a.wb]);
+        if (p.type != kSaoOff) ma = sao_merge_cost(sh, a, bx, by, sy, p);
+    }
+    if (lane_id() == 0) {
+        a.mcl[mb] = ml;
+        a.mca[mb] = ma;
+    }
+}
+
+// Stage 3: the merges in the reference's raster order (sao.cpp:553-583).
+// One wave per MB row walks it left to right, the left MB's decision in
+// registers; the upper row's decided words (device-scope 64-bit accesses:
+// bit 63 = published | merge_above << 51 | merge_left << 50 | chroma params
+// << 25 | luma params) are polled 64 at a time.  A merge cost is taken
+// from stage 2 when the neighbour kept its own luma params, else computed
+// here.  Dynamic LDS: 32 bytes per MB of the row.
+__global__ __launch_bounds__(64) void sao_merge_kernel(const SaoArgs a)
+{
+    extern __shared__ uint64_t row64[];
+    __shared__ SaoLds sh;
+    uint32_t *row = reinterpret_cast<uint32_t *>(row64);
+    const int lane = lane_id(), by = blockIdx.x, wb = a.wb;
+    uint32_t *r_ol = row, *r_oc = row + wb, *r_al = row + 5 * wb;
+    float *r_cur = reinterpret_cast<float *>(row + 2 * wb);
+    float *r_mcl = reinterpret_cast<float *>(row + 3 * wb);
+    float *r_mca = reinterpret_cast<float *>(row + 4 * wb);
+    uint64_t *r_aw = reinterpret_cast<uint64_t *>(row + 6 * wb);
+    for (int i = lane; i < wb; i += 64) {
+        const int m = by * wb + i;
+        r_ol[i] = a.own_l[m];
+        r_oc[i] = a.own_c[m];
+        r_cur[i] = a.cur[m];
+        r_mcl[i] = a.mcl[m];
+        r_mca[i] = a.mca[m];
+        r_al[i] = by > 0 ? a.own_l[m - wb] : 0u;
+        r_aw[i] = 0;
+    }
+    __syncthreads();
+    uint32_t pl = 0, pc = 0;               // the left MB's decided params
+    int avail = by > 0 ? 0 : wb;           // upper words known published below this column
+    int spins = 0;
+    bool timed_out = false;
+    for (int bx = 0; bx < wb; bx++) {
+        const int mb = by * wb + bx;
+        if (bx >= avail) {
+            int nap = 1;
+            for (;;) {
+                const int c = bx + lane;
+                uint64_t wv = 0;
+                if (c < wb) wv = ld_relaxed64(a.words + mb - wb + lane);
+                const uint64_t pub = __ballot(c < wb && (wv >> 63) != 0);
+                const int run = ~pub == 0 ? 64 : __builtin_ctzll(~pub);
+                if (lane < run) r_aw[c] = wv;
+                if (run > 0) {
+                    avail = bx + run;
+                    break;
                 }
+                if (++spins > kSpinLimit) {
+                    timed_out = true;
+                    avail = wb;
+                    break;
+                }
+                for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+                nap = min(nap * 2, kMaxNap);
+            }
+            __syncthreads();
+        }
+        float cur = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r_cur[bx])));
+        uint32_t fl = __builtin_amdgcn_readfirstlane(r_ol[bx]);
+        uint32_t fc = __builtin_amdgcn_readfirstlane(r_oc[bx]);
+        int merge = 0;
+        if (bx > 0 && (pl & 3) != kSaoOff) {
+            float mc;
+            if (pl == __builtin_amdgcn_readfirstlane(r_ol[bx - 1])) {
+                mc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r_mcl[bx])));
+            } else {
+                int sy[4];
+                __syncthreads();
+                sao_load_luma(sh, a, bx, by, sy);
+                __syncthreads();
+                mc = sao_merge_cost(sh, a, bx, by, sy, sao_unpack(pl));
+            }
+            if (mc < cur) {
+                merge = 1;
+                fl = pl;
+                fc = pc;
+                cur = mc;
             }
         }
         if (merge == 0 && by > 0) {
-            const SaoP A = sao_unpack((uint32_t)wa & kMask25);
-            if (A.type != kSaoOff) {
-                const float mc = (float)sao_block_ssd<16>(&sh.ry[0][0], sy, bx * 16, by * 16, a.w, a.h, A) +
-                                 a.lambda * 1.0f;
+            const uint64_t aw =
+                ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(r_aw[bx] >> 32)) << 32) |
+                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)r_aw[bx]);
+            const uint32_t al = (uint32_t)aw & kSaoMask25;
+            if ((al & 3) != kSaoOff) {
+                float mc;
+                if (al == __builtin_amdgcn_readfirstlane(r_al[bx])) {
+                    mc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r_mca[bx])));
+                } else {
+                    int sy[4];
+                    __syncthreads();
+                    sao_load_luma(sh, a, bx, by, sy);
+                    __syncthreads();
+                    mc = sao_merge_cost(sh, a, bx, by, sy, sao_unpack(al));
+                }
                 if (mc < cur) {
                     merge = 2;
-                    fl = (uint32_t)wa & kMask25;
-                    fc = (uint32_t)(wa >> 25) & kMask25;
+                    fl = al;
+                    fc = (uint32_t)(aw >> 25) & kSaoMask25;
                 }
             }
         }
@@ -2616,7 +2712,10 @@ __global__ __launch_bounds__(64) void sao_decide_kernel(const SaoArgs a)
         if (lane < 5)
             e[lane] = lane == 0 ? (uint32_t)(merge == 1) | (uint32_t)(merge == 2) << 8
                     : lane == 1 ? lw0 : lane == 2 ? lw1 : lane == 3 ? cw0 : cw1;
+        pl = fl;
+        pc = fc;
     }
+    if (timed_out && lane == 0) atomicOr(&a.counters[1], kSaoErrWait);
 }
 
 struct SaoApplyArgs {
@@ -2847,7 +2946,7 @@ hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, in
 }
 
 hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const DevPic &src,
-                             int32_t w, int32_t h, float lambda, const uint32_t *order,
+                             int32_t w, int32_t h, float lambda, uint32_t *scratch,
                              uint64_t *words, evxa_sao_info *table, int32_t *counters,
                              hipStream_t st)
 {
@@ -2863,12 +2962,18 @@ hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const D
     a.wb = w / 16;
     a.hb = h / 16;
     a.lambda = lambda;
-    a.order = order;
+    const int n = a.wb * a.hb;
+    a.own_l = scratch;
+    a.own_c = scratch + n;
+    a.cur = reinterpret_cast<float *>(scratch + 2 * n);
+    a.mcl = reinterpret_cast<float *>(scratch + 3 * n);
+    a.mca = reinterpret_cast<float *>(scratch + 4 * n);
     a.words = words;
     a.table = table;
     a.counters = counters;
-    const int n = a.wb * a.hb;
-    hipLaunchKernelGGL(sao_decide_kernel, dim3(min(n, 4096)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(sao_own_kernel, dim3(n), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(sao_cost_kernel, dim3(n), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(sao_merge_kernel, dim3(a.hb), dim3(64), (size_t)a.wb * 32, st, a);
     return hipGetLastError();
 }
 

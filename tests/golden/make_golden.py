@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from chicago_amd import synth  # noqa: E402
-from oracle.oracle import RECORD_DTYPE, REF_FIELDS, Picture, Reference  # noqa: E402
+from oracle.oracle import RECORD_DTYPE, REF_FIELDS, SAO_DTYPE, Picture, Reference  # noqa: E402
 
 
 def save(name, **arrays):
@@ -248,12 +248,63 @@ def deblock_fixtures():
     save("deblock.npz", **out)
 
 
+def sao_fixtures():
+    """sao.npz: the SAO filter under random tables (every type, class, band
+    and merge flag; samples outside [0, 255]), and the encoder's decisions
+    and filtered planes: for the deblocked reconstructions of deblock.npz
+    against their source frames, and for content mixing noise (edge
+    offsets), a constant bias (band offsets) and exact copies (off)."""
+    ref = Reference()
+    rng = np.random.default_rng(20261206)
+    out = {}
+    w, h = 176, 144
+    pic = tuple((p.astype(np.int32) + rng.integers(-40, 41, p.shape)).astype(np.int16)
+                for p in synth.blocky_picture(21, w, h, amp=20))
+    n = (w // 16) * (h // 16)
+    t = np.zeros(n, SAO_DTYPE)
+    for pre in ("luma", "chroma"):
+        t[pre + "_type"] = rng.choice([0, 1, 1, 2], n)
+        t[pre + "_eo_class"] = rng.integers(0, 4, n)
+        t[pre + "_band_position"] = rng.integers(0, 29, n)
+        t[pre + "_offsets"] = rng.integers(-7, 8, (n, 4))
+    t["merge_left"] = rng.random(n) < 0.2
+    t["merge_above"] = rng.random(n) < 0.2
+    out["filter_in_y"], out["filter_in_u"], out["filter_in_v"] = pic
+    out["filter_table"] = t
+    out["filter_y"], out["filter_u"], out["filter_v"] = ref.sao_filter_frame(t, *pic)
+    g = np.load(os.path.join(HERE, "frames.npz"))
+    d = np.load(os.path.join(HERE, "deblock.npz"))
+    for tag, cur, q in (("p1", 1, 8), ("p4", 5, 12), ("b", 3, 9)):
+        src = [g[f"f{cur}_{c}"] for c in "yuv"]
+        rec = [d[f"{tag}_{c}"] for c in "yuv"]
+        table, y, u, v = ref.sao_encode_frame(src, rec, ref.lam(q))
+        out[f"{tag}_table"], out[f"{tag}_y"], out[f"{tag}_u"], out[f"{tag}_v"] = table, y, u, v
+    src = synth.blocky_picture(22, w, h, amp=3)
+    rec = []
+    for k, p in enumerate(src):
+        mb = 16 if k == 0 else 8
+        r = p.astype(np.int32) + rng.integers(-4, 5, p.shape)
+        kind = np.repeat(np.repeat(rng.integers(0, 4, (p.shape[0] // mb, p.shape[1] // mb)), mb, 0), mb, 1)
+        r = np.where(kind == 1, p.astype(np.int32) + 5, r)
+        r = np.where(kind == 2, p, r)
+        rec.append(r.astype(np.int16))
+    out["mix_src_y"], out["mix_src_u"], out["mix_src_v"] = src
+    out["mix_rec_y"], out["mix_rec_u"], out["mix_rec_v"] = rec
+    for q in (6, 20):
+        table, y, u, v = ref.sao_encode_frame(src, rec, ref.lam(q))
+        out[f"mix{q}_table"], out[f"mix{q}_y"], out[f"mix{q}_u"], out[f"mix{q}_v"] = table, y, u, v
+    save("sao.npz", **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["recon"]:
         recon_fixtures()
     elif sys.argv[1:] == ["deblock"]:
         deblock_fixtures()
+    elif sys.argv[1:] == ["sao"]:
+        sao_fixtures()
     else:
         main()
         recon_fixtures()
         deblock_fixtures()
+        sao_fixtures()

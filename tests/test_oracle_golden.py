@@ -177,3 +177,40 @@ def test_deblock_synthetic_frames(oracle, tag):
     out = oracle.deblock_frame(d[f"{tag}_records"], *(d[f"{tag}_in_{c}"] for c in "yuv"))
     for c, plane in zip("yuv", out):
         assert np.array_equal(plane, d[f"{tag}_{c}"]), (tag, c)
+
+
+def _sao_same(a, b):
+    from oracle.oracle import SAO_DTYPE
+    return all(np.array_equal(a[f], b[f]) for f in SAO_DTYPE.names if "reserved" not in f)
+
+
+def test_sao_filter_table(oracle):
+    """sao.cpp:98-148 under a random table against the reference build
+    (sao.npz)."""
+    g = load("sao.npz")
+    out = oracle.sao_filter_frame(g["filter_table"], *(g[f"filter_in_{c}"] for c in "yuv"))
+    for c, plane in zip("yuv", out):
+        assert np.array_equal(plane, g[f"filter_{c}"]), c
+
+
+@pytest.mark.parametrize("tag,cur,q", [("p1", 1, 8), ("p4", 5, 12), ("b", 3, 9)])
+def test_sao_encode_deblocked_frames(oracle, tag, cur, q):
+    """sao.cpp:378-590 on the deblocked reconstructions of deblock.npz
+    against their source frames."""
+    f, d, g = load("frames.npz"), load("deblock.npz"), load("sao.npz")
+    table, *planes = oracle.sao_encode_frame([f[f"f{cur}_{c}"] for c in "yuv"],
+                                             [d[f"{tag}_{c}"] for c in "yuv"], oracle.lam(q))
+    assert _sao_same(table, g[f"{tag}_table"])
+    for c, plane in zip("yuv", planes):
+        assert np.array_equal(plane, g[f"{tag}_{c}"]), c
+
+
+@pytest.mark.parametrize("q", [6, 20])
+def test_sao_encode_mixed_content(oracle, q):
+    """Edge, band and no offsets, and merges left and up."""
+    g = load("sao.npz")
+    table, *planes = oracle.sao_encode_frame([g[f"mix_src_{c}"] for c in "yuv"],
+                                             [g[f"mix_rec_{c}"] for c in "yuv"], oracle.lam(q))
+    assert _sao_same(table, g[f"mix{q}_table"])
+    for c, plane in zip("yuv", planes):
+        assert np.array_equal(plane, g[f"mix{q}_{c}"]), c

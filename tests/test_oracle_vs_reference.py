@@ -169,3 +169,33 @@ def test_deblock_frame_random(oracle, ref, w, h, seed):
     for c, x, y in zip("yuv", a, b):
         assert np.array_equal(x, y), c
     assert any((x != p).any() for x, p in zip(a, pic)), "nothing filtered"
+
+
+@pytest.mark.parametrize("w,h,q", [(64, 48, 4), (176, 144, 12), (320, 192, 24)])
+def test_sao_random(oracle, ref, w, h, q):
+    """SAO decisions and filtered planes against sao_encode_frame itself,
+    and the filter alone under random tables."""
+    from oracle.oracle import SAO_DTYPE
+    rng = np.random.default_rng(q)
+    src = synth.blocky_picture(int(rng.integers(1 << 30)), w, h, amp=3)
+    rec = tuple((p.astype(np.int32) + rng.integers(-4, 5, p.shape) +
+                 (rng.random(p.shape) < 0.02) * 60).astype(np.int16) for p in src)
+    a = oracle.sao_encode_frame(src, rec, oracle.lam(q))
+    b = ref.sao_encode_frame(src, rec, ref.lam(q))
+    for f in SAO_DTYPE.names:
+        if "reserved" not in f:
+            assert np.array_equal(a[0][f], b[0][f]), f
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
+    n = (w // 16) * (h // 16)
+    t = np.zeros(n, SAO_DTYPE)
+    for pre in ("luma", "chroma"):
+        t[pre + "_type"] = rng.integers(0, 3, n)
+        t[pre + "_eo_class"] = rng.integers(0, 4, n)
+        t[pre + "_band_position"] = rng.integers(0, 29, n)
+        t[pre + "_offsets"] = rng.integers(-7, 8, (n, 4))
+    t["merge_left"] = rng.random(n) < 0.3
+    t["merge_above"] = rng.random(n) < 0.3
+    big = tuple((p.astype(np.int32) * 3 - 200).astype(np.int16) for p in rec)
+    for x, y in zip(oracle.sao_filter_frame(t, *big), ref.sao_filter_frame(t, *big)):
+        assert np.array_equal(x, y)
