@@ -1020,13 +1020,13 @@ namespace {
 constexpr int kSaoCounters = 64;
 
 // The SAO buffers of a stream, allocated on first use: sao_words holds the
-// decided words (one per MB), then kSaoCounters counters, then 5 words per
+// decided words (one per MB), then kSaoCounters counters, then 8 words per
 // MB of per-stage scratch.
 void sao_buffers(evxa_stream *s)
 {
     const size_t n = (size_t)s->wb * s->hb;
     lazy_alloc(s, s->sao_words, n * sizeof(uint64_t) + kSaoCounters * sizeof(int32_t) +
-                                    5 * n * sizeof(uint32_t));
+                                    8 * n * sizeof(uint32_t));
     lazy_alloc(s, s->sao_table, n * sizeof(evxa_sao_info));
 }
 

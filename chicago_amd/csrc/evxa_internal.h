@@ -119,7 +119,7 @@ hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, in
                         int16_t *out, int32_t w, int32_t h, int32_t *wide, hipStream_t st);
 // SAO decisions of a frame (sao.cpp:378-590): the unfiltered reconstruction
 // (packed luma and U planes), the frame's source picture, lambda; `scratch`
-// holds 5 words per MB, `words` (one per MB) starts at zero; decisions go to
+// holds 8 words per MB, `words` (one per MB) starts at zero; decisions go to
 // `table`; counters[1] (zero at the start) gets error flags (1: a wait
 // timed out, 2: an SAO chain deeper than three blocks).
 hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const DevPic &src,

@@ -2319,19 +2319,36 @@ __device__ __forceinline__ int sao_bits(const SaoP &p)
     return bits;
 }
 
+template <int CTRL>
+__device__ __forceinline__ int64_t dpp64(int64_t v)
+{
+    const uint32_t lo = (uint32_t)dpp<CTRL>((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)dpp<CTRL>((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int64_t lane_value64(int64_t v, int lane)
+{
+    const uint32_t lo = (uint32_t)lane_value((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)lane_value((int)(uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// A 64-bit sum over the wave (row reductions as row_sum, then the 4 rows).
 __device__ __forceinline__ int64_t wave_sum64(int64_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp64<0xB1>(v);
+    v += dpp64<0x4E>(v);
+    v += dpp64<0x141>(v);
+    v += dpp64<0x140>(v);
+    return lane_value64(v, 0) + lane_value64(v, 16) + lane_value64(v, 32) + lane_value64(v, 48);
 }
 
 struct SaoLds {
     int16_t ry[18][18];          // luma reconstruction, rows and columns -1 .. 16 of the MB
     int16_t ru[10][10];          // U, -1 .. 8
-    int32_t bsum[32], bcnt[32], boff[32];
-    int32_t esum[16], ecnt[16], eoff[16];    // [class * 4 + category]
-    unsigned long long bd[32], ed[4];        // SSD change per band / per edge class
+    int32_t boff[32];            // band offsets
+    long long bd[32];            // SSD change per band under its offset
 };
 
 // The rounded mean of src - rec over a class, clamped to +-7 (sao.cpp:199-214).
@@ -2340,6 +2357,15 @@ __device__ __forceinline__ int sao_mean(int sum, int cnt)
     if (cnt <= 0) return 0;
     const int avg = (sum + (sum >= 0 ? cnt / 2 : -cnt / 2)) / cnt;
     return min(max(avg, -7), 7);
+}
+
+__device__ __forceinline__ int wave_min(int v)
+{
+    v = min(v, dpp<0xB1>(v));
+    v = min(v, dpp<0x4E>(v));
+    v = min(v, dpp<0x141>(v));
+    v = min(v, dpp<0x140>(v));
+    return min(min(lane_value(v, 0), lane_value(v, 16)), min(lane_value(v, 32), lane_value(v, 48)));
 }
 
 // The corrected value of sample (x, y) (window position r, c) under p, the
@@ -2382,9 +2408,11 @@ __device__ __forceinline__ int64_t sao_block_ssd(const int16_t *win, const int (
 }
 
 // sao.cpp:316-331 with 163-239 and 241-314: the best params of one N x N
-// block.  Class statistics, offsets and the SSD change of every band and
-// edge class are gathered once; the 29 band and 4 edge candidates are then
-// priced in the reference's order (ties keep the earlier).  best_ssd and
+// block.  Each class's sum and count of src - rec, its offset and the SSD
+// change it brings are wave reductions: band by band over the bands
+// present (the lowest unprocessed band of the wave each pass), then per
+// edge class and category.  The 29 band and 4 edge candidates are then
+// priced in the reference's order (ties keep the earlier); best_ssd and
 // best_bits describe the winner.
 template <int N>
 __device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 64], int x0,
@@ -2393,27 +2421,18 @@ __device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 
     constexpr int K = N * N / 64, P = N + 2;
     const int lane = lane_id();
     if (lane < 32) {
-        sh.bsum[lane] = 0;
-        sh.bcnt[lane] = 0;
+        sh.boff[lane] = 0;
         sh.bd[lane] = 0;
     }
-    if (lane < 16) {
-        sh.esum[lane] = 0;
-        sh.ecnt[lane] = 0;
-    }
-    if (lane < 4) sh.ed[lane] = 0;
-    __syncthreads();
-    int v[K], band[K], cat[K][4];
+    int v[K], d[K], band[K], cat[K][4];
     int64_t base = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int idx = lane * K + k, r = idx / N, c = idx % N, x = x0 + c, y = y0 + r;
         v[k] = win[(r + 1) * P + c + 1];
-        const int d = sv[k] - v[k];
-        base += (int64_t)d * d;
+        d[k] = sv[k] - v[k];
+        base += (int64_t)d[k] * d[k];
         band[k] = sao_clamp(v[k]) >> 3;
-        atomicAdd(&sh.bsum[band[k]], d);
-        atomicAdd(&sh.bcnt[band[k]], 1);
 #pragma unroll
         for (int cls = 0; cls < 4; cls++) {
             int dx, dy;
@@ -2423,31 +2442,69 @@ __device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 
             cat[k][cls] = in ? sao_category(v[k], win[(r + 1 + dy) * P + c + 1 + dx],
                                             win[(r + 1 - dy) * P + c + 1 - dx])
                              : 4;
-            if (cat[k][cls] < 4) {
-                atomicAdd(&sh.esum[cls * 4 + cat[k][cls]], d);
-                atomicAdd(&sh.ecnt[cls * 4 + cat[k][cls]], 1);
-            }
-        }
-    }
-    __syncthreads();
-    if (lane < 32) sh.boff[lane] = sao_mean(sh.bsum[lane], sh.bcnt[lane]);
-    else if (lane < 48) sh.eoff[lane - 32] = sao_mean(sh.esum[lane - 32], sh.ecnt[lane - 32]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int d = sv[k] - v[k];
-        const int64_t d0 = (int64_t)d * d;
-        const int cb = sv[k] - sao_clamp(v[k] + sh.boff[band[k]]);
-        atomicAdd(&sh.bd[band[k]], (unsigned long long)((int64_t)cb * cb - d0));
-#pragma unroll
-        for (int cls = 0; cls < 4; cls++) {
-            if (cat[k][cls] < 4) {
-                const int ce = sv[k] - sao_clamp(v[k] + sh.eoff[cls * 4 + cat[k][cls]]);
-                atomicAdd(&sh.ed[cls], (unsigned long long)((int64_t)ce * ce - d0));
-            }
         }
     }
     base = wave_sum64(base);
+    __syncthreads();
+    // Band classes present, lowest first.
+    uint32_t done = 0;
+    for (;;) {
+        int mine = 32;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (!((done >> k) & 1)) mine = min(mine, band[k]);
+        const int b0 = wave_min(mine);
+        if (b0 >= 32) break;
+        int sum = 0, cnt = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (!((done >> k) & 1) && band[k] == b0) {
+                sum += d[k];
+                cnt++;
+            }
+        const int off = sao_mean(wave_sum(sum), wave_sum(cnt));
+        int64_t dl = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (!((done >> k) & 1) && band[k] == b0) {
+                const int e = sv[k] - sao_clamp(v[k] + off);
+                dl += (int64_t)e * e - (int64_t)d[k] * d[k];
+                done |= 1u << k;
+            }
+        dl = wave_sum64(dl);
+        if (lane == 0) {
+            sh.boff[b0] = off;
+            sh.bd[b0] = dl;
+        }
+    }
+    // Edge classes.
+    int eoff[4][4];
+    int64_t ed[4];
+#pragma unroll
+    for (int cls = 0; cls < 4; cls++) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            int sum = 0, cnt = 0;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (cat[k][cls] == ct) {
+                    sum += d[k];
+                    cnt++;
+                }
+            eoff[cls][ct] = sao_mean(wave_sum(sum), wave_sum(cnt));
+        }
+        int64_t dl = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int ct = cat[k][cls];
+            if (ct < 4) {
+                const int o = ct == 0 ? eoff[cls][0] : ct == 1 ? eoff[cls][1] : ct == 2 ? eoff[cls][2] : eoff[cls][3];
+                const int e = sv[k] - sao_clamp(v[k] + o);
+                dl += (int64_t)e * e - (int64_t)d[k] * d[k];
+            }
+        }
+        ed[cls] = wave_sum64(dl);
+    }
     __syncthreads();
     SaoP best{};
     float best_cost = (float)base + lambda * 2.0f;
@@ -2463,7 +2520,7 @@ __device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 
         for (int j = 0; j < 4; j++) {
             c.off[j] = sh.boff[bp + j];
             nz = nz || c.off[j] != 0;
-            ssd += (int64_t)sh.bd[bp + j];
+            ssd += sh.bd[bp + j];
         }
         if (!nz) continue;
         const int bits = sao_bits(c);
@@ -2475,6 +2532,7 @@ __device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 
             best_bits = bits;
         }
     }
+#pragma unroll
     for (int cls = 0; cls < 4; cls++) {
         SaoP c{};
         c.type = kSaoEdge;
@@ -2482,11 +2540,11 @@ __device__ SaoP sao_rdo(SaoLds &sh, const int16_t *win, const int (&sv)[N * N / 
         bool nz = false;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            c.off[j] = sh.eoff[cls * 4 + j];
+            c.off[j] = eoff[cls][j];
             nz = nz || c.off[j] != 0;
         }
         if (!nz) continue;
-        const int64_t ssd = base + (int64_t)sh.ed[cls];
+        const int64_t ssd = base + ed[cls];
         const int bits = sao_bits(c);
         const float cost = (float)ssd + lambda * (float)bits;
         if (cost < best_cost) {
@@ -2507,7 +2565,7 @@ struct SaoArgs {
     float lambda;
     uint32_t *own_l, *own_c;        // per MB: its own best params (packed)
     float *cur;                     // per MB: the cost of its own luma params
-    float *mcl, *mca;               // per MB: merge costs under the left / upper MB's own luma params
+    float *mc;                      // 5 planes of n: merge costs under neighbours' own luma params
     uint64_t *words;                // per MB: the decided params (zero until published)
     evxa_sao_info *table;
     int32_t *counters;              // [1] error flags
@@ -2574,8 +2632,11 @@ __global__ __launch_bounds__(64) void sao_own_kernel(const SaoArgs a)
     }
 }
 
-// Stage 2: each MB's merge costs under the left and upper MBs' own luma
-// params (used when the neighbour keeps them), one wave per MB.
+// Stage 2: each MB's merge costs under the own luma params of the MBs whose
+// decided params a merge can hand it: left, upper, and the ones a merging
+// neighbour takes them from (left of left, upper-left, upper of upper).
+// Slots of `mc` per MB: 0 left, 1 upper, 2 left of left, 3 upper-left,
+// 4 upper of upper (zero where absent or off).  One wave per MB.
 __global__ __launch_bounds__(64) void sao_cost_kernel(const SaoArgs a)
 {
     __shared__ SaoLds sh;
@@ -2583,137 +2644,131 @@ __global__ __launch_bounds__(64) void sao_cost_kernel(const SaoArgs a)
     int sy[4];
     sao_load_luma(sh, a, bx, by, sy);
     __syncthreads();
-    float ml = 0.0f, ma = 0.0f;
-    if (bx > 0) {
-        const SaoP p = sao_unpack(a.own_l[mb - 1]);
-        if (p.type != kSaoOff) ml = sao_merge_cost(sh, a, bx, by, sy, p);
+    const int src[5] = {bx > 0 ? mb - 1 : -1, by > 0 ? mb - a.wb : -1, bx > 1 ? mb - 2 : -1,
+                        bx > 0 && by > 0 ? mb - a.wb - 1 : -1, by > 1 ? mb - 2 * a.wb : -1};
+    float mc[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        mc[k] = 0.0f;
+        if (src[k] >= 0) {
+            const SaoP p = sao_unpack(a.own_l[src[k]]);
+            if (p.type != kSaoOff) mc[k] = sao_merge_cost(sh, a, bx, by, sy, p);
+        }
     }
-    if (by > 0) {
-        const SaoP p = sao_unpack(a.own_l[mb - a.wb]);
-        if (p.type != kSaoOff) ma = sao_merge_cost(sh, a, bx, by, sy, p);
-    }
-    if (lane_id() == 0) {
-        a.mcl[mb] = ml;
-        a.mca[mb] = ma;
+    if (lane_id() < 5) {
+        const int l = lane_id();
+        a.mc[(size_t)l * a.wb * a.hb + mb] = l == 0 ? mc[0] : l == 1 ? mc[1] : l == 2 ? mc[2] : l == 3 ? mc[3] : mc[4];
     }
 }
 
 // Stage 3: the merges in the reference's raster order (sao.cpp:553-583).
-// One wave per MB row walks it left to right, the left MB's decision in
-// registers; the upper row's decided words (device-scope 64-bit accesses:
-// bit 63 = published | merge_above << 51 | merge_left << 50 | chroma params
-// << 25 | luma params) are polled 64 at a time.  A merge cost is taken
-// from stage 2 when the neighbour kept its own luma params, else computed
-// here.  Dynamic LDS: 32 bytes per MB of the row.
+// One wave per MB row walks it left to right in chunks of 64 columns: lane j
+// holds column base + j's own params, cost and stage-2 merge costs, and the
+// upper row's own luma and decided word, read per column with readlane; the
+// left MB's decision stays in scalars.  The upper row's words (device-scope
+// 64-bit accesses: bit 63 = published | merge_above << 51 | merge_left << 50
+// | chroma params << 25 | luma params) are polled for the whole chunk at
+// once.  A merge cost is taken from stage 2 when the params handed over are
+// the own luma of an MB it was computed for, else computed here.
 __global__ __launch_bounds__(64) void sao_merge_kernel(const SaoArgs a)
 {
-    extern __shared__ uint64_t row64[];
     __shared__ SaoLds sh;
-    uint32_t *row = reinterpret_cast<uint32_t *>(row64);
-    const int lane = lane_id(), by = blockIdx.x, wb = a.wb;
-    uint32_t *r_ol = row, *r_oc = row + wb, *r_al = row + 5 * wb;
-    float *r_cur = reinterpret_cast<float *>(row + 2 * wb);
-    float *r_mcl = reinterpret_cast<float *>(row + 3 * wb);
-    float *r_mca = reinterpret_cast<float *>(row + 4 * wb);
-    uint64_t *r_aw = reinterpret_cast<uint64_t *>(row + 6 * wb);
-    for (int i = lane; i < wb; i += 64) {
-        const int m = by * wb + i;
-        r_ol[i] = a.own_l[m];
-        r_oc[i] = a.own_c[m];
-        r_cur[i] = a.cur[m];
-        r_mcl[i] = a.mcl[m];
-        r_mca[i] = a.mca[m];
-        r_al[i] = by > 0 ? a.own_l[m - wb] : 0u;
-        r_aw[i] = 0;
-    }
-    __syncthreads();
+    const int lane = lane_id(), by = blockIdx.x, wb = a.wb, n = a.wb * a.hb;
+    auto rl = [](uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); };
+    auto rlf = [](float v, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j)); };
     uint32_t pl = 0, pc = 0;               // the left MB's decided params
-    int avail = by > 0 ? 0 : wb;           // upper words known published below this column
+    uint32_t ol1 = 0, ol2 = 0, al1 = 0;    // own luma of the left, left-of-left and upper-left MBs
     int spins = 0;
     bool timed_out = false;
-    for (int bx = 0; bx < wb; bx++) {
-        const int mb = by * wb + bx;
-        if (bx >= avail) {
-            int nap = 1;
-            for (;;) {
-                const int c = bx + lane;
-                uint64_t wv = 0;
-                if (c < wb) wv = ld_relaxed64(a.words + mb - wb + lane);
-                const uint64_t pub = __ballot(c < wb && (wv >> 63) != 0);
-                const int run = ~pub == 0 ? 64 : __builtin_ctzll(~pub);
-                if (lane < run) r_aw[c] = wv;
-                if (run > 0) {
-                    avail = bx + run;
-                    break;
+    for (int cb = 0; cb < wb; cb += 64) {
+        const int col = cb + lane, m = by * wb + col;
+        const bool in = col < wb;
+        const uint32_t ol = in ? a.own_l[m] : 0u, oc = in ? a.own_c[m] : 0u;
+        const float cur0 = in ? a.cur[m] : 0.0f;
+        float mcv[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) mcv[k] = in ? a.mc[(size_t)k * n + m] : 0.0f;
+        const uint32_t al = in && by > 0 ? a.own_l[m - wb] : 0u;
+        const uint32_t al2 = in && by > 1 ? a.own_l[m - 2 * wb] : 0u;
+        uint64_t aw = 0;
+        int avail = by > 0 ? cb : wb;      // upper words of this chunk known published below this column
+        const int end = min(cb + 64, wb);
+        for (int bx = cb; bx < end; bx++) {
+            const int j = bx - cb, mb = by * wb + bx;
+            if (bx >= avail) {
+                int nap = 1;
+                for (;;) {
+                    // Lanes from j on poll their column's upper word.
+                    if (in && lane >= j && (aw >> 63) == 0) aw = ld_relaxed64(a.words + m - wb);
+                    const uint64_t pub = __ballot(in && (aw >> 63) != 0) >> j;
+                    const int run = ~pub == 0 ? 64 - j : __builtin_ctzll(~pub);
+                    if (run > 0) {
+                        avail = bx + run;
+                        break;
+                    }
+                    if (++spins > kSpinLimit) {
+                        timed_out = true;
+                        avail = wb;
+                        break;
+                    }
+                    for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+                    nap = min(nap * 2, kMaxNap);
                 }
-                if (++spins > kSpinLimit) {
-                    timed_out = true;
-                    avail = wb;
-                    break;
-                }
-                for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
-                nap = min(nap * 2, kMaxNap);
             }
-            __syncthreads();
-        }
-        float cur = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r_cur[bx])));
-        uint32_t fl = __builtin_amdgcn_readfirstlane(r_ol[bx]);
-        uint32_t fc = __builtin_amdgcn_readfirstlane(r_oc[bx]);
-        int merge = 0;
-        if (bx > 0 && (pl & 3) != kSaoOff) {
-            float mc;
-            if (pl == __builtin_amdgcn_readfirstlane(r_ol[bx - 1])) {
-                mc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r_mcl[bx])));
-            } else {
+            float cur = rlf(cur0, j);
+            uint32_t fl = rl(ol, j), fc = rl(oc, j);
+            const uint32_t al_j = rl(al, j), al2_j = rl(al2, j);
+            auto cost = [&](uint32_t p) -> float {
+                if (bx > 0 && p == ol1) return rlf(mcv[0], j);
+                if (by > 0 && p == al_j) return rlf(mcv[1], j);
+                if (bx > 1 && p == ol2) return rlf(mcv[2], j);
+                if (bx > 0 && by > 0 && p == al1) return rlf(mcv[3], j);
+                if (by > 1 && p == al2_j) return rlf(mcv[4], j);
                 int sy[4];
                 __syncthreads();
                 sao_load_luma(sh, a, bx, by, sy);
                 __syncthreads();
-                mc = sao_merge_cost(sh, a, bx, by, sy, sao_unpack(pl));
-            }
-            if (mc < cur) {
-                merge = 1;
-                fl = pl;
-                fc = pc;
-                cur = mc;
-            }
-        }
-        if (merge == 0 && by > 0) {
-            const uint64_t aw =
-                ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(r_aw[bx] >> 32)) << 32) |
-                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)r_aw[bx]);
-            const uint32_t al = (uint32_t)aw & kSaoMask25;
-            if ((al & 3) != kSaoOff) {
-                float mc;
-                if (al == __builtin_amdgcn_readfirstlane(r_al[bx])) {
-                    mc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r_mca[bx])));
-                } else {
-                    int sy[4];
-                    __syncthreads();
-                    sao_load_luma(sh, a, bx, by, sy);
-                    __syncthreads();
-                    mc = sao_merge_cost(sh, a, bx, by, sy, sao_unpack(al));
-                }
+                return sao_merge_cost(sh, a, bx, by, sy, sao_unpack(p));
+            };
+            int merge = 0;
+            if (bx > 0 && (pl & 3) != kSaoOff) {
+                const float mc = cost(pl);
                 if (mc < cur) {
-                    merge = 2;
-                    fl = al;
-                    fc = (uint32_t)(aw >> 25) & kSaoMask25;
+                    merge = 1;
+                    fl = pl;
+                    fc = pc;
+                    cur = mc;
                 }
             }
+            if (merge == 0 && by > 0) {
+                const uint64_t w = ((uint64_t)rl((uint32_t)(aw >> 32), j) << 32) | rl((uint32_t)aw, j);
+                const uint32_t ul = (uint32_t)w & kSaoMask25;
+                if ((ul & 3) != kSaoOff) {
+                    const float mc = cost(ul);
+                    if (mc < cur) {
+                        merge = 2;
+                        fl = ul;
+                        fc = (uint32_t)(w >> 25) & kSaoMask25;
+                    }
+                }
+            }
+            if (lane == 0)
+                st_relaxed64(a.words + mb, 1ull << 63 | (uint64_t)(merge == 2) << 51 |
+                                               (uint64_t)(merge == 1) << 50 | (uint64_t)fc << 25 | fl);
+            // The table entry (evxa_sao_info, five 32-bit words).
+            uint32_t lw0, lw1, cw0, cw1;
+            sao_to_words(sao_unpack(fl), lw0, lw1);
+            sao_to_words(sao_unpack(fc), cw0, cw1);
+            uint32_t *e = reinterpret_cast<uint32_t *>(a.table + mb);
+            if (lane < 5)
+                e[lane] = lane == 0 ? (uint32_t)(merge == 1) | (uint32_t)(merge == 2) << 8
+                        : lane == 1 ? lw0 : lane == 2 ? lw1 : lane == 3 ? cw0 : cw1;
+            pl = fl;
+            pc = fc;
+            ol2 = ol1;
+            ol1 = rl(ol, j);
+            al1 = al_j;
         }
-        if (lane == 0)
-            st_relaxed64(a.words + mb, 1ull << 63 | (uint64_t)(merge == 2) << 51 |
-                                           (uint64_t)(merge == 1) << 50 | (uint64_t)fc << 25 | fl);
-        // The table entry (evxa_sao_info, five 32-bit words).
-        uint32_t lw0, lw1, cw0, cw1;
-        sao_to_words(sao_unpack(fl), lw0, lw1);
-        sao_to_words(sao_unpack(fc), cw0, cw1);
-        uint32_t *e = reinterpret_cast<uint32_t *>(a.table + mb);
-        if (lane < 5)
-            e[lane] = lane == 0 ? (uint32_t)(merge == 1) | (uint32_t)(merge == 2) << 8
-                    : lane == 1 ? lw0 : lane == 2 ? lw1 : lane == 3 ? cw0 : cw1;
-        pl = fl;
-        pc = fc;
     }
     if (timed_out && lane == 0) atomicOr(&a.counters[1], kSaoErrWait);
 }
@@ -2966,14 +3021,13 @@ hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const D
     a.own_l = scratch;
     a.own_c = scratch + n;
     a.cur = reinterpret_cast<float *>(scratch + 2 * n);
-    a.mcl = reinterpret_cast<float *>(scratch + 3 * n);
-    a.mca = reinterpret_cast<float *>(scratch + 4 * n);
+    a.mc = reinterpret_cast<float *>(scratch + 3 * n);
     a.words = words;
     a.table = table;
     a.counters = counters;
     hipLaunchKernelGGL(sao_own_kernel, dim3(n), dim3(64), 0, st, a);
     hipLaunchKernelGGL(sao_cost_kernel, dim3(n), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(sao_merge_kernel, dim3(a.hb), dim3(64), (size_t)a.wb * 32, st, a);
+    hipLaunchKernelGGL(sao_merge_kernel, dim3(a.hb), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
