@@ -659,7 +659,7 @@ def run_recon(args):
     dec_s, app_s = float(np.mean(ms[:, 2])) / 1e3, float(np.mean(ms[:, 3])) / 1e3
     res["sao"] = {
         "workload": "SAO: the deblocked 1920x1080 reconstruction against its source frame",
-        "kernels": "sao_decide_kernel (per-MB search, raster-order merges), sao_apply_kernel",
+        "kernels": "decisions: sao_own_kernel, sao_cost_kernel, sao_merge_kernel; filter: sao_apply_kernel",
         "decide_us_avg": round(dec_s * 1e6, 2),
         "apply_us_avg": round(app_s * 1e6, 2),
         "reconstruct_deblock_sao_call_us": round(chain_s * 1e6, 2),
