@@ -353,6 +353,52 @@ class Reference(_Lib, _Recon):
     def lam(self, q):
         return self._lambda(q)
 
+    def analyze_frame_serialized(self, ftype, quality, src, refs, prev=None, target_ids=None):
+        """The unified block table (serialize.cpp:184-314) the reference's
+        own analysis leaves, arithmetic-coded as serialize_slice writes it."""
+        if not hasattr(self, "_afs"):
+            self._afs = self.f("analyze_frame_serialized", C.c_int32, C.POINTER(FrameJob),
+                               C.c_void_p, C.c_int32)
+        job = make_job(ftype, quality, src, refs, prev, target_ids)
+        out = np.zeros(1 << 22, np.uint8)
+        n = self._afs(C.byref(job), out.ctypes.data, out.size)
+        if n < 0:
+            raise RuntimeError(f"analyze_frame_serialized failed: {n}")
+        return out[:n].tobytes()
+
+    def serialize_records(self, records, width, height, ftype, prev=None, target_ids=None):
+        """The same serialization from block records (target ids mapped to
+        reference indexes as make_job assigns them)."""
+        if not hasattr(self, "_ser"):
+            self._ser = self.f("serialize_records", C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                               C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                               C.c_int32)
+        n_refs = 2 if ftype == 2 else 4
+        ids = target_ids if target_ids is not None else (
+            list(range(1, n_refs + 1)) if ftype == 1 else [0, 1])
+        tid = np.array(ids, np.uint8)
+        rec = np.ascontiguousarray(records)
+        pv = None if prev is None else np.ascontiguousarray(prev)
+        out = np.zeros(1 << 22, np.uint8)
+        n = self._ser(rec.ctypes.data, width // 16, height // 16,
+                      None if pv is None else pv.ctypes.data, int(ftype == 2), tid.ctypes.data,
+                      len(ids), out.ctypes.data, out.size)
+        if n < 0:
+            raise RuntimeError("serialize_records failed")
+        return out[:n].tobytes()
+
+    def serialize_sao(self, table, width, height):
+        """serialize_sao_table (serialize.cpp:564-622) of an SAO table."""
+        if not hasattr(self, "_sers"):
+            self._sers = self.f("serialize_sao", C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                                C.c_void_p, C.c_int32)
+        t = np.ascontiguousarray(table, SAO_DTYPE)
+        out = np.zeros(1 << 20, np.uint8)
+        n = self._sers(t.ctypes.data, width // 16, height // 16, out.ctypes.data, out.size)
+        if n < 0:
+            raise RuntimeError("serialize_sao failed")
+        return out[:n].tobytes()
+
     def analyze_frame_wpp(self, ftype, quality, src, refs, prev=None, target_ids=None,
                           threads=1):
         """analyze_frame on `threads` threads in the reference's wavefront row

@@ -39,6 +39,16 @@ void deblock_filter_values(int16 p3, int16 p2, int16 p1, int16 p0, int16 q0, int
                            int16 q3, int16 *out_p2, int16 *out_p1, int16 *out_p0, int16 *out_q0,
                            int16 *out_q1, int16 *out_q2, uint8 average_qp, uint8 strength,
                            bool is_luma);
+// serialize.cpp defines these at namespace scope without a header.
+evx_status serialize_block_table_unified(uint32 block_count, uint32 width_in_blocks,
+                                         evx_block_desc *block_table, bit_stream *feed_stream,
+                                         entropy_coder *coder, evx_metadata_contexts *meta,
+                                         const evx_block_desc *prev_block_table,
+                                         bool is_bidir_frame, bit_stream *output);
+evx_status serialize_sao_table(const evx_sao_info *sao_table, uint32 block_count,
+                               uint32 width_in_blocks, bit_stream *feed_stream,
+                               entropy_coder *coder, evx_metadata_contexts *meta,
+                               bit_stream *output);
 }  // namespace evx
 
 using namespace evx;
@@ -646,6 +656,60 @@ void copy_back(const image_set &set, int32_t width, int32_t height, int16_t *y, 
     memcpy(v, set.query_v_image()->query_data(), (size_t)(width / 2) * (height / 2) * sizeof(int16_t));
 }
 
+// The arithmetic-coded metadata of a frame as serialize_slice writes it
+// first (serialize.cpp:624-660): the unified block table, or the SAO table;
+// the coder is finished and the stream padded to a byte.  Returns the byte
+// count, or -1 when `cap` is too small.
+template <class F>
+int32_t coded_bytes(F &&write, uint8_t *out, int32_t cap)
+{
+    bit_stream feed(1 << 16), output((uint32)cap * 8 + 64);
+    entropy_coder coder;
+    coder.clear();
+    evx_metadata_contexts meta;
+    meta.clear();
+    write(&feed, &coder, &meta, &output);
+    coder.finish_encode(&output);
+    while (output.query_write_index() & 7) output.write_bit(0);
+    const int32_t n = (int32_t)(output.query_write_index() / 8);
+    if (n > cap) return -1;
+    memcpy(out, output.query_data(), (size_t)n);
+    return n;
+}
+
+// Block descriptors from records: every field the wire format reads; the
+// record's target ids become reference indexes through target_ids.
+void records_to_descs(const evo_record *r, size_t n, const uint8_t *target_ids, int32_t n_ids,
+                      std::vector<evx_block_desc> *out)
+{
+    auto index_of = [&](uint8_t id) -> uint8 {
+        for (int32_t k = 0; k < n_ids; k++)
+            if (target_ids[k] == id) return (uint8)k;
+        return id;
+    };
+    out->resize(n);
+    for (size_t i = 0; i < n; i++) {
+        evx_block_desc &d = (*out)[i];
+        clear_block_desc(&d);
+        d.block_type = (EVX_BLOCK_TYPE)r[i].block_type;
+        d.prediction_mode = r[i].prediction_mode;
+        d.prediction_target = index_of(r[i].prediction_target);
+        d.prediction_target_b = index_of(r[i].prediction_target_b);
+        d.motion_x = r[i].motion_x;
+        d.motion_y = r[i].motion_y;
+        d.motion_x_b = r[i].motion_x_b;
+        d.motion_y_b = r[i].motion_y_b;
+        d.sp_pred = r[i].sp_pred;
+        d.sp_amount = r[i].sp_amount;
+        d.sp_index = r[i].sp_index;
+        d.sp_pred_b = r[i].sp_pred_b;
+        d.sp_amount_b = r[i].sp_amount_b;
+        d.sp_index_b = r[i].sp_index_b;
+        d.q_index = r[i].q_index;
+        d.variance = r[i].variance;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -692,6 +756,66 @@ void evr_sao_encode_frame(const int16_t *src_y, const int16_t *src_u, const int1
         from_ref(t[i].chroma, &table[i].chroma);
     }
     copy_back(rec, width, height, rec_y, rec_u, rec_v);
+}
+
+
+// The block table the reference's own analysis of `job` leaves
+// (evx_block_desc, never converted), serialized as serialize_slice does.
+int32_t evr_analyze_frame_serialized(const evo_frame_job *job, uint8_t *out, int32_t cap)
+{
+    const size_t n = (size_t)(job->src.width / 16) * (job->src.height / 16);
+    std::vector<evo_record> rec(n);
+    std::vector<int16_t> co(n * 384), lv(n * 384);
+    frame_state fs(job, rec.data(), co.data(), lv.data());
+    bank_holder bh;
+    for (int32 k = 0; k < job->n_refs; k++)
+        bh.bank.prediction_cache[k].borrow(fs.refs.prediction_cache[k]);
+    for (uint32 by = 0; by < fs.hb; by++)
+        for (uint32 bx = 0; bx < fs.wb; bx++) {
+            const int32_t rc = analyze_mb(fs, &bh.bank, bx, by);
+            if (rc) return -rc - 1000;
+        }
+    return coded_bytes([&](bit_stream *feed, entropy_coder *coder, evx_metadata_contexts *meta,
+                           bit_stream *output) {
+        serialize_block_table_unified((uint32)n, fs.wb, fs.table.data(), feed, coder, meta,
+                                      fs.prev_ptr, job->type == 2, output);
+    }, out, cap);
+}
+
+// The same serialization from records (the block type and motion of
+// `prev`, when given, are the temporal MV predictor's table).
+int32_t evr_serialize_records(const evo_record *records, int32_t wb, int32_t hb,
+                              const evo_record *prev, int32_t is_bidir, const uint8_t *target_ids,
+                              int32_t n_ids, uint8_t *out, int32_t cap)
+{
+    const size_t n = (size_t)wb * hb;
+    std::vector<evx_block_desc> t, p;
+    records_to_descs(records, n, target_ids, n_ids, &t);
+    if (prev) records_to_descs(prev, n, target_ids, n_ids, &p);
+    return coded_bytes([&](bit_stream *feed, entropy_coder *coder, evx_metadata_contexts *meta,
+                           bit_stream *output) {
+        serialize_block_table_unified((uint32)n, (uint32)wb, t.data(), feed, coder, meta,
+                                      prev ? p.data() : nullptr, is_bidir != 0, output);
+    }, out, cap);
+}
+
+// serialize_sao_table (serialize.cpp:564-622) of a table.
+int32_t evr_serialize_sao(const evo_sao_info *table, int32_t wb, int32_t hb, uint8_t *out,
+                          int32_t cap)
+{
+    const size_t n = (size_t)wb * hb;
+    std::vector<evx_sao_info> t(n);
+    for (size_t i = 0; i < n; i++) {
+        clear_sao_info(&t[i]);
+        t[i].merge_left = table[i].merge_left != 0;
+        t[i].merge_above = table[i].merge_above != 0;
+        to_ref(table[i].luma, &t[i].luma);
+        to_ref(table[i].chroma, &t[i].chroma);
+    }
+    return coded_bytes([&](bit_stream *feed, entropy_coder *coder, evx_metadata_contexts *meta,
+                           bit_stream *output) {
+        serialize_sao_table(t.data(), (uint32)n, (uint32)wb, feed, coder, meta, output);
+    }, out, cap);
 }
 
 }  // extern "C"
