@@ -157,3 +157,17 @@ def test_deblock_refusals(ctx):
     with pytest.raises(evxa.EvxaError):
         s.deblock(2, synth.random_block_records(1, 64, 48))   # no such slot
     s.close()
+
+
+@pytest.mark.parametrize("w,h,seed", [(16, 16, 7), (16, 64, 8), (144, 16, 9), (4096, 2176, 10)])
+def test_deblock_edge_sizes(ctx, oracle, w, h, seed):
+    """One-MB, one-column and one-row frames (no interior edges in one
+    direction; the chroma planes are a single 8x8 block) and a 4096-wide
+    frame (full tiles only)."""
+    pic = synth.blocky_picture(seed, w, h, amp=12)
+    rec = synth.random_block_records(seed + 60, w, h)
+    s = ctx.open_stream(w, h, 1)
+    s.upload(0, *pic)
+    s.deblock(0, rec)
+    _check_planes(s.download(0), oracle.deblock_frame(rec, *pic), f"{w}x{h}")
+    s.close()

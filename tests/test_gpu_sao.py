@@ -140,3 +140,34 @@ def test_sao_refusals(ctx):
     with pytest.raises(evxa.EvxaError):
         s.sao(2, 0, 8)                         # no such slot
     s.close()
+
+
+@pytest.mark.parametrize("w,h,seed", [(16, 16, 5), (16, 64, 6), (144, 16, 7)])
+def test_sao_edge_sizes(ctx, oracle, w, h, seed):
+    """One-MB, one-column and one-row frames: no left or no upper
+    neighbours, edge classes at every border."""
+    rng = np.random.default_rng(seed)
+    src, rec = _content(rng, w, h)
+    s = ctx.open_stream(w, h, 2)
+    s.upload(0, *src)
+    s.upload(1, *rec)
+    table = s.sao(1, 0, 12)
+    want_t, *want = oracle.sao_encode_frame(src, rec, oracle.lam(12))
+    _check_table(table, want_t, f"{w}x{h}")
+    _check_planes(s.download(1), want, f"{w}x{h}")
+    s.close()
+
+
+def test_sao_filter_extreme_samples(ctx, oracle):
+    """Samples across the int16 range: band indices of clamped values,
+    categories of large differences, results clamped to [0, 255]."""
+    rng = np.random.default_rng(13)
+    w, h = 96, 64
+    pic = tuple(rng.integers(-32768, 32768, s).astype(np.int16)
+                for s in ((h, w), (h // 2, w // 2), (h // 2, w // 2)))
+    table = _random_table(rng, (w // 16) * (h // 16))
+    s = ctx.open_stream(w, h, 1)
+    s.upload(0, *pic)
+    s.sao_filter(0, table)
+    _check_planes(s.download(0), oracle.sao_filter_frame(table, *pic), "extreme")
+    s.close()
