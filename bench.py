@@ -691,6 +691,79 @@ def run_recon(args):
     return res
 
 
+def run_loop(args):
+    """Closed-loop P coding on the device: each 1920x1080 frame is analysed
+    against the reconstructions of up to 4 previous frames (temporal MV
+    predictor from the previous frame), then reconstructed from its own
+    levels, deblocked and SAO'd against its source; that picture is the
+    next frames' reference, so the decoded picture buffer stays in HBM.
+    Sources are resident before the timed region; each frame ends with the
+    SAO call's synchronisation.  The first two frames are checked against
+    the oracle's chain."""
+    from chicago_amd import evxa, synth
+
+    w, h = 1920, 1080
+    aw, ah = synth.align16(w), synth.align16(h)
+    k = args.loop_frames
+    frames = synth.sequence_c1(synth.seed_of(1, 0, 0), k + 1, w, h)
+    ctx = evxa.Context(0)
+    ring = 5                                  # 4 references + the frame being coded
+    s = ctx.open_stream(aw, ah, ring + k + 1)
+    src_slot = lambda t: ring + t
+    rec_slot = lambda t: t % ring
+    for t, f in enumerate(frames):
+        s.upload(src_slot(t), *f)
+    s.upload(rec_slot(0), *frames[0])        # frame 0 stands in for a coded intra frame
+
+    def code(t):
+        refs = [rec_slot(t - 1 - j) for j in range(min(4, t))]
+        s.analyze_p(args.quality, src_slot(t), refs, temporal_mvp=t > 1)
+        s.reconstruct_deblocked(rec_slot(t))
+        s.sao(rec_slot(t), src_slot(t), args.quality, table=False)
+
+    res = {"workload": f"closed loop: {k} 1920x1080 P frames, up to 4 reconstructed references, "
+                       f"quality {args.quality}; analysis, reconstruction, deblocking, SAO per frame"}
+    checked = []
+    if not args.no_cpu:
+        from oracle.oracle import Oracle, Picture
+        o = Oracle()
+        recons = [Picture(*frames[0])]
+        prev = None
+        for t in (1, 2):
+            code(t)
+            rec, _, lv = s.results()
+            refs = [recons[t - 1 - j] for j in range(min(4, t))]
+            orec, _, olv = o.analyze_frame(1, args.quality, Picture(*frames[t]), refs,
+                                           prev=prev if t > 1 else None)
+            same = all(np.array_equal(rec[f], orec[f]) for f in rec.dtype.names
+                       if not f.startswith("reserved")) and np.array_equal(lv, olv)
+            filt = o.deblock_frame(orec, *o.reconstruct_frame(1, args.quality, Picture(*frames[t]),
+                                                              refs, orec, olv))
+            _, *want = o.sao_encode_frame(frames[t], filt, o.lam(args.quality))
+            got = s.download(rec_slot(t))
+            checked.append(bool(same and all(np.array_equal(a, b) for a, b in zip(got, want))))
+            recons.append(Picture(*want))
+            prev = orec
+        res["bit_exact_vs_cpu"] = all(checked)
+    start = 3 if checked else 1
+    for t in range(start, min(start + 2, k + 1)):
+        code(t)                               # warm-up frames
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    for t in range(min(start + 2, k + 1), k + 1):
+        code(t)
+        n += 1
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    res["frames_timed"] = n
+    res["ms_per_frame"] = round(dt / max(n, 1) * 1e3, 3)
+    res["frames_per_s"] = round(n / dt, 2) if n else None
+    s.close()
+    ctx.close()
+    return res
+
+
 def calibrate_serving(ctx, args):
     """Sustained MB/s of the dispatcher on a backlog of mix frames (every
     stream of a 64-stream mix queued for several frames)."""
@@ -781,9 +854,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--quality", type=int, default=8)
-    ap.add_argument("--workload", choices=["plain", "batched", "gop", "serve", "recon", "all",
-                                           "traffic"],
+    ap.add_argument("--workload", choices=["plain", "batched", "gop", "serve", "recon", "loop",
+                                           "all", "traffic"],
                     default="all")
+    ap.add_argument("--loop-frames", type=int, default=24)
     ap.add_argument("--gop-frames", type=int, default=33)
     ap.add_argument("--serve-streams", type=int, default=200)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -830,6 +904,8 @@ def main():
         res["gop_b"] = run_gop(args)
     if args.workload in ("all", "recon"):
         res["reconstruct"] = run_recon(args)
+    if args.workload in ("all", "loop"):
+        res["closed_loop"] = run_loop(args)
     if args.workload in ("all", "serve"):
         res["serve"] = run_serve(args)
     print(json.dumps(res), flush=True)
