@@ -215,7 +215,8 @@ struct evxa_stream {
     DevBuf tables;         // 3 record tables: P ping, P pong, B
     DevBuf coeffs, levels;
     DevBuf searches;       // EVXA_MAX_REFS SearchRec per MB
-    DevBuf wide;           // per slot: nonzero if a sample lies outside [0, 2047]
+    DevBuf wide;           // per slot: the sample range words the analysis reads (2 words)
+    DevBuf spread;         // per slot: the spread range record its producers grow
     int32_t p_cur = 0;
     bool temporal_valid = false;
     std::atomic<int32_t> inflight{0};   // frames queued on a server and not yet complete
@@ -230,6 +231,11 @@ struct evxa_stream {
     DevBuf sao_table;      // evxa_sao_info per MB
     uint64_t bytes = 0;
 
+    int32_t *range_words(int32_t slot) const { return wide.as<int32_t>() + 2 * slot; }
+    int32_t *range_spread(int32_t slot) const
+    {
+        return spread.as<int32_t>() + (size_t)kRangeWords * slot;
+    }
     int16_t *slot_base(int32_t slot) const
     {
         return reinterpret_cast<int16_t *>(static_cast<char *>(pics.p) + (size_t)slot * slot_stride);
@@ -248,7 +254,7 @@ struct evxa_stream {
         p.v = ub + (size_t)gc.pitch * gc.rows + gc.origin();
         p.py = gy.pitch;
         p.pc = gc.pitch;
-        p.wide = wide.as<int32_t>() + slot;
+        p.wide = wide.as<int32_t>() + 2 * slot;
         return p;
     }
 };
@@ -260,6 +266,20 @@ void set_device(const evxa_context *ctx) { check(hipSetDevice(ctx->device), "hip
 void check_slot(const evxa_stream *s, int32_t slot)
 {
     if (slot < 0 || slot >= s->n_slots) fail(EVXA_ERROR_INVALIDARG, "slot out of range");
+}
+
+// A slot's range record: cleared before the kernels that write the picture
+// (they grow its spread record), folded into the two words the analysis
+// reads after them.
+void range_begin(evxa_stream *s, int32_t slot)
+{
+    check(launch_reset(s->range_spread(slot), kRangeWords, s->ctx->stream), "reset range record");
+}
+
+void range_end(evxa_stream *s, int32_t slot)
+{
+    check(launch_fold_range(s->range_spread(slot), s->range_words(slot), s->ctx->stream),
+          "fold range record");
 }
 
 // Allocate a buffer a stream creates on first use and charge it to the
@@ -556,7 +576,8 @@ float evxa_lambda(int32_t quality) { return lambda_of(quality); }
 uint64_t evxa_stream_bytes(int32_t width, int32_t height, int32_t n_slots)
 {
     if (width <= 0 || height <= 0 || n_slots <= 0) return 0;
-    return (uint64_t)n_slots * (align_up(picture_bytes(width, height), 4096) + sizeof(int32_t)) +
+    return (uint64_t)n_slots * (align_up(picture_bytes(width, height), 4096) +
+                                (2 + kRangeWords) * sizeof(int32_t)) +
            outputs_bytes(width, height);
 }
 
@@ -665,8 +686,12 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
         s->searches.alloc(n * EVXA_MAX_REFS * sizeof(SearchRec));
         check(hipMemsetAsync(s->searches.p, 0, n * EVXA_MAX_REFS * sizeof(SearchRec), ctx->stream),
               "clear search words");
-        s->wide.alloc((size_t)n_slots * sizeof(int32_t));
-        check(hipMemsetAsync(s->wide.p, 0, (size_t)n_slots * sizeof(int32_t), ctx->stream),
+        s->spread.alloc((size_t)n_slots * kRangeWords * sizeof(int32_t));
+        check(hipMemsetAsync(s->spread.p, 0, (size_t)n_slots * kRangeWords * sizeof(int32_t),
+                             ctx->stream),
+              "clear range records");
+        s->wide.alloc((size_t)n_slots * 2 * sizeof(int32_t));
+        check(hipMemsetAsync(s->wide.p, 0, (size_t)n_slots * 2 * sizeof(int32_t), ctx->stream),
               "clear range flags");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         s->bytes = need;
@@ -719,8 +744,8 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
         int16_t *dst[3] = {py, pu, pv};
         const int16_t *src[3] = {y, u, v};
         const int32_t strides[3] = {stride_y, stride_c, stride_c};
-        int32_t *wide = s->wide.as<int32_t>() + slot;
-        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
+        int32_t *wide = s->range_spread(slot);
+        range_begin(s, slot);
         for (int k = 0; k < 3; k++) {
             check(hipMemcpy2DAsync(dst[k] + g[k]->origin(), (size_t)g[k]->pitch * 2, src[k],
                                    (size_t)strides[k] * 2, (size_t)g[k]->w * 2, g[k]->h,
@@ -730,6 +755,7 @@ int evxa_upload_picture(evxa_stream *s, int32_t slot, const int16_t *y, const in
                                ctx->stream),
                   "range kernel");
         }
+        range_end(s, slot);
         check(launch_apron3(py + s->gy.origin(), pu + s->gc.origin(), pv + s->gc.origin(),
                             s->gy.pitch, s->gc.pitch, s->w, s->h, ctx->stream),
               "apron kernel");
@@ -924,10 +950,11 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
         lv = s->recon_levels.as<int16_t>();
     }
     const DevPic dst = s->pic(dst_slot);
-    int32_t *wide = s->wide.as<int32_t>() + dst_slot;
+    int32_t *wide = s->range_spread(dst_slot);
+    range_begin(s, dst_slot);
     if (!filter) {
-        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
         timed(ctx, [&] { check(launch_recon(j, lv, dst, wide, ctx->stream), "reconstruction kernel"); });
+        range_end(s, dst_slot);
         // The slot is a picture like an uploaded one: refresh its apron (the
         // range flag came with the reconstruction).
         check(launch_apron3(const_cast<int16_t *>(dst.y), const_cast<int16_t *>(dst.u),
@@ -936,7 +963,7 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
     } else {
         int16_t *src[3];
         deblock_input(s, src);
-        lazy_alloc(s, s->recon_flag, sizeof(int32_t));
+        lazy_alloc(s, s->recon_flag, kRangeWords * sizeof(int32_t));
         DevPic mid{};
         mid.y = src[0];
         mid.u = src[1];
@@ -946,7 +973,6 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
         const int32_t sp[3] = {s->w, s->w / 2, s->w / 2}, dp[3] = {dst.py, dst.pc, dst.pc};
         int16_t *const out[3] = {const_cast<int16_t *>(dst.y), const_cast<int16_t *>(dst.u),
                                  const_cast<int16_t *>(dst.v)};
-        check(launch_reset(wide, 1, ctx->stream), "reset range flag");
         timed(ctx, [&] {
             check(launch_recon(j, lv, mid, s->recon_flag.as<int32_t>(), ctx->stream),
                   "reconstruction kernel");
@@ -955,6 +981,7 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
             check(launch_deblock(src, sp, out, dp, s->w, s->h, j.records, wide, ctx->stream),
                   "deblock kernel");
         });
+        range_end(s, dst_slot);
     }
     // The caller's level buffer has been consumed once this returns.
     if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -1001,13 +1028,15 @@ int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
         int16_t *const dst[3] = {const_cast<int16_t *>(p.y), const_cast<int16_t *>(p.u),
                                  const_cast<int16_t *>(p.v)};
         const int32_t sp[3] = {s->w, s->w / 2, s->w / 2}, dp[3] = {p.py, p.pc, p.pc};
-        int32_t *wide = s->wide.as<int32_t>() + slot;
+        int32_t *wide = s->range_spread(slot);
         check(launch_copy3(p.y, p.u, p.v, p.py, p.pc, src[0], s->w, s->h, wide, ctx->stream),
               "copy kernel");
+        range_begin(s, slot);
         timed(ctx, [&] {
             check(launch_deblock(src, sp, dst, dp, s->w, s->h, rec, wide, ctx->stream),
                   "deblock kernel");
         });
+        range_end(s, slot);
         // The caller's records have been consumed once this returns.
         if (records) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
@@ -1045,11 +1074,13 @@ void sao_apply_slot(evxa_stream *s, int32_t slot, int16_t *const rec[3])
                              const_cast<int16_t *>(p.v)};
     const int32_t dp[3] = {p.py, p.pc, p.pc};
     int32_t *counters = sao_counters(s);
+    range_begin(s, slot);
     timed(ctx, [&] {
         check(launch_sao_apply(rec, dst, dp, s->w, s->h, s->sao_table.as<evxa_sao_info>(),
-                               s->wide.as<int32_t>() + slot, counters + 1, ctx->stream),
+                               s->range_spread(slot), counters + 1, ctx->stream),
               "SAO apply kernel");
     });
+    range_end(s, slot);
     check(launch_apron3(dst[0], dst[1], dst[2], p.py, p.pc, s->w, s->h, ctx->stream), "apron kernel");
     int32_t err = 0;
     check(hipMemcpyAsync(&err, counters + 1, sizeof(err), hipMemcpyDeviceToHost, ctx->stream),
@@ -1081,7 +1112,7 @@ int evxa_sao(evxa_stream *s, int32_t slot, int32_t src_slot, int32_t quality, ev
         const DevPic p = s->pic(slot);
         // The decisions and the filter read the unfiltered reconstruction.
         check(launch_copy3(p.y, p.u, p.v, p.py, p.pc, rec[0], s->w, s->h,
-                           s->wide.as<int32_t>() + slot, ctx->stream),
+                           s->range_spread(slot), ctx->stream),
               "copy kernel");
         check(launch_reset(reinterpret_cast<int32_t *>(s->sao_words.p), (int32_t)(2 * n) + kSaoCounters,
                            ctx->stream),
@@ -1119,7 +1150,7 @@ int evxa_sao_filter(evxa_stream *s, int32_t slot, const evxa_sao_info *table)
         check(launch_reset(sao_counters(s), kSaoCounters, ctx->stream), "reset SAO flags");
         const DevPic p = s->pic(slot);
         check(launch_copy3(p.y, p.u, p.v, p.py, p.pc, rec[0], s->w, s->h,
-                           s->wide.as<int32_t>() + slot, ctx->stream),
+                           s->range_spread(slot), ctx->stream),
               "copy kernel");
         sao_apply_slot(s, slot, rec);
     });

@@ -15,14 +15,22 @@ namespace evxa {
 // (0, 0); pitches are in elements and keep rows 128-byte aligned.
 constexpr int kApron = 24;
 
+// Spread range records (see note_range): kRangeWays word pairs, kRangeStride
+// words apart.
+constexpr int kRangeWays = 64;
+constexpr int kRangeStride = 16;
+constexpr int kRangeWords = kRangeWays * kRangeStride;
+
 struct DevPic {
     const int16_t *y;
     const int16_t *u;
     const int16_t *v;
     int32_t py;
     int32_t pc;
-    // Nonzero when some sample of the picture lies outside [0, 2047] (set at
-    // upload); pictures without it take the packed 16-bit metric paths.
+    // The picture's sample range record, two words: 32767 - min and
+    // max + 32768 (zero when cleared; written at upload, reconstruction and
+    // filtering).  A source / reference pair spanning at most 2048 values
+    // within [-8192, 8191] takes the packed 16-bit metric paths.
     const int32_t *wide;
 };
 
@@ -95,26 +103,28 @@ hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st);
 hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
                           int32_t grid, int32_t waves, hipStream_t st);
 // Reconstruction of the MBs of `job` (its records and prediction sources)
-// from `levels` into the picture `dst`; ORs 1 into *wide if a stored sample
-// lies outside [0, 2047] (clear it first).
+// from `levels` into the picture `dst`; grows the spread range record *wide
+// (clear it first) by the stored samples.
 hipError_t launch_recon(const DevJob &job, const int16_t *levels, const DevPic &dst,
                         int32_t *wide, hipStream_t st);
 // The aprons of a picture's three planes (interior origins, luma w x h).
 hipError_t launch_apron3(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int32_t pc, int32_t w,
                          int32_t h, hipStream_t st);
-// ORs 1 into *wide if a sample of the w x h plane lies outside [0, 2047].
+// Folds a spread range record into a picture's two range words.
+hipError_t launch_fold_range(const int32_t *spread, int32_t *range, hipStream_t st);
+// Grows the spread range record *wide by the samples of the w x h plane.
 hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,
                         hipStream_t st);
 // The in-loop deblocking filter of a frame (luma w x h, MB descriptors in
 // `records`) from the unfiltered planes src (pitches sp) into the picture
 // planes dst (pitches dp, distinct memory, with their kApron aprons, which
-// it refreshes); ORs 1 into *wide if a stored sample lies outside [0, 2047]
-// (clear it first).  w and h are multiples of 16.
+// it refreshes); grows the spread range record *wide (clear it
+// first) by the stored samples.  w and h are multiples of 16.
 hipError_t launch_deblock(int16_t *const src[3], const int32_t sp[3], int16_t *const dst[3],
                           const int32_t dp[3], int32_t w, int32_t h,
                           const evxa_block_record *records, int32_t *wide, hipStream_t st);
 // Copies a picture's three plane interiors (luma w x h) to packed planes at
-// `out` and clears *wide.
+// `out` (*wide is not used).
 hipError_t launch_copy3(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py, int32_t pc,
                         int16_t *out, int32_t w, int32_t h, int32_t *wide, hipStream_t st);
 // SAO decisions of a frame (sao.cpp:378-590): the unfiltered reconstruction
@@ -127,8 +137,8 @@ hipError_t launch_sao_decide(const int16_t *rec_y, const int16_t *rec_u, const D
                              uint64_t *words, evxa_sao_info *table, int32_t *counters,
                              hipStream_t st);
 // SAO of a frame per `table` from the unfiltered packed planes `rec` into
-// the picture planes `dst` (pitches dp); ORs 1 into *wide if a stored sample
-// lies outside [0, 2047] (clear it first) and error flags into *error.
+// the picture planes `dst` (pitches dp); grows the spread range record *wide
+// (clear it first) by the stored samples and error flags into *error.
 hipError_t launch_sao_apply(int16_t *const rec[3], int16_t *const dst[3], const int32_t dp[3],
                             int32_t w, int32_t h, const evxa_sao_info *table, int32_t *wide,
                             int32_t *error, hipStream_t st);

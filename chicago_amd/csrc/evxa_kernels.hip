@@ -123,6 +123,54 @@ __device__ __forceinline__ int lane_value(int v, int lane)
     return __builtin_amdgcn_readlane(v, lane);
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+    v = min(v, dpp<0xB1>(v));
+    v = min(v, dpp<0x4E>(v));
+    v = min(v, dpp<0x141>(v));
+    v = min(v, dpp<0x140>(v));
+    return min(min(lane_value(v, 0), lane_value(v, 16)), min(lane_value(v, 32), lane_value(v, 48)));
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    v = row_max(v);
+    return max(max(lane_value(v, 0), lane_value(v, 16)), max(lane_value(v, 32), lane_value(v, 48)));
+}
+
+// A picture's sample range is gathered in a spread record: kRangeWays pairs
+// of words, kRangeStride words apart (one cache line each), cleared to zero;
+// pair k holds 32767 - min and max + 32768 over the samples of the waves
+// whose workgroup index is k modulo kRangeWays, grown with atomicMax.  A
+// wave first reads its pair and updates a word only when it would grow it
+// (the words only grow, so a stale read costs an extra atomic but loses no
+// update).  fold_range_kernel then folds the pairs into the picture's two
+// words the analysis reads.  Called by every lane of a wave; lo / hi are
+// the lane's extremes (INT_MAX / INT_MIN when it stored nothing).
+__device__ __forceinline__ void note_range(int32_t *spread, int lo, int hi)
+{
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (lane_id() == 0 && hi >= lo) {
+        int32_t *pair = spread + ((blockIdx.x + blockIdx.y * gridDim.x) % kRangeWays) * kRangeStride;
+        const int e0 = 32767 - lo, e1 = hi + 32768;
+        if (e0 > __hip_atomic_load(&pair[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&pair[0], e0);
+        if (e1 > __hip_atomic_load(&pair[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&pair[1], e1);
+    }
+}
+
+// Whether a source / reference pair takes the packed int16 metric paths:
+// their samples together span at most 2048 values within [-8192, 8191], so
+// every difference is at most 2047 in magnitude and every half-pel sample
+// and sum those paths form fits in int16.
+__device__ __forceinline__ bool narrow_pair(const int32_t *a, const int32_t *b)
+{
+    const int lo = 32767 - max(a[0], b[0]), hi = max(a[1], b[1]) - 32768;
+    return hi < lo || (hi - lo <= 2047 && lo >= -8192 && hi <= 8191);
+}
+
 __device__ __forceinline__ int wave_sum(int v)
 {
     v = row_sum(v);
@@ -1638,7 +1686,7 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
         load_cand(job.src, geo, x, y, s);
         load_cand(rp, geo, x, y, pre);
         const WinPos win = stage_window<NW>(rp, x, y, kReachMax, sh.sp);
-        const bool narrow = __builtin_amdgcn_readfirstlane((*job.src.wide | *rp.wide) == 0) != 0;
+        const bool narrow = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, rp.wide)) != 0;
         // The temporal table's record of this MB (written by an earlier launch).
         uint32_t p0 = 0, p1 = 0;
         if (job.prev) {
@@ -1811,6 +1859,7 @@ __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
     }
     // Horizontal pass plus the prediction (or the prediction alone): lane =
     // (8x8 block, row), eight samples stored as one 16-byte row segment.
+    int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;   // extremes of the stored samples
     if (lane < 48) {
         const int blk = lane >> 3, j = lane & 7;
         int origin, pitch;
@@ -1822,18 +1871,15 @@ __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
             for (int k = 0; k < 8; k++) v[k] = sh.tmp[r + k];
         }
         uint32_t out[4];
-        bool outside = false;             // a stored sample outside [0, 2047]
 #pragma unroll
         for (int i = 0; i < 8; i += 2) {
             int a = sh.pred[r + i], b2 = sh.pred[r + i + 1];
             if (coded) { a += idct_sample(v, i); b2 += idct_sample(v, i + 1); }
             const int sa = (int16_t)a, sb = (int16_t)b2;
-            outside = outside || sa < 0 || sa > 2047 || sb < 0 || sb > 2047;
+            lo = min(lo, min(sa, sb));
+            hi = max(hi, max(sa, sb));
             out[i / 2] = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b2 << 16);
         }
-        // The picture's range flag (range_kernel's rule), one atomic per
-        // wave that stores such a sample.
-        if (__ballot(outside) != 0 && lane == 0) atomicOr(wide, 1);
         // DevPic views are read-only for the analysis; the destination slot
         // is this kernel's output.
         int16_t *row = const_cast<int16_t *>(
@@ -1843,6 +1889,7 @@ __global__ __launch_bounds__(64 * kReconMbs) void recon_kernel(const DevJob job,
         q.x = out[0]; q.y = out[1]; q.z = out[2]; q.w = out[3];
         __builtin_memcpy(row, &q, 16);
     }
+    note_range(wide, lo, hi);             // the picture's range record
 }
 
 __global__ void reset_kernel(int32_t *words, int32_t n)
@@ -1883,17 +1930,30 @@ __global__ void apron3_kernel(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py,
     }
 }
 
-// Flags pictures holding a sample outside [0, 2047] (one atomic per wave
-// that finds one).
+// Folds a spread range record into the picture's two words (one wave).
+__global__ __launch_bounds__(64) void fold_range_kernel(const int32_t *spread, int32_t *range)
+{
+    const int l = lane_id();
+    const int e0 = l < kRangeWays ? spread[l * kRangeStride] : 0;
+    const int e1 = l < kRangeWays ? spread[l * kRangeStride + 1] : 0;
+    const int m0 = wave_max(e0), m1 = wave_max(e1);
+    if (l == 0) {
+        range[0] = m0;
+        range[1] = m1;
+    }
+}
+
+// Grows a picture's spread range record by the samples of one w x h plane.
 __global__ void range_kernel(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h,
                              int32_t *wide)
 {
-    bool out = false;
+    int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
         const int v = plane0[(i / w) * pitch + i % w];
-        out = out || v < 0 || v > 2047;
+        lo = min(lo, v);
+        hi = max(hi, v);
     }
-    if (__ballot(out) != 0 && lane_id() == 0) atomicOr(wide, 1);
+    note_range(wide, lo, hi);
 }
 
 // -------------------------------------------------------------- deblock --
@@ -2146,7 +2206,7 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
     // The tile's samples, 8 per thread and step; rows 8b+3 from stage 1.
     int16_t *dst = a.dst[pl];
     const int dpitch = a.dp[pl];
-    bool outside = false;
+    int vmin = 0x7FFFFFFF, vmax = -0x7FFFFFFF - 1;   // extremes of the stored samples
     for (int i = tid; i < kDbTh * (kDbTw / 8); i += kDbThreads) {
         const int rr = i / (kDbTw / 8), q = i % (kDbTw / 8);
         const int r = R0 + rr, c = C0 + 8 * q;
@@ -2159,12 +2219,12 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int x0 = (int16_t)(d[k] & 0xFFFF), x1 = d[k] >> 16;
-            outside = outside || x0 < 0 || x0 > 2047 || x1 < 0 || x1 > 2047;
+            vmin = min(vmin, min(x0, x1));
+            vmax = max(vmax, max(x0, x1));
         }
         *reinterpret_cast<int4 *>(dst + (size_t)r * dpitch + c) = make_int4(lo.x, lo.y, hi.x, hi.y);
     }
-    // The picture's range flag (range_kernel's rule).
-    if (__ballot(outside) != 0 && lane_id() == 0) atomicOr(wide, 1);
+    note_range(wide, vmin, vmax);         // the picture's range record
 
     // The apron (kApron samples on every side, the clamped sampling of
     // interp.cpp:8-14): tiles on the plane's border replicate their final
@@ -2216,13 +2276,11 @@ __global__ __launch_bounds__(kDbThreads) void deblock_kernel(const DeblockArgs a
 
 // Copies the interiors of a picture's three planes (luma w x h) into packed
 // planes at `out` (luma, then the two chroma planes; 8 samples per thread
-// and step, blockIdx.y = plane) and clears the picture's range flag for the
-// deblocking kernel that writes the planes back.
+// and step, blockIdx.y = plane).
 __global__ void copy3_kernel(const int16_t *y, const int16_t *u, const int16_t *v, int32_t py,
                              int32_t pc, int16_t *out, int32_t w, int32_t h, int32_t *wide)
 {
     const int plane = blockIdx.y;
-    if (plane == 0 && blockIdx.x == 0 && threadIdx.x == 0) *wide = 0;
     const int pw = plane ? w / 2 : w, ph = plane ? h / 2 : h;
     const int16_t *src = plane == 0 ? y : plane == 1 ? u : v;
     const int pitch = plane ? pc : py;
@@ -2357,15 +2415,6 @@ __device__ __forceinline__ int sao_mean(int sum, int cnt)
     if (cnt <= 0) return 0;
     const int avg = (sum + (sum >= 0 ? cnt / 2 : -cnt / 2)) / cnt;
     return min(max(avg, -7), 7);
-}
-
-__device__ __forceinline__ int wave_min(int v)
-{
-    v = min(v, dpp<0xB1>(v));
-    v = min(v, dpp<0x4E>(v));
-    v = min(v, dpp<0x141>(v));
-    v = min(v, dpp<0x140>(v));
-    return min(min(lane_value(v, 0), lane_value(v, 16)), min(lane_value(v, 32), lane_value(v, 48)));
 }
 
 // The corrected value of sample (x, y) (window position r, c) under p, the
@@ -2838,14 +2887,15 @@ __global__ __launch_bounds__(256) void sao_apply_kernel(const SaoApplyArgs a)
 {
     const int pl = blockIdx.y;
     const int pw = pl ? a.w / 2 : a.w, ph = pl ? a.h / 2 : a.h;
-    bool outside = false;
+    int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < pw * ph; i += gridDim.x * blockDim.x) {
         const int x = i % pw, y = i / pw;
         const int v = sao_final<0>(a, pl, x, y);
-        outside = outside || v < 0 || v > 2047;
+        lo = min(lo, v);
+        hi = max(hi, v);
         a.dst[pl][(size_t)y * a.dp[pl] + x] = (int16_t)v;
     }
-    if (__ballot(outside) != 0 && lane_id() == 0) atomicOr(a.wide, 1);
+    note_range(a.wide, lo, hi);
 }
 
 // params: n x {x, y, dir_x, dir_y, quarter}
@@ -2884,7 +2934,7 @@ __global__ __launch_bounds__(64) void search_blocks_kernel(DevPic src, DevPic re
     load_cand(src, geo, p[0], p[1], s);
     load_cand(ref, geo, p[0], p[1], pre);
     const WinPos win = stage_window<1>(ref, p[0], p[1], kReachMax, sp);
-    const bool narrow = (*src.wide | *ref.wide) == 0;
+    const bool narrow = narrow_pair(src.wide, ref.wide);
     window_ready();
     SearchOut o = inter_search<1>(ref, w, h, narrow, s, geo, pre, win, p[0], p[1], (quality >> 2) + 1,
                                lambda, p[2], p[3], sp);
@@ -2955,6 +3005,12 @@ hipError_t launch_apron3(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int3
     const int n = 2 * kApron * (w + 2 * kApron) + 2 * kApron * h;
     const int blocks = min((n + 255) / 256, 512);
     hipLaunchKernelGGL(apron3_kernel, dim3(blocks, 3), dim3(256), 0, st, y0, u0, v0, py, pc, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_range(const int32_t *spread, int32_t *range, hipStream_t st)
+{
+    hipLaunchKernelGGL(fold_range_kernel, dim3(1), dim3(64), 0, st, spread, range);
     return hipGetLastError();
 }
 
