@@ -90,7 +90,7 @@ def test_deblock_after_reconstruct(ctx, oracle, ftype):
 
 def test_deblocked_picture_as_reference(ctx, oracle):
     """A dark blocky picture whose filtered samples go below zero (the
-    strength-1 taps overshoot): after deblocking, its apron and range flag
+    strength-1 taps overshoot): after deblocking, its apron and range record
     are refreshed, so a P frame searched against it matches the oracle's
     search against the filtered planes."""
     w, h = 160, 96
@@ -125,7 +125,7 @@ def test_deblocked_picture_as_reference(ctx, oracle):
 
 
 def test_fused_reconstruction_as_reference(ctx, oracle):
-    """reconstruct_deblocked refreshes the apron and range flag of the slot
+    """reconstruct_deblocked refreshes the apron and range record of the slot
     it writes: frame 2 searched against the filtered reconstruction of
     frame 1 matches the oracle."""
     frames = synth.sequence_c1(83, 3, 128, 96, local_range=10)

@@ -111,7 +111,7 @@ def test_sao_after_reconstruct_deblocked(ctx, oracle):
 
 
 def test_sao_picture_as_reference(ctx, oracle):
-    """After SAO the slot's apron and range flag are refreshed: a P frame
+    """After SAO the slot's apron and range record are refreshed: a P frame
     searched against it matches the oracle's search against the planes."""
     rng = np.random.default_rng(5)
     w, h = 160, 96
