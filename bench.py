@@ -749,6 +749,7 @@ def run_loop(args):
     for t in range(start, min(start + 2, k + 1)):
         code(t)                               # warm-up frames
     ctx.synchronize()
+    ctx.launch_times()
     t0 = time.perf_counter()
     n = 0
     for t in range(min(start + 2, k + 1), k + 1):
@@ -759,6 +760,14 @@ def run_loop(args):
     res["frames_timed"] = n
     res["ms_per_frame"] = round(dt / max(n, 1) * 1e3, 3)
     res["frames_per_s"] = round(n / dt, 2) if n else None
+    # Device time per frame from the events around each kernel group:
+    # analysis, reconstruction, deblocking, SAO decisions, SAO filter.
+    ms = np.array(ctx.launch_times())
+    if n and ms.size == 5 * n:
+        per = ms.reshape(n, 5).mean(axis=0)
+        res["device_ms_per_frame"] = {name: round(float(v), 3) for name, v in
+                                      zip(("analysis", "reconstruct", "deblock", "sao_decide",
+                                           "sao_apply"), per)}
     s.close()
     ctx.close()
     return res

@@ -226,7 +226,7 @@ struct evxa_stream {
     DevBuf recon_levels;   // levels handed in for reconstruction
     DevBuf deblock_src;    // unfiltered copy of the picture being deblocked (packed planes)
     DevBuf deblock_records;  // block descriptors handed in for deblocking
-    DevBuf recon_flag;     // range flag of a reconstruction headed for the filter (unused)
+    DevBuf recon_flag;     // range record of a reconstruction headed for the filter (unused)
     DevBuf sao_words;      // per MB: published SAO decision; then the SAO counters
     DevBuf sao_table;      // evxa_sao_info per MB
     uint64_t bytes = 0;
@@ -692,7 +692,7 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
               "clear range records");
         s->wide.alloc((size_t)n_slots * 2 * sizeof(int32_t));
         check(hipMemsetAsync(s->wide.p, 0, (size_t)n_slots * 2 * sizeof(int32_t), ctx->stream),
-              "clear range flags");
+              "clear range words");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         s->bytes = need;
         ctx->used += need;
@@ -956,7 +956,7 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
         timed(ctx, [&] { check(launch_recon(j, lv, dst, wide, ctx->stream), "reconstruction kernel"); });
         range_end(s, dst_slot);
         // The slot is a picture like an uploaded one: refresh its apron (the
-        // range flag came with the reconstruction).
+        // range record came with the reconstruction).
         check(launch_apron3(const_cast<int16_t *>(dst.y), const_cast<int16_t *>(dst.u),
                             const_cast<int16_t *>(dst.v), dst.py, dst.pc, s->w, s->h, ctx->stream),
               "apron kernel");

@@ -2882,7 +2882,7 @@ __device__ int sao_final(const SaoApplyArgs &a, int pl, int x, int y)
 }
 
 // SAO of a frame per its table, from the unfiltered planes into the picture
-// slot (blockIdx.y = plane), setting the slot's range flag.
+// slot (blockIdx.y = plane), growing the slot's spread range record.
 __global__ __launch_bounds__(256) void sao_apply_kernel(const SaoApplyArgs a)
 {
     const int pl = blockIdx.y;
