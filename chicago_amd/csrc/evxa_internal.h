@@ -27,8 +27,8 @@ struct DevPic {
     const int16_t *v;
     int32_t py;
     int32_t pc;
-    // The picture's sample range record, two words: 32767 - min and
-    // max + 32768 (zero when cleared; written at upload, reconstruction and
+    // The picture's sample range word, max << 16 | (min & 0xFFFF) (zero
+    // when never written; folded after upload, reconstruction and
     // filtering).  A source / reference pair spanning at most 2048 values
     // within [-8192, 8191] takes the packed 16-bit metric paths.
     const int32_t *wide;
@@ -110,7 +110,7 @@ hipError_t launch_recon(const DevJob &job, const int16_t *levels, const DevPic &
 // The aprons of a picture's three planes (interior origins, luma w x h).
 hipError_t launch_apron3(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py, int32_t pc, int32_t w,
                          int32_t h, hipStream_t st);
-// Folds a spread range record into a picture's two range words.
+// Folds a spread range record into a picture's range word.
 hipError_t launch_fold_range(const int32_t *spread, int32_t *range, hipStream_t st);
 // Grows the spread range record *wide by the samples of the w x h plane.
 hipError_t launch_range(const int16_t *plane0, int32_t pitch, int32_t w, int32_t h, int32_t *wide,

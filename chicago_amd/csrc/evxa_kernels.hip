@@ -164,11 +164,15 @@ __device__ __forceinline__ void note_range(int32_t *spread, int lo, int hi)
 // Whether a source / reference pair takes the packed int16 metric paths:
 // their samples together span at most 2048 values within [-8192, 8191], so
 // every difference is at most 2047 in magnitude and every half-pel sample
-// and sum those paths form fits in int16.
+// and sum those paths form fits in int16.  A picture's range word holds its
+// maximum in the high half and its minimum in the low half (zero for a
+// picture never written).
 __device__ __forceinline__ bool narrow_pair(const int32_t *a, const int32_t *b)
 {
-    const int lo = 32767 - max(a[0], b[0]), hi = max(a[1], b[1]) - 32768;
-    return hi < lo || (hi - lo <= 2047 && lo >= -8192 && hi <= 8191);
+    const int32_t wa = *a, wb = *b;
+    const int lo = min((int)(int16_t)(wa & 0xFFFF), (int)(int16_t)(wb & 0xFFFF));
+    const int hi = max(wa >> 16, wb >> 16);
+    return hi - lo <= 2047 && lo >= -8192 && hi <= 8191;
 }
 
 __device__ __forceinline__ int wave_sum(int v)
@@ -1681,12 +1685,13 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
         // Load the source block and the initial candidate, and stage the
         // largest search window around the MB, while the neighbours finish.
         const DevPic &rp = job.refs[ref];
+        // The metric path first: its two loads then overlap the staging.
+        const bool narrow = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, rp.wide)) != 0;
         const Geo geo = lane_geometry();
         Blk s, pre;
         load_cand(job.src, geo, x, y, s);
         load_cand(rp, geo, x, y, pre);
         const WinPos win = stage_window<NW>(rp, x, y, kReachMax, sh.sp);
-        const bool narrow = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, rp.wide)) != 0;
         // The temporal table's record of this MB (written by an earlier launch).
         uint32_t p0 = 0, p1 = 0;
         if (job.prev) {
@@ -1930,17 +1935,15 @@ __global__ void apron3_kernel(int16_t *y0, int16_t *u0, int16_t *v0, int32_t py,
     }
 }
 
-// Folds a spread range record into the picture's two words (one wave).
+// Folds a spread range record into the picture's range word, max << 16 |
+// (min & 0xFFFF), or zero when no sample was recorded (one wave).
 __global__ __launch_bounds__(64) void fold_range_kernel(const int32_t *spread, int32_t *range)
 {
     const int l = lane_id();
     const int e0 = l < kRangeWays ? spread[l * kRangeStride] : 0;
     const int e1 = l < kRangeWays ? spread[l * kRangeStride + 1] : 0;
-    const int m0 = wave_max(e0), m1 = wave_max(e1);
-    if (l == 0) {
-        range[0] = m0;
-        range[1] = m1;
-    }
+    const int lo = 32767 - wave_max(e0), hi = wave_max(e1) - 32768;
+    if (l == 0) range[0] = hi < lo ? 0 : (int32_t)((uint32_t)hi << 16 | ((uint32_t)lo & 0xFFFF));
 }
 
 // Grows a picture's spread range record by the samples of one w x h plane.
