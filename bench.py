@@ -193,12 +193,15 @@ def run_batched(args, ctx):
     mbs = sum(s.n_mbs for s in streams)
     log(f"batched: uploaded, {mbs} MBs per step")
 
-    tuned = None
+    tuned, modes = None, None
     if not args.no_tune:
         from chicago_amd import autotune
-        tuned, _ = autotune.tune(ctx, lambda: ctx.analyze_batch(streams, descs),
-                                 lambda: [s.results()[0] for s in streams], autotune.BATCH,
-                                 reps=2, log=log)
+        tuned, table = autotune.tune(ctx, lambda: ctx.analyze_batch(streams, descs),
+                                     lambda: [s.results()[0] for s in streams], autotune.BATCH,
+                                     reps=2, log=log)
+        modes = {k: {"ms_per_step": v["step_ms"], "tuning": v["tuning"],
+                     "frames_1080p_eq_per_s": round(mbs / 8160 / (v["step_ms"] / 1e3), 2)}
+                 for k, v in autotune.by_mode(table).items()}
 
     # Untimed eager step: outputs and work accounting.
     ctx.analyze_batch(streams, descs)
@@ -254,6 +257,7 @@ def run_batched(args, ctx):
                             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "replays_bit_identical": same,
         "launch_shape": tuned,
+        "modes": modes,
     }
     return res, streams, descs, seqs, outs
 
@@ -299,17 +303,21 @@ def run_plain(args):
             s.reset_temporal()
         s.analyze_p(q, t, frame_refs(t))
 
-    # Launch shape: the autotuner times one 4-reference frame (temporal table
-    # reset, so every shape sees the same inputs) under each candidate.
-    tuned = None
+    # Launch shape: the autotuner times frames 4-6 (4 references; temporal
+    # table reset first, so every shape sees the same inputs, then chained as
+    # in the timed run) under each candidate.
+    tuned, modes = None, None
     if not args.no_tune:
         from chicago_amd import autotune
 
         def tune_step():
             s.reset_temporal()
-            s.analyze_p(q, 5, frame_refs(5))
+            for t in (4, 5, 6):
+                s.analyze_p(q, t, frame_refs(t))
         tuned, table = autotune.tune(ctx, tune_step, lambda: s.results(), autotune.SINGLE,
                                      log=log)
+        modes = {k: {"ms_per_frame": round(v["step_ms"] / 3, 4), "tuning": v["tuning"]}
+                 for k, v in autotune.by_mode(table).items()}
 
     # Untimed pass: work accounting, outputs for the CPU comparison and
     # per-frame latency (enqueue -> device idle).
@@ -342,6 +350,17 @@ def run_plain(args):
     kms = ctx.launch_times()
     assert len(kms) == len(timed), (len(kms), len(timed))
     log(f"timed {len(timed)} frames in {wall:.3f}s")
+    sweep_mode = ctx.tuning()[0]["mode"] == evxa.MODE_SWEEP
+    sweeps = None
+    if sweep_mode:
+        # The sweeps of one frame of the run (frame 6 after 5, temporal table).
+        step(5)
+        step(6)
+        listed, searched, done_us = ctx.sweep_stats(times=True)
+        ctx.launch_times()
+        sweeps = {"frame": 6, "mbs_listed": listed, "mbs_searched": searched,
+                  "done_us": done_us, "searches_per_mb": round(
+                      sum(searched) / ((aw // 16) * (ah // 16)), 3)}
 
     ops_launch = float(np.mean([ops[t] for t in timed]))
     k_avg_s = float(np.mean(kms)) / 1e3
@@ -375,7 +394,7 @@ def run_plain(args):
             "unit": "TOP/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 5),
             "traffic": measured_traffic(),
-            "kernel": "analyze_kernel",
+            "kernel": "sweep_kernel" if sweep_mode else "analyze_kernel",
             "kernel_ms_avg": round(k_avg_s * 1e3, 4),
             "algorithmic_ops_per_launch": int(ops_launch),
             "algorithmic_bytes_per_launch": int(bytes_launch),
@@ -387,6 +406,8 @@ def run_plain(args):
             "what": "enqueue of one frame to device idle, sequential frames",
         },
         "launch_shape": tuned,
+        "modes": modes,
+        "sweeps": sweeps,
     }
     if not args.no_cpu:
         sample = list(range(args.cpu_start, args.cpu_start + args.cpu_frames))
@@ -397,6 +418,7 @@ def run_plain(args):
         result["cpu_baseline"] = cb
         result["bit_exact_vs_cpu"] = exact
         result["gpu_vs_cpu"] = round(result["value"] / cb["value"], 1)
+    args.plain_tuning = ctx.tuning()[0]
     s.close()
     ctx.close()
     return result
@@ -430,12 +452,14 @@ def run_gop(args):
         else:
             s.analyze_b(r.quality, r.index, r.refs[0], r.refs[1], target_ids=[0, 1])
 
-    tuned = None
+    tuned, modes = None, None
     if not args.no_tune:
         from chicago_amd import autotune
         rb = next(r for r in work if r.kind == "B")
-        tuned, _ = autotune.tune(ctx, lambda: step(rb), lambda: s.results(), autotune.SINGLE,
-                                 reps=2, log=log)
+        tuned, table = autotune.tune(ctx, lambda: step(rb), lambda: s.results(), autotune.SINGLE,
+                                     reps=2, log=log)
+        modes = {k: {"ms_per_b_frame": v["step_ms"], "tuning": v["tuning"]}
+                 for k, v in autotune.by_mode(table).items()}
 
     # Untimed pass: per-frame latency, work accounting, outputs of a sample.
     sample = {work[0].index, next(r.index for r in work if r.kind == "B")}
@@ -474,6 +498,7 @@ def run_gop(args):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "launch_shape": tuned,
+        "modes": modes,
     }
     if not args.no_cpu:
         from oracle.oracle import Oracle, Picture, Reference, reference_available
@@ -511,46 +536,58 @@ def run_serve(args):
     dispatcher.  Streams are admitted by HBM budget and by a compute capacity
     measured first: a burst of frames of the mix through the same dispatcher
     gives the sustainable MB/s, and admission keeps the offered MB/s within
-    `--serve-util` of it."""
-    from chicago_amd import evxa, serve
+    `--serve-util` of it.  Run in both modes (MB wavefront, sweeps); the one
+    with the lower p99 latency at the main load is reported first."""
+    from chicago_amd import autotune, evxa, serve
 
-    ctx = evxa.Context(0)
-    # Launch shape for serving: the fewest-latency shape of a single frame
-    # (two waves per search) unless the tuner is off.
-    if not args.no_tune:
-        ctx.set_tuning(waves_per_search=args.serve_waves)
-    cap = None
-    if args.serve_util > 0:
-        full = args.serve_capacity if args.serve_capacity > 0 else calibrate_serving(ctx, args)
-        cap = full * args.serve_util
-        log(f"serve: admission capacity {cap / 1e6:.2f} M MB/s")
-    res = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
-                    duration=args.serve_seconds, quality=args.quality,
-                    max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
-                    capacity_mbps=cap, log=log)
-    # The same schedule at lower admitted loads: latency against throughput.
-    if cap is not None and args.serve_curve:
-        res["curve"] = []
-        for u in (float(x) for x in args.serve_curve.split(",")):
-            r = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
-                          duration=args.serve_seconds, quality=args.quality,
-                          max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
-                          capacity_mbps=full * u, log=log)
-            res["curve"].append({"util": u, "frames_per_s": r["frames_per_s"],
-                                 "frames_1080p_eq_per_s": r["frames_1080p_eq_per_s"],
-                                 "admitted_lives": r["admitted_lives"],
-                                 "latency_ms": r["latency_ms"],
-                                 "latency_ms_by_height": r["latency_ms_by_height"]})
-            log(f"serve curve util {u}: {r['frames_per_s']} frames/s, p99 "
-                f"{r['latency_ms']['p99'] if r['latency_ms'] else None} ms")
-    res["workload"] = (f"serve: {args.serve_streams} live streams offered at 60 fps "
-                       f"(50% 720p / 35% 1080p / 15% 1440p, mean life 20 s, 2 ms jitter), "
-                       f"{args.serve_seconds:g} s schedule, 1 ref, quality {args.quality}, "
-                       f"{args.serve_inflight} launches in flight")
-    res["admission_capacity_mb_per_s"] = cap
-    res["launch_shape"] = ctx.tuning()[0]
-    ctx.close()
-    return res
+    runs = {}
+    for name, shape in (("sweep", autotune._sweep(args.serve_waves, 2)),
+                        ("wavefront", autotune._wavefront(waves_per_search=args.serve_waves))):
+        ctx = evxa.Context(0)
+        if not args.no_tune:
+            ctx.set_tuning(**shape)
+        cap = full = None
+        if args.serve_util > 0:
+            full = args.serve_capacity if args.serve_capacity > 0 else calibrate_serving(ctx, args)
+            cap = full * args.serve_util
+            log(f"serve ({name}): admission capacity {cap / 1e6:.2f} M MB/s")
+        res = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
+                        duration=args.serve_seconds, quality=args.quality,
+                        max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
+                        capacity_mbps=cap, log=log)
+        # The same schedule at lower admitted loads: latency against throughput.
+        if cap is not None and args.serve_curve:
+            res["curve"] = []
+            for u in (float(x) for x in args.serve_curve.split(",")):
+                r = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
+                              duration=args.serve_seconds, quality=args.quality,
+                              max_inflight=args.serve_inflight, max_frames=args.serve_max_frames,
+                              capacity_mbps=full * u, log=log)
+                res["curve"].append({"util": u, "frames_per_s": r["frames_per_s"],
+                                     "frames_1080p_eq_per_s": r["frames_1080p_eq_per_s"],
+                                     "admitted_lives": r["admitted_lives"],
+                                     "latency_ms": r["latency_ms"],
+                                     "latency_ms_by_height": r["latency_ms_by_height"]})
+                log(f"serve ({name}) curve util {u}: {r['frames_per_s']} frames/s, p99 "
+                    f"{r['latency_ms']['p99'] if r['latency_ms'] else None} ms")
+        res["workload"] = (f"serve: {args.serve_streams} live streams offered at 60 fps "
+                           f"(50% 720p / 35% 1080p / 15% 1440p, mean life 20 s, 2 ms jitter), "
+                           f"{args.serve_seconds:g} s schedule, 1 ref, quality {args.quality}, "
+                           f"{args.serve_inflight} launches in flight")
+        res["admission_capacity_mb_per_s"] = cap
+        res["launch_shape"] = ctx.tuning()[0]
+        res["mode"] = name
+        runs[name] = res
+        ctx.close()
+
+    def p99(r):
+        return r["latency_ms"]["p99"] if r.get("latency_ms") else float("inf")
+    best = min(runs.values(), key=p99)
+    other = [r for r in runs.values() if r is not best][0]
+    best["other_mode"] = {k: other.get(k) for k in (
+        "mode", "frames_per_s", "frames_1080p_eq_per_s", "admitted_lives", "latency_ms",
+        "admission_capacity_mb_per_s", "curve")}
+    return best
 
 
 def run_recon(args):
@@ -707,6 +744,8 @@ def run_loop(args):
     k = args.loop_frames
     frames = synth.sequence_c1(synth.seed_of(1, 0, 0), k + 1, w, h)
     ctx = evxa.Context(0)
+    if getattr(args, "plain_tuning", None):
+        ctx.set_tuning(**args.plain_tuning)   # the plain workload's tuned shape
     ring = 5                                  # 4 references + the frame being coded
     s = ctx.open_stream(aw, ah, ring + k + 1)
     src_slot = lambda t: ring + t

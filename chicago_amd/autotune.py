@@ -1,12 +1,21 @@
 """Autotuner of the analysis launch shape.
 
-The knobs (evxa_set_tuning): waves per (MB, reference) search (one wave
-evaluates a level's 8 candidates in two passes, two waves in one pass and
-exchange results through LDS), work lists per launch (row bands spread over
-the XCDs' L2 caches), and the per-list cap of tickets per virtual diagonal
-that staggers the frames of a batch.  Which shape is fastest depends on the
-workload: a single frame is bound by its MB dependency chain and wants the
-lowest per-MB latency, a batch of many frames wants throughput.
+The knobs (evxa_set_tuning):
+  * the mode: the MB wavefront (each MB waits for its neighbours' final
+    MVs) or sweeps to the predictor's fixpoint (no MB waits; MBs whose
+    predictor changed are searched again);
+  * waves per (MB, reference) search (one wave evaluates a level's 8
+    candidates in two passes, two waves in one pass and exchange results
+    through LDS);
+  * wavefront: work lists per launch (row bands spread over the XCDs' L2
+    caches) and the per-list cap of tickets per virtual diagonal that
+    staggers the frames of a batch;
+  * sweeps: sweep 0's order (raster, or anti-diagonal with a bounded wait
+    for the neighbours' MVs) and the tickets claimed at once.
+Which shape is fastest depends on the workload: a single frame is bound by
+its MB dependency chain in the wavefront and wants sweeps, a batch of many
+frames wants throughput, and content whose predictor changes travel far
+(zooms, hierarchical B) needs many sweeps.
 
 `tune` runs a repeatable step of the real workload under each candidate
 shape, times its launches with HIP events, requires every shape's outputs to
@@ -20,11 +29,36 @@ import itertools
 
 import numpy as np
 
+WAVEFRONT, SWEEP = 1, 2
+
+
+def _wavefront(**kw):
+    return dict(mode=WAVEFRONT, **kw)
+
+
+def _sweep(waves, batch, order=1, wait_us=0):
+    return dict(mode=SWEEP, waves_per_search=waves, claim_batch=batch, sweep_order=order,
+                sweep_reads=2, sweep_wait_us=wait_us)
+
+
 # Candidate shapes per workload class.
-SINGLE = [dict(waves_per_search=w, lists=l) for w, l in itertools.product((1, 2), (1, 8))]
-BATCH = [dict(waves_per_search=w, lists=8, cap_pct=c)
-         for w, c in itertools.product((1, 2), (100, 150, 200, 250, 300, 400, 800))]
-SERVE = [dict(waves_per_search=w) for w in (1, 2)]
+SINGLE = ([_wavefront(waves_per_search=w, lists=l) for w, l in itertools.product((1, 2), (1, 8))] +
+          [_sweep(w, b) for w, b in itertools.product((1, 2), (1, 2, 4))])
+BATCH = ([_wavefront(waves_per_search=w, lists=8, cap_pct=c)
+          for w, c in itertools.product((1, 2), (150, 250, 400))] +
+         [_sweep(1, 4), _sweep(1, 1, order=2, wait_us=20), _sweep(1, 1, order=2, wait_us=200),
+          _sweep(1, 2, order=2, wait_us=20)])
+SERVE = [_wavefront(waves_per_search=w) for w in (1, 2)] + [_sweep(w, 2) for w in (1, 2)]
+
+
+def by_mode(table):
+    """Fastest step of each mode in a tune() table."""
+    out = {}
+    for r in table:
+        name = "sweep" if r["tuning"].get("mode") == SWEEP else "wavefront"
+        if name not in out or r["step_ms"] < out[name]["step_ms"]:
+            out[name] = {"step_ms": r["step_ms"], "tuning": r["tuning"]}
+    return out
 
 
 def _same(a, b):

@@ -159,6 +159,8 @@ float lambda_of(int32_t quality)
 // another launch's tables.
 struct LaunchRes {
     DevBuf jobs, tickets, progress, counters;
+    DevBuf state;                    // sweep mode: per-MB words, keys, predictors and decisions
+    DevBuf lists;                    // sweep mode: the dynamic check and search lists (zero between launches)
     PinnedBuf h_jobs, h_tickets;
     hipEvent_t staged = nullptr;     // host staging may be reused after this
     hipEvent_t g_start = nullptr, g_stop = nullptr;   // captured timing (graphs)
@@ -167,6 +169,29 @@ struct LaunchRes {
     int32_t n_progress = 0;
     int32_t grid = 0;                // workgroups to launch for this composition
     int32_t waves = 1;               // waves per workgroup
+    bool sweep = false;              // sweep mode (else the MB wavefront)
+    int32_t n_reset = 0;             // progress words cleared before each launch
+    int32_t cap_check = 0;           // sweep mode: entries per check sub-list (MBs of the launch)
+    int32_t cap_search = 0;          // sweep mode: entries per search sub-list (tickets of the launch)
+    int32_t max_sweeps = 0;          // sweep mode: sweep counter blocks
+    int32_t sctr_off = 0;            // sweep mode: progress word of sweep block 0
+    int32_t reads = kSweepReadsGuess;  // sweep mode: how sweep 0 reads neighbours' words
+    int32_t wait_ticks = 0;          // sweep mode: fresh reads' longest wait (10 ns ticks)
+    int32_t batch = 4;               // sweep mode: tickets per claim
+    SweepArgs sweep_args() const
+    {
+        SweepArgs a;
+        a.check = lists.as<uint64_t>();
+        a.search = a.check + (size_t)2 * kSubLists * cap_check;
+        a.cap_check = cap_check;
+        a.cap_search = cap_search;
+        a.max_sweeps = max_sweeps;
+        a.sctr = progress.as<int32_t>() + sctr_off;
+        a.reads = reads;
+        a.wait_ticks = wait_ticks;
+        a.batch = batch;
+        return a;
+    }
     ~LaunchRes()
     {
         if (staged) (void)hipEventDestroy(staged);
@@ -184,8 +209,13 @@ struct evxa_context {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
-    int32_t resident[3] = {0, 0, 0}; // workgroups of 1- and 2-wave analysis kernels that fit
+    int32_t resident[2][3] = {};     // [sweep mode][waves]: workgroups of the analysis kernels that fit
     int32_t waves = 1;               // waves per search (1 or 2)
+    int32_t mode = EVXA_MODE_WAVEFRONT;  // predictor dependencies: wavefront or sweeps
+    int32_t sweep_reads = kSweepReadsFresh;  // sweep 0: guess table only, or fresh words too
+    int32_t sweep_wait_us = 0;       // sweep 0, fresh reads: longest wait for a neighbour
+    int32_t sweep_order = EVXA_SWEEP_ORDER_RASTER;  // sweep 0's ticket order
+    int32_t claim_batch = 2;         // sweep mode: tickets per claim
     int32_t grid = 0;                // most analysis workgroups per launch (0: all that fit)
     int32_t lists = 0;               // work lists per launch (0: automatic)
     int32_t cap_pct = 60;            // tickets per list and virtual diagonal, % of its workgroups
@@ -327,6 +357,10 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     if (d.type == EVXA_FRAME_P) {
         out = s->p_cur;
         j.prev = (d.temporal_mvp && s->temporal_valid) ? s->table(s->p_cur ^ 1) : nullptr;
+        // Sweep mode starts from the temporal table's motion (the information
+        // the predictor itself uses), never from an earlier result of the
+        // same frame.
+        j.guess = j.prev;
     } else {
         out = 2;
         j.prev = nullptr;
@@ -366,8 +400,15 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
 // have smaller v), and across jobs there are no dependencies, so the item
 // with the smallest unfinished v is always runnable: no list can deadlock.
 // Returns the most tickets any list holds on one virtual diagonal.
+//
+// Sweep mode appends the final pass's lists (one ticket per MB, reference 0,
+// in MB order, each MB in the list of its row) and fills the sweep fields
+// of the header.  Its sweep 0 has no dependencies: `raster` lists each
+// frame's rows in raster order instead (an XCD then works on few pictures
+// at a time), while the anti-diagonal order lets sweep 0 read the MVs its
+// neighbours decided before it.
 int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per_list_cap,
-                   bool stagger, std::vector<uint32_t> &out)
+                   bool stagger, bool sweep, bool raster, std::vector<uint32_t> &out)
 {
     int64_t total = 0;
     for (const auto &j : jobs) total += (int64_t)j.wb * j.hb * j.n_refs;
@@ -432,6 +473,18 @@ int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per
     for (int32_t l = 0; l < n_lists; l++)
         for (int32_t v : load[l]) widest = std::max(widest, v);
     std::vector<std::vector<uint32_t>> lists(n_lists);
+    // Raster order: each list takes its rows frame by frame (the references
+    // of a MB adjacent, the last one designated).
+    if (raster) horizon = 0;
+    for (size_t k = 0; raster && k < jobs.size(); k++) {
+        const DevJob &j = jobs[k];
+        for (int32_t by = 0; by < j.hb; by++)
+            for (int32_t bx = 0; bx < j.wb; bx++)
+                for (int32_t r = 0; r < j.n_refs; r++)
+                    lists[row_list[k][by]].push_back(((uint32_t)k << kJobShift) |
+                                                     ((uint32_t)r << kRefShift) |
+                                                     (uint32_t)(by * j.wb + bx));
+    }
     for (int32_t v = 0; v < horizon; v++)
         for (size_t k = 0; k < jobs.size(); k++) {
             const DevJob &j = jobs[k];
@@ -456,7 +509,51 @@ int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per
         out.insert(out.end(), lists[l].begin(), lists[l].end());
     }
     out[1 + n_lists] = off;
+    if (sweep) {
+        int32_t non_empty = 0, max_diag = 0;
+        for (int32_t l = 0; l < n_lists; l++) non_empty += lists[l].empty() ? 0 : 1;
+        for (const auto &j : jobs) max_diag = std::max(max_diag, diagonals(j));
+        for (int32_t l = 0; l < n_lists; l++) {
+            out[kWorkFinal + l] = off;
+            for (size_t k = 0; k < jobs.size(); k++) {
+                const DevJob &j = jobs[k];
+                for (int32_t by = 0; by < j.hb; by++) {
+                    if (row_list[k][by] != l) continue;
+                    for (int32_t bx = 0; bx < j.wb; bx++)
+                        out.push_back(((uint32_t)k << kJobShift) | (uint32_t)(by * j.wb + bx));
+                    off += (uint32_t)j.wb;
+                }
+            }
+        }
+        out[kWorkFinal + n_lists] = off;
+        out[kWorkNonEmpty] = (uint32_t)non_empty;
+        // A MB at dependency depth d is final after sweep d, and depths stay
+        // below the anti-diagonal count; sweep s + 2's block is addressed
+        // during sweep s.
+        out[kWorkMaxSweeps] = (uint32_t)(max_diag + 3);
+    }
     return widest;
+}
+
+// Progress words (int32, cleared per launch) of a frame of n_mb MBs in
+// sweep mode: the listed stamps (padded to 8 bytes), the predictor words of
+// both parities and the published predictors (64-bit each).
+int32_t sweep_words(int32_t n_mb) { return (int32_t)align_up((size_t)n_mb, 2) + 6 * n_mb; }
+
+// The reset and analysis kernels of a staged launch (also captured into
+// graphs).
+hipError_t launch_staged(LaunchRes &r, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    hipError_t e = launch_reset(r.progress.as<int32_t>(), r.n_reset, st);
+    if (e != hipSuccess) return e;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    e = r.sweep ? launch_sweep(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.progress.as<int32_t>(),
+                               r.sweep_args(), r.grid, r.waves, st)
+                : launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.progress.as<int32_t>(),
+                                 r.grid, r.waves, st);
+    if (e != hipSuccess) return e;
+    if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
+    return hipSuccess;
 }
 
 // Stage jobs + work list into `r` and enqueue reset + analysis kernels on
@@ -465,26 +562,35 @@ int32_t build_work(const std::vector<DevJob> &jobs, int32_t n_lists, int32_t per
 // diagonals per list (batches of whole streams) or starts all at once
 // (serving, where compositions change every step).  `ev0`/`ev1`, if given,
 // are recorded around the analysis kernel instead of the context's ring.
+// `launch` false stages only (graph capture launches it).
 void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool time_it,
              hipStream_t st = nullptr, int32_t grid_cap = 0, bool stagger = true,
-             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool launch = true)
 {
     if (!st) st = ctx->stream;
-    const int32_t fit = ctx->resident[ctx->waves];
+    const bool sweep = ctx->mode == EVXA_MODE_SWEEP;
+    const int32_t fit = ctx->resident[sweep ? 1 : 0][ctx->waves];
     const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
     const int32_t max_grid = grid_cap > 0 ? std::min(grid_cap, all) : all;
     if (jobs.empty()) return;
     if ((int)jobs.size() > kMaxJobs) fail(EVXA_ERROR_CAPACITY_LIMIT, "too many jobs in one launch");
     std::vector<uint64_t> key;
-    int32_t n_progress = 0;
+    int32_t n_words = 0;           // per-MB words cleared per launch
+    int64_t state_bytes = 0, n_mbs = 0, n_tickets = 0;
     for (const auto &j : jobs) {
+        n_mbs += (int64_t)j.wb * j.hb;
+        n_tickets += (int64_t)j.wb * j.hb * j.n_refs;
         if ((int64_t)j.wb * j.hb > (int64_t)kMbMask) fail(EVXA_ERROR_CAPACITY_LIMIT, "frame too large");
         key.push_back(((uint64_t)j.wb << 40) | ((uint64_t)j.hb << 8) | (uint32_t)j.n_refs);
-        n_progress += j.wb * j.hb * 2;          // one 64-bit predictor word per MB
+        // Wavefront: one 64-bit predictor word per MB.  Sweep: the listed
+        // stamp, the words of both parities and the predictor.
+        n_words += sweep ? sweep_words(j.wb * j.hb) : j.wb * j.hb * 2;
+        state_bytes += (int64_t)align_up((size_t)j.wb * j.hb * (sizeof(Decision) + 4), 256);
     }
     key.push_back(((uint64_t)max_grid << 32) | ((uint64_t)ctx->waves << 24) |
                   ((uint64_t)ctx->lists << 16) | ((uint64_t)(stagger ? 1 : 0) << 15) |
                   (uint64_t)(ctx->cap_pct & 0x7FFF));
+    key.push_back(sweep ? 1 + ctx->sweep_order : 0);
     if (!r.staged) check(hipEventCreateWithFlags(&r.staged, hipEventDisableTiming), "hipEventCreate");
     // The previous launch's host->device copies must have consumed the
     // staging buffers before they are rewritten.
@@ -492,19 +598,28 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
 
     if (key != r.key) {
         std::vector<uint32_t> t;
-        // One list for a single picture (its rows all stay in flight), one
-        // per XCD for batches.
-        const int32_t lists = ctx->lists > 0 ? ctx->lists : (jobs.size() > 1 ? kLists : 1);
+        // Wavefront: one list for a single picture (its rows all stay in
+        // flight), one per XCD for batches.  Sweeps: one per XCD.
+        const int32_t lists = ctx->lists > 0 ? ctx->lists : ((jobs.size() > 1 || sweep) ? kLists : 1);
         const int32_t n_lists = std::max(1, std::min(lists, max_grid));
         const int32_t cap = std::max(16, max_grid / n_lists * ctx->cap_pct / 100);
-        const int32_t widest = build_work(jobs, n_lists, cap, stagger, t);
-        // Per list, enough waves for its widest diagonal (plus slack for the
-        // MBs still finishing the previous one), never more than fit at once:
-        // waiting waves beyond that only add polling traffic.  The grid is a
-        // multiple of n_lists so that every list has the same number of waves.
-        const int32_t per_list = std::max(8, widest + widest / 2 + 4);
-        r.grid = n_lists * std::max(1, std::min(max_grid / n_lists, per_list));
+        const bool raster = sweep && ctx->sweep_order == EVXA_SWEEP_ORDER_RASTER;
+        const int32_t widest = build_work(jobs, n_lists, cap, stagger && !raster, sweep, raster, t);
+        if (sweep) {
+            // Every workgroup that fits: sweep 0 is throughput work.
+            r.grid = n_lists * std::max(1, max_grid / n_lists);
+            r.max_sweeps = (int32_t)t[kWorkMaxSweeps];
+        } else {
+            // Per list, enough waves for its widest diagonal (plus slack for
+            // the MBs still finishing the previous one), never more than fit
+            // at once: waiting waves beyond that only add polling traffic.
+            // The grid is a multiple of n_lists so that every list has the
+            // same number of waves.
+            const int32_t per_list = std::max(8, widest + widest / 2 + 4);
+            r.grid = n_lists * std::max(1, std::min(max_grid / n_lists, per_list));
+        }
         r.waves = ctx->waves;
+        r.sweep = sweep;
         r.h_tickets.ensure(t.size() * sizeof(uint32_t));
         memcpy(r.h_tickets.p, t.data(), t.size() * sizeof(uint32_t));
         if (r.tickets.n < t.size() * sizeof(uint32_t)) r.tickets.alloc(t.size() * sizeof(uint32_t));
@@ -514,19 +629,56 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
         r.n_tickets = (int32_t)t.size() - kWorkHeader;
         r.key = key;
     }
-    const size_t prog_bytes = (size_t)(n_progress + kCounterWords) * sizeof(int32_t);
-    if (r.progress.n < prog_bytes) r.progress.alloc(prog_bytes);
-    r.n_progress = n_progress;
+    // Progress words: the counter block, the per-MB words, then (sweep mode)
+    // the sweep counter blocks, of which blocks 0..2 are cleared here and
+    // the kernel clears the others as it goes.
+    r.n_progress = n_words;
+    r.reads = ctx->sweep_reads;
+    r.wait_ticks = ctx->sweep_wait_us * 100;
+    r.batch = ctx->claim_batch;
+    r.sctr_off = (int32_t)align_up((size_t)(kCounterWords + n_words), 64);
+    const size_t prog_words = r.sweep ? (size_t)r.sctr_off + (size_t)r.max_sweeps * kSweepStride
+                                      : (size_t)(kCounterWords + n_words);
+    r.n_reset = r.sweep ? r.sctr_off + 3 * kSweepStride : kCounterWords + n_words;
+    if (r.progress.n < prog_words * sizeof(int32_t)) r.progress.alloc(prog_words * sizeof(int32_t));
+    if (r.sweep) {
+        if (r.state.n < (size_t)state_bytes) r.state.alloc((size_t)state_bytes);
+        // Each sub-list can receive every MB (check) or ticket (search) of
+        // the launch: a workgroup fills only the sub-list of its index.
+        r.cap_check = (int32_t)n_mbs;
+        r.cap_search = (int32_t)n_tickets;
+        const size_t list_bytes = 2 * kSubLists * (size_t)(n_mbs + n_tickets) * sizeof(uint64_t);
+        if (r.lists.n < list_bytes) {
+            r.lists.alloc(list_bytes);
+            check(hipMemsetAsync(r.lists.p, 0, list_bytes, st), "clear ticket lists");
+        }
+    }
     int32_t *prog = r.progress.as<int32_t>();
     int32_t off = kCounterWords;
+    char *state = r.state.as<char>();
     for (auto &j : jobs) {
-        j.mvw = reinterpret_cast<uint64_t *>(prog + off);
-        off += j.wb * j.hb * 2;
-        // A lone picture is latency-bound: the wave of whichever reference
-        // finishes last can select and publish at once, without waiting for
-        // the designated wave to see its result.  In batches the extra read
-        // per search costs throughput, so only the designated wave selects.
-        j.eager = jobs.size() == 1 && j.type == EVXA_FRAME_P && j.n_refs > 1;
+        const size_t n_mb = (size_t)j.wb * j.hb;
+        if (r.sweep) {
+            j.mvw = nullptr;
+            const int32_t nl = (int32_t)align_up(n_mb, 2);
+            j.sw_listed = reinterpret_cast<uint32_t *>(prog + off);
+            j.sw_words = reinterpret_cast<uint64_t *>(prog + off + nl);
+            j.sw_pred = reinterpret_cast<uint64_t *>(prog + off + nl + 4 * (int32_t)n_mb);
+            off += sweep_words((int32_t)n_mb);
+            j.sw_dec = reinterpret_cast<Decision *>(state);
+            j.sw_key = reinterpret_cast<uint32_t *>(state + n_mb * sizeof(Decision));
+            state += align_up(n_mb * (sizeof(Decision) + 4), 256);
+            j.eager = 0;
+        } else {
+            j.mvw = reinterpret_cast<uint64_t *>(prog + off);
+            off += (int32_t)n_mb * 2;
+            // A lone picture is latency-bound: the wave of whichever
+            // reference finishes last can select and publish at once,
+            // without waiting for the designated wave to see its result.  In
+            // batches the extra read per search costs throughput, so only
+            // the designated wave selects.
+            j.eager = jobs.size() == 1 && j.type == EVXA_FRAME_P && j.n_refs > 1;
+        }
     }
     const size_t jb = jobs.size() * sizeof(DevJob);
     r.h_jobs.ensure(jb);
@@ -534,17 +686,13 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     if (r.jobs.n < jb) r.jobs.alloc(jb);
     check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, st), "copy jobs");
     check(hipEventRecord(r.staged, st), "hipEventRecord");
-    check(launch_reset(prog, n_progress + kCounterWords, st), "reset kernel");
-    const int32_t grid = r.grid;
+    if (!launch) return;
     std::pair<hipEvent_t, hipEvent_t> ev{ev0, ev1};
     if (time_it) {
         if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
         ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
     }
-    if (ev.first) check(hipEventRecord(ev.first, st), "hipEventRecord");
-    check(launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), prog, grid, r.waves, st),
-          "analysis kernel");
-    if (ev.second) check(hipEventRecord(ev.second, st), "hipEventRecord");
+    check(launch_staged(r, st, ev.first, ev.second), "analysis kernel");
     if (time_it) {
         ctx->ring_head++;
         ctx->last_start = ev.first;
@@ -557,6 +705,10 @@ void check_counters(evxa_context *ctx, LaunchRes &r)
     if (!r.progress.p) return;
     int32_t c[kCounterError + 1];
     check(hipMemcpy(c, r.progress.p, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    // A launch that stopped early may leave ticket entries behind.
+    if (c[kCounterError] && r.lists.p) check(hipMemset(r.lists.p, 0, r.lists.n), "clear ticket lists");
+    if (c[kCounterError] & kErrSweepLimit)
+        fail(EVXA_ERROR_EXECUTION_FAILURE, "analysis kernel: sweep limit reached");
     if (c[kCounterError] & kErrWaitTimeout)
         fail(EVXA_ERROR_OPERATION_TIMEDOUT, "analysis kernel: dependency wait timed out");
     if (c[kCounterError] & kErrSearchFailed)
@@ -567,7 +719,7 @@ void check_counters(evxa_context *ctx, LaunchRes &r)
 
 extern "C" {
 
-int32_t evxa_version(void) { return 100; }
+int32_t evxa_version(void) { return 101; }
 
 const char *evxa_last_error(void) { return g_last_error.c_str(); }
 
@@ -606,8 +758,20 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
             hbm_budget_bytes = (uint64_t)(free_b * 0.9);
         }
         ctx->budget = hbm_budget_bytes;
-        ctx->resident[1] = analyze_resident_groups(device_ordinal, 1);
-        ctx->resident[2] = analyze_resident_groups(device_ordinal, 2);
+        for (int m = 0; m < 2; m++)
+            for (int w = 1; w <= 2; w++) ctx->resident[m][w] = analyze_resident_groups(device_ordinal, w, m != 0);
+        if (const char *g = getenv("EVXA_MODE")) {
+            const int v = atoi(g);
+            if (v == EVXA_MODE_WAVEFRONT || v == EVXA_MODE_SWEEP) ctx->mode = v;
+        }
+        if (const char *g = getenv("EVXA_SWEEP_READS")) {
+            const int v = atoi(g);
+            if (v == kSweepReadsGuess || v == kSweepReadsFresh) ctx->sweep_reads = v;
+        }
+        if (const char *g = getenv("EVXA_SWEEP_WAIT_US")) {
+            const int v = atoi(g);
+            if (v >= 0 && v <= 100000) ctx->sweep_wait_us = v;
+        }
         if (const char *g = getenv("EVXA_WAVES")) {
             const int v = atoi(g);
             if (v == 1 || v == 2) ctx->waves = v;
@@ -1173,25 +1337,16 @@ int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams, const evxa
         // Stage the work list and job table outside capture (plain copies),
         // then capture only the reset + analysis kernels.
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        enqueue(ctx, *res, jobs, false);
+        enqueue(ctx, *res, jobs, false, nullptr, 0, true, nullptr, nullptr, false);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         hipGraph_t graph = nullptr;
-        check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        hipError_t e1 = launch_reset(res->progress.as<int32_t>(), res->n_progress + kCounterWords,
-                                     ctx->stream);
-        const int32_t grid = res->grid;
         check(hipEventCreate(&res->g_start), "hipEventCreate");
         check(hipEventCreate(&res->g_stop), "hipEventCreate");
-        hipError_t e2 = hipEventRecord(res->g_start, ctx->stream);
-        hipError_t e3 = launch_analyze(res->jobs.as<DevJob>(), res->tickets.as<uint32_t>(),
-                                       res->progress.as<int32_t>(), grid, res->waves, ctx->stream);
-        hipError_t e4 = hipEventRecord(res->g_stop, ctx->stream);
-        hipError_t e5 = hipStreamEndCapture(ctx->stream, &graph);
-        check(e1, "capture reset");
-        check(e2, "capture event");
-        check(e3, "capture analysis");
-        check(e4, "capture event");
-        check(e5, "end capture");
+        check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        const hipError_t e1 = launch_staged(*res, ctx->stream, res->g_start, res->g_stop);
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+        check(e1, "capture analysis");
+        check(e2, "end capture");
         hipGraphExec_t exec = nullptr;
         hipError_t e6 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -1366,6 +1521,7 @@ struct evxa_server {
     struct Slot {
         hipStream_t st = nullptr;
         hipEvent_t t0 = nullptr, t1 = nullptr;
+        hipEvent_t dep = nullptr;        // work queued on the context's stream before a launch
         LaunchRes res;
         PinnedBuf flags;                 // error word of the launch, copied back
         std::vector<Req> reqs;
@@ -1400,10 +1556,16 @@ void server_launch(evxa_server *srv, evxa_server::Slot &sl, std::vector<evxa_ser
     std::vector<DevJob> jobs;
     jobs.reserve(reqs.size());
     for (auto &r : reqs) jobs.push_back(make_job(r.s, r.d, true));
+    // Calls that return before their work is done (uploads aside: analysis,
+    // reconstruction, deblocking, SAO) queue it on the context's stream; the
+    // launch reads their pictures, range words and temporal tables, so it
+    // waits for everything queued there so far.
+    check(hipEventRecord(sl.dep, ctx->stream), "hipEventRecord");
+    check(hipStreamWaitEvent(sl.st, sl.dep, 0), "hipStreamWaitEvent");
     // Each launch is sized by its own work (the widest diagonal of its
     // lists), capped at half the device: launches of chain-bound frames need
     // few workgroups, and several in flight may share the device.
-    const int32_t fit = ctx->resident[ctx->waves];
+    const int32_t fit = ctx->resident[ctx->mode == EVXA_MODE_SWEEP ? 1 : 0][ctx->waves];
     const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
     const int32_t cap = std::max(1, all / std::min(srv->max_inflight, 2));
     enqueue(ctx, sl.res, jobs, false, sl.st, cap, false, sl.t0, nullptr);
@@ -1524,10 +1686,19 @@ int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t)
     return guarded([&] {
         if (!ctx || !t) fail(EVXA_ERROR_INVALIDARG, "null argument");
         if (t->waves_per_search < 0 || t->waves_per_search > 2 || t->max_groups < 0 ||
-            t->lists < 0 || t->lists > kLists || t->cap_pct < 0 || t->cap_pct > 10000)
+            t->lists < 0 || t->lists > kLists || t->cap_pct < 0 || t->cap_pct > 10000 ||
+            (t->mode != 0 && t->mode != EVXA_MODE_WAVEFRONT && t->mode != EVXA_MODE_SWEEP) ||
+            t->sweep_reads < 0 || t->sweep_reads > 2 || t->sweep_wait_us < -1 ||
+            t->sweep_wait_us > 100000 || t->sweep_order < 0 || t->sweep_order > 2 ||
+            t->claim_batch < 0 || t->claim_batch > kMaxClaimBatch)
             fail(EVXA_ERROR_INVALIDARG, "tuning out of range");
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (t->waves_per_search) ctx->waves = t->waves_per_search;
+        if (t->mode) ctx->mode = t->mode;
+        if (t->sweep_reads) ctx->sweep_reads = t->sweep_reads;
+        if (t->sweep_wait_us >= 0) ctx->sweep_wait_us = t->sweep_wait_us;
+        if (t->sweep_order) ctx->sweep_order = t->sweep_order;
+        if (t->claim_batch) ctx->claim_batch = t->claim_batch;
         ctx->grid = t->max_groups;
         ctx->lists = t->lists;
         ctx->cap_pct = t->cap_pct ? t->cap_pct : 60;
@@ -1543,7 +1714,58 @@ int evxa_get_tuning(evxa_context *ctx, evxa_tuning *t, int32_t *resident_groups)
         t->max_groups = ctx->grid;
         t->lists = ctx->lists;
         t->cap_pct = ctx->cap_pct;
-        if (resident_groups) *resident_groups = ctx->resident[ctx->waves];
+        t->mode = ctx->mode;
+        t->sweep_reads = ctx->sweep_reads;
+        t->sweep_wait_us = ctx->sweep_wait_us;
+        t->sweep_order = ctx->sweep_order;
+        t->claim_batch = ctx->claim_batch;
+        if (resident_groups) *resident_groups = ctx->resident[ctx->mode == EVXA_MODE_SWEEP ? 1 : 0][ctx->waves];
+    });
+}
+
+int evxa_sweep_stats(evxa_context *ctx, int32_t *listed, int32_t *searched, float *done_us,
+                     int32_t max, int32_t *n_sweeps)
+{
+    return guarded([&] {
+        if (!ctx || !n_sweeps || (max > 0 && (!listed || !searched)))
+            fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        *n_sweeps = 0;
+        const LaunchRes &r = ctx->res;
+        if (!r.sweep || !r.progress.p) return;
+        std::vector<int32_t> blk((size_t)r.max_sweeps * kSweepStride);
+        check(hipMemcpy(blk.data(), r.progress.as<int32_t>() + r.sctr_off, blk.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost),
+              "read sweep counters");
+        // Sweeps 0 and 1 take every MB; a later sweep ran iff MBs were listed
+        // for it (the block after the last sweep was cleared before that
+        // sweep and stays empty).
+        int32_t n = 0;
+        uint64_t t0;
+        memcpy(&t0, blk.data() + kSwStart, 8);
+        for (int32_t s = 0; s < r.max_sweeps; s++) {
+            const int32_t *b = blk.data() + (size_t)s * kSweepStride;
+            int32_t l = 0;
+            if (s <= 1) l = r.cap_check;
+            else
+                for (int k = 0; k < kSubLists; k++) l += b[k * kSubStride + kSubCheckTail];
+            if (s > 1 && l == 0) break;
+            if (n < max) {
+                listed[n] = l;
+                searched[n] = b[kSwSearched];
+                if (done_us) {
+                    uint64_t t1;
+                    memcpy(&t1, b + kSwTime, 8);
+                    // A sweep whose checks changed nothing ends without searches
+                    // (and without an end time of its own).
+                    done_us[n] = b[kSwFlag] ? (float)((double)(int64_t)(t1 - t0) / 100.0) : -1.0f;
+                }
+            }
+            n++;
+        }
+        *n_sweeps = n;
     });
 }
 
@@ -1567,6 +1789,7 @@ int evxa_server_create(evxa_context *ctx, int32_t max_inflight, int32_t max_fram
                 check(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking), "hipStreamCreate");
                 check(hipEventCreate(&sl->t0), "hipEventCreate");
                 check(hipEventCreate(&sl->t1), "hipEventCreate");
+                check(hipEventCreateWithFlags(&sl->dep, hipEventDisableTiming), "hipEventCreate");
                 sl->flags.ensure(64);
                 srv->slots.push_back(std::move(sl));
             }
@@ -1632,6 +1855,7 @@ int evxa_server_destroy(evxa_server *srv)
             if (sl->st) (void)hipStreamSynchronize(sl->st);
             if (sl->t0) (void)hipEventDestroy(sl->t0);
             if (sl->t1) (void)hipEventDestroy(sl->t1);
+            if (sl->dep) (void)hipEventDestroy(sl->dep);
             if (sl->st) (void)hipStreamDestroy(sl->st);
         }
         delete srv;

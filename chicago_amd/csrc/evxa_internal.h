@@ -50,6 +50,18 @@ struct SearchRec {
 };
 static_assert(sizeof(SearchRec) == 32, "SearchRec layout");
 
+// A MB's current decision in sweep mode (64 bytes): the searches its
+// record and prediction come from, written at every re-search and read by
+// the final pass.  w[0..2]: the primary search (P: the selected reference;
+// B: forward) and w[3..5]: the backward search (B), both in the SearchRec
+// word layout; w[6]: bits 0..31 the selection SAD, 32..33 the selected
+// reference, 34 bidirectional; w[7]: bits 0..31 integer candidates, 32..47
+// axial and 48..63 diagonal sub-pel directions over all searches.
+struct Decision {
+    uint64_t w[8];
+};
+static_assert(sizeof(Decision) == 64, "Decision layout");
+
 // One frame of work inside a launch.
 struct DevJob {
     DevPic src;
@@ -66,6 +78,20 @@ struct DevJob {
     uint64_t *mvw;                     // per MB: predictor word, zeroed per launch
     int32_t eager;                     // P with several references, alone in its launch:
                                        // every reference's wave tries the selection
+    // Sweep mode, per MB: the predictor words by sweep parity (2 per MB;
+    // parity 0 is zero at the start of a launch, so sweep 0 can tell which
+    // words are published), the key of the predictor the current decision
+    // was searched with, the predictor the MB's searches of the current sweep
+    // use (zero at the start), the last sweep the MB was listed for (zero at
+    // the start) and the current decision.  `guess` is the table whose block
+    // types and MVs stand in for the neighbours' records in sweep 0 (the
+    // temporal table of a P frame), or nullptr for none.
+    uint64_t *sw_words;
+    uint32_t *sw_key;
+    uint64_t *sw_pred;                 // bit 63 | the predictor (x low, y high) the MB's searches use
+    uint32_t *sw_listed;
+    Decision *sw_dec;
+    const evxa_block_record *guess;
 };
 
 // Work-list entry: job (10 bits) | reference (2 bits) | MB index (20 bits).
@@ -84,24 +110,98 @@ constexpr int kMaxJobs = 1 << (32 - kJobShift);
 // [0] n_lists, [1 + k] first ticket of list k (k = 0 .. n_lists, the last
 // entry closes the final list); tickets follow.
 constexpr int kLists = 8;
-constexpr int kWorkHeader = 16;
+constexpr int kWorkHeader = 32;
+// Sweep mode adds to the header: [kWorkFinal + k] first ticket of final-pass
+// list k (k = 0 .. n_lists; the final pass has one ticket per MB), the
+// number of sweep-0 lists that hold tickets and the sweep counter blocks
+// available.
+constexpr int kWorkFinal = 12;
+constexpr int kWorkNonEmpty = 24;
+constexpr int kWorkMaxSweeps = 25;
 
 // Counter block of a launch: [1] error flags; the next-ticket counter of
-// list k at kCounterList + k * kCounterListStride (own 256-byte line each).
+// list k at kCounterList + k * kCounterListStride (own 256-byte block each),
+// and in sweep mode, in the same block, the list's final-pass next-ticket
+// counter (+ kCounterFinal) and its count of finished sweep-0 tickets
+// (+ kCounterDone, a line of its own).
 constexpr int kCounterError = 1;
 constexpr int kCounterList = 64;
 constexpr int kCounterListStride = 64;
+constexpr int kCounterFinal = 16;
+constexpr int kCounterDone = 32;
+constexpr int kCounterCheck1 = 48;      // next chunk of sweep 1's check over the list's MBs
 constexpr int kCounterWords = kCounterList + kLists * kCounterListStride;
+
+// Sweep counters: block s (kSweepStride words) per sweep.  Sweeps s >= 1
+// have a check phase (the MBs listed for the sweep, 64 per claim, one lane
+// each; sweep 1 checks every MB, in the final pass's lists) and a search
+// phase (the tickets of the MBs whose predictor changed).
+// Both phases' lists are split into kSubLists sub-lists, filled by the
+// workgroups whose index is k modulo kSubLists and taken by any; sub-list k's
+// counters sit at k * kSubStride: [kSubChunkHead] next check chunk,
+// [kSubCheckTail] MBs listed, [kSubSearchTail] search tickets,
+// [kSubDesignated] MBs searched, and on a line of their own
+// [kSubSearchHead] next search ticket, [kSubDone] MBs finished.  Then, each
+// on its own 128-byte line: check phase finished (flag), check chunks
+// finished, search sub-lists finished (sweep 0: static lists finished), the
+// sweep's finished flag with its end time and (block 0) the launch's start
+// time, and the MBs the sweep searched.
+constexpr int kSubLists = 8;
+constexpr int kSubStride = 64;
+constexpr int kSubChunkHead = 0;
+constexpr int kSubCheckTail = 1;
+constexpr int kSubSearchTail = 2;
+constexpr int kSubDesignated = 3;
+constexpr int kSubSearchHead = 32;
+constexpr int kSubDone = 33;
+constexpr int kSwFlagC = kSubLists * kSubStride;   // 512
+constexpr int kSwDoneC = kSwFlagC + 32;
+constexpr int kSwSubDone = kSwFlagC + 64;
+constexpr int kSwFlag = kSwFlagC + 96;
+constexpr int kSwTime = kSwFlag + 2;    // s_memrealtime (100 MHz), 64-bit
+constexpr int kSwStart = kSwFlag + 4;   // block 0: the first ticket's claim, 64-bit
+constexpr int kSwSearched = kSwFlagC + 128;
+constexpr int kSweepStride = kSwFlagC + 160;
+constexpr int kCheckChunk = 64;         // listed MBs per check claim
+constexpr int kMaxClaimBatch = 16;      // tickets per claim, at most
 
 // Error flag bits.
 constexpr int32_t kErrSearchFailed = 1;
 constexpr int32_t kErrWaitTimeout = 2;
+constexpr int32_t kErrSweepLimit = 4;
+
+// How sweep 0 reads its neighbours' words: the guess table only (the
+// deterministic Jacobi iteration), or also words published earlier in sweep
+// 0, waiting up to a bounded time for them (a timing-dependent path to the
+// same fixpoint).
+constexpr int32_t kSweepReadsGuess = 1;
+constexpr int32_t kSweepReadsFresh = 2;
+
+// Sweep-mode resources of a launch beyond the counter block.  The dynamic
+// lists come in two parities (sweep s uses s & 1), each of kSubLists
+// sub-lists: check entries (one per listed MB, cap_check each) and search
+// tickets (cap_search each).  Entries are zero between uses.
+struct SweepArgs {
+    uint64_t *check;
+    uint64_t *search;
+    int32_t cap_check;
+    int32_t cap_search;
+    int32_t max_sweeps;   // sweep counter blocks available
+    int32_t *sctr;        // sweep counter blocks
+    int32_t reads;        // kSweepReadsGuess or kSweepReadsFresh
+    int32_t wait_ticks;   // fresh reads: longest wait, s_memrealtime ticks (10 ns)
+    int32_t batch;        // tickets per claim (sweep 0, searches, final pass)
+};
 
 // Kernel launchers (evxa_kernels.hip).
 hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st);
 // `waves` (1 or 2) per workgroup; `grid` workgroups.
 hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *counters,
                           int32_t grid, int32_t waves, hipStream_t st);
+// The same work in sweep mode (fixpoint of the MV predictor, then a final
+// pass of prediction, DCT and quantisation per MB).
+hipError_t launch_sweep(const DevJob *jobs, const uint32_t *work, int32_t *counters,
+                        const SweepArgs &sa, int32_t grid, int32_t waves, hipStream_t st);
 // Reconstruction of the MBs of `job` (its records and prediction sources)
 // from `levels` into the picture `dst`; grows the spread range record *wide
 // (clear it first) by the stored samples.
@@ -150,8 +250,8 @@ hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, in
 hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t *pred,
                                    const int32_t *quality, int16_t *coeffs, int16_t *levels,
                                    int32_t *qp, int32_t *variance, hipStream_t st);
-// Workgroups of the analysis kernel with `waves` waves each that fit the
-// device at once.
-int32_t analyze_resident_groups(int32_t device, int32_t waves);
+// Workgroups of the analysis kernel (wavefront or sweep mode) with `waves`
+// waves each that fit the device at once.
+int32_t analyze_resident_groups(int32_t device, int32_t waves, bool sweep);
 
 }  // namespace evxa

@@ -1426,21 +1426,29 @@ __device__ __forceinline__ void write_words(uint32_t *dst, const uint32_t *w, in
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 
+// Word i (0..2) of a search result in the SearchRec layout, bit 63 set.
+__device__ __forceinline__ uint64_t pack_search(const SearchOut &o, int i)
+{
+    const uint64_t pub = 1ull << 63;
+    if (i == 0) {
+        const uint32_t pos = (uint32_t)(uint16_t)o.best_x | ((uint32_t)(uint16_t)o.best_y << 16);
+        return pub | ((uint64_t)(uint32_t)o.mad << 32) | pos;
+    }
+    if (i == 1) return pub | ((uint64_t)(uint32_t)o.sad << 32) | fbits(o.cost);
+    const uint32_t cnt = (uint32_t)(o.n_int & 0xFFFF) | ((uint32_t)(o.n_ax & 0xFF) << 16) |
+                         ((uint32_t)(o.n_dg & 0xFF) << 24);
+    const uint32_t fl = (uint32_t)(o.sp_en & 1) | ((uint32_t)(o.sp_amt & 1) << 1) |
+                        ((uint32_t)(o.sp_idx & 7) << 2) | ((uint32_t)(o.btype & 0xFF) << 8);
+    return pub | ((uint64_t)fl << 32) | cnt;
+}
+
 // Lanes 0..2 store this search's three result words (SearchRec layout,
 // bit 63 set) for the wave that finishes the MB.
 __device__ __forceinline__ void publish_search(uint64_t *dst, const SearchOut &o)
 {
     const int lane = lane_id();
-    const uint64_t pub = 1ull << 63;
-    const uint32_t pos = (uint32_t)(uint16_t)o.best_x | ((uint32_t)(uint16_t)o.best_y << 16);
-    const uint32_t cnt = (uint32_t)(o.n_int & 0xFFFF) | ((uint32_t)(o.n_ax & 0xFF) << 16) |
-                         ((uint32_t)(o.n_dg & 0xFF) << 24);
-    const uint32_t fl = (uint32_t)(o.sp_en & 1) | ((uint32_t)(o.sp_amt & 1) << 1) |
-                        ((uint32_t)(o.sp_idx & 7) << 2) | ((uint32_t)(o.btype & 0xFF) << 8);
-    const uint64_t w0 = pub | ((uint64_t)(uint32_t)o.mad << 32) | pos;
-    const uint64_t w1 = pub | ((uint64_t)(uint32_t)o.sad << 32) | fbits(o.cost);
-    const uint64_t w2 = pub | ((uint64_t)fl << 32) | cnt;
-    if (lane < 3) st_relaxed64(dst + lane, lane == 0 ? w0 : lane == 1 ? w1 : w2);
+    const uint64_t w = pack_search(o, lane < 3 ? lane : 0);
+    if (lane < 3) st_relaxed64(dst + lane, w);
 }
 
 // Search result k of a MB, read wave-uniformly from the words in `sw`
@@ -1506,19 +1514,14 @@ __device__ __forceinline__ void try_publish(const DevJob &job, int mb, int x, in
                                              mv_word(best.best_x - x, best.best_y - y)));
 }
 
-// Selection among a MB's searches, record publication, residual prediction,
-// DCT and quantisation; run by the wave that searched the MB's last
-// reference, which passes its own result in `own`.  The results of the
-// other references are polled from job.searches (they were claimed earlier,
-// so their waves are running) and cleared for the next launch.
-__device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
-                          const Blk &src, const Geo &geo, const SearchOut &own)
+// The results of the searches of a MB's first n_refs - 1 references, which
+// their waves publish for the wave of the last one: lane l < 3 (n_refs - 1)
+// polls word l % 3 of reference l / 3 (they were claimed earlier, so their
+// waves are running), then clears what it read for the next search, and
+// this wave's own words too if it published them (`own_published`).
+__device__ uint64_t gather_searches(const DevJob &job, int idx, int32_t *counters, bool own_published)
 {
     const int lane = lane_id();
-    const int x = bx * 16, y = by * 16;
-    const int idx = by * job.wb + bx;
-    const float lambda = job.lambda;
-    // Lane l < 3 (n_refs - 1) polls word l % 3 of reference l / 3.
     const int n_words = 3 * (job.n_refs - 1);
     uint64_t sw = 0;
     if (n_words > 0) {
@@ -1535,17 +1538,62 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
             for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
             nap = min(nap * 2, kMaxNap);
         }
-        // Clear what was read, and this wave's own words if it published them.
         if (lane < n_words) st_relaxed64(p, 0ull);
-        if (job.eager && lane >= n_words && lane < n_words + 3) st_relaxed64(p, 0ull);
+        if (own_published && lane >= n_words && lane < n_words + 3) st_relaxed64(p, 0ull);
     }
-    auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
+    return sw;
+}
 
-    SearchOut best, other;
-    other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
-    int best_k = -1, best_sad = kInt32Max, bidir = 0;
-    int n_int = 0, n_ax = 0, n_dg = 0;
-    store_src(src, geo, sh.src);
+// A MB's selection (encode.cpp:183-248 for P, 386-443 for B) and the
+// searches it rests on.
+struct Sel {
+    SearchOut best, other;      // other: the backward search of a bidirectional MB
+    int best_k, best_sad, bidir;
+    int n_int, n_ax, n_dg;
+};
+
+// The two predictions of a B MB (motion.cpp:729-776) into sh.pred and
+// sh.pred_b; returns the SAD of their average against sh.src over Y, U and V
+// (motion.cpp:779-813).
+__device__ int b_predictions(const DevJob &job, const SearchOut &f, const SearchOut &b, MbLds &sh)
+{
+    const int lane = lane_id();
+    predict_mb(job.refs[0], f, sh.pred, sh.tmp);
+    predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
+    int bsad = 0;
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        const int a = (sh.pred[i] + sh.pred_b[i] + 1) >> 1;
+        bsad += abs(sh.src[i] - a);
+    }
+    return wave_sum(bsad);
+}
+
+// The B prediction the selection chose, into sh.pred (after b_predictions).
+__device__ __forceinline__ void b_choose(const Sel &sel, MbLds &sh)
+{
+    const int lane = lane_id();
+    wave_sync();
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        if (sel.bidir) sh.pred[i] = (int16_t)((sh.pred[i] + sh.pred_b[i] + 1) >> 1);
+        else if (sel.best_k == 1) sh.pred[i] = sh.pred_b[i];
+    }
+    wave_sync();
+}
+
+// Selection among a MB's searches, reference k's result from result(k); the
+// source MB is in sh.src.  For B frames the chosen prediction is left in
+// sh.pred.
+template <typename F>
+__device__ Sel select_mb(const DevJob &job, int x, int y, MbLds &sh, F result, int32_t *counters)
+{
+    const int lane = lane_id();
+    const float lambda = job.lambda;
+    Sel sel;
+    sel.other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
+    sel.best_k = -1;
+    sel.best_sad = kInt32Max;
+    sel.bidir = 0;
+    sel.n_int = 0; sel.n_ax = 0; sel.n_dg = 0;
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:215-247 against an empty (non-copy, INT32_MAX) sentinel
         // (the same rule as select_p, written out: the compiler schedules
@@ -1553,13 +1601,13 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
         bool best_copy = false;
         for (int k = 0; k < job.n_refs; k++) {
             const SearchOut r = result(k);
-            n_int += r.n_int; n_ax += r.n_ax; n_dg += r.n_dg;
+            sel.n_int += r.n_int; sel.n_ax += r.n_ax; sel.n_dg += r.n_dg;
             const int bias = __float2int_rz(__fmul_rn(lambda, __int2float_rn((k + 1) * 2)));
             const bool copy = (r.btype & EVXA_BLOCK_COPY) != 0;
             if (copy != best_copy) {
-                if (copy) { best = r; best_k = k; best_sad = r.sad; best_copy = true; }
-            } else if (r.sad + bias < best_sad) {
-                best = r; best_k = k; best_sad = r.sad; best_copy = copy;
+                if (copy) { sel.best = r; sel.best_k = k; sel.best_sad = r.sad; best_copy = true; }
+            } else if (r.sad + bias < sel.best_sad) {
+                sel.best = r; sel.best_k = k; sel.best_sad = r.sad; best_copy = copy;
             }
         }
     } else {
@@ -1567,35 +1615,43 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
         // candidate (motion.cpp:702-840) reuses both searches.
         const SearchOut f = result(0);
         const SearchOut b = result(1);
-        n_int = f.n_int + b.n_int; n_ax = f.n_ax + b.n_ax; n_dg = f.n_dg + b.n_dg;
-        if (f.sad < best_sad) { best = f; best_k = 0; best_sad = f.sad; }
-        if (b.sad < best_sad) { best = b; best_k = 1; best_sad = b.sad; }
-        predict_mb(job.refs[0], f, sh.pred, sh.tmp);
-        predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
-        int bsad = 0;
-        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
-            const int a = (sh.pred[i] + sh.pred_b[i] + 1) >> 1;
-            bsad += abs(sh.src[i] - a);
-        }
-        bsad = wave_sum(bsad);
+        sel.n_int = f.n_int + b.n_int; sel.n_ax = f.n_ax + b.n_ax; sel.n_dg = f.n_dg + b.n_dg;
+        if (f.sad < sel.best_sad) { sel.best = f; sel.best_k = 0; sel.best_sad = f.sad; }
+        if (b.sad < sel.best_sad) { sel.best = b; sel.best_k = 1; sel.best_sad = b.sad; }
+        const int bsad = b_predictions(job, f, b, sh);
         const int bias = __float2int_rz(__fmul_rn(lambda, 8.0f));
-        if (bsad + bias < best_sad) {
-            bidir = 1; best_sad = bsad; best = f; best_k = 0; other = b;
+        if (bsad + bias < sel.best_sad) {
+            sel.bidir = 1; sel.best_sad = bsad; sel.best = f; sel.best_k = 0; sel.other = b;
         }
-        wave_sync();
-        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
-            if (bidir) sh.pred[i] = (int16_t)((sh.pred[i] + sh.pred_b[i] + 1) >> 1);
-            else if (best_k == 1) sh.pred[i] = sh.pred_b[i];
-        }
-        wave_sync();
+        b_choose(sel, sh);
     }
-
-    if (best_k < 0 || best_sad == kInt32Max) {
+    if (sel.best_k < 0 || sel.best_sad == kInt32Max) {
         if (lane == 0) atomicOr(&counters[kCounterError], kErrSearchFailed);
-        best_k = 0;
-        best = SearchOut{x, y, best_sad, 0, 0, 0.0f, 0, 0, 0, EVXA_BLOCK_COPY, 0, 0, 0};
+        sel.best_k = 0;
+        sel.best = SearchOut{x, y, sel.best_sad, 0, 0, 0.0f, 0, 0, 0, EVXA_BLOCK_COPY, 0, 0, 0};
     }
+    return sel;
+}
 
+// The block type and MV of a selection, as the MV predictor reads them.
+__device__ __forceinline__ uint64_t sel_word(const Sel &sel, int x, int y)
+{
+    const int btype = sel.bidir ? EVXA_BLOCK_MOTION : sel.best.btype;
+    return pred_word((btype & EVXA_BLOCK_MOTION) != 0,
+                     mv_word(sel.best.best_x - x, sel.best.best_y - y));
+}
+
+// The record of a selected MB, then (for non-copy MBs) its residual
+// prediction, DCT and quantisation; `publish`: also store the MB's predictor
+// word for the MBs that wait on it.  sh.src holds the source MB, and for B
+// frames sh.pred the chosen prediction.
+__device__ void emit_mb(const DevJob &job, int bx, int by, MbLds &sh, const Sel &sel, bool publish)
+{
+    const int lane = lane_id();
+    const int x = bx * 16, y = by * 16;
+    const int idx = by * job.wb + bx;
+    const SearchOut &best = sel.best, &other = sel.other;
+    const int bidir = sel.bidir, best_k = sel.best_k;
     const uint32_t ids = job.target_ids;
     uint32_t w[16];
     const int btype = bidir ? EVXA_BLOCK_MOTION : best.btype;
@@ -1611,14 +1667,14 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
            ((uint32_t)(bidir ? other.sp_en : 0) << 24);
     w[4] = bidir ? ((uint32_t)other.sp_amt | ((uint32_t)other.sp_idx << 8)) : 0u;
     w[5] = 0;
-    w[6] = (uint32_t)best_sad;
+    w[6] = (uint32_t)sel.best_sad;
     w[7] = (uint32_t)best.mad;
     w[8] = fbits(best.cost);
     w[9] = bidir ? (uint32_t)other.sad : 0u;
     w[10] = bidir ? (uint32_t)other.mad : 0u;
     w[11] = bidir ? fbits(other.cost) : 0u;
-    w[12] = (uint32_t)n_int;
-    w[13] = (uint32_t)n_ax | ((uint32_t)n_dg << 16);
+    w[12] = (uint32_t)sel.n_int;
+    w[13] = (uint32_t)sel.n_ax | ((uint32_t)sel.n_dg << 16);
     w[14] = 0;
     w[15] = 0;
     uint32_t *rec = reinterpret_cast<uint32_t *>(job.records + idx);
@@ -1626,7 +1682,8 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
 
     // Publish what the MV predictor of later MBs reads (block type and MV)
     // as one device-scope word: they poll it directly.
-    if (lane == 0) st_relaxed64(job.mvw + idx, pred_word((btype & EVXA_BLOCK_MOTION) != 0, w[1]));
+    if (publish && lane == 0)
+        st_relaxed64(job.mvw + idx, pred_word((btype & EVXA_BLOCK_MOTION) != 0, w[1]));
 
     int16_t *co = job.coeffs + (size_t)idx * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)idx * EVXA_MB_SAMPLES;
@@ -1647,6 +1704,23 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
     w2[0] = 1u | (bidir ? 2u : 0u);
     write_words(rec + 14, w2, 1);
     wave_sync();
+}
+
+// Selection among a MB's searches, record publication, residual prediction,
+// DCT and quantisation; run by the wave that searched the MB's last
+// reference, which passes its own result in `own`.  The results of the
+// other references are polled from job.searches (they were claimed earlier,
+// so their waves are running) and cleared for the next launch.
+__device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
+                          const Blk &src, const Geo &geo, const SearchOut &own)
+{
+    const int x = bx * 16, y = by * 16;
+    const int idx = by * job.wb + bx;
+    const uint64_t sw = gather_searches(job, idx, counters, job.eager != 0);
+    auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
+    store_src(src, geo, sh.src);
+    const Sel sel = select_mb(job, x, y, sh, result, counters);
+    emit_mb(job, bx, by, sh, sel, true);
 }
 
 // The persistent analysis kernel: NW waves per workgroup search one
@@ -1736,7 +1810,9 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
         uint64_t *early = job.type == EVXA_FRAME_P && job.n_refs == 1 ? job.mvw + mb : nullptr;
         const SearchOut o = inter_search<NW>(rp, job.width, job.height, narrow, s, geo, pre, win,
                                              x, y, job.thresh, job.lambda, pmx, pmy, sh.sp, early);
-        // Wave 0 finishes the ticket; the other waves go on to the next claim.
+        // Wave 0 finishes the ticket in LDS the search used; the other waves
+        // go on to the next claim.
+        if (NW > 1) __syncthreads();
         if (NW > 1 && wave_id() != 0) continue;
         uint64_t *own_words =
             reinterpret_cast<uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref);
@@ -1752,6 +1828,769 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
             try_publish(job, mb, x, y);
         }
         finish_mb(job, bx, by, counters, sh.mb, s, geo, o);
+    }
+}
+
+// ------------------------------------------------------------ sweep mode --
+//
+// perform_inter_search depends on the frame's block table only through
+// compute_mv_predictor (motion.cpp:613-635), and a MB's predictor reads MBs
+// before it in raster order: left, top, top-right or top-left
+// (motion.cpp:271-346).  So the frame's records are the unique fixpoint of
+// "every MB's record is its searches under the predictor of its neighbours'
+// records": by induction along raster order, any table with that property
+// equals the sequential result.  Sweep mode reaches the fixpoint by Jacobi
+// sweeps, with no MB waiting on another:
+//  * sweep 0 searches every (MB, reference) pair, with a guess table (the
+//    temporal table, or none) standing in for the neighbours' records;
+//  * sweep s >= 1 first checks the MBs listed for it, one lane each: it
+//    recomputes the predictor from every MB's word (block type and MV) as of
+//    sweep s - 1, and keeps the MB's decision if the predictor as the search
+//    uses it -- absent, or the clamped candidate position -- is unchanged;
+//    then it searches the others again, one (MB, reference) ticket per
+//    workgroup;
+//  * a MB whose word changes lists itself and the four MBs that read it for
+//    the next sweep, and the iteration ends with a sweep that changes none.
+// A MB at dependency depth d is final after sweep d, so at most the frame's
+// anti-diagonal count plus one sweeps run.  A final pass then forms each
+// MB's prediction, DCT and quantisation once, from its last decision.
+//
+// Words by sweep parity: sweep s reads W[(s - 1) & 1] and writes W[s & 1];
+// after sweep s, W[s & 1] holds every MB's word as of sweep s (a listed MB
+// that keeps its predictor copies its word over).  Nothing is read and
+// written in the same sweep, so every reference of a MB computes the same
+// predictor, and each sweep's lists and counts are deterministic.
+//
+// Phases follow each other without a grid barrier: a wave moves on once
+// every item of the current phase has finished (a flag set by the wave that
+// finishes the last one), and only running waves hold items, so the launch
+// needs no co-residency.  The dynamic lists and their counters are split in
+// kSubLists sub-lists (a workgroup fills the one of its index and takes from
+// any), because device-scope atomics on one address serialise.  Cross-wave
+// data (words, keys, predictors, decisions, list entries, counters) move as
+// device-scope accesses, and every item's stores complete before it counts
+// itself finished.
+
+constexpr uint64_t kSweepWaitTicks = 100000000ull;   // 1 s of s_memrealtime (100 MHz)
+constexpr int kSweepMaxNap = 16;                     // longest back-off, s_sleep(8) units
+
+__device__ __forceinline__ int32_t ld_relaxed32(const int32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed32(int32_t *p, int32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t fetch_add(int32_t *p, int32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every store of this wave has completed (device-scope stores are then
+// visible to every agent-scope load).
+__device__ __forceinline__ void stores_done()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// One lane waits until *flag is set; false after kSweepWaitTicks.
+__device__ bool wait_flag(const int32_t *flag)
+{
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    int nap = 1;
+    while (ld_relaxed32(flag) == 0) {
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kSweepWaitTicks) return false;
+        for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(8);
+        nap = min(nap * 2, kSweepMaxNap);
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint64_t lane_u64(uint64_t w, int l)
+{
+    return ((uint64_t)(uint32_t)lane_value((int)(w >> 32), l) << 32) |
+           (uint32_t)lane_value((int)(uint32_t)w, l);
+}
+
+// Entry of a dynamic list: bit 63 set, the sweep in bits 32..46, the ticket
+// (job, reference, MB) below.  The claiming wave clears it, so the lists are
+// zero between uses.
+__device__ __forceinline__ uint64_t list_entry(int sweep, uint32_t ticket)
+{
+    return (1ull << 63) | ((uint64_t)(sweep & 0x7FFF) << 32) | ticket;
+}
+
+// Reads and clears the entry of sweep s at *slot (written by a wave that
+// counted its item finished before this phase began; the poll only guards
+// the ordering of the two device-scope stores).  Returns false if none came.
+__device__ __forceinline__ bool take_entry(uint64_t *slot, int s, uint32_t &ticket)
+{
+    const uint64_t want = list_entry(s, 0);
+    uint64_t e = ld_relaxed64(slot);
+    int spins = 0;
+    while ((e & 0xFFFFFFFF00000000ull) != want && ++spins < kSpinLimit) {
+        __builtin_amdgcn_s_sleep(1);
+        e = ld_relaxed64(slot);
+    }
+    if ((e & 0xFFFFFFFF00000000ull) != want) return false;
+    st_relaxed64(slot, 0ull);
+    ticket = (uint32_t)e;
+    return true;
+}
+
+__device__ __forceinline__ int32_t *sub_ctr(const SweepArgs &sa, int s, int k)
+{
+    return sa.sctr + (size_t)s * kSweepStride + k * kSubStride;
+}
+
+__device__ __forceinline__ uint64_t *check_slot(const SweepArgs &sa, int s, int k, int i)
+{
+    return sa.check + ((size_t)(s & 1) * kSubLists + k) * sa.cap_check + i;
+}
+
+__device__ __forceinline__ uint64_t *search_slot(const SweepArgs &sa, int s, int k, int i)
+{
+    return sa.search + ((size_t)(s & 1) * kSubLists + k) * sa.cap_search + i;
+}
+
+enum { kClaimSweep0 = 0, kClaimCheck = 1, kClaimSearch = 2, kClaimFinal = 3, kClaimExit = 4 };
+
+// Where a workgroup's claims stand (in LDS; only thread 0 writes it).
+struct ClaimState {
+    int phase;                  // 0: sweep 0, 1: check, 2: search, 3: final pass
+    int s;                      // current sweep
+    int total_chunks;           // check phase: chunks over all (sub-)lists
+    int chunks[kSubLists];      // check phase: chunks per (sub-)list
+    int tickets[kSubLists];     // search phase: tickets per sub-list
+    int done0[kLists];          // sweep 0: this workgroup's finished tickets per list, not yet counted
+    int b_kind, b_next, b_end, b_list;   // the claimed batch: kind, next and end item, list
+    uint32_t b_tk[kMaxClaimBatch];       // its tickets, in the order they are run
+};
+
+// Thread 0: the check phase of sweep s, or the final pass if no MB is
+// listed for it (the MB lists are final: sweep s - 1 has finished).  Sweep
+// 1 checks every MB, taking the final pass's static lists in chunks.
+__device__ void enter_check(ClaimState &c, const uint32_t *work, const SweepArgs &sa, int s)
+{
+    int any = 0;
+    c.total_chunks = 0;
+    for (int k = 0; k < kSubLists; k++) {
+        int n;
+        if (s == 1)
+            n = k < (int)work[0] ? (int)work[kWorkFinal + k + 1] - (int)work[kWorkFinal + k] : 0;
+        else
+            n = s < sa.max_sweeps ? min(ld_relaxed32(sub_ctr(sa, s, k) + kSubCheckTail), sa.cap_check) : 0;
+        c.chunks[k] = (n + kCheckChunk - 1) / kCheckChunk;
+        c.total_chunks += c.chunks[k];
+        any += n;
+    }
+    c.s = s;
+    c.phase = any > 0 ? 1 : 3;
+}
+
+// Thread 0: the search phase of sweep c.s, or the final pass if its check
+// phase changed no predictor (then no word changed either).
+__device__ void enter_search(ClaimState &c, const SweepArgs &sa)
+{
+    int any = 0;
+    for (int k = 0; k < kSubLists; k++) {
+        c.tickets[k] = min(ld_relaxed32(sub_ctr(sa, c.s, k) + kSubSearchTail), sa.cap_search);
+        any += c.tickets[k];
+    }
+    c.phase = any > 0 ? 2 : 3;
+}
+
+// Thread 0, once the static lists of sweep 0 are all taken: count the
+// workgroup's finished tickets (kept per list in LDS, so that no ticket pays
+// an atomic round trip) after its stores completed; the count that finishes
+// the last list sets the sweep's flag.  (Wave 0 makes every global store of
+// a ticket.)
+__device__ void flush_sweep0(ClaimState &c, const uint32_t *work, int32_t *counters, const SweepArgs &sa)
+{
+    stores_done();
+    for (int l = 0; l < (int)work[0]; l++) {
+        const int n = c.done0[l];
+        if (n == 0) continue;
+        c.done0[l] = 0;
+        const int size = (int)work[2 + l] - (int)work[1 + l];
+        if (fetch_add(counters + kCounterList + l * kCounterListStride + kCounterDone, n) + n == size &&
+            fetch_add(sa.sctr + kSwSubDone, 1) == (int)work[kWorkNonEmpty] - 1) {
+            st_relaxed64(reinterpret_cast<uint64_t *>(sa.sctr + kSwTime), __builtin_amdgcn_s_memrealtime());
+            st_relaxed32(sa.sctr + kSwFlag, 1);
+        }
+    }
+}
+
+// Thread 0: stage a claimed batch of n items from index t of its list (an
+// item is a ticket, or for the check phase a chunk).  Tickets run in two
+// passes: first those that never wait on another MB's decision (every
+// reference but the last), then the designated ones (the last reference),
+// which wait only on tickets of the first kind.  A designated ticket at the
+// head of a batch would otherwise hold back the batch's tail, which the
+// designated ticket of another wave's batch may wait on, and so on along a
+// chain of waves.  Returns false if a dynamic entry never came.
+__device__ bool stage_batch(ClaimState &c, const DevJob *jobs, const uint32_t *work, const SweepArgs &sa,
+                            int kind, int t, int n, int list)
+{
+    c.b_kind = kind;
+    c.b_list = list;
+    c.b_next = 0;
+    c.b_end = n;
+    if (kind == kClaimCheck) {
+        for (int i = 0; i < n; i++) c.b_tk[i] = (uint32_t)(t + i);
+        return true;
+    }
+    uint32_t tk[kMaxClaimBatch];
+    bool ok = true;
+    for (int i = 0; i < n; i++) {
+        if (kind == kClaimSearch) {
+            if (!take_entry(search_slot(sa, c.s, list, t + i), c.s, tk[i])) {
+                ok = false;
+                n = i;
+                break;
+            }
+        } else {
+            tk[i] = work[kWorkHeader + t + i];
+        }
+    }
+    if (kind == kClaimFinal) {
+        for (int i = 0; i < n; i++) c.b_tk[i] = tk[i];
+        c.b_end = n;
+        return ok;
+    }
+    int m = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int i = 0; i < n; i++) {
+            const int ref = (int)((tk[i] >> kRefShift) & 3);
+            const bool designated = ref == jobs[tk[i] >> kJobShift].n_refs - 1;
+            if (designated == (pass == 1)) c.b_tk[m++] = tk[i];
+        }
+    c.b_end = m;
+    return ok;
+}
+
+// Thread 0: the next item of the staged batch, if any, out = {kind, ticket
+// or chunk, sweep, list}.
+__device__ bool batch_item(ClaimState &c, uint32_t out[4])
+{
+    if (c.b_next >= c.b_end) return false;
+    out[0] = (uint32_t)c.b_kind;
+    out[1] = c.b_tk[c.b_next++];
+    out[2] = c.b_kind == kClaimSearch || c.b_kind == kClaimCheck ? (uint32_t)c.s : 0u;
+    out[3] = (uint32_t)c.b_list;
+    return true;
+}
+
+// Thread 0: the workgroup's next item, out = {kind, ticket or chunk, sweep,
+// list}.  Items are claimed in batches of sa.batch consecutive ones (a
+// device-scope atomic on one address serialises, so a claim per item would
+// bound the rate); a wave works through its batch in order, and since
+// dependencies point to earlier items, the wave holding the earliest
+// unfinished item never waits on another.  Static lists (sweep 0 and the
+// final pass) and sub-lists are taken own first, then the others'; between
+// phases it waits for the finished flag of the last.
+__device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, int32_t *counters,
+                      const SweepArgs &sa, int home, uint32_t out[4])
+{
+    if (batch_item(c, out)) return;
+    const int n_lists = (int)work[0];
+    const int kb = min(max(sa.batch, 1), kMaxClaimBatch);
+    for (;;) {
+        if (c.phase == 0 || c.phase == 3) {
+            const int hdr = c.phase == 0 ? 1 : kWorkFinal;
+            const int ctr = c.phase == 0 ? 0 : kCounterFinal;
+            const int h = home % n_lists;
+            for (int k = 0; k < n_lists; k++) {
+                const int l = h + k < n_lists ? h + k : h + k - n_lists;
+                const int lo = (int)work[hdr + l], hi = (int)work[hdr + l + 1];
+                int32_t *next = counters + kCounterList + l * kCounterListStride + ctr;
+                // Another list's counter is read before it is bumped (a
+                // stealing wave should not add traffic to a busy counter);
+                // the own list's is bumped directly.
+                if (k > 0 && lo + ld_relaxed32(next) >= hi) continue;
+                const int t = lo + fetch_add(next, kb);
+                if (t < hi) {
+                    if (c.phase == 0 && l == 0 && t == lo)
+                        st_relaxed64(reinterpret_cast<uint64_t *>(sa.sctr + kSwStart),
+                                     __builtin_amdgcn_s_memrealtime());
+                    stage_batch(c, jobs, work, sa, c.phase == 0 ? kClaimSweep0 : kClaimFinal, t,
+                                min(t + kb, hi) - t, l);
+                    batch_item(c, out);
+                    return;
+                }
+            }
+            if (c.phase == 3) break;
+            flush_sweep0(c, work, counters, sa);
+            if (!wait_flag(sa.sctr + kSwFlag)) break;
+            enter_check(c, work, sa, 1);
+            continue;
+        }
+        if (c.phase == 1) {
+            for (int i = 0; i < kSubLists; i++) {
+                const int k = (home + i) & (kSubLists - 1);
+                if (c.chunks[k] == 0) continue;
+                int32_t *next = c.s == 1 ? counters + kCounterList + k * kCounterListStride + kCounterCheck1
+                                         : sub_ctr(sa, c.s, k) + kSubChunkHead;
+                if (ld_relaxed32(next) >= c.chunks[k]) continue;
+                const int j = fetch_add(next, 1);
+                if (j < c.chunks[k]) {
+                    stage_batch(c, jobs, work, sa, kClaimCheck, j, 1, k);
+                    batch_item(c, out);
+                    return;
+                }
+            }
+            if (!wait_flag(sa.sctr + (size_t)c.s * kSweepStride + kSwFlagC)) break;
+            enter_search(c, sa);
+            continue;
+        }
+        for (int i = 0; i < kSubLists; i++) {
+            const int k = (home + i) & (kSubLists - 1);
+            if (c.tickets[k] == 0) continue;
+            int32_t *next = sub_ctr(sa, c.s, k) + kSubSearchHead;
+            if (i > 0 && ld_relaxed32(next) >= c.tickets[k]) continue;
+            const int t = fetch_add(next, kb);
+            if (t < c.tickets[k]) {
+                if (!stage_batch(c, jobs, work, sa, kClaimSearch, t, min(t + kb, c.tickets[k]) - t, k))
+                    atomicOr(&counters[kCounterError], kErrWaitTimeout);
+                if (batch_item(c, out)) return;
+                break;
+            }
+        }
+        if (!wait_flag(sa.sctr + (size_t)c.s * kSweepStride + kSwFlag)) break;
+        enter_check(c, work, sa, c.s + 1);
+    }
+    if (c.phase != 3) atomicOr(&counters[kCounterError], kErrWaitTimeout);
+    out[0] = kClaimExit;
+}
+
+// The word a MB of the guess table stands in with (no guess: no motion).
+__device__ __forceinline__ uint64_t guess_word(const DevJob &job, int idx)
+{
+    if (!job.guess) return pred_word(false, 0);
+    const uint32_t *g = reinterpret_cast<const uint32_t *>(job.guess + idx);
+    const uint32_t g0 = g[0], g1 = g[1];
+    return (g0 & EVXA_BLOCK_MOTION) ? pred_word(true, g1) : pred_word(false, 0);
+}
+
+// The words of the left, top, top-right and top-left MBs of (bx, by) and
+// its own, as of sweep s - 1 >= 0, and its predictor from them and the
+// temporal table (compute_mv_predictor, motion.cpp:271-346).  Per lane.
+__device__ void sweep_predictor(const DevJob &job, int mb, int s, int &pmx, int &pmy,
+                                uint64_t &own_old)
+{
+    const int bx = mb % job.wb, by = mb / job.wb;
+    const size_t n_mb = (size_t)job.wb * job.hb;
+    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
+    const bool has_tl = has_t && has_l;
+    const int idx[5] = {has_l ? mb - 1 : -1, has_t ? mb - job.wb : -1,
+                        has_tr ? mb - job.wb + 1 : -1, has_tl ? mb - job.wb - 1 : -1, mb};
+    uint64_t w[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        w[i] = pred_word(false, 0);
+        if (idx[i] >= 0) w[i] = ld_relaxed64(job.sw_words + (size_t)((s - 1) & 1) * n_mb + idx[i]);
+    }
+    uint32_t p0 = 0, p1 = 0;
+    if (job.prev) {
+        const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + (size_t)mb * 16;
+        p0 = prv[0];
+        p1 = prv[1];
+    }
+    mv_predictor(job, bx, by, w[0], w[1], w[2], w[3], p0, p1, pmx, pmy);
+    own_old = w[4];
+}
+
+// The key of a predictor as the search uses it: absent, or the clamped
+// candidate position (motion.cpp:613-635).
+__device__ __forceinline__ uint32_t pred_key(const DevJob &job, int mb, int pmx, int pmy)
+{
+    if (pmx == 0 && pmy == 0) return 0xFFFFFFFFu;
+    const int x = (mb % job.wb) * 16, y = (mb / job.wb) * 16;
+    return (uint32_t)min(max(x + pmx, 0), job.width - 16) |
+           ((uint32_t)min(max(y + pmy, 0), job.height - 16) << 16);
+}
+
+// A MB's word changed in sweep s: list it (for its carry-over) and the MBs
+// whose predictor reads it -- right, bottom-left, bottom, bottom-right --
+// for sweep s + 1, each once (the listed stamp), in sub-list `home`.
+__device__ void push_dependants(const DevJob &job, int jidx, int mb, int s, const SweepArgs &sa,
+                                int32_t *counters, int home)
+{
+    const int lane = lane_id();
+    if (s + 1 >= sa.max_sweeps) {
+        if (lane == 0) atomicOr(&counters[kCounterError], kErrSweepLimit);
+        return;
+    }
+    const int bx = mb % job.wb, by = mb / job.wb;
+    int m2 = -1;
+    if (lane == 0) m2 = mb;
+    else if (lane == 1 && bx + 1 < job.wb) m2 = mb + 1;
+    else if (lane == 2 && bx > 0 && by + 1 < job.hb) m2 = mb + job.wb - 1;
+    else if (lane == 3 && by + 1 < job.hb) m2 = mb + job.wb;
+    else if (lane == 4 && bx + 1 < job.wb && by + 1 < job.hb) m2 = mb + job.wb + 1;
+    bool need = false;
+    if (m2 >= 0)
+        need = __hip_atomic_fetch_max(reinterpret_cast<int32_t *>(job.sw_listed + m2), s + 1,
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < s + 1;
+    const uint64_t mask = __ballot(need);
+    const int cnt = __builtin_popcountll(mask);
+    if (cnt == 0) return;
+    int base = 0;
+    if (lane == 0) base = fetch_add(sub_ctr(sa, s + 1, home) + kSubCheckTail, cnt);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (base + cnt > sa.cap_check) {
+        if (lane == 0) atomicOr(&counters[kCounterError], kErrSweepLimit);
+        return;
+    }
+    if (need) {
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        st_relaxed64(check_slot(sa, s + 1, home, base + rank),
+                     list_entry(s + 1, ((uint32_t)jidx << kJobShift) | (uint32_t)m2));
+    }
+}
+
+// The MB's decision after its searches of sweep s (the designated wave):
+// selection, decision record, word for the next sweep and, from sweep 1 on,
+// its dependants listed if the word changed.  Sweep 0 also records the
+// predictor's key.
+__device__ void decide_mb(const DevJob &job, int jidx, int mb, int s, const SweepArgs &sa,
+                          MbLds &sh, const Blk &src, const Geo &geo, const SearchOut &own,
+                          uint64_t own_old, uint32_t key, int32_t *counters, int home)
+{
+    const int lane = lane_id();
+    const int x = (mb % job.wb) * 16, y = (mb / job.wb) * 16;
+    const size_t n_mb = (size_t)job.wb * job.hb;
+    const uint64_t sw = gather_searches(job, mb, counters, false);
+    auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
+    store_src(src, geo, sh.src);
+    const Sel sel = select_mb(job, x, y, sh, result, counters);
+    const uint64_t w = sel_word(sel, x, y);
+    // The decision (P: the selected search; B: both searches).
+    const bool is_b = job.type != EVXA_FRAME_P;
+    const SearchOut first = is_b ? result(0) : sel.best;
+    const SearchOut second = is_b ? result(1) : sel.best;
+    uint64_t dw;
+    if (lane < 3) dw = pack_search(first, lane);
+    else if (lane < 6) dw = is_b ? pack_search(second, lane - 3) : 0ull;
+    else if (lane == 6)
+        dw = (uint32_t)sel.best_sad | ((uint64_t)(sel.best_k & 3) << 32) | ((uint64_t)sel.bidir << 34);
+    else
+        dw = (uint32_t)sel.n_int | ((uint64_t)(uint16_t)sel.n_ax << 32) |
+             ((uint64_t)(uint16_t)sel.n_dg << 48);
+    if (lane < 8) st_relaxed64(&job.sw_dec[mb].w[lane], dw);
+    if (lane == 0) {
+        st_relaxed64(job.sw_words + (size_t)(s & 1) * n_mb + mb, w);
+        if (s == 0) st_relaxed32(reinterpret_cast<int32_t *>(job.sw_key + mb), (int32_t)key);
+    }
+    // Sweep 1 checks every MB; later sweeps the MBs listed.
+    if (s > 0 && w != own_old) push_dependants(job, jidx, mb, s, sa, counters, home);
+}
+
+// One check chunk of sweep s (one wave): up to kCheckChunk MBs listed in
+// sub-list k, one lane each.  A MB whose predictor key is unchanged keeps
+// its decision and carries its word over; the others get a new key and
+// predictor and their search tickets (all references, the last one
+// designated to decide) in sub-list `home`.  The wave that finishes the
+// phase's last chunk clears sweep s + 2's counters and sets the phase flag.
+__device__ void check_chunk(const DevJob *jobs, const uint32_t *work, int s, int k, int j,
+                            int total_chunks, const SweepArgs &sa, int32_t *counters, int home)
+{
+    const int lane = lane_id();
+    const int i = j * kCheckChunk + lane;
+    uint32_t tk = 0;
+    bool have = false;
+    if (s == 1) {
+        // Every MB: list k of the final pass.
+        const int lo = (int)work[kWorkFinal + k], hi = (int)work[kWorkFinal + k + 1];
+        if (lo + i < hi) {
+            tk = work[kWorkHeader + lo + i];
+            have = true;
+        }
+    } else {
+        const int n = min(ld_relaxed32(sub_ctr(sa, s, k) + kSubCheckTail), sa.cap_check);
+        if (i < n) {
+            have = take_entry(check_slot(sa, s, k, i), s, tk);
+            if (!have) atomicOr(&counters[kCounterError], kErrWaitTimeout);
+        }
+    }
+    int nr = 0, mb = 0, jidx = 0;
+    if (have) {
+        jidx = (int)(tk >> kJobShift);
+        mb = (int)(tk & kMbMask);
+        const DevJob &job = jobs[jidx];
+        int pmx, pmy;
+        uint64_t own_old;
+        sweep_predictor(job, mb, s, pmx, pmy, own_old);
+        const uint32_t key = pred_key(job, mb, pmx, pmy);
+        int32_t *key_p = reinterpret_cast<int32_t *>(job.sw_key + mb);
+        if ((uint32_t)ld_relaxed32(key_p) == key) {
+            st_relaxed64(job.sw_words + (size_t)(s & 1) * job.wb * job.hb + mb, own_old);
+        } else {
+            st_relaxed32(key_p, (int32_t)key);
+            st_relaxed64(job.sw_pred + mb, (1ull << 63) | (uint32_t)(uint16_t)pmx |
+                                               ((uint32_t)(uint16_t)pmy << 16));
+            nr = job.n_refs;
+        }
+    }
+    // Search tickets: an exclusive scan of the lanes' counts.
+    int incl = nr;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    const int n_mbs = __builtin_popcountll(__ballot(nr > 0));
+    if (total > 0) {
+        int base = 0;
+        if (lane == 0) {
+            base = fetch_add(sub_ctr(sa, s, home) + kSubSearchTail, total);
+            fetch_add(sub_ctr(sa, s, home) + kSubDesignated, n_mbs);
+            fetch_add(sa.sctr + (size_t)s * kSweepStride + kSwSearched, n_mbs);
+        }
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base + total > sa.cap_search) {
+            if (lane == 0) atomicOr(&counters[kCounterError], kErrSweepLimit);
+        } else {
+            for (int r = 0; r < nr; r++)
+                st_relaxed64(search_slot(sa, s, home, base + incl - nr + r),
+                             list_entry(s, ((uint32_t)jidx << kJobShift) | ((uint32_t)r << kRefShift) |
+                                               (uint32_t)mb));
+        }
+    }
+    stores_done();
+    int last = 0;
+    if (lane == 0)
+        last = fetch_add(sa.sctr + (size_t)s * kSweepStride + kSwDoneC, 1) == total_chunks - 1;
+    if (!__builtin_amdgcn_readfirstlane(last)) return;
+    // Sweep s + 2's counters are untouched until sweep s + 1's searches.
+    if (s + 2 < sa.max_sweeps) {
+        int32_t *b2 = sa.sctr + (size_t)(s + 2) * kSweepStride;
+        const int sub = lane / 6, f = lane % 6;
+        const int off = f < 4 ? f : (f == 4 ? kSubSearchHead : kSubDone);
+        if (lane < 6 * kSubLists) st_relaxed32(b2 + sub * kSubStride + off, 0);
+        else if (lane == 48) st_relaxed32(b2 + kSwFlagC, 0);
+        else if (lane == 49) st_relaxed32(b2 + kSwDoneC, 0);
+        else if (lane == 50) st_relaxed32(b2 + kSwSubDone, 0);
+        else if (lane == 51) st_relaxed32(b2 + kSwFlag, 0);
+        else if (lane == 52) st_relaxed32(b2 + kSwSearched, 0);
+        stores_done();
+    }
+    if (lane == 0) st_relaxed32(sa.sctr + (size_t)s * kSweepStride + kSwFlagC, 1);
+}
+
+// One (MB, reference) ticket of sweep s, from (sub-)list k.  In sweep 0 the
+// wave of reference 0 forms the MB's predictor from the words of the left,
+// top, top-right and top-left MBs -- words already published in sweep 0
+// (fresh reads, waiting up to sa.wait_ticks for missing ones), else the
+// guess table's -- and publishes it for the MB's other references, so that
+// all of them search under the same one; in later sweeps the MB's check has
+// set it.  The memory stage (source, initial candidate, window) is issued
+// first and overlaps it.  The designated wave (last reference) decides the
+// MB; in sweeps s >= 1 it then counts the MB finished, and the last MB of
+// the last sub-list ends the sweep.  One function for both, so the search
+// is compiled once.
+template <int NW>
+__device__ void search_ticket(const DevJob &job, int jidx, int ref, int mb, int s, int k,
+                              const SweepArgs &sa, Shared &sh, int32_t *counters, int home,
+                              uint32_t *s_pred)
+{
+    const int lane = lane_id();
+    const int bx = mb % job.wb, by = mb / job.wb;
+    const int x = bx * 16, y = by * 16;
+    const size_t n_mb = (size_t)job.wb * job.hb;
+    const bool designated = ref == job.n_refs - 1;
+    const DevPic &rp = job.refs[ref];
+    const bool narrow = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, rp.wide)) != 0;
+    const Geo geo = lane_geometry();
+    Blk src, pre;
+    load_cand(job.src, geo, x, y, src);
+    load_cand(rp, geo, x, y, pre);
+    const WinPos win = stage_window<NW>(rp, x, y, kReachMax, sh.sp);
+    int pmx, pmy;
+    if (s == 0 && ref == 0) {
+        const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
+        const bool has_tl = has_t && has_l;
+        const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
+                       : lane == 1 ? (has_t ? mb - job.wb : -1)
+                       : lane == 2 ? (has_tr ? mb - job.wb + 1 : -1)
+                       : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1) : -1;
+        uint64_t word = 0;
+        if (sa.reads == kSweepReadsFresh) {
+            if (nidx >= 0) word = ld_relaxed64(job.sw_words + nidx);
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            int nap = 1;
+            while (__ballot(nidx >= 0 && (word >> 63) == 0) != 0 &&
+                   __builtin_amdgcn_s_memrealtime() - t0 < (uint64_t)sa.wait_ticks) {
+                for (int q = 0; q < nap; q++) __builtin_amdgcn_s_sleep(1);
+                nap = min(nap * 2, kMaxNap);
+                if (nidx >= 0 && (word >> 63) == 0) word = ld_relaxed64(job.sw_words + nidx);
+            }
+        }
+        if (nidx >= 0 && (word >> 63) == 0) word = guess_word(job, nidx);
+        uint32_t p0 = 0, p1 = 0;
+        if (job.prev) {
+            const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + (size_t)mb * 16;
+            p0 = prv[0];
+            p1 = prv[1];
+        }
+        mv_predictor(job, bx, by, lane_u64(word, 0), lane_u64(word, 1), lane_u64(word, 2),
+                     lane_u64(word, 3), p0, p1, pmx, pmy);
+        pmx = __builtin_amdgcn_readfirstlane(pmx);
+        pmy = __builtin_amdgcn_readfirstlane(pmy);
+        if (job.n_refs > 1 && (NW == 1 || wave_id() == 0) && lane == 0)
+            st_relaxed64(job.sw_pred + mb, (1ull << 63) | (uint32_t)(uint16_t)pmx |
+                                               ((uint32_t)(uint16_t)pmy << 16));
+        if (NW > 1) {
+            // The waves of the workgroup may have seen different words
+            // published: wave 0's predictor is the MB's, handed over in LDS
+            // across window_ready's barrier.
+            if (wave_id() == 0 && lane == 0)
+                *s_pred = (uint32_t)(uint16_t)pmx | ((uint32_t)(uint16_t)pmy << 16);
+        }
+    } else {
+        // Set before this sweep (check), or by reference 0's wave, whose
+        // ticket was claimed earlier, so its wave is running.
+        uint64_t pw = ld_relaxed64(job.sw_pred + mb);
+        int spins = 0;
+        while ((pw >> 63) == 0 && ++spins < kSpinLimit) {
+            __builtin_amdgcn_s_sleep(1);
+            pw = ld_relaxed64(job.sw_pred + mb);
+        }
+        if ((pw >> 63) == 0 && lane == 0) atomicOr(&counters[kCounterError], kErrWaitTimeout);
+        const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pw);
+        pmx = (int16_t)(p & 0xFFFF);
+        pmy = (int16_t)(p >> 16);
+    }
+    uint64_t own_old = 0;
+    if (s > 0 && designated) own_old = ld_relaxed64(job.sw_words + (size_t)((s - 1) & 1) * n_mb + mb);
+    window_ready();
+    if (NW > 1 && s == 0 && ref == 0) {
+        const uint32_t p = (uint32_t)__builtin_amdgcn_readfirstlane((int)*s_pred);
+        pmx = (int16_t)(p & 0xFFFF);
+        pmy = (int16_t)(p >> 16);
+    }
+    const SearchOut o = inter_search<NW>(rp, job.width, job.height, narrow, src, geo, pre, win, x, y,
+                                         job.thresh, job.lambda, pmx, pmy, sh.sp, nullptr);
+    if (NW > 1) __syncthreads();             // both waves are done with the window
+    if (NW > 1 && wave_id() != 0) return;
+    if (!designated) {
+        publish_search(reinterpret_cast<uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref), o);
+        return;
+    }
+    own_old = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(own_old >> 32)) << 32) |
+              (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)own_old);
+    decide_mb(job, jidx, mb, s, sa, sh.mb, src, geo, o, own_old,
+              s == 0 ? pred_key(job, mb, pmx, pmy) : 0u, counters, home);
+    if (s == 0) {
+        if (lane == 0) fetch_add(sa.sctr + kSwSearched, 1);
+        return;
+    }
+    stores_done();
+    if (lane == 0) {
+        int32_t *c = sub_ctr(sa, s, k);
+        if (fetch_add(c + kSubDone, 1) == ld_relaxed32(c + kSubDesignated) - 1) {
+            int nonempty = 0;
+            for (int q = 0; q < kSubLists; q++) nonempty += ld_relaxed32(sub_ctr(sa, s, q) + kSubDesignated) > 0;
+            int32_t *blk = sa.sctr + (size_t)s * kSweepStride;
+            if (fetch_add(blk + kSwSubDone, 1) == nonempty - 1) {
+                st_relaxed64(reinterpret_cast<uint64_t *>(blk + kSwTime), __builtin_amdgcn_s_memrealtime());
+                st_relaxed32(blk + kSwFlag, 1);
+            }
+        }
+    }
+}
+
+// Final pass: a MB's prediction, DCT, quantisation and record from its last
+// decision (one wave).
+__device__ void final_mb(const DevJob &job, int mb, MbLds &sh)
+{
+    const int lane = lane_id();
+    const int bx = mb % job.wb, by = mb / job.wb;
+    const int x = bx * 16, y = by * 16;
+    const uint64_t dw = lane < 8 ? ld_relaxed64(&job.sw_dec[mb].w[lane]) : 0ull;
+    const Geo geo = lane_geometry();
+    Blk src;
+    load_cand(job.src, geo, x, y, src);
+    store_src(src, geo, sh.src);
+    const uint64_t w6 = lane_u64(dw, 6), w7 = lane_u64(dw, 7);
+    Sel sel;
+    sel.best_sad = (int)(uint32_t)w6;
+    sel.best_k = (int)((w6 >> 32) & 3);
+    sel.bidir = (int)((w6 >> 34) & 1);
+    sel.n_int = (int)(uint32_t)w7;
+    sel.n_ax = (int)((w7 >> 32) & 0xFFFF);
+    sel.n_dg = (int)(w7 >> 48);
+    sel.other = SearchOut{x, y, 0, 0, 0, 0.0f, 0, 0, 0, 0, 0, 0, 0};
+    const SearchOut a = unpack_search(dw, 0);
+    if (job.type == EVXA_FRAME_P) {
+        sel.best = a;
+    } else {
+        const SearchOut b = unpack_search(dw, 1);
+        wave_sync();
+        (void)b_predictions(job, a, b, sh);
+        sel.best = (sel.bidir || sel.best_k == 0) ? a : b;
+        if (sel.bidir) sel.other = b;
+        b_choose(sel, sh);
+    }
+    emit_mb(job, bx, by, sh, sel, false);
+}
+
+// The persistent sweep-mode kernel: sweep 0, then check and search phases
+// sweep after sweep, then the final pass.  NW waves per workgroup search one
+// (MB, reference) ticket together; wave 0 runs the per-MB stages.
+template <int NW>
+__global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void sweep_kernel(const DevJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ work,
+                                                        int32_t *counters, SweepArgs sa)
+{
+    __shared__ Shared sh;
+    __shared__ ClaimState cs;
+    __shared__ uint32_t s_claim[4];
+    __shared__ uint32_t s_pred;
+    const int home = (int)(blockIdx.x % kSubLists);
+    if (threadIdx.x == 0) {
+        cs.phase = 0;
+        cs.s = 0;
+        cs.b_next = cs.b_end = 0;
+        for (int l = 0; l < kLists; l++) cs.done0[l] = 0;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t o[4];
+            claim(cs, jobs, work, counters, sa, (int)blockIdx.x, o);
+            s_claim[0] = o[0]; s_claim[1] = o[1]; s_claim[2] = o[2]; s_claim[3] = o[3];
+        }
+        __syncthreads();
+        const int kind = __builtin_amdgcn_readfirstlane((int)s_claim[0]);
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_claim[1]);
+        const int sweep = __builtin_amdgcn_readfirstlane((int)s_claim[2]);
+        const int list = __builtin_amdgcn_readfirstlane((int)s_claim[3]);
+        const int total_chunks = __builtin_amdgcn_readfirstlane(cs.total_chunks);
+        __syncthreads();
+        if (kind == kClaimExit) break;
+        if (kind == kClaimCheck) {
+            if (NW == 1 || wave_id() == 0)
+                check_chunk(jobs, work, sweep, list, (int)e, total_chunks, sa, counters, home);
+            continue;
+        }
+        const int jidx = (int)(e >> kJobShift);
+        const DevJob &job = jobs[jidx];
+        const int mb = (int)(e & kMbMask);
+        const int ref = (int)((e >> kRefShift) & 3);
+        if (kind == kClaimFinal) {
+            if (NW == 1 || wave_id() == 0) final_mb(job, mb, sh.mb);
+        } else {
+            search_ticket<NW>(job, jidx, ref, mb, kind == kClaimSearch ? sweep : 0, list, sa, sh,
+                              counters, home, &s_pred);
+            if (kind == kClaimSweep0 && threadIdx.x == 0) cs.done0[list]++;
+        }
     }
 }
 
@@ -2993,6 +3832,16 @@ hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *cou
     return hipGetLastError();
 }
 
+hipError_t launch_sweep(const DevJob *jobs, const uint32_t *work, int32_t *counters,
+                        const SweepArgs &sa, int32_t grid, int32_t waves, hipStream_t st)
+{
+    if (waves == 2)
+        hipLaunchKernelGGL(sweep_kernel<2>, dim3(grid), dim3(128), 0, st, jobs, work, counters, sa);
+    else
+        hipLaunchKernelGGL(sweep_kernel<1>, dim3(grid), dim3(64), 0, st, jobs, work, counters, sa);
+    return hipGetLastError();
+}
+
 hipError_t launch_recon(const DevJob &job, const int16_t *levels, const DevPic &dst,
                         int32_t *wide, hipStream_t st)
 {
@@ -3137,16 +3986,20 @@ hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t 
     return hipGetLastError();
 }
 
-int32_t analyze_resident_groups(int32_t device, int32_t waves)
+int32_t analyze_resident_groups(int32_t device, int32_t waves, bool sweep)
 {
     int n_cu = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         n_cu <= 0)
         n_cu = 256;
     int per_cu = 0;
-    const hipError_t e = waves == 2
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel<2>, 128, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel<1>, 64, 0);
+    hipError_t e;
+    if (sweep)
+        e = waves == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel<2>, 128, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel<1>, 64, 0);
+    else
+        e = waves == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel<2>, 128, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, analyze_kernel<1>, 64, 0);
     if (e != hipSuccess || per_cu <= 0) per_cu = waves == 2 ? 2 : 4;
     return n_cu * per_cu;
 }

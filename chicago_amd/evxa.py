@@ -62,9 +62,21 @@ COMPLETION_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("batch_frames",
 assert COMPLETION_DTYPE.itemsize == 48
 
 
+MODE_WAVEFRONT = 1
+MODE_SWEEP = 2
+
+
+SWEEP_READS_GUESS = 1
+SWEEP_READS_FRESH = 2
+SWEEP_ORDER_RASTER = 1
+SWEEP_ORDER_DIAGONAL = 2
+
+
 class Tuning(C.Structure):
     _fields_ = [("waves_per_search", C.c_int32), ("max_groups", C.c_int32), ("lists", C.c_int32),
-                ("cap_pct", C.c_int32)]
+                ("cap_pct", C.c_int32), ("mode", C.c_int32), ("sweep_reads", C.c_int32),
+                ("sweep_wait_us", C.c_int32), ("sweep_order", C.c_int32),
+                ("claim_batch", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -88,7 +100,7 @@ EXPORTS = (
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
-    "evxa_set_tuning", "evxa_get_tuning")
+    "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats")
 
 
 class EvxaError(RuntimeError):
@@ -154,6 +166,8 @@ def load_library(path: str = LIB_PATH):
         "evxa_server_destroy": (C.c_int, [vp]),
         "evxa_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "evxa_get_tuning": (C.c_int, [vp, C.POINTER(Tuning), C.POINTER(C.c_int32)]),
+        "evxa_sweep_stats": (C.c_int, [vp, I32P, I32P, C.POINTER(C.c_float), C.c_int32,
+                                       C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -237,9 +251,26 @@ class Context:
     def bytes_used(self) -> int:
         return _lib.evxa_context_bytes(self.h)
 
-    def set_tuning(self, waves_per_search=0, max_groups=0, lists=0, cap_pct=0):
-        t = Tuning(waves_per_search, max_groups, lists, cap_pct)
+    def set_tuning(self, waves_per_search=0, max_groups=0, lists=0, cap_pct=0, mode=0,
+                   sweep_reads=0, sweep_wait_us=-1, sweep_order=0, claim_batch=0):
+        t = Tuning(waves_per_search, max_groups, lists, cap_pct, mode, sweep_reads, sweep_wait_us,
+                   sweep_order, claim_batch)
         _check(_lib.evxa_set_tuning(self.h, C.byref(t)))
+
+    def sweep_stats(self, max_n=4096, times=False):
+        """Per sweep of the last sweep-mode launch: (MBs listed, MBs searched)
+        lists, plus the microseconds from the first ticket to each sweep's
+        end if `times`; empty after a wavefront launch."""
+        t = np.zeros(max_n, np.int32)
+        m = np.zeros(max_n, np.int32)
+        us = np.zeros(max_n, np.float32)
+        n = C.c_int32()
+        _check(_lib.evxa_sweep_stats(self.h, _p32(t), _p32(m),
+                                     us.ctypes.data_as(C.POINTER(C.c_float)), max_n, C.byref(n)))
+        k = min(n.value, max_n)
+        if times:
+            return t[:k].tolist(), m[:k].tolist(), [round(float(x), 1) for x in us[:k]]
+        return t[:k].tolist(), m[:k].tolist()
 
     def tuning(self):
         """(current Tuning as a dict, workgroups of that shape resident at once)."""

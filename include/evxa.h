@@ -255,17 +255,60 @@ int evxa_dct_quant_blocks(evxa_context *ctx, int32_t n, const int16_t *src,
 /* ---- launch shape (no counterpart in the reference: its analysis runs on
  *      CPU threads, encode.cpp:817-941).  Set by the autotuner. ---------- */
 
+/* How a launch resolves the MV predictor's dependency on the left, top and
+ * top-right MBs' final records (compute_mv_predictor, motion.cpp:271-346).
+ * Both give the reference's sequential records bit for bit.
+ *   WAVEFRONT: MBs run in anti-diagonal order, each waiting for its
+ *              neighbours' published results (one search per MB and
+ *              reference; latency bound by the anti-diagonal chain).
+ *   SWEEP:     Jacobi sweeps to the unique fixpoint of "each MB's record is
+ *              its searches under its neighbours' records' predictor":
+ *              sweep 0 searches every MB with the temporal table standing in
+ *              for the neighbours, each later sweep searches again only MBs
+ *              whose predictor changed, then a final pass runs prediction,
+ *              DCT and quantisation once per MB.  No MB waits on another. */
+#define EVXA_MODE_WAVEFRONT 1
+#define EVXA_MODE_SWEEP 2
+
+/* Order of sweep 0's tickets: each frame's MBs in raster order, or in the
+ * wavefront's anti-diagonal order (with sweep_reads 2, most MBs then find
+ * their neighbours' MVs decided, and a bounded wait does the rest). */
+#define EVXA_SWEEP_ORDER_RASTER 1
+#define EVXA_SWEEP_ORDER_DIAGONAL 2
+
 typedef struct evxa_tuning {
     int32_t waves_per_search;     /* 1 or 2 waves search one (MB, reference) pair (0: keep) */
     int32_t max_groups;           /* workgroups per launch (0: all that fit the device) */
-    int32_t lists;                /* work lists per launch, 1..8 (0: one per XCD for batches) */
-    int32_t cap_pct;              /* tickets per list and diagonal, % of its workgroups (0: 60) */
+    int32_t lists;                /* work lists per launch, 1..8 (0: one per XCD for batches
+                                     and sweeps) */
+    int32_t cap_pct;              /* wavefront: tickets per list and diagonal, % of its
+                                     workgroups (0: 60) */
+    int32_t mode;                 /* EVXA_MODE_WAVEFRONT or EVXA_MODE_SWEEP (0: keep) */
+    int32_t sweep_reads;          /* sweep 0's neighbour MVs: 1 the temporal table's only (a
+                                     deterministic iteration), 2 also those already decided
+                                     in sweep 0 (fewer re-searches; 0: keep) */
+    int32_t sweep_wait_us;        /* with sweep_reads 2: longest wait for a neighbour's
+                                     sweep-0 MV, microseconds (-1: keep) */
+    int32_t sweep_order;          /* EVXA_SWEEP_ORDER_* (0: keep) */
+    int32_t claim_batch;          /* sweep mode: consecutive tickets a workgroup claims at
+                                     once, 1..16 (0: keep) */
 } evxa_tuning;
 
 int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t);
 /* Current tuning; *resident_groups (optional) receives the workgroups of
  * the current shape that fit the device at once. */
 int evxa_get_tuning(evxa_context *ctx, evxa_tuning *t, int32_t *resident_groups);
+
+/* The sweeps of the context's last sweep-mode launch (blocks until it is
+ * done): for sweep k < *n_sweeps (at most `max` reported), listed[k] the MBs
+ * whose predictor it checked (sweep 0: all), searched[k] the MBs it searched
+ * and, if done_us is not NULL, done_us[k] the microseconds from the launch's
+ * first ticket to the end of that sweep's searches (the device's constant
+ * 100 MHz clock; -1 for a sweep whose checks changed no predictor).
+ * *n_sweeps is 0 after a wavefront launch.  Diagnostic; no counterpart in
+ * the reference. */
+int evxa_sweep_stats(evxa_context *ctx, int32_t *listed, int32_t *searched, float *done_us,
+                     int32_t max, int32_t *n_sweeps);
 
 /* ---- serving: continuous batching (no counterpart in the reference, whose
  *      encoder analyses one frame at a time inside engine_encode_frame,

@@ -469,5 +469,128 @@ class Reference(_Lib, _Recon):
         return _run_frame(self._frame, ftype, quality, src, refs, prev, target_ids)
 
 
+def _words_table(words):
+    """A record table holding only block types (motion bit) and MVs."""
+    t = np.zeros(len(words), RECORD_DTYPE)
+    for i, (m, mx, my) in enumerate(words):
+        t["block_type"][i] = 2 if m else 0
+        t["motion_x"][i] = mx
+        t["motion_y"][i] = my
+    return t
+
+
+def sweep_model(o, ftype, quality, src, refs, prev=None):
+    """Model of the GPU's sweep mode on the oracle's searches: Jacobi sweeps
+    to the fixpoint of "each MB's record is its searches under the predictor
+    of its neighbours' records" (compute_mv_predictor, motion.cpp:271-346,
+    read by perform_inter_search only through motion.cpp:613-635).
+
+    Sweep 0 searches every MB with the temporal table (P frames) standing in
+    for the neighbours (the GPU's deterministic setting, sweep_reads 1);
+    sweep s reads every MB's word (motion bit, MV) as of sweep s - 1,
+    searches a listed MB again only if its predictor key (absent, or the
+    clamped candidate position) changed, and a MB whose word changed lists
+    itself and its right, bottom-left, bottom and bottom-right neighbours for
+    sweep s + 1; sweep 1 lists every MB.  Selection follows encode.cpp:183-248 (P,
+    against an empty sentinel) and 386-443 (B).
+
+    Returns (final words per MB as (motion, mv_x, mv_y), MBs listed per sweep
+    (sweep 0: all), MBs searched per sweep); the words must equal the
+    sequential records'."""
+    wb, hb = src.width // 16, src.height // 16
+    n = wb * hb
+    W, H = src.width, src.height
+    lam = np.float32(o.lam(quality))
+    tprev = prev if ftype == 1 else None
+
+    def word_of_rec(r):
+        return (1, int(r["motion_x"]), int(r["motion_y"])) if r["block_type"] & 2 else (0, 0, 0)
+
+    guess = [word_of_rec(r) for r in tprev] if tprev is not None else [(0, 0, 0)] * n
+
+    def decide(m, pmx, pmy):
+        x, y = (m % wb) * 16, (m // wb) * 16
+        blk = src.block(x, y)
+        res = [o.inter_search(blk, r, x, y, quality, pmx, pmy) for r in refs]
+        if ftype == 1:
+            best, best_sad, best_copy = -1, 2**31 - 1, False
+            for k, r in enumerate(res):
+                bias = int(lam * np.float32((k + 1) * 2))
+                copy = (r.block_type & 4) != 0
+                if copy != best_copy:
+                    if copy:
+                        best, best_sad, best_copy = k, r.sad, True
+                elif r.sad + bias < best_sad:
+                    best, best_sad, best_copy = k, r.sad, copy
+            r = res[best]
+            return (1, r.motion_x, r.motion_y) if r.block_type & 2 else (0, 0, 0)
+        f, b = res
+        best, best_sad = -1, 2**31 - 1
+        if f.sad < best_sad:
+            best, best_sad = 0, f.sad
+        if b.sad < best_sad:
+            best, best_sad = 1, b.sad
+        bsad, _ = o.bidir(blk, o.predict(refs[0], x, y, f), o.predict(refs[1], x, y, b))
+        if bsad + int(lam * np.float32(8.0)) < best_sad:
+            return (1, f.motion_x, f.motion_y)
+        r = res[best]
+        return (1, r.motion_x, r.motion_y) if r.block_type & 2 else (0, 0, 0)
+
+    def key_of(m, pmx, pmy):
+        if pmx == 0 and pmy == 0:
+            return None
+        x, y = (m % wb) * 16, (m // wb) * 16
+        return (min(max(x + pmx, 0), W - 16), min(max(y + pmy, 0), H - 16))
+
+    def dependants(m):
+        bx, by = m % wb, m // wb
+        out = [m]
+        if bx + 1 < wb:
+            out.append(m + 1)
+        if by + 1 < hb:
+            if bx > 0:
+                out.append(m + wb - 1)
+            out.append(m + wb)
+            if bx + 1 < wb:
+                out.append(m + wb + 1)
+        return out
+
+    words = [list(guess), list(guess)]
+    keys = [None] * n
+    n_listed, searched = [], []
+    tab = _words_table(guess)
+    for m in range(n):
+        pmx, pmy = o.mv_predictor(tab, wb, m, tprev)
+        keys[m] = key_of(m, pmx, pmy)
+        words[0][m] = decide(m, pmx, pmy)
+    n_listed.append(n)
+    searched.append(n)
+    listed = set(range(n))      # sweep 1 checks every MB
+    s = 1
+    while listed:
+        old = words[(s - 1) & 1]
+        cur = words[s & 1]
+        tab = _words_table(old)
+        nxt = set()
+        cnt = 0
+        for m in sorted(listed):
+            pmx, pmy = o.mv_predictor(tab, wb, m, tprev)
+            k = key_of(m, pmx, pmy)
+            if k == keys[m]:
+                cur[m] = old[m]
+                continue
+            keys[m] = k
+            cnt += 1
+            w = decide(m, pmx, pmy)
+            cur[m] = w
+            if w != old[m]:
+                nxt.update(dependants(m))
+        n_listed.append(len(listed))
+        searched.append(cnt)
+        listed = nxt
+        s += 1
+    return words[(s - 1) & 1], n_listed, searched
+
+
 def reference_available():
     return os.path.exists(REF_SO)

@@ -1,0 +1,69 @@
+"""Frame analysis at the benchmark workloads' full sizes, in both modes,
+bit-exact against the oracle: the 1280x720 single-reference P pair of the
+CPU-runnable workload, and 1920x1080 P frames with four references and the
+temporal table (the plain GPU workload)."""
+import numpy as np
+import pytest
+
+from chicago_amd import evxa, synth
+from oracle.oracle import Picture, sweep_model
+
+from test_gpu_parity import assert_records_equal
+
+pytestmark = pytest.mark.gpu
+
+MODES = [pytest.param(evxa.MODE_WAVEFRONT, id="wavefront"), pytest.param(evxa.MODE_SWEEP, id="sweep")]
+
+
+@pytest.fixture
+def set_mode(ctx):
+    before = ctx.tuning()[0]
+
+    def apply(mode, waves=2, reads=evxa.SWEEP_READS_FRESH):
+        ctx.set_tuning(waves_per_search=waves, mode=mode, sweep_reads=reads)
+    yield apply
+    ctx.set_tuning(**before)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_c0_pair_1280x720(ctx, oracle, set_mode, mode):
+    # Sweep 0 reads the guess table only, so the sweeps are deterministic
+    # and checked against the model below.
+    set_mode(mode, reads=evxa.SWEEP_READS_GUESS)
+    ref, cur = synth.pair_c0(1, 1280, 720)
+    s = ctx.open_stream(1280, 720, 2)
+    s.upload(0, *cur)
+    s.upload(1, *ref)
+    s.analyze_p(8, 0, [1])
+    rec, co, lv = s.results()
+    s.close()
+    want = oracle.analyze_frame(1, 8, Picture(*cur), [Picture(*ref)])
+    assert_records_equal(rec, want[0])
+    assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2])
+    if mode == evxa.MODE_SWEEP:
+        _, listed, searched = sweep_model(oracle, 1, 8, Picture(*cur), [Picture(*ref)])
+        assert ctx.sweep_stats() == (listed, searched)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_c1_1080p_four_refs_temporal(ctx, oracle, set_mode, mode):
+    set_mode(mode)
+    frames = synth.sequence_c1(1, 7, 1920, 1080)
+    h, w = frames[0][0].shape
+    assert (w, h) == (1920, 1088)
+    s = ctx.open_stream(w, h, len(frames))
+    for k, f in enumerate(frames):
+        s.upload(k, *f)
+    prev = None
+    for t in (5, 6):
+        refs = [t - 1 - k for k in range(4)]
+        if t == 5:
+            s.reset_temporal()
+        s.analyze_p(8, t, refs)
+        rec, co, lv = s.results()
+        want = oracle.analyze_frame(1, 8, Picture(*frames[t]), [Picture(*frames[r]) for r in refs],
+                                    prev=prev)
+        assert_records_equal(rec, want[0])
+        assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2]), t
+        prev = want[0]
+    s.close()

@@ -14,6 +14,17 @@ pytestmark = pytest.mark.gpu
 RECORD_FIELDS = [n for n in evxa.RECORD_DTYPE.names if not n.startswith("reserved")]
 
 
+@pytest.fixture(autouse=True, params=[evxa.MODE_WAVEFRONT, evxa.MODE_SWEEP],
+                ids=["wavefront", "sweep"])
+def mode(request, ctx):
+    """Every test of this module runs in both ways of resolving the MV
+    predictor's dependencies; both must give the oracle's outputs."""
+    before = ctx.tuning()[0]
+    ctx.set_tuning(mode=request.param)
+    yield request.param
+    ctx.set_tuning(**before)
+
+
 def _wide_picture(rng, w, h, lo, hi):
     return (rng.integers(lo, hi, (h, w)).astype(np.int16),
             rng.integers(lo, hi, (h // 2, w // 2)).astype(np.int16),
