@@ -207,6 +207,10 @@ struct evxa_stream;
 struct evxa_context {
     int32_t device = 0;
     hipStream_t stream = nullptr;
+    // The allocation-free path's scratch on this device (its address from
+    // hipGetSymbolAddress), and whether the last analysis used it.
+    char *scratch = nullptr;
+    bool scratch_last = false;
     uint64_t budget = 0;
     uint64_t used = 0;
     int32_t resident[2][3] = {};     // [sweep mode][waves]: workgroups of the analysis kernels that fit
@@ -568,6 +572,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool launch = true)
 {
     if (!st) st = ctx->stream;
+    if (st == ctx->stream) ctx->scratch_last = false;
     const bool sweep = ctx->mode == EVXA_MODE_SWEEP;
     const int32_t fit = ctx->resident[sweep ? 1 : 0][ctx->waves];
     const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
@@ -700,6 +705,24 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
     }
 }
 
+// Where the allocation-free path keeps a frame's launch in the scratch: the
+// counter block and predictor words first (fixed offset, so a later call can
+// read the error word), the job descriptor, then the work list; `end` is the
+// bytes used.
+struct NoallocLayout {
+    size_t prog, job, work, end;
+};
+
+NoallocLayout noalloc_layout(size_t n_mb, int32_t n_refs)
+{
+    NoallocLayout l;
+    l.prog = 0;
+    l.job = align_up((kCounterWords + 2 * n_mb) * sizeof(int32_t), 256);
+    l.work = l.job + align_up(sizeof(DevJob), 256);
+    l.end = l.work + (kWorkHeader + n_mb * (size_t)n_refs) * sizeof(uint32_t);
+    return l;
+}
+
 void check_counters(evxa_context *ctx, LaunchRes &r)
 {
     if (!r.progress.p) return;
@@ -715,11 +738,27 @@ void check_counters(evxa_context *ctx, LaunchRes &r)
         fail(EVXA_ERROR_EXECUTION_FAILURE, "analysis kernel: motion search failed");
 }
 
+// The error word of the context's last analysis launch (its queue is idle).
+void check_last_launch(evxa_context *ctx)
+{
+    if (!ctx->scratch_last) {
+        check_counters(ctx, ctx->res);
+        return;
+    }
+    int32_t c[kCounterError + 1];
+    check(hipMemcpy(c, ctx->scratch + noalloc_layout(0, 0).prog, sizeof(c), hipMemcpyDeviceToHost),
+          "read counters");
+    if (c[kCounterError] & kErrWaitTimeout)
+        fail(EVXA_ERROR_OPERATION_TIMEDOUT, "analysis kernel: dependency wait timed out");
+    if (c[kCounterError] & kErrSearchFailed)
+        fail(EVXA_ERROR_EXECUTION_FAILURE, "analysis kernel: motion search failed");
+}
+
 }  // namespace
 
 extern "C" {
 
-int32_t evxa_version(void) { return 101; }
+int32_t evxa_version(void) { return 102; }
 
 const char *evxa_last_error(void) { return g_last_error.c_str(); }
 
@@ -758,6 +797,9 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
             hbm_budget_bytes = (uint64_t)(free_b * 0.9);
         }
         ctx->budget = hbm_budget_bytes;
+        void *sp = nullptr;
+        check(noalloc_scratch(&sp), "hipGetSymbolAddress(scratch)");
+        ctx->scratch = static_cast<char *>(sp);
         for (int m = 0; m < 2; m++)
             for (int w = 1; w <= 2; w++) ctx->resident[m][w] = analyze_resident_groups(device_ordinal, w, m != 0);
         if (const char *g = getenv("EVXA_MODE")) {
@@ -965,6 +1007,59 @@ int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc)
     });
 }
 
+int evxa_analyze_frame_noalloc(evxa_stream *s, const evxa_frame_desc *desc)
+{
+    return guarded([&] {
+        if (!s || !desc) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        if (!ctx->scratch) fail(EVXA_ERROR_INVALID_RESOURCE, "no scratch on this device");
+        set_device(ctx);
+        // Validate and size before the stream's temporal state advances.
+        (void)make_job(s, *desc, false);
+        const size_t n_mb = (size_t)s->wb * s->hb;
+        const NoallocLayout lay = noalloc_layout(n_mb, desc->n_refs);
+        if (lay.end > kNoAllocScratchBytes) {
+            char msg[160];
+            snprintf(msg, sizeof(msg),
+                     "frame needs %zu bytes of scratch, the allocation-free path has %zu",
+                     lay.end, kNoAllocScratchBytes);
+            fail(EVXA_ERROR_CAPACITY_LIMIT, msg);
+        }
+        DevJob j = make_job(s, *desc, true);
+        int32_t *prog = reinterpret_cast<int32_t *>(ctx->scratch + lay.prog);
+        DevJob *jp = reinterpret_cast<DevJob *>(ctx->scratch + lay.job);
+        uint32_t *work = reinterpret_cast<uint32_t *>(ctx->scratch + lay.work);
+        j.mvw = reinterpret_cast<uint64_t *>(prog + kCounterWords);
+        j.eager = j.type == EVXA_FRAME_P && j.n_refs > 1;
+        // As the wavefront sizes one frame's launch: enough workgroups for
+        // the widest anti-diagonal plus slack, at most all that fit.
+        int32_t widest = 0;
+        for (int32_t v = 0; v < s->wb + 2 * (s->hb - 1); v++) {
+            int32_t c = 0;
+            for (int32_t by = 0; by < s->hb; by++) c += (v - 2 * by >= 0 && v - 2 * by < s->wb) ? 1 : 0;
+            widest = std::max(widest, c * j.n_refs);
+        }
+        const int32_t fit = ctx->resident[0][ctx->waves];
+        const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
+        const int32_t grid = std::max(1, std::min(all, std::max(8, widest + widest / 2 + 4)));
+        hipStream_t st = ctx->stream;
+        check(launch_reset(prog, (int32_t)(kCounterWords + 2 * n_mb), st), "reset kernel");
+        check(launch_set_job(j, jp, st), "job copy kernel");
+        check(launch_wave_list(s->wb, s->hb, j.n_refs, work, st), "work list kernel");
+        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+        check(hipEventRecord(ev.first, st), "hipEventRecord");
+        check(launch_analyze(jp, work, prog, grid, ctx->waves, st), "analysis kernel");
+        check(hipEventRecord(ev.second, st), "hipEventRecord");
+        ctx->ring_head++;
+        ctx->last_start = ev.first;
+        ctx->last_stop = ev.second;
+        ctx->scratch_last = true;
+    });
+}
+
 int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams, const evxa_frame_desc *descs,
                        int32_t n)
 {
@@ -995,7 +1090,7 @@ int evxa_synchronize(evxa_context *ctx)
         std::lock_guard<std::mutex> lk(ctx->mu);
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        check_counters(ctx, ctx->res);
+        check_last_launch(ctx);
     });
 }
 
@@ -1039,7 +1134,7 @@ int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeff
         if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        check_counters(ctx, ctx->res);
+        check_last_launch(ctx);
         const size_t n = (size_t)s->wb * s->hb;
         if (records)
             check(hipMemcpy(records, s->table(s->last_table), n * sizeof(evxa_block_record),

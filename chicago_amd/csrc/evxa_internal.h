@@ -250,6 +250,14 @@ hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, in
 hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t *pred,
                                    const int32_t *quality, int16_t *coeffs, int16_t *levels,
                                    int32_t *qp, int32_t *variance, hipStream_t st);
+// Allocation-free analysis: bytes of the per-GPU scratch array, its device
+// address (to be taken once per device, with that device current), the
+// copy of a job descriptor into it and the wavefront work list of one frame
+// (header + tickets, one list).
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
+hipError_t noalloc_scratch(void **ptr);
+hipError_t launch_set_job(const DevJob &job, DevJob *dst, hipStream_t st);
+hipError_t launch_wave_list(int32_t wb, int32_t hb, int32_t n_refs, uint32_t *work, hipStream_t st);
 // Workgroups of the analysis kernel (wavefront or sweep mode) with `waves`
 // waves each that fit the device at once.
 int32_t analyze_resident_groups(int32_t device, int32_t waves, bool sweep);

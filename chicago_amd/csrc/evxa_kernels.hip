@@ -3813,6 +3813,47 @@ __global__ __launch_bounds__(64) void dct_quant_blocks_kernel(int32_t n, const i
     if (lane_id() == 0) { qp[b] = q; variance[b] = v; }
 }
 
+// ------------------------------------------------- allocation-free path --
+//
+// The scratch of an analysis launch -- job descriptor, work list, counters
+// and predictor words -- for callers that cannot allocate: one array per
+// GPU, alive for the process.  Calls that overlap on one GPU share it and
+// race; every call clears what must start at zero.  Host code reaches it
+// only through the address hipGetSymbolAddress gives (noalloc_scratch).
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// Copies a job descriptor (a kernel argument) into the scratch.
+__global__ void set_job_kernel(DevJob job, DevJob *dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = job;
+}
+
+// The work buffer of one frame as the wavefront lists it (one list,
+// anti-diagonal order v = bx + 2 by, MBs of a diagonal by row, a MB's
+// references adjacent): MB (bx, by) is preceded by the MBs (x, y) with
+// x + 2 y < v -- min(max(v - 2 y, 0), wb) of them on row y -- and by those
+// of diagonal v on rows y < by.  One thread per MB writes its tickets.
+__global__ void wave_list_kernel(int32_t wb, int32_t hb, int32_t n_refs, uint32_t *work)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = wb * hb;
+    if (i == 0) {
+        work[0] = 1u;
+        work[1] = 0u;
+        work[2] = (uint32_t)(n * n_refs);
+    }
+    if (i >= n) return;
+    const int bx = i % wb, by = i / wb, v = bx + 2 * by;
+    int before = 0;
+    for (int y = 0; y < hb; y++) {
+        const int c = v - 2 * y;
+        before += c <= 0 ? 0 : (c < wb ? c : wb);
+        if (y < by && c >= 0 && c < wb) before++;
+    }
+    uint32_t *t = work + kWorkHeader + (size_t)before * n_refs;
+    for (int r = 0; r < n_refs; r++) t[r] = ((uint32_t)r << kRefShift) | (uint32_t)i;
+}
+
 }  // namespace
 
 hipError_t launch_reset(int32_t *words, int32_t n, hipStream_t st)
@@ -3983,6 +4024,24 @@ hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t 
 {
     hipLaunchKernelGGL(dct_quant_blocks_kernel, dim3(n), dim3(64), 0, st, n, src, pred, quality,
                        coeffs, levels, qp, variance);
+    return hipGetLastError();
+}
+
+hipError_t noalloc_scratch(void **ptr)
+{
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+hipError_t launch_set_job(const DevJob &job, DevJob *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(set_job_kernel, dim3(1), dim3(64), 0, st, job, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_list(int32_t wb, int32_t hb, int32_t n_refs, uint32_t *work, hipStream_t st)
+{
+    const int n = wb * hb;
+    hipLaunchKernelGGL(wave_list_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wb, hb, n_refs, work);
     return hipGetLastError();
 }
 

@@ -100,7 +100,7 @@ EXPORTS = (
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
-    "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats")
+    "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats", "evxa_analyze_frame_noalloc")
 
 
 class EvxaError(RuntimeError):
@@ -136,6 +136,7 @@ def load_library(path: str = LIB_PATH):
         "evxa_upload_picture": (C.c_int, [vp, C.c_int32, I16P, I16P, I16P, C.c_int32, C.c_int32]),
         "evxa_stream_reset_temporal": (C.c_int, [vp]),
         "evxa_analyze_frame": (C.c_int, [vp, C.POINTER(FrameDesc)]),
+        "evxa_analyze_frame_noalloc": (C.c_int, [vp, C.POINTER(FrameDesc)]),
         "evxa_analyze_batch": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32]),
         "evxa_synchronize": (C.c_int, [vp]),
         "evxa_last_launch_ms": (C.c_int, [vp, C.POINTER(C.c_float)]),
@@ -418,6 +419,11 @@ class Stream:
 
     def analyze(self, desc: FrameDesc):
         _check(_lib.evxa_analyze_frame(self.h, C.byref(desc)))
+
+    def analyze_noalloc(self, desc: FrameDesc):
+        """evxa_analyze_frame_noalloc: the wavefront analysis with its launch
+        scratch in the library's per-GPU array (no allocation)."""
+        _check(_lib.evxa_analyze_frame_noalloc(self.h, C.byref(desc)))
 
     def analyze_p(self, quality, src_slot, ref_slots, **kw):
         self.analyze(frame_desc(FRAME_P, quality, src_slot, ref_slots, **kw))

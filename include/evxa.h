@@ -157,6 +157,17 @@ int evxa_stream_reset_temporal(evxa_stream *s);
 
 /* Enqueue one frame: one kernel launch over the frame's MB wavefront. */
 int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc);
+/* evxa_analyze_frame for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation): the same
+ * outputs, from the MB wavefront, with the launch's scratch -- job
+ * descriptor, work list, counters, predictor words -- in one per-GPU device
+ * array of 1 MiB held for the life of the process, instead of buffers the
+ * context allocates on first use.  It calls no allocating routine.  Frames
+ * whose scratch does not fit (about 3840x2160 with 4 references is the
+ * largest) fail with EVXA_ERROR_CAPACITY_LIMIT.  Calls on one GPU share the
+ * array: calls that overlap on one GPU (from several contexts) race; calls
+ * on one context are serialised.  No counterpart in the reference. */
+int evxa_analyze_frame_noalloc(evxa_stream *s, const evxa_frame_desc *desc);
 /* Enqueue frames of distinct streams as one fused persistent launch. */
 int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams,
                        const evxa_frame_desc *descs, int32_t n);
