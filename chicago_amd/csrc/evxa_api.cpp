@@ -148,10 +148,29 @@ struct PinnedBuf {
     }
 };
 
+// compute_lambda (common.h:264-266), 0.85 * powf(2, (quality - 12) / 3),
+// as the fp32 bits the reference's own build computes for the frame
+// qualities 0..63 (set_quality keeps QPs within them, evx3enc.cpp:64-82):
+// a table, so no libm's powf rounding reaches the RD costs.
+constexpr uint32_t kLambdaBits[64] = {
+    0x3D59999Au, 0x3D891454u, 0x3DACB591u, 0x3DD9999Au, 0x3E091454u, 0x3E2CB591u,
+    0x3E59999Au, 0x3E891455u, 0x3EACB591u, 0x3ED9999Au, 0x3F091455u, 0x3F2CB591u,
+    0x3F59999Au, 0x3F891455u, 0x3FACB591u, 0x3FD9999Au, 0x40091455u, 0x402CB591u,
+    0x4059999Au, 0x40891454u, 0x40ACB591u, 0x40D9999Au, 0x41091454u, 0x412CB591u,
+    0x4159999Au, 0x41891456u, 0x41ACB590u, 0x41D9999Au, 0x42091456u, 0x422CB590u,
+    0x4259999Au, 0x42891456u, 0x42ACB590u, 0x42D9999Au, 0x43091456u, 0x432CB590u,
+    0x4359999Au, 0x43891452u, 0x43ACB593u, 0x43D9999Au, 0x44091452u, 0x442CB593u,
+    0x4459999Au, 0x44891452u, 0x44ACB593u, 0x44D9999Au, 0x45091452u, 0x452CB593u,
+    0x4559999Au, 0x45891452u, 0x45ACB593u, 0x45D9999Au, 0x46091452u, 0x462CB593u,
+    0x4659999Au, 0x46891452u, 0x46ACB593u, 0x46D9999Au, 0x47091452u, 0x472CB593u,
+    0x4759999Au, 0x47891458u, 0x47ACB58Bu, 0x47D9999Au,
+};
+
 float lambda_of(int32_t quality)
 {
-    // common.h:264-266, computed on the host with the same libm powf.
-    return 0.85f * powf(2.0f, ((float)quality - 12.0f) / 3.0f);
+    float f;
+    memcpy(&f, &kLambdaBits[quality & 63], sizeof(f));
+    return f;
 }
 
 // Device-side resources of one launch configuration: job table, work list,
@@ -337,7 +356,7 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
         fail(EVXA_ERROR_INVALIDARG, "P frames take 1..4 references");
     if (d.type == EVXA_FRAME_B && d.n_refs != 2)
         fail(EVXA_ERROR_INVALIDARG, "B frames take 2 references");
-    if (d.quality < 0 || d.quality > 255) fail(EVXA_ERROR_INVALIDARG, "quality");
+    if (d.quality < 0 || d.quality > 63) fail(EVXA_ERROR_INVALIDARG, "quality must be 0..63");
     check_slot(s, d.src_slot);
     DevJob j;
     memset(&j, 0, sizeof(j));
@@ -1520,7 +1539,7 @@ int evxa_search_blocks(evxa_stream *s, int32_t src_slot, int32_t ref_slot, int32
     return guarded([&] {
         if (!s || n < 0 || (n > 0 && (!xs || !ys || !pred_x || !pred_y || !out)))
             fail(EVXA_ERROR_INVALIDARG, "bad argument");
-        if (quality < 0 || quality > 255) fail(EVXA_ERROR_INVALIDARG, "quality");
+        if (quality < 0 || quality > 63) fail(EVXA_ERROR_INVALIDARG, "quality must be 0..63");
         if (n == 0) return;
         check_slot(s, src_slot);
         check_slot(s, ref_slot);

@@ -297,10 +297,11 @@ __device__ __forceinline__ void select_blk(bool take_b, const Blk &a, const Blk 
     }
 }
 
-// Per-lane part of an integer candidate (analysis.h:89-132, 180-202) when
-// every sample lies in [0, 2047] ("narrow" pictures): |differences| <= 2047,
-// so the differences, their sums over a lane's 16 samples and the 4x4
-// Hadamard (intermediates <= 16 * 2047) are exact in packed int16.  The
+// Per-lane part of an integer candidate (analysis.h:89-132, 180-202) for a
+// source / reference pair that narrow_pair accepts (together at most 2048
+// distinct values within [-8192, 8191]): |differences| <= 2047, so the
+// differences, their sums over a lane's 16 samples and the 4x4 Hadamard
+// (intermediates <= 16 * 2047 = 32752) are exact in packed int16.  The
 // Hadamard runs the vertical butterflies first (the set of coefficients does
 // not depend on the pass order); the last horizontal stage uses
 // |x + y| + |x - y| = 2 max(|x|, |y|).
@@ -701,8 +702,13 @@ __device__ __forceinline__ uint32_t group_bits(uint64_t m)
 // SAD and MAD of this lane's samples against the half-pel block of one
 // direction (its plane views) and against the quarter-pel block
 // (full + half + 1) >> 1 (interp.cpp:166-202), per lane before reduction.
-// Narrow pictures (samples in [0, 2047]) keep every half-pel sample in
-// [-1680, 3727], so differences and sums stay exact in packed int16.
+// For a pair narrow_pair accepts (range [lo, hi], hi - lo <= 2047, within
+// [-8192, 8191]), a 6-tap pass overshoots [lo, hi] by at most 10/32 of its
+// span (the negative taps' weight), 640, and the diagonal's second pass by
+// 10/32 of 2047 + 2 * 640 more, 1040: every half-pel sample lies within
+// 1680 of [lo, hi], in [-9872, 9871].  Differences to the source then stay
+// within 2047 + 1680 = 3727 and full + half + 1 within [-18064, 18063], all
+// exact in the packed int16 lanes.
 __device__ __forceinline__ void subpel_metrics(bool narrow, const Blk &s, const Blk &full,
                                                const PlaneView &vy, const PlaneView &vu,
                                                const PlaneView &vv, const Geo &g, int &sh, int &mh,

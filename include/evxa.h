@@ -101,7 +101,7 @@ typedef struct evxa_search_result {
 /* One frame of analysis work on a stream. */
 typedef struct evxa_frame_desc {
     int32_t type;                 /* EVXA_FRAME_P or EVXA_FRAME_B */
-    int32_t quality;              /* frame QP (evx3enc.cpp:64-82: q <= 16 maps 1:1) */
+    int32_t quality;              /* frame QP 0..63 (evx3enc.cpp:64-82: q <= 16 maps 1:1) */
     int32_t src_slot;             /* picture slot holding the current frame */
     int32_t n_refs;               /* P: 1..4 (most recent first); B: 2 (fwd, bwd) */
     int32_t ref_slots[EVXA_MAX_REFS];
@@ -127,7 +127,8 @@ int32_t evxa_version(void);
 uint64_t evxa_stream_bytes(int32_t width, int32_t height, int32_t n_slots);
 /* HBM currently charged to streams of this context. */
 uint64_t evxa_context_bytes(const evxa_context *ctx);
-/* fp32 lambda of common.h:264-266 for `quality`, as the kernels use it. */
+/* fp32 lambda of common.h:264-266 for `quality` 0..63, as the kernels use
+ * it: the reference build's own values, compiled in as a table. */
 float evxa_lambda(int32_t quality);
 
 /* ---- streams and picture slots (replaces evx_cache_bank's

@@ -89,10 +89,31 @@ static int32_t clip_i32(int32_t v, int32_t lo, int32_t hi)
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-/* common.h:264-266 */
+/* common.h:264-266: 0.85 * powf(2, (quality - 12) / 3) as the fp32 bits the
+ * reference's own build computes (tests/golden/lambda.npz) for the frame
+ * qualities 0..63 that set_quality produces (evx3enc.cpp:64-82), so that no
+ * host's libm rounding enters the RD costs; other qualities are outside the
+ * path and give NaN. */
+static const uint32_t kLambdaBits[64] = {
+    0x3D59999Au, 0x3D891454u, 0x3DACB591u, 0x3DD9999Au, 0x3E091454u, 0x3E2CB591u,
+    0x3E59999Au, 0x3E891455u, 0x3EACB591u, 0x3ED9999Au, 0x3F091455u, 0x3F2CB591u,
+    0x3F59999Au, 0x3F891455u, 0x3FACB591u, 0x3FD9999Au, 0x40091455u, 0x402CB591u,
+    0x4059999Au, 0x40891454u, 0x40ACB591u, 0x40D9999Au, 0x41091454u, 0x412CB591u,
+    0x4159999Au, 0x41891456u, 0x41ACB590u, 0x41D9999Au, 0x42091456u, 0x422CB590u,
+    0x4259999Au, 0x42891456u, 0x42ACB590u, 0x42D9999Au, 0x43091456u, 0x432CB590u,
+    0x4359999Au, 0x43891452u, 0x43ACB593u, 0x43D9999Au, 0x44091452u, 0x442CB593u,
+    0x4459999Au, 0x44891452u, 0x44ACB593u, 0x44D9999Au, 0x45091452u, 0x452CB593u,
+    0x4559999Au, 0x45891452u, 0x45ACB593u, 0x45D9999Au, 0x46091452u, 0x462CB593u,
+    0x4659999Au, 0x46891452u, 0x46ACB593u, 0x46D9999Au, 0x47091452u, 0x472CB593u,
+    0x4759999Au, 0x47891458u, 0x47ACB58Bu, 0x47D9999Au,
+};
+
 float evo_lambda(int32_t quality)
 {
-    return 0.85f * powf(2.0f, ((float)quality - 12.0f) / 3.0f);
+    float f;
+    uint32_t b = quality >= 0 && quality < 64 ? kLambdaBits[quality] : 0x7FC00000u;
+    memcpy(&f, &b, sizeof(f));
+    return f;
 }
 
 /* -------------------------------------------------------- block access -- */

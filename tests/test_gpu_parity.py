@@ -125,12 +125,18 @@ def _stretch(pics, lo, hi):
                   for c in p) for p in pics]
 
 
-# Sample ranges that select the metric paths: 8-bit content as generated
-# (packed 16-bit path), stretched onto [0, 2047] (packed path at the edge of
-# its range, largest differences), onto [1, 2048] (one past it: 32-bit path),
-# onto negative samples, and onto [-32000, 32000] (differences beyond int16,
-# which the Hadamard input truncates like analysis.h:102).
-RANGE_CASES = [None, (0, 2047), (1, 2048), (-3000, -1000), (-32000, 32000)]
+# Sample ranges that select the metric paths.  A source / reference pair
+# takes the packed 16-bit path when their samples together span at most 2048
+# values (max - min <= 2047) within [-8192, 8191], else the 32-bit path:
+#   packed: 8-bit content as generated; [0, 2047] and [1, 2048] (span 2047,
+#     the largest differences); [-3000, -1000]; and the limits of the window,
+#     [6144, 8191] and [-8192, -6145], where half-pel samples come closest
+#     to the int16 range;
+#   32-bit: [0, 2048] (span 2048, one value too many), [6145, 8192] (one past
+#     the top), and [-32000, 32000] (differences beyond int16, which the
+#     Hadamard input truncates like analysis.h:102).
+RANGE_CASES = [None, (0, 2047), (1, 2048), (-3000, -1000), (6144, 8191), (-8192, -6145),
+               (0, 2048), (6145, 8192), (-32000, 32000)]
 
 
 @pytest.mark.parametrize("span", RANGE_CASES)
@@ -154,6 +160,29 @@ def test_search_blocks_sample_ranges(ctx, oracle, span):
         r = oracle.inter_search(S.block(xs[k], ys[k]), R, xs[k], ys[k], 8, pxs[k], pys[k])
         for f in evxa.SEARCH_DTYPE.names:
             assert got[f][k] == getattr(r, f), (k, f, got[k], xs[k], ys[k], pxs[k], pys[k])
+    s.close()
+
+
+def test_search_blocks_disjoint_ranges(ctx, oracle):
+    """Each picture alone spans 2048 values, the pair together does not: the
+    pair must take the 32-bit path."""
+    w, h = 128, 96
+    ref, cur = _pair(78, w, h)
+    ref = _stretch([ref], 3000, 5047)[0]
+    cur = _stretch([cur], 0, 2047)[0]
+    s = ctx.open_stream(w, h, 2)
+    _upload(s, 0, cur)
+    _upload(s, 1, ref)
+    R, S = Picture(*ref), Picture(*cur)
+    xs = [bx * 16 for by in range(h // 16) for bx in range(w // 16)]
+    ys = [by * 16 for by in range(h // 16) for bx in range(w // 16)]
+    pxs = [3 * (k % 5) - 6 for k in range(len(xs))]
+    pys = [2 * (k % 3) - 2 for k in range(len(xs))]
+    got = s.perform_inter_search(0, 1, 8, xs, ys, pxs, pys)
+    for k in range(len(xs)):
+        r = oracle.inter_search(S.block(xs[k], ys[k]), R, xs[k], ys[k], 8, pxs[k], pys[k])
+        for f in evxa.SEARCH_DTYPE.names:
+            assert got[f][k] == getattr(r, f), (k, f)
     s.close()
 
 
