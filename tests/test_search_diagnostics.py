@@ -1,0 +1,253 @@
+"""The search record's diagnostic fields (mad, cost_bits, n_int, n_axial,
+n_diag), which the reference computes but does not expose, pinned on the CPU.
+
+A brute-force replay of perform_inter_search (motion.cpp:567-679 with its
+scans, motion.cpp:125-261, 381-406 and 450-483) written here in Python over
+the reference build's own block metrics, 6-tap interpolation and lambda
+(oracle/_ref) first reproduces the reference's own records (motion, sub-pel
+fields, SAD) exactly; the oracle's diagnostics must then equal what the
+replay tracked:
+
+* mad: the selection's best_mad, which equals compute_block_mad of the final
+  predicted block (integer or interpolated);
+* cost_bits: the fp32 bits of best_cost, SATD at the chosen integer position
+  plus lambda * (|mvx| + |mvy| + 1), product rounded to fp32 before the sum
+  (no FMA), or the SAD where lambda is 0;
+* n_int: integer candidates evaluated, counting the start position and the
+  predictor but not the re-evaluated centre of each level (its outcome is
+  fixed: it is the level's starting best);
+* n_axial / n_diag: sub-pel directions scanned (each half and quarter), by
+  whether the direction has one or two non-zero components.
+
+Frame records sum the counters over the references searched and carry mad
+and cost of the chosen search (and of the backward one for a bidir MB).
+Skipped where the reference build is absent.  CPU only."""
+import numpy as np
+import pytest
+
+from chicago_amd import synth
+from oracle.oracle import Picture, reference_available
+
+pytestmark = pytest.mark.skipif(not reference_available(), reason="oracle/_ref not built")
+
+MB = 16
+SAD_THRESHOLD = 8 * 1024      # motion.cpp:24
+SEARCH_RADIUS = 16            # motion.cpp:29
+INT32_MAX = 0x7FFFFFFF
+
+
+@pytest.fixture(scope="module")
+def ref():
+    from oracle.oracle import Reference
+    return Reference()
+
+
+def _f32_bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def frac_index(i, j):
+    # compute_motion_frac_index_from_direction, motion.cpp:68-91
+    i, j = i + 1, j + 1
+    if j == 0:
+        return i
+    if j == 1:
+        return 3 if i == 0 else 4
+    return i + 5
+
+
+class Replay:
+    """One perform_inter_search, candidate by candidate, with counters."""
+
+    def __init__(self, ref, src_blk, pic, x, y, quality):
+        self.ref, self.src, self.pic = ref, src_blk, pic
+        self.px, self.py = x, y
+        self.thresh = (quality >> 2) + 1
+        self.lam = np.float32(ref.lam(quality))
+        self.max_x, self.max_y = pic.width - MB, pic.height - MB
+        self.n_int = self.n_axial = self.n_diag = 0
+
+    def cost(self, sad, satd, cx, cy):
+        if self.lam > 0:
+            bits = np.float32(abs(cx - self.px) + abs(cy - self.py) + 1)
+            return np.float32(np.float32(satd) + np.float32(self.lam * bits))
+        return np.float32(sad)
+
+    def candidate(self, cx, cy, counted=True):
+        sad, mad, satd = self.ref.metrics(self.src, self.pic.block(cx, cy))
+        self.n_int += counted
+        ssd = (cx - self.px) ** 2 + (cy - self.py) ** 2
+        cost = self.cost(sad, satd, cx, cy)
+        if self.best_mad < self.thresh:
+            take = mad < self.best_mad or (mad == self.best_mad and ssd < self.best_ssd)
+        else:
+            take = (cost < self.best_cost or
+                    (cost == self.best_cost and ssd < self.best_ssd and sad < SAD_THRESHOLD)
+                    or mad < self.thresh)
+        if take:
+            self.bx, self.by = cx, cy
+            self.best_sad, self.best_mad, self.best_ssd, self.best_cost = sad, mad, ssd, cost
+
+    def level(self, step):
+        base_x, base_y = self.bx, self.by
+        for j in (-step, 0, step):
+            for i in (-step, 0, step):
+                cx, cy = base_x + i, base_y + j
+                if not (0 <= cx <= self.max_x and 0 <= cy <= self.max_y):
+                    continue
+                self.candidate(cx, cy, counted=(i, j) != (0, 0))
+                if self.best_mad < self.thresh:
+                    return
+
+    def subpel(self):
+        self.sp = (0, 0, 0)
+        for j in (-1, 0, 1):
+            for i in (-1, 0, 1):
+                if (i, j) == (0, 0):
+                    continue
+                tx, ty = self.bx + i, self.by + j
+                if not (0 <= tx <= self.max_x and 0 <= ty <= self.max_y):
+                    continue
+                if i and j:
+                    self.n_diag += 1
+                else:
+                    self.n_axial += 1
+                blks = self.ref.interp_many(self.pic, [self.bx] * 2, [self.by] * 2, [i] * 2,
+                                            [j] * 2, [0, 1])
+                for quarter in (0, 1):
+                    sad, mad, _ = self.ref.metrics(self.src, blks[quarter])
+                    if self.best_mad < self.thresh:
+                        take = mad < self.best_mad
+                    else:
+                        take = (sad < self.best_sad and sad < SAD_THRESHOLD) or mad < self.thresh
+                    if take:
+                        self.sp = (1, quarter, frac_index(i, j))
+                        self.best_sad, self.best_mad = sad, mad
+
+    def run(self, pmx, pmy):
+        x, y = self.px, self.py
+        self.bx, self.by = x, y
+        self.best_ssd = INT32_MAX
+        sad, mad, satd = self.ref.metrics(self.src, self.pic.block(x, y))
+        self.n_int = 1
+        self.best_sad, self.best_mad = sad, mad
+        self.best_cost = self.cost(sad, satd, x, y)
+        self.sp = (0, 0, 0)
+        if pmx or pmy:
+            self.candidate(min(max(x + pmx, 0), self.max_x), min(max(y + pmy, 0), self.max_y))
+        if self.best_mad >= self.thresh:
+            radius = SEARCH_RADIUS
+            if self.best_sad < SAD_THRESHOLD // 4:
+                radius = SEARCH_RADIUS // 4
+            elif self.best_sad < SAD_THRESHOLD // 2:
+                radius = SEARCH_RADIUS // 2
+            while radius > 0:
+                self.level(radius)
+                if self.best_mad < self.thresh:
+                    break
+                radius >>= 1
+            self.subpel()
+        bt = 0
+        if (self.bx, self.by) != (x, y) or self.sp[0]:
+            bt |= 2
+        if self.best_mad < self.thresh:
+            bt |= 4
+        self.block_type = bt
+        return self
+
+    def decisions(self):
+        return (self.block_type, self.bx - self.px, self.by - self.py) + self.sp + (self.best_sad,)
+
+    def predicted(self):
+        mx, my = self.bx, self.by
+        if not self.sp[0]:
+            return self.pic.block(mx, my)
+        idx = self.sp[2]
+        dx, dy = ((idx - 1, -1) if idx <= 2 else (-1, 0) if idx == 3 else (1, 0) if idx == 4
+                  else (idx - 6, 1))
+        return self.ref.interp_many(self.pic, [mx], [my], [dx], [dy], [self.sp[1]])[0]
+
+
+def _ref_decisions(w):
+    return (int(w["block_type"]), int(w["motion_x"]), int(w["motion_y"]), int(w["sp_pred"]),
+            int(w["sp_amount"]), int(w["sp_index"]), int(w["sad"]))
+
+
+def _cases():
+    """Pairs and qualities that reach every exit of the search: copies found
+    at the start or the predictor, each adaptive radius, the level scan's
+    early return, sub-pel wins and edge clamps."""
+    out = []
+    r_pic, c_pic = synth.pair_c0(4242, 128, 96)
+    out.append(("c0", Picture(*r_pic), Picture(*c_pic)))
+    frames = synth.sequence_c1(4343, 2, 144, 96, local_range=14)
+    out.append(("c1", Picture(*frames[0]), Picture(*frames[1])))
+    rng = np.random.default_rng(4444)
+    y = rng.integers(0, 256, (64, 96)).astype(np.int16)
+    u = rng.integers(0, 256, (32, 48)).astype(np.int16)
+    v = rng.integers(0, 256, (32, 48)).astype(np.int16)
+    out.append(("noise", Picture(y, u, v), Picture(np.roll(y, (2, -3), (0, 1)), u, v)))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("case", range(len(CASES)), ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("quality", [0, 8, 13, 31, 63])
+def test_search_diagnostics_match_replay(oracle, ref, case, quality):
+    _, R, S = CASES[case]
+    rng = np.random.default_rng(quality * 7 + case)
+    exits = set()
+    for yy in range(0, S.height, MB):
+        for xx in range(0, S.width, MB):
+            src = S.block(xx, yy)
+            for pm in ((0, 0), tuple(int(v) for v in rng.integers(-40, 41, 2)),
+                       (int(rng.integers(-3, 4)), 0)):
+                rp = Replay(ref, src, R, xx, yy, quality).run(*pm)
+                w = ref.inter_search(src, R, xx, yy, quality, *pm)
+                assert rp.decisions() == _ref_decisions(w), (xx, yy, pm)
+                o = oracle.inter_search(src, R, xx, yy, quality, *pm)
+                got = (o.mad, o.cost_bits, o.n_int, o.n_axial, o.n_diag)
+                want = (rp.best_mad, _f32_bits(rp.best_cost), rp.n_int, rp.n_axial, rp.n_diag)
+                assert got == want, (xx, yy, pm, got, want)
+                # Closed forms of the final state.
+                sad, mad, _ = ref.metrics(src, rp.predicted())
+                assert (o.sad, o.mad) == (sad, mad), (xx, yy, pm)
+                _, _, satd = ref.metrics(src, R.block(o.best_x, o.best_y))
+                assert o.cost_bits == _f32_bits(rp.cost(o.sad, satd, o.best_x, o.best_y)) or (
+                    rp.lam == 0), (xx, yy, pm)
+                exits.add((o.n_int == 1, o.n_axial + o.n_diag > 0, o.sp_enabled))
+    # Both the search-free copies and full searches with sub-pel scans occur.
+    assert any(e[1] for e in exits), exits
+
+
+@pytest.mark.parametrize("ftype,nrefs,quality", [(1, 1, 8), (1, 3, 11), (2, 2, 9)])
+def test_frame_record_diagnostics(oracle, ref, ftype, nrefs, quality):
+    """Per-MB diagnostics of the oracle's frame driver against the replay,
+    with each MB's predictor taken from the reference's own block table."""
+    frames = synth.sequence_c1(4545 + nrefs, 6, 112, 80, local_range=12)
+    pics = [Picture(*f) for f in frames]
+    if ftype == 1:
+        cur, refs = pics[5], [pics[4 - k] for k in range(nrefs)]
+        prev = ref.analyze_frame(1, quality, pics[4], [pics[3]])[0]
+    else:
+        cur, refs, prev = pics[3], [pics[2], pics[5]], None
+    want, _, _ = ref.analyze_frame(ftype, quality, cur, refs, prev=prev)
+    got, _, _ = oracle.analyze_frame(ftype, quality, cur, refs, prev=prev)
+    wb = cur.width // MB
+    ids = list(range(1, nrefs + 1)) if ftype == 1 else [0, 1]
+    for m in range(len(got)):
+        x, y = (m % wb) * MB, (m // wb) * MB
+        pmx, pmy = ref.mv_predictor(want, wb, m, prev if ftype == 1 else None)
+        reps = [Replay(ref, cur.block(x, y), r, x, y, quality).run(pmx, pmy) for r in refs]
+        g = got[m]
+        assert (int(g["n_int"]), int(g["n_axial"]), int(g["n_diag"])) == (
+            sum(r.n_int for r in reps), sum(r.n_axial for r in reps),
+            sum(r.n_diag for r in reps)), m
+        k = ids.index(int(want[m]["prediction_target"]))
+        assert (int(g["mad"]), int(g["cost_bits"])) == (reps[k].best_mad,
+                                                       _f32_bits(reps[k].best_cost)), m
+        if int(want[m]["prediction_mode"]) == 2:
+            assert (int(g["mad_b"]), int(g["cost_b_bits"])) == (
+                reps[1].best_mad, _f32_bits(reps[1].best_cost)), m
