@@ -252,6 +252,7 @@ struct evxa_context {
     hipEvent_t last_start = nullptr, last_stop = nullptr;
     std::map<int32_t, std::unique_ptr<LaunchRes>> graph_res;
     std::map<int32_t, hipGraphExec_t> graphs;
+    std::map<int32_t, std::vector<evxa_stream *>> graph_streams;
     int32_t next_graph = 1;
     std::vector<evxa_stream *> streams;
 };
@@ -283,6 +284,9 @@ struct evxa_stream {
     DevBuf sao_words;      // per MB: published SAO decision; then the SAO counters
     DevBuf sao_table;      // evxa_sao_info per MB
     uint64_t bytes = 0;
+    // Recorded after every asynchronous job that writes the stream's slots or
+    // outputs, on the HIP stream that job ran on (evxa_stream_wait).
+    hipEvent_t done = nullptr;
 
     int32_t *range_words(int32_t slot) const { return wide.as<int32_t>() + 2 * slot; }
     int32_t *range_spread(int32_t slot) const
@@ -315,6 +319,12 @@ struct evxa_stream {
 namespace {
 
 void set_device(const evxa_context *ctx) { check(hipSetDevice(ctx->device), "hipSetDevice"); }
+
+// The stream's work so far ends with what was just queued on `st`.
+void mark_done(evxa_stream *s, hipStream_t st)
+{
+    check(hipEventRecord(s->done, st), "hipEventRecord");
+}
 
 void check_slot(const evxa_stream *s, int32_t slot)
 {
@@ -777,7 +787,7 @@ void check_last_launch(evxa_context *ctx)
 
 extern "C" {
 
-int32_t evxa_version(void) { return 102; }
+int32_t evxa_version(void) { return 103; }
 
 const char *evxa_last_error(void) { return g_last_error.c_str(); }
 
@@ -918,6 +928,8 @@ int evxa_stream_open(evxa_context *ctx, int32_t width, int32_t height, int32_t n
         s->wide.alloc((size_t)n_slots * 2 * sizeof(int32_t));
         check(hipMemsetAsync(s->wide.p, 0, (size_t)n_slots * 2 * sizeof(int32_t), ctx->stream),
               "clear range words");
+        check(hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "hipEventCreate");
+        mark_done(s.get(), ctx->stream);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         s->bytes = need;
         ctx->used += need;
@@ -938,6 +950,9 @@ int evxa_stream_close(evxa_stream *s)
         ctx->used -= s->bytes;
         ctx->streams.erase(std::remove(ctx->streams.begin(), ctx->streams.end(), s),
                            ctx->streams.end());
+        for (auto &g : ctx->graph_streams)
+            g.second.erase(std::remove(g.second.begin(), g.second.end(), s), g.second.end());
+        if (s->done) (void)hipEventDestroy(s->done);
         delete s;
     });
 }
@@ -1023,6 +1038,7 @@ int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc)
         set_device(ctx);
         std::vector<DevJob> jobs{make_job(s, *desc, true)};
         enqueue(ctx, ctx->res, jobs, true);
+        mark_done(s, ctx->stream);
     });
 }
 
@@ -1076,6 +1092,7 @@ int evxa_analyze_frame_noalloc(evxa_stream *s, const evxa_frame_desc *desc)
         ctx->last_start = ev.first;
         ctx->last_stop = ev.second;
         ctx->scratch_last = true;
+        mark_done(s, st);
     });
 }
 
@@ -1099,6 +1116,7 @@ int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams, const evx
         jobs.reserve(n);
         for (int32_t a = 0; a < n; a++) jobs.push_back(make_job(streams[a], descs[a], true));
         enqueue(ctx, ctx->res, jobs, true);
+        for (int32_t a = 0; a < n; a++) mark_done(streams[a], ctx->stream);
     });
 }
 
@@ -1110,6 +1128,38 @@ int evxa_synchronize(evxa_context *ctx)
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         check_last_launch(ctx);
+    });
+}
+
+int evxa_stream_wait(evxa_stream *s)
+{
+    return guarded([&] {
+        if (!s) fail(EVXA_ERROR_INVALIDARG, "null stream");
+        hipEvent_t ev;
+        {
+            std::lock_guard<std::mutex> lk(s->ctx->mu);
+            set_device(s->ctx);
+            ev = s->done;
+        }
+        // Outside the lock: other threads keep queueing work meanwhile; the
+        // wait covers what was queued for this stream before the call.
+        check(hipEventSynchronize(ev), "hipEventSynchronize");
+    });
+}
+
+int evxa_stream_query(evxa_stream *s, int32_t *done)
+{
+    return guarded([&] {
+        if (!s || !done) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(s->ctx->mu);
+        set_device(s->ctx);
+        const hipError_t e = hipEventQuery(s->done);
+        if (e == hipErrorNotReady) {
+            *done = 0;
+            return;
+        }
+        check(e, "hipEventQuery");
+        *done = 1;
     });
 }
 
@@ -1261,6 +1311,7 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
         });
         range_end(s, dst_slot);
     }
+    mark_done(s, ctx->stream);
     // The caller's level buffer has been consumed once this returns.
     if (levels) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
@@ -1315,6 +1366,7 @@ int evxa_deblock(evxa_stream *s, int32_t slot, const evxa_block_record *records)
                   "deblock kernel");
         });
         range_end(s, slot);
+        mark_done(s, ctx->stream);
         // The caller's records have been consumed once this returns.
         if (records) check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
@@ -1403,6 +1455,7 @@ int evxa_sao(evxa_stream *s, int32_t slot, int32_t src_slot, int32_t quality, ev
                   "SAO decision kernels");
         });
         sao_apply_slot(s, slot, rec);
+        mark_done(s, ctx->stream);
         if (table)
             check(hipMemcpy(table, s->sao_table.p, n * sizeof(evxa_sao_info), hipMemcpyDeviceToHost),
                   "download SAO table");
@@ -1431,6 +1484,7 @@ int evxa_sao_filter(evxa_stream *s, int32_t slot, const evxa_sao_info *table)
                            s->range_spread(slot), ctx->stream),
               "copy kernel");
         sao_apply_slot(s, slot, rec);
+        mark_done(s, ctx->stream);
     });
 }
 
@@ -1468,6 +1522,7 @@ int evxa_graph_create(evxa_context *ctx, evxa_stream *const *streams, const evxa
         const int32_t id = ctx->next_graph++;
         ctx->graphs[id] = exec;
         ctx->graph_res[id] = std::move(res);
+        ctx->graph_streams[id].assign(streams, streams + n);
         *graph_id = id;
     });
 }
@@ -1481,6 +1536,7 @@ int evxa_graph_launch(evxa_context *ctx, int32_t graph_id)
         if (it == ctx->graphs.end()) fail(EVXA_ERROR_INVALIDARG, "no such graph");
         set_device(ctx);
         check(hipGraphLaunch(it->second, ctx->stream), "hipGraphLaunch");
+        for (evxa_stream *gs : ctx->graph_streams[graph_id]) mark_done(gs, ctx->stream);
         const LaunchRes &r = *ctx->graph_res[graph_id];
         ctx->last_start = r.g_start;
         ctx->last_stop = r.g_stop;
@@ -1499,6 +1555,7 @@ int evxa_graph_destroy(evxa_context *ctx, int32_t graph_id)
         (void)hipGraphExecDestroy(it->second);
         ctx->graphs.erase(it);
         ctx->graph_res.erase(graph_id);
+        ctx->graph_streams.erase(graph_id);
     });
 }
 
@@ -1687,6 +1744,7 @@ void server_launch(evxa_server *srv, evxa_server::Slot &sl, std::vector<evxa_ser
                          hipMemcpyDeviceToHost, sl.st),
           "copy launch flags");
     check(hipEventRecord(sl.t1, sl.st), "hipEventRecord");
+    for (auto &r : reqs) mark_done(r.s, sl.st);
     sl.reqs = std::move(reqs);
     sl.busy = true;
     sl.t_launch = clock_s();

@@ -100,7 +100,8 @@ EXPORTS = (
     "evxa_graph_create", "evxa_graph_launch", "evxa_graph_destroy", "evxa_interp_blocks",
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
-    "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats", "evxa_analyze_frame_noalloc")
+    "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats", "evxa_analyze_frame_noalloc",
+    "evxa_stream_wait", "evxa_stream_query")
 
 
 class EvxaError(RuntimeError):
@@ -139,6 +140,8 @@ def load_library(path: str = LIB_PATH):
         "evxa_analyze_frame_noalloc": (C.c_int, [vp, C.POINTER(FrameDesc)]),
         "evxa_analyze_batch": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32]),
         "evxa_synchronize": (C.c_int, [vp]),
+        "evxa_stream_wait": (C.c_int, [vp]),
+        "evxa_stream_query": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "evxa_last_launch_ms": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "evxa_launch_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32,
                                         C.POINTER(C.c_int32)]),
@@ -410,6 +413,16 @@ class Stream:
             assert records.shape == (self.n_mbs,)
             ptr = records.ctypes.data
         _check(_lib.evxa_deblock(self.h, slot, ptr))
+
+    def wait(self):
+        """Block until this stream's queued jobs are done (other streams'
+        work is not waited for)."""
+        _check(_lib.evxa_stream_wait(self.h))
+
+    def query(self) -> bool:
+        done = C.c_int32()
+        _check(_lib.evxa_stream_query(self.h, C.byref(done)))
+        return bool(done.value)
 
     def reset_temporal(self):
         _check(_lib.evxa_stream_reset_temporal(self.h))

@@ -174,6 +174,17 @@ int evxa_analyze_batch(evxa_context *ctx, evxa_stream *const *streams,
                        const evxa_frame_desc *descs, int32_t n);
 /* Block until the context's queue is idle; reports device-side failures. */
 int evxa_synchronize(evxa_context *ctx);
+/* Block until every job queued so far that writes this stream's slots or
+ * outputs (analysis alone, in a batch, a graph replay or a server launch;
+ * reconstruction, deblocking, SAO) has completed, without waiting for other
+ * streams' work: the wait(event) of a submitted job (the reference's frame
+ * call returns only when its row threads have joined, encode.cpp:934-935).  Device-side
+ * failures of the analysis are reported by evxa_synchronize,
+ * evxa_read_results and server completions.  The stream must stay open
+ * until the wait returns. */
+int evxa_stream_wait(evxa_stream *s);
+/* Non-blocking: *done = 1 when evxa_stream_wait would return at once. */
+int evxa_stream_query(evxa_stream *s, int32_t *done);
 /* Device time (ms) of the last analysis launch, from HIP events around it. */
 int evxa_last_launch_ms(evxa_context *ctx, float *ms);
 /* Waits for the queue, then reports the device times (ms, HIP events around

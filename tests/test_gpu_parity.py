@@ -304,6 +304,43 @@ def test_gop8_hierarchical_sequence(ctx, oracle):
     s.close()
 
 
+def test_reference_gop4_sequence(ctx, oracle):
+    """The reference encoder's own GOP (P anchor + 3 Bs, reference B in the
+    middle, evx3enc.cpp:279-383) over 11 frames, flush included, in decode
+    order: records carry the picture-buffer slots as prediction targets, P
+    anchors chain the previous non-B frame's table."""
+    from chicago_amd import gop
+    n = 11
+    frames = synth.sequence_c2(62, n, 112, 80, local_range=12)
+    h, w = frames[0][0].shape
+    s = ctx.open_stream(w, h, n)
+    for k, f in enumerate(frames):
+        _upload(s, k, f)
+    tables = {}
+    kinds = set()
+    for r in gop.reference_schedule(n, 10):
+        if r.kind == "I":
+            continue
+        cur = Picture(*frames[r.index])
+        refs = [Picture(*frames[k]) for k in r.refs]
+        if r.kind == "P":
+            prev = tables.get(r.temporal_from)
+            if prev is None:
+                s.reset_temporal()
+            s.analyze_p(r.quality, r.index, r.refs, target_ids=r.ref_slots)
+            want = oracle.analyze_frame(1, r.quality, cur, refs, prev=prev, target_ids=r.ref_slots)
+            tables[r.index] = want[0]
+        else:
+            s.analyze_b(r.quality, r.index, r.refs[0], r.refs[1], target_ids=r.ref_slots)
+            want = oracle.analyze_frame(2, r.quality, cur, refs, target_ids=r.ref_slots)
+        rec, co, lv = s.results()
+        assert_records_equal(rec, want[0])
+        assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2]), r
+        kinds.add((r.kind, r.is_reference, r.quality))
+    assert kinds == {("P", True, 10), ("B", True, 11), ("B", False, 12)}
+    s.close()
+
+
 def test_batch_uneven_streams_match_alone(ctx, oracle):
     """Streams of different sizes in one fused launch give the same results
     as each analysed alone, and as the oracle."""

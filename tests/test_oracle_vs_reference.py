@@ -199,3 +199,29 @@ def test_sao_random(oracle, ref, w, h, q):
     big = tuple((p.astype(np.int32) * 3 - 200).astype(np.int16) for p in rec)
     for x, y in zip(oracle.sao_filter_frame(t, *big), ref.sao_filter_frame(t, *big)):
         assert np.array_equal(x, y)
+
+
+def test_reference_gop4_sequence(oracle, ref):
+    """The reference encoder's own GOP schedule (chicago_amd.gop) over 10
+    frames, through the oracle and the reference build, frame by frame, with
+    the picture-buffer slots as prediction targets and the P anchors'
+    temporal tables chained as the schedule says."""
+    from chicago_amd import gop
+    frames = synth.sequence_c2(6006, 10, 112, 80, local_range=12)
+    pics = [Picture(*f) for f in frames]
+    tables = {}
+    for r in gop.reference_schedule(10, 9):
+        if r.kind == "I":
+            continue
+        ftype = 1 if r.kind == "P" else 2
+        prev = tables.get(r.temporal_from) if ftype == 1 else None
+        refs = [pics[k] for k in r.refs]
+        a = oracle.analyze_frame(ftype, r.quality, pics[r.index], refs, prev=prev,
+                                 target_ids=r.ref_slots)
+        b = ref.analyze_frame(ftype, r.quality, pics[r.index], refs, prev=prev,
+                              target_ids=r.ref_slots)
+        for f in REF_FIELDS:
+            assert np.array_equal(a[0][f], b[0][f]), (r, f)
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), r
+        if ftype == 1:
+            tables[r.index] = b[0]
