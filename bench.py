@@ -865,8 +865,8 @@ def run_loop(args):
     levels, deblocked and SAO'd against its source; that picture is the
     next frames' reference, so the decoded picture buffer stays in HBM.
     Sources are resident before the timed region; each frame ends with the
-    SAO call's synchronisation.  The first two frames are checked against
-    the oracle's chain."""
+    SAO call's synchronisation.  Frames 1, 2 and the last timed frame are
+    checked against a CPU chain run after the timed region."""
     from chicago_amd import evxa, synth
 
     w, h = 1920, 1080
@@ -892,29 +892,15 @@ def run_loop(args):
 
     res = {"workload": f"closed loop: {k} 1920x1080 P frames, up to 4 reconstructed references, "
                        f"quality {args.quality}; analysis, reconstruction, deblocking, SAO per frame"}
-    checked = []
-    if not args.no_cpu:
-        from oracle.oracle import Oracle, Picture
-        o = Oracle()
-        recons = [Picture(*frames[0])]
-        prev = None
-        for t in (1, 2):
-            code(t)
-            rec, _, lv = s.results()
-            refs = [recons[t - 1 - j] for j in range(min(4, t))]
-            orec, _, olv = o.analyze_frame(1, args.quality, Picture(*frames[t]), refs,
-                                           prev=prev if t > 1 else None)
-            same = all(np.array_equal(rec[f], orec[f]) for f in rec.dtype.names
-                       if not f.startswith("reserved")) and np.array_equal(lv, olv)
-            filt = o.deblock_frame(orec, *o.reconstruct_frame(1, args.quality, Picture(*frames[t]),
-                                                              refs, orec, olv))
-            _, *want = o.sao_encode_frame(frames[t], filt, o.lam(args.quality))
-            got = s.download(rec_slot(t))
-            checked.append(bool(same and all(np.array_equal(a, b) for a, b in zip(got, want))))
-            recons.append(Picture(*want))
-            prev = orec
-        res["bit_exact_vs_cpu"] = all(checked)
-    start = 3 if checked else 1
+    # Frames 1 and 2 run before the timed region and the last frame ends it;
+    # their records, levels and decoded pictures are compared with a CPU
+    # chain afterwards.
+    device_out = {}
+    for t in (1, 2):
+        code(t)
+        rec, _, lv = s.results()
+        device_out[t] = (rec, lv, s.download(rec_slot(t)))
+    start = 3
     for t in range(start, min(start + 2, k + 1)):
         code(t)                               # warm-up frames
     ctx.synchronize()
@@ -926,6 +912,9 @@ def run_loop(args):
         n += 1
     ctx.synchronize()
     dt = time.perf_counter() - t0
+    if n:
+        rec, _, lv = s.results()
+        device_out[k] = (rec, lv, s.download(rec_slot(k)))
     res["frames_timed"] = n
     res["ms_per_frame"] = round(dt / max(n, 1) * 1e3, 3)
     res["frames_per_s"] = round(n / dt, 2) if n else None
@@ -939,7 +928,48 @@ def run_loop(args):
                                            "sao_apply"), per)}
     s.close()
     ctx.close()
+    if not args.no_cpu:
+        res["bit_exact_vs_cpu"], res["checked_frames"] = loop_cpu_chain(args, frames, device_out)
     return res
+
+
+def loop_cpu_chain(args, frames, device_out):
+    """The closed loop on the CPU: each frame analysed against the CPU's own
+    decoded references, reconstructed, deblocked and SAO'd -- by the
+    reference build (its analysis on the host's threads) where it is built,
+    else by the oracle.  Compares the frames in `device_out` (records,
+    levels and decoded picture)."""
+    from oracle.oracle import Oracle, Picture, Reference, reference_available
+    o = Oracle()
+    ana = Reference() if reference_available() else None
+    q = args.quality
+    recons = [Picture(*frames[0])]
+    prev = None
+    checked, t0 = [], time.perf_counter()
+    fields = [f for f in device_out[1][0].dtype.names if not f.startswith("reserved")
+              and (ana is None or f not in ("mad", "cost_bits", "mad_b", "cost_b_bits", "n_int",
+                                             "n_axial", "n_diag"))]
+    for t in range(1, max(device_out) + 1):
+        refs = [recons[t - 1 - j] for j in range(min(4, t))]
+        cur = Picture(*frames[t])
+        p = prev if t > 1 else None
+        if ana is not None:
+            orec, _, olv = ana.analyze_frame_wpp(1, q, cur, refs, prev=p, threads=args.cpu_threads)
+        else:
+            orec, _, olv = o.analyze_frame(1, q, cur, refs, prev=p)
+        impl = ana or o
+        filt = impl.deblock_frame(orec, *impl.reconstruct_frame(1, q, cur, refs, orec, olv))
+        _, *want = impl.sao_encode_frame(frames[t], filt, impl.lam(q))
+        if t in device_out:
+            rec, lv, pic = device_out[t]
+            same = (all(np.array_equal(rec[f], orec[f]) for f in fields)
+                    and np.array_equal(lv, olv)
+                    and all(np.array_equal(a, b) for a, b in zip(pic, want)))
+            log(f"loop frame {t}: CPU chain bit-exact={same} ({time.perf_counter() - t0:.1f} s)")
+            checked.append((t, bool(same)))
+        recons.append(Picture(*want))
+        prev = orec
+    return all(c for _, c in checked), [t for t, _ in checked]
 
 
 def calibrate_serving(ctx, args):
