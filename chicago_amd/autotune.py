@@ -45,9 +45,10 @@ def _sweep(waves, batch, order=1, wait_us=0):
 SINGLE = ([_wavefront(waves_per_search=w, lists=l) for w, l in itertools.product((1, 2), (1, 8))] +
           [_sweep(w, b) for w, b in itertools.product((1, 2), (1, 2, 4))])
 BATCH = ([_wavefront(waves_per_search=w, lists=8, cap_pct=c)
-          for w, c in itertools.product((1, 2), (150, 250, 400))] +
-         [_sweep(1, 4), _sweep(1, 1, order=2, wait_us=20), _sweep(1, 1, order=2, wait_us=200),
-          _sweep(1, 2, order=2, wait_us=20)])
+          for w, c in itertools.product((1, 2), (100, 150, 250))] +
+         [_wavefront(waves_per_search=1, lists=8, cap_pct=c, max_groups=2048) for c in (150, 250)] +
+         [_sweep(1, 1, order=2, wait_us=w) for w in (0, 5, 20)] +
+         [_sweep(1, 2, order=2, wait_us=0), _sweep(2, 1, order=2, wait_us=0)])
 SERVE = [_wavefront(waves_per_search=w) for w in (1, 2)] + [_sweep(w, 2) for w in (1, 2)]
 
 

@@ -193,6 +193,7 @@ struct LaunchRes {
     int32_t cap_check = 0;           // sweep mode: entries per check sub-list (MBs of the launch)
     int32_t cap_search = 0;          // sweep mode: entries per search sub-list (tickets of the launch)
     int32_t max_sweeps = 0;          // sweep mode: sweep counter blocks
+    int32_t n_final = 0;             // sweep mode: MBs of the final pass
     int32_t sctr_off = 0;            // sweep mode: progress word of sweep block 0
     int32_t reads = kSweepReadsGuess;  // sweep mode: how sweep 0 reads neighbours' words
     int32_t wait_ticks = 0;          // sweep mode: fresh reads' longest wait (10 ns ticks)
@@ -581,7 +582,7 @@ hipError_t launch_staged(LaunchRes &r, hipStream_t st, hipEvent_t ev0, hipEvent_
     if (e != hipSuccess) return e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
     e = r.sweep ? launch_sweep(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.progress.as<int32_t>(),
-                               r.sweep_args(), r.grid, r.waves, st)
+                               r.sweep_args(), r.grid, r.waves, r.n_final, st)
                 : launch_analyze(r.jobs.as<DevJob>(), r.tickets.as<uint32_t>(), r.progress.as<int32_t>(),
                                  r.grid, r.waves, st);
     if (e != hipSuccess) return e;
@@ -643,6 +644,7 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
             // Every workgroup that fits: sweep 0 is throughput work.
             r.grid = n_lists * std::max(1, max_grid / n_lists);
             r.max_sweeps = (int32_t)t[kWorkMaxSweeps];
+            r.n_final = (int32_t)(t[kWorkFinal + t[0]] - t[kWorkFinal]);
         } else {
             // Per list, enough waves for its widest diagonal (plus slack for
             // the MBs still finishing the previous one), never more than fit

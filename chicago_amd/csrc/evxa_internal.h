@@ -121,13 +121,11 @@ constexpr int kWorkMaxSweeps = 25;
 
 // Counter block of a launch: [1] error flags; the next-ticket counter of
 // list k at kCounterList + k * kCounterListStride (own 256-byte block each),
-// and in sweep mode, in the same block, the list's final-pass next-ticket
-// counter (+ kCounterFinal) and its count of finished sweep-0 tickets
-// (+ kCounterDone, a line of its own).
+// and in sweep mode, in the same block, its count of finished sweep-0
+// tickets (+ kCounterDone, a line of its own).
 constexpr int kCounterError = 1;
 constexpr int kCounterList = 64;
 constexpr int kCounterListStride = 64;
-constexpr int kCounterFinal = 16;
 constexpr int kCounterDone = 32;
 constexpr int kCounterCheck1 = 48;      // next chunk of sweep 1's check over the list's MBs
 constexpr int kCounterWords = kCounterList + kLists * kCounterListStride;
@@ -201,7 +199,8 @@ hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *cou
 // The same work in sweep mode (fixpoint of the MV predictor, then a final
 // pass of prediction, DCT and quantisation per MB).
 hipError_t launch_sweep(const DevJob *jobs, const uint32_t *work, int32_t *counters,
-                        const SweepArgs &sa, int32_t grid, int32_t waves, hipStream_t st);
+                        const SweepArgs &sa, int32_t grid, int32_t waves, int32_t n_final,
+                        hipStream_t st);
 // Reconstruction of the MBs of `job` (its records and prediction sources)
 // from `levels` into the picture `dst`; grows the spread range record *wide
 // (clear it first) by the stored samples.

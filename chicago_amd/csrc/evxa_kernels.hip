@@ -77,7 +77,17 @@ constexpr QuantTables make_quant_tables()
 
 __constant__ QuantTables c_quant = make_quant_tables();
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+// The lane index, opaque to the optimiser: lane-dependent constants are then
+// recomputed where they are used instead of being hoisted out of the
+// persistent kernels' loops, where they would stay live across a whole
+// ticket (in the one-wave analysis kernel this took the VGPR count from 256,
+// with spills to scratch, to 136, and the waves per SIMD from 2 to 3).
+__device__ __forceinline__ int lane_id()
+{
+    int l = (int)(threadIdx.x & 63);
+    asm volatile("" : "+v"(l));
+    return l;
+}
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
 
 // Orders one wave's LDS accesses across its lanes: a wave's LDS
@@ -1734,7 +1744,7 @@ __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, 
 // budgets: the one-wave kernel may take 256 VGPRs (2 waves per SIMD); the
 // two-wave kernel fits 168 without spills, so 3 waves share each SIMD.
 template <int NW>
-__global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const DevJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * NW, 3) void analyze_kernel(const DevJob *__restrict__ jobs,
                                                           const uint32_t *__restrict__ work,
                                                           int32_t *counters)
 {
@@ -1858,8 +1868,9 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void analyze_kernel(const
 //  * a MB whose word changes lists itself and the four MBs that read it for
 //    the next sweep, and the iteration ends with a sweep that changes none.
 // A MB at dependency depth d is final after sweep d, so at most the frame's
-// anti-diagonal count plus one sweeps run.  A final pass then forms each
-// MB's prediction, DCT and quantisation once, from its last decision.
+// anti-diagonal count plus one sweeps run.  A final pass (final_kernel, a
+// launch of its own at high occupancy) then forms each MB's prediction, DCT
+// and quantisation once, from its last decision.
 //
 // Words by sweep parity: sweep s reads W[(s - 1) & 1] and writes W[s & 1];
 // after sweep s, W[s & 1] holds every MB's word as of sweep s (a listed MB
@@ -1964,11 +1975,11 @@ __device__ __forceinline__ uint64_t *search_slot(const SweepArgs &sa, int s, int
     return sa.search + ((size_t)(s & 1) * kSubLists + k) * sa.cap_search + i;
 }
 
-enum { kClaimSweep0 = 0, kClaimCheck = 1, kClaimSearch = 2, kClaimFinal = 3, kClaimExit = 4 };
+enum { kClaimSweep0 = 0, kClaimCheck = 1, kClaimSearch = 2, kClaimExit = 4 };
 
 // Where a workgroup's claims stand (in LDS; only thread 0 writes it).
 struct ClaimState {
-    int phase;                  // 0: sweep 0, 1: check, 2: search, 3: final pass
+    int phase;                  // 0: sweep 0, 1: check, 2: search, 3: done
     int s;                      // current sweep
     int total_chunks;           // check phase: chunks over all (sub-)lists
     int chunks[kSubLists];      // check phase: chunks per (sub-)list
@@ -1978,9 +1989,9 @@ struct ClaimState {
     uint32_t b_tk[kMaxClaimBatch];       // its tickets, in the order they are run
 };
 
-// Thread 0: the check phase of sweep s, or the final pass if no MB is
-// listed for it (the MB lists are final: sweep s - 1 has finished).  Sweep
-// 1 checks every MB, taking the final pass's static lists in chunks.
+// Thread 0: the check phase of sweep s, or the end of the sweeps if no MB
+// is listed for it (the MB lists are final: sweep s - 1 has finished).
+// Sweep 1 checks every MB, taking the final pass's static lists in chunks.
 __device__ void enter_check(ClaimState &c, const uint32_t *work, const SweepArgs &sa, int s)
 {
     int any = 0;
@@ -1999,8 +2010,8 @@ __device__ void enter_check(ClaimState &c, const uint32_t *work, const SweepArgs
     c.phase = any > 0 ? 1 : 3;
 }
 
-// Thread 0: the search phase of sweep c.s, or the final pass if its check
-// phase changed no predictor (then no word changed either).
+// Thread 0: the search phase of sweep c.s, or the end of the sweeps if its
+// check phase changed no predictor (then no word changed either).
 __device__ void enter_search(ClaimState &c, const SweepArgs &sa)
 {
     int any = 0;
@@ -2064,11 +2075,6 @@ __device__ bool stage_batch(ClaimState &c, const DevJob *jobs, const uint32_t *w
             tk[i] = work[kWorkHeader + t + i];
         }
     }
-    if (kind == kClaimFinal) {
-        for (int i = 0; i < n; i++) c.b_tk[i] = tk[i];
-        c.b_end = n;
-        return ok;
-    }
     int m = 0;
     for (int pass = 0; pass < 2; pass++)
         for (int i = 0; i < n; i++) {
@@ -2097,9 +2103,9 @@ __device__ bool batch_item(ClaimState &c, uint32_t out[4])
 // device-scope atomic on one address serialises, so a claim per item would
 // bound the rate); a wave works through its batch in order, and since
 // dependencies point to earlier items, the wave holding the earliest
-// unfinished item never waits on another.  Static lists (sweep 0 and the
-// final pass) and sub-lists are taken own first, then the others'; between
-// phases it waits for the finished flag of the last.
+// unfinished item never waits on another.  Sweep 0's static lists and the
+// sub-lists are taken own first, then the others'; between phases it waits
+// for the finished flag of the last.
 __device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, int32_t *counters,
                       const SweepArgs &sa, int home, uint32_t out[4])
 {
@@ -2107,30 +2113,28 @@ __device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, i
     const int n_lists = (int)work[0];
     const int kb = min(max(sa.batch, 1), kMaxClaimBatch);
     for (;;) {
-        if (c.phase == 0 || c.phase == 3) {
-            const int hdr = c.phase == 0 ? 1 : kWorkFinal;
-            const int ctr = c.phase == 0 ? 0 : kCounterFinal;
+        // The final pass is a launch of its own (final_kernel).
+        if (c.phase == 3) break;
+        if (c.phase == 0) {
             const int h = home % n_lists;
             for (int k = 0; k < n_lists; k++) {
                 const int l = h + k < n_lists ? h + k : h + k - n_lists;
-                const int lo = (int)work[hdr + l], hi = (int)work[hdr + l + 1];
-                int32_t *next = counters + kCounterList + l * kCounterListStride + ctr;
+                const int lo = (int)work[1 + l], hi = (int)work[2 + l];
+                int32_t *next = counters + kCounterList + l * kCounterListStride;
                 // Another list's counter is read before it is bumped (a
                 // stealing wave should not add traffic to a busy counter);
                 // the own list's is bumped directly.
                 if (k > 0 && lo + ld_relaxed32(next) >= hi) continue;
                 const int t = lo + fetch_add(next, kb);
                 if (t < hi) {
-                    if (c.phase == 0 && l == 0 && t == lo)
+                    if (l == 0 && t == lo)
                         st_relaxed64(reinterpret_cast<uint64_t *>(sa.sctr + kSwStart),
                                      __builtin_amdgcn_s_memrealtime());
-                    stage_batch(c, jobs, work, sa, c.phase == 0 ? kClaimSweep0 : kClaimFinal, t,
-                                min(t + kb, hi) - t, l);
+                    stage_batch(c, jobs, work, sa, kClaimSweep0, t, min(t + kb, hi) - t, l);
                     batch_item(c, out);
                     return;
                 }
             }
-            if (c.phase == 3) break;
             flush_sweep0(c, work, counters, sa);
             if (!wait_flag(sa.sctr + kSwFlag)) break;
             enter_check(c, work, sa, 1);
@@ -2549,10 +2553,11 @@ __device__ void final_mb(const DevJob &job, int mb, MbLds &sh)
 }
 
 // The persistent sweep-mode kernel: sweep 0, then check and search phases
-// sweep after sweep, then the final pass.  NW waves per workgroup search one
-// (MB, reference) ticket together; wave 0 runs the per-MB stages.
+// sweep after sweep (final_kernel follows it).  NW waves per workgroup
+// search one (MB, reference) ticket together; wave 0 runs the per-MB
+// stages.
 template <int NW>
-__global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void sweep_kernel(const DevJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * NW, 3) void sweep_kernel(const DevJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ work,
                                                         int32_t *counters, SweepArgs sa)
 {
@@ -2590,13 +2595,30 @@ __global__ __launch_bounds__(64 * NW, NW == 2 ? 3 : 2) void sweep_kernel(const D
         const DevJob &job = jobs[jidx];
         const int mb = (int)(e & kMbMask);
         const int ref = (int)((e >> kRefShift) & 3);
-        if (kind == kClaimFinal) {
-            if (NW == 1 || wave_id() == 0) final_mb(job, mb, sh.mb);
-        } else {
-            search_ticket<NW>(job, jidx, ref, mb, kind == kClaimSearch ? sweep : 0, list, sa, sh,
-                              counters, home, &s_pred);
-            if (kind == kClaimSweep0 && threadIdx.x == 0) cs.done0[list]++;
-        }
+        search_ticket<NW>(job, jidx, ref, mb, kind == kClaimSearch ? sweep : 0, list, sa, sh,
+                          counters, home, &s_pred);
+        if (kind == kClaimSweep0 && threadIdx.x == 0) cs.done0[list]++;
+    }
+}
+
+// Sweep mode's final pass, after sweep_kernel on the same queue: one wave
+// per MB of the final lists (every MB of the launch), kFinalMbs waves per
+// workgroup, no dependencies.  Skipped when the sweeps reported an error
+// (their decisions are then incomplete).
+constexpr int kFinalMbs = 4;
+
+__global__ __launch_bounds__(64 * kFinalMbs) void final_kernel(const DevJob *__restrict__ jobs,
+                                                              const uint32_t *__restrict__ work,
+                                                              const int32_t *counters)
+{
+    __shared__ MbLds sh[kFinalMbs];
+    if (__hip_atomic_load(&counters[kCounterError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return;
+    const int w = wave_id();
+    const int lo = (int)work[kWorkFinal], hi = (int)work[kWorkFinal + work[0]];
+    for (int t = lo + (int)blockIdx.x * kFinalMbs + w; t < hi; t += (int)gridDim.x * kFinalMbs) {
+        const uint32_t e = work[kWorkHeader + t];
+        final_mb(jobs[e >> kJobShift], (int)(e & kMbMask), sh[w]);
     }
 }
 
@@ -3880,12 +3902,19 @@ hipError_t launch_analyze(const DevJob *jobs, const uint32_t *work, int32_t *cou
 }
 
 hipError_t launch_sweep(const DevJob *jobs, const uint32_t *work, int32_t *counters,
-                        const SweepArgs &sa, int32_t grid, int32_t waves, hipStream_t st)
+                        const SweepArgs &sa, int32_t grid, int32_t waves, int32_t n_final,
+                        hipStream_t st)
 {
     if (waves == 2)
         hipLaunchKernelGGL(sweep_kernel<2>, dim3(grid), dim3(128), 0, st, jobs, work, counters, sa);
     else
         hipLaunchKernelGGL(sweep_kernel<1>, dim3(grid), dim3(64), 0, st, jobs, work, counters, sa);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n_final <= 0) return e;
+    // Enough workgroups for every MB at once up to a few rounds of the
+    // device; each wave then strides over the list.
+    const int32_t fgrid = std::min((n_final + kFinalMbs - 1) / kFinalMbs, 8192);
+    hipLaunchKernelGGL(final_kernel, dim3(fgrid), dim3(64 * kFinalMbs), 0, st, jobs, work, counters);
     return hipGetLastError();
 }
 
