@@ -11,7 +11,8 @@ The knobs (evxa_set_tuning):
     caches) and the per-list cap of tickets per virtual diagonal that
     staggers the frames of a batch;
   * sweeps: sweep 0's order (raster, or anti-diagonal with a bounded wait
-    for the neighbours' MVs) and the tickets claimed at once.
+    for the neighbours' MVs, frames staggered by the per-diagonal cap) and
+    the tickets claimed at once.
 Which shape is fastest depends on the workload: a single frame is bound by
 its MB dependency chain in the wavefront and wants sweeps, a batch of many
 frames wants throughput, and content whose predictor changes travel far
@@ -36,9 +37,9 @@ def _wavefront(**kw):
     return dict(mode=WAVEFRONT, **kw)
 
 
-def _sweep(waves, batch, order=1, wait_us=0):
+def _sweep(waves, batch, order=1, wait_us=0, **kw):
     return dict(mode=SWEEP, waves_per_search=waves, claim_batch=batch, sweep_order=order,
-                sweep_reads=2, sweep_wait_us=wait_us)
+                sweep_reads=2, sweep_wait_us=wait_us, **kw)
 
 
 # Candidate shapes per workload class.
@@ -47,8 +48,9 @@ SINGLE = ([_wavefront(waves_per_search=w, lists=l) for w, l in itertools.product
 BATCH = ([_wavefront(waves_per_search=w, lists=8, cap_pct=c)
           for w, c in itertools.product((1, 2), (100, 150, 250))] +
          [_wavefront(waves_per_search=1, lists=8, cap_pct=c, max_groups=2048) for c in (150, 250)] +
-         [_sweep(1, 1, order=2, wait_us=w) for w in (0, 5, 20)] +
-         [_sweep(1, 2, order=2, wait_us=0), _sweep(2, 1, order=2, wait_us=0)])
+         [_sweep(1, 1, order=2, wait_us=w, cap_pct=c) for w, c in
+          itertools.product((0, 5, 20), (250, 400))] +
+         [_sweep(2, 1, order=2, wait_us=5, cap_pct=250)])
 SERVE = [_wavefront(waves_per_search=w) for w in (1, 2)] + [_sweep(w, 2) for w in (1, 2)]
 
 
