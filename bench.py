@@ -428,8 +428,8 @@ def run_plain(args):
             "peak": VALU_PEAK_TOPS,
             "unit": "TOP/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 5),
-            "traffic": measured_traffic(),
-            "kernel": "sweep_kernel" if sweep_mode else "analyze_kernel",
+            "traffic": measured_traffic(sweep_mode),
+            "kernel": "sweep_kernel + final_kernel" if sweep_mode else "analyze_kernel",
             "kernel_ms_avg": round(k_avg_s * 1e3, 4),
             "algorithmic_ops_per_launch": int(ops_launch),
             "algorithmic_bytes_per_launch": int(bytes_launch),
@@ -1008,7 +1008,8 @@ def calibrate_serving(ctx, args):
 
 
 TRAFFIC_COPY_BYTES = 256 << 20
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "analyze_kernel_traffic_1080p.md")
+TRAFFIC_PROFILE = {False: os.path.join(ROOT, "profiles", "analyze_kernel_traffic_1080p.md"),
+                   True: os.path.join(ROOT, "profiles", "sweep_kernel_traffic_1080p.md")}
 
 
 def run_traffic(args):
@@ -1042,11 +1043,11 @@ def run_traffic(args):
     log(f"traffic: {TRAFFIC_COPY_BYTES} B copy x3, analysis of frames 1-5")
 
 
-def measured_traffic():
-    """HBM bytes per analysis launch (frame 5, 4 refs) from the committed
-    counter summary: the line `traffic_bytes_per_launch: N`."""
+def measured_traffic(sweep_mode=False):
+    """HBM bytes per analysis launch (frame 5, 4 refs) in the given mode from
+    the committed counter summary: the line `traffic_bytes_per_launch: N`."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILE[sweep_mode]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(line.split(":", 1)[1])

@@ -239,7 +239,7 @@ struct evxa_context {
     int32_t sweep_reads = kSweepReadsFresh;  // sweep 0: guess table only, or fresh words too
     int32_t sweep_wait_us = 0;       // sweep 0, fresh reads: longest wait for a neighbour
     int32_t sweep_order = EVXA_SWEEP_ORDER_RASTER;  // sweep 0's ticket order
-    int32_t claim_batch = 2;         // sweep mode: tickets per claim
+    int32_t claim_batch = 1;         // sweep mode: tickets per claim
     int32_t grid = 0;                // most analysis workgroups per launch (0: all that fit)
     int32_t lists = 0;               // work lists per launch (0: automatic)
     int32_t cap_pct = 60;            // tickets per list and virtual diagonal, % of its workgroups
@@ -860,6 +860,14 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
         if (const char *g = getenv("EVXA_CAP_PCT")) {
             const int v = atoi(g);
             if (v > 0) ctx->cap_pct = v;
+        }
+        if (const char *g = getenv("EVXA_SWEEP_ORDER")) {
+            const int v = atoi(g);
+            if (v == EVXA_SWEEP_ORDER_RASTER || v == EVXA_SWEEP_ORDER_DIAGONAL) ctx->sweep_order = v;
+        }
+        if (const char *g = getenv("EVXA_CLAIM_BATCH")) {
+            const int v = atoi(g);
+            if (v >= 1 && v <= kMaxClaimBatch) ctx->claim_batch = v;
         }
         *out = ctx.release();
     });

@@ -304,8 +304,8 @@ typedef struct evxa_tuning {
     int32_t max_groups;           /* workgroups per launch (0: all that fit the device) */
     int32_t lists;                /* work lists per launch, 1..8 (0: one per XCD for batches
                                      and sweeps) */
-    int32_t cap_pct;              /* wavefront: tickets per list and diagonal, % of its
-                                     workgroups (0: 60) */
+    int32_t cap_pct;              /* tickets per list and diagonal, % of its workgroups
+                                     (wavefront, and sweep 0 in diagonal order; 0: 60) */
     int32_t mode;                 /* EVXA_MODE_WAVEFRONT or EVXA_MODE_SWEEP (0: keep) */
     int32_t sweep_reads;          /* sweep 0's neighbour MVs: 1 the temporal table's only (a
                                      deterministic iteration), 2 also those already decided
@@ -314,7 +314,7 @@ typedef struct evxa_tuning {
                                      sweep-0 MV, microseconds (-1: keep) */
     int32_t sweep_order;          /* EVXA_SWEEP_ORDER_* (0: keep) */
     int32_t claim_batch;          /* sweep mode: consecutive tickets a workgroup claims at
-                                     once, 1..16 (0: keep) */
+                                     once, 1..16 (default 1; 0: keep) */
 } evxa_tuning;
 
 int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t);
