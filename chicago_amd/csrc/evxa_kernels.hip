@@ -268,14 +268,21 @@ __device__ __forceinline__ Geo lane_geometry()
     return g;
 }
 
-// Four consecutive samples as one 8-byte access (global or LDS; the address
-// only needs 2-byte alignment, which gfx950 serves as one access).
+// Four consecutive samples of a picture in global memory as one 8-byte
+// access (the address only needs 2-byte alignment, which gfx950 serves as
+// one access).  The pointer is taken as a global one, so the load is a
+// global_load: a flat load would also count against the LDS wait counter,
+// and every wait for an LDS read would then wait for it too.
+struct __attribute__((packed, aligned(2))) Pair32 {
+    uint32_t lo, hi;
+};
+typedef const __attribute__((address_space(1))) Pair32 *GlobalPair32;
+
 __device__ __forceinline__ void load4(const int16_t *p, uint32_t &lo, uint32_t &hi)
 {
-    uint2 q;
-    __builtin_memcpy(&q, p, 8);
-    lo = q.x;
-    hi = q.y;
+    const GlobalPair32 q = (GlobalPair32)(p);
+    lo = q->lo;
+    hi = q->hi;
 }
 
 // The block at (x, y) of planes with pitches (py, pc) in the lane layout.
@@ -862,14 +869,19 @@ __device__ __forceinline__ bool fold_level(Best &b, int base_x, int base_y, int 
 // coherent side of the L2s.  Each such word is one 64-bit access carrying
 // its own published bit, so a reader needs no flag, no ordering between
 // words and no wait on the writer's side.
+// (Every such word lives in global memory: the accesses take the pointer as
+// a global one, so they are global, not flat, instructions and do not count
+// against the LDS wait counter.)
+#define EVXA_GLOBAL(T, p) ((__attribute__((address_space(1))) T *)(p))
+
 __device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(EVXA_GLOBAL(const uint64_t, p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(EVXA_GLOBAL(uint64_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A MB's predictor word: bit 63 = published, bits 32..39 = the block type's
@@ -1889,21 +1901,21 @@ __global__ __launch_bounds__(64 * NW, 3) void analyze_kernel(const DevJob *__res
 // itself finished.
 
 constexpr uint64_t kSweepWaitTicks = 100000000ull;   // 1 s of s_memrealtime (100 MHz)
-constexpr int kSweepMaxNap = 16;                     // longest back-off, s_sleep(8) units
+constexpr int kSweepMaxNap = 4;                      // longest back-off, s_sleep(8) units
 
 __device__ __forceinline__ int32_t ld_relaxed32(const int32_t *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(EVXA_GLOBAL(const int32_t, p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void st_relaxed32(int32_t *p, int32_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(EVXA_GLOBAL(int32_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int32_t fetch_add(int32_t *p, int32_t v)
 {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(EVXA_GLOBAL(int32_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Every store of this wave has completed (device-scope stores are then
