@@ -789,7 +789,7 @@ void check_last_launch(evxa_context *ctx)
 
 extern "C" {
 
-int32_t evxa_version(void) { return 103; }
+int32_t evxa_version(void) { return 104; }
 
 const char *evxa_last_error(void) { return g_last_error.c_str(); }
 
@@ -1625,6 +1625,39 @@ int evxa_search_blocks(evxa_stream *s, int32_t src_slot, int32_t ref_slot, int32
                                    lambda_of(quality), n, dp.as<int32_t>(),
                                    dout.as<evxa_search_result>(), ctx->stream),
               "search kernel");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check(hipMemcpy(out, dout.p, (size_t)n * sizeof(evxa_search_result), hipMemcpyDeviceToHost),
+              "copy out");
+    });
+}
+
+int evxa_intra_search_blocks(evxa_stream *s, int32_t src_slot, int32_t pic_slot, int32_t quality,
+                             int32_t n, const int32_t *xs, const int32_t *ys,
+                             evxa_search_result *out)
+{
+    return guarded([&] {
+        if (!s || n < 0 || (n > 0 && (!xs || !ys || !out))) fail(EVXA_ERROR_INVALIDARG, "bad argument");
+        if (quality < 0 || quality > 63) fail(EVXA_ERROR_INVALIDARG, "quality must be 0..63");
+        if (n == 0) return;
+        check_slot(s, src_slot);
+        check_slot(s, pic_slot);
+        std::vector<int32_t> p((size_t)n * 2);
+        for (int32_t k = 0; k < n; k++) {
+            if (xs[k] < 0 || xs[k] > s->w - 16 || ys[k] < 0 || ys[k] > s->h - 16)
+                fail(EVXA_ERROR_INVALIDARG, "MB position outside the picture");
+            p[k * 2] = xs[k];
+            p[k * 2 + 1] = ys[k];
+        }
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        set_device(ctx);
+        DevBuf dp(p.size() * 4), dout((size_t)n * sizeof(evxa_search_result));
+        check(hipMemcpy(dp.p, p.data(), p.size() * 4, hipMemcpyHostToDevice), "copy params");
+        check(launch_intra_blocks(s->pic(src_slot), s->pic(pic_slot), s->w, s->h, quality,
+                                  lambda_of(quality), n, dp.as<int32_t>(),
+                                  dout.as<evxa_search_result>(), ctx->stream),
+              "intra search kernel");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         check(hipMemcpy(out, dout.p, (size_t)n * sizeof(evxa_search_result), hipMemcpyDeviceToHost),
               "copy out");

@@ -246,6 +246,11 @@ hipError_t launch_interp_blocks(DevPic pic, int32_t w, int32_t h, int32_t n, con
 hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, int32_t quality,
                                 float lambda, int32_t n, const int32_t *params,
                                 evxa_search_result *out, hipStream_t st);
+// calculate_intra_prediction of n MBs of `src` over `pic` (the picture being
+// reconstructed); params holds (x, y) per MB.
+hipError_t launch_intra_blocks(DevPic src, DevPic pic, int32_t w, int32_t h, int32_t quality,
+                               float lambda, int32_t n, const int32_t *params,
+                               evxa_search_result *out, hipStream_t st);
 hipError_t launch_dct_quant_blocks(int32_t n, const int16_t *src, const int16_t *pred,
                                    const int32_t *quality, int16_t *coeffs, int16_t *levels,
                                    int32_t *qp, int32_t *variance, hipStream_t st);

@@ -3833,6 +3833,176 @@ __global__ __launch_bounds__(64) void search_blocks_kernel(DevPic src, DevPic re
     }
 }
 
+// ---------------------------------------------------- intra block search --
+//
+// calculate_intra_prediction (motion.cpp:485-565): a block search of the MB
+// at (x, y) in the picture being reconstructed, over the part already
+// reconstructed -- a candidate that reaches below row y - 16 and right of
+// column x - 16 is skipped (motion.cpp:361-365, 433-438).  The state starts
+// from the block against zero (luma SAD, MAD INT32_MAX, cost the SAD); a
+// 3x3 scan at step 16 over rows y - 32 .. y and columns x - 16 .. x + 16
+// (motion.cpp:525-527) is refined by 3x3 levels at steps 8, 4, 2, 1 around
+// the best until a copy (MAD < T) is found, then the sub-pel step runs in
+// every case (motion.cpp:540).  Candidates are evaluated four at a time (one
+// per 16-lane group) in 32-bit arithmetic and folded one by one in scan
+// order with evaluate_motion_candidate's and
+// evaluate_subpel_motion_candidate's acceptance (motion.cpp:125-261).
+
+struct IntraState {
+    int x, y, sad, mad, ssd;
+    uint32_t cost;                 // fp32 bits (costs are >= 0)
+    int n_int;
+};
+
+__device__ __forceinline__ bool intra_unstable(int cx, int cy, int x, int y)
+{
+    return cy > y - 16 && cx > x - 16;
+}
+
+// One 3x3 scan (perform_intra_motion_search, motion.cpp:348-379) of step
+// `step` whose first position is (left, top) from the current best; true
+// once the best is a copy (the scan stops there).
+__device__ bool intra_scan(const DevPic &pic, int pw, int ph, const Blk &s, const Geo &geo, int x,
+                           int y, int thresh, float lambda, int left, int top, int step,
+                           IntraState &st)
+{
+    const int g = lane_id() >> 4;
+    const int base_x = st.x, base_y = st.y, max_x = pw - 16, max_y = ph - 16;
+    for (int c = 0; c < 3; c++) {
+        const int p = 4 * c + g;
+        const int cx = base_x + left + (p % 3) * step, cy = base_y + top + (p / 3) * step;
+        const bool valid = p < 9 && !intra_unstable(cx, cy, x, y) && cx >= 0 && cx <= max_x &&
+                           cy >= 0 && cy <= max_y;
+        Blk r;
+        load_cand(pic, geo, valid ? cx : 0, valid ? cy : 0, r);
+        int sad, mad, satd;
+        cand_metrics(false, s, r, sad, mad, satd);
+        const uint32_t cst = cost_bits(sad, satd, cx - x, cy - y, lambda);
+        const uint64_t vm = __ballot(valid);
+        for (int k = 0; k < 4; k++) {
+            if (((vm >> (16 * k)) & 1u) == 0) continue;
+            const int pk = 4 * c + k;
+            const int ik = left + (pk % 3) * step, jk = top + (pk / 3) * step;
+            const int kx = base_x + ik, ky = base_y + jk;
+            const int ksad = lane_value(sad, 16 * k), kmad = lane_value(mad, 16 * k);
+            const uint32_t kcost = (uint32_t)lane_value((int)cst, 16 * k);
+            const int ssd = (kx - x) * (kx - x) + (ky - y) * (ky - y);
+            st.n_int += (ik != 0 || jk != 0) ? 1 : 0;
+            bool take;
+            if (st.mad < thresh)
+                take = kmad < st.mad || (kmad == st.mad && ssd < st.ssd);
+            else   // C precedence: && binds tighter than || (motion.cpp:166-168)
+                take = (kcost < st.cost || ((kcost == st.cost && ssd < st.ssd) && ksad < kSadThreshold)) ||
+                       kmad < thresh;
+            if (take) {
+                st.x = kx; st.y = ky;
+                st.sad = ksad; st.mad = kmad; st.ssd = ssd; st.cost = kcost;
+            }
+            if (st.mad < thresh) return true;
+        }
+    }
+    return false;
+}
+
+// A block of 384 samples in LDS (Y 16x16, U 8x8, V 8x8) in the lane layout.
+__device__ __forceinline__ void load_mb_lds(const int16_t *mb, const Geo &g, Blk &b)
+{
+#pragma unroll
+    for (int r = 0; r < 4; r++) lds_load4(mb, (g.oy + r) * 16 + g.ox, b.y[2 * r], b.y[2 * r + 1]);
+    lds_load4(mb, 256 + g.cr * 8 + g.cc, b.u[0], b.u[1]);
+    lds_load4(mb, 320 + g.cr * 8 + g.cc, b.v[0], b.v[1]);
+}
+
+__device__ __forceinline__ int frac_index_of(int i, int j)
+{
+    // compute_motion_frac_index_from_direction (motion.cpp:68-91)
+    i++; j++;
+    if (j == 0) return i;
+    if (j == 1) return i == 0 ? 3 : 4;
+    return i + 5;
+}
+
+__device__ SearchOut intra_search(const DevPic &pic, int pw, int ph, const Blk &s, const Geo &geo,
+                                  int x, int y, int thresh, float lambda, MbLds &sh)
+{
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) a += abs(lo16(s.y[k])) + abs(hi16(s.y[k]));
+    const int sad0 = lane_value(row_sum(a), 0);       // compute_block_sad(src), analysis.h:134-145
+    IntraState st{x, y, sad0, kInt32Max, kInt32Max, __float_as_uint(__int2float_rn(sad0)), 0};
+    intra_scan(pic, pw, ph, s, geo, x, y, thresh, lambda, -kSearchRadius, -2 * kSearchRadius,
+               kSearchRadius, st);
+    if (st.mad >= thresh)
+        for (int step = kSearchRadius / 2; step > 0; step >>= 1)
+            if (intra_scan(pic, pw, ph, s, geo, x, y, thresh, lambda, -step, -step, step, st)) break;
+
+    // perform_intra_subpixel_motion_search (motion.cpp:408-448): half then
+    // quarter of each direction around the integer best.
+    int sp_en = 0, sp_amt = 0, sp_idx = 0, n_ax = 0, n_dg = 0;
+    for (int j = -1; j <= 1; j++)
+        for (int i = -1; i <= 1; i++) {
+            const int tx = st.x + i, ty = st.y + j;
+            if ((i == 0 && j == 0) || intra_unstable(tx, ty, x, y)) continue;
+            if (tx < 0 || tx > pw - 16 || ty < 0 || ty > ph - 16) continue;
+            if (i != 0 && j != 0) n_dg++; else n_ax++;
+            for (int q = 0; q < 2; q++) {
+                SearchOut r{st.x, st.y, 0, 0, 0, 0.0f, 1, q, frac_index_of(i, j), 0, 0, 0, 0};
+                predict_mb(pic, r, sh.pred, sh.tmp);
+                Blk c;
+                load_mb_lds(sh.pred, geo, c);
+                int sad, mad, satd;
+                cand_metrics(false, s, c, sad, mad, satd);
+                sad = lane_value(sad, 0);
+                mad = lane_value(mad, 0);
+                const bool take = st.mad < thresh ? mad < st.mad
+                                                  : ((sad < st.sad && sad < kSadThreshold) || mad < thresh);
+                if (take) {
+                    sp_en = 1; sp_amt = q; sp_idx = frac_index_of(i, j);
+                    st.sad = sad; st.mad = mad;
+                }
+                wave_sync();            // every lane has read sh.pred before the next block
+            }
+        }
+    SearchOut o;
+    o.best_x = st.x; o.best_y = st.y;
+    o.sad = st.sad; o.mad = st.mad; o.ssd = st.ssd;
+    o.cost = __uint_as_float(st.cost);
+    o.sp_en = sp_en; o.sp_amt = sp_amt; o.sp_idx = sp_idx;
+    o.btype = EVXA_BLOCK_INTRA;
+    if (st.x != x || st.y != y || sp_en) o.btype |= EVXA_BLOCK_MOTION;
+    if (st.mad < thresh) o.btype |= EVXA_BLOCK_COPY;
+    o.n_int = st.n_int; o.n_ax = n_ax; o.n_dg = n_dg;
+    return o;
+}
+
+// One wave per MB: intra searches of the MBs at params[2b], params[2b + 1]
+// of `src` over `pic`.
+__global__ __launch_bounds__(64) void intra_blocks_kernel(DevPic src, DevPic pic, int32_t w,
+                                                          int32_t h, int32_t quality, float lambda,
+                                                          int32_t n, const int32_t *params,
+                                                          evxa_search_result *out)
+{
+    __shared__ MbLds sh;
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const int x = params[2 * b], y = params[2 * b + 1];
+    const Geo geo = lane_geometry();
+    Blk s;
+    load_cand(src, geo, x, y, s);
+    const SearchOut o = intra_search(pic, w, h, s, geo, x, y, (quality >> 2) + 1, lambda, sh);
+    if (lane_id() == 0) {
+        evxa_search_result r;
+        r.best_x = o.best_x; r.best_y = o.best_y;
+        r.sad = o.sad; r.mad = o.mad; r.ssd = o.ssd;
+        r.cost_bits = fbits(o.cost);
+        r.sp_enabled = o.sp_en; r.sp_amount = o.sp_amt; r.sp_index = o.sp_idx;
+        r.block_type = o.btype;
+        r.motion_x = o.best_x - x; r.motion_y = o.best_y - y;
+        r.n_int = o.n_int; r.n_axial = o.n_ax; r.n_diag = o.n_dg;
+        out[b] = r;
+    }
+}
+
 __global__ __launch_bounds__(64) void dct_quant_blocks_kernel(int32_t n, const int16_t *src,
                                                               const int16_t *pred,
                                                               const int32_t *quality,
@@ -4061,6 +4231,15 @@ hipError_t launch_search_blocks(DevPic src, DevPic ref, int32_t w, int32_t h, in
                                 evxa_search_result *out, hipStream_t st)
 {
     hipLaunchKernelGGL(search_blocks_kernel, dim3(n), dim3(64), 0, st, src, ref, w, h, quality,
+                       lambda, n, params, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_intra_blocks(DevPic src, DevPic pic, int32_t w, int32_t h, int32_t quality,
+                               float lambda, int32_t n, const int32_t *params,
+                               evxa_search_result *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(intra_blocks_kernel, dim3(n), dim3(64), 0, st, src, pic, w, h, quality,
                        lambda, n, params, out);
     return hipGetLastError();
 }

@@ -101,7 +101,7 @@ EXPORTS = (
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
     "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats", "evxa_analyze_frame_noalloc",
-    "evxa_stream_wait", "evxa_stream_query")
+    "evxa_stream_wait", "evxa_stream_query", "evxa_intra_search_blocks")
 
 
 class EvxaError(RuntimeError):
@@ -160,6 +160,8 @@ def load_library(path: str = LIB_PATH):
                                          I16P]),
         "evxa_search_blocks": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, I32P,
                                          I32P, I16P, I16P, vp]),
+        "evxa_intra_search_blocks": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                               I32P, I32P, vp]),
         "evxa_dct_quant_blocks": (C.c_int, [vp, C.c_int32, I16P, I16P, I32P, I16P, I16P, I32P,
                                             I32P]),
         "evxa_clock": (C.c_double, []),
@@ -470,6 +472,16 @@ class Stream:
         _check(_lib.evxa_search_blocks(self.h, src_slot, ref_slot, quality, n, _p32(xs),
                                        _p32(ys), _p16(px), _p16(py),
                                        out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def intra_search(self, src_slot, pic_slot, quality, xs, ys):
+        """calculate_intra_prediction of the MBs at (xs, ys) of src_slot over
+        pic_slot as the picture being reconstructed."""
+        xs, ys = (np.ascontiguousarray(np.atleast_1d(a), np.int32) for a in (xs, ys))
+        out = np.zeros(len(xs), SEARCH_DTYPE)
+        _check(_lib.evxa_intra_search_blocks(self.h, src_slot, pic_slot, quality, len(xs),
+                                             _p32(xs), _p32(ys),
+                                             out.ctypes.data_as(C.c_void_p)))
         return out
 
 

@@ -267,7 +267,18 @@ int evxa_interp_blocks(evxa_stream *s, int32_t slot, int32_t n, const int32_t *x
 int evxa_search_blocks(evxa_stream *s, int32_t src_slot, int32_t ref_slot, int32_t quality,
                        int32_t n, const int32_t *xs, const int32_t *ys, const int16_t *pred_x,
                        const int16_t *pred_y, evxa_search_result *out);
-/* sub_transform_macroblock + query_block_quantization_parameter +
+/* calculate_intra_prediction (motion.cpp:485-565) for n MBs: the intra-frame
+ * block search of the source MB at (xs[k], ys[k]) of `src_slot` over the
+ * picture in `pic_slot` as the picture being reconstructed (the
+ * reference's own picture-buffer slot, motion.cpp:510-515: rows above and
+ * columns left of the MB hold the reconstruction; the search never reads
+ * below row y - 16 to the right of column x - 16).  Results as
+ * evxa_search_result: block_type has EVXA_BLOCK_INTRA, best_x/best_y the
+ * matched block in pic_slot. */
+int evxa_intra_search_blocks(evxa_stream *s, int32_t src_slot, int32_t pic_slot, int32_t quality,
+                             int32_t n, const int32_t *xs, const int32_t *ys,
+                             evxa_search_result *out);
+/* sub_transform_macroblock + query_block_quantization_parameter +/* sub_transform_macroblock + query_block_quantization_parameter +
  * compute_block_variance2 + quantize_macroblock (inter) for n residual
  * pairs (macroblock.h:272-277, quantize.cpp:97-118, 372-397;
  * analysis.h:253-275).  src/pred/coeffs/levels hold n * 384 samples. */

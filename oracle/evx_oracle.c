@@ -506,6 +506,97 @@ int32_t evo_inter_search(const int16_t src[EVO_MB_SAMPLES], const evo_planes *re
     return s.best_sad;
 }
 
+/* ------------------------------------------------------- intra search -- */
+
+/* The intra searches read the picture being reconstructed: a candidate
+ * that reaches below row y - 16 and right of column x - 16 covers MBs not
+ * yet reconstructed and is skipped (motion.cpp:361-365, 433-438). */
+static int intra_unstable(const search_state *s, int32_t cx, int32_t cy)
+{
+    return cy > s->py - EVO_MB && cx > s->px - EVO_MB;
+}
+
+/* motion.cpp:348-379 */
+static void intra_scan(search_state *s, int32_t left, int32_t top, int32_t right, int32_t bottom,
+                       int32_t step)
+{
+    int32_t base_x = s->best_x, base_y = s->best_y;
+    int32_t max_x = s->ref->width - EVO_MB, max_y = s->ref->height - EVO_MB;
+    for (int32_t j = top; j <= bottom; j += step)
+        for (int32_t i = left; i <= right; i += step) {
+            int32_t cx = base_x + i, cy = base_y + j;
+            if (intra_unstable(s, cx, cy)) continue;
+            if (cx < 0 || cx > max_x || cy < 0 || cy > max_y) continue;
+            eval_candidate(s, cx, cy, i != 0 || j != 0);
+            if (s->best_mad < s->thresh) return;
+        }
+}
+
+/* motion.cpp:408-448 */
+static void intra_subpel_search(search_state *s)
+{
+    int16_t blk[EVO_MB_SAMPLES];
+    int32_t max_x = s->ref->width - EVO_MB, max_y = s->ref->height - EVO_MB;
+    s->sp_index = 0; s->sp_amount = 0; s->sp_enabled = 0;
+    for (int32_t j = -1; j <= 1; j++)
+        for (int32_t i = -1; i <= 1; i++) {
+            int32_t tx = s->best_x + i, ty = s->best_y + j;
+            if (i == 0 && j == 0) continue;
+            if (intra_unstable(s, tx, ty)) continue;
+            if (tx < 0 || tx > max_x || ty < 0 || ty > max_y) continue;
+            if (i != 0 && j != 0) s->n_diag++; else s->n_axial++;
+            for (int32_t q = 0; q < 2; q++) {
+                evo_interp_block(s->ref, s->best_x, s->best_y, i, j, q, blk);
+                int32_t sad = evo_block_sad(s->src, blk);
+                int32_t mad = evo_block_mad(s->src, blk);
+                subpel_accept(s, sad, mad, i, j, q);
+            }
+        }
+}
+
+int32_t evo_intra_search(const int16_t src[EVO_MB_SAMPLES], const evo_planes *pic, int32_t x,
+                         int32_t y, int32_t quality, evo_search_result *res)
+{
+    /* motion.cpp:485-565 */
+    search_state s;
+    memset(&s, 0, sizeof(s));
+    s.ref = pic; s.src = src; s.px = x; s.py = y;
+    s.thresh = (quality >> 2) + 1;
+    s.lambda = evo_lambda(quality);
+    s.best_x = x; s.best_y = y;
+    /* compute_block_sad(src) (analysis.h:134-145): luma only. */
+    int32_t sad0 = 0;
+    for (int32_t k = 0; k < 256; k++) sad0 += src[k] < 0 ? -src[k] : src[k];
+    s.best_sad = sad0;
+    s.best_mad = EVO_INT32_MAX;
+    s.best_ssd = EVO_INT32_MAX;
+    s.best_cost = (float)sad0;
+
+    /* The triangle above and to the left: rows y - 32 .. y, columns
+     * x - 16 .. x + 16, step 16 (motion.cpp:525-527). */
+    intra_scan(&s, -EVO_SEARCH_RADIUS, -2 * EVO_SEARCH_RADIUS, EVO_SEARCH_RADIUS, 0, EVO_SEARCH_RADIUS);
+    if (s.best_mad >= s.thresh) {
+        for (int32_t i = EVO_SEARCH_RADIUS / 2; i > 0; i >>= 1) {
+            intra_scan(&s, -i, -i, i, i, i);
+            if (s.best_mad < s.thresh) break;
+        }
+    }
+    /* motion.cpp:540: the sub-pel step runs whatever the integer scans found. */
+    intra_subpel_search(&s);
+
+    int32_t bt = EVO_BT_INTRA;
+    if (s.best_x != x || s.best_y != y || s.sp_enabled) bt |= EVO_BT_MOTION;
+    if (s.best_mad < s.thresh) bt |= EVO_BT_COPY;
+    res->best_x = s.best_x; res->best_y = s.best_y;
+    res->sad = s.best_sad; res->mad = s.best_mad; res->ssd = s.best_ssd;
+    memcpy(&res->cost_bits, &s.best_cost, 4);
+    res->sp_enabled = s.sp_enabled; res->sp_amount = s.sp_amount; res->sp_index = s.sp_index;
+    res->block_type = bt;
+    res->motion_x = s.best_x - x; res->motion_y = s.best_y - y;
+    res->n_int = s.n_int; res->n_axial = s.n_axial; res->n_diag = s.n_diag;
+    return s.best_sad;
+}
+
 void evo_predict_block(const evo_planes *ref, int32_t x, int32_t y, const evo_search_result *r,
                        int16_t out[EVO_MB_SAMPLES])
 {

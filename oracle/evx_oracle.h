@@ -133,6 +133,12 @@ int32_t evo_inter_search(const int16_t src[EVO_MB_SAMPLES],
                          int32_t quality, int16_t pred_mx, int16_t pred_my,
                          evo_search_result *res);
 
+/* calculate_intra_prediction (motion.cpp:485-565): the intra-frame block
+ * search of the MB at (x, y) over `pic`, the picture being reconstructed
+ * (rows above and columns left of the MB hold reconstructed samples). */
+int32_t evo_intra_search(const int16_t src[EVO_MB_SAMPLES], const evo_planes *pic,
+                         int32_t x, int32_t y, int32_t quality, evo_search_result *res);
+
 /* MB prediction for a search result (encode.cpp:696-723). */
 void evo_predict_block(const evo_planes *ref, int32_t x, int32_t y,
                        const evo_search_result *r, int16_t out[EVO_MB_SAMPLES]);

@@ -216,6 +216,8 @@ class Oracle(_Lib, _Recon):
                             C.c_void_p, I16P, I16P)
         self._search = self.f("inter_search", C.c_int32, I16P, P, C.c_int32, C.c_int32,
                               C.c_int32, C.c_int16, C.c_int16, C.POINTER(SearchResult))
+        self._intra = self.f("intra_search", C.c_int32, I16P, P, C.c_int32, C.c_int32, C.c_int32,
+                             C.POINTER(SearchResult))
         self._predict = self.f("predict_block", None, P, C.c_int32, C.c_int32,
                                C.POINTER(SearchResult), I16P)
         self._bidir = self.f("bidir_average", C.c_int32, I16P, I16P, I16P, I16P)
@@ -257,6 +259,14 @@ class Oracle(_Lib, _Recon):
         r = SearchResult()
         s = np.ascontiguousarray(src_blk, np.int16)
         self._search(_ptr16(s), C.byref(ref.planes), x, y, quality, pmx, pmy, C.byref(r))
+        return r
+
+    def intra_search(self, src_blk, pic, x, y, quality):
+        """calculate_intra_prediction of the MB at (x, y) over `pic`, the
+        picture being reconstructed."""
+        r = SearchResult()
+        s = np.ascontiguousarray(src_blk, np.int16)
+        self._intra(_ptr16(s), C.byref(pic.planes), x, y, quality, C.byref(r))
         return r
 
     def predict(self, ref, x, y, r):
@@ -342,6 +352,8 @@ class Reference(_Lib, _Recon):
                             C.c_uint32, C.c_void_p, I16P, I16P)
         self._search = self.f("inter_search", C.c_int32, I16P, P, C.c_int32, C.c_int32,
                               C.c_int32, C.c_int16, C.c_int16, C.c_void_p)
+        self._intra = self.f("intra_search", C.c_int32, I16P, P, C.c_int32, C.c_int32, C.c_int32,
+                             C.c_void_p)
         self._bidir = self.f("bidir", C.c_int32, I16P, P, P, C.c_int32, C.c_int32, C.c_int32,
                              C.c_int16, C.c_int16, C.c_void_p, I16P)
         self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
@@ -455,6 +467,12 @@ class Reference(_Lib, _Recon):
         rec = np.zeros(1, RECORD_DTYPE)
         s = np.ascontiguousarray(src_blk, np.int16)
         self._search(_ptr16(s), C.byref(ref.planes), x, y, quality, pmx, pmy, rec.ctypes.data)
+        return rec[0]
+
+    def intra_search(self, src_blk, pic, x, y, quality):
+        rec = np.zeros(1, RECORD_DTYPE)
+        s = np.ascontiguousarray(src_blk, np.int16)
+        self._intra(_ptr16(s), C.byref(pic.planes), x, y, quality, rec.ctypes.data)
         return rec[0]
 
     def bidir(self, src_blk, fwd, bwd, x, y, quality, pmx, pmy):

@@ -303,6 +303,27 @@ int32_t evr_inter_search(const int16_t *src, const evo_planes *ref, int32_t x, i
     return sad;
 }
 
+// calculate_intra_prediction (motion.cpp:485-565) with `pic` as the picture
+// being reconstructed, the frame's own picture-buffer slot (motion.cpp:510-515).
+int32_t evr_intra_search(const int16_t *src, const evo_planes *pic, int32_t x, int32_t y,
+                         int32_t quality, evo_record *out)
+{
+    bank_holder h;
+    fill_set(&h.bank.prediction_cache[0], *pic);
+    evx_frame frame;
+    clear_frame(&frame);
+    frame.type = EVX_FRAME_INTER;
+    frame.quality = (uint16)quality;
+    frame.dpb_slot = 0;
+    std::vector<int16_t> s(src, src + 384);
+    macroblock ms = view(s.data());
+    evx_block_desc d;
+    int32 sad = calculate_intra_prediction(frame, ms, x, y, &h.bank, &d);
+    desc_to_record(d, out);
+    out->sad = sad;
+    return sad;
+}
+
 int32_t evr_bidir(const int16_t *src, const evo_planes *fwd, const evo_planes *bwd, int32_t x,
                   int32_t y, int32_t quality, int16_t pmx, int16_t pmy, evo_record *out,
                   int16_t *avg)

@@ -6,7 +6,7 @@ sources plus oracle/ref_shim.cpp.  Inputs are stored next to the outputs, so
 the fixtures never depend on a random generator at test time.
 
     python tests/golden/make_golden.py          # every fixture
-    python tests/golden/make_golden.py recon    # recon.npz only
+    python tests/golden/make_golden.py recon    # recon.npz only (likewise deblock, sao, intra)
 """
 import os
 import sys
@@ -296,8 +296,40 @@ def sao_fixtures():
     save("sao.npz", **out)
 
 
+def intra_fixtures():
+    """calculate_intra_prediction (motion.cpp:485-565) for every MB of two
+    pictures: natural texture searched in its previous frame, and content
+    repeating every 24 samples (exact and near-exact block copies above and
+    to the left), at several qualities."""
+    ref = Reference()
+    rng = np.random.default_rng(20261204)
+    frames = synth.sequence_c1(int(rng.integers(1 << 30)), 2, 112, 80, local_range=12)
+    tile = synth.texture_frame(rng, 48, 48)
+    tiled = tuple(np.tile(t[:n, :n], (80 // n // (1 if k == 0 else 2) + 1,
+                                      112 // n // (1 if k == 0 else 2) + 1))
+                  [: (80 if k == 0 else 40), : (112 if k == 0 else 56)].astype(np.int16)
+                  for k, (t, n) in enumerate(zip(tile, (24, 12, 12))))
+    noisy = tuple((p + rng.integers(-1, 2, p.shape)).astype(np.int16) for p in tiled)
+    out = {}
+    for tag, src, pic in (("tex", frames[1], frames[0]), ("tile", tiled, noisy)):
+        S, P = Picture(*src), Picture(*pic)
+        params, recs = [], []
+        for q in (0, 8, 13, 40):
+            for yy in range(0, S.height, 16):
+                for xx in range(0, S.width, 16):
+                    recs.append(ref.intra_search(S.block(xx, yy), P, xx, yy, q))
+                    params.append((xx, yy, q))
+        out[f"{tag}_src_y"], out[f"{tag}_src_u"], out[f"{tag}_src_v"] = src
+        out[f"{tag}_pic_y"], out[f"{tag}_pic_u"], out[f"{tag}_pic_v"] = pic
+        out[f"{tag}_params"] = np.array(params, np.int32)
+        out[f"{tag}_records"] = np.array(recs, RECORD_DTYPE)
+    save("intra.npz", **out)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["recon"]:
+    if sys.argv[1:] == ["intra"]:
+        intra_fixtures()
+    elif sys.argv[1:] == ["recon"]:
         recon_fixtures()
     elif sys.argv[1:] == ["deblock"]:
         deblock_fixtures()
@@ -308,3 +340,4 @@ if __name__ == "__main__":
         recon_fixtures()
         deblock_fixtures()
         sao_fixtures()
+        intra_fixtures()

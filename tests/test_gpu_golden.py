@@ -4,7 +4,7 @@ per-block entries (interpolation, DCT + QP + quantisation, searches), whole
 analysed P and B frames in both ways of resolving the MV predictor,
 reconstruction with the frames' own and perturbed levels, and SAO.  These
 pin the device's results to the reference directly, not through the oracle
-built on the box."""
+built on the box.  The intra block search (intra.npz) is pinned the same way."""
 import os
 
 import numpy as np
@@ -98,6 +98,23 @@ def test_search(ctx, tag):
         sel = np.nonzero(params[:, 2] == q)[0]
         p = params[sel]
         got = s.perform_inter_search(0, 1, int(q), p[:, 0], p[:, 1], p[:, 3], p[:, 4])
+        for k, i in enumerate(sel):
+            assert _search_fields(got[k]) == _record_fields(want[i]), (tag, params[i].tolist())
+    s.close()
+
+
+@pytest.mark.parametrize("tag", ["tex", "tile"])
+def test_intra_search(ctx, tag):
+    """evxa_intra_search_blocks against intra.npz (the reference build's
+    calculate_intra_prediction records)."""
+    g = load("intra.npz")
+    s = _stream(ctx, [_planes(g, f"{tag}_src"), _planes(g, f"{tag}_pic")])
+    params = g[f"{tag}_params"]
+    want = g[f"{tag}_records"].view(evxa.RECORD_DTYPE)
+    for q in np.unique(params[:, 2]):
+        sel = np.nonzero(params[:, 2] == q)[0]
+        p = params[sel]
+        got = s.intra_search(0, 1, int(q), p[:, 0], p[:, 1])
         for k, i in enumerate(sel):
             assert _search_fields(got[k]) == _record_fields(want[i]), (tag, params[i].tolist())
     s.close()

@@ -76,6 +76,25 @@ def test_inter_search(oracle, tag):
         assert copies > 0
 
 
+@pytest.mark.parametrize("tag", ["tex", "tile"])
+def test_intra_search(oracle, tag):
+    """calculate_intra_prediction against intra.npz (the reference build's
+    records for every MB at qualities 0, 8, 13 and 40)."""
+    g = load("intra.npz")
+    S = Picture(g[f"{tag}_src_y"], g[f"{tag}_src_u"], g[f"{tag}_src_v"])
+    P = Picture(g[f"{tag}_pic_y"], g[f"{tag}_pic_u"], g[f"{tag}_pic_v"])
+    kinds = set()
+    for (x, y, q), want in zip(g[f"{tag}_params"], g[f"{tag}_records"]):
+        r = oracle.intra_search(S.block(int(x), int(y)), P, int(x), int(y), int(q))
+        got = (r.block_type, r.motion_x, r.motion_y, r.sp_enabled, r.sp_amount, r.sp_index, r.sad)
+        exp = tuple(int(want[f]) for f in ("block_type", "motion_x", "motion_y", "sp_pred",
+                                            "sp_amount", "sp_index", "sad"))
+        assert got == exp, (tag, x, y, q)
+        kinds.add(r.block_type)
+    if tag == "tile":
+        assert {1, 3, 7} <= kinds
+
+
 def test_bidir(oracle):
     g = load("bidir.npz")
     F = Picture(g["fwd_y"], g["fwd_u"], g["fwd_v"])

@@ -225,3 +225,56 @@ def test_reference_gop4_sequence(oracle, ref):
         assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), r
         if ftype == 1:
             tables[r.index] = b[0]
+
+
+def _intra_cases():
+    """(name, source picture, picture being reconstructed): natural texture
+    against its previous frame, against itself plus noise, and flat blocky
+    content where block copies abound; plus 16-bit samples."""
+    frames = synth.sequence_c1(7007, 2, 144, 96, local_range=12)
+    rng = np.random.default_rng(7008)
+    noisy = tuple((p + rng.integers(-2, 3, p.shape)).astype(np.int16) for p in frames[1])
+    blocky = synth.blocky_picture(7009, 144, 96)
+    blocky2 = synth.blocky_picture(7010, 144, 96)
+    wide = tuple((p.astype(np.int32) * 90 - 9000).clip(-32768, 32767).astype(np.int16) for p in frames[1])
+    wide_ref = tuple((p.astype(np.int32) * 90 - 9000).clip(-32768, 32767).astype(np.int16)
+                     for p in frames[0])
+    # Content that repeats every 24 samples (12 in chroma): exact and
+    # near-exact block copies above and to the left, at offsets the step
+    # scans reach only through their finer levels.
+    tile = synth.texture_frame(rng, 48, 48)
+    tiled = tuple(np.tile(t[: (24 if k == 0 else 12), : (24 if k == 0 else 12)],
+                          (96 // (24 if k == 0 else 12) // (1 if k == 0 else 2) + 1,
+                           144 // (24 if k == 0 else 12) // (1 if k == 0 else 2) + 1))
+                  [: (96 if k == 0 else 48), : (144 if k == 0 else 72)].astype(np.int16)
+                  for k, t in enumerate(tile))
+    tiled_noise = tuple((p + rng.integers(-1, 2, p.shape)).astype(np.int16) for p in tiled)
+    return [("texture", frames[1], frames[0]), ("self_noise", frames[1], noisy),
+            ("blocky", blocky, blocky), ("blocky_other", blocky, blocky2),
+            ("tiled", tiled, tiled), ("tiled_noise", tiled, tiled_noise),
+            ("wide", wide, wide_ref)]
+
+
+INTRA_CASES = _intra_cases()
+
+
+@pytest.mark.parametrize("case", range(len(INTRA_CASES)), ids=[c[0] for c in INTRA_CASES])
+@pytest.mark.parametrize("quality", [0, 8, 13, 30, 63])
+def test_intra_search_random(oracle, ref, case, quality):
+    """calculate_intra_prediction (motion.cpp:485-565) over every MB of the
+    picture, edges included, oracle = reference build."""
+    _, src, pic = INTRA_CASES[case]
+    S, P = Picture(*src), Picture(*pic)
+    kinds = set()
+    for yy in range(0, S.height, 16):
+        for xx in range(0, S.width, 16):
+            r = oracle.intra_search(S.block(xx, yy), P, xx, yy, quality)
+            w = ref.intra_search(S.block(xx, yy), P, xx, yy, quality)
+            assert (r.block_type, r.motion_x, r.motion_y, r.sp_enabled, r.sp_amount, r.sp_index,
+                    r.sad) == (w["block_type"], w["motion_x"], w["motion_y"], w["sp_pred"],
+                               w["sp_amount"], w["sp_index"], w["sad"]), (xx, yy)
+            kinds.add((int(r.block_type), int(r.sp_enabled)))
+    if INTRA_CASES[case][0].startswith("tiled") and quality in (8, 13):
+        # Moves, copies and sub-pel wins all occur.
+        assert any(k & 4 for k, _ in kinds) and any(k == 3 for k, _ in kinds), kinds
+        assert any(sp for _, sp in kinds), kinds

@@ -21,6 +21,9 @@ replay tracked:
 
 Frame records sum the counters over the references searched and carry mad
 and cost of the chosen search (and of the backward one for a bidir MB).
+The intra search (calculate_intra_prediction) is replayed the same way; its
+initial state is the block against zero (luma SAD, mad INT32_MAX, cost the
+SAD), so mad stays INT32_MAX when no candidate is taken.
 Skipped where the reference build is absent.  CPU only."""
 import numpy as np
 import pytest
@@ -168,6 +171,78 @@ class Replay:
         return self.ref.interp_many(self.pic, [mx], [my], [dx], [dy], [self.sp[1]])[0]
 
 
+class IntraReplay(Replay):
+    """calculate_intra_prediction (motion.cpp:485-565): the scans of
+    perform_intra_motion_search and its sub-pel step skip candidates that
+    reach the MBs not yet reconstructed (motion.cpp:361-365, 433-438)."""
+
+    def unstable(self, cx, cy):
+        return cy > self.py - MB and cx > self.px - MB
+
+    def scan(self, left, top, right, bottom, step):
+        base_x, base_y = self.bx, self.by
+        for j in range(top, bottom + 1, step):
+            for i in range(left, right + 1, step):
+                cx, cy = base_x + i, base_y + j
+                if self.unstable(cx, cy):
+                    continue
+                if not (0 <= cx <= self.max_x and 0 <= cy <= self.max_y):
+                    continue
+                self.candidate(cx, cy, counted=(i, j) != (0, 0))
+                if self.best_mad < self.thresh:
+                    return
+
+    def subpel(self):
+        self.sp = (0, 0, 0)
+        for j in (-1, 0, 1):
+            for i in (-1, 0, 1):
+                if (i, j) == (0, 0) or self.unstable(self.bx + i, self.by + j):
+                    continue
+                tx, ty = self.bx + i, self.by + j
+                if not (0 <= tx <= self.max_x and 0 <= ty <= self.max_y):
+                    continue
+                if i and j:
+                    self.n_diag += 1
+                else:
+                    self.n_axial += 1
+                blks = self.ref.interp_many(self.pic, [self.bx] * 2, [self.by] * 2, [i] * 2,
+                                            [j] * 2, [0, 1])
+                for quarter in (0, 1):
+                    sad, mad, _ = self.ref.metrics(self.src, blks[quarter])
+                    if self.best_mad < self.thresh:
+                        take = mad < self.best_mad
+                    else:
+                        take = (sad < self.best_sad and sad < SAD_THRESHOLD) or mad < self.thresh
+                    if take:
+                        self.sp = (1, quarter, frac_index(i, j))
+                        self.best_sad, self.best_mad = sad, mad
+
+    def run_intra(self):
+        x, y = self.px, self.py
+        self.bx, self.by = x, y
+        self.n_int = 0
+        self.best_sad = int(np.abs(self.src[:256].astype(np.int64)).sum())
+        self.best_mad = INT32_MAX
+        self.best_ssd = INT32_MAX
+        self.best_cost = np.float32(self.best_sad)
+        self.scan(-SEARCH_RADIUS, -2 * SEARCH_RADIUS, SEARCH_RADIUS, 0, SEARCH_RADIUS)
+        if self.best_mad >= self.thresh:
+            step = SEARCH_RADIUS // 2
+            while step > 0:
+                self.scan(-step, -step, step, step, step)
+                if self.best_mad < self.thresh:
+                    break
+                step >>= 1
+        self.subpel()
+        bt = 1
+        if (self.bx, self.by) != (x, y) or self.sp[0]:
+            bt |= 2
+        if self.best_mad < self.thresh:
+            bt |= 4
+        self.block_type = bt
+        return self
+
+
 def _ref_decisions(w):
     return (int(w["block_type"]), int(w["motion_x"]), int(w["motion_y"]), int(w["sp_pred"]),
             int(w["sp_amount"]), int(w["sp_index"]), int(w["sad"]))
@@ -251,3 +326,22 @@ def test_frame_record_diagnostics(oracle, ref, ftype, nrefs, quality):
         if int(want[m]["prediction_mode"]) == 2:
             assert (int(g["mad_b"]), int(g["cost_b_bits"])) == (
                 reps[1].best_mad, _f32_bits(reps[1].best_cost)), m
+
+
+@pytest.mark.parametrize("quality", [0, 8, 13, 63])
+def test_intra_search_diagnostics_match_replay(oracle, ref, quality):
+    """The same for calculate_intra_prediction, over content with block
+    copies (the tiled cases) and without."""
+    import test_oracle_vs_reference as t
+    for name, src, pic in t.INTRA_CASES:
+        S, P = Picture(*src), Picture(*pic)
+        for yy in range(0, S.height, MB):
+            for xx in range(0, S.width, MB):
+                blk = S.block(xx, yy)
+                rp = IntraReplay(ref, blk, P, xx, yy, quality).run_intra()
+                w = ref.intra_search(blk, P, xx, yy, quality)
+                assert rp.decisions() == _ref_decisions(w), (name, xx, yy)
+                o = oracle.intra_search(blk, P, xx, yy, quality)
+                got = (o.mad, o.cost_bits, o.n_int, o.n_axial, o.n_diag)
+                want = (rp.best_mad, _f32_bits(rp.best_cost), rp.n_int, rp.n_axial, rp.n_diag)
+                assert got == want, (name, xx, yy, got, want)
