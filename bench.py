@@ -78,6 +78,15 @@ def frame_bytes(w, h, n_refs):
     return pic * (1 + n_refs) + n * (384 * 2 * 2 + 64)
 
 
+def shape_table(table):
+    """The autotuner's table (every candidate launch shape with its step
+    time), compacted for the JSON line."""
+    if not table:
+        return None
+    return [{"tuning": {k: v for k, v in r["tuning"].items()}, "step_ms": r["step_ms"],
+             "bit_identical": r.get("bit_identical", True)} for r in table]
+
+
 def cpu_model():
     """The host CPU as /proc/cpuinfo and lscpu describe it (model name,
     family/model/stepping, sockets x cores x threads), plus the cores this
@@ -228,7 +237,7 @@ def run_batched(args, ctx):
     mbs = sum(s.n_mbs for s in streams)
     log(f"batched: uploaded, {mbs} MBs per step")
 
-    tuned, modes = None, None
+    tuned, modes, table = None, None, None
     if not args.no_tune:
         from chicago_amd import autotune
         tuned, table = autotune.tune(ctx, lambda: ctx.analyze_batch(streams, descs),
@@ -292,6 +301,7 @@ def run_batched(args, ctx):
                             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "replays_bit_identical": same,
         "launch_shape": tuned,
+        "launch_shapes": shape_table(table),
         "modes": modes,
     }
     return res, streams, descs, seqs, outs
@@ -341,7 +351,7 @@ def run_plain(args):
     # Launch shape: the autotuner times frames 4-6 (4 references; temporal
     # table reset first, so every shape sees the same inputs, then chained as
     # in the timed run) under each candidate.
-    tuned, modes = None, None
+    tuned, modes, table = None, None, None
     if not args.no_tune:
         from chicago_amd import autotune
 
@@ -441,6 +451,7 @@ def run_plain(args):
             "what": "enqueue of one frame to device idle, sequential frames",
         },
         "launch_shape": tuned,
+        "launch_shapes": shape_table(table),
         "modes": modes,
         "sweeps": sweeps,
     }
@@ -582,7 +593,7 @@ def run_gop(args):
         else:
             s.analyze_b(r.quality, r.index, r.refs[0], r.refs[1], target_ids=[0, 1])
 
-    tuned, modes = None, None
+    tuned, modes, table = None, None, None
     if not args.no_tune:
         from chicago_amd import autotune
         rb = next(r for r in work if r.kind == "B")
@@ -628,6 +639,7 @@ def run_gop(args):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "launch_shape": tuned,
+        "launch_shapes": shape_table(table),
         "modes": modes,
     }
     if not args.no_cpu:
@@ -671,15 +683,21 @@ def run_serve(args):
     from chicago_amd import autotune, evxa, serve
 
     runs = {}
-    for name, shape in (("sweep", autotune._sweep(args.serve_waves, 2)),
-                        ("wavefront", autotune._wavefront(waves_per_search=args.serve_waves))):
+    for name, mode in (("sweep", autotune.SWEEP), ("wavefront", autotune.WAVEFRONT)):
         ctx = evxa.Context(0)
-        if not args.no_tune:
-            ctx.set_tuning(**shape)
+        shapes = [c for c in autotune.SERVE if c["mode"] == mode]
+        table = None
+        if args.no_tune:
+            ctx.set_tuning(**shapes[0])
         cap = full = None
         if args.serve_util > 0:
-            full = args.serve_capacity if args.serve_capacity > 0 else calibrate_serving(ctx, args)
+            if args.serve_capacity > 0:
+                full = args.serve_capacity
+            else:
+                full, table = calibrate_serving(ctx, args, None if args.no_tune else shapes)
             cap = full * args.serve_util
+        elif not args.no_tune:
+            _, table = calibrate_serving(ctx, args, shapes)
             log(f"serve ({name}): admission capacity {cap / 1e6:.2f} M MB/s")
         res = serve.run(ctx, seed=args.serve_seed, n_live=args.serve_streams,
                         duration=args.serve_seconds, quality=args.quality,
@@ -706,6 +724,7 @@ def run_serve(args):
                            f"{args.serve_inflight} launches in flight")
         res["admission_capacity_mb_per_s"] = cap
         res["launch_shape"] = ctx.tuning()[0]
+        res["launch_shapes"] = table
         res["mode"] = name
         runs[name] = res
         ctx.close()
@@ -716,7 +735,7 @@ def run_serve(args):
     other = [r for r in runs.values() if r is not best][0]
     best["other_mode"] = {k: other.get(k) for k in (
         "mode", "frames_per_s", "frames_1080p_eq_per_s", "admitted_lives", "latency_ms",
-        "admission_capacity_mb_per_s", "curve")}
+        "admission_capacity_mb_per_s", "curve", "launch_shape", "launch_shapes")}
     return best
 
 
@@ -972,9 +991,12 @@ def loop_cpu_chain(args, frames, device_out):
     return all(c for _, c in checked), [t for t, _ in checked]
 
 
-def calibrate_serving(ctx, args):
+def calibrate_serving(ctx, args, shapes=None):
     """Sustained MB/s of the dispatcher on a backlog of mix frames (every
-    stream of a 64-stream mix queued for several frames)."""
+    stream of a 64-stream mix queued for several frames).  With `shapes`,
+    the burst runs under each launch shape (outputs compared bit for bit
+    with the first shape's) and the fastest is left applied; returns the
+    capacity and the table."""
     from chicago_amd import evxa, serve, synth
     frames = serve.pool_content(args.serve_seed, [m[:2] for m in serve.C4_MIX], 4)
     descs = serve.descs_for(4, args.quality)
@@ -988,23 +1010,43 @@ def calibrate_serving(ctx, args):
             s.upload(k, *f)
         streams.append(s)
     n_per = 6
-    # The burst runs two launches in flight whatever the serving setting, so
-    # the capacity (and the admitted load) does not depend on it.
-    with evxa.Server(ctx, 2, args.serve_max_frames) as srv:
-        t0 = time.perf_counter()
-        for k in range(1, n_per + 1):
-            for i, s in enumerate(streams):
-                srv.submit(s, descs[k % 4], i * 100 + k)
-        n = 0
-        while n < 64 * n_per:
-            n += len(srv.poll())
-            time.sleep(0.0005)
-        wall = time.perf_counter() - t0
     mbs = sum(s.n_mbs for s in streams) * n_per
+
+    def burst():
+        # Two launches in flight whatever the serving setting, so the
+        # capacity (and the admitted load) does not depend on it.
+        with evxa.Server(ctx, 2, args.serve_max_frames) as srv:
+            t0 = time.perf_counter()
+            for k in range(1, n_per + 1):
+                for i, s in enumerate(streams):
+                    srv.submit(s, descs[k % 4], i * 100 + k)
+            n = 0
+            while n < 64 * n_per:
+                n += len(srv.poll())
+                time.sleep(0.0005)
+            return time.perf_counter() - t0
+
+    table, best, base = [], None, None
+    for shape in (shapes or [None]):
+        if shape is not None:
+            ctx.set_tuning(**shape)
+        burst()                                   # warm-up (and work lists)
+        wall = burst()
+        out = [s.results(coeffs=False, levels=False)[0] for s in streams[:8]]
+        same = base is None or all(np.array_equal(a, b) for a, b in zip(out, base))
+        base = base if base is not None else out
+        cap = mbs / wall
+        table.append({"tuning": shape, "mb_per_s": round(cap), "bit_identical": same})
+        log(f"serve: calibration {shape} {cap / 1e6:.2f} M MB/s, bit-identical={same}")
+        if not same:
+            raise RuntimeError(f"launch shape {shape} changed the outputs")
+        if best is None or cap > best[0]:
+            best = (cap, shape)
+    if best[1] is not None:
+        ctx.set_tuning(**best[1])
     for s in streams:
         s.close()
-    log(f"serve: calibration {mbs / wall / 1e6:.2f} M MB/s")
-    return mbs / wall
+    return best[0], table
 
 
 TRAFFIC_COPY_BYTES = 256 << 20
@@ -1076,7 +1118,6 @@ def main():
     ap.add_argument("--serve-util", type=float, default=0.6)
     ap.add_argument("--serve-capacity", type=float, default=0.0,
                     help="admission capacity in MB/s (0: measure it with a burst)")
-    ap.add_argument("--serve-waves", type=int, default=2)
     ap.add_argument("--serve-curve", default="0.15,0.3",
                     help="further admitted-load fractions to run (latency against throughput)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")

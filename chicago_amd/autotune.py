@@ -12,7 +12,10 @@ The knobs (evxa_set_tuning):
     staggers the frames of a batch;
   * sweeps: sweep 0's order (raster, or anti-diagonal with a bounded wait
     for the neighbours' MVs, frames staggered by the per-diagonal cap) and
-    the tickets claimed at once.
+    the tickets claimed at once;
+  * both: the reach of the reference window each search stages in LDS
+    (a smaller window costs less to stage; levels that leave it load from
+    HBM or stage their own).
 Which shape is fastest depends on the workload: a single frame is bound by
 its MB dependency chain in the wavefront and wants sweeps, a batch of many
 frames wants throughput, and content whose predictor changes travel far
@@ -33,25 +36,33 @@ import numpy as np
 WAVEFRONT, SWEEP = 1, 2
 
 
-def _wavefront(**kw):
-    return dict(mode=WAVEFRONT, **kw)
+REACH = 15          # the kernels' largest window reach (the default)
 
 
-def _sweep(waves, batch, order=1, wait_us=0, **kw):
+def _wavefront(reach=REACH, **kw):
+    return dict(mode=WAVEFRONT, window_reach=reach, **kw)
+
+
+def _sweep(waves, batch, order=1, wait_us=0, reach=REACH, **kw):
     return dict(mode=SWEEP, waves_per_search=waves, claim_batch=batch, sweep_order=order,
-                sweep_reads=2, sweep_wait_us=wait_us, **kw)
+                sweep_reads=2, sweep_wait_us=wait_us, window_reach=reach, **kw)
 
 
-# Candidate shapes per workload class.
+# Candidate shapes per workload class.  Every candidate sets every knob it
+# depends on (set_tuning keeps a knob given as 0).
 SINGLE = ([_wavefront(waves_per_search=w, lists=l) for w, l in itertools.product((1, 2), (1, 8))] +
-          [_sweep(w, b) for w, b in itertools.product((1, 2), (1, 2, 4))])
+          [_sweep(w, b) for w, b in itertools.product((1, 2), (1, 2))] +
+          [_sweep(w, 1, reach=r) for w, r in itertools.product((1, 2), (7, 11))])
 BATCH = ([_wavefront(waves_per_search=w, lists=8, cap_pct=c)
           for w, c in itertools.product((1, 2), (100, 150, 250))] +
+         [_wavefront(waves_per_search=1, lists=8, cap_pct=150, reach=r) for r in (7, 11)] +
          [_wavefront(waves_per_search=1, lists=8, cap_pct=c, max_groups=2048) for c in (150, 250)] +
          [_sweep(1, 1, order=2, wait_us=w, cap_pct=c) for w, c in
           itertools.product((0, 5, 20), (250, 400))] +
+         [_sweep(1, 1, order=2, wait_us=5, cap_pct=250, reach=r) for r in (7, 11)] +
          [_sweep(2, 1, order=2, wait_us=5, cap_pct=250)])
-SERVE = [_wavefront(waves_per_search=w) for w in (1, 2)] + [_sweep(w, 2) for w in (1, 2)]
+SERVE = ([_wavefront(waves_per_search=w) for w in (1, 2)] + [_sweep(w, 1) for w in (1, 2)] +
+         [_sweep(1, 1, reach=r) for r in (7, 11)])
 
 
 def by_mode(table):

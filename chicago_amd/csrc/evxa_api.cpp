@@ -240,6 +240,7 @@ struct evxa_context {
     int32_t sweep_wait_us = 0;       // sweep 0, fresh reads: longest wait for a neighbour
     int32_t sweep_order = EVXA_SWEEP_ORDER_RASTER;  // sweep 0's ticket order
     int32_t claim_batch = 1;         // sweep mode: tickets per claim
+    int32_t reach = 15;              // reach of the window each ticket stages (kernels' maximum)
     int32_t grid = 0;                // most analysis workgroups per launch (0: all that fit)
     int32_t lists = 0;               // work lists per launch (0: automatic)
     int32_t cap_pct = 60;            // tickets per list and virtual diagonal, % of its workgroups
@@ -384,6 +385,7 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     j.quality = d.quality;
     j.n_refs = d.n_refs;
     j.thresh = (d.quality >> 2) + 1;   // motion.cpp:589
+    j.reach = s->ctx->reach;
     j.lambda = lambda_of(d.quality);
     j.target_ids = (uint32_t)d.target_ids[0] | ((uint32_t)d.target_ids[1] << 8) |
                    ((uint32_t)d.target_ids[2] << 16) | ((uint32_t)d.target_ids[3] << 24);
@@ -789,7 +791,7 @@ void check_last_launch(evxa_context *ctx)
 
 extern "C" {
 
-int32_t evxa_version(void) { return 104; }
+int32_t evxa_version(void) { return 105; }
 
 const char *evxa_last_error(void) { return g_last_error.c_str(); }
 
@@ -1905,7 +1907,8 @@ int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t)
             (t->mode != 0 && t->mode != EVXA_MODE_WAVEFRONT && t->mode != EVXA_MODE_SWEEP) ||
             t->sweep_reads < 0 || t->sweep_reads > 2 || t->sweep_wait_us < -1 ||
             t->sweep_wait_us > 100000 || t->sweep_order < 0 || t->sweep_order > 2 ||
-            t->claim_batch < 0 || t->claim_batch > kMaxClaimBatch)
+            t->claim_batch < 0 || t->claim_batch > kMaxClaimBatch || t->window_reach < 0 ||
+            t->window_reach > 15)
             fail(EVXA_ERROR_INVALIDARG, "tuning out of range");
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (t->waves_per_search) ctx->waves = t->waves_per_search;
@@ -1914,6 +1917,7 @@ int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t)
         if (t->sweep_wait_us >= 0) ctx->sweep_wait_us = t->sweep_wait_us;
         if (t->sweep_order) ctx->sweep_order = t->sweep_order;
         if (t->claim_batch) ctx->claim_batch = t->claim_batch;
+        if (t->window_reach) ctx->reach = t->window_reach;
         ctx->grid = t->max_groups;
         ctx->lists = t->lists;
         ctx->cap_pct = t->cap_pct ? t->cap_pct : 60;
@@ -1934,6 +1938,7 @@ int evxa_get_tuning(evxa_context *ctx, evxa_tuning *t, int32_t *resident_groups)
         t->sweep_wait_us = ctx->sweep_wait_us;
         t->sweep_order = ctx->sweep_order;
         t->claim_batch = ctx->claim_batch;
+        t->window_reach = ctx->reach;
         if (resident_groups) *resident_groups = ctx->resident[ctx->mode == EVXA_MODE_SWEEP ? 1 : 0][ctx->waves];
     });
 }

@@ -78,6 +78,7 @@ struct DevJob {
     uint64_t *mvw;                     // per MB: predictor word, zeroed per launch
     int32_t eager;                     // P with several references, alone in its launch:
                                        // every reference's wave tries the selection
+    int32_t reach;                     // reach of the search window staged per ticket (1..15)
     // Sweep mode, per MB: the predictor words by sweep parity (2 per MB;
     // parity 0 is zero at the start of a launch, so sweep 0 can tell which
     // words are published), the key of the predictor the current decision

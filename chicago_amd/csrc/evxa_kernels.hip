@@ -1793,7 +1793,7 @@ __global__ __launch_bounds__(64 * NW, 3) void analyze_kernel(const DevJob *__res
         Blk s, pre;
         load_cand(job.src, geo, x, y, s);
         load_cand(rp, geo, x, y, pre);
-        const WinPos win = stage_window<NW>(rp, x, y, kReachMax, sh.sp);
+        const WinPos win = stage_window<NW>(rp, x, y, job.reach, sh.sp);
         // The temporal table's record of this MB (written by an earlier launch).
         uint32_t p0 = 0, p1 = 0;
         if (job.prev) {
@@ -2434,7 +2434,7 @@ __device__ void search_ticket(const DevJob &job, int jidx, int ref, int mb, int 
     Blk src, pre;
     load_cand(job.src, geo, x, y, src);
     load_cand(rp, geo, x, y, pre);
-    const WinPos win = stage_window<NW>(rp, x, y, kReachMax, sh.sp);
+    const WinPos win = stage_window<NW>(rp, x, y, job.reach, sh.sp);
     int pmx, pmy;
     if (s == 0 && ref == 0) {
         const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;

@@ -76,7 +76,7 @@ class Tuning(C.Structure):
     _fields_ = [("waves_per_search", C.c_int32), ("max_groups", C.c_int32), ("lists", C.c_int32),
                 ("cap_pct", C.c_int32), ("mode", C.c_int32), ("sweep_reads", C.c_int32),
                 ("sweep_wait_us", C.c_int32), ("sweep_order", C.c_int32),
-                ("claim_batch", C.c_int32)]
+                ("claim_batch", C.c_int32), ("window_reach", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -258,9 +258,10 @@ class Context:
         return _lib.evxa_context_bytes(self.h)
 
     def set_tuning(self, waves_per_search=0, max_groups=0, lists=0, cap_pct=0, mode=0,
-                   sweep_reads=0, sweep_wait_us=-1, sweep_order=0, claim_batch=0):
+                   sweep_reads=0, sweep_wait_us=-1, sweep_order=0, claim_batch=0,
+                   window_reach=0):
         t = Tuning(waves_per_search, max_groups, lists, cap_pct, mode, sweep_reads, sweep_wait_us,
-                   sweep_order, claim_batch)
+                   sweep_order, claim_batch, window_reach)
         _check(_lib.evxa_set_tuning(self.h, C.byref(t)))
 
     def sweep_stats(self, max_n=4096, times=False):

@@ -326,6 +326,10 @@ typedef struct evxa_tuning {
     int32_t sweep_order;          /* EVXA_SWEEP_ORDER_* (0: keep) */
     int32_t claim_batch;          /* sweep mode: consecutive tickets a workgroup claims at
                                      once, 1..16 (default 1; 0: keep) */
+    int32_t window_reach;         /* reach of the reference window each (MB, reference)
+                                     search stages in LDS around the MB, 1..15 samples
+                                     (default 15; 0: keep).  Levels whose candidates leave
+                                     it load from HBM (step 16) or stage their own. */
 } evxa_tuning;
 
 int evxa_set_tuning(evxa_context *ctx, const evxa_tuning *t);
