@@ -437,7 +437,7 @@ struct HalfPlanes {
 constexpr int kReachMax = 15;
 constexpr int kWinLS = 54;    // luma row stride (>= 2 * 15 + 23, even)
 constexpr int kWinLR = 52;    // luma rows (2 * 15 + 22)
-constexpr int kWinCS = 32;    // chroma row stride (>= 15 + 16, even)
+constexpr int kWinCS = 34;    // chroma row stride (>= 15 + 16, even; 17 dwords spread rows over the LDS banks)
 constexpr int kWinCR = 30;    // chroma rows (15 + 15)
 
 struct Window {
