@@ -278,7 +278,7 @@ int evxa_search_blocks(evxa_stream *s, int32_t src_slot, int32_t ref_slot, int32
 int evxa_intra_search_blocks(evxa_stream *s, int32_t src_slot, int32_t pic_slot, int32_t quality,
                              int32_t n, const int32_t *xs, const int32_t *ys,
                              evxa_search_result *out);
-/* sub_transform_macroblock + query_block_quantization_parameter +/* sub_transform_macroblock + query_block_quantization_parameter +
+/* sub_transform_macroblock + query_block_quantization_parameter +
  * compute_block_variance2 + quantize_macroblock (inter) for n residual
  * pairs (macroblock.h:272-277, quantize.cpp:97-118, 372-397;
  * analysis.h:253-275).  src/pred/coeffs/levels hold n * 384 samples. */
