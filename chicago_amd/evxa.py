@@ -34,7 +34,7 @@ RECORD_DTYPE = np.dtype([
     ("prediction_target_b", "u1"), ("motion_x", "<i2"), ("motion_y", "<i2"),
     ("motion_x_b", "<i2"), ("motion_y_b", "<i2"), ("sp_pred", "u1"), ("sp_amount", "u1"),
     ("sp_index", "u1"), ("sp_pred_b", "u1"), ("sp_amount_b", "u1"), ("sp_index_b", "u1"),
-    ("q_index", "u1"), ("reserved0", "u1"), ("variance", "<i2"), ("reserved1", "<i2"),
+    ("q_index", "u1"), ("intra_mode", "u1"), ("variance", "<i2"), ("reserved1", "<i2"),
     ("sad", "<i4"), ("mad", "<i4"), ("cost_bits", "<u4"), ("sad_b", "<i4"), ("mad_b", "<i4"),
     ("cost_b_bits", "<u4"), ("n_int", "<u4"), ("n_axial", "<u2"), ("n_diag", "<u2"),
     ("flags", "<u4"), ("reserved2", "<u4")])

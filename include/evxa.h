@@ -58,7 +58,7 @@ typedef struct evxa_block_record {
     int16_t motion_x, motion_y;
     int16_t motion_x_b, motion_y_b;
     uint8_t sp_pred, sp_amount, sp_index, sp_pred_b;
-    uint8_t sp_amount_b, sp_index_b, q_index, reserved0;
+    uint8_t sp_amount_b, sp_index_b, q_index, intra_mode;
     int16_t variance;
     int16_t reserved1;
     int32_t sad;

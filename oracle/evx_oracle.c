@@ -1333,3 +1333,662 @@ void evo_sao_encode_frame(const int16_t *src_y, const int16_t *src_u, const int1
         }
     evo_sao_filter_frame(table, width, height, rec_y, rec_u, rec_v);
 }
+
+/* ===================================================================== */
+/* Mode decision and the in-loop encode of a frame (encode.cpp:39-370,   */
+/* 479-748, 751-813; decode.cpp:31-433; intra_pred.cpp; the trellis of   */
+/* quantize.cpp:478-682).                                                */
+/* ===================================================================== */
+
+#define EVO_BT_INTRA_DEFAULT EVO_BT_INTRA   /* types.h:83 */
+
+/* Intra quantisation matrix, quantize.cpp:13-23. */
+static const int16_t kIntraQm[64] = {
+     8, 10, 12, 15, 19, 23, 25, 27,
+    10, 12, 14, 17, 21, 25, 27, 28,
+    12, 14, 18, 21, 24, 26, 28, 30,
+    15, 17, 21, 24, 26, 28, 30, 32,
+    19, 21, 24, 26, 28, 30, 32, 35,
+    23, 25, 26, 28, 30, 32, 35, 38,
+    25, 27, 28, 30, 32, 35, 38, 41,
+    27, 28, 30, 32, 35, 38, 41, 45};
+
+/* Intra deadzone weights, quantize.cpp:42-52. */
+static const uint8_t kIntraDeadzone[64] = {
+    0, 1, 1, 1, 1, 1, 2, 2,
+    1, 1, 1, 1, 1, 2, 2, 2,
+    1, 1, 1, 1, 2, 2, 2, 2,
+    1, 1, 1, 2, 2, 2, 2, 3,
+    1, 1, 2, 2, 2, 2, 3, 3,
+    1, 2, 2, 2, 2, 3, 3, 3,
+    2, 2, 2, 2, 3, 3, 3, 3,
+    2, 2, 2, 3, 3, 3, 3, 3};
+
+/* evx_math.h:64 with the byte log2 of 69-87: floor(log2(n)) + 1 of the
+ * value truncated to 8 bits, log2(0) = 0. */
+static uint32_t required_bits_u8(uint32_t n)
+{
+    return floor_log2_u32(n & 0xFFu) + 1;
+}
+
+/* ------------------------------------------------ intra prediction -- */
+
+/* intra_pred.cpp:12-52: the row above, the column left and the corner of
+ * the n x n block at (cx, cy) of one plane, 128 where absent. */
+static void intra_edges(const int16_t *d, int32_t stride, int32_t cx, int32_t cy, int32_t n,
+                        int16_t *top, int16_t *left, int16_t *corner, int *has_top,
+                        int *has_left)
+{
+    *has_top = cy > 0;
+    *has_left = cx > 0;
+    for (int32_t i = 0; i < n; i++) {
+        top[i] = *has_top ? d[(cy - 1) * stride + cx + i] : 128;
+        left[i] = *has_left ? d[(cy + i) * stride + cx - 1] : 128;
+    }
+    if (*has_top && *has_left) *corner = d[(cy - 1) * stride + cx - 1];
+    else if (*has_top) *corner = top[0];
+    else if (*has_left) *corner = left[0];
+    else *corner = 128;
+}
+
+/* intra_pred.cpp:58-171: one plane's n x n prediction (out has stride n). */
+static void intra_predict_plane(int32_t mode, const int16_t *top, const int16_t *left,
+                                int16_t corner, int has_top, int has_left, int32_t n,
+                                int16_t *out)
+{
+    int16_t ref[33];
+    if (mode == 4 || mode == 5 || mode == 6) {
+        /* left[n-1] .. left[0], corner, top[0] .. top[n-1] */
+        for (int32_t i = 0; i < n; i++) ref[i] = left[n - 1 - i];
+        ref[n] = corner;
+        for (int32_t i = 0; i < n; i++) ref[n + 1 + i] = top[i];
+    }
+    for (int32_t y = 0; y < n; y++)
+        for (int32_t x = 0; x < n; x++) {
+            int32_t v;
+            switch (mode) {
+            case 0: { /* DC */
+                int32_t sum = 0, count = 0;
+                if (has_top) { for (int32_t i = 0; i < n; i++) sum += top[i]; count += n; }
+                if (has_left) { for (int32_t i = 0; i < n; i++) sum += left[i]; count += n; }
+                v = count > 0 ? (int16_t)((sum + count / 2) / count) : 128;
+            } break;
+            case 1: v = top[x]; break;            /* vertical */
+            case 2: v = left[y]; break;           /* horizontal */
+            case 3: {                             /* diagonal down-left */
+                int32_t i0 = x + y;
+                int32_t a = top[i0 < n - 1 ? i0 : n - 1];
+                int32_t b = top[i0 + 1 < n - 1 ? i0 + 1 : n - 1];
+                int32_t c = top[i0 + 2 < n - 1 ? i0 + 2 : n - 1];
+                v = (a + 2 * b + c + 2) >> 2;
+            } break;
+            case 4: {                             /* diagonal down-right */
+                int32_t d = x - y + n;
+                v = (ref[d - 1] + 2 * ref[d] + ref[d + 1] + 2) >> 2;
+            } break;
+            case 5:                               /* vertical-right */
+            case 6: {                             /* horizontal-down */
+                int32_t base = mode == 5 ? x - (y >> 1) : (x >> 1) - y;
+                int32_t odd = mode == 5 ? (y & 1) : (x & 1);
+                int32_t d = base + n;
+                if (d < 1) d = 1;
+                if (d > 2 * n - 1) d = 2 * n - 1;
+                v = odd ? (ref[d - 1] + 2 * ref[d] + ref[d + 1] + 2) >> 2
+                        : (ref[d] + ref[d + 1] + 1) >> 1;
+            } break;
+            default: v = 128; break;
+            }
+            out[y * n + x] = (int16_t)v;
+        }
+}
+
+void evo_intra_prediction(int32_t mode, const evo_planes *pic, int32_t x, int32_t y,
+                          int16_t out[EVO_MB_SAMPLES])
+{
+    /* intra_pred.cpp:177-215 */
+    int16_t top[16], left[16], corner;
+    int has_top, has_left;
+    intra_edges(pic->y, pic->stride_y, x, y, 16, top, left, &corner, &has_top, &has_left);
+    intra_predict_plane(mode, top, left, corner, has_top, has_left, 16, out);
+    intra_edges(pic->u, pic->stride_c, x >> 1, y >> 1, 8, top, left, &corner, &has_top, &has_left);
+    intra_predict_plane(mode, top, left, corner, has_top, has_left, 8, out + 256);
+    intra_edges(pic->v, pic->stride_c, x >> 1, y >> 1, 8, top, left, &corner, &has_top, &has_left);
+    intra_predict_plane(mode, top, left, corner, has_top, has_left, 8, out + 320);
+}
+
+void evo_intra_mode_candidates(const evo_planes *pic, int32_t x, int32_t y, uint8_t cand[3])
+{
+    /* intra_pred.cpp:304-326 (also the candidate set of 227-268): luma edge
+     * gradients pick a horizontal, vertical or diagonal family. */
+    int16_t top[16], left[16], corner;
+    int has_top, has_left;
+    intra_edges(pic->y, pic->stride_y, x, y, 16, top, left, &corner, &has_top, &has_left);
+    int32_t gh = 0, gv = 0;
+    if (has_top)
+        for (int i = 1; i < 16; i++) gh += abs32(top[i] - top[i - 1]);
+    if (has_left)
+        for (int i = 1; i < 16; i++) gv += abs32(left[i] - left[i - 1]);
+    cand[0] = 0;
+    if (gv > gh * 3 / 2) { cand[1] = 2; cand[2] = 6; }
+    else if (gh > gv * 3 / 2) { cand[1] = 1; cand[2] = 5; }
+    else { cand[1] = 3; cand[2] = 4; }
+}
+
+int32_t evo_select_intra_mode(const int16_t src[EVO_MB_SAMPLES], const evo_planes *pic,
+                              int32_t x, int32_t y, int32_t *sad)
+{
+    /* intra_pred.cpp:217-302 (EVX_ENABLE_FAST_INTRA_MODE): the first of the
+     * three candidates with the smallest luma SAD. */
+    uint8_t cand[3];
+    int16_t pred[EVO_MB_SAMPLES];
+    evo_intra_mode_candidates(pic, x, y, cand);
+    int32_t best_sad = EVO_INT32_MAX, best_mode = 0;
+    for (int c = 0; c < 3; c++) {
+        evo_intra_prediction(cand[c], pic, x, y, pred);
+        int32_t s = evo_block_sad(src, pred);
+        if (s < best_sad) { best_sad = s; best_mode = cand[c]; }
+    }
+    if (sad) *sad = best_sad;
+    return best_mode;
+}
+
+/* ------------------------------------------ intra (de)quantisation -- */
+
+/* quantize.cpp:74-95 */
+static int32_t luma_dc_scale(int32_t qp)
+{
+    if (qp < 5) return (qp << 1) > 2 ? (qp << 1) : 2;
+    if (qp < 9) return qp << 1;
+    if (qp < 25) return qp + 8;
+    return (qp << 1) - 16;
+}
+
+static int32_t chroma_dc_scale(int32_t qp)
+{
+    if (qp < 5) return (qp << 1) > 2 ? (qp << 1) : 2;
+    if (qp < 25) return (qp + 13) >> 1;
+    return qp - 6;
+}
+
+/* quantize.cpp:120-200 */
+static void quantize_intra_8x8(int32_t qp, int32_t chroma, const int16_t *src, int32_t sp,
+                               int16_t *dst, int32_t dp)
+{
+    for (int j = 0; j < 8; j++)
+        for (int k = 0; k < 8; k++) {
+            int16_t qf = (int16_t)rounded_div(src[j * sp + k] * EVO_QUANT_SCALE, kIntraQm[j * 8 + k]);
+            if (qp >= 16 && !(k == 0 && j == 0)) {
+                int16_t dz = (int16_t)((sign16(qf) * qp * kIntraDeadzone[j * 8 + k]) >> 1);
+                qf = (int16_t)(qf - dz);
+            }
+            dst[j * dp + k] = (int16_t)rounded_div(qf, qp << 1);
+        }
+    dst[0] = (int16_t)rounded_div(src[0], chroma ? chroma_dc_scale(qp) : luma_dc_scale(qp));
+}
+
+/* quantize.cpp:271-301 */
+static void dequantize_intra_8x8(int32_t qp, int32_t chroma, const int16_t *src, int32_t sp,
+                                 int16_t *dst, int32_t dp)
+{
+    for (int j = 0; j < 8; j++)
+        for (int k = 0; k < 8; k++)
+            dst[j * dp + k] = (int16_t)((2 * src[j * sp + k] * kIntraQm[j * 8 + k] * qp) / EVO_QUANT_SCALE);
+    dst[0] = (int16_t)(src[0] * (chroma ? chroma_dc_scale(qp) : luma_dc_scale(qp)));
+}
+
+static int is_intra_quant(int32_t bt)
+{
+    /* quantize.cpp:457, 469: intra matrices for intra blocks without motion. */
+    return (bt & EVO_BT_INTRA) && !(bt & EVO_BT_MOTION);
+}
+
+void evo_quantize_mb(int32_t qp, int32_t block_type, const int16_t coeffs[EVO_MB_SAMPLES],
+                     int16_t levels[EVO_MB_SAMPLES])
+{
+    /* quantize.cpp:454-464 with 345-370 */
+    if (!is_intra_quant(block_type)) {
+        evo_quantize_inter_mb(qp, coeffs, levels);
+        return;
+    }
+    int32_t cqp = qp + EVO_CHROMA_QP_OFFSET;
+    if (cqp > EVO_MAX_QUANT_LEVELS - 1) cqp = EVO_MAX_QUANT_LEVELS - 1;
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        quantize_intra_8x8(qp, 0, coeffs + off, 16, levels + off, 16);
+    }
+    quantize_intra_8x8(cqp, 1, coeffs + 256, 8, levels + 256, 8);
+    quantize_intra_8x8(cqp, 1, coeffs + 320, 8, levels + 320, 8);
+}
+
+void evo_dequantize_mb(int32_t qp, int32_t block_type, const int16_t levels[EVO_MB_SAMPLES],
+                       int16_t coeffs[EVO_MB_SAMPLES])
+{
+    /* quantize.cpp:466-476 with 399-424 */
+    if (!is_intra_quant(block_type)) {
+        evo_dequantize_inter_mb(qp, levels, coeffs);
+        return;
+    }
+    int32_t cqp = qp + EVO_CHROMA_QP_OFFSET;
+    if (cqp > EVO_MAX_QUANT_LEVELS - 1) cqp = EVO_MAX_QUANT_LEVELS - 1;
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        dequantize_intra_8x8(qp, 0, levels + off, 16, coeffs + off, 16);
+    }
+    dequantize_intra_8x8(cqp, 1, levels + 256, 8, coeffs + 256, 8);
+    dequantize_intra_8x8(cqp, 1, levels + 320, 8, coeffs + 320, 8);
+}
+
+/* ------------------------------------------------------------ trellis -- */
+
+/* scan.h:52-62 (zig-zag), 117-127 (rows), 137-147 (columns); the scan of a
+ * block follows its intra mode (scan.h:154-159). */
+static void scan_order(int32_t intra_mode, uint8_t order[64])
+{
+    if (intra_mode == 1 || intra_mode == 5) {
+        for (int p = 0; p < 64; p++) order[p] = (uint8_t)p;
+    } else if (intra_mode == 2 || intra_mode == 6) {
+        for (int p = 0; p < 64; p++) order[p] = (uint8_t)((p & 7) * 8 + (p >> 3));
+    } else {
+        /* JPEG zig-zag: anti-diagonals d = r + c, odd ones walked downwards. */
+        int p = 0;
+        for (int d = 0; d < 15; d++)
+            for (int t = 0; t <= d; t++) {
+                int r = (d & 1) ? t : d - t;
+                int c = d - r;
+                if (r < 8 && c < 8) order[p++] = (uint8_t)(r * 8 + c);
+            }
+    }
+}
+
+/* quantize.cpp:481-514 */
+static int16_t trellis_dequant(int16_t level, int32_t qp, int32_t pos, int intra, int chroma)
+{
+    if (level == 0) return 0;
+    if (intra) {
+        if (pos == 0) return (int16_t)(level * (chroma ? chroma_dc_scale(qp) : luma_dc_scale(qp)));
+        return (int16_t)((2 * level * kIntraQm[pos] * qp) / EVO_QUANT_SCALE);
+    }
+    return (int16_t)((2 * level * kInterQm[pos] * qp) / EVO_QUANT_SCALE);
+}
+
+/* quantize.cpp:517-532 (EVX_ENABLE_SIGMAP_CODING) */
+static uint32_t coeff_bits(int16_t level)
+{
+    if (level == 0) return 1;
+    int32_t a = abs32(level);
+    uint32_t v = (uint32_t)(a < 255 ? a : 255);
+    uint32_t bits = 2;
+    if (v > 1) bits += 1;
+    if (v > 2) bits += 1 + 2 * required_bits_u8(v - 2);
+    else bits += 1;
+    return bits;
+}
+
+static float trellis_cost(int16_t coeff, int16_t level, int32_t qp, int32_t pos, int intra,
+                          int chroma, float lambda)
+{
+    int32_t diff = (int32_t)coeff - (int32_t)trellis_dequant(level, qp, pos, intra, chroma);
+    int32_t sq = (int32_t)((uint32_t)diff * (uint32_t)diff);
+    float rate = lambda * (float)coeff_bits(level);
+    return (float)sq + rate;
+}
+
+/* quantize.cpp:536-613 on one 8x8 block (coefficients and levels with
+ * stride sp). */
+static void trellis_8x8(int32_t qp, int intra, int chroma, const int16_t *co, int16_t *lv,
+                        int32_t sp, const uint8_t *scan, float lambda)
+{
+#define AT(a, pos) (a)[((pos) >> 3) * sp + ((pos) & 7)]
+    int32_t last = -1;
+    for (int32_t p = 63; p >= 0; p--)
+        if (AT(lv, scan[p]) != 0) { last = p; break; }
+    if (last < 0) return;
+    for (int32_t p = last; p >= 1; p--) {
+        int32_t pos = scan[p];
+        int16_t level = AT(lv, pos);
+        if (level == 0) continue;
+        float keep = trellis_cost(AT(co, pos), level, qp, pos, intra, chroma, lambda);
+        float zero = trellis_cost(AT(co, pos), 0, qp, pos, intra, chroma, lambda);
+        if (zero <= keep) AT(lv, pos) = 0;
+        else break;
+    }
+    last = -1;
+    for (int32_t p = 63; p >= 0; p--)
+        if (AT(lv, scan[p]) != 0) { last = p; break; }
+    if (last < 0) return;
+    for (int32_t p = 1; p <= last; p++) {
+        int32_t pos = scan[p];
+        int16_t level = AT(lv, pos);
+        if (level == 0) continue;
+        int16_t reduced = (int16_t)(level - sign16(level));
+        float keep = trellis_cost(AT(co, pos), level, qp, pos, intra, chroma, lambda);
+        float red = trellis_cost(AT(co, pos), reduced, qp, pos, intra, chroma, lambda);
+        if (red < keep) AT(lv, pos) = reduced;
+    }
+#undef AT
+}
+
+void evo_trellis_mb(int32_t qp, int32_t block_type, int32_t intra_mode, float lambda,
+                    const int16_t coeffs[EVO_MB_SAMPLES], int16_t levels[EVO_MB_SAMPLES])
+{
+    /* quantize.cpp:615-680 */
+    uint8_t scan[64];
+    scan_order(intra_mode, scan);
+    int intra = is_intra_quant(block_type);
+    int32_t cqp = qp + EVO_CHROMA_QP_OFFSET;
+    if (cqp > EVO_MAX_QUANT_LEVELS - 1) cqp = EVO_MAX_QUANT_LEVELS - 1;
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        trellis_8x8(qp, intra, 0, coeffs + off, levels + off, 16, scan, lambda);
+    }
+    trellis_8x8(cqp, intra, 1, coeffs + 256, levels + 256, 8, scan, lambda);
+    trellis_8x8(cqp, intra, 1, coeffs + 320, levels + 320, 8, scan, lambda);
+}
+
+/* --------------------------------------------------------- rate model -- */
+
+static uint32_t mv_bits(int32_t mx, int32_t my, int32_t sp_pred)
+{
+    /* encode.cpp:49-53, 65-69 */
+    uint32_t bits = 2 * required_bits_u8((uint32_t)(abs32(mx) + 1)) +
+                    2 * required_bits_u8((uint32_t)(abs32(my) + 1));
+    return bits + (sp_pred ? 5 : 1);
+}
+
+uint32_t evo_block_bits(const evo_record *d, const int16_t block[EVO_MB_SAMPLES])
+{
+    /* encode.cpp:39-110 (EVX_ENABLE_SIGMAP_CODING, EVX_ENABLE_MPM) */
+    int32_t bt = d->block_type;
+    uint32_t bits = 3;
+    if (!(bt & EVO_BT_INTRA)) bits += 2;
+    if (bt & EVO_BT_MOTION) bits += mv_bits(d->motion_x, d->motion_y, d->sp_pred);
+    if (bt & EVO_BT_COPY) return bits;
+    if (bt == EVO_BT_INTRA_DEFAULT) bits += 3;
+    bits += 2 * required_bits_u8((uint32_t)d->q_index + 1);
+    for (int b = 0; b < 4; b++) {
+        int off = (b >> 1) * 8 * 16 + (b & 1) * 8;
+        uint32_t nz = 0, cb = 0;
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                int16_t v = block[off + r * 16 + c];
+                if (v != 0) {
+                    nz++;
+                    cb += coeff_bits(v);
+                }
+            }
+        bits += required_bits_u8(nz + 1) * 2 + 1;
+        bits += cb;
+    }
+    return bits;
+}
+
+/* ------------------------------------------------- frame encode driver -- */
+
+typedef struct encode_ctx {
+    const evo_encode_job *job;
+    const evo_planes *pic;         /* the picture being reconstructed */
+    float lambda;
+} encode_ctx;
+
+/* The MB prediction a descriptor names (encode.cpp:485-723, decode.cpp:
+ * 35-433): intra modes over the picture being reconstructed, intra motion
+ * over the same picture, inter blocks over their references (averaged for
+ * bidirectional records).  Returns 0 if the record names an unknown target. */
+static int prediction_of(const encode_ctx *e, const evo_record *d, int32_t x, int32_t y,
+                         int16_t out[EVO_MB_SAMPLES])
+{
+    const evo_encode_job *job = e->job;
+    int32_t bt = d->block_type;
+    if (bt == EVO_BT_INTRA_DEFAULT) {
+        if (d->intra_mode != EVO_INTRA_NONE) evo_intra_prediction(d->intra_mode, e->pic, x, y, out);
+        else memset(out, 0, EVO_MB_SAMPLES * sizeof(int16_t));
+        return 1;
+    }
+    evo_search_result s;
+    memset(&s, 0, sizeof(s));
+    s.motion_x = d->motion_x; s.motion_y = d->motion_y;
+    s.sp_enabled = d->sp_pred; s.sp_amount = d->sp_amount; s.sp_index = d->sp_index;
+    if (bt & EVO_BT_INTRA) {
+        evo_predict_block(e->pic, x, y, &s, out);
+        return 1;
+    }
+    int k = -1, kb = -1;
+    for (int i = 0; i < job->n_refs; i++) {
+        if (job->target_ids[i] == d->prediction_target && k < 0) k = i;
+        if (job->target_ids[i] == d->prediction_target_b && kb < 0) kb = i;
+    }
+    if (k < 0) return 0;
+    evo_predict_block(&job->refs[k], x, y, &s, out);
+    if (job->type == 2 && d->prediction_mode == 2) {
+        int16_t pb[EVO_MB_SAMPLES];
+        if (kb < 0) return 0;
+        s.motion_x = d->motion_x_b; s.motion_y = d->motion_y_b;
+        s.sp_enabled = d->sp_pred_b; s.sp_amount = d->sp_amount_b; s.sp_index = d->sp_index_b;
+        evo_predict_block(&job->refs[kb], x, y, &s, pb);
+        for (int i = 0; i < EVO_MB_SAMPLES; i++) out[i] = (int16_t)((out[i] + pb[i] + 1) >> 1);
+    }
+    return 1;
+}
+
+/* encode_block (encode.cpp:479-748): residual, QP, variance, quantisation
+ * and trellis into lv; copies are left alone (lv zeroed here). */
+static int encode_mb(const encode_ctx *e, const int16_t *src, evo_record *d, int32_t x,
+                     int32_t y, int16_t co[EVO_MB_SAMPLES], int16_t lv[EVO_MB_SAMPLES])
+{
+    int16_t pred[EVO_MB_SAMPLES];
+    if (d->block_type & EVO_BT_COPY) {
+        memset(lv, 0, EVO_MB_SAMPLES * sizeof(int16_t));
+        memset(co, 0, EVO_MB_SAMPLES * sizeof(int16_t));
+        return 1;
+    }
+    if (!prediction_of(e, d, x, y, pred)) return 0;
+    evo_sub_transform_mb(src, pred, co);
+    int32_t qp = evo_block_qp(e->job->quality, co);
+    d->q_index = (uint8_t)qp;
+    d->variance = (int16_t)evo_variance2(co);
+    evo_quantize_mb(qp, d->block_type, co, lv);
+    evo_trellis_mb(qp, d->block_type, d->intra_mode, e->lambda, co, lv);
+    return 1;
+}
+
+/* decode_block (decode.cpp:31-433): the reconstruction of one MB. */
+static int decode_mb(const encode_ctx *e, const evo_record *d, int32_t x, int32_t y,
+                     const int16_t lv[EVO_MB_SAMPLES], int16_t out[EVO_MB_SAMPLES])
+{
+    int16_t pred[EVO_MB_SAMPLES], co[EVO_MB_SAMPLES];
+    if (!prediction_of(e, d, x, y, pred)) return 0;
+    if (d->block_type & EVO_BT_COPY) {
+        memcpy(out, pred, sizeof(pred));
+        return 1;
+    }
+    evo_dequantize_mb(d->q_index, d->block_type, lv, co);
+    evo_inverse_transform_add_mb(co, pred, out);
+    return 1;
+}
+
+/* compute_rdo_cost (encode.cpp:128-166) of a candidate, which gets its
+ * q_index and variance as encode_block sets them.  Distortion is the luma
+ * SSD of the reconstruction.  The reference decodes into the buffer that
+ * holds the candidate's levels (both are views of the staging cache at
+ * (0, 0), encode.cpp:150-156), so the residual rate is estimated from the
+ * reconstructed luma samples. */
+static int rdo_cost(const encode_ctx *e, const int16_t *src, evo_record *d, int32_t x, int32_t y,
+                    float *cost)
+{
+    int16_t co[EVO_MB_SAMPLES], lv[EVO_MB_SAMPLES], rec[EVO_MB_SAMPLES];
+    if (!encode_mb(e, src, d, x, y, co, lv)) return 0;
+    if (!decode_mb(e, d, x, y, lv, rec)) return 0;
+    uint32_t ssd = 0;
+    for (int k = 0; k < 256; k++) {
+        int32_t t = src[k] - rec[k];
+        ssd += (uint32_t)t * (uint32_t)t;
+    }
+    uint32_t rate = evo_block_bits(d, rec);
+    float rt = e->lambda * (float)rate;
+    *cost = (float)(int32_t)ssd + rt;
+    return 1;
+}
+
+static void desc_clear(evo_record *d)
+{
+    /* clear_block_desc (common.cpp:78-94) */
+    memset(d, 0, sizeof(*d));
+    d->block_type = EVO_BT_INTRA_DEFAULT;
+    d->intra_mode = EVO_INTRA_NONE;
+}
+
+static void desc_from_search(evo_record *d, const evo_search_result *r, uint8_t target)
+{
+    desc_clear(d);
+    d->block_type = (uint8_t)r->block_type;
+    d->prediction_target = target;
+    d->motion_x = (int16_t)r->motion_x;
+    d->motion_y = (int16_t)r->motion_y;
+    d->sp_pred = (uint8_t)r->sp_enabled;
+    d->sp_amount = (uint8_t)r->sp_amount;
+    d->sp_index = (uint8_t)r->sp_index;
+}
+
+int32_t evo_encode_frame(const evo_encode_job *job, int16_t *pic_y, int16_t *pic_u,
+                         int16_t *pic_v, evo_record *records, int16_t *levels)
+{
+    int32_t w = job->src.width, h = job->src.height, wb = w / EVO_MB, hb = h / EVO_MB;
+    evo_planes pic = {pic_y, pic_u, pic_v, w, h, w, w / 2};
+    encode_ctx e = {job, &pic, evo_lambda(job->quality)};
+    if (job->type < 0 || job->type > 2 || job->n_refs != (job->type == 0 ? 0 : job->type))
+        return 3;
+    int16_t src[EVO_MB_SAMPLES], co[EVO_MB_SAMPLES], rec[EVO_MB_SAMPLES];
+    int16_t pf[EVO_MB_SAMPLES], pb[EVO_MB_SAMPLES], avg[EVO_MB_SAMPLES];
+
+    for (int32_t by = 0; by < hb; by++)
+        for (int32_t bx = 0; bx < wb; bx++) {
+            uint32_t idx = (uint32_t)(by * wb + bx);
+            int32_t x = bx * EVO_MB, y = by * EVO_MB;
+            evo_load_block(&job->src, x, y, src);
+            evo_record best;
+            evo_search_result ri, r[2];
+            uint32_t n_int, n_ax, n_dg;
+
+            /* calculate_intra_prediction (motion.cpp:485-565) over the
+             * picture being reconstructed: every frame type starts with it
+             * (encode.cpp:175, 362). */
+            int32_t best_sad = evo_intra_search(src, &pic, x, y, job->quality, &ri);
+            desc_from_search(&best, &ri, 0);
+            n_int = ri.n_int; n_ax = ri.n_axial; n_dg = ri.n_diag;
+
+            if (job->type == 1) {
+                /* encode.cpp:183-210: one reference, with the temporal table. */
+                int16_t pmx, pmy;
+                evo_mv_predictor(records, (uint32_t)wb, idx, job->prev, &pmx, &pmy);
+                int32_t sad = evo_inter_search(src, &job->refs[0], x, y, job->quality, pmx, pmy, &r[0]);
+                n_int += r[0].n_int; n_ax += r[0].n_axial; n_dg += r[0].n_diag;
+                int inter_copy = (r[0].block_type & EVO_BT_COPY) != 0;
+                int best_copy = (best.block_type & EVO_BT_COPY) != 0;
+                if (inter_copy ^ best_copy) {
+                    if (inter_copy) { desc_from_search(&best, &r[0], job->target_ids[0]); best_sad = sad; }
+                } else if (sad < best_sad) {
+                    desc_from_search(&best, &r[0], job->target_ids[0]);
+                    best_sad = sad;
+                }
+            } else if (job->type == 2) {
+                /* encode.cpp:372-458: forward, backward, then the average of
+                 * the same two searches (motion.cpp:702-840). */
+                int16_t pmx, pmy;
+                evo_mv_predictor(records, (uint32_t)wb, idx, NULL, &pmx, &pmy);
+                int32_t fs = evo_inter_search(src, &job->refs[0], x, y, job->quality, pmx, pmy, &r[0]);
+                int32_t bs = evo_inter_search(src, &job->refs[1], x, y, job->quality, pmx, pmy, &r[1]);
+                n_int += r[0].n_int + r[1].n_int;
+                n_ax += r[0].n_axial + r[1].n_axial;
+                n_dg += r[0].n_diag + r[1].n_diag;
+                if (fs < best_sad) {
+                    desc_from_search(&best, &r[0], job->target_ids[0]);
+                    best.prediction_mode = 0;
+                    best_sad = fs;
+                }
+                if (bs < best_sad) {
+                    desc_from_search(&best, &r[1], job->target_ids[1]);
+                    best.prediction_mode = 1;
+                    best_sad = bs;
+                }
+                evo_predict_block(&job->refs[0], x, y, &r[0], pf);
+                evo_predict_block(&job->refs[1], x, y, &r[1], pb);
+                int32_t bi_sad = evo_bidir_average(src, pf, pb, avg);
+                if (bi_sad + (int32_t)(e.lambda * 8.0f) < best_sad) {
+                    desc_clear(&best);
+                    best.block_type = EVO_BT_MOTION;   /* EVX_BLOCK_INTER_MOTION_DELTA */
+                    best.prediction_mode = 2;
+                    best.prediction_target = job->target_ids[0];
+                    best.motion_x = (int16_t)r[0].motion_x;
+                    best.motion_y = (int16_t)r[0].motion_y;
+                    best.sp_pred = (uint8_t)r[0].sp_enabled;
+                    best.sp_amount = (uint8_t)r[0].sp_amount;
+                    best.sp_index = (uint8_t)r[0].sp_index;
+                    best.prediction_target_b = job->target_ids[1];
+                    best.motion_x_b = (int16_t)r[1].motion_x;
+                    best.motion_y_b = (int16_t)r[1].motion_y;
+                    best.sp_pred_b = (uint8_t)r[1].sp_enabled;
+                    best.sp_amount_b = (uint8_t)r[1].sp_amount;
+                    best.sp_index_b = (uint8_t)r[1].sp_index;
+                    best_sad = bi_sad;
+                }
+            }
+            if (best_sad == EVO_INT32_MAX) return 10;
+
+            /* The directional intra candidate (encode.cpp:254-272, 446-468). */
+            int32_t dir_sad;
+            int32_t mode = evo_select_intra_mode(src, &pic, x, y, &dir_sad);
+            if (dir_sad < best_sad) {
+                desc_clear(&best);
+                best.intra_mode = (uint8_t)mode;
+                best_sad = dir_sad;
+            }
+
+            float best_cost = 0.0f;
+            if (job->type == 1) {
+                /* RDO (encode.cpp:274-348, EVX_ENABLE_RDO_COPY_EVAL and
+                 * EVX_ENABLE_RDO_INTRA_MODES): the SAD winner, then the plain
+                 * intra block unless the winner is a directional one, then
+                 * the three gradient-pruned modes; strictly lower cost wins. */
+                if (!rdo_cost(&e, src, &best, x, y, &best_cost)) return 3;
+                if (best.block_type != EVO_BT_INTRA_DEFAULT) {
+                    evo_record cand;
+                    float c;
+                    desc_clear(&cand);
+                    if (!rdo_cost(&e, src, &cand, x, y, &c)) return 3;
+                    if (c < best_cost) { best = cand; best_cost = c; }
+                }
+                uint8_t modes[3];
+                evo_intra_mode_candidates(&pic, x, y, modes);
+                for (int c = 0; c < 3; c++) {
+                    evo_record cand;
+                    float cc;
+                    desc_clear(&cand);
+                    cand.intra_mode = modes[c];
+                    if (!rdo_cost(&e, src, &cand, x, y, &cc)) return 3;
+                    if (cc < best_cost) { best = cand; best_cost = cc; }
+                }
+            }
+
+            /* encode_block, then decode_block into the picture
+             * (encode.cpp:795-811). */
+            int16_t *lv = levels + (size_t)idx * EVO_MB_SAMPLES;
+            if (!encode_mb(&e, src, &best, x, y, co, lv)) return 3;
+            if (!decode_mb(&e, &best, x, y, lv, rec)) return 3;
+            for (int j = 0; j < 16; j++)
+                memcpy(pic_y + (size_t)(y + j) * w + x, rec + j * 16, 16 * sizeof(int16_t));
+            for (int j = 0; j < 8; j++) {
+                memcpy(pic_u + (size_t)(y / 2 + j) * (w / 2) + x / 2, rec + 256 + j * 8, 8 * sizeof(int16_t));
+                memcpy(pic_v + (size_t)(y / 2 + j) * (w / 2) + x / 2, rec + 320 + j * 8, 8 * sizeof(int16_t));
+            }
+
+            best.sad = best_sad;
+            memcpy(&best.cost_bits, &best_cost, 4);
+            best.n_int = n_int;
+            best.n_axial = (uint16_t)n_ax;
+            best.n_diag = (uint16_t)n_dg;
+            best.flags = ((best.block_type & EVO_BT_COPY) ? 0u : 1u) |
+                         (best.prediction_mode == 2 ? 2u : 0u) | 4u;
+            records[idx] = best;
+        }
+    return 0;
+}

@@ -38,6 +38,11 @@ extern "C" {
 #define EVO_BT_MOTION 0x2
 #define EVO_BT_COPY   0x4
 
+/* Directional intra modes, intra_pred.h:42-49: DC, vertical, horizontal,
+ * diagonal down-left, diagonal down-right, vertical-right, horizontal-down;
+ * 7 = none (a plain intra block, transformed without prediction). */
+#define EVO_INTRA_NONE 7
+
 /* A planar int16 4:2:0 picture.  width/height are the luma size (multiples
  * of 16, already aligned by the caller with zero rows, image.cpp:89);
  * chroma planes are width/2 x height/2.  Strides are in elements. */
@@ -75,7 +80,8 @@ typedef struct evo_record {
     int16_t motion_x, motion_y;
     int16_t motion_x_b, motion_y_b;
     uint8_t sp_pred, sp_amount, sp_index, sp_pred_b;
-    uint8_t sp_amount_b, sp_index_b, q_index, reserved0;
+    uint8_t sp_amount_b, sp_index_b, q_index;
+    uint8_t intra_mode;            /* intra_pred.h:42-49 (encode mode); 0 in analysis mode */
     int16_t variance;
     int16_t reserved1;
     int32_t sad;                   /* SAD used by the final selection */
@@ -247,6 +253,58 @@ void evo_sao_filter_frame(const evo_sao_info *table, int32_t width, int32_t heig
 void evo_sao_encode_frame(const int16_t *src_y, const int16_t *src_u, const int16_t *src_v,
                           int16_t *rec_y, int16_t *rec_u, int16_t *rec_v, int32_t width,
                           int32_t height, float lambda, evo_sao_info *table);
+
+/* ---- Mode decision and in-loop encode (the reference's encode_slice) ---- */
+
+/* generate_intra_prediction (intra_pred.cpp:177-215) of `mode` for the MB at
+ * (x, y) from the neighbouring samples of `pic` (128 where absent). */
+void evo_intra_prediction(int32_t mode, const evo_planes *pic, int32_t x, int32_t y,
+                          int16_t out[EVO_MB_SAMPLES]);
+/* select_intra_mode_candidates (intra_pred.cpp:304-326). */
+void evo_intra_mode_candidates(const evo_planes *pic, int32_t x, int32_t y, uint8_t cand[3]);
+/* select_best_intra_mode (intra_pred.cpp:217-302, fast mode): the mode, and
+ * its luma SAD in *sad. */
+int32_t evo_select_intra_mode(const int16_t src[EVO_MB_SAMPLES], const evo_planes *pic,
+                              int32_t x, int32_t y, int32_t *sad);
+/* quantize_macroblock / inverse_quantize_macroblock (quantize.cpp:454-476):
+ * intra matrices and DC scales for intra blocks without motion, inter
+ * otherwise. */
+void evo_quantize_mb(int32_t qp, int32_t block_type, const int16_t coeffs[EVO_MB_SAMPLES],
+                     int16_t levels[EVO_MB_SAMPLES]);
+void evo_dequantize_mb(int32_t qp, int32_t block_type, const int16_t levels[EVO_MB_SAMPLES],
+                       int16_t coeffs[EVO_MB_SAMPLES]);
+/* trellis_optimize_macroblock (quantize.cpp:615-680) in place on levels,
+ * with the scan of the block's intra mode (scan.h:154-159). */
+void evo_trellis_mb(int32_t qp, int32_t block_type, int32_t intra_mode, float lambda,
+                    const int16_t coeffs[EVO_MB_SAMPLES], int16_t levels[EVO_MB_SAMPLES]);
+/* estimate_block_bits (encode.cpp:39-110) of a descriptor and the luma of
+ * `block`. */
+uint32_t evo_block_bits(const evo_record *desc, const int16_t block[EVO_MB_SAMPLES]);
+
+/* One frame of the in-loop encode. */
+typedef struct evo_encode_job {
+    int32_t type;                  /* 0 = I, 1 = P (one reference), 2 = B (fwd, bwd) */
+    int32_t quality;
+    evo_planes src;
+    int32_t n_refs;                /* 0, 1 or 2, as the type says */
+    evo_planes refs[2];
+    uint8_t target_ids[2];         /* prediction_target of each reference */
+    const evo_record *prev;        /* temporal MVP table (P only) or NULL */
+} evo_encode_job;
+
+/* encode_slice (encode.cpp:751-813) of one frame: per MB in raster order,
+ * classify_block (P and I, encode.cpp:168-370, with RDO for P) or
+ * classify_block_bidir (B, 372-476), encode_block with trellis (479-748),
+ * and decode_block (decode.cpp:31-433) into the picture being reconstructed
+ * (pic_*, packed: luma stride width, chroma width / 2), which holds the
+ * previous content of its picture-buffer slot on entry.  records: the
+ * descriptors (q_index and variance as encode_block sets them; sad the SAD
+ * stage's best, cost_bits the RDO cost of P frames; flags bit0 residual
+ * coded, bit1 bidirectional, bit2 encode mode); levels: 384 per MB after
+ * trellis, zero for copies.  Returns 0, 3 for a bad job or 10 if a search
+ * fails. */
+int32_t evo_encode_frame(const evo_encode_job *job, int16_t *pic_y, int16_t *pic_u,
+                         int16_t *pic_v, evo_record *records, int16_t *levels);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,7 @@ RECORD_DTYPE = np.dtype([
     ("prediction_target_b", "u1"), ("motion_x", "<i2"), ("motion_y", "<i2"),
     ("motion_x_b", "<i2"), ("motion_y_b", "<i2"), ("sp_pred", "u1"), ("sp_amount", "u1"),
     ("sp_index", "u1"), ("sp_pred_b", "u1"), ("sp_amount_b", "u1"), ("sp_index_b", "u1"),
-    ("q_index", "u1"), ("reserved0", "u1"), ("variance", "<i2"), ("reserved1", "<i2"),
+    ("q_index", "u1"), ("intra_mode", "u1"), ("variance", "<i2"), ("reserved1", "<i2"),
     ("sad", "<i4"), ("mad", "<i4"), ("cost_bits", "<u4"), ("sad_b", "<i4"), ("mad_b", "<i4"),
     ("cost_b_bits", "<u4"), ("n_int", "<u4"), ("n_axial", "<u2"), ("n_diag", "<u2"),
     ("flags", "<u4"), ("reserved2", "<u4")])
@@ -63,6 +63,12 @@ REF_FIELDS = ("block_type", "prediction_mode", "prediction_target", "prediction_
 class FrameJob(C.Structure):
     _fields_ = [("type", C.c_int32), ("quality", C.c_int32), ("src", Planes),
                 ("n_refs", C.c_int32), ("refs", Planes * 4), ("target_ids", C.c_uint8 * 4),
+                ("prev", C.c_void_p)]
+
+
+class EncodeJob(C.Structure):
+    _fields_ = [("type", C.c_int32), ("quality", C.c_int32), ("src", Planes),
+                ("n_refs", C.c_int32), ("refs", Planes * 2), ("target_ids", C.c_uint8 * 2),
                 ("prev", C.c_void_p)]
 
 
@@ -198,7 +204,89 @@ class _Recon:
         return oy, ou, ov
 
 
-class Oracle(_Lib, _Recon):
+class _Encode:
+    """Mode decision and in-loop encode entry points shared by the
+    restatement and the reference build (same names and layout)."""
+
+    def _bind_encode(self, encode_extra=()):
+        P = C.POINTER(Planes)
+        self._ipred = self.f("intra_prediction", None, C.c_int32, P, C.c_int32, C.c_int32, I16P)
+        self._icand = self.f("intra_mode_candidates", None, P, C.c_int32, C.c_int32, C.c_void_p)
+        self._isel = self.f("select_intra_mode", C.c_int32, I16P, P, C.c_int32, C.c_int32, I32P)
+        self._quant_mb = self.f("quantize_mb", None, C.c_int32, C.c_int32, I16P, I16P)
+        self._dequant_mb = self.f("dequantize_mb", None, C.c_int32, C.c_int32, I16P, I16P)
+        self._trellis = self.f("trellis_mb", None, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                               I16P, I16P)
+        self._encode = self.f("encode_frame", C.c_int32, C.POINTER(EncodeJob), I16P, I16P, I16P,
+                              C.c_void_p, I16P, *encode_extra)
+
+    def intra_prediction(self, mode, pic, x, y):
+        out = np.empty(384, np.int16)
+        self._ipred(mode, C.byref(pic.planes), x, y, _ptr16(out))
+        return out
+
+    def intra_mode_candidates(self, pic, x, y):
+        out = np.zeros(3, np.uint8)
+        self._icand(C.byref(pic.planes), x, y, out.ctypes.data)
+        return [int(v) for v in out]
+
+    def select_intra_mode(self, src_blk, pic, x, y):
+        """(mode, luma SAD) of select_best_intra_mode."""
+        sad = C.c_int32()
+        s = np.ascontiguousarray(src_blk, np.int16)
+        mode = self._isel(_ptr16(s), C.byref(pic.planes), x, y, C.byref(sad))
+        return mode, sad.value
+
+    def quantize_mb(self, qp, block_type, coeffs):
+        out = np.empty(384, np.int16)
+        self._quant_mb(qp, block_type, _ptr16(np.ascontiguousarray(coeffs, np.int16)),
+                       _ptr16(out))
+        return out
+
+    def dequantize_mb(self, qp, block_type, levels):
+        out = np.empty(384, np.int16)
+        self._dequant_mb(qp, block_type, _ptr16(np.ascontiguousarray(levels, np.int16)),
+                         _ptr16(out))
+        return out
+
+    def trellis(self, qp, block_type, intra_mode, lam, coeffs, levels):
+        out = np.array(levels, np.int16, order="C")
+        self._trellis(qp, block_type, intra_mode, C.c_float(lam),
+                      _ptr16(np.ascontiguousarray(coeffs, np.int16)), _ptr16(out))
+        return out
+
+    def _encode_call(self, job, y, u, v, recs, levels):
+        return self._encode(job, _ptr16(y), _ptr16(u), _ptr16(v), recs.ctypes.data,
+                            _ptr16(levels))
+
+    def encode_frame(self, ftype, quality, src, refs, pic, prev=None, target_ids=None):
+        """encode_slice of one frame (0 = I, 1 = P with refs[0], 2 = B with
+        refs[0] forward and refs[1] backward).  pic: the (y, u, v) planes the
+        picture-buffer slot holds before the frame.  Returns (records,
+        levels, (y, u, v) reconstruction)."""
+        job = EncodeJob()
+        job.type = ftype
+        job.quality = quality
+        job.src = src.planes
+        job.n_refs = len(refs)
+        for k, r in enumerate(refs):
+            job.refs[k] = r.planes
+        ids = target_ids if target_ids is not None else [1, 2][:len(refs)]
+        for k, t in enumerate(ids):
+            job.target_ids[k] = t
+        pv = None if prev is None else np.ascontiguousarray(prev)
+        job.prev = None if pv is None else pv.ctypes.data
+        y, u, v = (np.array(a, np.int16, order="C") for a in pic)
+        n = (src.width // 16) * (src.height // 16)
+        recs = np.zeros(n, RECORD_DTYPE)
+        levels = np.zeros((n, 384), np.int16)
+        rc = self._encode_call(C.byref(job), y, u, v, recs, levels)
+        if rc != 0:
+            raise RuntimeError(f"encode_frame failed: {rc}")
+        return recs, levels, (y, u, v)
+
+
+class Oracle(_Lib, _Recon, _Encode):
     """The C restatement."""
     prefix = "evo_"
 
@@ -228,6 +316,7 @@ class Oracle(_Lib, _Recon):
         self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
                              I16P, I16P)
         self._bind_recon()
+        self._bind_encode()
 
     def lam(self, q):
         return self._lambda(q)
@@ -332,7 +421,7 @@ def _run_frame(fn, ftype, quality, src, refs, prev, target_ids):
     return recs, coeffs, levels
 
 
-class Reference(_Lib, _Recon):
+class Reference(_Lib, _Recon, _Encode):
     """The reference's own functions (oracle/_ref, built from its sources)."""
     prefix = "evr_"
 
@@ -361,6 +450,12 @@ class Reference(_Lib, _Recon):
         self._frame_wpp = self.f("analyze_frame_wpp", C.c_int32, C.POINTER(FrameJob),
                                  C.c_void_p, I16P, I16P, C.c_int32)
         self._bind_recon()
+        self._bind_encode((C.c_int32,))
+        self.encode_threads = 1
+
+    def _encode_call(self, job, y, u, v, recs, levels):
+        return self._encode(job, _ptr16(y), _ptr16(u), _ptr16(v), recs.ctypes.data,
+                            _ptr16(levels), self.encode_threads)
 
     def lam(self, q):
         return self._lambda(q)

@@ -18,10 +18,12 @@
 #include "common.h"
 #include "imageset.h"
 #include "interp.h"
+#include "intra_pred.h"
 #include "macroblock.h"
 #include "motion.h"
 #include "quantize.h"
 #include "sao.h"
+#include "scan.h"
 #include "transform.h"
 
 #include <algorithm>
@@ -49,6 +51,9 @@ evx_status serialize_sao_table(const evx_sao_info *sao_table, uint32 block_count
                                uint32 width_in_blocks, bit_stream *feed_stream,
                                entropy_coder *coder, evx_metadata_contexts *meta,
                                bit_stream *output);
+// encode.cpp defines the slice encoders at namespace scope without a header.
+evx_status encode_slice(const evx_frame &frame, evx_context *context);
+evx_status encode_slice_wpp(const evx_frame &frame, evx_context *context);
 }  // namespace evx
 
 using namespace evx;
@@ -837,6 +842,133 @@ int32_t evr_serialize_sao(const evo_sao_info *table, int32_t wb, int32_t hb, uin
                            bit_stream *output) {
         serialize_sao_table(t.data(), (uint32)n, (uint32)wb, feed, coder, meta, output);
     }, out, cap);
+}
+
+
+// ---- Mode decision and in-loop encode: the reference's own functions ----
+
+// generate_intra_prediction (intra_pred.cpp:177-215) into an MB-order buffer.
+void evr_intra_prediction(int32_t mode, const evo_planes *pic, int32_t x, int32_t y,
+                          int16_t *out)
+{
+    image_set set;
+    fill_set(&set, *pic);
+    std::vector<int16_t> buf(384);
+    macroblock m = view(buf.data());
+    generate_intra_prediction((uint8)mode, set, x, y, &m);
+    copy_out(m, out);
+}
+
+// select_intra_mode_candidates (intra_pred.cpp:304-326).
+void evr_intra_mode_candidates(const evo_planes *pic, int32_t x, int32_t y, uint8_t *cand)
+{
+    image_set set;
+    fill_set(&set, *pic);
+    select_intra_mode_candidates(set, x, y, cand);
+}
+
+// select_best_intra_mode (intra_pred.cpp:217-302).
+int32_t evr_select_intra_mode(const int16_t *src, const evo_planes *pic, int32_t x, int32_t y,
+                              int32_t *sad)
+{
+    image_set set;
+    fill_set(&set, *pic);
+    std::vector<int16_t> s(src, src + 384), scratch(384);
+    macroblock ms = view(s.data()), mt = view(scratch.data());
+    return select_best_intra_mode(ms, set, x, y, &mt, sad);
+}
+
+// inverse_quantize_macroblock (quantize.cpp:466-476); evr_quantize_mb above
+// is its forward counterpart.
+void evr_dequantize_mb(int32_t qp, int32_t block_type, const int16_t *levels, int16_t *coeffs)
+{
+    std::vector<int16_t> l(levels, levels + 384), c(384);
+    macroblock ml = view(l.data()), mc = view(c.data());
+    inverse_quantize_macroblock((uint8)qp, (EVX_BLOCK_TYPE)block_type, ml, &mc);
+    copy_out(mc, coeffs);
+}
+
+// trellis_optimize_macroblock (quantize.cpp:615-680) with the scan of
+// select_scan_table_8x8 (scan.h:154-159), levels in place.
+void evr_trellis_mb(int32_t qp, int32_t block_type, int32_t intra_mode, float lambda,
+                    const int16_t *coeffs, int16_t *levels)
+{
+    std::vector<int16_t> c(coeffs, coeffs + 384), l(levels, levels + 384);
+    macroblock mc = view(c.data()), ml = view(l.data());
+    trellis_optimize_macroblock((uint8)qp, (EVX_BLOCK_TYPE)block_type, mc,
+                                select_scan_table_8x8((uint8)intra_mode), lambda, &ml);
+    copy_out(ml, levels);
+}
+
+// encode_slice (encode.cpp:751-813) of one frame, run by the reference
+// itself on a context set up as engine_encode_frame leaves it: the source in
+// the input cache, the references in prediction slots 0 (forward) and 1
+// (backward), the picture being reconstructed in slot 2 (its previous
+// content from pic_*), the temporal table in prev_block_table.  threads > 1
+// runs encode_slice_wpp (encode.cpp:817-941, on hardware_concurrency
+// workers) instead.  Descriptors come back as records (reference slot
+// numbers mapped to job->target_ids), levels per MB (zero for copies, whose
+// output-cache area encode_block leaves untouched) and the reconstruction
+// in pic_*.
+int32_t evr_encode_frame(const evo_encode_job *job, int16_t *pic_y, int16_t *pic_u,
+                         int16_t *pic_v, evo_record *records, int16_t *levels, int32_t threads)
+{
+    const int32_t w = job->src.width, h = job->src.height;
+    const uint32 wb = (uint32)w / 16, hb = (uint32)h / 16, n = wb * hb;
+    evx_context ctx;
+    if (initialize_context((uint32)w, (uint32)h, &ctx) != EVX_SUCCESS) return 10;
+    fill_set(&ctx.cache_bank.input_cache, job->src);
+    for (int32 k = 0; k < job->n_refs && k < 2; k++)
+        fill_set(&ctx.cache_bank.prediction_cache[k], job->refs[k]);
+    evo_planes pic = {pic_y, pic_u, pic_v, w, h, w, w / 2};
+    fill_set(&ctx.cache_bank.prediction_cache[2], pic);
+    {
+        image_set &o = ctx.cache_bank.output_cache;
+        memset(o.query_y_image()->query_data(), 0, (size_t)o.query_y_image()->query_row_pitch() * h);
+        memset(o.query_u_image()->query_data(), 0, (size_t)o.query_u_image()->query_row_pitch() * (h / 2));
+        memset(o.query_v_image()->query_data(), 0, (size_t)o.query_v_image()->query_row_pitch() * (h / 2));
+    }
+    for (uint32 k = 0; k < n; k++) {
+        clear_block_desc(&ctx.block_table[k]);
+        clear_block_desc(&ctx.prev_block_table[k]);
+        if (job->prev) {
+            ctx.prev_block_table[k].block_type = (EVX_BLOCK_TYPE)job->prev[k].block_type;
+            ctx.prev_block_table[k].motion_x = job->prev[k].motion_x;
+            ctx.prev_block_table[k].motion_y = job->prev[k].motion_y;
+        }
+    }
+    evx_frame frame;
+    clear_frame(&frame);
+    frame.type = (EVX_FRAME_TYPE)job->type;
+    frame.quality = (uint16)job->quality;
+    frame.dpb_slot = 2;
+    frame.ref_slot_fwd = 0;
+    frame.ref_slot_bwd = 1;
+    const evx_status st = threads > 1 ? encode_slice_wpp(frame, &ctx) : encode_slice(frame, &ctx);
+    if (st != EVX_SUCCESS) {
+        clear_context(&ctx);
+        return 10;
+    }
+    auto id_of = [&](uint8 slot) -> uint8_t {
+        return slot < 2 && (int32)slot < job->n_refs ? job->target_ids[slot] : (uint8_t)(0xF0 | slot);
+    };
+    const image_set &o = ctx.cache_bank.output_cache;
+    for (uint32 k = 0; k < n; k++) {
+        const evx_block_desc &d = ctx.block_table[k];
+        evo_record *r = &records[k];
+        desc_to_record(d, r);
+        r->intra_mode = d.intra_mode;
+        if (!(d.block_type & 1)) {
+            r->prediction_target = id_of(d.prediction_target);
+            if (d.prediction_mode == 2) r->prediction_target_b = id_of(d.prediction_target_b);
+        }
+        macroblock m;
+        create_macroblock(o, (int32)(k % wb) * 16, (int32)(k / wb) * 16, &m);
+        copy_out(m, levels + (size_t)k * 384);
+    }
+    copy_back(ctx.cache_bank.prediction_cache[2], w, h, pic_y, pic_u, pic_v);
+    clear_context(&ctx);
+    return 0;
 }
 
 }  // extern "C"

@@ -6,7 +6,8 @@ sources plus oracle/ref_shim.cpp.  Inputs are stored next to the outputs, so
 the fixtures never depend on a random generator at test time.
 
     python tests/golden/make_golden.py          # every fixture
-    python tests/golden/make_golden.py recon    # recon.npz only (likewise deblock, sao, intra)
+    python tests/golden/make_golden.py recon    # recon.npz only (likewise deblock, sao, intra,
+                                                # encode)
 """
 import os
 import sys
@@ -326,9 +327,63 @@ def intra_fixtures():
     save("intra.npz", **out)
 
 
+ENCODE_FIELDS = REF_FIELDS[:-3] + ("intra_mode",)   # the descriptor fields
+
+
+def encode_fixtures():
+    """encode_slice (encode.cpp:751-813) of I, P and B frames run by the
+    reference itself: natural texture with motion, and tiled content with
+    small noise (intra block copies), at qualities 8 and 20 (20 takes the
+    deadzones that start at qp 16) and, for the tiles, 1 (reconstructions
+    close enough to the source for copy blocks), and a flat picture with one
+    textured square moving right (copy blocks of every kind).  Each chain is I (over a random stale
+    picture-buffer slot), P against the I reconstruction with the I records
+    as the temporal table, then B between the two reconstructions."""
+    ref = Reference()
+    rng = np.random.default_rng(20261205)
+    w, h = 112, 80
+    nat = synth.sequence_c1(int(rng.integers(1 << 30)), 3, w, h, local_range=12)
+    tile = synth.texture_frame(rng, 48, 48)
+    tiled = [tuple(np.tile(t[:n, :n], (h // n // (1 if k == 0 else 2) + 1,
+                                        w // n // (1 if k == 0 else 2) + 1))
+                   [: (h if k == 0 else h // 2), : (w if k == 0 else w // 2)].astype(np.int16)
+                   for k, (t, n) in enumerate(zip(tile, (24, 12, 12))))]
+    for _ in range(2):
+        tiled.append(tuple((p + rng.integers(-2, 3, p.shape)).astype(np.int16) for p in tiled[-1]))
+    flat = []
+    for k in range(3):
+        y = np.full((h, w), 100, np.int16)
+        y[16:48, 32 + 4 * k:64 + 4 * k] = nat[0][0][16:48, 32:64]
+        flat.append((y, np.full((h // 2, w // 2), 128, np.int16),
+                     np.full((h // 2, w // 2), 120, np.int16)))
+    out = {}
+    for tag, frames, qualities in (("nat", nat, (8, 20)), ("tile", tiled, (1, 8, 20)),
+                                   ("flat", flat, (8,))):
+        for q in qualities:
+            stale = wide_picture(rng, w, h, 0, 256)
+            key = f"{tag}{q}"
+            for i, pl in enumerate(frames):
+                out[f"{key}_src{i}_y"], out[f"{key}_src{i}_u"], out[f"{key}_src{i}_v"] = pl
+            out[f"{key}_stale_y"], out[f"{key}_stale_u"], out[f"{key}_stale_v"] = stale
+            ri, li, pi = ref.encode_frame(0, q, Picture(*frames[0]), [], stale)
+            rp, lp, pp = ref.encode_frame(1, q, Picture(*frames[1]), [Picture(*pi)], stale,
+                                          prev=ri, target_ids=[5])
+            rb, lb, pb = ref.encode_frame(2, q, Picture(*frames[2]), [Picture(*pi), Picture(*pp)],
+                                          stale, target_ids=[5, 6])
+            for t, (r, lv, pic) in (("I", (ri, li, pi)), ("P", (rp, lp, pp)), ("B", (rb, lb, pb))):
+                out[f"{key}_{t}_fields"] = np.stack([r[f].astype(np.int64) for f in ENCODE_FIELDS],
+                                                    axis=-1)
+                out[f"{key}_{t}_levels"] = lv
+                out[f"{key}_{t}_y"], out[f"{key}_{t}_u"], out[f"{key}_{t}_v"] = pic
+    out["fields"] = np.array(ENCODE_FIELDS)
+    save("encode.npz", **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["intra"]:
         intra_fixtures()
+    elif sys.argv[1:] == ["encode"]:
+        encode_fixtures()
     elif sys.argv[1:] == ["recon"]:
         recon_fixtures()
     elif sys.argv[1:] == ["deblock"]:
@@ -341,3 +396,4 @@ if __name__ == "__main__":
         deblock_fixtures()
         sao_fixtures()
         intra_fixtures()
+        encode_fixtures()
