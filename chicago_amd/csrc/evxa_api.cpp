@@ -231,6 +231,9 @@ struct evxa_context {
     // hipGetSymbolAddress), and whether the last analysis used it.
     char *scratch = nullptr;
     bool scratch_last = false;
+    // The counter block of the last in-loop encode, when that was the last
+    // launch whose errors evxa_synchronize reports.
+    const int32_t *enc_last = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
     int32_t resident[2][3] = {};     // [sweep mode][waves]: workgroups of the analysis kernels that fit
@@ -285,6 +288,10 @@ struct evxa_stream {
     DevBuf recon_flag;     // range record of a reconstruction headed for the filter (unused)
     DevBuf sao_words;      // per MB: published SAO decision; then the SAO counters
     DevBuf sao_table;      // evxa_sao_info per MB
+    // In-loop encode: counter block, per-MB words, then the MB order (built
+    // once, on first use).
+    DevBuf enc;
+    bool enc_order = false;
     uint64_t bytes = 0;
     // Recorded after every asynchronous job that writes the stream's slots or
     // outputs, on the HIP stream that job ran on (evxa_stream_wait).
@@ -604,7 +611,10 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool launch = true)
 {
     if (!st) st = ctx->stream;
-    if (st == ctx->stream) ctx->scratch_last = false;
+    if (st == ctx->stream) {
+        ctx->scratch_last = false;
+        ctx->enc_last = nullptr;
+    }
     const bool sweep = ctx->mode == EVXA_MODE_SWEEP;
     const int32_t fit = ctx->resident[sweep ? 1 : 0][ctx->waves];
     const int32_t all = ctx->grid > 0 ? std::min(ctx->grid, fit) : fit;
@@ -774,6 +784,15 @@ void check_counters(evxa_context *ctx, LaunchRes &r)
 // The error word of the context's last analysis launch (its queue is idle).
 void check_last_launch(evxa_context *ctx)
 {
+    if (ctx->enc_last) {
+        int32_t c[kCounterError + 1];
+        check(hipMemcpy(c, ctx->enc_last, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+        if (c[kCounterError] & kErrWaitTimeout)
+            fail(EVXA_ERROR_OPERATION_TIMEDOUT, "encode kernel: dependency wait timed out");
+        if (c[kCounterError] & kErrSearchFailed)
+            fail(EVXA_ERROR_EXECUTION_FAILURE, "encode kernel: block search failed");
+        return;
+    }
     if (!ctx->scratch_last) {
         check_counters(ctx, ctx->res);
         return;
@@ -960,6 +979,7 @@ int evxa_stream_close(evxa_stream *s)
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         ctx->used -= s->bytes;
+        if (s->enc.p && ctx->enc_last == s->enc.as<int32_t>()) ctx->enc_last = nullptr;
         ctx->streams.erase(std::remove(ctx->streams.begin(), ctx->streams.end(), s),
                            ctx->streams.end());
         for (auto &g : ctx->graph_streams)
@@ -1054,6 +1074,101 @@ int evxa_analyze_frame(evxa_stream *s, const evxa_frame_desc *desc)
     });
 }
 
+int evxa_encode_frame(evxa_stream *s, const evxa_encode_desc *d)
+{
+    return guarded([&] {
+        if (!s || !d) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        evxa_context *ctx = s->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        const int32_t n_refs = d->type == EVXA_FRAME_I ? 0 : d->type == EVXA_FRAME_P ? 1 : 2;
+        if (d->type != EVXA_FRAME_I && d->type != EVXA_FRAME_P && d->type != EVXA_FRAME_B)
+            fail(EVXA_ERROR_INVALIDARG, "frame type");
+        if (d->quality < 0 || d->quality > 63) fail(EVXA_ERROR_INVALIDARG, "quality must be 0..63");
+        if (s->w < 48 || s->h < 48) fail(EVXA_ERROR_INVALIDARG, "the encode needs frames of at least 48x48");
+        check_slot(s, d->src_slot);
+        check_slot(s, d->dst_slot);
+        if (d->dst_slot == d->src_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is the source");
+        for (int k = 0; k < n_refs; k++) {
+            check_slot(s, d->ref_slots[k]);
+            if (d->ref_slots[k] == d->dst_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is a reference");
+        }
+        set_device(ctx);
+        const size_t n_mb = (size_t)s->wb * s->hb;
+        const size_t words_off = 256, order_off = words_off + align_up(n_mb * 8, 256);
+        lazy_alloc(s, s->enc, order_off + n_mb * 4);
+        char *eb = static_cast<char *>(s->enc.p);
+        int32_t *counters = reinterpret_cast<int32_t *>(eb);
+        uint64_t *words = reinterpret_cast<uint64_t *>(eb + words_off);
+        uint32_t *order = reinterpret_cast<uint32_t *>(eb + order_off);
+
+        DevJob j;
+        memset(&j, 0, sizeof(j));
+        j.src = s->pic(d->src_slot);
+        for (int k = 0; k < n_refs; k++) j.refs[k] = s->pic(d->ref_slots[k]);
+        j.width = s->w;
+        j.height = s->h;
+        j.wb = s->wb;
+        j.hb = s->hb;
+        j.type = d->type;
+        j.quality = d->quality;
+        j.n_refs = n_refs;
+        j.thresh = (d->quality >> 2) + 1;   // motion.cpp:589
+        j.lambda = lambda_of(d->quality);
+        j.target_ids = (uint32_t)d->target_ids[0] | ((uint32_t)d->target_ids[1] << 8);
+        // The reference keeps one temporal table, the block table of the last
+        // non-B frame (encode.cpp:990-997); I and P frames write the P table
+        // in use, B frames their own.
+        const int out = d->type == EVXA_FRAME_B ? 2 : s->p_cur;
+        j.prev = (d->type == EVXA_FRAME_P && d->temporal_mvp && s->temporal_valid)
+                     ? s->table(s->p_cur ^ 1) : nullptr;
+        j.records = s->table(out);
+        j.coeffs = s->coeffs.as<int16_t>();
+        j.levels = s->levels.as<int16_t>();
+        j.mvw = words;
+        const DevPic pic = s->pic(d->dst_slot);
+
+        // Enough workgroups for twice the widest diagonal of the MB order
+        // (the MBs that can run at once), at most all that fit.
+        const int32_t slope = kEncodeLag + 1;
+        int32_t widest = 1;
+        for (int32_t v = 0; v < s->wb + slope * (s->hb - 1); v++) {
+            int32_t c = 0;
+            for (int32_t by = 0; by < s->hb; by++) c += (v - slope * by >= 0 && v - slope * by < s->wb) ? 1 : 0;
+            widest = std::max(widest, c);
+        }
+        const int32_t grid = std::max(1, std::min(encode_resident_groups(ctx->device), 2 * widest + 8));
+
+        hipStream_t st = ctx->stream;
+        range_begin(s, d->dst_slot);
+        check(launch_reset(counters, (int32_t)(words_off / 4 + 2 * n_mb), st), "reset kernel");
+        if (!s->enc_order) {
+            check(launch_encode_order(s->wb, s->hb, order, st), "encode order kernel");
+            s->enc_order = true;
+        }
+        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+        check(hipEventRecord(ev.first, st), "hipEventRecord");
+        check(launch_encode(j, pic, order, counters, s->range_spread(d->dst_slot), grid, st),
+              "encode kernel");
+        check(hipEventRecord(ev.second, st), "hipEventRecord");
+        ctx->ring_head++;
+        ctx->last_start = ev.first;
+        ctx->last_stop = ev.second;
+        range_end(s, d->dst_slot);
+        ctx->scratch_last = false;
+        ctx->enc_last = counters;
+        s->last_table = out;
+        s->last_job = j;
+        for (int k = 0; k < EVXA_MAX_REFS; k++) s->last_refs[k] = k < n_refs ? d->ref_slots[k] : -1;
+        if (d->type != EVXA_FRAME_B && d->update_temporal) {
+            s->p_cur ^= 1;
+            s->temporal_valid = true;
+        }
+        mark_done(s, st);
+    });
+}
+
 int evxa_analyze_frame_noalloc(evxa_stream *s, const evxa_frame_desc *desc)
 {
     return guarded([&] {
@@ -1104,6 +1219,7 @@ int evxa_analyze_frame_noalloc(evxa_stream *s, const evxa_frame_desc *desc)
         ctx->last_start = ev.first;
         ctx->last_stop = ev.second;
         ctx->scratch_last = true;
+        ctx->enc_last = nullptr;
         mark_done(s, st);
     });
 }

@@ -504,13 +504,12 @@ typedef __attribute__((address_space(3))) void lds_void;
 // Copies rows of `row_dw` dwords (row r of the source at g + r * pitch_dw)
 // into LDS packed with stride row_dw, n_rows rows, straight from memory to
 // LDS (global_load_lds_dword: lane i of the k-th load writes dword 64k + i).
-// All loads are issued back to back; they complete under vmcnt(0).
-// Copies rows of `row_dw` dwords (row r of the source at g + r * pitch_dw)
-// into LDS packed with stride row_dw, n_rows rows, straight from memory to
-// LDS (global_load_lds_dword: lane i of the k-th load writes dword 64k + i).
 // The W waves of the workgroup take every W-th load.  All loads are issued
-// back to back; they complete under vmcnt(0).
-template <int ROW_DW, int W>
+// back to back; they complete under vmcnt(0).  AUX is the loads' cache
+// policy (kLoadDevice: device scope, for pictures other workgroups of the
+// same launch write).
+constexpr int kLoadDevice = 16;     // sc1
+template <int ROW_DW, int W, int AUX = 0>
 __device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_rows, uint32_t *lds)
 {
     const int lane = lane_id();
@@ -523,7 +522,7 @@ __device__ __forceinline__ void dma_rows(const uint32_t *g, int pitch_dw, int n_
     for (int k = w0; k * 64 < n; k += W) {
         if (k * 64 + lane < n)
             __builtin_amdgcn_global_load_lds((global_void *)(g + r * pitch_dw + c),
-                                             (lds_void *)(lds + k * 64), 4, 0, 0);
+                                             (lds_void *)(lds + k * 64), 4, 0, AUX);
         r += kStepR;
         c += kStepC;
         if (c >= ROW_DW) { c -= ROW_DW; r++; }
@@ -1204,12 +1203,38 @@ union alignas(16) Shared {
     MbLds mb;
 };
 
+// A window of a picture staged in LDS: its planes (row strides py, pc) and
+// where they start in the picture, luma (lx, ly) and chroma (cx, cy).  Reads
+// index the arrays relative to those origins; every read stays inside.
+struct WinView {
+    const int16_t *y, *u, *v;
+    int32_t py, pc;
+    int32_t lx, ly, cx, cy;
+};
+
+// Where a picture view's plane arrays start, in picture coordinates: a
+// device picture's pointers address sample (0, 0) itself.
+__device__ __forceinline__ void view_origin(const DevPic &, int &lx, int &ly, int &cx, int &cy)
+{
+    lx = ly = cx = cy = 0;
+}
+__device__ __forceinline__ void view_origin(const WinView &w, int &lx, int &ly, int &cx, int &cy)
+{
+    lx = w.lx; ly = w.ly; cx = w.cx; cy = w.cy;
+}
+
 // Residual prediction for a search result into `out` (encode.cpp:696-723;
 // interp.cpp:166-202 when sub-pel).  `tmp` holds the diagonal H pass.
-__device__ void predict_mb(const DevPic &ref, const SearchOut &r, int16_t *out, int16_t *tmp)
+//
+// `ref` is a device picture or a window of one staged in LDS (WinView).
+template <class Pic>
+__device__ void predict_mb(const Pic &ref, const SearchOut &r, int16_t *out, int16_t *tmp)
 {
     const int lane = lane_id();
-    const int bx = r.best_x, by = r.best_y, cbx = bx >> 1, cby = by >> 1;
+    int olx, oly, ocx, ocy;
+    view_origin(ref, olx, oly, ocx, ocy);
+    const int bx = r.best_x - olx, by = r.best_y - oly;
+    const int cbx = (r.best_x >> 1) - ocx, cby = (r.best_y >> 1) - ocy;
     int dx = 0, dy = 0;
     if (r.sp_en) frac_direction(r.sp_idx, dx, dy);
     const int xo = dx > 0 ? 0 : -1, yo = dy > 0 ? 0 : -1;
@@ -1300,13 +1325,11 @@ __device__ __forceinline__ void block_origin(int blk, int &origin, int &pitch)
     else { origin = 256 + (blk - 4) * 64; pitch = 8; }
 }
 
-// Residual, forward DCT, adaptive QP, variance and inter quantisation of one
-// MB held in LDS (encode.cpp:717-728; transform.cpp:435-452, 584-594;
-// quantize.cpp:97-118, 217-252, 372-397; analysis.h:253-275).
-// Coefficients and levels go to global memory; returns qp and variance2.
-__device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res, int16_t *tmp,
-                          int quality, int16_t *co_out, int16_t *lv_out, int &qp_out,
-                          int &var_out)
+// Residual and forward DCT of one MB held in LDS (encode.cpp:717-728;
+// transform.cpp:435-452, 584-594): the coefficients replace the residual
+// in `res`.
+__device__ __forceinline__ void fdct_mb(const int16_t *src, const int16_t *pred, int16_t *res,
+                                        int16_t *tmp)
 {
     const int lane = lane_id();
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) res[i] = (int16_t)(src[i] - pred[i]);
@@ -1325,7 +1348,13 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
         dct_line(tmp + col, pitch, res + col, pitch);
     }
     wave_sync();
+}
 
+// compute_block_variance2 and query_block_quantization_parameter over the
+// coefficients in `res` (analysis.h:253-275; quantize.cpp:97-118).
+__device__ __forceinline__ void mb_qp(const int16_t *res, int quality, int &qp_out, int &var_out)
+{
+    const int lane = lane_id();
     // compute_block_variance2 over the 16x16 luma coefficients, (0,0)
     // skipped; the int32 sums wrap like the reference's.
     int sum = 0, cnt = 0;
@@ -1355,7 +1384,22 @@ __device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res,
     const int q8 = quality & 0xFF;
     int delta = ((int)(int16_t)(index - q8)) >> 1;
     delta = min(max(delta, -6), 6);
-    const int qp = min(max(q8 + delta, 1), 63);
+    qp_out = min(max(q8 + delta, 1), 63);
+    var_out = var2;
+}
+
+// Residual, forward DCT, adaptive QP, variance and inter quantisation of one
+// MB held in LDS (encode.cpp:717-728; transform.cpp:435-452, 584-594;
+// quantize.cpp:97-118, 217-252, 372-397; analysis.h:253-275).
+// Coefficients and levels go to global memory; returns qp and variance2.
+__device__ void dct_quant(const int16_t *src, const int16_t *pred, int16_t *res, int16_t *tmp,
+                          int quality, int16_t *co_out, int16_t *lv_out, int &qp_out,
+                          int &var_out)
+{
+    const int lane = lane_id();
+    fdct_mb(src, pred, res, tmp);
+    int qp, var2;
+    mb_qp(res, quality, qp, var2);
     const int cqp = min(qp + 2, 63);
     const uint32_t m_luma = magic_of(qp << 1), m_chroma = magic_of(cqp << 1);
 
@@ -3859,10 +3903,23 @@ __device__ __forceinline__ bool intra_unstable(int cx, int cy, int x, int y)
     return cy > y - 16 && cx > x - 16;
 }
 
+// The block at (x, y) of a staged window, in the lane layout.
+__device__ __forceinline__ void load_cand(const WinView &p, const Geo &g, int x, int y, Blk &b)
+{
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        lds_load4(p.y, (y - p.ly + g.oy + r) * p.py + x - p.lx + g.ox, b.y[2 * r], b.y[2 * r + 1]);
+    const int co = ((y >> 1) - p.cy + g.cr) * p.pc + (x >> 1) - p.cx + g.cc;
+    lds_load4(p.u, co, b.u[0], b.u[1]);
+    lds_load4(p.v, co, b.v[0], b.v[1]);
+}
+
 // One 3x3 scan (perform_intra_motion_search, motion.cpp:348-379) of step
 // `step` whose first position is (left, top) from the current best; true
-// once the best is a copy (the scan stops there).
-__device__ bool intra_scan(const DevPic &pic, int pw, int ph, const Blk &s, const Geo &geo, int x,
+// once the best is a copy (the scan stops there).  `pic` is the picture
+// being reconstructed: a device picture, or a window of it in LDS.
+template <class Pic>
+__device__ bool intra_scan(const Pic &pic, int pw, int ph, const Blk &s, const Geo &geo, int x,
                            int y, int thresh, float lambda, int left, int top, int step,
                            IntraState &st)
 {
@@ -3874,7 +3931,7 @@ __device__ bool intra_scan(const DevPic &pic, int pw, int ph, const Blk &s, cons
         const bool valid = p < 9 && !intra_unstable(cx, cy, x, y) && cx >= 0 && cx <= max_x &&
                            cy >= 0 && cy <= max_y;
         Blk r;
-        load_cand(pic, geo, valid ? cx : 0, valid ? cy : 0, r);
+        load_cand(pic, geo, valid ? cx : x, valid ? cy : y, r);   // invalid slots read the MB's own position
         int sad, mad, satd;
         cand_metrics(false, s, r, sad, mad, satd);
         const uint32_t cst = cost_bits(sad, satd, cx - x, cy - y, lambda);
@@ -3922,7 +3979,8 @@ __device__ __forceinline__ int frac_index_of(int i, int j)
     return i + 5;
 }
 
-__device__ SearchOut intra_search(const DevPic &pic, int pw, int ph, const Blk &s, const Geo &geo,
+template <class Pic>
+__device__ SearchOut intra_search(const Pic &pic, int pw, int ph, const Blk &s, const Geo &geo,
                                   int x, int y, int thresh, float lambda, MbLds &sh)
 {
     int a = 0;
@@ -4021,6 +4079,874 @@ __global__ __launch_bounds__(64) void dct_quant_blocks_kernel(int32_t n, const i
     dct_quant(sh.src, sh.pred, sh.res, sh.tmp, quality[b], coeffs + (size_t)b * EVXA_MB_SAMPLES,
               levels + (size_t)b * EVXA_MB_SAMPLES, q, v);
     if (lane_id() == 0) { qp[b] = q; variance[b] = v; }
+}
+
+// ------------------------------------------------------- in-loop encode --
+//
+// encode_slice (encode.cpp:751-813) of one frame on the device: per MB the
+// mode decision of classify_block (I and P, encode.cpp:168-370, with RDO
+// for P) or classify_block_bidir (B, encode.cpp:372-476), encode_block with
+// the trellis (encode.cpp:479-748; quantize.cpp:454-464, 615-680), and
+// decode_block into the picture being reconstructed (decode.cpp:31-433).
+//
+// The intra-frame block search (motion.cpp:485-565) and the directional
+// intra modes (intra_pred.cpp:177-326) read that picture while it is being
+// written, so MBs run in a wavefront: MB (bx, by) starts once MB (bx - 1, by)
+// and MB (bx + kEncodeLag, by - 1) are done (the reference's row workers keep
+// the same kind of lag, encode.cpp:864-875).  An intra candidate lies on or
+// above row y - 16, or on or left of column x - 16 (motion.cpp:361-365,
+// 433-438), within 15 samples of a position of the first 3x3 scan of step
+// 16, whose reach is column x + 16 and row y - 32; with the 6-tap apron its
+// reads stay within columns x - 33 .. x + 49 and rows y - 49 .. y + 33.  So
+// the MB reads nothing right of MB bx + 3 in the row above, and of the rows
+// below only columns x - 33 .. x + 2, which no MB has written yet when the
+// lag is 3 (row by + 1 has at most reached MB bx - 4).  Each MB therefore
+// sees the picture exactly as the sequential encoder does: reconstructed
+// samples above and to the left, the slot's previous content elsewhere.
+//
+// One wave per MB.  Pictures written in the launch move between workgroups
+// (and XCDs) only through device-scope accesses: the reconstruction, and the
+// apron of MBs on the picture border, are stored with device-scope stores
+// that complete before the MB's predictor word is published, and a MB
+// stages the window of the picture it reads (luma 84 x 84, chroma 46 x 45
+// around it) into LDS with device-scope loads after its dependencies are
+// published.  The window keeps the apron's clamped reads: its origin moves
+// inside the picture's apron at the borders.
+
+// The window holds every read of the intra search, the intra predictions
+// and the intra block copies: a candidate (cx, cy) lies in [x - 31, x + 31] x
+// [y - 47, y + 15], and its 6-tap reads reach 3 samples before and 18 after
+// it (luma), 3 before and 10 after (chroma, at (cx >> 1, cy >> 1)).
+constexpr int kIwLS = 84, kIwLR = 84;    // luma: columns x - 34 .. x + 49, rows y - 50 .. y + 33
+constexpr int kIwCS = 46, kIwCR = 45;    // chroma: columns x/2 - 20 .. x/2 + 25, rows y/2 - 27 .. y/2 + 17
+constexpr int kIntraNone = 7;            // intra_pred.h:42-49: no directional prediction
+constexpr uint64_t kEncWaitTicks = 200000000ull;   // 2 s of s_memrealtime (100 MHz)
+
+struct IntraWin {
+    alignas(16) int16_t y[kIwLR * kIwLS + 4];   // + one dword read past the last sample
+    alignas(16) int16_t u[kIwCR * kIwCS + 4];
+    alignas(16) int16_t v[kIwCR * kIwCS + 4];
+};
+
+// The neighbour samples of the directional intra modes (intra_pred.cpp:
+// 12-52): per plane the row above, the column left and the corner, 128
+// where absent, and the DC value (intra_pred.cpp:58-75).
+struct IntraEdges {
+    int16_t top[3][16];
+    int16_t left[3][16];
+    int16_t corner[3];
+    int16_t dc[3];
+    int32_t has_top, has_left;
+};
+
+// The decision of a MB as a block descriptor (common.h:87-115).
+struct EncDesc {
+    int btype, pmode, tgt, tgt_b;
+    int mvx, mvy, mvx_b, mvy_b;
+    int sp, sa, si, sp_b, sa_b, si_b;
+    int imode, qp, var;
+};
+
+struct EncLds {
+    IntraWin iw;
+    SubpelLds sp;
+    MbLds mb;
+    IntraEdges edges;
+    int16_t co[2][EVXA_MB_SAMPLES];      // coefficients of a candidate and of the best
+    int16_t lv[2][EVXA_MB_SAMPLES];      // levels after the trellis
+    int16_t rec[2][EVXA_MB_SAMPLES];     // reconstruction
+};
+
+struct EncArgs {
+    DevJob job;                 // type 0 (I), 1 (P, one reference) or 2 (B)
+    DevPic pic;                 // the picture being reconstructed (its slot's previous content)
+    const uint32_t *order;      // MB indices in dependency order
+    int32_t *counters;          // [0] next ticket, [kCounterError] error flags
+    int32_t *spread;            // the picture's spread range record
+};
+
+// Intra matrix (quantize.cpp:13-23), its dead-zone weights (quantize.cpp:
+// 42-52), the reciprocal multipliers of the matrix, and the zig-zag scan
+// (scan.h:52-62: anti-diagonals, odd ones walked downwards).
+struct IntraTables {
+    int16_t qm[64];
+    uint8_t dz[64];
+    uint32_t qm_magic[64];
+    uint8_t zigzag[64];
+};
+
+constexpr IntraTables make_intra_tables()
+{
+    IntraTables t{
+        {8, 10, 12, 15, 19, 23, 25, 27, 10, 12, 14, 17, 21, 25, 27, 28,
+         12, 14, 18, 21, 24, 26, 28, 30, 15, 17, 21, 24, 26, 28, 30, 32,
+         19, 21, 24, 26, 28, 30, 32, 35, 23, 25, 26, 28, 30, 32, 35, 38,
+         25, 27, 28, 30, 32, 35, 38, 41, 27, 28, 30, 32, 35, 38, 41, 45},
+        {0, 1, 1, 1, 1, 1, 2, 2, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 2, 2, 2, 2,
+         1, 1, 1, 2, 2, 2, 2, 3, 1, 1, 2, 2, 2, 2, 3, 3, 1, 2, 2, 2, 2, 3, 3, 3,
+         2, 2, 2, 2, 3, 3, 3, 3, 2, 2, 2, 3, 3, 3, 3, 3},
+        {},
+        {}};
+    for (int i = 0; i < 64; i++) t.qm_magic[i] = 0xFFFFFFFFu / (uint32_t)t.qm[i] + 1u;
+    int p = 0;
+    for (int d = 0; d < 15; d++)
+        for (int k = 0; k <= d; k++) {
+            const int r = (d & 1) ? k : d - k, c = d - r;
+            if (r < 8 && c < 8) t.zigzag[p++] = (uint8_t)(r * 8 + c);
+        }
+    return t;
+}
+
+__constant__ IntraTables c_intra = make_intra_tables();
+
+// quantize.cpp:74-95: the DC scales of intra blocks.
+__device__ __forceinline__ int luma_dc_scale(int qp)
+{
+    if (qp < 5) return max(qp << 1, 2);
+    if (qp < 9) return qp << 1;
+    if (qp < 25) return qp + 8;
+    return (qp << 1) - 16;
+}
+
+__device__ __forceinline__ int chroma_dc_scale(int qp)
+{
+    if (qp < 5) return max(qp << 1, 2);
+    if (qp < 25) return (qp + 13) >> 1;
+    return qp - 6;
+}
+
+// evx_math.h:64 with the byte log2 of 69-87: floor(log2(n & 0xFF)) + 1,
+// log2(0) = 0.
+__device__ __forceinline__ int bits_u8(uint32_t n)
+{
+    n &= 0xFFu;
+    return (n ? 31 - __builtin_clz(n) : 0) + 1;
+}
+
+// quantize.cpp:517-532 (EVX_ENABLE_SIGMAP_CODING): the bits of a level.
+__device__ __forceinline__ int coeff_bits(int level)
+{
+    if (level == 0) return 1;
+    const int v = min(abs(level), 255);
+    return 2 + (v > 1 ? 1 : 0) + (v > 2 ? 1 + 2 * bits_u8((uint32_t)(v - 2)) : 1);
+}
+
+// Intra matrices for intra blocks without motion, inter otherwise
+// (quantize.cpp:457, 469).
+__device__ __forceinline__ bool intra_quant(int btype)
+{
+    return (btype & EVXA_BLOCK_INTRA) != 0 && (btype & EVXA_BLOCK_MOTION) == 0;
+}
+
+// Scan position p of an 8x8 block as a raster index, by the block's intra
+// mode (scan.h:154-159): rows for vertical modes, columns for horizontal
+// ones, zig-zag otherwise.
+__device__ __forceinline__ int scan_pos(int imode, int p)
+{
+    if (imode == 1 || imode == 5) return p;
+    if (imode == 2 || imode == 6) return (p & 7) * 8 + (p >> 3);
+    return c_intra.zigzag[p];
+}
+
+// Sample k of an MB in MB order: its 8x8 block's raster position and
+// whether it is chroma.
+__device__ __forceinline__ int mb_pos(int i, bool &chroma)
+{
+    chroma = i >= 256;
+    return chroma ? (i - 256) & 63 : ((i >> 4) & 7) * 8 + (i & 7);
+}
+
+// Inverse quantisation of one level (quantize.cpp:271-301, 322-332; the
+// trellis's own restatement, quantize.cpp:481-514, is the same arithmetic).
+__device__ __forceinline__ int dequant(int level, int pos, int bq, bool intra, bool chroma)
+{
+    if (intra && pos == 0) return (int16_t)(level * (chroma ? chroma_dc_scale(bq) : luma_dc_scale(bq)));
+    const int qm = intra ? c_intra.qm[pos] : c_quant.qm[pos];
+    return (int16_t)((2 * level * qm * bq) / 16);
+}
+
+// Quantisation of one coefficient: quantize_intra_block_8x8
+// (quantize.cpp:120-200) or quantize_inter_block_8x8 (217-252).
+__device__ __forceinline__ int quant(int sv, int pos, int bq, uint32_t mq, bool intra, bool chroma)
+{
+    if (intra) {
+        if (pos == 0) {
+            const int d = chroma ? chroma_dc_scale(bq) : luma_dc_scale(bq);
+            return (int16_t)rdiv_magic(sv, d, magic_of(d));
+        }
+        int qf = (int16_t)rdiv_magic(sv * 16, c_intra.qm[pos], c_intra.qm_magic[pos]);
+        if (bq >= 16) {
+            const int sg = (qf > 0) - (qf < 0);
+            qf = (int16_t)(qf - (int16_t)((sg * bq * c_intra.dz[pos]) >> 1));
+        }
+        return (int16_t)rdiv_magic(qf, bq << 1, mq);
+    }
+    const int qf = (int16_t)rdiv_magic(sv * 16, c_quant.qm[pos], c_quant.qm_magic[pos]);
+    int w = c_quant.dz[pos];
+    if (bq >= 16) w = min(w + 1, 6);
+    const int sg = (qf > 0) - (qf < 0);
+    const int dz = (int16_t)((sg * bq * w) >> 1);
+    return (int16_t)rdiv_magic(qf - dz, bq << 1, mq);
+}
+
+// The trellis's cost of `level` for coefficient `co` (quantize.cpp:
+// 536-560): squared error of the inverse-quantised level, as an int32 that
+// wraps, plus lambda times the level's bits; fp32 multiply then add.
+__device__ __forceinline__ float trellis_cost(int co, int level, int pos, int bq, bool intra,
+                                              bool chroma, float lambda)
+{
+    const int deq = level == 0 ? 0 : dequant(level, pos, bq, intra, chroma);
+    const int diff = co - deq;
+    const int sq = (int)((uint32_t)diff * (uint32_t)diff);
+    return __fadd_rn(__int2float_rn(sq), __fmul_rn(lambda, __int2float_rn(coeff_bits(level))));
+}
+
+// trellis_optimize_macroblock (quantize.cpp:615-680) on the levels `lv` of
+// coefficients `co`, one 8x8 block at a time, lane p holding scan position
+// p.  The backward pass walks from the last nonzero level down to position
+// 1, zeroing while zero costs no more than the level, and stops at the
+// first that it keeps: so it keeps every nonzero level at or below the
+// highest position whose level it would keep.  The forward pass then lowers
+// each nonzero level (positions >= 1) by one when that costs less; its
+// decisions do not depend on each other.
+__device__ void trellis_mb(int qp, bool intra, int imode, float lambda, const int16_t *co,
+                           int16_t *lv)
+{
+    const int p = lane_id();
+    const int cqp = min(qp + 2, 63);
+    for (int blk = 0; blk < 6; blk++) {
+        int origin, pitch;
+        block_origin(blk, origin, pitch);
+        const bool chroma = blk >= 4;
+        const int bq = chroma ? cqp : qp;
+        const int pos = scan_pos(imode, p);
+        const int idx = origin + (pos >> 3) * pitch + (pos & 7);
+        const int c = co[idx];
+        int level = lv[idx];
+        const uint64_t nz = __ballot(level != 0);
+        if (nz == 0) continue;
+        const int last = 63 - __builtin_clzll(nz);
+        const float keep = trellis_cost(c, level, pos, bq, intra, chroma, lambda);
+        const float zero = trellis_cost(c, 0, pos, bq, intra, chroma, lambda);
+        const uint64_t kept = __ballot(level != 0 && p >= 1 && p <= last && !(zero <= keep));
+        const int stop = kept ? 63 - __builtin_clzll(kept) : 0;
+        if (p >= 1 && p > stop) level = 0;
+        if (p >= 1 && level != 0) {
+            const int reduced = (int16_t)(level - ((level > 0) - (level < 0)));
+            if (trellis_cost(c, reduced, pos, bq, intra, chroma, lambda) < keep) level = reduced;
+        }
+        lv[idx] = (int16_t)level;
+    }
+    wave_sync();
+}
+
+// The descriptor's residual coding (encode.cpp:717-739 with
+// quantize_macroblock and the trellis) and its reconstruction
+// (decode.cpp: inverse quantisation, inverse DCT added to the prediction):
+// from sh.mb.src and the prediction in sh.mb.pred into co/lv/rec of `slot`.
+// Copies are not transformed (encode.cpp:742-744): zero coefficients and
+// levels, the prediction as the reconstruction.
+__device__ void code_mb(const DevJob &job, EncDesc &d, EncLds &sh, int slot)
+{
+    const int lane = lane_id();
+    int16_t *co = sh.co[slot], *lv = sh.lv[slot], *rec = sh.rec[slot];
+    if (d.btype & EVXA_BLOCK_COPY) {
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+            co[i] = 0;
+            lv[i] = 0;
+            rec[i] = sh.mb.pred[i];
+        }
+        wave_sync();
+        return;
+    }
+    fdct_mb(sh.mb.src, sh.mb.pred, sh.mb.res, sh.mb.tmp);
+    int qp, var2;
+    mb_qp(sh.mb.res, job.quality, qp, var2);
+    d.qp = qp;
+    d.var = var2;
+    const bool intra = intra_quant(d.btype);
+    const int cqp = min(qp + 2, 63);
+    const uint32_t m_luma = magic_of(qp << 1), m_chroma = magic_of(cqp << 1);
+#pragma unroll 1
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        bool chroma;
+        const int pos = mb_pos(i, chroma);
+        const int sv = sh.mb.res[i];
+        co[i] = (int16_t)sv;
+        lv[i] = (int16_t)quant(sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra, chroma);
+    }
+    wave_sync();
+    trellis_mb(qp, intra, d.imode, job.lambda, co, lv);
+    // Inverse quantisation into sh.mb.res, then the inverse DCT: vertical
+    // pass (lane = block column) into tmp, horizontal pass plus the
+    // prediction (lane = block row) into rec, stored as int16.
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        bool chroma;
+        const int pos = mb_pos(i, chroma);
+        sh.mb.res[i] = (int16_t)dequant(lv[i], pos, chroma ? cqp : qp, intra, chroma);
+    }
+    wave_sync();
+    if (lane < 48) {
+        int origin, pitch;
+        block_origin(lane >> 3, origin, pitch);
+        const int c = origin + (lane & 7);
+        int v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = sh.mb.res[c + k * pitch];
+#pragma unroll
+        for (int i = 0; i < 8; i++) sh.mb.tmp[c + i * pitch] = (int16_t)idct_sample(v, i);
+    }
+    wave_sync();
+    if (lane < 48) {
+        int origin, pitch;
+        block_origin(lane >> 3, origin, pitch);
+        const int r = origin + (lane & 7) * pitch;
+        int v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = sh.mb.tmp[r + k];
+#pragma unroll
+        for (int i = 0; i < 8; i++) rec[r + i] = (int16_t)(sh.mb.pred[r + i] + idct_sample(v, i));
+    }
+    wave_sync();
+}
+
+// estimate_block_bits (encode.cpp:39-110; EVX_ENABLE_SIGMAP_CODING,
+// EVX_ENABLE_MPM) of a descriptor, the residual part counted on the luma of
+// `blk`: the reference estimates it from the buffer that holds the
+// candidate's reconstruction by then (encode.cpp:150-156).
+__device__ uint32_t block_bits(const EncDesc &d, const int16_t *blk)
+{
+    uint32_t bits = 3;
+    if (!(d.btype & EVXA_BLOCK_INTRA)) bits += 2;
+    if (d.btype & EVXA_BLOCK_MOTION)
+        bits += 2 * bits_u8((uint32_t)(abs(d.mvx) + 1)) + 2 * bits_u8((uint32_t)(abs(d.mvy) + 1)) +
+                (d.sp ? 5 : 1);
+    if (d.btype & EVXA_BLOCK_COPY) return bits;
+    if (d.btype == EVXA_BLOCK_INTRA) bits += 3;
+    bits += 2 * bits_u8((uint32_t)d.qp + 1);
+    // Lane l holds luma samples 4l .. 4l + 3: row l >> 2, columns
+    // (l & 3) * 4 .. + 3, in 8x8 block ((l >> 5) << 1) | ((l >> 1) & 1).
+    const int lane = lane_id();
+    int nz = 0, cb = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int v = blk[4 * lane + k];
+        if (v != 0) {
+            nz++;
+            cb += coeff_bits(v);
+        }
+    }
+    const int mine = ((lane >> 5) << 1) | ((lane >> 1) & 1);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int n = wave_sum(mine == b ? nz : 0);
+        const int c = wave_sum(mine == b ? cb : 0);
+        bits += (uint32_t)(bits_u8((uint32_t)n + 1) * 2 + 1 + c);
+    }
+    return bits;
+}
+
+// Gathers the directional modes' neighbour samples of the MB at (x, y)
+// from the window (intra_pred.cpp:12-52) and the DC values
+// (intra_pred.cpp:58-75: truncating division, 128 without neighbours).
+__device__ void gather_edges(const WinView &wv, int x, int y, IntraEdges &e)
+{
+    const int lane = lane_id();
+    const bool has_top = y > 0, has_left = x > 0;   // the chroma block's neighbours exist alike
+    if (lane < 48) {
+        const int pl = lane >> 4, i = lane & 15, n = pl ? 8 : 16;
+        if (i < n) {
+            const int16_t *d = pl == 0 ? wv.y : pl == 1 ? wv.u : wv.v;
+            const int pitch = pl ? wv.pc : wv.py;
+            const int cx = pl ? (x >> 1) - wv.cx : x - wv.lx, cy = pl ? (y >> 1) - wv.cy : y - wv.ly;
+            e.top[pl][i] = has_top ? d[(cy - 1) * pitch + cx + i] : (int16_t)128;
+            e.left[pl][i] = has_left ? d[(cy + i) * pitch + cx - 1] : (int16_t)128;
+            if (i == 0) {
+                int16_t c;
+                if (has_top && has_left) c = d[(cy - 1) * pitch + cx - 1];
+                else if (has_top) c = d[(cy - 1) * pitch + cx];
+                else if (has_left) c = d[cy * pitch + cx - 1];
+                else c = 128;
+                e.corner[pl] = c;
+            }
+        }
+    }
+    if (lane == 0) {
+        e.has_top = has_top;
+        e.has_left = has_left;
+    }
+    wave_sync();
+    if (lane < 3) {
+        const int n = lane ? 8 : 16;
+        int sum = 0, count = 0;
+        if (has_top) { for (int i = 0; i < n; i++) sum += e.top[lane][i]; count += n; }
+        if (has_left) { for (int i = 0; i < n; i++) sum += e.left[lane][i]; count += n; }
+        e.dc[lane] = count > 0 ? (int16_t)((sum + count / 2) / count) : (int16_t)128;
+    }
+    wave_sync();
+}
+
+// generate_intra_prediction (intra_pred.cpp:58-215) of `mode` into `out`.
+__device__ void intra_predict(int mode, const IntraEdges &e, int16_t *out)
+{
+    const int lane = lane_id();
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        const int pl = i < 256 ? 0 : (i < 320 ? 1 : 2);
+        const int n = pl ? 8 : 16;
+        const int k = pl ? (i - 256) & 63 : i;
+        const int x = pl ? k & 7 : k & 15, y = pl ? k >> 3 : k >> 4;
+        const int16_t *top = e.top[pl], *left = e.left[pl];
+        // left[n - 1] .. left[0], corner, top[0] .. top[n - 1]
+        auto ref = [&](int j) {
+            return j < n ? (int)left[n - 1 - j] : j == n ? (int)e.corner[pl] : (int)top[j - n - 1];
+        };
+        int v;
+        switch (mode) {
+        case 0: v = e.dc[pl]; break;
+        case 1: v = top[x]; break;
+        case 2: v = left[y]; break;
+        case 3: {
+            const int i0 = x + y;
+            v = (top[min(i0, n - 1)] + 2 * top[min(i0 + 1, n - 1)] + top[min(i0 + 2, n - 1)] + 2) >> 2;
+        } break;
+        case 4: {
+            const int d = x - y + n;
+            v = (ref(d - 1) + 2 * ref(d) + ref(d + 1) + 2) >> 2;
+        } break;
+        case 5:
+        case 6: {
+            const int base = mode == 5 ? x - (y >> 1) : (x >> 1) - y;
+            const int odd = mode == 5 ? (y & 1) : (x & 1);
+            const int d = min(max(base + n, 1), 2 * n - 1);
+            v = odd ? (ref(d - 1) + 2 * ref(d) + ref(d + 1) + 2) >> 2 : (ref(d) + ref(d + 1) + 1) >> 1;
+        } break;
+        default: v = 128; break;
+        }
+        out[i] = (int16_t)v;
+    }
+    wave_sync();
+}
+
+// select_intra_mode_candidates (intra_pred.cpp:304-326): DC plus the pair of
+// the luma edge gradients' family, packed as three bytes.
+__device__ __forceinline__ uint32_t intra_candidates(const IntraEdges &e)
+{
+    int gh = 0, gv = 0;
+    if (e.has_top)
+        for (int i = 1; i < 16; i++) gh += abs(e.top[0][i] - e.top[0][i - 1]);
+    if (e.has_left)
+        for (int i = 1; i < 16; i++) gv += abs(e.left[0][i] - e.left[0][i - 1]);
+    if (gv > gh * 3 / 2) return 0u | (2u << 8) | (6u << 16);
+    if (gh > gv * 3 / 2) return 0u | (1u << 8) | (5u << 16);
+    return 0u | (3u << 8) | (4u << 16);
+}
+
+__device__ __forceinline__ int luma_sad(const int16_t *a, const int16_t *b)
+{
+    const int lane = lane_id();
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += abs(a[4 * lane + k] - b[4 * lane + k]);
+    return wave_sum(s);
+}
+
+__device__ __forceinline__ EncDesc desc_clear()
+{
+    // clear_block_desc (common.cpp:78-94)
+    EncDesc d{};
+    d.btype = EVXA_BLOCK_INTRA;
+    d.imode = kIntraNone;
+    return d;
+}
+
+__device__ __forceinline__ EncDesc desc_of(const SearchOut &r, int x, int y, int tgt, int pmode)
+{
+    EncDesc d = desc_clear();
+    d.btype = r.btype;
+    d.pmode = pmode;
+    d.tgt = tgt;
+    d.mvx = r.best_x - x;
+    d.mvy = r.best_y - y;
+    d.sp = r.sp_en;
+    d.sa = r.sp_amt;
+    d.si = r.sp_idx;
+    return d;
+}
+
+__device__ __forceinline__ SearchOut motion_of(int x, int y, int mvx, int mvy, int sp, int sa, int si)
+{
+    return SearchOut{x + mvx, y + mvy, 0, 0, 0, 0.0f, sp, sa, si, 0, 0, 0, 0};
+}
+
+// The prediction a descriptor names into sh.mb.pred (encode.cpp:485-723;
+// decode.cpp:35-433): a directional mode over the window's neighbours, zero
+// for a plain intra block, an intra block copy from the window, or its
+// reference(s) for inter blocks (the first reference with the target id;
+// bidirectional blocks average both).
+__device__ void predict_desc(const DevJob &job, const EncDesc &d, int x, int y, const WinView &wv,
+                             EncLds &sh)
+{
+    const int lane = lane_id();
+    if (d.btype == EVXA_BLOCK_INTRA) {
+        if (d.imode != kIntraNone) {
+            intra_predict(d.imode, sh.edges, sh.mb.pred);
+        } else {
+            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.mb.pred[i] = 0;
+            wave_sync();
+        }
+        return;
+    }
+    const SearchOut r = motion_of(x, y, d.mvx, d.mvy, d.sp, d.sa, d.si);
+    if (d.btype & EVXA_BLOCK_INTRA) {
+        predict_mb(wv, r, sh.mb.pred, sh.mb.tmp);
+        return;
+    }
+    auto ref_of = [&](int target) {
+        int k = 0;
+        for (int i = job.n_refs - 1; i >= 0; i--)
+            if ((int)((job.target_ids >> (8 * i)) & 0xFF) == target) k = i;
+        return k;
+    };
+    predict_mb(job.refs[ref_of(d.tgt)], r, sh.mb.pred, sh.mb.tmp);
+    if (job.type == EVXA_FRAME_B && d.pmode == 2) {
+        const SearchOut b = motion_of(x, y, d.mvx_b, d.mvy_b, d.sp_b, d.sa_b, d.si_b);
+        predict_mb(job.refs[ref_of(d.tgt_b)], b, sh.mb.pred_b, sh.mb.tmp);
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64)
+            sh.mb.pred[i] = (int16_t)((sh.mb.pred[i] + sh.mb.pred_b[i] + 1) >> 1);
+        wave_sync();
+    }
+}
+
+// compute_rdo_cost (encode.cpp:128-166): the candidate coded and
+// reconstructed into `slot`, luma SSD of the reconstruction (an int32 that
+// wraps) plus lambda times its estimated bits, fp32 multiply then add.
+__device__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y, const WinView &wv,
+                          EncLds &sh, int slot)
+{
+    predict_desc(job, d, x, y, wv, sh);
+    code_mb(job, d, sh, slot);
+    const int lane = lane_id();
+    uint32_t ssd = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int t = sh.mb.src[4 * lane + k] - sh.rec[slot][4 * lane + k];
+        ssd += (uint32_t)t * (uint32_t)t;
+    }
+    ssd = (uint32_t)wave_sum((int)ssd);
+    const uint32_t rate = block_bits(d, sh.rec[slot]);
+    return __fadd_rn(__int2float_rn((int)ssd), __fmul_rn(job.lambda, __uint2float_rn(rate)));
+}
+
+// Stores the MB's reconstruction into the picture (and the picture's apron
+// next to it for MBs on the border) with device-scope stores, and grows
+// the picture's range record by it.
+__device__ void store_reconstruction(const EncArgs &a, int bx, int by, const int16_t *rec)
+{
+    const DevJob &job = a.job;
+    const DevPic &pic = a.pic;
+    const int lane = lane_id();
+    const int x = bx * 16, y = by * 16;
+    int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;
+    for (int i = lane; i < 192; i += 64) {
+        int16_t *dst;
+        const int16_t *s;
+        if (i < 128) {
+            const int r = i >> 3, c = (i & 7) * 2;
+            dst = const_cast<int16_t *>(pic.y) + (y + r) * pic.py + x + c;
+            s = rec + r * 16 + c;
+        } else {
+            const int j = i - 128, pl = j >> 5, k = j & 31, r = k >> 2, c = (k & 3) * 2;
+            dst = const_cast<int16_t *>(pl ? pic.v : pic.u) + ((y >> 1) + r) * pic.pc + (x >> 1) + c;
+            s = rec + 256 + 64 * pl + r * 8 + c;
+        }
+        const int v0 = s[0], v1 = s[1];
+        lo = min(lo, min(v0, v1));
+        hi = max(hi, max(v0, v1));
+        st_relaxed32(reinterpret_cast<int32_t *>(dst),
+                     (int32_t)((uint32_t)(uint16_t)v0 | ((uint32_t)(uint16_t)v1 << 16)));
+    }
+    note_range(a.spread, lo, hi);
+    if (bx > 0 && bx < job.wb - 1 && by > 0 && by < job.hb - 1) return;
+    // The apron part this MB owns: its rows and columns extended to the
+    // apron's edge on each border side it touches, the interior excluded;
+    // every value is the clamped sample, which lies in this MB.
+    for (int pl = 0; pl < 3; pl++) {
+        const int n = pl ? 8 : 16;
+        const int X0 = pl ? x >> 1 : x, Y0 = pl ? y >> 1 : y;
+        const int PW = pl ? job.width >> 1 : job.width, PH = pl ? job.height >> 1 : job.height;
+        const int pitch = pl ? pic.pc : pic.py;
+        int16_t *base = const_cast<int16_t *>(pl == 0 ? pic.y : pl == 1 ? pic.u : pic.v);
+        const int16_t *src = rec + (pl == 0 ? 0 : pl == 1 ? 256 : 320);
+        const int xl = bx == 0 ? -kApron : X0, xh = bx == job.wb - 1 ? PW + kApron : X0 + n;
+        const int yl = by == 0 ? -kApron : Y0, yh = by == job.hb - 1 ? PH + kApron : Y0 + n;
+        const int nw = (xh - xl) >> 1;
+        const int total = nw * (yh - yl);
+        for (int i = lane; i < total; i += 64) {
+            const int r = yl + i / nw, c = xl + 2 * (i % nw);
+            if (r >= Y0 && r < Y0 + n && c >= X0 && c < X0 + n) continue;
+            const int sr = min(max(r, 0), PH - 1) - Y0;
+            const int c0 = min(max(c, 0), PW - 1) - X0, c1 = min(max(c + 1, 0), PW - 1) - X0;
+            const int v0 = src[sr * n + c0], v1 = src[sr * n + c1];
+            st_relaxed32(reinterpret_cast<int32_t *>(base + r * pitch + c),
+                         (int32_t)((uint32_t)(uint16_t)v0 | ((uint32_t)(uint16_t)v1 << 16)));
+        }
+    }
+}
+
+// One MB of the in-loop encode (see the section comment).  Inlined into the
+// kernel, so that `sh` is known to be LDS and its accesses are LDS ones.
+__device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
+{
+    const DevJob &job = a.job;
+    const int lane = lane_id();
+    const int bx = mb % job.wb, by = mb / job.wb;
+    const int x = bx * 16, y = by * 16;
+    const int W = job.width, H = job.height;
+    const Geo geo = lane_geometry();
+
+    // While the dependencies finish: the source block, and for P and B the
+    // first reference's block at (x, y) and its search window.
+    Blk s, pre;
+    load_cand(job.src, geo, x, y, s);
+    store_src(s, geo, sh.mb.src);
+    WinPos w0{};
+    bool narrow0 = false;
+    if (job.type != 0) {
+        narrow0 = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[0].wide)) != 0;
+        load_cand(job.refs[0], geo, x, y, pre);
+        w0 = stage_window<1>(job.refs[0], x, y, kReachMax, sh.sp);
+    }
+    uint32_t p0 = 0, p1 = 0;
+    if (job.type == EVXA_FRAME_P && job.prev) {
+        const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + (size_t)mb * 16;
+        p0 = prv[0];
+        p1 = prv[1];
+    }
+
+    // Wait for MB (bx - 1, by) and MB (min(bx + lag, wb - 1), by - 1); the
+    // rest of what this MB reads was done before them.
+    const int dep = lane == 0 ? (bx > 0 ? mb - 1 : -1)
+                  : lane == 1 ? (by > 0 ? (by - 1) * job.wb + min(bx + kEncodeLag, job.wb - 1) : -1)
+                  : -1;
+    {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        int nap = 1;
+        bool timed_out = false;
+        for (;;) {
+            uint64_t w = 1ull << 63;
+            if (dep >= 0) w = ld_relaxed64(job.mvw + dep);
+            if (__ballot((w >> 63) == 0) == 0) break;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kEncWaitTicks) { timed_out = true; break; }
+            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+            nap = min(nap * 2, kMaxNap);
+        }
+        if (timed_out && lane == 0) atomicOr(&a.counters[kCounterError], kErrWaitTimeout);
+    }
+    // The MV predictor's neighbours (left, top, top-right, top-left).
+    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
+    const bool has_tl = has_t && has_l;
+    const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
+                   : lane == 1 ? (has_t ? mb - job.wb : -1)
+                   : lane == 2 ? (has_tr ? mb - job.wb + 1 : -1)
+                   : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1) : -1;
+    const uint64_t word = nidx >= 0 ? ld_relaxed64(job.mvw + nidx) : 0;
+    int pmx = 0, pmy = 0;
+    if (job.type != 0) {
+        mv_predictor(job, bx, by, lane_u64(word, 0), lane_u64(word, 1), lane_u64(word, 2),
+                     lane_u64(word, 3), p0, p1, pmx, pmy);
+        pmx = __builtin_amdgcn_readfirstlane(pmx);
+        pmy = __builtin_amdgcn_readfirstlane(pmy);
+    }
+
+    // The window of the picture being reconstructed, origin moved inside the
+    // apron at the borders (its reads are then the clamped ones).
+    const int x0 = min(max(x - 34, -kApron), W + kApron - kIwLS);
+    const int y0 = min(max(y - 50, -kApron), H + kApron - kIwLR);
+    const int cx0 = min(max((x >> 1) - 20, -kApron), (W >> 1) + kApron - kIwCS);
+    const int cy0 = min(max((y >> 1) - 27, -kApron), (H >> 1) + kApron - kIwCR);
+    dma_rows<kIwLS / 2, 1, kLoadDevice>(
+        reinterpret_cast<const uint32_t *>(a.pic.y + y0 * a.pic.py + x0), a.pic.py / 2, kIwLR,
+        reinterpret_cast<uint32_t *>(sh.iw.y));
+    dma_rows<kIwCS / 2, 1, kLoadDevice>(
+        reinterpret_cast<const uint32_t *>(a.pic.u + cy0 * a.pic.pc + cx0), a.pic.pc / 2, kIwCR,
+        reinterpret_cast<uint32_t *>(sh.iw.u));
+    dma_rows<kIwCS / 2, 1, kLoadDevice>(
+        reinterpret_cast<const uint32_t *>(a.pic.v + cy0 * a.pic.pc + cx0), a.pic.pc / 2, kIwCR,
+        reinterpret_cast<uint32_t *>(sh.iw.v));
+    window_ready();
+    const WinView wv{sh.iw.y, sh.iw.u, sh.iw.v, kIwLS, kIwCS, x0, y0, cx0, cy0};
+
+    // calculate_intra_prediction first, for every frame type (encode.cpp:175,
+    // 362, 380).
+    const int thresh = job.thresh;
+    const float lambda = job.lambda;
+    const SearchOut ri = intra_search(wv, W, H, s, geo, x, y, thresh, lambda, sh.mb);
+    EncDesc best = desc_of(ri, x, y, 0, 0);
+    int best_sad = ri.sad;
+    int n_int = ri.n_int, n_ax = ri.n_ax, n_dg = ri.n_dg;
+    const int tgt0 = job.target_ids & 0xFF, tgt1 = (job.target_ids >> 8) & 0xFF;
+
+    if (job.type == EVXA_FRAME_P) {
+        // encode.cpp:183-210: the copy state decides first, then the SAD.
+        const SearchOut r = inter_search<1>(job.refs[0], W, H, narrow0, s, geo, pre, w0, x, y,
+                                            thresh, lambda, pmx, pmy, sh.sp);
+        n_int += r.n_int; n_ax += r.n_ax; n_dg += r.n_dg;
+        const bool inter_copy = (r.btype & EVXA_BLOCK_COPY) != 0;
+        const bool best_copy = (best.btype & EVXA_BLOCK_COPY) != 0;
+        if (inter_copy != best_copy) {
+            if (inter_copy) { best = desc_of(r, x, y, tgt0, 0); best_sad = r.sad; }
+        } else if (r.sad < best_sad) {
+            best = desc_of(r, x, y, tgt0, 0);
+            best_sad = r.sad;
+        }
+    } else if (job.type == EVXA_FRAME_B) {
+        // encode.cpp:386-458: forward, backward, then their average.
+        const SearchOut f = inter_search<1>(job.refs[0], W, H, narrow0, s, geo, pre, w0, x, y,
+                                            thresh, lambda, pmx, pmy, sh.sp);
+        const bool narrow1 =
+            __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[1].wide)) != 0;
+        Blk pre1;
+        load_cand(job.refs[1], geo, x, y, pre1);
+        const WinPos w1 = stage_window<1>(job.refs[1], x, y, kReachMax, sh.sp);
+        window_ready();
+        const SearchOut b = inter_search<1>(job.refs[1], W, H, narrow1, s, geo, pre1, w1, x, y,
+                                            thresh, lambda, pmx, pmy, sh.sp);
+        n_int += f.n_int + b.n_int;
+        n_ax += f.n_ax + b.n_ax;
+        n_dg += f.n_dg + b.n_dg;
+        if (f.sad < best_sad) { best = desc_of(f, x, y, tgt0, 0); best_sad = f.sad; }
+        if (b.sad < best_sad) { best = desc_of(b, x, y, tgt1, 1); best_sad = b.sad; }
+        const int bsad = b_predictions(job, f, b, sh.mb);
+        if (bsad + __float2int_rz(__fmul_rn(lambda, 8.0f)) < best_sad) {
+            best = desc_clear();
+            best.btype = EVXA_BLOCK_MOTION;
+            best.pmode = 2;
+            best.tgt = tgt0;
+            best.mvx = f.best_x - x; best.mvy = f.best_y - y;
+            best.sp = f.sp_en; best.sa = f.sp_amt; best.si = f.sp_idx;
+            best.tgt_b = tgt1;
+            best.mvx_b = b.best_x - x; best.mvy_b = b.best_y - y;
+            best.sp_b = b.sp_en; best.sa_b = b.sp_amt; best.si_b = b.sp_idx;
+            best_sad = bsad;
+        }
+    }
+    if (best_sad == kInt32Max && lane == 0) atomicOr(&a.counters[kCounterError], kErrSearchFailed);
+
+    // The directional intra candidate (encode.cpp:254-272, 446-468;
+    // select_best_intra_mode, intra_pred.cpp:217-302: the first candidate
+    // with the smallest luma SAD).
+    gather_edges(wv, x, y, sh.edges);
+    const uint32_t cands = intra_candidates(sh.edges);
+    int dir_sad = kInt32Max, dir_mode = 0;
+    for (int c = 0; c < 3; c++) {
+        const int m = (cands >> (8 * c)) & 0xFF;
+        intra_predict(m, sh.edges, sh.mb.pred);
+        const int sd = luma_sad(sh.mb.src, sh.mb.pred);
+        if (sd < dir_sad) { dir_sad = sd; dir_mode = m; }
+        wave_sync();
+    }
+    if (dir_sad < best_sad) {
+        best = desc_clear();
+        best.imode = dir_mode;
+        best_sad = dir_sad;
+    }
+
+    // RDO for P frames (encode.cpp:274-348, EVX_ENABLE_RDO_COPY_EVAL and
+    // EVX_ENABLE_RDO_INTRA_MODES): the SAD winner, the plain intra block
+    // unless the winner is one of the intra-default kind, then the three
+    // mode candidates; a strictly lower cost wins.  Otherwise the winner is
+    // coded once.
+    int slot = 0;
+    float best_cost = 0.0f;
+    if (job.type == EVXA_FRAME_P) {
+        best_cost = rdo_cost(job, best, x, y, wv, sh, 0);
+        if (best.btype != EVXA_BLOCK_INTRA) {
+            EncDesc c = desc_clear();
+            const float cc = rdo_cost(job, c, x, y, wv, sh, slot ^ 1);
+            if (cc < best_cost) { best = c; best_cost = cc; slot ^= 1; }
+        }
+        for (int k = 0; k < 3; k++) {
+            EncDesc c = desc_clear();
+            c.imode = (cands >> (8 * k)) & 0xFF;
+            const float cc = rdo_cost(job, c, x, y, wv, sh, slot ^ 1);
+            if (cc < best_cost) { best = c; best_cost = cc; slot ^= 1; }
+        }
+    } else {
+        predict_desc(job, best, x, y, wv, sh);
+        code_mb(job, best, sh, 0);
+    }
+
+    // Outputs: the reconstruction into the picture, coefficients, levels,
+    // the record; then the MB's word, once every store has completed.
+    store_reconstruction(a, bx, by, sh.rec[slot]);
+    int16_t *co = job.coeffs + (size_t)mb * EVXA_MB_SAMPLES;
+    int16_t *lv = job.levels + (size_t)mb * EVXA_MB_SAMPLES;
+    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+        co[i] = sh.co[slot][i];
+        lv[i] = sh.lv[slot][i];
+    }
+    uint32_t w[16];
+    w[0] = (uint32_t)best.btype | ((uint32_t)best.pmode << 8) | ((uint32_t)best.tgt << 16) |
+           ((uint32_t)best.tgt_b << 24);
+    w[1] = mv_word(best.mvx, best.mvy);
+    w[2] = mv_word(best.mvx_b, best.mvy_b);
+    w[3] = (uint32_t)best.sp | ((uint32_t)best.sa << 8) | ((uint32_t)best.si << 16) |
+           ((uint32_t)best.sp_b << 24);
+    w[4] = (uint32_t)best.sa_b | ((uint32_t)best.si_b << 8) | ((uint32_t)best.qp << 16) |
+           ((uint32_t)best.imode << 24);
+    w[5] = (uint32_t)(uint16_t)(int16_t)best.var;
+    w[6] = (uint32_t)best_sad;
+    w[7] = 0;
+    w[8] = fbits(best_cost);
+    w[9] = 0; w[10] = 0; w[11] = 0;
+    w[12] = (uint32_t)n_int;
+    w[13] = (uint32_t)n_ax | ((uint32_t)n_dg << 16);
+    w[14] = ((best.btype & EVXA_BLOCK_COPY) ? 0u : 1u) | (best.pmode == 2 ? 2u : 0u) | 4u;
+    w[15] = 0;
+    write_words(reinterpret_cast<uint32_t *>(job.records + mb), w, 16);
+    stores_done();
+    if (lane == 0)
+        st_relaxed64(job.mvw + mb, pred_word((best.btype & EVXA_BLOCK_MOTION) != 0, w[1]));
+}
+
+// The persistent in-loop encode kernel: one wave per workgroup takes the
+// next MB of the dependency order until none is left.  A MB waits only on
+// MBs earlier in the order, which running waves hold, so the launch cannot
+// deadlock whatever the number of resident workgroups.
+__global__ __launch_bounds__(64) void encode_kernel(const EncArgs a)
+{
+    __shared__ EncLds sh;
+    __shared__ int32_t s_word;
+    const int n_mb = a.job.wb * a.job.hb;
+    for (;;) {
+        if (threadIdx.x == 0) s_word = atomicAdd(&a.counters[0], 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_word);
+        __syncthreads();
+        if (t >= n_mb) break;
+        const int mb = (int)__builtin_amdgcn_readfirstlane((int)a.order[t]);
+        encode_mb(a, mb, sh);
+    }
+}
+
+// The encode's dependency order: MB (bx, by) at diagonal v = bx + (lag + 1) by,
+// MBs of a diagonal by row.  MB (bx, by) is preceded by the MBs (x, y) of
+// smaller diagonals -- min(max(v - (lag + 1) y, 0), wb) of them on row y --
+// and by those of diagonal v on rows y < by.  Its dependencies, MB (bx - 1,
+// by) and MB (min(bx + lag, wb - 1), by - 1), lie on smaller diagonals.
+__global__ void encode_order_kernel(int32_t wb, int32_t hb, uint32_t *order)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= wb * hb) return;
+    const int slope = kEncodeLag + 1;
+    const int bx = i % wb, by = i / wb, v = bx + slope * by;
+    int before = 0;
+    for (int y = 0; y < hb; y++) {
+        const int c = v - slope * y;
+        before += c <= 0 ? 0 : (c < wb ? c : wb);
+        if (y < by && c >= 0 && c < wb) before++;
+    }
+    order[before] = (uint32_t)i;
 }
 
 // ------------------------------------------------- allocation-free path --
@@ -4269,6 +5195,39 @@ hipError_t launch_wave_list(int32_t wb, int32_t hb, int32_t n_refs, uint32_t *wo
     const int n = wb * hb;
     hipLaunchKernelGGL(wave_list_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wb, hb, n_refs, work);
     return hipGetLastError();
+}
+
+hipError_t launch_encode_order(int32_t wb, int32_t hb, uint32_t *order, hipStream_t st)
+{
+    const int n = wb * hb;
+    hipLaunchKernelGGL(encode_order_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wb, hb, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode(const DevJob &job, const DevPic &pic, const uint32_t *order,
+                         int32_t *counters, int32_t *spread, int32_t grid, hipStream_t st)
+{
+    EncArgs a;
+    a.job = job;
+    a.pic = pic;
+    a.order = order;
+    a.counters = counters;
+    a.spread = spread;
+    hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+int32_t encode_resident_groups(int32_t device)
+{
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        n_cu <= 0)
+        n_cu = 256;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel, 64, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return n_cu * per_cu;
 }
 
 int32_t analyze_resident_groups(int32_t device, int32_t waves, bool sweep)

@@ -22,6 +22,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libevxa.so")
 
+FRAME_I = 0
 FRAME_P = 1
 FRAME_B = 2
 MB_SAMPLES = 384
@@ -89,6 +90,13 @@ class FrameDesc(C.Structure):
                 ("update_temporal", C.c_int32)]
 
 
+class EncodeDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("quality", C.c_int32), ("src_slot", C.c_int32),
+                ("dst_slot", C.c_int32), ("ref_slots", C.c_int32 * 2),
+                ("target_ids", C.c_uint8 * 2), ("reserved", C.c_uint8 * 2),
+                ("temporal_mvp", C.c_int32), ("update_temporal", C.c_int32)]
+
+
 # Every symbol include/evxa.h declares.
 EXPORTS = (
     "evxa_create", "evxa_destroy", "evxa_last_error", "evxa_version", "evxa_stream_bytes",
@@ -101,7 +109,7 @@ EXPORTS = (
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
     "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats", "evxa_analyze_frame_noalloc",
-    "evxa_stream_wait", "evxa_stream_query", "evxa_intra_search_blocks")
+    "evxa_stream_wait", "evxa_stream_query", "evxa_intra_search_blocks", "evxa_encode_frame")
 
 
 class EvxaError(RuntimeError):
@@ -138,6 +146,7 @@ def load_library(path: str = LIB_PATH):
         "evxa_stream_reset_temporal": (C.c_int, [vp]),
         "evxa_analyze_frame": (C.c_int, [vp, C.POINTER(FrameDesc)]),
         "evxa_analyze_frame_noalloc": (C.c_int, [vp, C.POINTER(FrameDesc)]),
+        "evxa_encode_frame": (C.c_int, [vp, C.POINTER(EncodeDesc)]),
         "evxa_analyze_batch": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32]),
         "evxa_synchronize": (C.c_int, [vp]),
         "evxa_stream_wait": (C.c_int, [vp]),
@@ -222,6 +231,27 @@ def frame_desc(ftype, quality, src_slot, ref_slots, target_ids=None, temporal_mv
     d.temporal_mvp = int(bool(temporal_mvp)) if ftype == FRAME_P else 0
     if update_temporal is None:
         update_temporal = ftype == FRAME_P
+    d.update_temporal = int(bool(update_temporal))
+    return d
+
+
+def encode_desc(ftype, quality, src_slot, dst_slot, ref_slots=(), target_ids=None,
+                temporal_mvp=True, update_temporal=None):
+    """One frame of the in-loop encode: FRAME_I, FRAME_P (ref_slots[0]) or
+    FRAME_B (forward, backward)."""
+    d = EncodeDesc()
+    d.type = ftype
+    d.quality = quality
+    d.src_slot = src_slot
+    d.dst_slot = dst_slot
+    for k, s in enumerate(ref_slots):
+        d.ref_slots[k] = s
+    ids = target_ids if target_ids is not None else [1, 2][:len(ref_slots)]
+    for k, t in enumerate(ids):
+        d.target_ids[k] = t
+    d.temporal_mvp = int(bool(temporal_mvp)) if ftype == FRAME_P else 0
+    if update_temporal is None:
+        update_temporal = ftype != FRAME_B
     d.update_temporal = int(bool(update_temporal))
     return d
 
@@ -440,6 +470,15 @@ class Stream:
         """evxa_analyze_frame_noalloc: the wavefront analysis with its launch
         scratch in the library's per-GPU array (no allocation)."""
         _check(_lib.evxa_analyze_frame_noalloc(self.h, C.byref(desc)))
+
+    def encode(self, desc: EncodeDesc):
+        """evxa_encode_frame: the in-loop encode of one frame into
+        desc.dst_slot; results() then returns its descriptors, coefficients
+        and post-trellis levels."""
+        _check(_lib.evxa_encode_frame(self.h, C.byref(desc)))
+
+    def encode_frame(self, ftype, quality, src_slot, dst_slot, ref_slots=(), **kw):
+        self.encode(encode_desc(ftype, quality, src_slot, dst_slot, ref_slots, **kw))
 
     def analyze_p(self, quality, src_slot, ref_slots, **kw):
         self.analyze(frame_desc(FRAME_P, quality, src_slot, ref_slots, **kw))

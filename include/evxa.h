@@ -33,6 +33,7 @@ extern "C" {
 #define EVXA_ERROR_OPERATION_TIMEDOUT 9
 #define EVXA_ERROR_EXECUTION_FAILURE 10
 
+#define EVXA_FRAME_I 0
 #define EVXA_FRAME_P 1
 #define EVXA_FRAME_B 2
 #define EVXA_MAX_REFS 4
@@ -109,6 +110,23 @@ typedef struct evxa_frame_desc {
     int32_t temporal_mvp;         /* P: use the stream's temporal table if one exists */
     int32_t update_temporal;      /* P: this frame's table becomes the temporal table */
 } evxa_frame_desc;
+
+/* One frame of the in-loop encode (evxa_encode_frame). */
+typedef struct evxa_encode_desc {
+    int32_t type;                 /* EVXA_FRAME_I, EVXA_FRAME_P (one reference) or
+                                     EVXA_FRAME_B (forward, backward) */
+    int32_t quality;              /* frame QP 0..63 */
+    int32_t src_slot;             /* the current frame */
+    int32_t dst_slot;             /* the picture-buffer slot reconstructed into; it holds
+                                     its previous content on entry, as in the reference */
+    int32_t ref_slots[2];         /* P: [0]; B: [0] forward, [1] backward */
+    uint8_t target_ids[2];        /* written to prediction_target(_b) */
+    uint8_t reserved[2];
+    int32_t temporal_mvp;         /* P: use the stream's temporal table if one exists */
+    int32_t update_temporal;      /* I and P: this frame's records become the temporal
+                                     table (the reference copies every non-B frame's
+                                     block table, encode.cpp:990-997) */
+} evxa_encode_desc;
 
 typedef struct evxa_context evxa_context;
 typedef struct evxa_stream evxa_stream;
@@ -245,6 +263,29 @@ int evxa_sao(evxa_stream *s, int32_t slot, int32_t src_slot, int32_t quality,
  * `slot` in place, as the decoder's sao_filter_image does after deblocking
  * (sao.cpp:98-148; decode.cpp:589). */
 int evxa_sao_filter(evxa_stream *s, int32_t slot, const evxa_sao_info *table);
+
+/* The in-loop encode of one frame, as the reference's encode_slice
+ * (encode.cpp:751-813) runs it: per MB in raster-order semantics, the mode
+ * decision of classify_block (I and P, encode.cpp:168-370: the intra-frame
+ * block search over the picture being reconstructed, motion.cpp:485-565;
+ * for P the inter search with the temporal table; the directional intra
+ * mode, intra_pred.cpp:217-326; RDO over the SAD winner, the plain intra
+ * block and the three mode candidates, encode.cpp:128-166, 274-348) or of
+ * classify_block_bidir (B, encode.cpp:372-476), encode_block with the
+ * trellis (encode.cpp:479-748; quantize.cpp:454-464, 615-680), then
+ * decode_block into `dst_slot` (decode.cpp:31-433).  One launch; MBs run in
+ * a wavefront whose lag keeps every MB's reads of `dst_slot` equal to the
+ * sequential encoder's.  Outputs: the reconstruction in `dst_slot` (before
+ * deblocking and SAO, which evxa_deblock and evxa_sao then apply), and,
+ * through evxa_read_results, one descriptor per MB (q_index and variance as
+ * encode_block sets them; sad the SAD-stage best; cost_bits the RDO cost of
+ * P frames; flags bit0 residual coded, bit1 bidirectional, bit2 encode),
+ * the coefficients and the post-trellis levels (zero for copies).
+ * dst_slot must differ from the source and the references; frames are at
+ * least 48 x 48.  Replaces the encoder's encode_slice / encode_slice_wpp
+ * call in engine_encode_frame (encode.cpp:958-962) and its row workers
+ * (encode.cpp:817-941). */
+int evxa_encode_frame(evxa_stream *s, const evxa_encode_desc *desc);
 
 /* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
 

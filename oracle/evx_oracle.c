@@ -1850,7 +1850,7 @@ static void desc_from_search(evo_record *d, const evo_search_result *r, uint8_t 
 }
 
 int32_t evo_encode_frame(const evo_encode_job *job, int16_t *pic_y, int16_t *pic_u,
-                         int16_t *pic_v, evo_record *records, int16_t *levels)
+                         int16_t *pic_v, evo_record *records, int16_t *levels, int16_t *coeffs)
 {
     int32_t w = job->src.width, h = job->src.height, wb = w / EVO_MB, hb = h / EVO_MB;
     evo_planes pic = {pic_y, pic_u, pic_v, w, h, w, w / 2};
@@ -1973,6 +1973,7 @@ int32_t evo_encode_frame(const evo_encode_job *job, int16_t *pic_y, int16_t *pic
              * (encode.cpp:795-811). */
             int16_t *lv = levels + (size_t)idx * EVO_MB_SAMPLES;
             if (!encode_mb(&e, src, &best, x, y, co, lv)) return 3;
+            if (coeffs) memcpy(coeffs + (size_t)idx * EVO_MB_SAMPLES, co, sizeof(co));
             if (!decode_mb(&e, &best, x, y, lv, rec)) return 3;
             for (int j = 0; j < 16; j++)
                 memcpy(pic_y + (size_t)(y + j) * w + x, rec + j * 16, 16 * sizeof(int16_t));

@@ -301,10 +301,11 @@ typedef struct evo_encode_job {
  * descriptors (q_index and variance as encode_block sets them; sad the SAD
  * stage's best, cost_bits the RDO cost of P frames; flags bit0 residual
  * coded, bit1 bidirectional, bit2 encode mode); levels: 384 per MB after
- * trellis, zero for copies.  Returns 0, 3 for a bad job or 10 if a search
- * fails. */
+ * trellis, zero for copies; coeffs (may be NULL): the transform
+ * coefficients encode_block quantised, zero for copies.  Returns 0, 3 for a
+ * bad job or 10 if a search fails. */
 int32_t evo_encode_frame(const evo_encode_job *job, int16_t *pic_y, int16_t *pic_u,
-                         int16_t *pic_v, evo_record *records, int16_t *levels);
+                         int16_t *pic_v, evo_record *records, int16_t *levels, int16_t *coeffs);
 
 #ifdef __cplusplus
 }

@@ -255,11 +255,21 @@ class _Encode:
                       _ptr16(np.ascontiguousarray(coeffs, np.int16)), _ptr16(out))
         return out
 
-    def _encode_call(self, job, y, u, v, recs, levels):
+    def _encode_call(self, job, y, u, v, recs, levels, coeffs=None):
         return self._encode(job, _ptr16(y), _ptr16(u), _ptr16(v), recs.ctypes.data,
-                            _ptr16(levels))
+                            _ptr16(levels), None if coeffs is None else _ptr16(coeffs))
 
-    def encode_frame(self, ftype, quality, src, refs, pic, prev=None, target_ids=None):
+    def encode_frame_coeffs(self, ftype, quality, src, refs, pic, prev=None, target_ids=None):
+        """encode_frame, plus the coefficients each MB's levels quantise
+        (n_mbs x 384, zero for copies): (records, levels, planes, coeffs)."""
+        n = (src.width // 16) * (src.height // 16)
+        co = np.zeros((n, 384), np.int16)
+        recs, lv, pic = self.encode_frame(ftype, quality, src, refs, pic, prev, target_ids,
+                                          _coeffs=co)
+        return recs, lv, pic, co
+
+    def encode_frame(self, ftype, quality, src, refs, pic, prev=None, target_ids=None,
+                     _coeffs=None):
         """encode_slice of one frame (0 = I, 1 = P with refs[0], 2 = B with
         refs[0] forward and refs[1] backward).  pic: the (y, u, v) planes the
         picture-buffer slot holds before the frame.  Returns (records,
@@ -280,7 +290,10 @@ class _Encode:
         n = (src.width // 16) * (src.height // 16)
         recs = np.zeros(n, RECORD_DTYPE)
         levels = np.zeros((n, 384), np.int16)
-        rc = self._encode_call(C.byref(job), y, u, v, recs, levels)
+        if _coeffs is None:
+            rc = self._encode_call(C.byref(job), y, u, v, recs, levels)
+        else:
+            rc = self._encode_call(C.byref(job), y, u, v, recs, levels, _coeffs)
         if rc != 0:
             raise RuntimeError(f"encode_frame failed: {rc}")
         return recs, levels, (y, u, v)
@@ -316,7 +329,7 @@ class Oracle(_Lib, _Recon, _Encode):
         self._frame = self.f("analyze_frame", C.c_int32, C.POINTER(FrameJob), C.c_void_p,
                              I16P, I16P)
         self._bind_recon()
-        self._bind_encode()
+        self._bind_encode((I16P,))
 
     def lam(self, q):
         return self._lambda(q)
