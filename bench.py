@@ -991,6 +991,166 @@ def loop_cpu_chain(args, frames, device_out):
     return all(c for _, c in checked), [t for t, _ in checked]
 
 
+ENCODE_FIELDS = ("block_type", "prediction_mode", "prediction_target", "prediction_target_b",
+                 "motion_x", "motion_y", "motion_x_b", "motion_y_b", "sp_pred", "sp_amount",
+                 "sp_index", "sp_pred_b", "sp_amount_b", "sp_index_b", "q_index", "variance",
+                 "intra_mode")
+
+
+def run_encode(args):
+    """The reference encoder's frame loop on the device at 1920x1080: its own
+    GOP (an I frame, then per GOP a P anchor, a reference B and two
+    non-reference Bs, in decode order, with its picture-buffer slots and B
+    qualities), each frame coded by the in-loop encode (mode decision with
+    the intra-frame block search, RDO for P, trellis, reconstruction into
+    its slot over the slot's previous picture), then deblocked and SAO'd in
+    place, as engine_encode_frame does (encode.cpp:945-983).  The sequence
+    runs once untimed (its first frames and its last are checked against
+    the reference build's own chain on the host) and once timed."""
+    from chicago_amd import evxa, gop, synth
+
+    w, h = 1920, 1080
+    aw, ah = synth.align16(w), synth.align16(h)
+    n = args.encode_frames
+    q = args.quality
+    frames = synth.sequence_c1(synth.seed_of(1, 0, 0), n, w, h)
+    roles = gop.reference_schedule(n, q)
+    ctx = evxa.Context(0)
+    n_dpb = 4
+    s = ctx.open_stream(aw, ah, n_dpb + n)
+    src_slot = lambda d: n_dpb + d
+    for d, f in enumerate(frames):
+        s.upload(src_slot(d), *f)
+    kind = {"I": evxa.FRAME_I, "P": evxa.FRAME_P, "B": evxa.FRAME_B}
+
+    def code(r):
+        refs = [] if r.kind == "I" else r.ref_slots
+        s.encode_frame(kind[r.kind], r.quality, src_slot(r.index), r.slot, refs,
+                       target_ids=refs, temporal_mvp=r.kind == "P" and r.temporal_from >= 0)
+        s.deblock(r.slot)
+        s.sao(r.slot, src_slot(r.index), r.quality, table=False)
+
+    res = {"workload": f"encode: the reference encoder's GOP (I, then P + reference B + 2 Bs per "
+                       f"GOP) over {n} 1920x1080 frames, quality {q}; per frame the in-loop encode "
+                       f"(encode_slice), deblocking and SAO on the device",
+           "frames": n, "decode_order": "".join(r.kind for r in roles)}
+    check = sorted({0, 1, 2, 3, 4, len(roles) - 1})
+    device_out = {}
+    for k, r in enumerate(roles):
+        code(r)
+        if k in check:
+            rec, _, lv = s.results()
+            device_out[k] = (rec, lv, s.download(r.slot))
+    ctx.synchronize()
+    ctx.launch_times()
+    t0 = time.perf_counter()
+    for r in roles:
+        code(r)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    res["ms_per_frame"] = round(dt / len(roles) * 1e3, 3)
+    res["frames_per_s"] = round(len(roles) / dt, 2)
+    ms = np.array(ctx.launch_times())
+    if ms.size == 4 * len(roles):
+        per = ms.reshape(len(roles), 4)
+        res["device_ms_per_frame"] = {name: round(float(v), 3) for name, v in
+                                      zip(("encode", "deblock", "sao_decide", "sao_apply"),
+                                          per.mean(axis=0))}
+        by_kind = {}
+        for kk in "IPB":
+            rows = [i for i, r in enumerate(roles) if r.kind == kk]
+            if rows:
+                by_kind[kk] = round(float(per[rows, 0].mean()), 3)
+        res["encode_ms_by_type"] = by_kind
+        # The MB wavefront's length: MB (bx, by) runs after (bx + 3, by - 1).
+        wb, hb = aw // 16, ah // 16
+        steps = wb + 4 * (hb - 1)
+        res["wavefront_steps"] = steps
+        res["us_per_wavefront_step"] = round(float(per[:, 0].mean()) * 1e3 / steps, 2)
+    s.close()
+    ctx.close()
+    if not args.no_cpu:
+        res.update(encode_cpu_chain(args, frames, roles, device_out))
+    return res
+
+
+def encode_cpu_chain(args, frames, roles, device_out):
+    """The same frame loop on the host -- the reference build's encode_slice
+    (its row workers on the host's threads), deblocking and SAO, where the
+    reference is built, else the oracle's -- from zeroed picture-buffer
+    slots; compares the frames in `device_out` (descriptors, post-trellis
+    levels, decoded picture) and times encode_slice per frame on one thread
+    and on the host's threads over a bounded sample."""
+    from oracle.oracle import Oracle, Picture, Reference, reference_available
+    impl = Reference() if reference_available() else Oracle()
+    kind = {"I": 0, "P": 1, "B": 2}
+    h, w = frames[0][0].shape
+    zero = (np.zeros((h, w), np.int16), np.zeros((h // 2, w // 2), np.int16),
+            np.zeros((h // 2, w // 2), np.int16))
+    dpb = [zero] * 4
+    tables = {}
+    checked = []
+    if isinstance(impl, Reference):
+        impl.encode_threads = args.cpu_threads
+    t0 = time.perf_counter()
+    for k, r in enumerate(roles):
+        refs = [Picture(*dpb[sl]) for sl in ([] if r.kind == "I" else r.ref_slots)]
+        prev = tables.get(r.temporal_from) if r.kind == "P" else None
+        cur = Picture(*frames[r.index])
+        recs, lv, pic = impl.encode_frame(kind[r.kind], r.quality, cur, refs, dpb[r.slot],
+                                          prev=prev, target_ids=r.ref_slots or None)
+        filt = impl.deblock_frame(recs, *pic)
+        _, *dec = impl.sao_encode_frame(frames[r.index], filt, impl.lam(r.quality))
+        dpb[r.slot] = tuple(dec)
+        if r.kind != "B":
+            tables[r.index] = recs
+        if k in device_out:
+            grec, glv, gpic = device_out[k]
+            same = (all(np.array_equal(grec[f], recs[f]) for f in ENCODE_FIELDS)
+                    and np.array_equal(glv, lv)
+                    and all(np.array_equal(a, b) for a, b in zip(gpic, dec)))
+            log(f"encode frame {k} ({r.kind}{r.index}): CPU chain bit-exact={same} "
+                f"({time.perf_counter() - t0:.1f} s)")
+            checked.append((k, bool(same)))
+        if k >= max(device_out):
+            break
+    out = {"bit_exact_vs_cpu": all(c for _, c in checked),
+           "checked_frames_decode_order": [k for k, _ in checked]}
+    # CPU baseline: encode_slice alone on the first P anchor and the first
+    # reference B of the sequence, one thread and the host's threads.
+    if isinstance(impl, Reference):
+        sample = [k for k, r in enumerate(roles) if r.kind in "PB"][:2]
+        base = {}
+        for threads in (1, args.cpu_threads):
+            impl.encode_threads = threads
+            dpb = [zero] * 4
+            tables = {}
+            spent, nf = 0.0, 0
+            for k, r in enumerate(roles[:max(sample) + 1]):
+                refs = [Picture(*dpb[sl]) for sl in ([] if r.kind == "I" else r.ref_slots)]
+                prev = tables.get(r.temporal_from) if r.kind == "P" else None
+                t1 = time.perf_counter()
+                recs, lv, pic = impl.encode_frame(kind[r.kind], r.quality,
+                                                  Picture(*frames[r.index]), refs, dpb[r.slot],
+                                                  prev=prev, target_ids=r.ref_slots or None)
+                if k in sample:
+                    spent += time.perf_counter() - t1
+                    nf += 1
+                filt = impl.deblock_frame(recs, *pic)
+                _, *dec = impl.sao_encode_frame(frames[r.index], filt, impl.lam(r.quality))
+                dpb[r.slot] = tuple(dec)
+                if r.kind != "B":
+                    tables[r.index] = recs
+            base[threads] = nf / spent
+        out["cpu_baseline"] = {
+            "encode_slice_frames_per_s_1_thread": round(base[1], 3),
+            f"encode_slice_frames_per_s_{args.cpu_threads}_threads": round(base[args.cpu_threads], 3),
+            "sample": f"the first P anchor and the first reference B (1920x1080), reference build, "
+                      f"row workers on 1 and {args.cpu_threads} thread(s)",
+            "kind": "reference"}
+    return out
+
+
 def calibrate_serving(ctx, args, shapes=None):
     """Sustained MB/s of the dispatcher on a backlog of mix frames (every
     stream of a 64-stream mix queued for several frames).  With `shapes`,
@@ -1106,8 +1266,9 @@ def main():
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--quality", type=int, default=8)
     ap.add_argument("--workload", choices=["plain", "batched", "pair", "gop", "serve", "recon",
-                                           "loop", "all", "traffic"],
+                                           "loop", "encode", "all", "traffic"],
                     default="all")
+    ap.add_argument("--encode-frames", type=int, default=13)
     ap.add_argument("--loop-frames", type=int, default=24)
     ap.add_argument("--gop-frames", type=int, default=33)
     ap.add_argument("--serve-streams", type=int, default=200)
@@ -1162,6 +1323,8 @@ def main():
         res["reconstruct"] = run_recon(args)
     if args.workload in ("all", "loop"):
         res["closed_loop"] = run_loop(args)
+    if args.workload in ("all", "encode"):
+        res["encode"] = run_encode(args)
     if args.workload in ("all", "serve"):
         res["serve"] = run_serve(args)
     print(json.dumps(res), flush=True)
