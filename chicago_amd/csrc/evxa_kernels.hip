@@ -911,7 +911,7 @@ __device__ __forceinline__ uint32_t mv_word(int mvx, int mvy)
 // slots' results through LDS.  The ordered fold then runs on wave-uniform
 // values, identically in every wave.
 template <int NW>
-__device__ SearchOut inter_search(const DevPic &ref, int pw, int ph, bool narrow, const Blk &s,
+__device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int ph, bool narrow, const Blk &s,
                                   const Geo &geo, const Blk &pre, WinPos w, int x, int y,
                                   int thresh, float lambda, int pmx, int pmy, SubpelLds &sp,
                                   uint64_t *early = nullptr)
@@ -1627,7 +1627,7 @@ struct Sel {
 // The two predictions of a B MB (motion.cpp:729-776) into sh.pred and
 // sh.pred_b; returns the SAD of their average against sh.src over Y, U and V
 // (motion.cpp:779-813).
-__device__ int b_predictions(const DevJob &job, const SearchOut &f, const SearchOut &b, MbLds &sh)
+__device__ __forceinline__ int b_predictions(const DevJob &job, const SearchOut &f, const SearchOut &b, MbLds &sh)
 {
     const int lane = lane_id();
     predict_mb(job.refs[0], f, sh.pred, sh.tmp);
@@ -4346,7 +4346,7 @@ __device__ void trellis_mb(int qp, bool intra, int imode, float lambda, const in
 // from sh.mb.src and the prediction in sh.mb.pred into co/lv/rec of `slot`.
 // Copies are not transformed (encode.cpp:742-744): zero coefficients and
 // levels, the prediction as the reconstruction.
-__device__ void code_mb(const DevJob &job, EncDesc &d, EncLds &sh, int slot)
+__device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, EncLds &sh, int slot)
 {
     const int lane = lane_id();
     int16_t *co = sh.co[slot], *lv = sh.lv[slot], *rec = sh.rec[slot];
@@ -4449,7 +4449,7 @@ __device__ uint32_t block_bits(const EncDesc &d, const int16_t *blk)
 // Gathers the directional modes' neighbour samples of the MB at (x, y)
 // from the window (intra_pred.cpp:12-52) and the DC values
 // (intra_pred.cpp:58-75: truncating division, 128 without neighbours).
-__device__ void gather_edges(const WinView &wv, int x, int y, IntraEdges &e)
+__device__ __forceinline__ void gather_edges(const WinView &wv, int x, int y, IntraEdges &e)
 {
     const int lane = lane_id();
     const bool has_top = y > 0, has_left = x > 0;   // the chroma block's neighbours exist alike
@@ -4487,7 +4487,7 @@ __device__ void gather_edges(const WinView &wv, int x, int y, IntraEdges &e)
 }
 
 // generate_intra_prediction (intra_pred.cpp:58-215) of `mode` into `out`.
-__device__ void intra_predict(int mode, const IntraEdges &e, int16_t *out)
+__device__ __forceinline__ void intra_predict(int mode, const IntraEdges &e, int16_t *out)
 {
     const int lane = lane_id();
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
@@ -4583,7 +4583,7 @@ __device__ __forceinline__ SearchOut motion_of(int x, int y, int mvx, int mvy, i
 // for a plain intra block, an intra block copy from the window, or its
 // reference(s) for inter blocks (the first reference with the target id;
 // bidirectional blocks average both).
-__device__ void predict_desc(const DevJob &job, const EncDesc &d, int x, int y, const WinView &wv,
+__device__ __forceinline__ void predict_desc(const DevJob &job, const EncDesc &d, int x, int y, const WinView &wv,
                              EncLds &sh)
 {
     const int lane = lane_id();
@@ -4620,11 +4620,13 @@ __device__ void predict_desc(const DevJob &job, const EncDesc &d, int x, int y, 
 // compute_rdo_cost (encode.cpp:128-166): the candidate coded and
 // reconstructed into `slot`, luma SSD of the reconstruction (an int32 that
 // wraps) plus lambda times its estimated bits, fp32 multiply then add.
-__device__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y, const WinView &wv,
-                          EncLds &sh, int slot)
+// With `rdo` false only the coding and reconstruction (cost 0).
+__device__ __forceinline__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y, const WinView &wv,
+                          EncLds &sh, int slot, bool rdo)
 {
     predict_desc(job, d, x, y, wv, sh);
     code_mb(job, d, sh, slot);
+    if (!rdo) return 0.0f;
     const int lane = lane_id();
     uint32_t ssd = 0;
 #pragma unroll
@@ -4640,7 +4642,7 @@ __device__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y, const Win
 // Stores the MB's reconstruction into the picture (and the picture's apron
 // next to it for MBs on the border) with device-scope stores, and grows
 // the picture's range record by it.
-__device__ void store_reconstruction(const EncArgs &a, int bx, int by, const int16_t *rec)
+__device__ __forceinline__ void store_reconstruction(const EncArgs &a, int bx, int by, const int16_t *rec)
 {
     const DevJob &job = a.job;
     const DevPic &pic = a.pic;
@@ -4691,6 +4693,131 @@ __device__ void store_reconstruction(const EncArgs &a, int bx, int by, const int
                          (int32_t)((uint32_t)(uint16_t)v0 | ((uint32_t)(uint16_t)v1 << 16)));
         }
     }
+}
+
+// calculate_intra_prediction (motion.cpp:485-565) over the window of the
+// picture being reconstructed, as intra_search, with the sub-pel step of
+// perform_intra_subpixel_motion_search (motion.cpp:408-448) evaluated the
+// way inter_search does its own: the h, v and d half-pel planes around the
+// integer best are built once from the window, every lane group takes two
+// of the 8 directions (half and quarter of each), and the 16 candidates are
+// folded in their order in closed form.  The acceptance rule is the same as
+// the inter sub-pel step's (a candidate is taken iff its MAD is below the
+// running best's in copy state; otherwise iff its MAD is below T, or its
+// SAD is below the running best's and below 8192); only directions whose
+// target is stable and inside the picture take part.
+__device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw, int ph,
+                                                      const Blk &s, const Geo &geo, int x, int y,
+                                                      int thresh, float lambda, SubpelLds &sp)
+{
+    const int lane = lane_id();
+    const int g = lane >> 4;
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) a += abs(lo16(s.y[k])) + abs(hi16(s.y[k]));
+    const int sad0 = lane_value(row_sum(a), 0);       // compute_block_sad(src), analysis.h:134-145
+    IntraState st{x, y, sad0, kInt32Max, kInt32Max, __float_as_uint(__int2float_rn(sad0)), 0};
+    intra_scan(wv, pw, ph, s, geo, x, y, thresh, lambda, -kSearchRadius, -2 * kSearchRadius,
+               kSearchRadius, st);
+    if (st.mad >= thresh)
+        for (int step = kSearchRadius / 2; step > 0; step >>= 1)
+            if (intra_scan(wv, pw, ph, s, geo, x, y, thresh, lambda, -step, -step, step, st)) break;
+
+    int sp_en = 0, sp_amt = 0, sp_idx = 0, n_ax = 0, n_dg = 0;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int tx = st.x + slot_i(k), ty = st.y + slot_j(k);
+        if (!intra_unstable(tx, ty, x, y) && tx >= 0 && tx <= pw - 16 && ty >= 0 && ty <= ph - 16)
+            mask |= 1u << k;
+    }
+    if (mask) {
+        const int bx = st.x, by = st.y;
+        build_hv<16, 1>(wv.y + (by - wv.ly) * wv.py + (bx - wv.lx), wv.py, sp.y);
+        const int co = ((by >> 1) - wv.cy) * wv.pc + ((bx >> 1) - wv.cx);
+        build_hv<8, 1>(wv.u + co, wv.pc, sp.u);
+        build_hv<8, 1>(wv.v + co, wv.pc, sp.v);
+        Blk full;
+        load_cand(wv, geo, bx, by, full);
+        wave_sync();
+        build_d<16, 1>(sp.y);
+        build_d<8, 1>(sp.u);
+        build_d<8, 1>(sp.v);
+        wave_sync();
+        // Half and quarter SAD/MAD of candidate j = 2 * direction + quarter
+        // in lane j (j < 16): group g evaluates directions g and g + 4.
+        const int ia = slot_i(g), ja = slot_j(g), ib = slot_i(g + 4), jb = slot_j(g + 4);
+        int m[8];
+#pragma unroll
+        for (int slot = 0; slot < 2; slot++) {
+            const int di = slot ? ib : ia, dj = slot ? jb : ja;
+            int sh, mh, sq, mq;
+            subpel_metrics(false, s, full, plane_view<16>(sp.y, di, dj), plane_view<8>(sp.u, di, dj),
+                           plane_view<8>(sp.v, di, dj), geo, sh, mh, sq, mq);
+            m[slot * 4 + 0] = row_sum(sh);
+            m[slot * 4 + 1] = row_max(mh);
+            m[slot * 4 + 2] = row_sum(sq);
+            m[slot * 4 + 3] = row_max(mq);
+        }
+        const int j = lane & 15, jd = j >> 1, jq = j & 1;
+        const int from = ((jd & 3) << 4) << 2;
+        int pulled[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) pulled[i] = __builtin_amdgcn_ds_bpermute(from, m[i]);
+        const int base = (jd >= 4 ? 4 : 0) + (jq ? 2 : 0);
+        const int jsad = base == 0 ? pulled[0] : base == 2 ? pulled[2] : base == 4 ? pulled[4] : pulled[6];
+        const int jmad = base == 0 ? pulled[1] : base == 2 ? pulled[3] : base == 4 ? pulled[5] : pulled[7];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool valid = ((mask >> k) & 1u) != 0;
+            const bool diag = slot_i(k) != 0 && slot_j(k) != 0;
+            n_dg += (valid && diag) ? 1 : 0;
+            n_ax += (valid && !diag) ? 1 : 0;
+        }
+        const bool vj = lane < 16 && ((mask >> (lane >> 1)) & 1u) != 0;
+        auto row_min = [](int v) {
+            v = min(v, dpp<0xB1>(v));
+            v = min(v, dpp<0x4E>(v));
+            v = min(v, dpp<0x141>(v));
+            v = min(v, dpp<0x140>(v));
+            return lane_value(v, 0);
+        };
+        int win = -1;
+        if (st.mad < thresh) {
+            const int m2 = row_min(vj ? jmad : kInt32Max);
+            if (m2 < st.mad) win = (int)__builtin_ctzll(__ballot(vj && jmad == m2));
+        } else {
+            const uint64_t cm = __ballot(vj && jmad < thresh);
+            if (cm) {
+                const int js = (int)__builtin_ctzll(cm);
+                const bool after = vj && lane >= js;
+                const int m2 = row_min(after ? jmad : kInt32Max);
+                win = (int)__builtin_ctzll(__ballot(after && jmad == m2));
+            } else {
+                const bool ok = vj && jsad < kSadThreshold;
+                const int m1 = row_min(ok ? jsad : kInt32Max);
+                if (m1 < st.sad) win = (int)__builtin_ctzll(__ballot(ok && jsad == m1));
+            }
+        }
+        if (win >= 0) {
+            sp_en = 1;
+            sp_amt = win & 1;
+            sp_idx = win >> 1;
+            st.sad = lane_value(jsad, win);
+            st.mad = lane_value(jmad, win);
+        }
+        wave_sync();            // the planes are free again
+    }
+    SearchOut o;
+    o.best_x = st.x; o.best_y = st.y;
+    o.sad = st.sad; o.mad = st.mad; o.ssd = st.ssd;
+    o.cost = __uint_as_float(st.cost);
+    o.sp_en = sp_en; o.sp_amt = sp_amt; o.sp_idx = sp_idx;
+    o.btype = EVXA_BLOCK_INTRA;
+    if (st.x != x || st.y != y || sp_en) o.btype |= EVXA_BLOCK_MOTION;
+    if (st.mad < thresh) o.btype |= EVXA_BLOCK_COPY;
+    o.n_int = st.n_int; o.n_ax = n_ax; o.n_dg = n_dg;
+    return o;
 }
 
 // One MB of the in-loop encode (see the section comment).  Inlined into the
@@ -4780,40 +4907,41 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     // 362, 380).
     const int thresh = job.thresh;
     const float lambda = job.lambda;
-    const SearchOut ri = intra_search(wv, W, H, s, geo, x, y, thresh, lambda, sh.mb);
+    const SearchOut ri = intra_search_win(wv, W, H, s, geo, x, y, thresh, lambda, sh.sp);
     EncDesc best = desc_of(ri, x, y, 0, 0);
     int best_sad = ri.sad;
     int n_int = ri.n_int, n_ax = ri.n_ax, n_dg = ri.n_dg;
     const int tgt0 = job.target_ids & 0xFF, tgt1 = (job.target_ids >> 8) & 0xFF;
 
+    // The inter searches, one call site for both references (P: forward
+    // only; B: forward then backward), so the search inlines.
+    SearchOut r[2];
+    WinPos wk = w0;
+    bool narrowk = narrow0;
+    for (int k = 0; k < job.n_refs; k++) {
+        if (k > 0) {
+            narrowk = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[k].wide)) != 0;
+            load_cand(job.refs[k], geo, x, y, pre);
+            wk = stage_window<1>(job.refs[k], x, y, kReachMax, sh.sp);
+            window_ready();
+        }
+        r[k] = inter_search<1>(job.refs[k], W, H, narrowk, s, geo, pre, wk, x, y, thresh, lambda,
+                               pmx, pmy, sh.sp);
+        n_int += r[k].n_int; n_ax += r[k].n_ax; n_dg += r[k].n_dg;
+    }
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:183-210: the copy state decides first, then the SAD.
-        const SearchOut r = inter_search<1>(job.refs[0], W, H, narrow0, s, geo, pre, w0, x, y,
-                                            thresh, lambda, pmx, pmy, sh.sp);
-        n_int += r.n_int; n_ax += r.n_ax; n_dg += r.n_dg;
-        const bool inter_copy = (r.btype & EVXA_BLOCK_COPY) != 0;
+        const bool inter_copy = (r[0].btype & EVXA_BLOCK_COPY) != 0;
         const bool best_copy = (best.btype & EVXA_BLOCK_COPY) != 0;
         if (inter_copy != best_copy) {
-            if (inter_copy) { best = desc_of(r, x, y, tgt0, 0); best_sad = r.sad; }
-        } else if (r.sad < best_sad) {
-            best = desc_of(r, x, y, tgt0, 0);
-            best_sad = r.sad;
+            if (inter_copy) { best = desc_of(r[0], x, y, tgt0, 0); best_sad = r[0].sad; }
+        } else if (r[0].sad < best_sad) {
+            best = desc_of(r[0], x, y, tgt0, 0);
+            best_sad = r[0].sad;
         }
     } else if (job.type == EVXA_FRAME_B) {
         // encode.cpp:386-458: forward, backward, then their average.
-        const SearchOut f = inter_search<1>(job.refs[0], W, H, narrow0, s, geo, pre, w0, x, y,
-                                            thresh, lambda, pmx, pmy, sh.sp);
-        const bool narrow1 =
-            __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[1].wide)) != 0;
-        Blk pre1;
-        load_cand(job.refs[1], geo, x, y, pre1);
-        const WinPos w1 = stage_window<1>(job.refs[1], x, y, kReachMax, sh.sp);
-        window_ready();
-        const SearchOut b = inter_search<1>(job.refs[1], W, H, narrow1, s, geo, pre1, w1, x, y,
-                                            thresh, lambda, pmx, pmy, sh.sp);
-        n_int += f.n_int + b.n_int;
-        n_ax += f.n_ax + b.n_ax;
-        n_dg += f.n_dg + b.n_dg;
+        const SearchOut &f = r[0], &b = r[1];
         if (f.sad < best_sad) { best = desc_of(f, x, y, tgt0, 0); best_sad = f.sad; }
         if (b.sad < best_sad) { best = desc_of(b, x, y, tgt1, 1); best_sad = b.sad; }
         const int bsad = b_predictions(job, f, b, sh.mb);
@@ -4856,24 +4984,22 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     // unless the winner is one of the intra-default kind, then the three
     // mode candidates; a strictly lower cost wins.  Otherwise the winner is
     // coded once.
+    // Candidate c: 0 the winner, 1 the plain intra block, 2..4 the modes;
+    // one call site, so the coding stages inline.
     int slot = 0;
     float best_cost = 0.0f;
-    if (job.type == EVXA_FRAME_P) {
-        best_cost = rdo_cost(job, best, x, y, wv, sh, 0);
-        if (best.btype != EVXA_BLOCK_INTRA) {
-            EncDesc c = desc_clear();
-            const float cc = rdo_cost(job, c, x, y, wv, sh, slot ^ 1);
-            if (cc < best_cost) { best = c; best_cost = cc; slot ^= 1; }
+    const bool rdo = job.type == EVXA_FRAME_P;
+    const int n_cand = rdo ? 5 : 1;
+    for (int c = 0; c < n_cand; c++) {
+        if (c == 1 && best.btype == EVXA_BLOCK_INTRA) continue;
+        EncDesc d = best;
+        if (c > 0) {
+            d = desc_clear();
+            if (c > 1) d.imode = (cands >> (8 * (c - 2))) & 0xFF;
         }
-        for (int k = 0; k < 3; k++) {
-            EncDesc c = desc_clear();
-            c.imode = (cands >> (8 * k)) & 0xFF;
-            const float cc = rdo_cost(job, c, x, y, wv, sh, slot ^ 1);
-            if (cc < best_cost) { best = c; best_cost = cc; slot ^= 1; }
-        }
-    } else {
-        predict_desc(job, best, x, y, wv, sh);
-        code_mb(job, best, sh, 0);
+        const int to = c == 0 ? 0 : slot ^ 1;
+        const float cost = rdo_cost(job, d, x, y, wv, sh, to, rdo);
+        if (c == 0 || cost < best_cost) { best = d; best_cost = cost; slot = to; }
     }
 
     // Outputs: the reconstruction into the picture, coefficients, levels,
