@@ -559,6 +559,24 @@ __device__ __forceinline__ void window_ready()
     __syncthreads();
 }
 
+// The LDS ordering a search needs between its steps: the workgroup's, or,
+// for a wave that searches alone inside a larger workgroup (SOLO), its own
+// (a wave's LDS accesses complete in order, and its LDS copies have landed
+// once its vmcnt is zero).
+template <bool SOLO>
+__device__ __forceinline__ void group_sync()
+{
+    if constexpr (SOLO) wave_sync();
+    else __syncthreads();
+}
+
+template <bool SOLO>
+__device__ __forceinline__ void window_ready()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
+    group_sync<SOLO>();
+}
+
 // Four consecutive samples from sample index i of an LDS array with a
 // dword-aligned base: three aligned dword reads and two funnel shifts, so
 // no LDS access straddles a dword whatever the parity of i.
@@ -910,7 +928,7 @@ __device__ __forceinline__ uint32_t mv_word(int mvx, int mvy)
 // waves, group g of wave h takes slot 4h + g and the waves exchange their
 // slots' results through LDS.  The ordered fold then runs on wave-uniform
 // values, identically in every wave.
-template <int NW>
+template <int NW, bool SOLO = false>
 __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int ph, bool narrow, const Blk &s,
                                   const Geo &geo, const Blk &pre, WinPos w, int x, int y,
                                   int thresh, float lambda, int pmx, int pmy, SubpelLds &sp,
@@ -980,7 +998,7 @@ __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int
             bool in_win = win_holds_level(w, base_x, base_y, step, max_x, max_y);
             if (!in_win && step <= 8) {
                 w = stage_window<NW>(ref, base_x, base_y, 2 * step - 1, sp);
-                window_ready();
+                window_ready<SOLO>();
                 in_win = true;
             }
             uint32_t mask = 0;
@@ -1037,7 +1055,7 @@ __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int
                     xc.sad[parity][sa] = sad_a;
                     xc.mad[parity][sa] = mad_a;
                 }
-                __syncthreads();
+                group_sync<SOLO>();
                 const int k = lane & 7;
                 const uint32_t kc = xc.cost[parity][k];
                 const int ks = xc.sad[parity][k], km = xc.mad[parity][k];
@@ -1067,7 +1085,7 @@ __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int
             const int bx = b.x, by = b.y;
             if (!win_holds_subpel(w, bx, by)) {
                 w = stage_window<NW>(ref, bx, by, 0, sp);
-                window_ready();
+                window_ready<SOLO>();
             }
             const int wlx = bx - w.x0, wly = by - w.y0;
             const int wcx = (bx >> 1) - w.cx0, wcy = (by >> 1) - w.cy0;
@@ -1076,11 +1094,11 @@ __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int
             build_hv<8, NW>(sp.win.v + wcy * kWinCS + wcx, kWinCS, sp.v);
             Blk full;
             load_cand_win(sp, w, geo, bx, by, full);
-            __syncthreads();
+            group_sync<SOLO>();
             build_d<16, NW>(sp.y);
             build_d<8, NW>(sp.u);
             build_d<8, NW>(sp.v);
-            __syncthreads();
+            group_sync<SOLO>();
             // Half and quarter SAD/MAD of candidate j = 2 * direction +
             // (quarter) in lane j (j < 16).
             int jsad, jmad;
@@ -1117,7 +1135,7 @@ __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int
                     sp.xch.sub[sa][0] = sh; sp.xch.sub[sa][1] = mh;
                     sp.xch.sub[sa][2] = sq; sp.xch.sub[sa][3] = mq;
                 }
-                __syncthreads();
+                group_sync<SOLO>();
                 jsad = sp.xch.sub[jd][2 * jq];
                 jmad = sp.xch.sub[jd][2 * jq + 1];
             }
@@ -1174,7 +1192,7 @@ __device__ __forceinline__ SearchOut inter_search(const DevPic &ref, int pw, int
                 b.mad = lane_value(jmad, win);
             }
         }
-        __syncthreads();
+        group_sync<SOLO>();
     }
 
     SearchOut o;
@@ -4147,14 +4165,29 @@ struct EncDesc {
     int imode, qp, var;
 };
 
+// Waves of the encode's workgroup and the RDO's candidates.
+constexpr int kEncWaves = 4;
+constexpr int kEncCands = 5;          // the SAD winner, the plain intra block, three modes
+
+// What the waves of a MB's workgroup exchange.
+struct EncXchg {
+    SearchOut intra, inter[2];
+    int32_t dir_mode, dir_sad, cands, pmx, pmy;
+    float cost[kEncCands];
+    int32_t qp[kEncCands], var[kEncCands];
+};
+
 struct EncLds {
-    IntraWin iw;
-    SubpelLds sp;
-    MbLds mb;
+    IntraWin iw;                          // the picture being reconstructed, around the MB
+    SubpelLds sp[2];                      // the inter searches (waves 1 and 2)
+    HalfPlanes<16> hy;                    // the intra search's half-pel planes (wave 0)
+    HalfPlanes<8> hu, hv;
+    MbLds mb[kEncWaves];                  // per wave: source, predictions, residual, scratch
     IntraEdges edges;
-    int16_t co[2][EVXA_MB_SAMPLES];      // coefficients of a candidate and of the best
-    int16_t lv[2][EVXA_MB_SAMPLES];      // levels after the trellis
-    int16_t rec[2][EVXA_MB_SAMPLES];     // reconstruction
+    EncXchg x;
+    int16_t co[kEncCands][EVXA_MB_SAMPLES];   // per candidate: coefficients,
+    int16_t lv[kEncCands][EVXA_MB_SAMPLES];   // levels after the trellis,
+    int16_t rec[kEncCands][EVXA_MB_SAMPLES];  // reconstruction
 };
 
 struct EncArgs {
@@ -4346,22 +4379,22 @@ __device__ void trellis_mb(int qp, bool intra, int imode, float lambda, const in
 // from sh.mb.src and the prediction in sh.mb.pred into co/lv/rec of `slot`.
 // Copies are not transformed (encode.cpp:742-744): zero coefficients and
 // levels, the prediction as the reconstruction.
-__device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, EncLds &sh, int slot)
+__device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, MbLds &mb, int16_t *co,
+                                        int16_t *lv, int16_t *rec)
 {
     const int lane = lane_id();
-    int16_t *co = sh.co[slot], *lv = sh.lv[slot], *rec = sh.rec[slot];
     if (d.btype & EVXA_BLOCK_COPY) {
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
             co[i] = 0;
             lv[i] = 0;
-            rec[i] = sh.mb.pred[i];
+            rec[i] = mb.pred[i];
         }
         wave_sync();
         return;
     }
-    fdct_mb(sh.mb.src, sh.mb.pred, sh.mb.res, sh.mb.tmp);
+    fdct_mb(mb.src, mb.pred, mb.res, mb.tmp);
     int qp, var2;
-    mb_qp(sh.mb.res, job.quality, qp, var2);
+    mb_qp(mb.res, job.quality, qp, var2);
     d.qp = qp;
     d.var = var2;
     const bool intra = intra_quant(d.btype);
@@ -4371,19 +4404,19 @@ __device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, EncLds &s
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         bool chroma;
         const int pos = mb_pos(i, chroma);
-        const int sv = sh.mb.res[i];
+        const int sv = mb.res[i];
         co[i] = (int16_t)sv;
         lv[i] = (int16_t)quant(sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra, chroma);
     }
     wave_sync();
     trellis_mb(qp, intra, d.imode, job.lambda, co, lv);
-    // Inverse quantisation into sh.mb.res, then the inverse DCT: vertical
+    // Inverse quantisation into mb.res, then the inverse DCT: vertical
     // pass (lane = block column) into tmp, horizontal pass plus the
     // prediction (lane = block row) into rec, stored as int16.
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         bool chroma;
         const int pos = mb_pos(i, chroma);
-        sh.mb.res[i] = (int16_t)dequant(lv[i], pos, chroma ? cqp : qp, intra, chroma);
+        mb.res[i] = (int16_t)dequant(lv[i], pos, chroma ? cqp : qp, intra, chroma);
     }
     wave_sync();
     if (lane < 48) {
@@ -4392,9 +4425,9 @@ __device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, EncLds &s
         const int c = origin + (lane & 7);
         int v[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = sh.mb.res[c + k * pitch];
+        for (int k = 0; k < 8; k++) v[k] = mb.res[c + k * pitch];
 #pragma unroll
-        for (int i = 0; i < 8; i++) sh.mb.tmp[c + i * pitch] = (int16_t)idct_sample(v, i);
+        for (int i = 0; i < 8; i++) mb.tmp[c + i * pitch] = (int16_t)idct_sample(v, i);
     }
     wave_sync();
     if (lane < 48) {
@@ -4403,9 +4436,9 @@ __device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, EncLds &s
         const int r = origin + (lane & 7) * pitch;
         int v[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = sh.mb.tmp[r + k];
+        for (int k = 0; k < 8; k++) v[k] = mb.tmp[r + k];
 #pragma unroll
-        for (int i = 0; i < 8; i++) rec[r + i] = (int16_t)(sh.mb.pred[r + i] + idct_sample(v, i));
+        for (int i = 0; i < 8; i++) rec[r + i] = (int16_t)(mb.pred[r + i] + idct_sample(v, i));
     }
     wave_sync();
 }
@@ -4583,22 +4616,22 @@ __device__ __forceinline__ SearchOut motion_of(int x, int y, int mvx, int mvy, i
 // for a plain intra block, an intra block copy from the window, or its
 // reference(s) for inter blocks (the first reference with the target id;
 // bidirectional blocks average both).
-__device__ __forceinline__ void predict_desc(const DevJob &job, const EncDesc &d, int x, int y, const WinView &wv,
-                             EncLds &sh)
+__device__ __forceinline__ void predict_desc(const DevJob &job, const EncDesc &d, int x, int y,
+                                             const WinView &wv, const IntraEdges &edges, MbLds &mb)
 {
     const int lane = lane_id();
     if (d.btype == EVXA_BLOCK_INTRA) {
         if (d.imode != kIntraNone) {
-            intra_predict(d.imode, sh.edges, sh.mb.pred);
+            intra_predict(d.imode, edges, mb.pred);
         } else {
-            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) sh.mb.pred[i] = 0;
+            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) mb.pred[i] = 0;
             wave_sync();
         }
         return;
     }
     const SearchOut r = motion_of(x, y, d.mvx, d.mvy, d.sp, d.sa, d.si);
     if (d.btype & EVXA_BLOCK_INTRA) {
-        predict_mb(wv, r, sh.mb.pred, sh.mb.tmp);
+        predict_mb(wv, r, mb.pred, mb.tmp);
         return;
     }
     auto ref_of = [&](int target) {
@@ -4607,12 +4640,12 @@ __device__ __forceinline__ void predict_desc(const DevJob &job, const EncDesc &d
             if ((int)((job.target_ids >> (8 * i)) & 0xFF) == target) k = i;
         return k;
     };
-    predict_mb(job.refs[ref_of(d.tgt)], r, sh.mb.pred, sh.mb.tmp);
+    predict_mb(job.refs[ref_of(d.tgt)], r, mb.pred, mb.tmp);
     if (job.type == EVXA_FRAME_B && d.pmode == 2) {
         const SearchOut b = motion_of(x, y, d.mvx_b, d.mvy_b, d.sp_b, d.sa_b, d.si_b);
-        predict_mb(job.refs[ref_of(d.tgt_b)], b, sh.mb.pred_b, sh.mb.tmp);
+        predict_mb(job.refs[ref_of(d.tgt_b)], b, mb.pred_b, mb.tmp);
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64)
-            sh.mb.pred[i] = (int16_t)((sh.mb.pred[i] + sh.mb.pred_b[i] + 1) >> 1);
+            mb.pred[i] = (int16_t)((mb.pred[i] + mb.pred_b[i] + 1) >> 1);
         wave_sync();
     }
 }
@@ -4621,21 +4654,22 @@ __device__ __forceinline__ void predict_desc(const DevJob &job, const EncDesc &d
 // reconstructed into `slot`, luma SSD of the reconstruction (an int32 that
 // wraps) plus lambda times its estimated bits, fp32 multiply then add.
 // With `rdo` false only the coding and reconstruction (cost 0).
-__device__ __forceinline__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y, const WinView &wv,
-                          EncLds &sh, int slot, bool rdo)
+__device__ __forceinline__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y,
+                                          const WinView &wv, const IntraEdges &edges, MbLds &mb,
+                                          int16_t *co, int16_t *lv, int16_t *rec, bool rdo)
 {
-    predict_desc(job, d, x, y, wv, sh);
-    code_mb(job, d, sh, slot);
+    predict_desc(job, d, x, y, wv, edges, mb);
+    code_mb(job, d, mb, co, lv, rec);
     if (!rdo) return 0.0f;
     const int lane = lane_id();
     uint32_t ssd = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const int t = sh.mb.src[4 * lane + k] - sh.rec[slot][4 * lane + k];
+        const int t = mb.src[4 * lane + k] - rec[4 * lane + k];
         ssd += (uint32_t)t * (uint32_t)t;
     }
     ssd = (uint32_t)wave_sum((int)ssd);
-    const uint32_t rate = block_bits(d, sh.rec[slot]);
+    const uint32_t rate = block_bits(d, rec);
     return __fadd_rn(__int2float_rn((int)ssd), __fmul_rn(job.lambda, __uint2float_rn(rate)));
 }
 
@@ -4708,7 +4742,8 @@ __device__ __forceinline__ void store_reconstruction(const EncArgs &a, int bx, i
 // target is stable and inside the picture take part.
 __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw, int ph,
                                                       const Blk &s, const Geo &geo, int x, int y,
-                                                      int thresh, float lambda, SubpelLds &sp)
+                                                      int thresh, float lambda, HalfPlanes<16> &hy,
+                                                      HalfPlanes<8> &hu, HalfPlanes<8> &hv)
 {
     const int lane = lane_id();
     const int g = lane >> 4;
@@ -4733,16 +4768,16 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
     }
     if (mask) {
         const int bx = st.x, by = st.y;
-        build_hv<16, 1>(wv.y + (by - wv.ly) * wv.py + (bx - wv.lx), wv.py, sp.y);
+        build_hv<16, 1>(wv.y + (by - wv.ly) * wv.py + (bx - wv.lx), wv.py, hy);
         const int co = ((by >> 1) - wv.cy) * wv.pc + ((bx >> 1) - wv.cx);
-        build_hv<8, 1>(wv.u + co, wv.pc, sp.u);
-        build_hv<8, 1>(wv.v + co, wv.pc, sp.v);
+        build_hv<8, 1>(wv.u + co, wv.pc, hu);
+        build_hv<8, 1>(wv.v + co, wv.pc, hv);
         Blk full;
         load_cand(wv, geo, bx, by, full);
         wave_sync();
-        build_d<16, 1>(sp.y);
-        build_d<8, 1>(sp.u);
-        build_d<8, 1>(sp.v);
+        build_d<16, 1>(hy);
+        build_d<8, 1>(hu);
+        build_d<8, 1>(hv);
         wave_sync();
         // Half and quarter SAD/MAD of candidate j = 2 * direction + quarter
         // in lane j (j < 16): group g evaluates directions g and g + 4.
@@ -4752,8 +4787,8 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
         for (int slot = 0; slot < 2; slot++) {
             const int di = slot ? ib : ia, dj = slot ? jb : ja;
             int sh, mh, sq, mq;
-            subpel_metrics(false, s, full, plane_view<16>(sp.y, di, dj), plane_view<8>(sp.u, di, dj),
-                           plane_view<8>(sp.v, di, dj), geo, sh, mh, sq, mq);
+            subpel_metrics(false, s, full, plane_view<16>(hy, di, dj), plane_view<8>(hu, di, dj),
+                           plane_view<8>(hv, di, dj), geo, sh, mh, sq, mq);
             m[slot * 4 + 0] = row_sum(sh);
             m[slot * 4 + 1] = row_max(mh);
             m[slot * 4 + 2] = row_sum(sq);
@@ -4820,42 +4855,55 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
     return o;
 }
 
-// One MB of the in-loop encode (see the section comment).  Inlined into the
-// kernel, so that `sh` is known to be LDS and its accesses are LDS ones.
+// One MB of the in-loop encode (see the section comment), by the four waves
+// of the workgroup.  Wave 0 waits for the MB's dependencies; then all
+// waves stage the window of the picture being reconstructed, and the
+// independent parts run side by side: wave 0 the intra-frame block search,
+// wave 1 the (forward) inter search, wave 2 the backward one (B), wave 3 the
+// directional intra mode.  Every wave then makes the same selection from
+// what they left in LDS, codes the RDO candidates (P; wave w takes
+// candidates w and w + 4) or the winner (I, B; wave 0), and wave 0 stores
+// the MB and publishes its word.  Inlined into the kernel, so that `sh` is
+// known to be LDS.
 __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
 {
     const DevJob &job = a.job;
     const int lane = lane_id();
+    const int wv_id = __builtin_amdgcn_readfirstlane(wave_id());
     const int bx = mb % job.wb, by = mb / job.wb;
     const int x = bx * 16, y = by * 16;
     const int W = job.width, H = job.height;
     const Geo geo = lane_geometry();
+    MbLds &mine = sh.mb[wv_id];
 
-    // While the dependencies finish: the source block, and for P and B the
-    // first reference's block at (x, y) and its search window.
+    // While the dependencies finish: every wave loads the source block; the
+    // searching waves load their reference's block at (x, y) and stage its
+    // window.
     Blk s, pre;
     load_cand(job.src, geo, x, y, s);
-    store_src(s, geo, sh.mb.src);
-    WinPos w0{};
-    bool narrow0 = false;
-    if (job.type != 0) {
-        narrow0 = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[0].wide)) != 0;
-        load_cand(job.refs[0], geo, x, y, pre);
-        w0 = stage_window<1>(job.refs[0], x, y, kReachMax, sh.sp);
-    }
-    uint32_t p0 = 0, p1 = 0;
-    if (job.type == EVXA_FRAME_P && job.prev) {
-        const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + (size_t)mb * 16;
-        p0 = prv[0];
-        p1 = prv[1];
+    store_src(s, geo, mine.src);
+    const int my_ref = wv_id - 1;                  // wave 1: reference 0, wave 2: reference 1
+    const bool searcher = my_ref >= 0 && my_ref < job.n_refs;
+    WinPos wk{};
+    bool narrowk = false;
+    if (searcher) {
+        narrowk = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[my_ref].wide)) != 0;
+        load_cand(job.refs[my_ref], geo, x, y, pre);
+        wk = stage_window<1>(job.refs[my_ref], x, y, kReachMax, sh.sp[my_ref]);
     }
 
-    // Wait for MB (bx - 1, by) and MB (min(bx + lag, wb - 1), by - 1); the
-    // rest of what this MB reads was done before them.
-    const int dep = lane == 0 ? (bx > 0 ? mb - 1 : -1)
-                  : lane == 1 ? (by > 0 ? (by - 1) * job.wb + min(bx + kEncodeLag, job.wb - 1) : -1)
-                  : -1;
-    {
+    if (wv_id == 0) {
+        uint32_t p0 = 0, p1 = 0;
+        if (job.type == EVXA_FRAME_P && job.prev) {
+            const uint32_t *prv = reinterpret_cast<const uint32_t *>(job.prev) + (size_t)mb * 16;
+            p0 = prv[0];
+            p1 = prv[1];
+        }
+        // Wait for MB (bx - 1, by) and MB (min(bx + lag, wb - 1), by - 1);
+        // the rest of what this MB reads was done before them.
+        const int dep = lane == 0 ? (bx > 0 ? mb - 1 : -1)
+                      : lane == 1 ? (by > 0 ? (by - 1) * job.wb + min(bx + kEncodeLag, job.wb - 1) : -1)
+                      : -1;
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         int nap = 1;
         bool timed_out = false;
@@ -4868,83 +4916,114 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
             nap = min(nap * 2, kMaxNap);
         }
         if (timed_out && lane == 0) atomicOr(&a.counters[kCounterError], kErrWaitTimeout);
+        // The MV predictor's neighbours (left, top, top-right, top-left).
+        const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
+        const bool has_tl = has_t && has_l;
+        const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
+                       : lane == 1 ? (has_t ? mb - job.wb : -1)
+                       : lane == 2 ? (has_tr ? mb - job.wb + 1 : -1)
+                       : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1) : -1;
+        const uint64_t word = nidx >= 0 ? ld_relaxed64(job.mvw + nidx) : 0;
+        int pmx = 0, pmy = 0;
+        if (job.type != 0)
+            mv_predictor(job, bx, by, lane_u64(word, 0), lane_u64(word, 1), lane_u64(word, 2),
+                         lane_u64(word, 3), p0, p1, pmx, pmy);
+        if (lane == 0) {
+            sh.x.pmx = pmx;
+            sh.x.pmy = pmy;
+        }
     }
-    // The MV predictor's neighbours (left, top, top-right, top-left).
-    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
-    const bool has_tl = has_t && has_l;
-    const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
-                   : lane == 1 ? (has_t ? mb - job.wb : -1)
-                   : lane == 2 ? (has_tr ? mb - job.wb + 1 : -1)
-                   : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1) : -1;
-    const uint64_t word = nidx >= 0 ? ld_relaxed64(job.mvw + nidx) : 0;
-    int pmx = 0, pmy = 0;
-    if (job.type != 0) {
-        mv_predictor(job, bx, by, lane_u64(word, 0), lane_u64(word, 1), lane_u64(word, 2),
-                     lane_u64(word, 3), p0, p1, pmx, pmy);
-        pmx = __builtin_amdgcn_readfirstlane(pmx);
-        pmy = __builtin_amdgcn_readfirstlane(pmy);
-    }
+    __syncthreads();                              // the dependencies are done
 
     // The window of the picture being reconstructed, origin moved inside the
-    // apron at the borders (its reads are then the clamped ones).
+    // apron at the borders (its reads are then the clamped ones); the four
+    // waves share the copy.
     const int x0 = min(max(x - 34, -kApron), W + kApron - kIwLS);
     const int y0 = min(max(y - 50, -kApron), H + kApron - kIwLR);
     const int cx0 = min(max((x >> 1) - 20, -kApron), (W >> 1) + kApron - kIwCS);
     const int cy0 = min(max((y >> 1) - 27, -kApron), (H >> 1) + kApron - kIwCR);
-    dma_rows<kIwLS / 2, 1, kLoadDevice>(
+    dma_rows<kIwLS / 2, kEncWaves, kLoadDevice>(
         reinterpret_cast<const uint32_t *>(a.pic.y + y0 * a.pic.py + x0), a.pic.py / 2, kIwLR,
         reinterpret_cast<uint32_t *>(sh.iw.y));
-    dma_rows<kIwCS / 2, 1, kLoadDevice>(
+    dma_rows<kIwCS / 2, kEncWaves, kLoadDevice>(
         reinterpret_cast<const uint32_t *>(a.pic.u + cy0 * a.pic.pc + cx0), a.pic.pc / 2, kIwCR,
         reinterpret_cast<uint32_t *>(sh.iw.u));
-    dma_rows<kIwCS / 2, 1, kLoadDevice>(
+    dma_rows<kIwCS / 2, kEncWaves, kLoadDevice>(
         reinterpret_cast<const uint32_t *>(a.pic.v + cy0 * a.pic.pc + cx0), a.pic.pc / 2, kIwCR,
         reinterpret_cast<uint32_t *>(sh.iw.v));
     window_ready();
     const WinView wv{sh.iw.y, sh.iw.u, sh.iw.v, kIwLS, kIwCS, x0, y0, cx0, cy0};
-
-    // calculate_intra_prediction first, for every frame type (encode.cpp:175,
-    // 362, 380).
     const int thresh = job.thresh;
     const float lambda = job.lambda;
-    const SearchOut ri = intra_search_win(wv, W, H, s, geo, x, y, thresh, lambda, sh.sp);
+    const int pmx = sh.x.pmx, pmy = sh.x.pmy;
+
+    if (wv_id == 0) {
+        // calculate_intra_prediction, for every frame type (encode.cpp:175,
+        // 362, 380).
+        const SearchOut ri = intra_search_win(wv, W, H, s, geo, x, y, thresh, lambda, sh.hy,
+                                              sh.hu, sh.hv);
+        if (lane == 0) sh.x.intra = ri;
+    } else if (searcher) {
+        const SearchOut r = inter_search<1, true>(job.refs[my_ref], W, H, narrowk, s, geo, pre, wk,
+                                                  x, y, thresh, lambda, pmx, pmy, sh.sp[my_ref]);
+        if (lane == 0) sh.x.inter[my_ref] = r;
+        // A B frame's bidirectional candidate averages both predictions
+        // (motion.cpp:729-776): each search's wave forms its own.
+        if (job.type == EVXA_FRAME_B) predict_mb(job.refs[my_ref], r, mine.pred_b, mine.tmp);
+    } else if (wv_id == kEncWaves - 1) {
+        // The directional intra candidate (encode.cpp:254-272, 446-468;
+        // select_best_intra_mode, intra_pred.cpp:217-302: the first
+        // candidate with the smallest luma SAD).
+        gather_edges(wv, x, y, sh.edges);
+        const uint32_t cands = intra_candidates(sh.edges);
+        int dir_sad = kInt32Max, dir_mode = 0;
+        for (int c = 0; c < 3; c++) {
+            const int m = (cands >> (8 * c)) & 0xFF;
+            intra_predict(m, sh.edges, mine.pred);
+            const int sd = luma_sad(mine.src, mine.pred);
+            if (sd < dir_sad) { dir_sad = sd; dir_mode = m; }
+            wave_sync();
+        }
+        if (lane == 0) {
+            sh.x.dir_sad = dir_sad;
+            sh.x.dir_mode = dir_mode;
+            sh.x.cands = (int32_t)cands;
+        }
+    }
+    __syncthreads();
+
+    // The selection, made alike by every wave from what the waves left.
+    const SearchOut ri = sh.x.intra;
     EncDesc best = desc_of(ri, x, y, 0, 0);
     int best_sad = ri.sad;
     int n_int = ri.n_int, n_ax = ri.n_ax, n_dg = ri.n_dg;
     const int tgt0 = job.target_ids & 0xFF, tgt1 = (job.target_ids >> 8) & 0xFF;
-
-    // The inter searches, one call site for both references (P: forward
-    // only; B: forward then backward), so the search inlines.
-    SearchOut r[2];
-    WinPos wk = w0;
-    bool narrowk = narrow0;
-    for (int k = 0; k < job.n_refs; k++) {
-        if (k > 0) {
-            narrowk = __builtin_amdgcn_readfirstlane(narrow_pair(job.src.wide, job.refs[k].wide)) != 0;
-            load_cand(job.refs[k], geo, x, y, pre);
-            wk = stage_window<1>(job.refs[k], x, y, kReachMax, sh.sp);
-            window_ready();
-        }
-        r[k] = inter_search<1>(job.refs[k], W, H, narrowk, s, geo, pre, wk, x, y, thresh, lambda,
-                               pmx, pmy, sh.sp);
-        n_int += r[k].n_int; n_ax += r[k].n_ax; n_dg += r[k].n_dg;
-    }
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:183-210: the copy state decides first, then the SAD.
-        const bool inter_copy = (r[0].btype & EVXA_BLOCK_COPY) != 0;
+        const SearchOut r = sh.x.inter[0];
+        n_int += r.n_int; n_ax += r.n_ax; n_dg += r.n_dg;
+        const bool inter_copy = (r.btype & EVXA_BLOCK_COPY) != 0;
         const bool best_copy = (best.btype & EVXA_BLOCK_COPY) != 0;
         if (inter_copy != best_copy) {
-            if (inter_copy) { best = desc_of(r[0], x, y, tgt0, 0); best_sad = r[0].sad; }
-        } else if (r[0].sad < best_sad) {
-            best = desc_of(r[0], x, y, tgt0, 0);
-            best_sad = r[0].sad;
+            if (inter_copy) { best = desc_of(r, x, y, tgt0, 0); best_sad = r.sad; }
+        } else if (r.sad < best_sad) {
+            best = desc_of(r, x, y, tgt0, 0);
+            best_sad = r.sad;
         }
     } else if (job.type == EVXA_FRAME_B) {
         // encode.cpp:386-458: forward, backward, then their average.
-        const SearchOut &f = r[0], &b = r[1];
+        const SearchOut f = sh.x.inter[0], b = sh.x.inter[1];
+        n_int += f.n_int + b.n_int;
+        n_ax += f.n_ax + b.n_ax;
+        n_dg += f.n_dg + b.n_dg;
         if (f.sad < best_sad) { best = desc_of(f, x, y, tgt0, 0); best_sad = f.sad; }
         if (b.sad < best_sad) { best = desc_of(b, x, y, tgt1, 1); best_sad = b.sad; }
-        const int bsad = b_predictions(job, f, b, sh.mb);
+        int bsad = 0;
+        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+            const int av = (sh.mb[1].pred_b[i] + sh.mb[2].pred_b[i] + 1) >> 1;
+            bsad += abs(mine.src[i] - av);
+        }
+        bsad = wave_sum(bsad);
         if (bsad + __float2int_rz(__fmul_rn(lambda, 8.0f)) < best_sad) {
             best = desc_clear();
             best.btype = EVXA_BLOCK_MOTION;
@@ -4958,49 +5037,55 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
             best_sad = bsad;
         }
     }
-    if (best_sad == kInt32Max && lane == 0) atomicOr(&a.counters[kCounterError], kErrSearchFailed);
-
-    // The directional intra candidate (encode.cpp:254-272, 446-468;
-    // select_best_intra_mode, intra_pred.cpp:217-302: the first candidate
-    // with the smallest luma SAD).
-    gather_edges(wv, x, y, sh.edges);
-    const uint32_t cands = intra_candidates(sh.edges);
-    int dir_sad = kInt32Max, dir_mode = 0;
-    for (int c = 0; c < 3; c++) {
-        const int m = (cands >> (8 * c)) & 0xFF;
-        intra_predict(m, sh.edges, sh.mb.pred);
-        const int sd = luma_sad(sh.mb.src, sh.mb.pred);
-        if (sd < dir_sad) { dir_sad = sd; dir_mode = m; }
-        wave_sync();
-    }
-    if (dir_sad < best_sad) {
+    if (best_sad == kInt32Max && wv_id == 0 && lane == 0)
+        atomicOr(&a.counters[kCounterError], kErrSearchFailed);
+    const uint32_t cands = (uint32_t)sh.x.cands;
+    if (sh.x.dir_sad < best_sad) {
         best = desc_clear();
-        best.imode = dir_mode;
-        best_sad = dir_sad;
+        best.imode = sh.x.dir_mode;
+        best_sad = sh.x.dir_sad;
     }
 
     // RDO for P frames (encode.cpp:274-348, EVX_ENABLE_RDO_COPY_EVAL and
-    // EVX_ENABLE_RDO_INTRA_MODES): the SAD winner, the plain intra block
-    // unless the winner is one of the intra-default kind, then the three
-    // mode candidates; a strictly lower cost wins.  Otherwise the winner is
-    // coded once.
-    // Candidate c: 0 the winner, 1 the plain intra block, 2..4 the modes;
-    // one call site, so the coding stages inline.
-    int slot = 0;
-    float best_cost = 0.0f;
+    // EVX_ENABLE_RDO_INTRA_MODES): candidate 0 the SAD winner, 1 the plain
+    // intra block unless the winner is one of the intra-default kind, 2..4
+    // the three mode candidates; the first at the lowest cost wins (a later
+    // one only when strictly lower).  I and B frames code the winner once.
     const bool rdo = job.type == EVXA_FRAME_P;
-    const int n_cand = rdo ? 5 : 1;
-    for (int c = 0; c < n_cand; c++) {
-        if (c == 1 && best.btype == EVXA_BLOCK_INTRA) continue;
+    const int n_cand = rdo ? kEncCands : 1;
+    auto cand_desc = [&](int c) {
         EncDesc d = best;
         if (c > 0) {
             d = desc_clear();
             if (c > 1) d.imode = (cands >> (8 * (c - 2))) & 0xFF;
         }
-        const int to = c == 0 ? 0 : slot ^ 1;
-        const float cost = rdo_cost(job, d, x, y, wv, sh, to, rdo);
-        if (c == 0 || cost < best_cost) { best = d; best_cost = cost; slot = to; }
+        return d;
+    };
+    const bool has_plain = best.btype != EVXA_BLOCK_INTRA;
+    for (int c = wv_id; c < n_cand; c += kEncWaves) {
+        if (c == 1 && !has_plain) continue;
+        EncDesc d = cand_desc(c);
+        const float cost = rdo_cost(job, d, x, y, wv, sh.edges, mine, sh.co[c], sh.lv[c],
+                                    sh.rec[c], rdo);
+        if (lane == 0) {
+            sh.x.cost[c] = cost;
+            sh.x.qp[c] = d.qp;
+            sh.x.var[c] = d.var;
+        }
     }
+    __syncthreads();
+    if (wv_id != 0) return;
+
+    int slot = 0;
+    float best_cost = sh.x.cost[0];
+    for (int c = 1; c < n_cand; c++) {
+        if (c == 1 && !has_plain) continue;
+        if (sh.x.cost[c] < best_cost) { best_cost = sh.x.cost[c]; slot = c; }
+    }
+    best = cand_desc(slot);
+    best.qp = sh.x.qp[slot];
+    best.var = sh.x.var[slot];
+    if (!rdo) best_cost = 0.0f;
 
     // Outputs: the reconstruction into the picture, coefficients, levels,
     // the record; then the MB's word, once every store has completed.
@@ -5035,11 +5120,11 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         st_relaxed64(job.mvw + mb, pred_word((best.btype & EVXA_BLOCK_MOTION) != 0, w[1]));
 }
 
-// The persistent in-loop encode kernel: one wave per workgroup takes the
+// The persistent in-loop encode kernel: a workgroup of four waves takes the
 // next MB of the dependency order until none is left.  A MB waits only on
-// MBs earlier in the order, which running waves hold, so the launch cannot
-// deadlock whatever the number of resident workgroups.
-__global__ __launch_bounds__(64) void encode_kernel(const EncArgs a)
+// MBs earlier in the order, which running workgroups hold, so the launch
+// cannot deadlock whatever the number of resident workgroups.
+__global__ __launch_bounds__(64 * kEncWaves) void encode_kernel(const EncArgs a)
 {
     __shared__ EncLds sh;
     __shared__ int32_t s_word;
@@ -5339,7 +5424,7 @@ hipError_t launch_encode(const DevJob &job, const DevPic &pic, const uint32_t *o
     a.order = order;
     a.counters = counters;
     a.spread = spread;
-    hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(64 * kEncWaves), 0, st, a);
     return hipGetLastError();
 }
 
@@ -5350,7 +5435,8 @@ int32_t encode_resident_groups(int32_t device)
         n_cu <= 0)
         n_cu = 256;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel, 64, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel, 64 * kEncWaves, 0) !=
+            hipSuccess ||
         per_cu <= 0)
         per_cu = 1;
     return n_cu * per_cu;
