@@ -4165,39 +4165,6 @@ struct EncDesc {
     int imode, qp, var;
 };
 
-// Waves of the encode's workgroup and the RDO's candidates.
-constexpr int kEncWaves = 4;
-constexpr int kEncCands = 5;          // the SAD winner, the plain intra block, three modes
-
-// What the waves of a MB's workgroup exchange.
-struct EncXchg {
-    SearchOut intra, inter[2];
-    int32_t dir_mode, dir_sad, cands, pmx, pmy;
-    float cost[kEncCands];
-    int32_t qp[kEncCands], var[kEncCands];
-};
-
-struct EncLds {
-    IntraWin iw;                          // the picture being reconstructed, around the MB
-    SubpelLds sp[2];                      // the inter searches (waves 1 and 2)
-    HalfPlanes<16> hy;                    // the intra search's half-pel planes (wave 0)
-    HalfPlanes<8> hu, hv;
-    MbLds mb[kEncWaves];                  // per wave: source, predictions, residual, scratch
-    IntraEdges edges;
-    EncXchg x;
-    int16_t co[kEncCands][EVXA_MB_SAMPLES];   // per candidate: coefficients,
-    int16_t lv[kEncCands][EVXA_MB_SAMPLES];   // levels after the trellis,
-    int16_t rec[kEncCands][EVXA_MB_SAMPLES];  // reconstruction
-};
-
-struct EncArgs {
-    DevJob job;                 // type 0 (I), 1 (P, one reference) or 2 (B)
-    DevPic pic;                 // the picture being reconstructed (its slot's previous content)
-    const uint32_t *order;      // MB indices in dependency order
-    int32_t *counters;          // [0] next ticket, [kCounterError] error flags
-    int32_t *spread;            // the picture's spread range record
-};
-
 // Intra matrix (quantize.cpp:13-23), its dead-zone weights (quantize.cpp:
 // 42-52), the reciprocal multipliers of the matrix, and the zig-zag scan
 // (scan.h:52-62: anti-diagonals, odd ones walked downwards).
@@ -4231,6 +4198,64 @@ constexpr IntraTables make_intra_tables()
 }
 
 __constant__ IntraTables c_intra = make_intra_tables();
+
+// The quantiser tables and the zig-zag scan as the encode's coding stage
+// reads them: copied into LDS once per workgroup, since every lookup is
+// indexed per lane inside the serial loops of the quantisation, the
+// trellis and the inverse quantisation.
+struct CodeTables {
+    QuantTables inter;
+    IntraTables intra;
+};
+
+__device__ __forceinline__ void load_code_tables(CodeTables &t)
+{
+    const uint32_t *a = reinterpret_cast<const uint32_t *>(&c_quant);
+    const uint32_t *b = reinterpret_cast<const uint32_t *>(&c_intra);
+    uint32_t *ta = reinterpret_cast<uint32_t *>(&t.inter);
+    uint32_t *tb = reinterpret_cast<uint32_t *>(&t.intra);
+    for (int i = threadIdx.x; i < (int)(sizeof(QuantTables) / 4); i += blockDim.x) ta[i] = a[i];
+    for (int i = threadIdx.x; i < (int)(sizeof(IntraTables) / 4); i += blockDim.x) tb[i] = b[i];
+}
+
+// Waves of the encode's workgroup and the RDO's candidates.
+constexpr int kEncWaves = 4;
+// Coded candidates (slots): 0 the intra-frame block search's result, 1 the
+// (forward) inter search's, 2 the backward search's (B) or the plain intra
+// block (P), 3..5 the directional modes (P: the three candidates; I and B:
+// the chosen one in 3).
+constexpr int kEncCands = 6;
+constexpr int kSlotIntra = 0, kSlotInter = 1, kSlotPlain = 2, kSlotBwd = 2, kSlotMode = 3;
+
+// What the waves of a MB's workgroup exchange.
+struct EncXchg {
+    SearchOut intra, inter[2];
+    int32_t dir_mode, dir_sad, cands, pmx, pmy;
+    float cost[kEncCands];
+    int32_t qp[kEncCands], var[kEncCands];
+};
+
+struct EncLds {
+    IntraWin iw;                          // the picture being reconstructed, around the MB
+    SubpelLds sp[2];                      // the inter searches (waves 1 and 2)
+    HalfPlanes<16> hy;                    // the intra search's half-pel planes (wave 0)
+    HalfPlanes<8> hu, hv;
+    MbLds mb[kEncWaves];                  // per wave: source, predictions, residual, scratch
+    IntraEdges edges;
+    EncXchg x;
+    CodeTables tables;
+    int16_t co[kEncCands][EVXA_MB_SAMPLES];   // per candidate: coefficients,
+    int16_t lv[kEncCands][EVXA_MB_SAMPLES];   // levels after the trellis,
+    int16_t rec[kEncCands][EVXA_MB_SAMPLES];  // reconstruction
+};
+
+struct EncArgs {
+    DevJob job;                 // type 0 (I), 1 (P, one reference) or 2 (B)
+    DevPic pic;                 // the picture being reconstructed (its slot's previous content)
+    const uint32_t *order;      // MB indices in dependency order
+    int32_t *counters;          // [0] next ticket, [kCounterError] error flags
+    int32_t *spread;            // the picture's spread range record
+};
 
 // quantize.cpp:74-95: the DC scales of intra blocks.
 __device__ __forceinline__ int luma_dc_scale(int qp)
@@ -4274,11 +4299,11 @@ __device__ __forceinline__ bool intra_quant(int btype)
 // Scan position p of an 8x8 block as a raster index, by the block's intra
 // mode (scan.h:154-159): rows for vertical modes, columns for horizontal
 // ones, zig-zag otherwise.
-__device__ __forceinline__ int scan_pos(int imode, int p)
+__device__ __forceinline__ int scan_pos(const CodeTables &t, int imode, int p)
 {
     if (imode == 1 || imode == 5) return p;
     if (imode == 2 || imode == 6) return (p & 7) * 8 + (p >> 3);
-    return c_intra.zigzag[p];
+    return t.intra.zigzag[p];
 }
 
 // Sample k of an MB in MB order: its 8x8 block's raster position and
@@ -4291,31 +4316,33 @@ __device__ __forceinline__ int mb_pos(int i, bool &chroma)
 
 // Inverse quantisation of one level (quantize.cpp:271-301, 322-332; the
 // trellis's own restatement, quantize.cpp:481-514, is the same arithmetic).
-__device__ __forceinline__ int dequant(int level, int pos, int bq, bool intra, bool chroma)
+__device__ __forceinline__ int dequant(const CodeTables &t, int level, int pos, int bq, bool intra,
+                                       bool chroma)
 {
     if (intra && pos == 0) return (int16_t)(level * (chroma ? chroma_dc_scale(bq) : luma_dc_scale(bq)));
-    const int qm = intra ? c_intra.qm[pos] : c_quant.qm[pos];
+    const int qm = intra ? t.intra.qm[pos] : t.inter.qm[pos];
     return (int16_t)((2 * level * qm * bq) / 16);
 }
 
 // Quantisation of one coefficient: quantize_intra_block_8x8
 // (quantize.cpp:120-200) or quantize_inter_block_8x8 (217-252).
-__device__ __forceinline__ int quant(int sv, int pos, int bq, uint32_t mq, bool intra, bool chroma)
+__device__ __forceinline__ int quant(const CodeTables &t, int sv, int pos, int bq, uint32_t mq,
+                                     bool intra, bool chroma)
 {
     if (intra) {
         if (pos == 0) {
             const int d = chroma ? chroma_dc_scale(bq) : luma_dc_scale(bq);
             return (int16_t)rdiv_magic(sv, d, magic_of(d));
         }
-        int qf = (int16_t)rdiv_magic(sv * 16, c_intra.qm[pos], c_intra.qm_magic[pos]);
+        int qf = (int16_t)rdiv_magic(sv * 16, t.intra.qm[pos], t.intra.qm_magic[pos]);
         if (bq >= 16) {
             const int sg = (qf > 0) - (qf < 0);
-            qf = (int16_t)(qf - (int16_t)((sg * bq * c_intra.dz[pos]) >> 1));
+            qf = (int16_t)(qf - (int16_t)((sg * bq * t.intra.dz[pos]) >> 1));
         }
         return (int16_t)rdiv_magic(qf, bq << 1, mq);
     }
-    const int qf = (int16_t)rdiv_magic(sv * 16, c_quant.qm[pos], c_quant.qm_magic[pos]);
-    int w = c_quant.dz[pos];
+    const int qf = (int16_t)rdiv_magic(sv * 16, t.inter.qm[pos], t.inter.qm_magic[pos]);
+    int w = t.inter.dz[pos];
     if (bq >= 16) w = min(w + 1, 6);
     const int sg = (qf > 0) - (qf < 0);
     const int dz = (int16_t)((sg * bq * w) >> 1);
@@ -4325,10 +4352,10 @@ __device__ __forceinline__ int quant(int sv, int pos, int bq, uint32_t mq, bool 
 // The trellis's cost of `level` for coefficient `co` (quantize.cpp:
 // 536-560): squared error of the inverse-quantised level, as an int32 that
 // wraps, plus lambda times the level's bits; fp32 multiply then add.
-__device__ __forceinline__ float trellis_cost(int co, int level, int pos, int bq, bool intra,
-                                              bool chroma, float lambda)
+__device__ __forceinline__ float trellis_cost(const CodeTables &t, int co, int level, int pos,
+                                              int bq, bool intra, bool chroma, float lambda)
 {
-    const int deq = level == 0 ? 0 : dequant(level, pos, bq, intra, chroma);
+    const int deq = level == 0 ? 0 : dequant(t, level, pos, bq, intra, chroma);
     const int diff = co - deq;
     const int sq = (int)((uint32_t)diff * (uint32_t)diff);
     return __fadd_rn(__int2float_rn(sq), __fmul_rn(lambda, __int2float_rn(coeff_bits(level))));
@@ -4342,8 +4369,8 @@ __device__ __forceinline__ float trellis_cost(int co, int level, int pos, int bq
 // highest position whose level it would keep.  The forward pass then lowers
 // each nonzero level (positions >= 1) by one when that costs less; its
 // decisions do not depend on each other.
-__device__ void trellis_mb(int qp, bool intra, int imode, float lambda, const int16_t *co,
-                           int16_t *lv)
+__device__ void trellis_mb(const CodeTables &t, int qp, bool intra, int imode, float lambda,
+                           const int16_t *co, int16_t *lv)
 {
     const int p = lane_id();
     const int cqp = min(qp + 2, 63);
@@ -4352,21 +4379,21 @@ __device__ void trellis_mb(int qp, bool intra, int imode, float lambda, const in
         block_origin(blk, origin, pitch);
         const bool chroma = blk >= 4;
         const int bq = chroma ? cqp : qp;
-        const int pos = scan_pos(imode, p);
+        const int pos = scan_pos(t, imode, p);
         const int idx = origin + (pos >> 3) * pitch + (pos & 7);
         const int c = co[idx];
         int level = lv[idx];
         const uint64_t nz = __ballot(level != 0);
         if (nz == 0) continue;
         const int last = 63 - __builtin_clzll(nz);
-        const float keep = trellis_cost(c, level, pos, bq, intra, chroma, lambda);
-        const float zero = trellis_cost(c, 0, pos, bq, intra, chroma, lambda);
+        const float keep = trellis_cost(t, c, level, pos, bq, intra, chroma, lambda);
+        const float zero = trellis_cost(t, c, 0, pos, bq, intra, chroma, lambda);
         const uint64_t kept = __ballot(level != 0 && p >= 1 && p <= last && !(zero <= keep));
         const int stop = kept ? 63 - __builtin_clzll(kept) : 0;
         if (p >= 1 && p > stop) level = 0;
         if (p >= 1 && level != 0) {
             const int reduced = (int16_t)(level - ((level > 0) - (level < 0)));
-            if (trellis_cost(c, reduced, pos, bq, intra, chroma, lambda) < keep) level = reduced;
+            if (trellis_cost(t, c, reduced, pos, bq, intra, chroma, lambda) < keep) level = reduced;
         }
         lv[idx] = (int16_t)level;
     }
@@ -4379,8 +4406,8 @@ __device__ void trellis_mb(int qp, bool intra, int imode, float lambda, const in
 // from sh.mb.src and the prediction in sh.mb.pred into co/lv/rec of `slot`.
 // Copies are not transformed (encode.cpp:742-744): zero coefficients and
 // levels, the prediction as the reconstruction.
-__device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, MbLds &mb, int16_t *co,
-                                        int16_t *lv, int16_t *rec)
+__device__ __forceinline__ void code_mb(const DevJob &job, const CodeTables &t, EncDesc &d,
+                                        MbLds &mb, int16_t *co, int16_t *lv, int16_t *rec)
 {
     const int lane = lane_id();
     if (d.btype & EVXA_BLOCK_COPY) {
@@ -4406,17 +4433,17 @@ __device__ __forceinline__ void code_mb(const DevJob &job, EncDesc &d, MbLds &mb
         const int pos = mb_pos(i, chroma);
         const int sv = mb.res[i];
         co[i] = (int16_t)sv;
-        lv[i] = (int16_t)quant(sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra, chroma);
+        lv[i] = (int16_t)quant(t, sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra, chroma);
     }
     wave_sync();
-    trellis_mb(qp, intra, d.imode, job.lambda, co, lv);
+    trellis_mb(t, qp, intra, d.imode, job.lambda, co, lv);
     // Inverse quantisation into mb.res, then the inverse DCT: vertical
     // pass (lane = block column) into tmp, horizontal pass plus the
     // prediction (lane = block row) into rec, stored as int16.
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         bool chroma;
         const int pos = mb_pos(i, chroma);
-        mb.res[i] = (int16_t)dequant(lv[i], pos, chroma ? cqp : qp, intra, chroma);
+        mb.res[i] = (int16_t)dequant(t, lv[i], pos, chroma ? cqp : qp, intra, chroma);
     }
     wave_sync();
     if (lane < 48) {
@@ -4650,16 +4677,50 @@ __device__ __forceinline__ void predict_desc(const DevJob &job, const EncDesc &d
     }
 }
 
+// The prediction of the intra-frame block search's result into `out`, from
+// what the search built: the half-pel planes around its integer best (the
+// same samples interpolate_subpixel_6tap forms, averaged with the integer
+// block for quarter-pel) or the integer block of the window.
+__device__ __forceinline__ void predict_intra_result(const WinView &wv, const SearchOut &r,
+                                                     const HalfPlanes<16> &hy,
+                                                     const HalfPlanes<8> &hu,
+                                                     const HalfPlanes<8> &hv, int16_t *out)
+{
+    int dx = 0, dy = 0;
+    if (r.sp_en) frac_direction(r.sp_idx, dx, dy);
+    const PlaneView vy = plane_view<16>(hy, dx, dy), vu = plane_view<8>(hu, dx, dy),
+                    vv = plane_view<8>(hv, dx, dy);
+    for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64) {
+        const bool luma = i < 256;
+        const int k = luma ? i : (i - 256) & 63;
+        const int col = luma ? k & 15 : k & 7, row = luma ? k >> 4 : k >> 3;
+        const int16_t *pl = luma ? wv.y : (i < 320 ? wv.u : wv.v);
+        const int pitch = luma ? wv.py : wv.pc;
+        const int X = (luma ? r.best_x - wv.lx : (r.best_x >> 1) - wv.cx) + col;
+        const int Y = (luma ? r.best_y - wv.ly : (r.best_y >> 1) - wv.cy) + row;
+        const int full = pl[Y * pitch + X];
+        int v = full;
+        if (r.sp_en) {
+            const PlaneView &pv = luma ? vy : (i < 320 ? vu : vv);
+            const int half = pv.arr[pv.off + row * pv.stride + col];
+            v = r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half;
+        }
+        out[i] = (int16_t)v;
+    }
+    wave_sync();
+}
+
 // compute_rdo_cost (encode.cpp:128-166): the candidate coded and
 // reconstructed into `slot`, luma SSD of the reconstruction (an int32 that
 // wraps) plus lambda times its estimated bits, fp32 multiply then add.
 // With `rdo` false only the coding and reconstruction (cost 0).
-__device__ __forceinline__ float rdo_cost(const DevJob &job, EncDesc &d, int x, int y,
-                                          const WinView &wv, const IntraEdges &edges, MbLds &mb,
-                                          int16_t *co, int16_t *lv, int16_t *rec, bool rdo)
+__device__ __forceinline__ float rdo_cost(const DevJob &job, const CodeTables &t, EncDesc &d, int x,
+                                          int y, const WinView &wv, const IntraEdges &edges,
+                                          MbLds &mb, int16_t *co, int16_t *lv, int16_t *rec,
+                                          bool rdo, bool predicted = false)
 {
-    predict_desc(job, d, x, y, wv, edges, mb);
-    code_mb(job, d, mb, co, lv, rec);
+    if (!predicted) predict_desc(job, d, x, y, wv, edges, mb);
+    code_mb(job, t, d, mb, co, lv, rec);
     if (!rdo) return 0.0f;
     const int lane = lane_id();
     uint32_t ssd = 0;
@@ -4729,6 +4790,126 @@ __device__ __forceinline__ void store_reconstruction(const EncArgs &a, int bx, i
     }
 }
 
+// One 3x3 level of the intra-frame block search (motion.cpp:348-379) over
+// the window, in two passes and a closed-form fold.  The level's positions
+// in scan order without its centre are slots 0..7: for the first scan
+// (first = true: columns x - 16 .. x + 16, rows y - 32 .. y, step 16) the
+// row above the MB, the row of the MB above, then (x - 16, y) and
+// (x + 16, y); for the refinements the 3x3 ring of step `step` around the
+// best.  The centre is never an effective candidate: the first scan's is
+// the MB itself, which is unstable, and a refinement's is the running best,
+// whose re-evaluation leaves the state as it is (same cost and distance; a
+// copy would have ended the search) or is the MB itself (nothing taken
+// yet).  Group g evaluates slot g, then slot g + 4.  The fold is the
+// sequential rule's closed form: the first valid slot with MAD < T ends the
+// level in copy state; otherwise the running best ends at the minimum cost,
+// the first slot at that cost replaced by later equal-cost slots with a
+// strictly smaller distance and SAD < 8192 (motion.cpp:125-178).  Returns
+// true on a copy.
+__device__ __forceinline__ bool intra_level(const WinView &wv, int pw, int ph, const Blk &s,
+                                            const Geo &geo, int x, int y, int thresh, float lambda,
+                                            bool first, int step, IntraState &st)
+{
+    const int lane = lane_id();
+    const int g = lane >> 4;
+    auto off = [&](int k, int &i, int &j) {
+        if (first) {
+            i = k < 6 ? (k % 3 - 1) * 16 : (k == 6 ? -16 : 16);
+            j = k < 6 ? (k / 3) * 16 - 32 : 0;
+        } else {
+            i = slot_i_rt(k) * step;
+            j = slot_j_rt(k) * step;
+        }
+    };
+    const int base_x = st.x, base_y = st.y, max_x = pw - 16, max_y = ph - 16;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int i, j;
+        off(k, i, j);
+        const int cx = base_x + i, cy = base_y + j;
+        if (!intra_unstable(cx, cy, x, y) && cx >= 0 && cx <= max_x && cy >= 0 && cy <= max_y)
+            mask |= 1u << k;
+    }
+    if (mask == 0) return false;
+    int m[2][4];
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+        const int k = g + 4 * pass;
+        int i, j;
+        off(k, i, j);
+        const bool valid = ((mask >> k) & 1u) != 0;
+        const int cx = valid ? base_x + i : x, cy = valid ? base_y + j : y;
+        Blk r;
+        load_cand(wv, geo, cx, cy, r);
+        int sad, mad, satd;
+        cand_metrics(false, s, r, sad, mad, satd);
+        m[pass][0] = sad;
+        m[pass][1] = mad;
+        m[pass][2] = (int)cost_bits(sad, satd, cx - x, cy - y, lambda);
+    }
+    // Slot k's values into lane k (k < 8): slot g of pass p sits in lane 16 g.
+    const int from = ((lane & 3) << 4) << 2;
+    const bool second = (lane & 4) != 0;
+    int ks, km, kc;
+    {
+        const int s0 = __builtin_amdgcn_ds_bpermute(from, m[0][0]);
+        const int s1 = __builtin_amdgcn_ds_bpermute(from, m[1][0]);
+        const int d0 = __builtin_amdgcn_ds_bpermute(from, m[0][1]);
+        const int d1 = __builtin_amdgcn_ds_bpermute(from, m[1][1]);
+        const int c0 = __builtin_amdgcn_ds_bpermute(from, m[0][2]);
+        const int c1 = __builtin_amdgcn_ds_bpermute(from, m[1][2]);
+        ks = second ? s1 : s0;
+        km = second ? d1 : d0;
+        kc = second ? c1 : c0;
+    }
+    const bool kv = lane < 8 && ((mask >> (lane & 7)) & 1u) != 0;
+    auto dist = [&](int cx, int cy) { return (cx - x) * (cx - x) + (cy - y) * (cy - y); };
+    auto take = [&](int k) {
+        int i, j;
+        off(k, i, j);
+        st.x = base_x + i;
+        st.y = base_y + j;
+        st.sad = lane_value(ks, k);
+        st.mad = lane_value(km, k);
+        st.ssd = dist(st.x, st.y);
+        st.cost = (uint32_t)lane_value(kc, k);
+    };
+    const uint32_t copym = (uint32_t)__ballot(kv && km < thresh) & 0xFFu;
+    if (copym) {
+        const int p = __builtin_ctz(copym);
+        st.n_int += __builtin_popcount(mask & ((2u << p) - 1u));
+        take(p);
+        return true;
+    }
+    st.n_int += __builtin_popcount(mask);
+    uint32_t ml = kv ? (uint32_t)kc : 0xFFFFFFFFu;
+    ml = min(ml, (uint32_t)dpp<0xB1>((int)ml));
+    ml = min(ml, (uint32_t)dpp<0x4E>((int)ml));
+    ml = min(ml, (uint32_t)dpp<0x141>((int)ml));
+    const uint32_t mn = min(st.cost, (uint32_t)lane_value((int)ml, 0));
+    uint32_t eq = (uint32_t)__ballot(kv && (uint32_t)kc == mn) & 0xFFu;
+    if (eq == 0) return false;                     // the running best stays
+    bool have = st.cost == mn;
+    while (eq) {
+        const int q = __builtin_ctz(eq);
+        eq &= eq - 1;
+        int i, j;
+        off(q, i, j);
+        const int ex = base_x + i, ey = base_y + j;
+        const int esad = lane_value(ks, q), essd = dist(ex, ey);
+        if (!have || (essd < st.ssd && esad < kSadThreshold)) {
+            st.x = ex; st.y = ey;
+            st.sad = esad;
+            st.mad = lane_value(km, q);
+            st.ssd = essd;
+            st.cost = mn;
+        }
+        have = true;
+    }
+    return false;
+}
+
 // calculate_intra_prediction (motion.cpp:485-565) over the window of the
 // picture being reconstructed, as intra_search, with the sub-pel step of
 // perform_intra_subpixel_motion_search (motion.cpp:408-448) evaluated the
@@ -4752,11 +4933,9 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
     for (int k = 0; k < 8; k++) a += abs(lo16(s.y[k])) + abs(hi16(s.y[k]));
     const int sad0 = lane_value(row_sum(a), 0);       // compute_block_sad(src), analysis.h:134-145
     IntraState st{x, y, sad0, kInt32Max, kInt32Max, __float_as_uint(__int2float_rn(sad0)), 0};
-    intra_scan(wv, pw, ph, s, geo, x, y, thresh, lambda, -kSearchRadius, -2 * kSearchRadius,
-               kSearchRadius, st);
-    if (st.mad >= thresh)
+    if (!intra_level(wv, pw, ph, s, geo, x, y, thresh, lambda, true, kSearchRadius, st))
         for (int step = kSearchRadius / 2; step > 0; step >>= 1)
-            if (intra_scan(wv, pw, ph, s, geo, x, y, thresh, lambda, -step, -step, step, st)) break;
+            if (intra_level(wv, pw, ph, s, geo, x, y, thresh, lambda, false, step, st)) break;
 
     int sp_en = 0, sp_amt = 0, sp_idx = 0, n_ax = 0, n_dg = 0;
     uint32_t mask = 0;
@@ -4931,6 +5110,7 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         if (lane == 0) {
             sh.x.pmx = pmx;
             sh.x.pmy = pmy;
+            sh.x.cands = -1;                      // wave 3 publishes this MB's mode candidates
         }
     }
     __syncthreads();                              // the dependencies are done
@@ -4957,12 +5137,35 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     const float lambda = job.lambda;
     const int pmx = sh.x.pmx, pmy = sh.x.pmy;
 
+    // The searches, each followed by the coding of its result, which is a
+    // candidate of the decision whatever the others find (P frames' RDO
+    // codes every candidate; I and B frames code the winner, which is one of
+    // these unless it is the bidirectional average).  Costs are those of
+    // compute_rdo_cost (encode.cpp:128-166); only P frames use them.
+    const bool rdo = job.type == EVXA_FRAME_P;
+    const int tgt0 = job.target_ids & 0xFF, tgt1 = (job.target_ids >> 8) & 0xFF;
+    auto code_slot = [&](EncDesc d, int slot, bool predicted = false) {
+        const float cost = rdo_cost(job, sh.tables, d, x, y, wv, sh.edges, mine, sh.co[slot],
+                                    sh.lv[slot], sh.rec[slot], rdo, predicted);
+        if (lane == 0) {
+            sh.x.cost[slot] = cost;
+            sh.x.qp[slot] = d.qp;
+            sh.x.var[slot] = d.var;
+        }
+    };
     if (wv_id == 0) {
         // calculate_intra_prediction, for every frame type (encode.cpp:175,
         // 362, 380).
         const SearchOut ri = intra_search_win(wv, W, H, s, geo, x, y, thresh, lambda, sh.hy,
                                               sh.hu, sh.hv);
         if (lane == 0) sh.x.intra = ri;
+        const EncDesc di = desc_of(ri, x, y, 0, 0);
+        if (di.btype == EVXA_BLOCK_INTRA) {
+            code_slot(di, kSlotIntra);                   // nothing found: the plain intra block
+        } else {
+            predict_intra_result(wv, ri, sh.hy, sh.hu, sh.hv, mine.pred);
+            code_slot(di, kSlotIntra, true);
+        }
     } else if (searcher) {
         const SearchOut r = inter_search<1, true>(job.refs[my_ref], W, H, narrowk, s, geo, pre, wk,
                                                   x, y, thresh, lambda, pmx, pmy, sh.sp[my_ref]);
@@ -4970,6 +5173,7 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         // A B frame's bidirectional candidate averages both predictions
         // (motion.cpp:729-776): each search's wave forms its own.
         if (job.type == EVXA_FRAME_B) predict_mb(job.refs[my_ref], r, mine.pred_b, mine.tmp);
+        code_slot(desc_of(r, x, y, my_ref ? tgt1 : tgt0, my_ref), my_ref ? kSlotBwd : kSlotInter);
     } else if (wv_id == kEncWaves - 1) {
         // The directional intra candidate (encode.cpp:254-272, 446-468;
         // select_best_intra_mode, intra_pred.cpp:217-302: the first
@@ -4987,17 +5191,42 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         if (lane == 0) {
             sh.x.dir_sad = dir_sad;
             sh.x.dir_mode = dir_mode;
-            sh.x.cands = (int32_t)cands;
+            __hip_atomic_store(&sh.x.cands, (int32_t)cands, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
         }
+        // P: the first two mode candidates; I and B: the chosen mode.
+        for (int c = 0; c < (rdo ? 2 : 1); c++) {
+            EncDesc d = desc_clear();
+            d.imode = rdo ? (int)((cands >> (8 * c)) & 0xFF) : dir_mode;
+            code_slot(d, kSlotMode + c);
+        }
+    } else if (rdo) {
+        // The other P candidates: the plain intra block (unused when the
+        // winner is of the intra-default kind) and the third mode, which
+        // needs wave 3's neighbour samples.
+        code_slot(desc_clear(), kSlotPlain);
+        int32_t cw;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        while ((cw = __hip_atomic_load(&sh.x.cands, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) < 0) {
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kEncWaitTicks) {
+                if (lane == 0) atomicOr(&a.counters[kCounterError], kErrWaitTimeout);
+                cw = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        EncDesc d = desc_clear();
+        d.imode = (int)(((uint32_t)__builtin_amdgcn_readfirstlane(cw) >> 16) & 0xFF);
+        code_slot(d, kSlotMode + 2);
     }
     __syncthreads();
 
     // The selection, made alike by every wave from what the waves left.
     const SearchOut ri = sh.x.intra;
     EncDesc best = desc_of(ri, x, y, 0, 0);
-    int best_sad = ri.sad;
+    int best_sad = ri.sad, best_slot = kSlotIntra;
     int n_int = ri.n_int, n_ax = ri.n_ax, n_dg = ri.n_dg;
-    const int tgt0 = job.target_ids & 0xFF, tgt1 = (job.target_ids >> 8) & 0xFF;
+    bool bidir = false;
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:183-210: the copy state decides first, then the SAD.
         const SearchOut r = sh.x.inter[0];
@@ -5005,10 +5234,11 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         const bool inter_copy = (r.btype & EVXA_BLOCK_COPY) != 0;
         const bool best_copy = (best.btype & EVXA_BLOCK_COPY) != 0;
         if (inter_copy != best_copy) {
-            if (inter_copy) { best = desc_of(r, x, y, tgt0, 0); best_sad = r.sad; }
+            if (inter_copy) { best = desc_of(r, x, y, tgt0, 0); best_sad = r.sad; best_slot = kSlotInter; }
         } else if (r.sad < best_sad) {
             best = desc_of(r, x, y, tgt0, 0);
             best_sad = r.sad;
+            best_slot = kSlotInter;
         }
     } else if (job.type == EVXA_FRAME_B) {
         // encode.cpp:386-458: forward, backward, then their average.
@@ -5016,8 +5246,8 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         n_int += f.n_int + b.n_int;
         n_ax += f.n_ax + b.n_ax;
         n_dg += f.n_dg + b.n_dg;
-        if (f.sad < best_sad) { best = desc_of(f, x, y, tgt0, 0); best_sad = f.sad; }
-        if (b.sad < best_sad) { best = desc_of(b, x, y, tgt1, 1); best_sad = b.sad; }
+        if (f.sad < best_sad) { best = desc_of(f, x, y, tgt0, 0); best_sad = f.sad; best_slot = kSlotInter; }
+        if (b.sad < best_sad) { best = desc_of(b, x, y, tgt1, 1); best_sad = b.sad; best_slot = kSlotBwd; }
         int bsad = 0;
         for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
             const int av = (sh.mb[1].pred_b[i] + sh.mb[2].pred_b[i] + 1) >> 1;
@@ -5035,6 +5265,7 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
             best.mvx_b = b.best_x - x; best.mvy_b = b.best_y - y;
             best.sp_b = b.sp_en; best.sa_b = b.sp_amt; best.si_b = b.sp_idx;
             best_sad = bsad;
+            bidir = true;
         }
     }
     if (best_sad == kInt32Max && wv_id == 0 && lane == 0)
@@ -5044,52 +5275,54 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         best = desc_clear();
         best.imode = sh.x.dir_mode;
         best_sad = sh.x.dir_sad;
+        bidir = false;
+        best_slot = kSlotMode;
+        if (rdo)
+            for (int c = 0; c < 3; c++)
+                if ((int)((cands >> (8 * c)) & 0xFF) == best.imode) { best_slot = kSlotMode + c; break; }
+    }
+    if (wv_id != 0) return;
+    if (bidir) {
+        // The bidirectional winner of a B frame: coded now, from the two
+        // predictions the searches' waves formed.
+        best_slot = kSlotIntra;
+        code_slot(best, best_slot);
     }
 
     // RDO for P frames (encode.cpp:274-348, EVX_ENABLE_RDO_COPY_EVAL and
-    // EVX_ENABLE_RDO_INTRA_MODES): candidate 0 the SAD winner, 1 the plain
-    // intra block unless the winner is one of the intra-default kind, 2..4
-    // the three mode candidates; the first at the lowest cost wins (a later
-    // one only when strictly lower).  I and B frames code the winner once.
-    const bool rdo = job.type == EVXA_FRAME_P;
-    const int n_cand = rdo ? kEncCands : 1;
-    auto cand_desc = [&](int c) {
-        EncDesc d = best;
-        if (c > 0) {
-            d = desc_clear();
-            if (c > 1) d.imode = (cands >> (8 * (c - 2))) & 0xFF;
+    // EVX_ENABLE_RDO_INTRA_MODES): the SAD winner, the plain intra block
+    // unless the winner is one of the intra-default kind, then the three
+    // mode candidates; a later candidate wins only at a strictly lower cost.
+    int slot = best_slot;
+    float best_cost = 0.0f;
+    if (rdo) {
+        best_cost = sh.x.cost[best_slot];
+        if (best.btype != EVXA_BLOCK_INTRA && sh.x.cost[kSlotPlain] < best_cost) {
+            best_cost = sh.x.cost[kSlotPlain];
+            slot = kSlotPlain;
+            best = desc_clear();
         }
-        return d;
-    };
-    const bool has_plain = best.btype != EVXA_BLOCK_INTRA;
-    for (int c = wv_id; c < n_cand; c += kEncWaves) {
-        if (c == 1 && !has_plain) continue;
-        EncDesc d = cand_desc(c);
-        const float cost = rdo_cost(job, d, x, y, wv, sh.edges, mine, sh.co[c], sh.lv[c],
-                                    sh.rec[c], rdo);
-        if (lane == 0) {
-            sh.x.cost[c] = cost;
-            sh.x.qp[c] = d.qp;
-            sh.x.var[c] = d.var;
-        }
+        for (int c = 0; c < 3; c++)
+            if (sh.x.cost[kSlotMode + c] < best_cost) {
+                best_cost = sh.x.cost[kSlotMode + c];
+                slot = kSlotMode + c;
+                best = desc_clear();
+                best.imode = (cands >> (8 * c)) & 0xFF;
+            }
     }
-    __syncthreads();
-    if (wv_id != 0) return;
-
-    int slot = 0;
-    float best_cost = sh.x.cost[0];
-    for (int c = 1; c < n_cand; c++) {
-        if (c == 1 && !has_plain) continue;
-        if (sh.x.cost[c] < best_cost) { best_cost = sh.x.cost[c]; slot = c; }
-    }
-    best = cand_desc(slot);
     best.qp = sh.x.qp[slot];
     best.var = sh.x.var[slot];
-    if (!rdo) best_cost = 0.0f;
 
     // Outputs: the reconstruction into the picture, coefficients, levels,
     // the record; then the MB's word, once every store has completed.
     store_reconstruction(a, bx, by, sh.rec[slot]);
+    // The MB's word goes out once the reconstruction's stores have completed
+    // (what later MBs read of this one); coefficients, levels and the record
+    // are read after the launch.
+    stores_done();
+    if (lane == 0)
+        st_relaxed64(job.mvw + mb, pred_word((best.btype & EVXA_BLOCK_MOTION) != 0,
+                                             mv_word(best.mvx, best.mvy)));
     int16_t *co = job.coeffs + (size_t)mb * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)mb * EVXA_MB_SAMPLES;
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
@@ -5115,9 +5348,6 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     w[14] = ((best.btype & EVXA_BLOCK_COPY) ? 0u : 1u) | (best.pmode == 2 ? 2u : 0u) | 4u;
     w[15] = 0;
     write_words(reinterpret_cast<uint32_t *>(job.records + mb), w, 16);
-    stores_done();
-    if (lane == 0)
-        st_relaxed64(job.mvw + mb, pred_word((best.btype & EVXA_BLOCK_MOTION) != 0, w[1]));
 }
 
 // The persistent in-loop encode kernel: a workgroup of four waves takes the
@@ -5129,6 +5359,7 @@ __global__ __launch_bounds__(64 * kEncWaves) void encode_kernel(const EncArgs a)
     __shared__ EncLds sh;
     __shared__ int32_t s_word;
     const int n_mb = a.job.wb * a.job.hb;
+    load_code_tables(sh.tables);        // read after the claim's barrier
     for (;;) {
         if (threadIdx.x == 0) s_word = atomicAdd(&a.counters[0], 1);
         __syncthreads();
