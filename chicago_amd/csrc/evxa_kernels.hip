@@ -4226,11 +4226,13 @@ constexpr int kEncWaves = 4;
 // the chosen one in 3).
 constexpr int kEncCands = 6;
 constexpr int kSlotIntra = 0, kSlotInter = 1, kSlotPlain = 2, kSlotBwd = 2, kSlotMode = 3;
+constexpr int kSlotBidir = 4;         // B: the average of both searches
 
 // What the waves of a MB's workgroup exchange.
 struct EncXchg {
     SearchOut intra, inter[2];
     int32_t dir_mode, dir_sad, cands, pmx, pmy;
+    int32_t preds;                        // B: bit k set once search k's prediction is in LDS
     float cost[kEncCands];
     int32_t qp[kEncCands], var[kEncCands];
 };
@@ -4710,6 +4712,54 @@ __device__ __forceinline__ void predict_intra_result(const WinView &wv, const Se
     wave_sync();
 }
 
+// The prediction of an inter search's result into `out` (predict_mb's
+// samples): for sub-pel results the half-pel planes the search built around
+// its integer best (averaged with the reference's integer block for
+// quarter-pel), else the reference's integer block.
+__device__ __forceinline__ void predict_inter_result(const DevPic &ref, const SearchOut &r,
+                                                     const SubpelLds &sp, int16_t *out)
+{
+    int dx = 0, dy = 0;
+    if (r.sp_en) frac_direction(r.sp_idx, dx, dy);
+    const PlaneView vy = plane_view<16>(sp.y, dx, dy), vu = plane_view<8>(sp.u, dx, dy),
+                    vv = plane_view<8>(sp.v, dx, dy);
+    for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64) {
+        const bool luma = i < 256;
+        const int k = luma ? i : (i - 256) & 63;
+        const int col = luma ? k & 15 : k & 7, row = luma ? k >> 4 : k >> 3;
+        const int16_t *pl = luma ? ref.y : (i < 320 ? ref.u : ref.v);
+        const int pitch = luma ? ref.py : ref.pc;
+        const int X = (luma ? r.best_x : r.best_x >> 1) + col;
+        const int Y = (luma ? r.best_y : r.best_y >> 1) + row;
+        const int full = pl[Y * pitch + X];
+        int v = full;
+        if (r.sp_en) {
+            const PlaneView &pv = luma ? vy : (i < 320 ? vu : vv);
+            const int half = pv.arr[pv.off + row * pv.stride + col];
+            v = r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half;
+        }
+        out[i] = (int16_t)v;
+    }
+    wave_sync();
+}
+
+// The bidirectional descriptor of a B MB from its two searches
+// (encode.cpp:446-458; motion.cpp:702-840).
+__device__ __forceinline__ EncDesc bidir_desc(const SearchOut &f, const SearchOut &b, int x, int y,
+                                              int tgt0, int tgt1)
+{
+    EncDesc d = desc_clear();
+    d.btype = EVXA_BLOCK_MOTION;
+    d.pmode = 2;
+    d.tgt = tgt0;
+    d.mvx = f.best_x - x; d.mvy = f.best_y - y;
+    d.sp = f.sp_en; d.sa = f.sp_amt; d.si = f.sp_idx;
+    d.tgt_b = tgt1;
+    d.mvx_b = b.best_x - x; d.mvy_b = b.best_y - y;
+    d.sp_b = b.sp_en; d.sa_b = b.sp_amt; d.si_b = b.sp_idx;
+    return d;
+}
+
 // compute_rdo_cost (encode.cpp:128-166): the candidate coded and
 // reconstructed into `slot`, luma SSD of the reconstruction (an int32 that
 // wraps) plus lambda times its estimated bits, fp32 multiply then add.
@@ -5111,6 +5161,7 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
             sh.x.pmx = pmx;
             sh.x.pmy = pmy;
             sh.x.cands = -1;                      // wave 3 publishes this MB's mode candidates
+            sh.x.preds = 0;
         }
     }
     __syncthreads();                              // the dependencies are done
@@ -5169,11 +5220,23 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     } else if (searcher) {
         const SearchOut r = inter_search<1, true>(job.refs[my_ref], W, H, narrowk, s, geo, pre, wk,
                                                   x, y, thresh, lambda, pmx, pmy, sh.sp[my_ref]);
-        if (lane == 0) sh.x.inter[my_ref] = r;
-        // A B frame's bidirectional candidate averages both predictions
-        // (motion.cpp:729-776): each search's wave forms its own.
-        if (job.type == EVXA_FRAME_B) predict_mb(job.refs[my_ref], r, mine.pred_b, mine.tmp);
-        code_slot(desc_of(r, x, y, my_ref ? tgt1 : tgt0, my_ref), my_ref ? kSlotBwd : kSlotInter);
+        // The search's prediction, from its planes (the bidirectional
+        // candidate of a B frame averages both searches', motion.cpp:729-776).
+        predict_inter_result(job.refs[my_ref], r, sh.sp[my_ref], mine.pred_b);
+        if (lane == 0) {
+            sh.x.inter[my_ref] = r;
+            __hip_atomic_fetch_or(&sh.x.preds, 1 << my_ref, __ATOMIC_RELEASE,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        const EncDesc d = desc_of(r, x, y, my_ref ? tgt1 : tgt0, my_ref);
+        // Coded from that prediction when the descriptor names this search's
+        // reference (the first with its target id, as decode_block finds it).
+        const bool same_ref = my_ref == 0 || tgt1 != tgt0;
+        if (same_ref) {
+            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) mine.pred[i] = mine.pred_b[i];
+            wave_sync();
+        }
+        code_slot(d, my_ref ? kSlotBwd : kSlotInter, same_ref);
     } else if (wv_id == kEncWaves - 1) {
         // The directional intra candidate (encode.cpp:254-272, 446-468;
         // select_best_intra_mode, intra_pred.cpp:217-302: the first
@@ -5199,6 +5262,28 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
             EncDesc d = desc_clear();
             d.imode = rdo ? (int)((cands >> (8 * c)) & 0xFF) : dir_mode;
             code_slot(d, kSlotMode + c);
+        }
+        if (job.type == EVXA_FRAME_B) {
+            // B: the bidirectional candidate too, once both searches'
+            // predictions are in LDS.
+            int32_t ready;
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            while ((ready = __hip_atomic_load(&sh.x.preds, __ATOMIC_ACQUIRE,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP)) != 3) {
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kEncWaitTicks) {
+                    if (lane == 0) atomicOr(&a.counters[kCounterError], kErrWaitTimeout);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            const EncDesc d = bidir_desc(sh.x.inter[0], sh.x.inter[1], x, y, tgt0, tgt1);
+            const bool own = tgt1 != tgt0;   // else decode_block takes both from the first reference
+            if (own) {
+                for (int i = lane; i < EVXA_MB_SAMPLES; i += 64)
+                    mine.pred[i] = (int16_t)((sh.mb[1].pred_b[i] + sh.mb[2].pred_b[i] + 1) >> 1);
+                wave_sync();
+            }
+            code_slot(d, kSlotBidir, own);
         }
     } else if (rdo) {
         // The other P candidates: the plain intra block (unused when the
@@ -5226,7 +5311,6 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     EncDesc best = desc_of(ri, x, y, 0, 0);
     int best_sad = ri.sad, best_slot = kSlotIntra;
     int n_int = ri.n_int, n_ax = ri.n_ax, n_dg = ri.n_dg;
-    bool bidir = false;
     if (job.type == EVXA_FRAME_P) {
         // encode.cpp:183-210: the copy state decides first, then the SAD.
         const SearchOut r = sh.x.inter[0];
@@ -5255,17 +5339,9 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         }
         bsad = wave_sum(bsad);
         if (bsad + __float2int_rz(__fmul_rn(lambda, 8.0f)) < best_sad) {
-            best = desc_clear();
-            best.btype = EVXA_BLOCK_MOTION;
-            best.pmode = 2;
-            best.tgt = tgt0;
-            best.mvx = f.best_x - x; best.mvy = f.best_y - y;
-            best.sp = f.sp_en; best.sa = f.sp_amt; best.si = f.sp_idx;
-            best.tgt_b = tgt1;
-            best.mvx_b = b.best_x - x; best.mvy_b = b.best_y - y;
-            best.sp_b = b.sp_en; best.sa_b = b.sp_amt; best.si_b = b.sp_idx;
+            best = bidir_desc(f, b, x, y, tgt0, tgt1);
             best_sad = bsad;
-            bidir = true;
+            best_slot = kSlotBidir;
         }
     }
     if (best_sad == kInt32Max && wv_id == 0 && lane == 0)
@@ -5275,19 +5351,12 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
         best = desc_clear();
         best.imode = sh.x.dir_mode;
         best_sad = sh.x.dir_sad;
-        bidir = false;
         best_slot = kSlotMode;
         if (rdo)
             for (int c = 0; c < 3; c++)
                 if ((int)((cands >> (8 * c)) & 0xFF) == best.imode) { best_slot = kSlotMode + c; break; }
     }
     if (wv_id != 0) return;
-    if (bidir) {
-        // The bidirectional winner of a B frame: coded now, from the two
-        // predictions the searches' waves formed.
-        best_slot = kSlotIntra;
-        code_slot(best, best_slot);
-    }
 
     // RDO for P frames (encode.cpp:274-348, EVX_ENABLE_RDO_COPY_EVAL and
     // EVX_ENABLE_RDO_INTRA_MODES): the SAD winner, the plain intra block
