@@ -1069,9 +1069,73 @@ def run_encode(args):
         res["us_per_wavefront_step"] = round(float(per[:, 0].mean()) * 1e3 / steps, 2)
     s.close()
     ctx.close()
+    if args.encode_streams > 1:
+        res["batched"] = run_encode_batched(args, frames, roles, device_out)
     if not args.no_cpu:
         res.update(encode_cpu_chain(args, frames, roles, device_out))
     return res
+
+
+def run_encode_batched(args, frames0, roles, device_out):
+    """The same frame loop for several 1920x1080 streams at once: each step
+    codes one frame of every stream in one encode launch (evxa_encode_batch),
+    then deblocks and SAOs each.  Stream 0 codes the single-stream sequence,
+    and its checked frames must equal the single-stream run's; the others
+    code sequences of their own seeds."""
+    from chicago_amd import evxa, synth
+    w, h = 1920, 1080
+    aw, ah = synth.align16(w), synth.align16(h)
+    n_streams, n = args.encode_streams, args.encode_frames
+    seqs = [frames0] + [synth.sequence_c1(synth.seed_of(1, k, 0), n, w, h)
+                        for k in range(1, n_streams)]
+    ctx = evxa.Context(0)
+    n_dpb = 4
+    streams = []
+    for k in range(n_streams):
+        s = ctx.open_stream(aw, ah, n_dpb + n)
+        for d, f in enumerate(seqs[k]):
+            s.upload(n_dpb + d, *f)
+        streams.append(s)
+    kind = {"I": evxa.FRAME_I, "P": evxa.FRAME_P, "B": evxa.FRAME_B}
+
+    def step(r):
+        refs = [] if r.kind == "I" else r.ref_slots
+        d = evxa.encode_desc(kind[r.kind], r.quality, n_dpb + r.index, r.slot, refs,
+                             target_ids=refs, temporal_mvp=r.kind == "P" and r.temporal_from >= 0)
+        ctx.encode_batch(streams, [d] * n_streams)
+        for s in streams:
+            s.deblock(r.slot)
+            s.sao(r.slot, n_dpb + r.index, r.quality, table=False)
+
+    same = True
+    for k, r in enumerate(roles):
+        step(r)
+        if k in device_out:
+            rec, _, lv = streams[0].results()
+            pic = streams[0].download(r.slot)
+            want = device_out[k]
+            same = same and (np.array_equal(rec, want[0]) and np.array_equal(lv, want[1])
+                             and all(np.array_equal(a, b) for a, b in zip(pic, want[2])))
+    ctx.synchronize()
+    ctx.launch_times()
+    t0 = time.perf_counter()
+    for r in roles:
+        step(r)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ms = np.array(ctx.launch_times())
+    per_step = 1 + 3 * n_streams          # encode, then deblock and SAO (2) per stream
+    out = {"workload": f"{n_streams} 1920x1080 streams, the same GOP, one encode launch per step "
+                       f"(one frame of every stream), deblocking and SAO per stream",
+           "streams": n_streams, "ms_per_step": round(dt / len(roles) * 1e3, 3),
+           "frames_per_s": round(n_streams * len(roles) / dt, 2),
+           "stream0_equals_single_stream_run": bool(same)}
+    if ms.size == per_step * len(roles):
+        out["encode_ms_per_step"] = round(float(ms.reshape(len(roles), per_step)[:, 0].mean()), 3)
+    for s in streams:
+        s.close()
+    ctx.close()
+    return out
 
 
 def encode_cpu_chain(args, frames, roles, device_out):
@@ -1269,6 +1333,8 @@ def main():
                                            "loop", "encode", "all", "traffic"],
                     default="all")
     ap.add_argument("--encode-frames", type=int, default=13)
+    ap.add_argument("--encode-streams", type=int, default=8,
+                    help="streams of the batched encode leg (1: none)")
     ap.add_argument("--loop-frames", type=int, default=24)
     ap.add_argument("--gop-frames", type=int, default=33)
     ap.add_argument("--serve-streams", type=int, default=200)

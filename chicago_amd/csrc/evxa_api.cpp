@@ -234,6 +234,20 @@ struct evxa_context {
     // The counter block of the last in-loop encode, when that was the last
     // launch whose errors evxa_synchronize reports.
     const int32_t *enc_last = nullptr;
+    // In-loop encode launches: job table, MB order (cached per composition),
+    // counter block and per-MB words, coefficient candidates per workgroup.
+    struct EncodeRes {
+        DevBuf jobs, order, progress, coscratch;
+        PinnedBuf h_jobs, h_order;
+        hipEvent_t staged = nullptr;
+        std::vector<uint64_t> key;
+        int32_t n_tickets = 0, grid = 0;
+        ~EncodeRes()
+        {
+            if (staged) (void)hipEventDestroy(staged);
+        }
+    } enc;
+    int32_t enc_resident = 0;        // encode workgroups that fit the device
     uint64_t budget = 0;
     uint64_t used = 0;
     int32_t resident[2][3] = {};     // [sweep mode][waves]: workgroups of the analysis kernels that fit
@@ -288,10 +302,7 @@ struct evxa_stream {
     DevBuf recon_flag;     // range record of a reconstruction headed for the filter (unused)
     DevBuf sao_words;      // per MB: published SAO decision; then the SAO counters
     DevBuf sao_table;      // evxa_sao_info per MB
-    // In-loop encode: counter block, per-MB words, then the MB order (built
-    // once, on first use).
-    DevBuf enc;
-    bool enc_order = false;
+
     uint64_t bytes = 0;
     // Recorded after every asynchronous job that writes the stream's slots or
     // outputs, on the HIP stream that job ran on (evxa_stream_wait).
@@ -806,6 +817,159 @@ void check_last_launch(evxa_context *ctx)
         fail(EVXA_ERROR_EXECUTION_FAILURE, "analysis kernel: motion search failed");
 }
 
+// The frames' in-loop encode in one launch (the caller holds ctx->mu).
+// Validates every descriptor before anything is queued.
+void encode_launch(evxa_context *ctx, evxa_stream *const *streams, const evxa_encode_desc *descs,
+                   int32_t n)
+{
+    if (n > (int32_t)(1u << (32 - kMbBits))) fail(EVXA_ERROR_CAPACITY_LIMIT, "too many frames in one launch");
+    for (int32_t k = 0; k < n; k++) {
+        evxa_stream *s = streams[k];
+        const evxa_encode_desc *d = &descs[k];
+        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
+        if (d->type != EVXA_FRAME_I && d->type != EVXA_FRAME_P && d->type != EVXA_FRAME_B)
+            fail(EVXA_ERROR_INVALIDARG, "frame type");
+        if (d->quality < 0 || d->quality > 63) fail(EVXA_ERROR_INVALIDARG, "quality must be 0..63");
+        if (s->w < 48 || s->h < 48) fail(EVXA_ERROR_INVALIDARG, "the encode needs frames of at least 48x48");
+        if ((int64_t)s->wb * s->hb > (int64_t)kMbMask) fail(EVXA_ERROR_CAPACITY_LIMIT, "frame too large");
+        check_slot(s, d->src_slot);
+        check_slot(s, d->dst_slot);
+        if (d->dst_slot == d->src_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is the source");
+        const int32_t n_refs = d->type == EVXA_FRAME_I ? 0 : d->type == EVXA_FRAME_P ? 1 : 2;
+        for (int r = 0; r < n_refs; r++) {
+            check_slot(s, d->ref_slots[r]);
+            if (d->ref_slots[r] == d->dst_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is a reference");
+        }
+    }
+    set_device(ctx);
+    auto &r = ctx->enc;
+    if (!r.staged) check(hipEventCreateWithFlags(&r.staged, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventSynchronize(r.staged), "hipEventSynchronize");   // staging free again
+    if (ctx->enc_resident <= 0) ctx->enc_resident = encode_resident_groups(ctx->device);
+    hipStream_t st = ctx->stream;
+
+    // The MB order of the composition: diagonal v = bx + (lag + 1) by of
+    // each frame, frames interleaved by v, a diagonal's MBs by row; every
+    // MB's dependencies (same frame, smaller v) come first.
+    const int32_t slope = kEncodeLag + 1;
+    std::vector<uint64_t> key;
+    size_t n_mb_total = 0;
+    for (int32_t k = 0; k < n; k++) {
+        key.push_back(((uint64_t)streams[k]->wb << 32) | (uint32_t)streams[k]->hb);
+        n_mb_total += (size_t)streams[k]->wb * streams[k]->hb;
+    }
+    if (key != r.key) {
+        int32_t max_v = 0;
+        for (int32_t k = 0; k < n; k++)
+            max_v = std::max(max_v, streams[k]->wb + slope * (streams[k]->hb - 1));
+        std::vector<uint32_t> order;
+        order.reserve(n_mb_total);
+        int32_t widest = 1;
+        for (int32_t v = 0; v < max_v; v++) {
+            int32_t on_v = 0;
+            for (int32_t k = 0; k < n; k++) {
+                const int32_t wb = streams[k]->wb, hb = streams[k]->hb;
+                for (int32_t by = 0; by < hb; by++) {
+                    const int32_t bx = v - slope * by;
+                    if (bx < 0) break;
+                    if (bx < wb) {
+                        order.push_back(((uint32_t)k << kMbBits) | (uint32_t)(by * wb + bx));
+                        on_v++;
+                    }
+                }
+            }
+            widest = std::max(widest, on_v);
+        }
+        r.h_order.ensure(order.size() * sizeof(uint32_t));
+        memcpy(r.h_order.p, order.data(), order.size() * sizeof(uint32_t));
+        if (r.order.n < order.size() * sizeof(uint32_t)) r.order.alloc(order.size() * sizeof(uint32_t));
+        check(hipMemcpyAsync(r.order.p, r.h_order.p, order.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, st),
+              "copy encode order");
+        r.n_tickets = (int32_t)order.size();
+        // Enough workgroups for twice the widest diagonal (the MBs that can
+        // run at once, and the next ones waiting), at most all that fit.
+        r.grid = std::max(1, std::min(ctx->enc_resident, 2 * widest + 8));
+        r.key = key;
+    }
+    // Counter block, then the per-MB words of every frame.
+    const size_t prog_words = kCounterWords + 2 * n_mb_total;
+    if (r.progress.n < prog_words * sizeof(int32_t)) r.progress.alloc(prog_words * sizeof(int32_t));
+    const size_t cos_bytes = (size_t)ctx->enc_resident * kEncodeCands * EVXA_MB_SAMPLES * sizeof(int16_t);
+    if (r.coscratch.n < cos_bytes) r.coscratch.alloc(cos_bytes);
+    int32_t *counters = r.progress.as<int32_t>();
+    uint64_t *words = reinterpret_cast<uint64_t *>(counters + kCounterWords);
+
+    std::vector<EncJob> jobs(n);
+    std::vector<int> outs(n);
+    for (int32_t k = 0; k < n; k++) {
+        evxa_stream *s = streams[k];
+        const evxa_encode_desc *d = &descs[k];
+        const int32_t n_refs = d->type == EVXA_FRAME_I ? 0 : d->type == EVXA_FRAME_P ? 1 : 2;
+        DevJob &j = jobs[k].job;
+        memset(&j, 0, sizeof(j));
+        j.src = s->pic(d->src_slot);
+        for (int r2 = 0; r2 < n_refs; r2++) j.refs[r2] = s->pic(d->ref_slots[r2]);
+        j.width = s->w;
+        j.height = s->h;
+        j.wb = s->wb;
+        j.hb = s->hb;
+        j.type = d->type;
+        j.quality = d->quality;
+        j.n_refs = n_refs;
+        j.thresh = (d->quality >> 2) + 1;   // motion.cpp:589
+        j.lambda = lambda_of(d->quality);
+        j.target_ids = (uint32_t)d->target_ids[0] | ((uint32_t)d->target_ids[1] << 8);
+        // The reference keeps one temporal table, the block table of the
+        // last non-B frame (encode.cpp:990-997); I and P frames write the P
+        // table in use, B frames their own.
+        outs[k] = d->type == EVXA_FRAME_B ? 2 : s->p_cur;
+        j.prev = (d->type == EVXA_FRAME_P && d->temporal_mvp && s->temporal_valid)
+                     ? s->table(s->p_cur ^ 1) : nullptr;
+        j.records = s->table(outs[k]);
+        j.coeffs = s->coeffs.as<int16_t>();
+        j.levels = s->levels.as<int16_t>();
+        j.mvw = words;
+        words += (size_t)s->wb * s->hb;
+        jobs[k].pic = s->pic(d->dst_slot);
+        jobs[k].spread = s->range_spread(d->dst_slot);
+    }
+    const size_t jb = jobs.size() * sizeof(EncJob);
+    r.h_jobs.ensure(jb);
+    memcpy(r.h_jobs.p, jobs.data(), jb);
+    if (r.jobs.n < jb) r.jobs.alloc(jb);
+    check(hipMemcpyAsync(r.jobs.p, r.h_jobs.p, jb, hipMemcpyHostToDevice, st), "copy encode jobs");
+    check(hipEventRecord(r.staged, st), "hipEventRecord");
+    for (int32_t k = 0; k < n; k++) range_begin(streams[k], descs[k].dst_slot);
+    check(launch_reset(counters, (int32_t)prog_words, st), "reset kernel");
+    if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
+    const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
+    check(hipEventRecord(ev.first, st), "hipEventRecord");
+    check(launch_encode(r.jobs.as<EncJob>(), r.order.as<uint32_t>(), r.n_tickets, counters,
+                        r.coscratch.as<int16_t>(), r.grid, st),
+          "encode kernel");
+    check(hipEventRecord(ev.second, st), "hipEventRecord");
+    ctx->ring_head++;
+    ctx->last_start = ev.first;
+    ctx->last_stop = ev.second;
+    ctx->scratch_last = false;
+    ctx->enc_last = counters;
+    for (int32_t k = 0; k < n; k++) {
+        evxa_stream *s = streams[k];
+        const evxa_encode_desc *d = &descs[k];
+        range_end(s, d->dst_slot);
+        s->last_table = outs[k];
+        s->last_job = jobs[k].job;
+        const int32_t n_refs = d->type == EVXA_FRAME_I ? 0 : d->type == EVXA_FRAME_P ? 1 : 2;
+        for (int r2 = 0; r2 < EVXA_MAX_REFS; r2++) s->last_refs[r2] = r2 < n_refs ? d->ref_slots[r2] : -1;
+        if (d->type != EVXA_FRAME_B && d->update_temporal) {
+            s->p_cur ^= 1;
+            s->temporal_valid = true;
+        }
+        mark_done(s, st);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -979,7 +1143,6 @@ int evxa_stream_close(evxa_stream *s)
         set_device(ctx);
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         ctx->used -= s->bytes;
-        if (s->enc.p && ctx->enc_last == s->enc.as<int32_t>()) ctx->enc_last = nullptr;
         ctx->streams.erase(std::remove(ctx->streams.begin(), ctx->streams.end(), s),
                            ctx->streams.end());
         for (auto &g : ctx->graph_streams)
@@ -1080,92 +1243,23 @@ int evxa_encode_frame(evxa_stream *s, const evxa_encode_desc *d)
         if (!s || !d) fail(EVXA_ERROR_INVALIDARG, "null argument");
         evxa_context *ctx = s->ctx;
         std::lock_guard<std::mutex> lk(ctx->mu);
-        if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
-        const int32_t n_refs = d->type == EVXA_FRAME_I ? 0 : d->type == EVXA_FRAME_P ? 1 : 2;
-        if (d->type != EVXA_FRAME_I && d->type != EVXA_FRAME_P && d->type != EVXA_FRAME_B)
-            fail(EVXA_ERROR_INVALIDARG, "frame type");
-        if (d->quality < 0 || d->quality > 63) fail(EVXA_ERROR_INVALIDARG, "quality must be 0..63");
-        if (s->w < 48 || s->h < 48) fail(EVXA_ERROR_INVALIDARG, "the encode needs frames of at least 48x48");
-        check_slot(s, d->src_slot);
-        check_slot(s, d->dst_slot);
-        if (d->dst_slot == d->src_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is the source");
-        for (int k = 0; k < n_refs; k++) {
-            check_slot(s, d->ref_slots[k]);
-            if (d->ref_slots[k] == d->dst_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is a reference");
-        }
-        set_device(ctx);
-        const size_t n_mb = (size_t)s->wb * s->hb;
-        const size_t words_off = 256, order_off = words_off + align_up(n_mb * 8, 256);
-        lazy_alloc(s, s->enc, order_off + n_mb * 4);
-        char *eb = static_cast<char *>(s->enc.p);
-        int32_t *counters = reinterpret_cast<int32_t *>(eb);
-        uint64_t *words = reinterpret_cast<uint64_t *>(eb + words_off);
-        uint32_t *order = reinterpret_cast<uint32_t *>(eb + order_off);
+        encode_launch(ctx, &s, d, 1);
+    });
+}
 
-        DevJob j;
-        memset(&j, 0, sizeof(j));
-        j.src = s->pic(d->src_slot);
-        for (int k = 0; k < n_refs; k++) j.refs[k] = s->pic(d->ref_slots[k]);
-        j.width = s->w;
-        j.height = s->h;
-        j.wb = s->wb;
-        j.hb = s->hb;
-        j.type = d->type;
-        j.quality = d->quality;
-        j.n_refs = n_refs;
-        j.thresh = (d->quality >> 2) + 1;   // motion.cpp:589
-        j.lambda = lambda_of(d->quality);
-        j.target_ids = (uint32_t)d->target_ids[0] | ((uint32_t)d->target_ids[1] << 8);
-        // The reference keeps one temporal table, the block table of the last
-        // non-B frame (encode.cpp:990-997); I and P frames write the P table
-        // in use, B frames their own.
-        const int out = d->type == EVXA_FRAME_B ? 2 : s->p_cur;
-        j.prev = (d->type == EVXA_FRAME_P && d->temporal_mvp && s->temporal_valid)
-                     ? s->table(s->p_cur ^ 1) : nullptr;
-        j.records = s->table(out);
-        j.coeffs = s->coeffs.as<int16_t>();
-        j.levels = s->levels.as<int16_t>();
-        j.mvw = words;
-        const DevPic pic = s->pic(d->dst_slot);
-
-        // Enough workgroups for twice the widest diagonal of the MB order
-        // (the MBs that can run at once), at most all that fit.
-        const int32_t slope = kEncodeLag + 1;
-        int32_t widest = 1;
-        for (int32_t v = 0; v < s->wb + slope * (s->hb - 1); v++) {
-            int32_t c = 0;
-            for (int32_t by = 0; by < s->hb; by++) c += (v - slope * by >= 0 && v - slope * by < s->wb) ? 1 : 0;
-            widest = std::max(widest, c);
+int evxa_encode_batch(evxa_context *ctx, evxa_stream *const *streams, const evxa_encode_desc *descs,
+                      int32_t n)
+{
+    return guarded([&] {
+        if (!ctx || n < 0 || (n > 0 && (!streams || !descs))) fail(EVXA_ERROR_INVALIDARG, "null argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (int32_t a = 0; a < n; a++) {
+            if (!streams[a] || streams[a]->ctx != ctx) fail(EVXA_ERROR_INVALIDARG, "foreign stream");
+            for (int32_t b = 0; b < a; b++)
+                if (streams[a] == streams[b])
+                    fail(EVXA_ERROR_INVALIDARG, "a stream appears twice in one batch");
         }
-        const int32_t grid = std::max(1, std::min(encode_resident_groups(ctx->device), 2 * widest + 8));
-
-        hipStream_t st = ctx->stream;
-        range_begin(s, d->dst_slot);
-        check(launch_reset(counters, (int32_t)(words_off / 4 + 2 * n_mb), st), "reset kernel");
-        if (!s->enc_order) {
-            check(launch_encode_order(s->wb, s->hb, order, st), "encode order kernel");
-            s->enc_order = true;
-        }
-        if (ctx->ring_head - ctx->ring_tail == evxa_context::kRing) ctx->ring_tail++;
-        const auto ev = ctx->ring[ctx->ring_head % evxa_context::kRing];
-        check(hipEventRecord(ev.first, st), "hipEventRecord");
-        check(launch_encode(j, pic, order, counters, s->range_spread(d->dst_slot), grid, st),
-              "encode kernel");
-        check(hipEventRecord(ev.second, st), "hipEventRecord");
-        ctx->ring_head++;
-        ctx->last_start = ev.first;
-        ctx->last_stop = ev.second;
-        range_end(s, d->dst_slot);
-        ctx->scratch_last = false;
-        ctx->enc_last = counters;
-        s->last_table = out;
-        s->last_job = j;
-        for (int k = 0; k < EVXA_MAX_REFS; k++) s->last_refs[k] = k < n_refs ? d->ref_slots[k] : -1;
-        if (d->type != EVXA_FRAME_B && d->update_temporal) {
-            s->p_cur ^= 1;
-            s->temporal_valid = true;
-        }
-        mark_done(s, st);
+        if (n > 0) encode_launch(ctx, streams, descs, n);
     });
 }
 

@@ -263,15 +263,25 @@ constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
 hipError_t noalloc_scratch(void **ptr);
 hipError_t launch_set_job(const DevJob &job, DevJob *dst, hipStream_t st);
 hipError_t launch_wave_list(int32_t wb, int32_t hb, int32_t n_refs, uint32_t *work, hipStream_t st);
-// The in-loop encode of one frame (encode_slice): `job` (type 0 I, 1 P with
-// one reference, 2 B; records, coefficients, levels and the per-MB words
-// `mvw`, zero at the start), `pic` the picture being reconstructed (written
-// in place, apron included), `order` the MB order encode_order writes,
-// counters[0] (zero at the start) the next ticket and counters[kCounterError]
-// error flags, `spread` the picture's spread range record (cleared first).
-hipError_t launch_encode_order(int32_t wb, int32_t hb, uint32_t *order, hipStream_t st);
-hipError_t launch_encode(const DevJob &job, const DevPic &pic, const uint32_t *order,
-                         int32_t *counters, int32_t *spread, int32_t grid, hipStream_t st);
+// One frame of an in-loop encode launch (encode_slice): `job` (type 0 I, 1
+// P with one reference, 2 B; records, coefficients, levels and the per-MB
+// words `mvw`, zero at the start), `pic` the picture being reconstructed
+// (written in place, apron included) and `spread` its spread range record
+// (cleared first).
+struct EncJob {
+    DevJob job;
+    DevPic pic;
+    int32_t *spread;
+};
+// Coefficient candidates each encode workgroup keeps in `coscratch`
+// (kEncodeCands x EVXA_MB_SAMPLES int16 per workgroup).
+constexpr int32_t kEncodeCands = 6;
+// The encode of the frames in `jobs` (device memory): `order` lists
+// n_tickets entries job << kMbBits | MB in dependency order; counters[0]
+// (zero at the start) is the next ticket, counters[kCounterError] error
+// flags.
+hipError_t launch_encode(const EncJob *jobs, const uint32_t *order, int32_t n_tickets,
+                         int32_t *counters, int16_t *coscratch, int32_t grid, hipStream_t st);
 // Workgroups of the encode kernel that fit the device at once.
 int32_t encode_resident_groups(int32_t device);
 // Rows of MBs an encode MB may lead the row below it by (its wavefront lag).

@@ -4224,7 +4224,7 @@ constexpr int kEncWaves = 4;
 // (forward) inter search's, 2 the backward search's (B) or the plain intra
 // block (P), 3..5 the directional modes (P: the three candidates; I and B:
 // the chosen one in 3).
-constexpr int kEncCands = 6;
+constexpr int kEncCands = kEncodeCands;
 constexpr int kSlotIntra = 0, kSlotInter = 1, kSlotPlain = 2, kSlotBwd = 2, kSlotMode = 3;
 constexpr int kSlotBidir = 4;         // B: the average of both searches
 
@@ -4246,17 +4246,16 @@ struct EncLds {
     IntraEdges edges;
     EncXchg x;
     CodeTables tables;
-    int16_t co[kEncCands][EVXA_MB_SAMPLES];   // per candidate: coefficients,
-    int16_t lv[kEncCands][EVXA_MB_SAMPLES];   // levels after the trellis,
-    int16_t rec[kEncCands][EVXA_MB_SAMPLES];  // reconstruction
+    int16_t lv[kEncCands][EVXA_MB_SAMPLES];   // per candidate: levels after the trellis,
+    int16_t rec[kEncCands][EVXA_MB_SAMPLES];  // reconstruction (coefficients: EncArgs::coscratch)
 };
 
 struct EncArgs {
-    DevJob job;                 // type 0 (I), 1 (P, one reference) or 2 (B)
-    DevPic pic;                 // the picture being reconstructed (its slot's previous content)
-    const uint32_t *order;      // MB indices in dependency order
+    const EncJob *jobs;         // the frames of the launch
+    const uint32_t *order;      // tickets: job << kMbBits | MB, in dependency order
     int32_t *counters;          // [0] next ticket, [kCounterError] error flags
-    int32_t *spread;            // the picture's spread range record
+    int32_t n_tickets;
+    int16_t *coscratch;         // per workgroup, kEncCands x 384 coefficients
 };
 
 // quantize.cpp:74-95: the DC scales of intra blocks.
@@ -4405,7 +4404,8 @@ __device__ void trellis_mb(const CodeTables &t, int qp, bool intra, int imode, f
 // The descriptor's residual coding (encode.cpp:717-739 with
 // quantize_macroblock and the trellis) and its reconstruction
 // (decode.cpp: inverse quantisation, inverse DCT added to the prediction):
-// from sh.mb.src and the prediction in sh.mb.pred into co/lv/rec of `slot`.
+// from sh.mb.src and the prediction in sh.mb.pred into co (global memory),
+// lv and rec (LDS).
 // Copies are not transformed (encode.cpp:742-744): zero coefficients and
 // levels, the prediction as the reconstruction.
 __device__ __forceinline__ void code_mb(const DevJob &job, const CodeTables &t, EncDesc &d,
@@ -4438,7 +4438,7 @@ __device__ __forceinline__ void code_mb(const DevJob &job, const CodeTables &t, 
         lv[i] = (int16_t)quant(t, sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra, chroma);
     }
     wave_sync();
-    trellis_mb(t, qp, intra, d.imode, job.lambda, co, lv);
+    trellis_mb(t, qp, intra, d.imode, job.lambda, mb.res, lv);
     // Inverse quantisation into mb.res, then the inverse DCT: vertical
     // pass (lane = block column) into tmp, horizontal pass plus the
     // prediction (lane = block row) into rec, stored as int16.
@@ -4787,10 +4787,11 @@ __device__ __forceinline__ float rdo_cost(const DevJob &job, const CodeTables &t
 // Stores the MB's reconstruction into the picture (and the picture's apron
 // next to it for MBs on the border) with device-scope stores, and grows
 // the picture's range record by it.
-__device__ __forceinline__ void store_reconstruction(const EncArgs &a, int bx, int by, const int16_t *rec)
+__device__ __forceinline__ void store_reconstruction(const EncJob &ej, int bx, int by,
+                                                     const int16_t *rec)
 {
-    const DevJob &job = a.job;
-    const DevPic &pic = a.pic;
+    const DevJob &job = ej.job;
+    const DevPic &pic = ej.pic;
     const int lane = lane_id();
     const int x = bx * 16, y = by * 16;
     int lo = 0x7FFFFFFF, hi = -0x7FFFFFFF - 1;
@@ -4812,7 +4813,7 @@ __device__ __forceinline__ void store_reconstruction(const EncArgs &a, int bx, i
         st_relaxed32(reinterpret_cast<int32_t *>(dst),
                      (int32_t)((uint32_t)(uint16_t)v0 | ((uint32_t)(uint16_t)v1 << 16)));
     }
-    note_range(a.spread, lo, hi);
+    note_range(ej.spread, lo, hi);
     if (bx > 0 && bx < job.wb - 1 && by > 0 && by < job.hb - 1) return;
     // The apron part this MB owns: its rows and columns extended to the
     // apron's edge on each border side it touches, the interior excluded;
@@ -5094,9 +5095,9 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
 // candidates w and w + 4) or the winner (I, B; wave 0), and wave 0 stores
 // the MB and publishes its word.  Inlined into the kernel, so that `sh` is
 // known to be LDS.
-__device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
+__device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, int mb, EncLds &sh)
 {
-    const DevJob &job = a.job;
+    const DevJob &job = ej.job;
     const int lane = lane_id();
     const int wv_id = __builtin_amdgcn_readfirstlane(wave_id());
     const int bx = mb % job.wb, by = mb / job.wb;
@@ -5174,13 +5175,13 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     const int cx0 = min(max((x >> 1) - 20, -kApron), (W >> 1) + kApron - kIwCS);
     const int cy0 = min(max((y >> 1) - 27, -kApron), (H >> 1) + kApron - kIwCR);
     dma_rows<kIwLS / 2, kEncWaves, kLoadDevice>(
-        reinterpret_cast<const uint32_t *>(a.pic.y + y0 * a.pic.py + x0), a.pic.py / 2, kIwLR,
+        reinterpret_cast<const uint32_t *>(ej.pic.y + y0 * ej.pic.py + x0), ej.pic.py / 2, kIwLR,
         reinterpret_cast<uint32_t *>(sh.iw.y));
     dma_rows<kIwCS / 2, kEncWaves, kLoadDevice>(
-        reinterpret_cast<const uint32_t *>(a.pic.u + cy0 * a.pic.pc + cx0), a.pic.pc / 2, kIwCR,
+        reinterpret_cast<const uint32_t *>(ej.pic.u + cy0 * ej.pic.pc + cx0), ej.pic.pc / 2, kIwCR,
         reinterpret_cast<uint32_t *>(sh.iw.u));
     dma_rows<kIwCS / 2, kEncWaves, kLoadDevice>(
-        reinterpret_cast<const uint32_t *>(a.pic.v + cy0 * a.pic.pc + cx0), a.pic.pc / 2, kIwCR,
+        reinterpret_cast<const uint32_t *>(ej.pic.v + cy0 * ej.pic.pc + cx0), ej.pic.pc / 2, kIwCR,
         reinterpret_cast<uint32_t *>(sh.iw.v));
     window_ready();
     const WinView wv{sh.iw.y, sh.iw.u, sh.iw.v, kIwLS, kIwCS, x0, y0, cx0, cy0};
@@ -5196,8 +5197,9 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
     const bool rdo = job.type == EVXA_FRAME_P;
     const int tgt0 = job.target_ids & 0xFF, tgt1 = (job.target_ids >> 8) & 0xFF;
     auto code_slot = [&](EncDesc d, int slot, bool predicted = false) {
-        const float cost = rdo_cost(job, sh.tables, d, x, y, wv, sh.edges, mine, sh.co[slot],
-                                    sh.lv[slot], sh.rec[slot], rdo, predicted);
+        int16_t *co = a.coscratch + ((size_t)blockIdx.x * kEncCands + slot) * EVXA_MB_SAMPLES;
+        const float cost = rdo_cost(job, sh.tables, d, x, y, wv, sh.edges, mine, co, sh.lv[slot],
+                                    sh.rec[slot], rdo, predicted);
         if (lane == 0) {
             sh.x.cost[slot] = cost;
             sh.x.qp[slot] = d.qp;
@@ -5384,7 +5386,7 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
 
     // Outputs: the reconstruction into the picture, coefficients, levels,
     // the record; then the MB's word, once every store has completed.
-    store_reconstruction(a, bx, by, sh.rec[slot]);
+    store_reconstruction(ej, bx, by, sh.rec[slot]);
     // The MB's word goes out once the reconstruction's stores have completed
     // (what later MBs read of this one); coefficients, levels and the record
     // are read after the launch.
@@ -5394,8 +5396,9 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
                                              mv_word(best.mvx, best.mvy)));
     int16_t *co = job.coeffs + (size_t)mb * EVXA_MB_SAMPLES;
     int16_t *lv = job.levels + (size_t)mb * EVXA_MB_SAMPLES;
+    const int16_t *cos = a.coscratch + ((size_t)blockIdx.x * kEncCands + slot) * EVXA_MB_SAMPLES;
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
-        co[i] = sh.co[slot][i];
+        co[i] = cos[i];
         lv[i] = sh.lv[slot][i];
     }
     uint32_t w[16];
@@ -5420,45 +5423,26 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, int mb, EncLds &sh)
 }
 
 // The persistent in-loop encode kernel: a workgroup of four waves takes the
-// next MB of the dependency order until none is left.  A MB waits only on
+// next MB of the launch's dependency order (the MBs of all its frames) until
+// none is left.  A MB waits only on
 // MBs earlier in the order, which running workgroups hold, so the launch
 // cannot deadlock whatever the number of resident workgroups.
 __global__ __launch_bounds__(64 * kEncWaves) void encode_kernel(const EncArgs a)
 {
     __shared__ EncLds sh;
     __shared__ int32_t s_word;
-    const int n_mb = a.job.wb * a.job.hb;
     load_code_tables(sh.tables);        // read after the claim's barrier
     for (;;) {
         if (threadIdx.x == 0) s_word = atomicAdd(&a.counters[0], 1);
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(s_word);
         __syncthreads();
-        if (t >= n_mb) break;
-        const int mb = (int)__builtin_amdgcn_readfirstlane((int)a.order[t]);
-        encode_mb(a, mb, sh);
+        if (t >= a.n_tickets) break;
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.order[t]);
+        encode_mb(a, a.jobs[e >> kMbBits], (int)(e & kMbMask), sh);
     }
 }
 
-// The encode's dependency order: MB (bx, by) at diagonal v = bx + (lag + 1) by,
-// MBs of a diagonal by row.  MB (bx, by) is preceded by the MBs (x, y) of
-// smaller diagonals -- min(max(v - (lag + 1) y, 0), wb) of them on row y --
-// and by those of diagonal v on rows y < by.  Its dependencies, MB (bx - 1,
-// by) and MB (min(bx + lag, wb - 1), by - 1), lie on smaller diagonals.
-__global__ void encode_order_kernel(int32_t wb, int32_t hb, uint32_t *order)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= wb * hb) return;
-    const int slope = kEncodeLag + 1;
-    const int bx = i % wb, by = i / wb, v = bx + slope * by;
-    int before = 0;
-    for (int y = 0; y < hb; y++) {
-        const int c = v - slope * y;
-        before += c <= 0 ? 0 : (c < wb ? c : wb);
-        if (y < by && c >= 0 && c < wb) before++;
-    }
-    order[before] = (uint32_t)i;
-}
 
 // ------------------------------------------------- allocation-free path --
 //
@@ -5708,22 +5692,15 @@ hipError_t launch_wave_list(int32_t wb, int32_t hb, int32_t n_refs, uint32_t *wo
     return hipGetLastError();
 }
 
-hipError_t launch_encode_order(int32_t wb, int32_t hb, uint32_t *order, hipStream_t st)
-{
-    const int n = wb * hb;
-    hipLaunchKernelGGL(encode_order_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wb, hb, order);
-    return hipGetLastError();
-}
-
-hipError_t launch_encode(const DevJob &job, const DevPic &pic, const uint32_t *order,
-                         int32_t *counters, int32_t *spread, int32_t grid, hipStream_t st)
+hipError_t launch_encode(const EncJob *jobs, const uint32_t *order, int32_t n_tickets,
+                         int32_t *counters, int16_t *coscratch, int32_t grid, hipStream_t st)
 {
     EncArgs a;
-    a.job = job;
-    a.pic = pic;
+    a.jobs = jobs;
     a.order = order;
     a.counters = counters;
-    a.spread = spread;
+    a.n_tickets = n_tickets;
+    a.coscratch = coscratch;
     hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(64 * kEncWaves), 0, st, a);
     return hipGetLastError();
 }

@@ -109,7 +109,8 @@ EXPORTS = (
     "evxa_search_blocks", "evxa_dct_quant_blocks", "evxa_clock", "evxa_server_create",
     "evxa_server_submit", "evxa_server_poll", "evxa_server_pending", "evxa_server_destroy",
     "evxa_set_tuning", "evxa_get_tuning", "evxa_sweep_stats", "evxa_analyze_frame_noalloc",
-    "evxa_stream_wait", "evxa_stream_query", "evxa_intra_search_blocks", "evxa_encode_frame")
+    "evxa_stream_wait", "evxa_stream_query", "evxa_intra_search_blocks", "evxa_encode_frame",
+    "evxa_encode_batch")
 
 
 class EvxaError(RuntimeError):
@@ -147,6 +148,7 @@ def load_library(path: str = LIB_PATH):
         "evxa_analyze_frame": (C.c_int, [vp, C.POINTER(FrameDesc)]),
         "evxa_analyze_frame_noalloc": (C.c_int, [vp, C.POINTER(FrameDesc)]),
         "evxa_encode_frame": (C.c_int, [vp, C.POINTER(EncodeDesc)]),
+        "evxa_encode_batch": (C.c_int, [vp, C.POINTER(vp), C.POINTER(EncodeDesc), C.c_int32]),
         "evxa_analyze_batch": (C.c_int, [vp, C.POINTER(vp), C.POINTER(FrameDesc), C.c_int32]),
         "evxa_synchronize": (C.c_int, [vp]),
         "evxa_stream_wait": (C.c_int, [vp]),
@@ -341,6 +343,14 @@ class Context:
         _check(_lib.evxa_analyze_batch(self.h, arr, d, n))
         for s, dd in zip(streams, descs):
             s._note(dd)
+
+    def encode_batch(self, streams, descs):
+        """evxa_encode_batch: the in-loop encode of one frame of each stream
+        in one launch."""
+        n = len(streams)
+        hs = (C.c_void_p * n)(*[s.h for s in streams])
+        ds = (EncodeDesc * n)(*descs)
+        _check(_lib.evxa_encode_batch(self.h, hs, ds, n))
 
     def graph_create(self, streams, descs) -> int:
         n = len(streams)

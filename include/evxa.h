@@ -286,6 +286,15 @@ int evxa_sao_filter(evxa_stream *s, int32_t slot, const evxa_sao_info *table);
  * call in engine_encode_frame (encode.cpp:958-962) and its row workers
  * (encode.cpp:817-941). */
 int evxa_encode_frame(evxa_stream *s, const evxa_encode_desc *desc);
+/* evxa_encode_frame for frames of `n` distinct streams in one launch: their
+ * MBs interleave in one dependency order (each frame's wavefront runs as it
+ * would alone), so the device holds the MBs of many frames at once.  Each
+ * stream's outputs are those of evxa_encode_frame; every descriptor is
+ * validated before anything is queued.  No counterpart in the reference
+ * (its encoder codes one frame at a time); for the many-encoder case of
+ * evxa_analyze_batch. */
+int evxa_encode_batch(evxa_context *ctx, evxa_stream *const *streams,
+                      const evxa_encode_desc *descs, int32_t n);
 
 /* ---- graph-captured steps (hipGraph of a fixed batch composition) ---- */
 

@@ -158,3 +158,38 @@ def test_encode_refusals(ctx):
             small.encode_frame(evxa.FRAME_I, 8, 0, 1)
     finally:
         small.close()
+
+
+def test_encode_batch_matches_alone(ctx, oracle):
+    """evxa_encode_batch: frames of streams of uneven sizes (and types) coded
+    in one launch give each stream's outputs of coding it alone."""
+    print("encode batch", flush=True)
+    sizes = [(128, 96), (64, 48), (320, 176), (96, 64)]
+    seqs = [synth.sequence_c1(100 + k, 3, w, h, local_range=10) for k, (w, h) in enumerate(sizes)]
+    stale = [tuple(np.full_like(p, 17 * k) for p in seqs[k][0]) for k in range(len(sizes))]
+    streams = []
+    try:
+        for k, (w, h) in enumerate(sizes):
+            s = ctx.open_stream(w, h, 6)
+            for f in range(3):
+                s.upload(f, *seqs[k][f])
+            for f in (3, 4, 5):
+                s.upload(f, *stale[k])
+            streams.append(s)
+        steps = [
+            [evxa.encode_desc(evxa.FRAME_I, 8, 0, 3) for _ in sizes],
+            [evxa.encode_desc(evxa.FRAME_P, 8, 1, 4, [3], target_ids=[5]) for _ in sizes],
+            [evxa.encode_desc(evxa.FRAME_B, 8, 2, 5, [3, 4], target_ids=[5, 6]) for _ in sizes],
+        ]
+        got = [{} for _ in sizes]
+        for t, descs in zip("IPB", steps):
+            ctx.encode_batch(streams, descs)
+            for k, s in enumerate(streams):
+                got[k][t] = (*s.results(), s.download(3 + "IPB".index(t)))
+        for k in range(len(sizes)):
+            alone = gpu_chain(ctx, seqs[k], stale[k], 8)
+            assert_chain_equal(got[k], alone, ("batch vs alone", k))
+        assert_chain_equal(got[2], oracle_chain(oracle, seqs[2], stale[2], 8), "batch vs oracle")
+    finally:
+        for s in streams:
+            s.close()
