@@ -4233,6 +4233,7 @@ struct EncXchg {
     SearchOut intra, inter[2];
     int32_t dir_mode, dir_sad, cands, pmx, pmy;
     int32_t preds;                        // B: bit k set once search k's prediction is in LDS
+    int32_t bsad;                         // B: SAD of the average of both predictions
     float cost[kEncCands];
     int32_t qp[kEncCands], var[kEncCands];
 };
@@ -4328,13 +4329,10 @@ __device__ __forceinline__ int dequant(const CodeTables &t, int level, int pos, 
 // Quantisation of one coefficient: quantize_intra_block_8x8
 // (quantize.cpp:120-200) or quantize_inter_block_8x8 (217-252).
 __device__ __forceinline__ int quant(const CodeTables &t, int sv, int pos, int bq, uint32_t mq,
-                                     bool intra, bool chroma)
+                                     bool intra, bool chroma, int dcs, uint32_t mdc)
 {
     if (intra) {
-        if (pos == 0) {
-            const int d = chroma ? chroma_dc_scale(bq) : luma_dc_scale(bq);
-            return (int16_t)rdiv_magic(sv, d, magic_of(d));
-        }
+        if (pos == 0) return (int16_t)rdiv_magic(sv, dcs, mdc);   // dcs: the DC scale, mdc its magic
         int qf = (int16_t)rdiv_magic(sv * 16, t.intra.qm[pos], t.intra.qm_magic[pos]);
         if (bq >= 16) {
             const int sg = (qf > 0) - (qf < 0);
@@ -4350,18 +4348,6 @@ __device__ __forceinline__ int quant(const CodeTables &t, int sv, int pos, int b
     return (int16_t)rdiv_magic(qf - dz, bq << 1, mq);
 }
 
-// The trellis's cost of `level` for coefficient `co` (quantize.cpp:
-// 536-560): squared error of the inverse-quantised level, as an int32 that
-// wraps, plus lambda times the level's bits; fp32 multiply then add.
-__device__ __forceinline__ float trellis_cost(const CodeTables &t, int co, int level, int pos,
-                                              int bq, bool intra, bool chroma, float lambda)
-{
-    const int deq = level == 0 ? 0 : dequant(t, level, pos, bq, intra, chroma);
-    const int diff = co - deq;
-    const int sq = (int)((uint32_t)diff * (uint32_t)diff);
-    return __fadd_rn(__int2float_rn(sq), __fmul_rn(lambda, __int2float_rn(coeff_bits(level))));
-}
-
 // trellis_optimize_macroblock (quantize.cpp:615-680) on the levels `lv` of
 // coefficients `co`, one 8x8 block at a time, lane p holding scan position
 // p.  The backward pass walks from the last nonzero level down to position
@@ -4375,28 +4361,53 @@ __device__ void trellis_mb(const CodeTables &t, int qp, bool intra, int imode, f
 {
     const int p = lane_id();
     const int cqp = min(qp + 2, 63);
+    // Scan position p's raster position is the same in every block; so are
+    // its matrix entry and whether it is the DC.
+    const int pos = scan_pos(t, imode, p);
+    const int qm = intra ? t.intra.qm[pos] : t.inter.qm[pos];
+    const bool dc_scaled = intra && pos == 0;
+    int c[6], level[6];
+#pragma unroll
     for (int blk = 0; blk < 6; blk++) {
         int origin, pitch;
         block_origin(blk, origin, pitch);
+        const int idx = origin + (pos >> 3) * pitch + (pos & 7);
+        c[blk] = co[idx];
+        level[blk] = lv[idx];
+    }
+#pragma unroll
+    for (int blk = 0; blk < 6; blk++) {
         const bool chroma = blk >= 4;
         const int bq = chroma ? cqp : qp;
-        const int pos = scan_pos(t, imode, p);
-        const int idx = origin + (pos >> 3) * pitch + (pos & 7);
-        const int c = co[idx];
-        int level = lv[idx];
-        const uint64_t nz = __ballot(level != 0);
+        const int dcs = chroma ? chroma_dc_scale(bq) : luma_dc_scale(bq);
+        // quantize.cpp:481-514 (the inverse quantisation the costs use) and
+        // 536-560.
+        auto cost = [&](int lvl) {
+            const int deq = lvl == 0 ? 0
+                          : dc_scaled ? (int)(int16_t)(lvl * dcs)
+                                      : (int)(int16_t)((2 * lvl * qm * bq) / 16);
+            const int diff = c[blk] - deq;
+            const int sq = (int)((uint32_t)diff * (uint32_t)diff);
+            return __fadd_rn(__int2float_rn(sq), __fmul_rn(lambda, __int2float_rn(coeff_bits(lvl))));
+        };
+        int lvl = level[blk];
+        const uint64_t nz = __ballot(lvl != 0);
         if (nz == 0) continue;
         const int last = 63 - __builtin_clzll(nz);
-        const float keep = trellis_cost(t, c, level, pos, bq, intra, chroma, lambda);
-        const float zero = trellis_cost(t, c, 0, pos, bq, intra, chroma, lambda);
-        const uint64_t kept = __ballot(level != 0 && p >= 1 && p <= last && !(zero <= keep));
+        const float keep = cost(lvl);
+        const float zero = cost(0);
+        const uint64_t kept = __ballot(lvl != 0 && p >= 1 && p <= last && !(zero <= keep));
         const int stop = kept ? 63 - __builtin_clzll(kept) : 0;
-        if (p >= 1 && p > stop) level = 0;
-        if (p >= 1 && level != 0) {
-            const int reduced = (int16_t)(level - ((level > 0) - (level < 0)));
-            if (trellis_cost(t, c, reduced, pos, bq, intra, chroma, lambda) < keep) level = reduced;
+        if (p >= 1 && p > stop) lvl = 0;
+        if (p >= 1 && lvl != 0) {
+            const int reduced = (int16_t)(lvl - ((lvl > 0) - (lvl < 0)));
+            if (cost(reduced) < keep) lvl = reduced;
         }
-        lv[idx] = (int16_t)level;
+        if (lvl != level[blk]) {
+            int origin, pitch;
+            block_origin(blk, origin, pitch);
+            lv[origin + (pos >> 3) * pitch + (pos & 7)] = (int16_t)lvl;
+        }
     }
     wave_sync();
 }
@@ -4429,13 +4440,17 @@ __device__ __forceinline__ void code_mb(const DevJob &job, const CodeTables &t, 
     const bool intra = intra_quant(d.btype);
     const int cqp = min(qp + 2, 63);
     const uint32_t m_luma = magic_of(qp << 1), m_chroma = magic_of(cqp << 1);
-#pragma unroll 1
-    for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+    const int dc_luma = luma_dc_scale(qp), dc_chroma = chroma_dc_scale(cqp);
+    const uint32_t mdc_luma = magic_of(dc_luma), mdc_chroma = magic_of(dc_chroma);
+#pragma unroll
+    for (int k = 0; k < EVXA_MB_SAMPLES / 64; k++) {
+        const int i = lane + 64 * k;
         bool chroma;
         const int pos = mb_pos(i, chroma);
         const int sv = mb.res[i];
         co[i] = (int16_t)sv;
-        lv[i] = (int16_t)quant(t, sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra, chroma);
+        lv[i] = (int16_t)quant(t, sv, pos, chroma ? cqp : qp, chroma ? m_chroma : m_luma, intra,
+                               chroma, chroma ? dc_chroma : dc_luma, chroma ? mdc_chroma : mdc_luma);
     }
     wave_sync();
     trellis_mb(t, qp, intra, d.imode, job.lambda, mb.res, lv);
@@ -4498,11 +4513,14 @@ __device__ uint32_t block_bits(const EncDesc &d, const int16_t *blk)
             cb += coeff_bits(v);
         }
     }
+    // nz (at most 64) and cb (at most 64 * 21) of a block packed in one
+    // word: one reduction per block.
     const int mine = ((lane >> 5) << 1) | ((lane >> 1) & 1);
+    const int packed = nz | (cb << 8);
 #pragma unroll
     for (int b = 0; b < 4; b++) {
-        const int n = wave_sum(mine == b ? nz : 0);
-        const int c = wave_sum(mine == b ? cb : 0);
+        const int v = wave_sum(mine == b ? packed : 0);
+        const int n = v & 0xFF, c = v >> 8;
         bits += (uint32_t)(bits_u8((uint32_t)n + 1) * 2 + 1 + c);
     }
     return bits;
@@ -5130,30 +5148,30 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, in
             p1 = prv[1];
         }
         // Wait for MB (bx - 1, by) and MB (min(bx + lag, wb - 1), by - 1);
-        // the rest of what this MB reads was done before them.
-        const int dep = lane == 0 ? (bx > 0 ? mb - 1 : -1)
-                      : lane == 1 ? (by > 0 ? (by - 1) * job.wb + min(bx + kEncodeLag, job.wb - 1) : -1)
-                      : -1;
-        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        int nap = 1;
-        bool timed_out = false;
-        for (;;) {
-            uint64_t w = 1ull << 63;
-            if (dep >= 0) w = ld_relaxed64(job.mvw + dep);
-            if (__ballot((w >> 63) == 0) == 0) break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > kEncWaitTicks) { timed_out = true; break; }
-            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
-            nap = min(nap * 2, kMaxNap);
-        }
-        if (timed_out && lane == 0) atomicOr(&a.counters[kCounterError], kErrWaitTimeout);
-        // The MV predictor's neighbours (left, top, top-right, top-left).
+        // the rest of what this MB reads was done before them.  Lanes 0..3
+        // poll the MV predictor's neighbours (left, top, top-right,
+        // top-left), lane 4 the MB the lag waits for, so the words the
+        // predictor reads are at hand when the wait ends.
         const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < job.wb;
         const bool has_tl = has_t && has_l;
         const int nidx = lane == 0 ? (has_l ? mb - 1 : -1)
                        : lane == 1 ? (has_t ? mb - job.wb : -1)
                        : lane == 2 ? (has_tr ? mb - job.wb + 1 : -1)
-                       : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1) : -1;
-        const uint64_t word = nidx >= 0 ? ld_relaxed64(job.mvw + nidx) : 0;
+                       : lane == 3 ? (has_tl ? mb - job.wb - 1 : -1)
+                       : lane == 4 ? (has_t ? (by - 1) * job.wb + min(bx + kEncodeLag, job.wb - 1) : -1)
+                       : -1;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        int nap = 1;
+        bool timed_out = false;
+        uint64_t word = 0;
+        for (;;) {
+            if (nidx >= 0 && (word >> 63) == 0) word = ld_relaxed64(job.mvw + nidx);
+            if (__ballot(nidx >= 0 && (word >> 63) == 0) == 0) break;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kEncWaitTicks) { timed_out = true; break; }
+            for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(1);
+            nap = min(nap * 2, kMaxNap);
+        }
+        if (timed_out && lane == 0) atomicOr(&a.counters[kCounterError], kErrWaitTimeout);
         int pmx = 0, pmy = 0;
         if (job.type != 0)
             mv_predictor(job, bx, by, lane_u64(word, 0), lane_u64(word, 1), lane_u64(word, 2),
@@ -5280,11 +5298,18 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, in
             }
             const EncDesc d = bidir_desc(sh.x.inter[0], sh.x.inter[1], x, y, tgt0, tgt1);
             const bool own = tgt1 != tgt0;   // else decode_block takes both from the first reference
-            if (own) {
-                for (int i = lane; i < EVXA_MB_SAMPLES; i += 64)
-                    mine.pred[i] = (int16_t)((sh.mb[1].pred_b[i] + sh.mb[2].pred_b[i] + 1) >> 1);
-                wave_sync();
+            // The average of the searches' predictions and its SAD against
+            // the source over Y, U and V (motion.cpp:778-813), which the
+            // selection compares.
+            int bsad = 0;
+            for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
+                const int av = (sh.mb[1].pred_b[i] + sh.mb[2].pred_b[i] + 1) >> 1;
+                bsad += abs(mine.src[i] - av);
+                if (own) mine.pred[i] = (int16_t)av;
             }
+            bsad = wave_sum(bsad);
+            if (lane == 0) sh.x.bsad = bsad;
+            wave_sync();
             code_slot(d, kSlotBidir, own);
         }
     } else if (rdo) {
@@ -5334,12 +5359,7 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, in
         n_dg += f.n_dg + b.n_dg;
         if (f.sad < best_sad) { best = desc_of(f, x, y, tgt0, 0); best_sad = f.sad; best_slot = kSlotInter; }
         if (b.sad < best_sad) { best = desc_of(b, x, y, tgt1, 1); best_sad = b.sad; best_slot = kSlotBwd; }
-        int bsad = 0;
-        for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
-            const int av = (sh.mb[1].pred_b[i] + sh.mb[2].pred_b[i] + 1) >> 1;
-            bsad += abs(mine.src[i] - av);
-        }
-        bsad = wave_sum(bsad);
+        const int bsad = sh.x.bsad;
         if (bsad + __float2int_rz(__fmul_rn(lambda, 8.0f)) < best_sad) {
             best = bidir_desc(f, b, x, y, tgt0, tgt1);
             best_sad = bsad;
