@@ -4115,14 +4115,15 @@ __global__ __launch_bounds__(64) void dct_quant_blocks_kernel(int32_t n, const i
 // above row y - 16, or on or left of column x - 16 (motion.cpp:361-365,
 // 433-438), within 15 samples of a position of the first 3x3 scan of step
 // 16, whose reach is column x + 16 and row y - 32; with the 6-tap apron its
-// reads stay within columns x - 33 .. x + 49 and rows y - 49 .. y + 33.  So
+// reads stay within columns x - 34 .. x + 49 and rows y - 50 .. y + 33.  So
 // the MB reads nothing right of MB bx + 3 in the row above, and of the rows
-// below only columns x - 33 .. x + 2, which no MB has written yet when the
+// below only columns x - 34 .. x + 2, which no MB has written yet when the
 // lag is 3 (row by + 1 has at most reached MB bx - 4).  Each MB therefore
 // sees the picture exactly as the sequential encoder does: reconstructed
 // samples above and to the left, the slot's previous content elsewhere.
+// The frames of a batch interleave their MBs in one such order.
 //
-// One wave per MB.  Pictures written in the launch move between workgroups
+// Four waves per MB (encode_mb).  Pictures written in the launch move between workgroups
 // (and XCDs) only through device-scope accesses: the reconstruction, and the
 // apron of MBs on the picture border, are stored with device-scope stores
 // that complete before the MB's predictor word is published, and a MB
