@@ -295,6 +295,7 @@ struct evxa_stream {
     std::atomic<int32_t> inflight{0};   // frames queued on a server and not yet complete
     int32_t last_table = -1;
     DevJob last_job{};     // the last analysed frame (for reconstruction)
+    bool last_encoded = false;   // the last frame came from the in-loop encode
     int32_t last_refs[EVXA_MAX_REFS] = {-1, -1, -1, -1};
     DevBuf recon_levels;   // levels handed in for reconstruction
     DevBuf deblock_src;    // unfiltered copy of the picture being deblocked (packed planes)
@@ -426,6 +427,7 @@ DevJob make_job(evxa_stream *s, const evxa_frame_desc &d, bool advance)
     if (advance) {
         s->last_table = out;
         s->last_job = j;
+        s->last_encoded = false;
         for (int k = 0; k < EVXA_MAX_REFS; k++) s->last_refs[k] = k < d.n_refs ? d.ref_slots[k] : -1;
         if (d.type == EVXA_FRAME_P && d.update_temporal) {
             s->p_cur ^= 1;
@@ -960,6 +962,7 @@ void encode_launch(evxa_context *ctx, evxa_stream *const *streams, const evxa_en
         range_end(s, d->dst_slot);
         s->last_table = outs[k];
         s->last_job = jobs[k].job;
+        s->last_encoded = true;
         const int32_t n_refs = d->type == EVXA_FRAME_I ? 0 : d->type == EVXA_FRAME_P ? 1 : 2;
         for (int r2 = 0; r2 < EVXA_MAX_REFS; r2++) s->last_refs[r2] = r2 < n_refs ? d->ref_slots[r2] : -1;
         if (d->type != EVXA_FRAME_B && d->update_temporal) {
@@ -1479,6 +1482,9 @@ void reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels, bool f
     evxa_context *ctx = s->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (s->last_table < 0) fail(EVXA_ERROR_INVALID_RESOURCE, "no frame analysed yet");
+    if (s->last_encoded)
+        fail(EVXA_ERROR_INVALID_RESOURCE,
+             "the last frame came from the in-loop encode, which reconstructs it itself");
     if (s->inflight.load() > 0) fail(EVXA_ERROR_INVALID_RESOURCE, "stream has a frame in flight");
     check_slot(s, dst_slot);
     const DevJob &j = s->last_job;

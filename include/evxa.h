@@ -225,6 +225,8 @@ int evxa_read_results(evxa_stream *s, evxa_block_record *records, int16_t *coeff
  * levels.  The slot then serves as a reference like an uploaded picture,
  * so the decoded picture buffer stays on the device.  dst_slot must not be
  * a reference of that frame, and its references need distinct target ids.
+ * After evxa_encode_frame, which reconstructs its frame itself, it fails
+ * with EVXA_ERROR_INVALID_RESOURCE.
  * Replaces the encoder's per-MB reconstruction call, decode_block after
  * quantisation (encode.cpp:806, 918 in the row workers). */
 int evxa_reconstruct(evxa_stream *s, int32_t dst_slot, const int16_t *levels);

@@ -152,6 +152,16 @@ def test_encode_refusals(ctx):
             s.encode_frame(evxa.FRAME_I, 64, 0, 1)
     finally:
         s.close()
+    # evxa_reconstruct after an encode (which reconstructs its frame itself)
+    s = ctx.open_stream(64, 48, 3)
+    try:
+        s.encode_frame(evxa.FRAME_I, 8, 0, 1)
+        with pytest.raises(evxa.EvxaError):
+            s.reconstruct(2)
+        s.deblock(1)                     # the encode's own descriptors
+        s.wait()
+    finally:
+        s.close()
     small = ctx.open_stream(32, 32, 2)
     try:
         with pytest.raises(evxa.EvxaError):
