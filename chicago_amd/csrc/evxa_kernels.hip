@@ -1964,6 +1964,15 @@ __global__ __launch_bounds__(64 * NW, 3) void analyze_kernel(const DevJob *__res
 
 constexpr uint64_t kSweepWaitTicks = 100000000ull;   // 1 s of s_memrealtime (100 MHz)
 constexpr int kSweepMaxNap = 4;                      // longest back-off, s_sleep(8) units
+// Workgroups past the first kTailActive poll a phase's flag kSweepFarNap
+// times less often, and sit out a phase with fewer items than their index:
+// thousands of waves polling one address, or bumping one counter, queue in
+// front of the few that still work.  One still sitting out after kJoinTicks
+// (200 us) claims like the others, so every item is taken whichever
+// workgroups run.
+constexpr int kSweepFarNap = 16;
+constexpr int kTailActive = 256;
+constexpr uint64_t kJoinTicks = 20000;
 
 __device__ __forceinline__ int32_t ld_relaxed32(const int32_t *p)
 {
@@ -1989,17 +1998,25 @@ __device__ __forceinline__ void stores_done()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// One lane waits until *flag is set; false after kSweepWaitTicks.
-__device__ bool wait_flag(const int32_t *flag)
+// One lane waits until *flag is set, with back-off up to max_nap units;
+// false after `limit` ticks.
+__device__ bool wait_flag(const int32_t *flag, int max_nap = kSweepMaxNap,
+                          uint64_t limit = kSweepWaitTicks)
 {
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     int nap = 1;
     while (ld_relaxed32(flag) == 0) {
-        if (__builtin_amdgcn_s_memrealtime() - t0 > kSweepWaitTicks) return false;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > limit) return false;
         for (int k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(8);
-        nap = min(nap * 2, kSweepMaxNap);
+        nap = min(nap * 2, max_nap);
     }
     return true;
+}
+
+// The back-off of this workgroup's waits between phases.
+__device__ __forceinline__ int phase_nap()
+{
+    return (int)blockIdx.x < kTailActive ? kSweepMaxNap : kSweepFarNap;
 }
 
 __device__ __forceinline__ uint64_t lane_u64(uint64_t w, int l)
@@ -2055,6 +2072,7 @@ enum { kClaimSweep0 = 0, kClaimCheck = 1, kClaimSearch = 2, kClaimExit = 4 };
 struct ClaimState {
     int phase;                  // 0: sweep 0, 1: check, 2: search, 3: done
     int s;                      // current sweep
+    int sit_out;                // this workgroup claims nothing in the phase (unless it joins in)
     int total_chunks;           // check phase: chunks over all (sub-)lists
     int chunks[kSubLists];      // check phase: chunks per (sub-)list
     int tickets[kSubLists];     // search phase: tickets per sub-list
@@ -2082,11 +2100,12 @@ __device__ void enter_check(ClaimState &c, const uint32_t *work, const SweepArgs
     }
     c.s = s;
     c.phase = any > 0 ? 1 : 3;
+    c.sit_out = (int)blockIdx.x >= max(kTailActive, c.total_chunks);
 }
 
 // Thread 0: the search phase of sweep c.s, or the end of the sweeps if its
 // check phase changed no predictor (then no word changed either).
-__device__ void enter_search(ClaimState &c, const SweepArgs &sa)
+__device__ void enter_search(ClaimState &c, const SweepArgs &sa, int kb)
 {
     int any = 0;
     for (int k = 0; k < kSubLists; k++) {
@@ -2094,6 +2113,7 @@ __device__ void enter_search(ClaimState &c, const SweepArgs &sa)
         any += c.tickets[k];
     }
     c.phase = any > 0 ? 2 : 3;
+    c.sit_out = (int)blockIdx.x >= max(kTailActive, (any + kb - 1) / kb);
 }
 
 // Thread 0, once the static lists of sweep 0 are all taken: count the
@@ -2180,7 +2200,7 @@ __device__ bool batch_item(ClaimState &c, uint32_t out[4])
 // unfinished item never waits on another.  Sweep 0's static lists and the
 // sub-lists are taken own first, then the others'; between phases it waits
 // for the finished flag of the last.
-__device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, int32_t *counters,
+__device__ __noinline__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, int32_t *counters,
                       const SweepArgs &sa, int home, uint32_t out[4])
 {
     if (batch_item(c, out)) return;
@@ -2210,11 +2230,17 @@ __device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, i
                 }
             }
             flush_sweep0(c, work, counters, sa);
-            if (!wait_flag(sa.sctr + kSwFlag)) break;
+            if (!wait_flag(sa.sctr + kSwFlag, phase_nap())) break;
             enter_check(c, work, sa, 1);
             continue;
         }
         if (c.phase == 1) {
+            const int32_t *flag = sa.sctr + (size_t)c.s * kSweepStride + kSwFlagC;
+            if (c.sit_out) {
+                c.sit_out = 0;
+                if (wait_flag(flag, kSweepFarNap, kJoinTicks)) enter_search(c, sa, kb);
+                continue;
+            }
             for (int i = 0; i < kSubLists; i++) {
                 const int k = (home + i) & (kSubLists - 1);
                 if (c.chunks[k] == 0) continue;
@@ -2228,8 +2254,14 @@ __device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, i
                     return;
                 }
             }
-            if (!wait_flag(sa.sctr + (size_t)c.s * kSweepStride + kSwFlagC)) break;
-            enter_search(c, sa);
+            if (!wait_flag(flag, phase_nap())) break;
+            enter_search(c, sa, kb);
+            continue;
+        }
+        const int32_t *flag = sa.sctr + (size_t)c.s * kSweepStride + kSwFlag;
+        if (c.sit_out) {
+            c.sit_out = 0;
+            if (wait_flag(flag, kSweepFarNap, kJoinTicks)) enter_check(c, work, sa, c.s + 1);
             continue;
         }
         for (int i = 0; i < kSubLists; i++) {
@@ -2245,7 +2277,7 @@ __device__ void claim(ClaimState &c, const DevJob *jobs, const uint32_t *work, i
                 break;
             }
         }
-        if (!wait_flag(sa.sctr + (size_t)c.s * kSweepStride + kSwFlag)) break;
+        if (!wait_flag(flag, phase_nap())) break;
         enter_check(c, work, sa, c.s + 1);
     }
     if (c.phase != 3) atomicOr(&counters[kCounterError], kErrWaitTimeout);
@@ -2643,6 +2675,7 @@ __global__ __launch_bounds__(64 * NW, 3) void sweep_kernel(const DevJob *__restr
     if (threadIdx.x == 0) {
         cs.phase = 0;
         cs.s = 0;
+        cs.sit_out = 0;
         cs.b_next = cs.b_end = 0;
         for (int l = 0; l < kLists; l++) cs.done0[l] = 0;
     }
