@@ -1057,6 +1057,10 @@ int evxa_create(int32_t device_ordinal, uint64_t hbm_budget_bytes, evxa_context 
             const int v = atoi(g);
             if (v >= 1 && v <= kMaxClaimBatch) ctx->claim_batch = v;
         }
+        if (const char *g = getenv("EVXA_WINDOW_REACH")) {
+            const int v = atoi(g);
+            if (v >= 1 && v <= 15) ctx->reach = v;
+        }
         *out = ctx.release();
     });
 }
