@@ -2714,7 +2714,7 @@ __global__ __launch_bounds__(64 * NW, 3) void sweep_kernel(const DevJob *__restr
 // (their decisions are then incomplete).
 constexpr int kFinalMbs = 4;
 
-__global__ __launch_bounds__(64 * kFinalMbs) void final_kernel(const DevJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * kFinalMbs, 8) void final_kernel(const DevJob *__restrict__ jobs,
                                                               const uint32_t *__restrict__ work,
                                                               const int32_t *counters)
 {
