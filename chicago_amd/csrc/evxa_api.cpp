@@ -842,6 +842,11 @@ void encode_launch(evxa_context *ctx, evxa_stream *const *streams, const evxa_en
             check_slot(s, d->ref_slots[r]);
             if (d->ref_slots[r] == d->dst_slot) fail(EVXA_ERROR_INVALIDARG, "dst_slot is a reference");
         }
+        // A target id names the decoder's picture-buffer slot, so two
+        // references share one only if they are the same picture (the
+        // reference encoder's targets are its slots, encode.cpp:397, 418).
+        if (n_refs == 2 && d->target_ids[0] == d->target_ids[1] && d->ref_slots[0] != d->ref_slots[1])
+            fail(EVXA_ERROR_INVALIDARG, "two reference slots under one target id");
     }
     set_device(ctx);
     auto &r = ctx->enc;

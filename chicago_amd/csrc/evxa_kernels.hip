@@ -5140,13 +5140,15 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
 // One MB of the in-loop encode (see the section comment), by the four waves
 // of the workgroup.  Wave 0 waits for the MB's dependencies; then all
 // waves stage the window of the picture being reconstructed, and the
-// independent parts run side by side: wave 0 the intra-frame block search,
-// wave 1 the (forward) inter search, wave 2 the backward one (B), wave 3 the
-// directional intra mode.  Every wave then makes the same selection from
-// what they left in LDS, codes the RDO candidates (P; wave w takes
-// candidates w and w + 4) or the winner (I, B; wave 0), and wave 0 stores
-// the MB and publishes its word.  Inlined into the kernel, so that `sh` is
-// known to be LDS.
+// independent parts run side by side, each followed by the coding of what
+// it found: wave 0 the intra-frame block search, wave 1 the (forward) inter
+// search, wave 2 the backward one (B) or, in P frames, the plain intra block
+// and the third mode candidate, wave 3 the directional intra modes (P: the
+// first two candidates; B: also the bidirectional average once both
+// searches are in).  Every wave then makes the same selection from what
+// they left in LDS; wave 0 takes the RDO choice among the coded candidates
+// (P) or the selected one (I, B), stores the MB and publishes its word.
+// Inlined into the kernel, so that `sh` is known to be LDS.
 __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, int mb, EncLds &sh)
 {
     const DevJob &job = ej.job;

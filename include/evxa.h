@@ -120,7 +120,9 @@ typedef struct evxa_encode_desc {
     int32_t dst_slot;             /* the picture-buffer slot reconstructed into; it holds
                                      its previous content on entry, as in the reference */
     int32_t ref_slots[2];         /* P: [0]; B: [0] forward, [1] backward */
-    uint8_t target_ids[2];        /* written to prediction_target(_b) */
+    uint8_t target_ids[2];        /* written to prediction_target(_b): the decoder's slot
+                                     of each reference, so a B frame's two are equal only
+                                     when ref_slots are (else EVXA_ERROR_INVALIDARG) */
     uint8_t reserved[2];
     int32_t temporal_mvp;         /* P: use the stream's temporal table if one exists */
     int32_t update_temporal;      /* I and P: this frame's records become the temporal

@@ -128,6 +128,38 @@ def test_encode_vs_oracle(ctx, oracle, kind, quality, w, h):
                        (kind, quality, w, h))
 
 
+def test_encode_b_one_picture_both_ways(ctx, oracle):
+    """A B frame whose forward and backward references are one slot under one
+    target id, against the oracle (which matches the reference build there)."""
+    print("encode B, one reference picture both ways", flush=True)
+    frames = synth.sequence_c1(77, 3, 96, 64, local_range=10)
+    rng = np.random.default_rng(78)
+    stale = random_pictures(rng, 96, 64, "u8")
+    s = ctx.open_stream(96, 64, 4)
+    try:
+        s.upload(0, *frames[0])
+        s.upload(1, *frames[2])
+        for k in (2, 3):
+            s.upload(k, *stale)
+        s.encode_frame(evxa.FRAME_I, 12, 0, 2)
+        gi = (*s.results(), s.download(2))
+        s.encode_frame(evxa.FRAME_B, 12, 1, 3, [2, 2], target_ids=[2, 2])
+        gb = (*s.results(), s.download(3))
+    finally:
+        s.close()
+    ri, li, pi, ci = oracle.encode_frame_coeffs(0, 12, Picture(*frames[0]), [], stale)
+    rb, lb, pb, cb = oracle.encode_frame_coeffs(2, 12, Picture(*frames[2]),
+                                                [Picture(*pi), Picture(*pi)], stale,
+                                                target_ids=[2, 2])
+    for tag, got, want in (("I", gi, (ri, ci, li, pi)), ("B", gb, (rb, cb, lb, pb))):
+        rec, co, lv, pic = got
+        for f in FIELDS:
+            assert np.array_equal(rec[f], want[0][f]), (tag, f)
+        assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2]), tag
+        for a, b in zip(pic, want[3]):
+            assert np.array_equal(a, b), tag
+
+
 def test_encode_720p_vs_oracle(ctx, oracle):
     """1280x720 (the CPU-runnable pair's size): I then P frames, every MB."""
     print("encode 1280x720", flush=True)
@@ -150,6 +182,9 @@ def test_encode_refusals(ctx):
             s.encode(evxa.encode_desc(3, 8, 0, 1))
         with pytest.raises(evxa.EvxaError):
             s.encode_frame(evxa.FRAME_I, 64, 0, 1)
+        # Two reference pictures under one target id (the decoder's slot).
+        with pytest.raises(evxa.EvxaError):
+            s.encode_frame(evxa.FRAME_B, 8, 0, 1, [2, 0], target_ids=[4, 4])
     finally:
         s.close()
     # evxa_reconstruct after an encode (which reconstructs its frame itself)

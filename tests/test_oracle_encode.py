@@ -172,6 +172,30 @@ def test_encode_vs_reference(oracle, ref, kind, quality):
     ref.encode_threads = 1
 
 
+@needs_ref
+@pytest.mark.parametrize("quality", [6, 16])
+def test_encode_b_one_picture_both_ways(oracle, ref, quality):
+    """A B frame whose forward and backward references are one picture under
+    one target id (the only way two references share an id: the id names
+    the decoder's slot): every field, level and sample as the reference's."""
+    rng = np.random.default_rng(quality + 900)
+    w, h = 96, 64
+    frames = synth.sequence_c1(int(rng.integers(1 << 30)), 3, w, h, local_range=10)
+    stale = random_pictures(rng, w, h, "u8")
+    out = {}
+    for name, enc in (("oracle", oracle), ("ref", ref)):
+        ri, li, pi = enc.encode_frame(0, quality, Picture(*frames[0]), [], stale)
+        rb, lb, pb = enc.encode_frame(2, quality, Picture(*frames[2]),
+                                      [Picture(*pi), Picture(*pi)], stale, target_ids=[2, 2])
+        out[name] = (rb, lb, pb)
+    a, b = out["oracle"], out["ref"]
+    for f in ENCODE_FIELDS:
+        assert np.array_equal(a[0][f], b[0][f]), (quality, f)
+    assert np.array_equal(a[1], b[1]), quality
+    for p, q in zip(a[2], b[2]):
+        assert np.array_equal(p, q), quality
+
+
 # ---- closed forms (no reference needed) ----
 
 def test_intra_prediction_closed_forms(oracle):
