@@ -110,6 +110,7 @@ def random_pictures(rng, w, h, kind):
 @pytest.mark.parametrize("kind,quality,w,h", [
     ("tex", 4, 128, 96), ("tex", 12, 128, 96), ("tex", 24, 128, 96),
     ("u8", 8, 128, 96), ("wide", 12, 128, 96),
+    ("tex", 0, 128, 96), ("wide", 40, 128, 96), ("u8", 63, 128, 96),
     ("tex", 8, 48, 48), ("tex", 8, 320, 176), ("tex", 20, 256, 64)])
 def test_encode_vs_oracle(ctx, oracle, kind, quality, w, h):
     """Fresh chains (natural motion, noise, 8-bit noise, values outside 8

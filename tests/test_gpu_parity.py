@@ -209,7 +209,8 @@ def _run_p(ctx, cur, refs, quality, prev_frames=()):
     return out
 
 
-@pytest.mark.parametrize("quality,noise", [(8, True), (12, True), (8, False), (13, False)])
+@pytest.mark.parametrize("quality,noise", [(8, True), (12, True), (8, False), (13, False),
+                                           (0, True), (63, True), (63, False)])
 def test_frame_p_single_ref(ctx, oracle, quality, noise):
     w, h = 160, 96
     ref, cur = _pair(100 + quality, w, h, noise)
@@ -222,13 +223,13 @@ def test_frame_p_single_ref(ctx, oracle, quality, noise):
         assert (rec["block_type"] & 4).any(), "expected copy blocks on noise-free content"
 
 
-@pytest.mark.parametrize("n_refs", [2, 4])
-def test_frame_p_multi_ref(ctx, oracle, n_refs):
+@pytest.mark.parametrize("n_refs,quality", [(2, 8), (4, 8), (3, 31), (4, 0)])
+def test_frame_p_multi_ref(ctx, oracle, n_refs, quality):
     frames = synth.sequence_c1(7, n_refs + 1, 160, 96, local_range=12)
     cur = frames[-1]
     refs = [frames[-2 - k] for k in range(n_refs)]
-    rec, co, lv = _run_p(ctx, cur, refs, 8)
-    orec, oco, olv = oracle.analyze_frame(1, 8, Picture(*cur), [Picture(*r) for r in refs])
+    rec, co, lv = _run_p(ctx, cur, refs, quality)
+    orec, oco, olv = oracle.analyze_frame(1, quality, Picture(*cur), [Picture(*r) for r in refs])
     assert_records_equal(rec, orec)
     assert np.array_equal(co, oco)
     assert np.array_equal(lv, olv)
@@ -254,7 +255,7 @@ def test_frame_p_temporal_chain(ctx, oracle):
     s.close()
 
 
-@pytest.mark.parametrize("quality", [9, 13])
+@pytest.mark.parametrize("quality", [9, 13, 0, 40, 63])
 def test_frame_b(ctx, oracle, quality):
     frames = synth.sequence_c1(31, 3, 160, 96, local_range=12)
     fwd, cur, bwd = frames

@@ -88,6 +88,25 @@ def test_sweep_b(sweep_ctx, oracle, waves, reads):
     s.close()
 
 
+@pytest.mark.parametrize("quality", [0, 31, 63])
+def test_sweep_extreme_qualities(sweep_ctx, oracle, quality):
+    """P (two references, then the temporal table) and B frames at the ends
+    of the quality range, where the copy threshold and lambda are extreme;
+    the sweeps as the model lists them."""
+    ctx = sweep_ctx
+    ctx.set_tuning(waves_per_search=1, mode=evxa.MODE_SWEEP, sweep_reads=evxa.SWEEP_READS_GUESS,
+                   sweep_wait_us=0)
+    frames = synth.sequence_c1(40 + quality, 4, 160, 96, local_range=12)
+    s = _open(ctx, frames)
+    s.analyze_p(quality, 2, [1, 0])
+    prev = _check_frame(ctx, oracle, s, 1, quality, frames[2], [frames[1], frames[0]], None, "P")
+    s.analyze_p(quality, 3, [2, 1])
+    _check_frame(ctx, oracle, s, 1, quality, frames[3], [frames[2], frames[1]], prev, "P temporal")
+    s.analyze_b(quality, 1, 0, 2, target_ids=[0, 1])
+    _check_frame(ctx, oracle, s, 2, quality, frames[1], [frames[0], frames[2]], None, "B")
+    s.close()
+
+
 def test_sweep_no_motion_single_sweep(sweep_ctx, oracle):
     """A frame equal to its reference: every MB is a copy without motion,
     so sweep 1's check of every MB finds no predictor changed and nothing is

@@ -140,7 +140,7 @@ def test_trellis_vs_reference(oracle, ref):
 
 @needs_ref
 @pytest.mark.parametrize("kind", ["tex", "u8", "wide"])
-@pytest.mark.parametrize("quality", [4, 12, 24])
+@pytest.mark.parametrize("quality", [0, 4, 12, 24, 40, 63])
 def test_encode_vs_reference(oracle, ref, kind, quality):
     """Fresh chains I -> P -> B at 128x96 (natural motion, noise and values
     outside 8 bits): every descriptor field, level and reconstructed sample."""

@@ -72,7 +72,9 @@ def test_inter_search_random(oracle, ref, quality):
                                                w["sad"])
 
 
-@pytest.mark.parametrize("ftype,nrefs,quality", [(1, 1, 8), (1, 3, 9), (1, 4, 14), (2, 2, 10)])
+@pytest.mark.parametrize("ftype,nrefs,quality", [(1, 1, 8), (1, 3, 9), (1, 4, 14), (2, 2, 10),
+                                                  (1, 2, 0), (1, 1, 63), (1, 4, 31), (2, 2, 40),
+                                                  (2, 2, 63)])
 def test_frames_random(oracle, ref, ftype, nrefs, quality):
     frames = synth.sequence_c1(1000 + nrefs * 7 + quality, 6, 144, 80, local_range=14)
     pics = [Picture(*f) for f in frames]
