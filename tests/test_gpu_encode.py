@@ -204,6 +204,23 @@ def test_encode_refusals(ctx):
             small.encode_frame(evxa.FRAME_I, 8, 0, 1)
     finally:
         small.close()
+    # evxa_encode_batch: one stream twice in a batch, a stream of another
+    # context.
+    s = ctx.open_stream(64, 48, 3)
+    other = evxa.Context(0)
+    t = other.open_stream(64, 48, 3)
+    try:
+        d = evxa.encode_desc(evxa.FRAME_I, 8, 0, 1)
+        with pytest.raises(evxa.EvxaError):
+            ctx.encode_batch([s, s], [d, d])
+        with pytest.raises(evxa.EvxaError):
+            ctx.encode_batch([s, t], [d, d])
+        ctx.encode_batch([s], [d])
+        s.wait()
+    finally:
+        t.close()
+        other.close()
+        s.close()
 
 
 def test_encode_batch_matches_alone(ctx, oracle):
