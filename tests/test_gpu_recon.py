@@ -47,6 +47,50 @@ def test_reconstruct_p(ctx, oracle, n_refs):
     s.close()
 
 
+TINY = [(16, 16), (32, 16), (16, 32), (48, 16), (16, 48), (64, 16)]
+MODES = [pytest.param(evxa.MODE_WAVEFRONT, id="wavefront"), pytest.param(evxa.MODE_SWEEP, id="sweep")]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("w,h", TINY)
+def test_tiny_frames(ctx, oracle, mode, w, h):
+    """Frames of one MB row or column (16x16 up to 64x16): P with two
+    references, then B, analysed in each mode and reconstructed, against
+    the oracle (which matches the reference build at these sizes)."""
+    print(f"tiny {w}x{h} mode {mode}", flush=True)
+    before = ctx.tuning()[0]
+    ctx.set_tuning(mode=mode)
+    try:
+        frames = synth.sequence_c1(w * 7 + h, 4, w, h, local_range=6)
+        pics = [Picture(*f) for f in frames]
+        s = ctx.open_stream(w, h, 5)
+        try:
+            for k, f in enumerate(frames):
+                s.upload(k, *f)
+            for q in (0, 8, 63):
+                for ftype, cur, refs in ((1, 3, [2, 1]), (2, 1, [0, 2])):
+                    if ftype == 1:
+                        s.analyze_p(q, cur, refs, temporal_mvp=0)
+                    else:
+                        s.analyze_b(q, cur, refs[0], refs[1], target_ids=[0, 1])
+                    rec, co, lv = s.results()
+                    want = oracle.analyze_frame(ftype, q, pics[cur], [pics[r] for r in refs],
+                                                target_ids=None if ftype == 1 else [0, 1])
+                    for f in rec.dtype.names:
+                        if not f.startswith("reserved"):
+                            assert np.array_equal(rec[f], want[0][f]), (w, h, q, ftype, f)
+                    assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2]), (w, h, q, ftype)
+                    s.reconstruct(4)
+                    got = s.download(4)
+                    planes = oracle.reconstruct_frame(ftype, q, pics[cur], [pics[r] for r in refs],
+                                                      rec, lv)
+                    _check_planes(got, planes, f"{w}x{h} q{q} type {ftype}")
+        finally:
+            s.close()
+    finally:
+        ctx.set_tuning(**before)
+
+
 def test_reconstruct_b(ctx, oracle):
     frames = synth.sequence_c1(47, 3, 160, 96, local_range=12)
     fwd, _, bwd = frames

@@ -124,6 +124,31 @@ def test_dequantize_idct_random(oracle, ref):
         assert np.array_equal(oracle.idct_add(co, pred), ref.idct_add(co, pred))
 
 
+@pytest.mark.parametrize("w,h", [(16, 16), (32, 16), (16, 32), (48, 16), (16, 48), (64, 16)])
+def test_frames_tiny(oracle, ref, w, h):
+    """Frames of one MB row or column: P with one and two references and B,
+    natural and 8-bit noise content, at qualities 0, 8 and 63."""
+    rng = np.random.default_rng(w * 3 + h)
+    for kind in ("tex", "noise"):
+        if kind == "tex":
+            fr = synth.sequence_c1(w * 7 + h, 4, w, h, local_range=6)
+        else:
+            fr = [tuple(rng.integers(0, 256, sh).astype(np.int16)
+                        for sh in ((h, w), (h // 2, w // 2), (h // 2, w // 2))) for _ in range(4)]
+        P = [Picture(*f) for f in fr]
+        for q in (0, 8, 63):
+            for ftype, cur, refs in ((1, 3, [P[2]]), (1, 3, [P[2], P[1]]), (2, 1, [P[0], P[2]])):
+                a = oracle.analyze_frame(ftype, q, P[cur], refs)
+                b = ref.analyze_frame(ftype, q, P[cur], refs)
+                for f in REF_FIELDS:
+                    assert np.array_equal(a[0][f], b[0][f]), (w, h, kind, q, ftype, len(refs), f)
+                assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), (w, h, kind, q)
+                got = oracle.reconstruct_frame(ftype, q, P[cur], refs, a[0], a[2])
+                want = ref.reconstruct_frame(ftype, q, P[cur], refs, a[0], a[2])
+                for g, wnt in zip(got, want):
+                    assert np.array_equal(g, wnt), (w, h, kind, q, ftype, "reconstruction")
+
+
 @pytest.mark.parametrize("ftype,nrefs,quality", [(1, 1, 8), (1, 3, 10), (2, 2, 12)])
 def test_reconstruct_frame_random(oracle, ref, ftype, nrefs, quality):
     """Reconstruction of an analysed frame, with its own levels and with
