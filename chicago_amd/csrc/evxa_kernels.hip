@@ -5147,7 +5147,8 @@ __device__ __forceinline__ SearchOut intra_search_win(const WinView &wv, int pw,
 // first two candidates; B: also the bidirectional average once both
 // searches are in).  Every wave then makes the same selection from what
 // they left in LDS; wave 0 takes the RDO choice among the coded candidates
-// (P) or the selected one (I, B), stores the MB and publishes its word.
+// (P) or the selected one (I, B; a B frame's intra-frame search candidate
+// is coded only then, if it won), stores the MB and publishes its word.
 // Inlined into the kernel, so that `sh` is known to be LDS.
 __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, int mb, EncLds &sh)
 {
@@ -5260,12 +5261,9 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, in
             sh.x.var[slot] = d.var;
         }
     };
-    if (wv_id == 0) {
-        // calculate_intra_prediction, for every frame type (encode.cpp:175,
-        // 362, 380).
-        const SearchOut ri = intra_search_win(wv, W, H, s, geo, x, y, thresh, lambda, sh.hy,
-                                              sh.hu, sh.hv);
-        if (lane == 0) sh.x.intra = ri;
+    // The intra-frame search's candidate, coded from the search's planes
+    // (they stay in LDS until the MB is stored).
+    auto code_intra = [&](const SearchOut &ri) {
         const EncDesc di = desc_of(ri, x, y, 0, 0);
         if (di.btype == EVXA_BLOCK_INTRA) {
             code_slot(di, kSlotIntra);                   // nothing found: the plain intra block
@@ -5273,6 +5271,18 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, in
             predict_intra_result(wv, ri, sh.hy, sh.hu, sh.hv, mine.pred);
             code_slot(di, kSlotIntra, true);
         }
+    };
+    // B frames code it only if it wins (about one MB in twenty): coded
+    // speculatively it would make wave 0, which searches longest, the last
+    // to reach the barrier.
+    const bool defer_intra = job.type == EVXA_FRAME_B;
+    if (wv_id == 0) {
+        // calculate_intra_prediction, for every frame type (encode.cpp:175,
+        // 362, 380).
+        const SearchOut ri = intra_search_win(wv, W, H, s, geo, x, y, thresh, lambda, sh.hy,
+                                              sh.hu, sh.hv);
+        if (lane == 0) sh.x.intra = ri;
+        if (!defer_intra) code_intra(ri);
     } else if (searcher) {
         const SearchOut r = inter_search<1, true>(job.refs[my_ref], W, H, narrowk, s, geo, pre, wk,
                                                   x, y, thresh, lambda, pmx, pmy, sh.sp[my_ref]);
@@ -5415,6 +5425,10 @@ __device__ __forceinline__ void encode_mb(const EncArgs &a, const EncJob &ej, in
                 if ((int)((cands >> (8 * c)) & 0xFF) == best.imode) { best_slot = kSlotMode + c; break; }
     }
     if (wv_id != 0) return;
+    if (defer_intra && best_slot == kSlotIntra) {
+        code_intra(ri);
+        wave_sync();                              // lane 0's cost, QP and variance
+    }
 
     // RDO for P frames (encode.cpp:274-348, EVX_ENABLE_RDO_COPY_EVAL and
     // EVX_ENABLE_RDO_INTRA_MODES): the SAD winner, the plain intra block
