@@ -20,9 +20,9 @@ def test_frame_p_single_ref(ctx2, oracle, quality, noise):
     base.test_frame_p_single_ref(ctx2, oracle, quality, noise)
 
 
-@pytest.mark.parametrize("n_refs", [2, 4])
-def test_frame_p_multi_ref(ctx2, oracle, n_refs):
-    base.test_frame_p_multi_ref(ctx2, oracle, n_refs)
+@pytest.mark.parametrize("n_refs,quality", [(2, 8), (4, 8), (3, 31), (4, 0)])
+def test_frame_p_multi_ref(ctx2, oracle, n_refs, quality):
+    base.test_frame_p_multi_ref(ctx2, oracle, n_refs, quality)
 
 
 @pytest.mark.parametrize("span", base.RANGE_CASES[1:])
@@ -34,7 +34,7 @@ def test_frame_p_temporal_chain(ctx2, oracle):
     base.test_frame_p_temporal_chain(ctx2, oracle)
 
 
-@pytest.mark.parametrize("quality", [9, 13])
+@pytest.mark.parametrize("quality", [9, 13, 63])
 def test_frame_b(ctx2, oracle, quality):
     base.test_frame_b(ctx2, oracle, quality)
 
