@@ -170,6 +170,16 @@ def test_encode_720p_vs_oracle(ctx, oracle):
     assert_chain_equal(got, oracle_chain(oracle, frames, stale, 8), "720p")
 
 
+def test_encode_1080p_vs_oracle(ctx, oracle):
+    """1920x1088 (the 1080p frames of the bench), I, P and B: every field,
+    coefficient, level and sample of all 8160 MBs."""
+    print("encode 1920x1088", flush=True)
+    frames = synth.sequence_c1(72, 3, 1920, 1088)
+    stale = tuple(np.zeros_like(p) for p in frames[0])
+    got = gpu_chain(ctx, frames, stale, 8)
+    assert_chain_equal(got, oracle_chain(oracle, frames, stale, 8), "1080p")
+
+
 def test_encode_refusals(ctx):
     """Bad jobs are refused before anything runs: the destination is the
     source or a reference, a frame type, a quality, a frame below 48x48."""
