@@ -1,8 +1,9 @@
 """Frame analysis at the benchmark workloads' full sizes, in both modes,
 bit-exact against the oracle: the 1280x720 single-reference P pair of the
 CPU-runnable workload, 1920x1080 P frames with four references and the
-temporal table (the plain GPU workload), and the first GOP-8 frames of the
-3840x2160 zoom sequence with hierarchical B (the GOP workload)."""
+temporal table (the plain GPU workload), the first GOP-8 frames of the
+3840x2160 zoom sequence with hierarchical B (the GOP workload), and the
+batched workload's 64-stream composition in one fused launch."""
 import numpy as np
 import pytest
 
@@ -110,3 +111,49 @@ def test_c2_4k_gop8_anchor_and_bs(ctx, c2_case, set_mode, mode):
             assert np.array_equal(co, want[1]) and np.array_equal(lv, want[2]), r.index
     finally:
         s.close()
+
+
+@pytest.fixture(scope="module")
+def c3_content():
+    """One 5-frame sequence per size of the batched workload's mix."""
+    return {(w, h): synth.sequence_c1(synth.seed_of(3, w, h), 5, w, h)
+            for (w, h) in ((640, 360), (1280, 720), (1920, 1080), (2560, 1440))}
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_c3_64_streams_batch_matches_alone(ctx, oracle, c3_content, set_mode, mode):
+    """The batched workload's composition (16x 640x360, 24x 1280x720, 16x
+    1920x1080, 8x 2560x1440; stream i with 1 + i mod 4 references) in one
+    fused launch: every stream's outputs equal its launch alone, and the
+    largest and smallest streams with four references equal the oracle."""
+    set_mode(mode)
+    mix = [(640, 360, 16), (1280, 720, 24), (1920, 1080, 16), (2560, 1440, 8)]
+    sizes = [(w, h) for w, h, n in mix for _ in range(n)]
+    streams, descs = [], []
+    try:
+        for i, (w, h) in enumerate(sizes):
+            frames = c3_content[(w, h)]
+            s = ctx.open_stream(synth.align16(w), synth.align16(h), 5)
+            for t, f in enumerate(frames):
+                s.upload(t, *f)
+            streams.append(s)
+            descs.append(evxa.frame_desc(evxa.FRAME_P, 8, 4, [3 - k for k in range(1 + i % 4)],
+                                         temporal_mvp=False, update_temporal=False))
+        print(f"64-stream batch, mode {mode}", flush=True)
+        ctx.analyze_batch(streams, descs)
+        batch = [s.results() for s in streams]
+        for i, (s, d) in enumerate(zip(streams, descs)):
+            s.analyze(d)
+            alone = s.results()
+            assert_records_equal(batch[i][0], alone[0])
+            assert np.array_equal(batch[i][1], alone[1]) and np.array_equal(batch[i][2], alone[2]), i
+        for i in (3, len(sizes) - 1):       # 640x360 and 2560x1440, four references each
+            w, h = sizes[i]
+            frames = c3_content[(w, h)]
+            want = oracle.analyze_frame(1, 8, Picture(*frames[4]),
+                                        [Picture(*frames[3 - k]) for k in range(4)])
+            assert_records_equal(batch[i][0], want[0])
+            assert np.array_equal(batch[i][1], want[1]) and np.array_equal(batch[i][2], want[2]), i
+    finally:
+        for s in streams:
+            s.close()
