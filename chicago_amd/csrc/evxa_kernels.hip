@@ -1220,6 +1220,9 @@ union alignas(16) Shared {
     SubpelLds sp;
     MbLds mb;
 };
+// The MB arrays overlap only the staged window: a search's half-pel planes
+// survive the selection that follows it.
+static_assert(sizeof(MbLds) <= offsetof(SubpelLds, y), "MB arrays reach the half-pel planes");
 
 // A window of a picture staged in LDS: its planes (row strides py, pc) and
 // where they start in the picture, luma (lx, ly) and chroma (cx, cy).  Reads
@@ -1642,14 +1645,49 @@ struct Sel {
     int n_int, n_ax, n_dg;
 };
 
+// The prediction of an inter search's result into `out` (predict_mb's
+// samples): for sub-pel results the half-pel planes the search built around
+// its integer best (averaged with the reference's integer block for
+// quarter-pel), else the reference's integer block.
+__device__ __forceinline__ void predict_inter_result(const DevPic &ref, const SearchOut &r,
+                                                     const SubpelLds &sp, int16_t *out)
+{
+    int dx = 0, dy = 0;
+    if (r.sp_en) frac_direction(r.sp_idx, dx, dy);
+    const PlaneView vy = plane_view<16>(sp.y, dx, dy), vu = plane_view<8>(sp.u, dx, dy),
+                    vv = plane_view<8>(sp.v, dx, dy);
+    for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64) {
+        const bool luma = i < 256;
+        const int k = luma ? i : (i - 256) & 63;
+        const int col = luma ? k & 15 : k & 7, row = luma ? k >> 4 : k >> 3;
+        const int16_t *pl = luma ? ref.y : (i < 320 ? ref.u : ref.v);
+        const int pitch = luma ? ref.py : ref.pc;
+        const int X = (luma ? r.best_x : r.best_x >> 1) + col;
+        const int Y = (luma ? r.best_y : r.best_y >> 1) + row;
+        const int full = pl[Y * pitch + X];
+        int v = full;
+        if (r.sp_en) {
+            const PlaneView &pv = luma ? vy : (i < 320 ? vu : vv);
+            const int half = pv.arr[pv.off + row * pv.stride + col];
+            v = r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half;
+        }
+        out[i] = (int16_t)v;
+    }
+    wave_sync();
+}
+
 // The two predictions of a B MB (motion.cpp:729-776) into sh.pred and
 // sh.pred_b; returns the SAD of their average against sh.src over Y, U and V
-// (motion.cpp:779-813).
-__device__ __forceinline__ int b_predictions(const DevJob &job, const SearchOut &f, const SearchOut &b, MbLds &sh)
+// (motion.cpp:779-813).  `planes_b`: the half-pel planes the backward search
+// left in LDS (past the MB arrays), which give its prediction without
+// interpolating again.
+__device__ __forceinline__ int b_predictions(const DevJob &job, const SearchOut &f, const SearchOut &b,
+                                             MbLds &sh, const SubpelLds *planes_b = nullptr)
 {
     const int lane = lane_id();
     predict_mb(job.refs[0], f, sh.pred, sh.tmp);
-    predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
+    if (planes_b) predict_inter_result(job.refs[1], b, *planes_b, sh.pred_b);
+    else predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
     int bsad = 0;
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         const int a = (sh.pred[i] + sh.pred_b[i] + 1) >> 1;
@@ -1674,7 +1712,8 @@ __device__ __forceinline__ void b_choose(const Sel &sel, MbLds &sh)
 // source MB is in sh.src.  For B frames the chosen prediction is left in
 // sh.pred.
 template <typename F>
-__device__ Sel select_mb(const DevJob &job, int x, int y, MbLds &sh, F result, int32_t *counters)
+__device__ Sel select_mb(const DevJob &job, int x, int y, MbLds &sh, F result, int32_t *counters,
+                         const SubpelLds *planes_b = nullptr)
 {
     const int lane = lane_id();
     const float lambda = job.lambda;
@@ -1708,7 +1747,7 @@ __device__ Sel select_mb(const DevJob &job, int x, int y, MbLds &sh, F result, i
         sel.n_int = f.n_int + b.n_int; sel.n_ax = f.n_ax + b.n_ax; sel.n_dg = f.n_dg + b.n_dg;
         if (f.sad < sel.best_sad) { sel.best = f; sel.best_k = 0; sel.best_sad = f.sad; }
         if (b.sad < sel.best_sad) { sel.best = b; sel.best_k = 1; sel.best_sad = b.sad; }
-        const int bsad = b_predictions(job, f, b, sh);
+        const int bsad = b_predictions(job, f, b, sh, planes_b);
         const int bias = __float2int_rz(__fmul_rn(lambda, 8.0f));
         if (bsad + bias < sel.best_sad) {
             sel.bidir = 1; sel.best_sad = bsad; sel.best = f; sel.best_k = 0; sel.other = b;
@@ -1802,14 +1841,15 @@ __device__ void emit_mb(const DevJob &job, int bx, int by, MbLds &sh, const Sel 
 // other references are polled from job.searches (they were claimed earlier,
 // so their waves are running) and cleared for the next launch.
 __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
-                          const Blk &src, const Geo &geo, const SearchOut &own)
+                          const Blk &src, const Geo &geo, const SearchOut &own,
+                          const SubpelLds *planes_b)
 {
     const int x = bx * 16, y = by * 16;
     const int idx = by * job.wb + bx;
     const uint64_t sw = gather_searches(job, idx, counters, job.eager != 0);
     auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
     store_src(src, geo, sh.src);
-    const Sel sel = select_mb(job, x, y, sh, result, counters);
+    const Sel sel = select_mb(job, x, y, sh, result, counters, planes_b);
     emit_mb(job, bx, by, sh, sel, true);
 }
 
@@ -1917,7 +1957,9 @@ __global__ __launch_bounds__(64 * NW, 3) void analyze_kernel(const DevJob *__res
             publish_search(own_words, o);
             try_publish(job, mb, x, y);
         }
-        finish_mb(job, bx, by, counters, sh.mb, s, geo, o);
+        // A B MB's wave is the backward search's: its planes are still in LDS.
+        finish_mb(job, bx, by, counters, sh.mb, s, geo, o,
+                  job.type == EVXA_FRAME_B ? &sh.sp : nullptr);
     }
 }
 
@@ -4764,36 +4806,6 @@ __device__ __forceinline__ void predict_intra_result(const WinView &wv, const Se
     wave_sync();
 }
 
-// The prediction of an inter search's result into `out` (predict_mb's
-// samples): for sub-pel results the half-pel planes the search built around
-// its integer best (averaged with the reference's integer block for
-// quarter-pel), else the reference's integer block.
-__device__ __forceinline__ void predict_inter_result(const DevPic &ref, const SearchOut &r,
-                                                     const SubpelLds &sp, int16_t *out)
-{
-    int dx = 0, dy = 0;
-    if (r.sp_en) frac_direction(r.sp_idx, dx, dy);
-    const PlaneView vy = plane_view<16>(sp.y, dx, dy), vu = plane_view<8>(sp.u, dx, dy),
-                    vv = plane_view<8>(sp.v, dx, dy);
-    for (int i = lane_id(); i < EVXA_MB_SAMPLES; i += 64) {
-        const bool luma = i < 256;
-        const int k = luma ? i : (i - 256) & 63;
-        const int col = luma ? k & 15 : k & 7, row = luma ? k >> 4 : k >> 3;
-        const int16_t *pl = luma ? ref.y : (i < 320 ? ref.u : ref.v);
-        const int pitch = luma ? ref.py : ref.pc;
-        const int X = (luma ? r.best_x : r.best_x >> 1) + col;
-        const int Y = (luma ? r.best_y : r.best_y >> 1) + row;
-        const int full = pl[Y * pitch + X];
-        int v = full;
-        if (r.sp_en) {
-            const PlaneView &pv = luma ? vy : (i < 320 ? vu : vv);
-            const int half = pv.arr[pv.off + row * pv.stride + col];
-            v = r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half;
-        }
-        out[i] = (int16_t)v;
-    }
-    wave_sync();
-}
 
 // The bidirectional descriptor of a B MB from its two searches
 // (encode.cpp:446-458; motion.cpp:702-840).
