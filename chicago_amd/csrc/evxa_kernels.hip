@@ -2419,7 +2419,8 @@ __device__ void push_dependants(const DevJob &job, int jidx, int mb, int s, cons
 // predictor's key.
 __device__ void decide_mb(const DevJob &job, int jidx, int mb, int s, const SweepArgs &sa,
                           MbLds &sh, const Blk &src, const Geo &geo, const SearchOut &own,
-                          uint64_t own_old, uint32_t key, int32_t *counters, int home)
+                          uint64_t own_old, uint32_t key, int32_t *counters, int home,
+                          const SubpelLds *planes_b)
 {
     const int lane = lane_id();
     const int x = (mb % job.wb) * 16, y = (mb / job.wb) * 16;
@@ -2427,7 +2428,7 @@ __device__ void decide_mb(const DevJob &job, int jidx, int mb, int s, const Swee
     const uint64_t sw = gather_searches(job, mb, counters, false);
     auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
     store_src(src, geo, sh.src);
-    const Sel sel = select_mb(job, x, y, sh, result, counters);
+    const Sel sel = select_mb(job, x, y, sh, result, counters, planes_b);
     const uint64_t w = sel_word(sel, x, y);
     // The decision (P: the selected search; B: both searches).
     const bool is_b = job.type != EVXA_FRAME_P;
@@ -2645,7 +2646,8 @@ __device__ void search_ticket(const DevJob &job, int jidx, int ref, int mb, int 
     own_old = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(own_old >> 32)) << 32) |
               (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)own_old);
     decide_mb(job, jidx, mb, s, sa, sh.mb, src, geo, o, own_old,
-              s == 0 ? pred_key(job, mb, pmx, pmy) : 0u, counters, home);
+              s == 0 ? pred_key(job, mb, pmx, pmy) : 0u, counters, home,
+              job.type == EVXA_FRAME_B ? &sh.sp : nullptr);   // the backward search's planes
     if (s == 0) {
         if (lane == 0) fetch_add(sa.sctr + kSwSearched, 1);
         return;
