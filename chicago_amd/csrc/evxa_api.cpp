@@ -2199,7 +2199,7 @@ int evxa_sweep_stats(evxa_context *ctx, int32_t *listed, int32_t *searched, floa
             if (s > 1 && l == 0) break;
             if (n < max) {
                 listed[n] = l;
-                searched[n] = b[kSwSearched];
+                searched[n] = s == 0 ? r.cap_check : b[kSwSearched];   // sweep 0 searches every MB
                 if (done_us) {
                     uint64_t t1;
                     memcpy(&t1, b + kSwTime, 8);

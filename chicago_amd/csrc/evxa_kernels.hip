@@ -2648,10 +2648,9 @@ __device__ void search_ticket(const DevJob &job, int jidx, int ref, int mb, int 
     decide_mb(job, jidx, mb, s, sa, sh.mb, src, geo, o, own_old,
               s == 0 ? pred_key(job, mb, pmx, pmy) : 0u, counters, home,
               job.type == EVXA_FRAME_B ? &sh.sp : nullptr);   // the backward search's planes
-    if (s == 0) {
-        if (lane == 0) fetch_add(sa.sctr + kSwSearched, 1);
-        return;
-    }
+    // Sweep 0 searches every MB: its count is the launch's MB count, and no
+    // ticket pays an atomic on one shared address for it.
+    if (s == 0) return;
     stores_done();
     if (lane == 0) {
         int32_t *c = sub_ctr(sa, s, k);
