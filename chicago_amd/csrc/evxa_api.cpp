@@ -180,6 +180,7 @@ struct LaunchRes {
     DevBuf jobs, tickets, progress, counters;
     DevBuf state;                    // sweep mode: per-MB words, keys, predictors and decisions
     DevBuf lists;                    // sweep mode: the dynamic check and search lists (zero between launches)
+    DevBuf fpred;                    // MB wavefront: the B jobs' forward predictions
     PinnedBuf h_jobs, h_tickets;
     hipEvent_t staged = nullptr;     // host staging may be reused after this
     hipEvent_t g_start = nullptr, g_stop = nullptr;   // captured timing (graphs)
@@ -715,10 +716,23 @@ void enqueue(evxa_context *ctx, LaunchRes &r, std::vector<DevJob> &jobs, bool ti
         }
     }
     int32_t *prog = r.progress.as<int32_t>();
+    // MB wavefront, B frames: the forward search's wave stores its prediction
+    // for the deciding wave, which then needs no interpolation of its own.
+    size_t fpred_bytes = 0;
+    if (!r.sweep)
+        for (const auto &j : jobs)
+            if (j.type == EVXA_FRAME_B) fpred_bytes += (size_t)j.wb * j.hb * EVXA_MB_SAMPLES * sizeof(int16_t);
+    if (r.fpred.n < fpred_bytes) r.fpred.alloc(fpred_bytes);
+    int16_t *fpred = r.fpred.as<int16_t>();
     int32_t off = kCounterWords;
     char *state = r.state.as<char>();
     for (auto &j : jobs) {
         const size_t n_mb = (size_t)j.wb * j.hb;
+        j.fpred = nullptr;
+        if (!r.sweep && j.type == EVXA_FRAME_B) {
+            j.fpred = fpred;
+            fpred += n_mb * EVXA_MB_SAMPLES;
+        }
         if (r.sweep) {
             j.mvw = nullptr;
             const int32_t nl = (int32_t)align_up(n_mb, 2);

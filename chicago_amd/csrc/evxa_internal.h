@@ -75,6 +75,8 @@ struct DevJob {
     int16_t *coeffs;
     int16_t *levels;
     SearchRec *searches;               // EVXA_MAX_REFS per MB
+    int16_t *fpred;                    // MB wavefront, B frames: per MB the forward search's prediction
+                                       // (EVXA_MB_SAMPLES), stored before its result; else nullptr
     uint64_t *mvw;                     // per MB: predictor word, zeroed per launch
     int32_t eager;                     // P with several references, alone in its launch:
                                        // every reference's wave tries the selection

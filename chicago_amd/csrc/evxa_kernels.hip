@@ -901,6 +901,25 @@ __device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
     __hip_atomic_store(EVXA_GLOBAL(uint64_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ int32_t ld_relaxed32(const int32_t *p)
+{
+    return __hip_atomic_load(EVXA_GLOBAL(const int32_t, p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed32(int32_t *p, int32_t v)
+{
+    __hip_atomic_store(EVXA_GLOBAL(int32_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every store of this wave has completed (device-scope stores are then
+// visible to every agent-scope load).
+__device__ __forceinline__ void stores_done()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // A MB's predictor word: bit 63 = published, bits 32..39 = the block type's
 // motion flag (the only type bit the MV predictor reads), bits 0..31 = the
 // integer MV (x low, y high; record word 1).  Zero until published.
@@ -1223,6 +1242,8 @@ union alignas(16) Shared {
 // The MB arrays overlap only the staged window: a search's half-pel planes
 // survive the selection that follows it.
 static_assert(sizeof(MbLds) <= offsetof(SubpelLds, y), "MB arrays reach the half-pel planes");
+static_assert(sizeof(MbLds) <= offsetof(SubpelLds, win) + offsetof(Window, u),
+              "MB arrays reach the window's chroma area");
 
 // A window of a picture staged in LDS: its planes (row strides py, pc) and
 // where they start in the picture, luma (lx, ly) and chroma (cx, cy).  Reads
@@ -1311,6 +1332,33 @@ __device__ void predict_mb(const Pic &ref, const SearchOut &r, int16_t *out, int
         out[i] = (int16_t)(r.sp_amt ? (int16_t)((full + half + 1) >> 1) : half);
     }
     wave_sync();
+}
+
+// predict_mb of a device picture over a copy of the region it reads, staged
+// into LDS (`stage`, kPredStage samples) by one DMA round instead of rounds
+// of 2-byte loads from the picture: luma rows and columns -3 .. +18 of the
+// result's integer position, chroma -3 .. +10 of its half (the 6-tap reach
+// in every direction), from even columns so that rows copy as dwords; the
+// picture's 24-sample apron holds every row.
+constexpr int kPredLS = 24, kPredLR = 22, kPredCS = 16, kPredCR = 14;
+constexpr int kPredStage = kPredLS * kPredLR + 2 * kPredCS * kPredCR;
+static_assert(kPredStage <= kWinCR * kWinCS, "a window's chroma plane holds the staging");
+__device__ void predict_mb_lds(const DevPic &ref, const SearchOut &r, int16_t *out, int16_t *tmp,
+                               int16_t *stage)
+{
+    const int lx = (r.best_x - 3) & ~1, ly = r.best_y - 3;
+    const int cx = ((r.best_x >> 1) - 3) & ~1, cy = (r.best_y >> 1) - 3;
+    int16_t *sy = stage, *su = stage + kPredLS * kPredLR, *sv = su + kPredCS * kPredCR;
+    dma_rows<kPredLS / 2, 1>(reinterpret_cast<const uint32_t *>(ref.y + ly * ref.py + lx), ref.py / 2,
+                             kPredLR, reinterpret_cast<uint32_t *>(sy));
+    dma_rows<kPredCS / 2, 1>(reinterpret_cast<const uint32_t *>(ref.u + cy * ref.pc + cx), ref.pc / 2,
+                             kPredCR, reinterpret_cast<uint32_t *>(su));
+    dma_rows<kPredCS / 2, 1>(reinterpret_cast<const uint32_t *>(ref.v + cy * ref.pc + cx), ref.pc / 2,
+                             kPredCR, reinterpret_cast<uint32_t *>(sv));
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0): the copies have landed
+    wave_sync();
+    const WinView wv{sy, su, sv, kPredLS, kPredCS, lx, ly, cx, cy};
+    predict_mb(wv, r, out, tmp);
 }
 
 // transform.cpp:264-284 on a line of 8 samples.  The 8-term products fold
@@ -1678,16 +1726,31 @@ __device__ __forceinline__ void predict_inter_result(const DevPic &ref, const Se
 
 // The two predictions of a B MB (motion.cpp:729-776) into sh.pred and
 // sh.pred_b; returns the SAD of their average against sh.src over Y, U and V
-// (motion.cpp:779-813).  `planes_b`: the half-pel planes the backward search
-// left in LDS (past the MB arrays), which give its prediction without
-// interpolating again.
+// (motion.cpp:779-813).  `search`: the LDS the backward search just used
+// (the MB arrays overlap only the start of its window): its half-pel planes
+// give the backward prediction without interpolating again, and its
+// window's chroma area stages the forward prediction's reads.  `fpred`: the
+// forward prediction as the forward search's wave stored it, else nullptr.
 __device__ __forceinline__ int b_predictions(const DevJob &job, const SearchOut &f, const SearchOut &b,
-                                             MbLds &sh, const SubpelLds *planes_b = nullptr)
+                                             MbLds &sh, SubpelLds *search = nullptr,
+                                             const int16_t *fpred = nullptr)
 {
     const int lane = lane_id();
-    predict_mb(job.refs[0], f, sh.pred, sh.tmp);
-    if (planes_b) predict_inter_result(job.refs[1], b, *planes_b, sh.pred_b);
-    else predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
+    if (search) {
+        if (fpred) {
+            const int32_t *src = reinterpret_cast<const int32_t *>(fpred);
+            int32_t *dst = reinterpret_cast<int32_t *>(sh.pred);
+#pragma unroll
+            for (int k = 0; k < EVXA_MB_SAMPLES / 128; k++) dst[lane + 64 * k] = ld_relaxed32(src + lane + 64 * k);
+            wave_sync();
+        } else {
+            predict_mb_lds(job.refs[0], f, sh.pred, sh.tmp, search->win.u);
+        }
+        predict_inter_result(job.refs[1], b, *search, sh.pred_b);
+    } else {
+        predict_mb(job.refs[0], f, sh.pred, sh.tmp);
+        predict_mb(job.refs[1], b, sh.pred_b, sh.tmp);
+    }
     int bsad = 0;
     for (int i = lane; i < EVXA_MB_SAMPLES; i += 64) {
         const int a = (sh.pred[i] + sh.pred_b[i] + 1) >> 1;
@@ -1713,7 +1776,7 @@ __device__ __forceinline__ void b_choose(const Sel &sel, MbLds &sh)
 // sh.pred.
 template <typename F>
 __device__ Sel select_mb(const DevJob &job, int x, int y, MbLds &sh, F result, int32_t *counters,
-                         const SubpelLds *planes_b = nullptr)
+                         SubpelLds *search = nullptr, const int16_t *fpred = nullptr)
 {
     const int lane = lane_id();
     const float lambda = job.lambda;
@@ -1747,7 +1810,7 @@ __device__ Sel select_mb(const DevJob &job, int x, int y, MbLds &sh, F result, i
         sel.n_int = f.n_int + b.n_int; sel.n_ax = f.n_ax + b.n_ax; sel.n_dg = f.n_dg + b.n_dg;
         if (f.sad < sel.best_sad) { sel.best = f; sel.best_k = 0; sel.best_sad = f.sad; }
         if (b.sad < sel.best_sad) { sel.best = b; sel.best_k = 1; sel.best_sad = b.sad; }
-        const int bsad = b_predictions(job, f, b, sh, planes_b);
+        const int bsad = b_predictions(job, f, b, sh, search, fpred);
         const int bias = __float2int_rz(__fmul_rn(lambda, 8.0f));
         if (bsad + bias < sel.best_sad) {
             sel.bidir = 1; sel.best_sad = bsad; sel.best = f; sel.best_k = 0; sel.other = b;
@@ -1842,14 +1905,15 @@ __device__ void emit_mb(const DevJob &job, int bx, int by, MbLds &sh, const Sel 
 // so their waves are running) and cleared for the next launch.
 __device__ void finish_mb(const DevJob &job, int bx, int by, int32_t *counters, MbLds &sh,
                           const Blk &src, const Geo &geo, const SearchOut &own,
-                          const SubpelLds *planes_b)
+                          SubpelLds *search)
 {
     const int x = bx * 16, y = by * 16;
     const int idx = by * job.wb + bx;
     const uint64_t sw = gather_searches(job, idx, counters, job.eager != 0);
     auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
     store_src(src, geo, sh.src);
-    const Sel sel = select_mb(job, x, y, sh, result, counters, planes_b);
+    const Sel sel = select_mb(job, x, y, sh, result, counters, search,
+                              job.fpred ? job.fpred + (size_t)idx * EVXA_MB_SAMPLES : nullptr);
     emit_mb(job, bx, by, sh, sel, true);
 }
 
@@ -1948,7 +2012,17 @@ __global__ __launch_bounds__(64 * NW, 3) void analyze_kernel(const DevJob *__res
             reinterpret_cast<uint64_t *>(job.searches + (size_t)mb * EVXA_MAX_REFS + ref);
         if (ref < job.n_refs - 1) {
             // Hand the result to the wave of the MB's last reference (a later
-            // ticket of the same list, so never one this wave waits on).
+            // ticket of the same list, so never one this wave waits on); in
+            // B frames its prediction first, from this search's planes (the
+            // MB arrays overlap only the window), complete before the result.
+            if (job.fpred) {
+                predict_inter_result(rp, o, sh.sp, sh.mb.pred);
+                const int32_t *src = reinterpret_cast<const int32_t *>(sh.mb.pred);
+                int32_t *dst = reinterpret_cast<int32_t *>(job.fpred + (size_t)mb * EVXA_MB_SAMPLES);
+#pragma unroll
+                for (int k = 0; k < EVXA_MB_SAMPLES / 128; k++) st_relaxed32(dst + lane + 64 * k, src[lane + 64 * k]);
+                stores_done();
+            }
             publish_search(own_words, o);
             if (job.eager) try_publish(job, mb, x, y);
             continue;
@@ -2016,29 +2090,12 @@ constexpr int kSweepFarNap = 16;
 constexpr int kTailActive = 256;
 constexpr uint64_t kJoinTicks = 20000;
 
-__device__ __forceinline__ int32_t ld_relaxed32(const int32_t *p)
-{
-    return __hip_atomic_load(EVXA_GLOBAL(const int32_t, p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void st_relaxed32(int32_t *p, int32_t v)
-{
-    __hip_atomic_store(EVXA_GLOBAL(int32_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __device__ __forceinline__ int32_t fetch_add(int32_t *p, int32_t v)
 {
     return __hip_atomic_fetch_add(EVXA_GLOBAL(int32_t, p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Every store of this wave has completed (device-scope stores are then
-// visible to every agent-scope load).
-__device__ __forceinline__ void stores_done()
-{
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0)
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
 
 // One lane waits until *flag is set, with back-off up to max_nap units;
 // false after `limit` ticks.
@@ -2420,7 +2477,7 @@ __device__ void push_dependants(const DevJob &job, int jidx, int mb, int s, cons
 __device__ void decide_mb(const DevJob &job, int jidx, int mb, int s, const SweepArgs &sa,
                           MbLds &sh, const Blk &src, const Geo &geo, const SearchOut &own,
                           uint64_t own_old, uint32_t key, int32_t *counters, int home,
-                          const SubpelLds *planes_b)
+                          SubpelLds *search)
 {
     const int lane = lane_id();
     const int x = (mb % job.wb) * 16, y = (mb / job.wb) * 16;
@@ -2428,7 +2485,7 @@ __device__ void decide_mb(const DevJob &job, int jidx, int mb, int s, const Swee
     const uint64_t sw = gather_searches(job, mb, counters, false);
     auto result = [&](int k) { return k == job.n_refs - 1 ? own : unpack_search(sw, k); };
     store_src(src, geo, sh.src);
-    const Sel sel = select_mb(job, x, y, sh, result, counters, planes_b);
+    const Sel sel = select_mb(job, x, y, sh, result, counters, search);
     const uint64_t w = sel_word(sel, x, y);
     // The decision (P: the selected search; B: both searches).
     const bool is_b = job.type != EVXA_FRAME_P;
