@@ -1266,11 +1266,14 @@ __device__ __forceinline__ void view_origin(const WinView &w, int &lx, int &ly, 
 }
 
 // Residual prediction for a search result into `out` (encode.cpp:696-723;
-// interp.cpp:166-202 when sub-pel).  `tmp` holds the diagonal H pass.
+// interp.cpp:166-202 when sub-pel).  `tmp` holds the diagonal H pass.  Neither
+// overlaps the picture or window read (restrict: the loads of later samples
+// need not wait for the stores of earlier ones).
 //
 // `ref` is a device picture or a window of one staged in LDS (WinView).
 template <class Pic>
-__device__ void predict_mb(const Pic &ref, const SearchOut &r, int16_t *out, int16_t *tmp)
+__device__ void predict_mb(const Pic &ref, const SearchOut &r, int16_t *__restrict__ out,
+                           int16_t *__restrict__ tmp)
 {
     const int lane = lane_id();
     int olx, oly, ocx, ocy;
